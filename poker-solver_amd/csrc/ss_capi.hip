@@ -1,0 +1,414 @@
+// ss_capi.hip -- extern "C" entry points of libstreet_solve.so (include/street_solve.h).
+#include <chrono>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "../../include/street_solve_debug.h"
+#include "ss_eval.h"
+#include "ss_internal.h"
+
+namespace ss {
+const char *last_error_str();
+
+struct StreamGuard {
+    hipStream_t st = nullptr;
+    int rc = SS_OK;
+    StreamGuard() {
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess)
+            rc = set_error(SS_ERR_HIP, "hipStreamCreate failed");
+    }
+    ~StreamGuard() {
+        if (st) (void)hipStreamDestroy(st);
+    }
+};
+
+static double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// Plain form with optional exploitability-targeted early stop.
+static int solve_plain(const HostProblem &hp, const SSSolveConfig &cfg, SSOutput *out, SSReport *rep) {
+    StreamGuard g;
+    if (g.rc) return g.rc;
+    double t0 = now_ms();
+    Slot s;
+    int rc = s.setup(hp, g.st);
+    if (rc) return rc;
+    const int T = std::max(0, cfg.max_iterations);
+    const bool target = cfg.target_exploitability > 0;
+    const int chunk = target ? std::max(1, cfg.check_every > 0 ? cfg.check_every : 50) : std::max(1, T);
+    double kms = 0, expl = -1, gains[SS_MAX_PLAYERS] = {0, 0, 0, 0, 0, 0};
+    int done = 0, converged = 0, expl_at = -1;
+    while (done < T) {
+        int k = std::min(chunk, T - done);
+        rc = plain_run(s, k, done + k, g.st, &kms);
+        if (rc) return rc;
+        done += k;
+        if (target) {
+            rc = plain_exploitability(s, g.st, &expl, gains);
+            if (rc) return rc;
+            expl_at = done;
+            if (expl >= 0 && expl <= cfg.target_exploitability) {
+                converged = 1;
+                break;
+            }
+        }
+    }
+    if (!(cfg.flags & SS_FLAG_NO_EXPL) && expl_at != done) {
+        rc = plain_exploitability(s, g.st, &expl, gains);
+        if (rc) return rc;
+    }
+    if (out) {
+        rc = read_output(s, s.desc, g.st, out, 0);
+        if (rc) return rc;
+    }
+    if (rep) {
+        std::memset(rep, 0, sizeof(*rep));
+        rep->iterations = done;
+        rep->converged = converged;
+        rep->exploitability = expl;
+        for (int p = 0; p < SS_MAX_PLAYERS; ++p) rep->nashconv[p] = gains[p];
+        rep->elapsed_ms = now_ms() - t0;
+        rep->kernel_ms = kms;
+        rep->algo_bytes = s.plan.algo_bytes_iter * done;
+    }
+    return SS_OK;
+}
+
+// Batched form: all subgames advance together, one persistent launch per
+// iteration; `chunk` launches are captured once in a hipGraph and replayed.
+int solve_batch(const HostProblem *hps, int n, const SSSolveConfig &cfg, SSOutput *outs, SSReport *reps,
+                double *wall_ms_out) {
+    StreamGuard g;
+    if (g.rc) return g.rc;
+    double t0 = now_ms();
+    std::vector<std::unique_ptr<Slot>> slots(n);
+    const int T = std::max(0, cfg.max_iterations);
+    const bool target = cfg.target_exploitability > 0;
+    const bool want_final = !(cfg.flags & SS_FLAG_NO_EXPL);
+    const int chunk = target ? std::max(1, cfg.check_every > 0 ? cfg.check_every : 50) : std::max(1, std::min(T, 50));
+    for (int i = 0; i < n; ++i) {
+        slots[i].reset(new Slot());
+        int rc = slots[i]->setup(hps[i], g.st);
+        if (rc) return set_error(rc, "subgame %d: %s", i, last_error_str());
+        if (target || want_final) {
+            rc = slots[i]->alloc_expl(g.st, 1);
+            if (rc) return rc;
+        }
+    }
+    std::vector<SlotDesc> descs(n);
+    for (int i = 0; i < n; ++i) {
+        descs[i] = slots[i]->desc;
+        descs[i].iter = 0;
+        descs[i].iter_end = T;
+        descs[i].final_iter = T;
+        descs[i].snap_every = target ? chunk : 0;
+        descs[i].active = 1;
+    }
+    SlotDesc *d_slots = nullptr;
+    int *d_want = nullptr;
+    double *scratch = nullptr;
+    struct Free {
+        void *p[3];
+        ~Free() {
+            for (void *q : p)
+                if (q) (void)hipFree(q);
+        }
+    } fr{{nullptr, nullptr, nullptr}};
+    SS_HIP(hipMalloc(&d_slots, sizeof(SlotDesc) * std::max(1, n)));
+    fr.p[0] = d_slots;
+    SS_HIP(hipMalloc(&d_want, sizeof(int) * std::max(1, n)));
+    fr.p[1] = d_want;
+    if (target || want_final) {
+        SS_HIP(hipMalloc(&scratch, sizeof(double) * (size_t)std::max(1, n) *
+                                       (2 * SS_NCOMBO + 3 * 2704 + 1328 + 3 * 52 + 256)));
+        fr.p[2] = scratch;
+    }
+    SS_HIP(hipMemcpyAsync(d_slots, descs.data(), sizeof(SlotDesc) * n, hipMemcpyHostToDevice, g.st));
+    SS_HIP(hipStreamSynchronize(g.st));
+
+    // capture `chunk` iterations
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    SS_HIP(hipStreamBeginCapture(g.st, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < chunk; ++k) {
+        int rc = launch_batched(d_slots, n, 1, n, g.st);
+        if (rc) {
+            (void)hipStreamEndCapture(g.st, &graph);
+            return rc;
+        }
+    }
+    SS_HIP(hipStreamEndCapture(g.st, &graph));
+    SS_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+
+    std::vector<int> iters(n, 0), conv(n, 0), live(n, 1), want(n, 0);
+    std::vector<double> expl(n, -1.0), gains((size_t)n * SS_MAX_PLAYERS, 0.0);
+    std::vector<int> expl_at(n, -1);
+    hipEvent_t e0, e1;
+    SS_HIP(hipEventCreate(&e0));
+    SS_HIP(hipEventCreate(&e1));
+    double kms = 0;
+    auto evaluate = [&](const std::vector<int> &sel, int at) -> int {
+        SS_HIP(hipMemcpyAsync(d_want, sel.data(), sizeof(int) * n, hipMemcpyHostToDevice, g.st));
+        int rc = launch_batched_expl(d_slots, n, scratch, d_want, n, g.st);
+        if (rc) return rc;
+        std::vector<double> xo(2 * SS_MAX_PLAYERS + 1);
+        for (int i = 0; i < n; ++i) {
+            if (!sel[i]) continue;
+            SS_HIP(hipMemcpyAsync(xo.data(), slots[i]->desc.xout, sizeof(double) * (2 * slots[i]->plan.NP + 1),
+                                  hipMemcpyDeviceToHost, g.st));
+            SS_HIP(hipStreamSynchronize(g.st));
+            expl[i] = expl_from_xout(xo.data(), slots[i]->plan.NP, slots[i]->plan.starting_pot,
+                                     &gains[(size_t)i * SS_MAX_PLAYERS]);
+            expl_at[i] = at;
+        }
+        return SS_OK;
+    };
+
+    int done = 0;
+    while (done < T) {
+        SS_HIP(hipEventRecord(e0, g.st));
+        SS_HIP(hipGraphLaunch(exec, g.st));
+        SS_HIP(hipEventRecord(e1, g.st));
+        SS_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        SS_HIP(hipEventElapsedTime(&ms, e0, e1));
+        kms += ms;
+        int step = std::min(chunk, T - done);
+        done += step;
+        for (int i = 0; i < n; ++i)
+            if (live[i]) iters[i] = done;
+        if (target) {
+            for (int i = 0; i < n; ++i) want[i] = live[i];
+            int rc = evaluate(want, done);
+            if (rc) return rc;
+            int nlive = 0;
+            for (int i = 0; i < n; ++i) {
+                if (!live[i]) continue;
+                if (expl[i] >= 0 && expl[i] <= cfg.target_exploitability) {
+                    conv[i] = 1;
+                    live[i] = 0;
+                    static const int kZero = 0;
+                    SS_HIP(hipMemcpyAsync(&d_slots[i].active, &kZero, sizeof(int), hipMemcpyHostToDevice, g.st));
+                } else {
+                    ++nlive;
+                }
+            }
+            SS_HIP(hipStreamSynchronize(g.st));
+            if (nlive == 0) break;
+        }
+    }
+    if (want_final) {
+        for (int i = 0; i < n; ++i) want[i] = expl_at[i] != iters[i];
+        // re-activate retired slots for the evaluation only if they still need one (none do)
+        int rc = evaluate(want, done);
+        if (rc) return rc;
+    }
+    double wall = now_ms() - t0;
+    for (int i = 0; i < n; ++i) {
+        if (outs) {
+            int rc = read_output(*slots[i], slots[i]->desc, g.st, &outs[i], 0);
+            if (rc) return rc;
+        }
+        if (reps) {
+            SSReport &r = reps[i];
+            std::memset(&r, 0, sizeof(r));
+            r.iterations = iters[i];
+            r.converged = conv[i];
+            r.exploitability = expl[i];
+            for (int p = 0; p < SS_MAX_PLAYERS; ++p) r.nashconv[p] = gains[(size_t)i * SS_MAX_PLAYERS + p];
+            r.elapsed_ms = wall;
+            r.kernel_ms = kms;
+            r.algo_bytes = slots[i]->plan.algo_bytes_iter * iters[i];
+        }
+    }
+    if (wall_ms_out) *wall_ms_out = wall;
+    (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return SS_OK;
+}
+
+}  // namespace ss
+
+using namespace ss;
+
+static int legacy_check(const SSTreeData *td) {
+    if (!td) return set_error(SS_ERR_ARG, "tree_data is NULL");
+    if (td->num_players < 2 || td->num_players > SS_MAX_PLAYERS)
+        return set_error(SS_ERR_ARG, "num_players %d", td->num_players);
+    for (int p = 0; p < td->num_players; ++p)
+        if (td->num_hands[p] < 0 || td->num_hands[p] > SS_MAX_HANDS)
+            return set_error(SS_ERR_ARG, "player %d has %d hands (legacy ABI max %d)", p, td->num_hands[p],
+                             SS_MAX_HANDS);
+    return SS_OK;
+}
+
+extern "C" {
+
+int ss_build_tree(const int *board, int num_board, int starting_pot, int effective_stack, const float *bet_sizes,
+                  int num_bet_sizes, int num_players, const int *acting_order, int is_river, int use_cont_strats,
+                  SSTreeData *tree_out) {
+    clear_error();
+    return build_tree(board, num_board, starting_pot, effective_stack, bet_sizes, num_bet_sizes, num_players,
+                      acting_order, is_river, use_cont_strats, SS_MAX_RAISES, tree_out);
+}
+
+int ss_build_tree_ex(const int *board, int num_board, int starting_pot, int effective_stack, const float *bet_sizes,
+                     int num_bet_sizes, int num_players, const int *acting_order, int is_river, int use_cont_strats,
+                     int max_raises, SSTreeData *tree_out) {
+    clear_error();
+    return build_tree(board, num_board, starting_pot, effective_stack, bet_sizes, num_bet_sizes, num_players,
+                      acting_order, is_river, use_cont_strats, max_raises, tree_out);
+}
+
+int ss_solve_gpu(SSTreeData *tree_data, int max_iterations, SSOutput *output) {
+    clear_error();
+    int rc = legacy_check(tree_data);
+    if (rc) return rc;
+    if (!output) return set_error(SS_ERR_ARG, "output is NULL");
+    HostProblem hp = from_legacy(tree_data);
+    SSSolveConfig cfg;
+    std::memset(&cfg, 0, sizeof(cfg));
+    cfg.max_iterations = max_iterations;
+    cfg.flags = SS_FLAG_NO_EXPL;
+    return solve_plain(hp, cfg, output, nullptr);
+}
+
+void ss_free_tree(SSTreeData *tree_data) { free_tree(tree_data); }
+
+void ss_free_output(SSOutput *out) {
+    if (!out) return;
+    std::free(out->root_strategy);
+    std::free(out->root_ev);
+    std::free(out->avg_strategies);
+    std::free(out->avg_strategy_node_ids);
+    std::memset(out, 0, sizeof(*out));
+}
+
+int ss_solve_gpu_ex(const SSProblem *problem, const SSSolveConfig *config, SSOutput *output, SSReport *report) {
+    clear_error();
+    if (!config) return set_error(SS_ERR_ARG, "config is NULL");
+    HostProblem hp;
+    int rc = from_ex(problem, &hp);
+    if (rc) return rc;
+    return solve_plain(hp, *config, output, report);
+}
+
+int ss_solve_batch_ex(const SSProblem *problems, int n, const SSSolveConfig *config, SSOutput *outputs,
+                      SSReport *reports) {
+    clear_error();
+    if (!config || n < 0 || (n > 0 && !problems)) return set_error(SS_ERR_ARG, "bad batch arguments");
+    if (n == 0) return SS_OK;
+    std::vector<HostProblem> hps(n);
+    for (int i = 0; i < n; ++i) {
+        int rc = from_ex(&problems[i], &hps[i]);
+        if (rc) return rc;
+    }
+    return solve_batch(hps.data(), n, *config, outputs, reports, nullptr);
+}
+
+int64_t ss_problem_footprint(const SSProblem *problem) {
+    clear_error();
+    HostProblem hp;
+    int rc = from_ex(problem, &hp);
+    if (rc) return rc;
+    return estimate_footprint(hp);
+}
+
+const char *ss_last_error(void) { return last_error_str(); }
+
+const char *ss_version(void) { return "street_solve-mi355x 0.1 (gfx950)"; }
+
+// ---- test entry points (include/street_solve_debug.h) ----------------------
+
+int ssdbg_eval7(const int *cards, int n, uint32_t *out, int on_device) {
+    clear_error();
+    if (n < 0 || (n > 0 && (!cards || !out))) return set_error(SS_ERR_ARG, "bad arguments");
+    if (on_device) return dbg_eval7_device(cards, n, out);
+    static uint8_t tab[8192];
+    static bool ready = false;
+    if (!ready) {
+        fill_straight_table(tab, 0, 1);
+        ready = true;
+    }
+    for (int i = 0; i < n; ++i) out[i] = eval7_lut(cards + 7 * i, tab);
+    return SS_OK;
+}
+
+int ssdbg_combo_ranks(const int *board5, uint32_t *out1326) {
+    clear_error();
+    if (!board5 || !out1326) return set_error(SS_ERR_ARG, "NULL argument");
+    return dbg_combo_ranks_device(board5, out1326);
+}
+
+int ssdbg_eval_census(unsigned long long counts[10], unsigned long long *rank_sum) {
+    clear_error();
+    return dbg_eval_census_device(counts, rank_sum);
+}
+
+int ssdbg_solve_state(const SSProblem *problem, int iterations, int form, float *regrets, float *strat_sum,
+                      float *root_ev) {
+    clear_error();
+    HostProblem hp;
+    int rc = from_ex(problem, &hp);
+    if (rc) return rc;
+    StreamGuard g;
+    if (g.rc) return g.rc;
+    Slot s;
+    rc = s.setup(hp, g.st);
+    if (rc) return rc;
+    if (form == 0) {
+        double ms = 0;
+        rc = plain_run(s, iterations, iterations, g.st, &ms);
+        if (rc) return rc;
+    } else {
+        SlotDesc d = s.desc;
+        d.iter = 0;
+        d.iter_end = iterations;
+        d.final_iter = iterations;
+        d.snap_every = 0;
+        d.active = 1;
+        SS_HIP(hipMemcpyAsync(s.d_desc, &d, sizeof(d), hipMemcpyHostToDevice, g.st));
+        rc = launch_batched(s.d_desc, 1, iterations, 1, g.st);
+        if (rc) return rc;
+    }
+    const Plan &pl = s.plan;
+    const size_t Mp = pl.Mp;
+    std::vector<float> R((size_t)pl.A * Mp), SSv((size_t)pl.A * Mp), cfv0(Mp);
+    SS_HIP(hipMemcpyAsync(R.data(), s.desc.R, sizeof(float) * R.size(), hipMemcpyDeviceToHost, g.st));
+    SS_HIP(hipMemcpyAsync(SSv.data(), s.desc.SS, sizeof(float) * SSv.size(), hipMemcpyDeviceToHost, g.st));
+    SS_HIP(hipMemcpyAsync(cfv0.data(), s.desc.cfv, sizeof(float) * Mp, hipMemcpyDeviceToHost, g.st));
+    SS_HIP(hipStreamSynchronize(g.st));
+    // reference layout [N][8][maxh]
+    for (int n = 0; n < pl.N; ++n)
+        for (int a = 0; a < SS_MAX_ACTIONS; ++a)
+            for (int h = 0; h < pl.maxh; ++h) {
+                size_t o = ((size_t)n * SS_MAX_ACTIONS + a) * pl.maxh + h;
+                const DevNode &nd = pl.nodes[n];
+                bool has = nd.type == SS_NODE_DECISION && a < nd.nchild;
+                if (regrets) regrets[o] = has ? R[(size_t)(nd.act_base + a) * Mp + h] : 0.f;
+                if (strat_sum) strat_sum[o] = has ? SSv[(size_t)(nd.act_base + a) * Mp + h] : 0.f;
+            }
+    if (root_ev)
+        for (int h = 0; h < pl.maxh; ++h) root_ev[h] = h < pl.nh[pl.NP - 1] ? cfv0[h] : 0.f;
+    return SS_OK;
+}
+
+int ssdbg_plan_bytes(const SSProblem *problem, int64_t *algo_bytes_per_iter, int64_t *footprint) {
+    clear_error();
+    HostProblem hp;
+    int rc = from_ex(problem, &hp);
+    if (rc) return rc;
+    Plan pl;
+    rc = make_plan(hp, pl);
+    if (rc) return rc;
+    if (algo_bytes_per_iter) *algo_bytes_per_iter = pl.algo_bytes_iter;
+    if (footprint) *footprint = estimate_footprint(hp);
+    return SS_OK;
+}
+
+}  // extern "C"

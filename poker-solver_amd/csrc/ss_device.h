@@ -1,0 +1,91 @@
+// ss_device.h -- data layout shared by the host runtime and the HIP kernels.
+//
+// One subgame ("slot") lives in one device arena. Its hand axis is padded to
+// Mp (a multiple of 64, one wavefront), so every (node x hand) sweep is a
+// coalesced stream of 64-float rows.
+//
+//   nodes     DevNode[N]            tree, struct per node (64 B)
+//   children  int[C]                child ids, reference layout
+//   dec_lvl   int[D]                decision nodes grouped by BFS depth,
+//   dec_off   int[L+1]                level l = dec_lvl[dec_off[l] .. dec_off[l+1])
+//   term      int[T]                fold / showdown / leaf nodes
+//   hinfo     uint32[NP][Mp]        packed hand: c0 | c1<<6 | combo<<12 | live<<23
+//   weights   float[NP][Mp]         root reach (raw input weights)
+//   R, SS     float[A][Mp]          regrets / strategy sums, one row per action
+//                                   of each decision node (rows act_base..+nchild)
+//   cfv       float[N][Mp]          counterfactual values of the current pass
+//   reach     float[NP][N][Mp]      reach of every player at every node
+//   rsig      float[8][Mp]          strategy at node 0 taken before the last pass
+//   river tables (showdown closed forms, see ss_kernels.hip):
+//   cstr      uint32[1326]          7-card rank of each combo (0 = blocked)
+//   gsort     int16[ng]             live combos by ascending rank
+//   glt, gle  int16[1326]           # live combos ranked <, <= the combo
+//   clist     uint8[52][52]         per card: partner cards by ascending rank
+//   clen      uint8[52]
+//   plt, ple  uint8[52][1326]       per card c and combo k: # entries of c's
+//                                   list ranked <, <= rank(k)
+#pragma once
+#include <stdint.h>
+
+#define SS_NCOMBO 1326
+#define SS_LIVE_BIT (1u << 23)
+
+namespace ss {
+
+struct DevNode {
+    int type, player, nchild, first_child;
+    int pot, leaf_idx, act_base, active_mask;
+    int bets[6];
+    int depth, pad;
+};
+
+struct SlotDesc {
+    // shape
+    int N, NP, Mp, maxh;
+    int A, L, T, D;
+    int is_river, has_leaf, num_leaves, root_player;
+    int nh[6];
+    int ng, odometer;          // odometer: k>=3 terminals present (slow path)
+    // progress (device-updated; iteration-dependent data never travels as a kernel argument)
+    int iter;                  // last completed iteration
+    int iter_end;              // the slot stops advancing at this iteration
+    int final_iter;            // iteration whose last pass snapshots rsig
+    int snap_every;            // also snapshot rsig at iterations divisible by this (0 = off)
+    int active;
+    // arrays
+    const DevNode *nodes;
+    const int *children;
+    const int *dec_lvl;
+    const int *dec_off;
+    const int *term;
+    const uint32_t *hinfo;
+    const float *weights;
+    const float *leaf_values;
+    const int *frozen;
+    float *R;
+    float *SS;
+    float *cfv;
+    float *reach;
+    float *rsig;
+    const uint32_t *cstr;
+    const int16_t *gsort;
+    const int16_t *glt;
+    const int16_t *gle;
+    const uint8_t *clist;
+    const uint8_t *clen;
+    const uint8_t *plt;
+    const uint8_t *ple;
+    // exploitability scratch (FP64), allocated on demand
+    float *avg;                // [A][Mp]
+    double *xreach;            // [NP][N][Mp]
+    double *xev;               // [N][Mp]
+    double *xbr;               // [N][Mp]
+    double *xout;              // [2*NP + 1]: gains, values, joint mass
+};
+
+__host__ __device__ inline int combo_id(int a, int b) {
+    if (a > b) { int t = a; a = b; b = t; }
+    return 51 * a - (a * (a - 1)) / 2 + (b - a - 1);
+}
+
+}  // namespace ss

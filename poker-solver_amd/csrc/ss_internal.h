@@ -1,0 +1,110 @@
+// ss_internal.h -- host-side declarations shared by the library's sources.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/street_solve.h"
+#include "ss_device.h"
+
+namespace ss {
+
+int set_error(int code, const char *fmt, ...);
+void clear_error();
+int log_level();
+
+#define SS_HIP(call)                                                                             \
+    do {                                                                                         \
+        hipError_t e_ = (call);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            return ::ss::set_error(SS_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                                   __FILE__, __LINE__);                                          \
+    } while (0)
+
+int build_tree(const int *board, int num_board, int starting_pot, int effective_stack,
+               const float *bet_sizes, int num_bet_sizes, int num_players, const int *acting_order,
+               int is_river, int use_cont_strats, int max_raises, SSTreeData *out);
+void free_tree(SSTreeData *td);
+
+// Input of one solve, independent of which ABI delivered it.
+struct HostProblem {
+    const SSTreeData *tree = nullptr;
+    int np = 0;
+    int nh[SS_MAX_PLAYERS] = {0, 0, 0, 0, 0, 0};
+    const int *hands[SS_MAX_PLAYERS] = {};     // [nh][2]
+    const float *weights[SS_MAX_PLAYERS] = {};
+    const float *leaf_values = nullptr;          // [num_leaves][np][maxh]
+    const int *frozen = nullptr;                 // [num_nodes]
+};
+
+HostProblem from_legacy(const SSTreeData *td);
+int from_ex(const SSProblem *p, HostProblem *out);
+
+// Host-side plan of a slot: every device array, filled before upload.
+struct Plan {
+    int N = 0, NP = 0, Mp = 0, maxh = 0, A = 0, L = 0, T = 0, D = 0;
+    int is_river = 0, has_leaf = 0, num_leaves = 0, root_player = 0, ng = 0, odometer = 0;
+    int nh[6] = {0, 0, 0, 0, 0, 0};
+    int board[5] = {0, 0, 0, 0, 0};
+    int num_board = 0;
+    int starting_pot = 0;
+    std::vector<DevNode> nodes;
+    std::vector<int> children, dec_lvl, dec_off, term, dec_index;
+    std::vector<uint32_t> hinfo;
+    std::vector<float> weights;
+    const float *leaf_src = nullptr;
+    size_t leaf_count = 0;
+    const int *frozen_src = nullptr;
+    int64_t algo_bytes_iter = 0;   // algorithmic bytes of one iteration
+};
+
+int make_plan(const HostProblem &hp, Plan &pl);
+
+// River tables built on the host from device-computed combo ranks.
+struct RiverTables {
+    std::vector<uint32_t> cstr;   // [1326]
+    std::vector<int16_t> gsort, glt, gle;
+    std::vector<uint8_t> clist, clen, plt, ple;
+    int ng = 0;
+};
+void build_river_tables(const uint32_t *cstr, RiverTables &rt);
+
+// A subgame resident on the device.
+struct Slot {
+    Plan plan;
+    SlotDesc desc{};              // host copy (pointers are device pointers)
+    SlotDesc *d_desc = nullptr;   // device copy (plain form); batched forms copy desc into their array
+    char *arena = nullptr;
+    size_t arena_bytes = 0;
+    char *xarena = nullptr;       // FP64 exploitability buffers
+    size_t xarena_bytes = 0;
+    double *xscratch = nullptr;   // per-workgroup FP64 terminal scratch
+    int xscratch_blocks = 0;
+
+    ~Slot();
+    int setup(const HostProblem &hp, hipStream_t st);
+    int alloc_expl(hipStream_t st, int blocks);
+    int64_t footprint() const { return (int64_t)arena_bytes + (int64_t)xarena_bytes; }
+};
+
+int64_t estimate_footprint(const HostProblem &hp);
+
+// plain form
+int plain_run(Slot &s, int iters, int final_iter, hipStream_t st, double *kernel_ms);
+int plain_exploitability(Slot &s, hipStream_t st, double *expl_pct, double *gains);
+int read_output(Slot &s, const SlotDesc &desc_now, hipStream_t st, SSOutput *out, int legacy_root_ev);
+
+// batched helpers
+int launch_batched(SlotDesc *d_slots, int nslots, int iters, int grid, hipStream_t st);
+int launch_batched_expl(SlotDesc *d_slots, int nslots, double *scratch, const int *d_want, int grid,
+                        hipStream_t st);
+double expl_from_xout(const double *xout, int np, int starting_pot, double *gains);
+
+// debug / test entry points implemented in the kernels unit
+int dbg_eval7_device(const int *cards, int n, uint32_t *out);
+int dbg_combo_ranks_device(const int *board, uint32_t *out);
+int dbg_eval_census_device(unsigned long long counts[10], unsigned long long *sum);
+
+}  // namespace ss

@@ -1,0 +1,950 @@
+// ss_kernels.hip -- CDNA4 (gfx950) kernels of the single-street Linear-CFR solver.
+//
+// Semantics follow src/cuda/street_solve.cu (kernels :575-1185, loop
+// :1471-1550); the data layout is this library's own (ss_device.h).
+//
+// Per traverser pass:
+//   forward   one launch per BFS level of decision nodes: regret matching of
+//             the acting player fused into the reach sweep (the reference's
+//             separate regret-match pass :575 and reach pass :650), every
+//             non-traverser's reach advanced in the same item.
+//   terminals one workgroup per terminal node. An opponent's reach is
+//             scattered into combo space (1326 slots, LDS); card removal then
+//             has closed forms (no O(M^k) enumeration):
+//               1 opponent:  Z(h) = S - C[x] - C[y] + M[x,y]
+//               2 opponents: D(h) = Z1 Z2 - sum_c (C1[c]-M1[c,x]-M1[c,y])(C2[c]-..)
+//                                   + Z_{M1*M2}
+//             and showdowns split the same sums by strength class (<, =)
+//             with prefix sums over strength-sorted combo lists. 3+ opponents
+//             use the reference odometer (small ranges only; host-guarded).
+//   backward  one launch per level, bottom-up: value, regret update with the
+//             Linear-CFR discount t/(t+1), strategy sum += t * sigma (:1138-1185).
+// The batched kernel runs all of this for one subgame per workgroup with
+// barriers between levels; the FP64 sweeps evaluate the best response.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "../../include/street_solve.h"
+#include "ss_device.h"
+#include "ss_eval.h"
+
+namespace ss {
+
+#define SS_BLOCK 256
+#define NT_DEC 8
+
+// ---------------------------------------------------------------------------
+// hand-rank kernels
+// ---------------------------------------------------------------------------
+
+// cstr[k] for all 1326 combos k on a 5-card board (0 when a card is on the
+// board). The straight table lives in LDS.
+__global__ void __launch_bounds__(SS_BLOCK) k_combo_ranks(const int *board, uint32_t *cstr) {
+    __shared__ uint8_t tab[8192];
+    fill_straight_table(tab, threadIdx.x, blockDim.x);
+    __syncthreads();
+    int b0 = board[0], b1 = board[1], b2 = board[2], b3 = board[3], b4 = board[4];
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < SS_NCOMBO; k += gridDim.x * blockDim.x) {
+        // invert combo_id
+        int a = 0, rem = k;
+        while (rem >= 51 - a) { rem -= 51 - a; ++a; }
+        int b = a + 1 + rem;
+        bool dead = a == b0 || a == b1 || a == b2 || a == b3 || a == b4 || b == b0 || b == b1 || b == b2 ||
+                    b == b3 || b == b4;
+        int c[7] = {b0, b1, b2, b3, b4, a, b};
+        cstr[k] = dead ? 0u : eval7_lut(c, tab);
+    }
+}
+
+// ranks of explicit 7-card hands (test entry)
+__global__ void __launch_bounds__(SS_BLOCK) k_eval7(const int *cards, int n, uint32_t *out) {
+    __shared__ uint8_t tab[8192];
+    fill_straight_table(tab, threadIdx.x, blockDim.x);
+    __syncthreads();
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        out[i] = eval7_lut(cards + 7 * i, tab);
+}
+
+// Census over all C(52,7) hands: grid over the three lowest cards.
+__global__ void __launch_bounds__(SS_BLOCK) k_eval_census(unsigned long long *counts, unsigned long long *sum) {
+    __shared__ uint8_t tab[8192];
+    __shared__ unsigned long long lc[10];
+    __shared__ unsigned long long ls;
+    fill_straight_table(tab, threadIdx.x, blockDim.x);
+    if (threadIdx.x < 10) lc[threadIdx.x] = 0;
+    if (threadIdx.x == 0) ls = 0;
+    __syncthreads();
+    // blockIdx.x indexes (c0, c1); threads stride over c2
+    int c0 = 0, rem = blockIdx.x;
+    while (rem >= 51 - c0) { rem -= 51 - c0; ++c0; }
+    int c1 = c0 + 1 + rem;
+    unsigned long long my[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    unsigned long long mysum = 0;
+    for (int c2 = c1 + 1 + threadIdx.x; c2 < 52; c2 += blockDim.x) {
+        int c[7] = {c0, c1, c2, 0, 0, 0, 0};
+        for (c[3] = c2 + 1; c[3] < 52; ++c[3])
+            for (c[4] = c[3] + 1; c[4] < 52; ++c[4])
+                for (c[5] = c[4] + 1; c[5] < 52; ++c[5])
+                    for (c[6] = c[5] + 1; c[6] < 52; ++c[6]) {
+                        uint32_t v = eval7_lut(c, tab);
+                        my[v >> 20]++;
+                        mysum += v;
+                    }
+    }
+    for (int i = 0; i < 10; ++i)
+        if (my[i]) atomicAdd(&lc[i], my[i]);
+    atomicAdd(&ls, mysum);
+    __syncthreads();
+    if (threadIdx.x < 10) atomicAdd(&counts[threadIdx.x], lc[threadIdx.x]);
+    if (threadIdx.x == 0) atomicAdd(sum, ls);
+}
+
+// ---------------------------------------------------------------------------
+// regret matching (street_solve.cu:575-614, :621-644). The reference's
+// pruning predicate maps regrets < -10000 to 0, which matches what any
+// negative regret contributes, so it is not evaluated.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void regret_match(const float *__restrict__ R, int row0, int na, int Mp, int h,
+                                             const int *frozen, int n, float *sig) {
+    float sum = 0.f;
+#pragma unroll
+    for (int a = 0; a < NT_DEC; ++a) {
+        if (a < na) {
+            float r = R[(size_t)(row0 + a) * Mp + h];
+            sig[a] = r > 0.f ? r : 0.f;
+            sum += sig[a];
+        }
+    }
+    if (sum > 0.f) {
+        float inv = 1.0f / sum;
+#pragma unroll
+        for (int a = 0; a < NT_DEC; ++a)
+            if (a < na) sig[a] *= inv;
+    } else {
+        float u = 1.0f / (float)na;
+#pragma unroll
+        for (int a = 0; a < NT_DEC; ++a)
+            if (a < na) sig[a] = u;
+    }
+    if (frozen) {
+        int fa = frozen[n];
+        if (fa >= 0) {
+#pragma unroll
+            for (int a = 0; a < NT_DEC; ++a)
+                if (a < na) sig[a] = a == fa ? 1.f : 0.f;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// forward: reach of every non-traverser through one level (items = node x hand)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void forward_items(const SlotDesc *__restrict__ s, int level, int trav, int begin,
+                                              int stride) {
+    const int Mp = s->Mp, N = s->N, NP = s->NP;
+    const int lo = s->dec_off[level];
+    const int total = (s->dec_off[level + 1] - lo) * Mp;
+    for (int it = begin; it < total; it += stride) {
+        int ni = it / Mp, h = it - ni * Mp;
+        int n = s->dec_lvl[lo + ni];
+        const DevNode &nd = s->nodes[n];
+        int actor = nd.player, na = nd.nchild, fc = nd.first_child;
+        float sig[NT_DEC];
+        if (actor != trav && actor >= 0 && actor < NP && h < s->nh[actor])
+            regret_match(s->R, nd.act_base, na, Mp, h, s->frozen, n, sig);
+        for (int rp = 0; rp < NP; ++rp) {
+            if (rp == trav || h >= s->nh[rp]) continue;
+            float *rr = s->reach + (size_t)rp * N * Mp;
+            float pr = rr[(size_t)n * Mp + h];
+            for (int a = 0; a < na; ++a) {
+                int c = s->children[fc + a];
+                rr[(size_t)c * Mp + h] = rp == actor ? pr * sig[a] : pr;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// backward: value + regret / strategy-sum update through one level
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void backward_items(const SlotDesc *__restrict__ s, int level, int trav, int iter,
+                                               int begin, int stride) {
+    const int Mp = s->Mp;
+    const int lo = s->dec_off[level];
+    const int total = (s->dec_off[level + 1] - lo) * Mp;
+    const int nh = s->nh[trav];
+    const float d = (float)iter / ((float)iter + 1.0f);
+    const float fit = (float)iter;
+    for (int it = begin; it < total; it += stride) {
+        int ni = it / Mp, h = it - ni * Mp;
+        if (h >= nh) continue;
+        int n = s->dec_lvl[lo + ni];
+        const DevNode &nd = s->nodes[n];
+        int na = nd.nchild, fc = nd.first_child;
+        float v = 0.f;
+        if (nd.player == trav) {
+            float sig[NT_DEC], cv[NT_DEC];
+            regret_match(s->R, nd.act_base, na, Mp, h, s->frozen, n, sig);
+            for (int a = 0; a < na; ++a) {
+                cv[a] = s->cfv[(size_t)s->children[fc + a] * Mp + h];
+                v += sig[a] * cv[a];
+            }
+            for (int a = 0; a < na; ++a) {
+                size_t row = (size_t)(nd.act_base + a) * Mp + h;
+                float r = s->R[row];
+                r += cv[a] - v;
+                r *= d;
+                s->R[row] = r;
+                s->SS[row] += fit * sig[a];
+            }
+        } else {
+            for (int a = 0; a < na; ++a) v += s->cfv[(size_t)s->children[fc + a] * Mp + h];
+        }
+        s->cfv[(size_t)n * Mp + h] = v;
+    }
+}
+
+// strategy at node 0, kept for the output's root_strategy (street_solve.cu:1563-1573)
+__device__ __forceinline__ void root_sigma_items(const SlotDesc *__restrict__ s, int begin, int stride) {
+    const DevNode &nd = s->nodes[0];
+    if (nd.act_base < 0 || nd.player < 0 || nd.player >= s->NP) return;
+    for (int h = begin; h < s->Mp; h += stride) {
+        float sig[NT_DEC];
+        if (h < s->nh[nd.player]) {
+            regret_match(s->R, nd.act_base, nd.nchild, s->Mp, h, s->frozen, 0, sig);
+            for (int a = 0; a < nd.nchild; ++a) s->rsig[(size_t)a * s->Mp + h] = sig[a];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// terminal values
+// ---------------------------------------------------------------------------
+template <typename T>
+struct TermBuf {
+    T *M1, *M2;          // [1326] opponent reach in combo space
+    T *PC1, *PC2, *PCP;  // [52*52] per-card prefix over strength-sorted partners
+    T *PG;               // [1328] prefix over strength-sorted combos
+    T *C1, *C2, *CP;     // [52] per-card totals
+    T *red;              // [SS_BLOCK] scan scratch
+};
+
+template <typename T>
+__device__ __forceinline__ T tsum52(const T *C) {
+    T s = 0;
+    for (int c = 0; c < 52; ++c) s += C[c];
+    return s;
+}
+
+// exclusive block scan of M[gsort[j]] into PG[0..ng]
+template <typename T>
+__device__ void scan_sorted(const SlotDesc *__restrict__ s, const T *M, T *PG, T *red, int tid, int nt) {
+    const int ng = s->ng;
+    const int chunk = (ng + nt - 1) / nt;
+    const int j0 = tid * chunk, j1 = min(ng, j0 + chunk);
+    T part = 0;
+    for (int j = j0; j < j1; ++j) part += M[s->gsort[j]];
+    red[tid] = part;
+    __syncthreads();
+    for (int off = 1; off < nt; off <<= 1) {
+        T v = tid >= off ? red[tid - off] : (T)0;
+        __syncthreads();
+        red[tid] += v;
+        __syncthreads();
+    }
+    T run = tid > 0 ? red[tid - 1] : (T)0;
+    for (int j = j0; j < j1; ++j) {
+        PG[j] = run;
+        run += M[s->gsort[j]];
+    }
+    if (tid == nt - 1) PG[ng] = red[nt - 1];
+    __syncthreads();
+}
+
+// reference odometer for 3+ opponents, one traverser hand (street_solve.cu
+// :764-823 fold/leaf mass, :936-1002 showdown)
+template <typename T>
+__device__ T odometer(const SlotDesc *__restrict__ s, int n, int trav, int h, const int *opp, int k,
+                      const T *reach, bool showdown, T pot, T bet) {
+    const int Mp = s->Mp, N = s->N;
+    uint32_t hi = s->hinfo[(size_t)trav * Mp + h];
+    int x = hi & 63, y = (hi >> 6) & 63;
+    uint32_t hs = showdown ? s->cstr[(hi >> 12) & 2047] : 0;
+    int idx[6] = {0, 0, 0, 0, 0, 0};
+    T acc = 0;
+    for (;;) {
+        uint64_t used = (1ull << x) | (1ull << y);
+        T prod = 1;
+        bool ok = true;
+        uint32_t best = hs;
+        int ties = 0;
+        for (int i = 0; i < k && ok; ++i) {
+            int q = opp[i];
+            uint32_t oi = s->hinfo[(size_t)q * Mp + idx[i]];
+            if (!(oi & SS_LIVE_BIT)) { ok = false; break; }
+            uint64_t m = (1ull << (oi & 63)) | (1ull << ((oi >> 6) & 63));
+            if (used & m) { ok = false; break; }
+            used |= m;
+            prod *= reach[((size_t)q * N + n) * Mp + idx[i]];
+            if (showdown) {
+                uint32_t os = s->cstr[(oi >> 12) & 2047];
+                if (os > best) { best = os; ties = 0; }
+                else if (os == best) ++ties;
+            }
+        }
+        if (ok && prod > (T)0) {
+            if (!showdown) acc += prod;
+            else if (best > hs) acc += prod * (-bet);
+            else acc += prod * (pot / (T)(ties + 1) - bet);
+        }
+        int c = k - 1;
+        while (c >= 0) {
+            if (++idx[c] < s->nh[opp[c]]) break;
+            idx[c] = 0;
+            --c;
+        }
+        if (c < 0) break;
+    }
+    return acc;
+}
+
+// Scatter opponent q's reach at node n into combo space.
+template <typename T>
+__device__ __forceinline__ void scatter(const SlotDesc *__restrict__ s, int q, int n, const T *reach, T *M,
+                                        int tid, int nt) {
+    const int Mp = s->Mp;
+    for (int o = tid; o < s->nh[q]; o += nt) {
+        uint32_t oi = s->hinfo[(size_t)q * Mp + o];
+        if (oi & SS_LIVE_BIT) atomicAdd(&M[(oi >> 12) & 2047], reach[((size_t)q * s->N + n) * Mp + o]);
+    }
+}
+
+// per-card totals (non-showdown) of M: C[c] = sum_d M[c,d]; optionally of M1*M2
+template <typename T>
+__device__ __forceinline__ void card_totals(const T *M1, const T *M2, T *C1, T *C2, T *CP, int tid, int nt) {
+    for (int c = tid; c < 52; c += nt) {
+        T a = 0, b = 0, p = 0;
+        for (int d = 0; d < 52; ++d) {
+            if (d == c) continue;
+            int k = combo_id(c, d);
+            T m1 = M1[k];
+            a += m1;
+            if (M2) {
+                T m2 = M2[k];
+                b += m2;
+                p += m1 * m2;
+            }
+        }
+        C1[c] = a;
+        if (M2) { C2[c] = b; CP[c] = p; }
+    }
+}
+
+// per-card prefix sums over strength-sorted partner lists (showdown)
+template <typename T>
+__device__ __forceinline__ void card_prefix(const SlotDesc *__restrict__ s, const T *M1, const T *M2, T *PC1,
+                                            T *PC2, T *PCP, T *C1, T *C2, T *CP, int tid, int nt) {
+    for (int c = tid; c < 52; c += nt) {
+        int len = s->clen[c];
+        T a = 0, b = 0, p = 0;
+        PC1[c * 52] = 0;
+        if (M2) { PC2[c * 52] = 0; PCP[c * 52] = 0; }
+        for (int j = 0; j < len; ++j) {
+            int k = combo_id(c, s->clist[c * 52 + j]);
+            T m1 = M1[k];
+            a += m1;
+            PC1[c * 52 + j + 1] = a;
+            if (M2) {
+                T m2 = M2[k];
+                b += m2;
+                p += m1 * m2;
+                PC2[c * 52 + j + 1] = b;
+                PCP[c * 52 + j + 1] = p;
+            }
+        }
+        C1[c] = a;
+        if (M2) { C2[c] = b; CP[c] = p; }
+    }
+}
+
+// Kinds of terminal evaluation
+#define TK_NODE 0      // the node's own type (fold / showdown / leaf)
+#define TK_MASS 1      // opponent mass only, all non-traversers, times `scale`
+
+// Writes out[h] for h < nh[trav] of node n for traverser trav.
+// kind TK_MASS: out[h] = scale[h] * mass(all other players at node n).
+template <typename T>
+__device__ void terminal_node(const SlotDesc *__restrict__ s, int n, int trav, const T *reach, T *out,
+                              TermBuf<T> B, int kind, const float *scale, int tid, int nt) {
+    const DevNode &nd = s->nodes[n];
+    const int Mp = s->Mp, NP = s->NP, nh = s->nh[trav];
+    const int type = kind == TK_MASS ? SS_NODE_FOLD : nd.type;
+    int opp[6], k = 0;
+    for (int q = 0; q < NP; ++q) {
+        if (q == trav) continue;
+        if (type != SS_NODE_FOLD && !((nd.active_mask >> q) & 1)) continue;   // :866-873, :1037-1044
+        opp[k++] = q;
+    }
+    bool zero = (type == SS_NODE_SHOWDOWN && (!s->is_river || !((nd.active_mask >> trav) & 1))) ||
+                (type == SS_NODE_LEAF && (!s->has_leaf || nd.leaf_idx < 0));
+    if (zero) {
+        for (int h = tid; h < nh; h += nt) out[h] = 0;
+        return;
+    }
+    const T pot = (T)nd.pot, bet = (T)nd.bets[trav];
+    if (k >= 3) {
+        for (int h = tid; h < nh; h += nt) {
+            T v;
+            if (type == SS_NODE_SHOWDOWN) {
+                uint32_t hi = s->hinfo[(size_t)trav * Mp + h];
+                v = (hi & SS_LIVE_BIT) && s->cstr[(hi >> 12) & 2047]
+                        ? odometer<T>(s, n, trav, h, opp, k, reach, true, pot, bet) : (T)0;
+            } else {
+                T m = odometer<T>(s, n, trav, h, opp, k, reach, false, pot, bet);
+                v = m * (kind == TK_MASS ? (T)scale[h]
+                         : type == SS_NODE_FOLD ? (trav == nd.player ? (T)(nd.pot - nd.bets[trav]) : -(T)nd.bets[trav])
+                         : (T)s->leaf_values[((size_t)nd.leaf_idx * NP + trav) * s->maxh + h]);
+            }
+            out[h] = v;
+        }
+        return;
+    }
+    // ---- closed forms, k <= 2 ----
+    const bool sd = type == SS_NODE_SHOWDOWN;
+    for (int i = tid; i < SS_NCOMBO; i += nt) {
+        B.M1[i] = 0;
+        if (k == 2) B.M2[i] = 0;
+    }
+    __syncthreads();
+    if (k >= 1) scatter<T>(s, opp[0], n, reach, B.M1, tid, nt);
+    if (k == 2) scatter<T>(s, opp[1], n, reach, B.M2, tid, nt);
+    __syncthreads();
+    if (sd) card_prefix<T>(s, B.M1, k == 2 ? B.M2 : nullptr, B.PC1, B.PC2, B.PCP, B.C1, B.C2, B.CP, tid, nt);
+    else if (k >= 1) card_totals<T>(B.M1, k == 2 ? B.M2 : nullptr, B.C1, B.C2, B.CP, tid, nt);
+    __syncthreads();
+    if (sd && k == 1) scan_sorted<T>(s, B.M1, B.PG, B.red, tid, nt);
+
+    T S1 = 0, S2 = 0, SP = 0;
+    if (k >= 1) S1 = (T)0.5 * tsum52(B.C1);
+    if (k == 2) { S2 = (T)0.5 * tsum52(B.C2); SP = (T)0.5 * tsum52(B.CP); }
+
+    for (int h = tid; h < nh; h += nt) {
+        uint32_t hi = s->hinfo[(size_t)trav * Mp + h];
+        int x = hi & 63, y = (hi >> 6) & 63;
+        int kk = combo_id(x, y);
+        T v;
+        if (!sd) {
+            T m;
+            if (k == 0) m = 1;
+            else {
+                T z1 = S1 - B.C1[x] - B.C1[y] + B.M1[kk];
+                if (k == 1) m = z1;
+                else {
+                    T z2 = S2 - B.C2[x] - B.C2[y] + B.M2[kk];
+                    T zp = SP - B.CP[x] - B.CP[y] + B.M1[kk] * B.M2[kk];
+                    T X = 0;
+                    for (int c = 0; c < 52; ++c) {
+                        if (c == x || c == y) continue;
+                        int kx = combo_id(c, x), ky = combo_id(c, y);
+                        X += (B.C1[c] - B.M1[kx] - B.M1[ky]) * (B.C2[c] - B.M2[kx] - B.M2[ky]);
+                    }
+                    m = z1 * z2 - X + zp;
+                }
+            }
+            T f;
+            if (kind == TK_MASS) f = (T)scale[h];
+            else if (type == SS_NODE_FOLD)                                   // :726-733
+                f = trav == nd.player ? (T)(nd.pot - nd.bets[trav]) : -(T)nd.bets[trav];
+            else f = (T)s->leaf_values[((size_t)nd.leaf_idx * NP + trav) * s->maxh + h];
+            v = f * m;
+        } else {
+            uint32_t hs = (hi & SS_LIVE_BIT) ? s->cstr[kk] : 0u;
+            if (hs == 0) { out[h] = 0; continue; }                            // :861
+            if (k == 0) {
+                v = pot - bet;
+            } else if (k == 1) {                                               // :886-903
+                int lx = s->plt[x * SS_NCOMBO + kk], ly = s->plt[y * SS_NCOMBO + kk];
+                int ex = s->ple[x * SS_NCOMBO + kk], ey = s->ple[y * SS_NCOMBO + kk];
+                T lt = B.PG[s->glt[kk]] - B.PC1[x * 52 + lx] - B.PC1[y * 52 + ly];
+                T le = B.PG[s->gle[kk]] - B.PC1[x * 52 + ex] - B.PC1[y * 52 + ey] + B.M1[kk];
+                T z = S1 - B.C1[x] - B.C1[y] + B.M1[kk];
+                v = pot * lt + pot * (T)0.5 * (le - lt) - bet * z;
+            } else {                                                           // :936-1004, n_opp = 2
+                T h1l = 0, h1e = 0, h2l = 0, h2e = 0, hpl = 0, hpe = 0;
+                T xll = 0, xle = 0, xel = 0, xee = 0, xaa = 0;
+                T a_lx = 0, a_ex = 0, a_ly = 0, a_ey = 0, b_lx = 0, b_ex = 0, b_ly = 0, b_ey = 0;
+                T p_lx = 0, p_ex = 0, p_ly = 0, p_ey = 0;
+                for (int c = 0; c < 52; ++c) {
+                    if (s->clen[c] == 0) continue;
+                    int l = s->plt[c * SS_NCOMBO + kk], e = s->ple[c * SS_NCOMBO + kk];
+                    T al = B.PC1[c * 52 + l], ae = B.PC1[c * 52 + e] - al;
+                    T bl = B.PC2[c * 52 + l], be = B.PC2[c * 52 + e] - bl;
+                    T pl = B.PCP[c * 52 + l], pe = B.PCP[c * 52 + e] - pl;
+                    h1l += al; h1e += ae; h2l += bl; h2e += be; hpl += pl; hpe += pe;
+                    if (c == x) { a_lx = al; a_ex = ae; b_lx = bl; b_ex = be; p_lx = pl; p_ex = pe; continue; }
+                    if (c == y) { a_ly = al; a_ey = ae; b_ly = bl; b_ey = be; p_ly = pl; p_ey = pe; continue; }
+                    T aa = B.C1[c], ba = B.C2[c];
+                    int kx = combo_id(c, x), ky = combo_id(c, y);
+                    uint32_t rx = s->cstr[kx], ry = s->cstr[ky];
+                    T m1x = B.M1[kx], m1y = B.M1[ky], m2x = B.M2[kx], m2y = B.M2[ky];
+                    if (rx < hs) { al -= m1x; bl -= m2x; } else if (rx == hs) { ae -= m1x; be -= m2x; }
+                    if (ry < hs) { al -= m1y; bl -= m2y; } else if (ry == hs) { ae -= m1y; be -= m2y; }
+                    aa -= m1x + m1y;
+                    ba -= m2x + m2y;
+                    xll += al * bl; xle += al * be; xel += ae * bl; xee += ae * be; xaa += aa * ba;
+                }
+                T m1k = B.M1[kk], m2k = B.M2[kk];
+                T Al = (T)0.5 * h1l - a_lx - a_ly, Ae = (T)0.5 * h1e - a_ex - a_ey + m1k;
+                T Bl = (T)0.5 * h2l - b_lx - b_ly, Be = (T)0.5 * h2e - b_ex - b_ey + m2k;
+                T Pl = (T)0.5 * hpl - p_lx - p_ly, Pe = (T)0.5 * hpe - p_ex - p_ey + m1k * m2k;
+                T Aa = S1 - B.C1[x] - B.C1[y] + m1k, Ba = S2 - B.C2[x] - B.C2[y] + m2k;
+                T Pa = SP - B.CP[x] - B.CP[y] + m1k * m2k;
+                T wll = Al * Bl - xll + Pl;
+                T wle = Al * Be - xle;
+                T wel = Ae * Bl - xel;
+                T wee = Ae * Be - xee + Pe;
+                T z = Aa * Ba - xaa + Pa;
+                v = pot * (wll + (wle + wel) * (T)0.5 + wee / (T)3) - bet * z;
+            }
+        }
+        out[h] = v;
+    }
+    __syncthreads();
+}
+
+// LDS terminal buffers for the FP32 sweep (55 KB)
+struct TermShared {
+    float M1[SS_NCOMBO], M2[SS_NCOMBO];
+    float PC1[52 * 52], PC2[52 * 52], PCP[52 * 52];
+    float PG[1328];
+    float C1[52], C2[52], CP[52];
+    float red[SS_BLOCK];
+};
+
+__device__ __forceinline__ TermBuf<float> lds_buf(TermShared &t) {
+    TermBuf<float> b;
+    b.M1 = t.M1; b.M2 = t.M2; b.PC1 = t.PC1; b.PC2 = t.PC2; b.PCP = t.PCP;
+    b.PG = t.PG; b.C1 = t.C1; b.C2 = t.C2; b.CP = t.CP; b.red = t.red;
+    return b;
+}
+
+// FP64 buffers live in global scratch: one slice per workgroup
+#define XSCRATCH_DOUBLES (2 * SS_NCOMBO + 3 * 2704 + 1328 + 3 * 52 + SS_BLOCK)
+__device__ __forceinline__ TermBuf<double> global_buf(double *base) {
+    TermBuf<double> b;
+    b.M1 = base; b.M2 = b.M1 + SS_NCOMBO;
+    b.PC1 = b.M2 + SS_NCOMBO; b.PC2 = b.PC1 + 2704; b.PCP = b.PC2 + 2704;
+    b.PG = b.PCP + 2704;
+    b.C1 = b.PG + 1328; b.C2 = b.C1 + 52; b.CP = b.C2 + 52;
+    b.red = b.CP + 52;
+    return b;
+}
+
+// ---------------------------------------------------------------------------
+// plain form: one subgame, level-synchronous launches
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(SS_BLOCK) k_forward(const SlotDesc *__restrict__ s, int level, int trav) {
+    forward_items(s, level, trav, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+}
+
+__global__ void __launch_bounds__(SS_BLOCK) k_terminals(const SlotDesc *__restrict__ s, int trav) {
+    __shared__ TermShared sh;
+    TermBuf<float> b = lds_buf(sh);
+    for (int j = blockIdx.x; j < s->T; j += gridDim.x) {
+        int n = s->term[j];
+        terminal_node<float>(s, n, trav, s->reach, s->cfv + (size_t)n * s->Mp, b, TK_NODE, nullptr,
+                             threadIdx.x, blockDim.x);
+    }
+}
+
+__global__ void __launch_bounds__(SS_BLOCK) k_backward(const SlotDesc *__restrict__ s, int level, int trav) {
+    int iter = s->iter + 1;
+    backward_items(s, level, trav, iter, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+}
+
+__global__ void __launch_bounds__(SS_BLOCK) k_root_sigma(const SlotDesc *__restrict__ s) {
+    root_sigma_items(s, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+}
+
+__global__ void k_bump(SlotDesc *s) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) s->iter += 1;
+}
+
+// ---------------------------------------------------------------------------
+// batched form: one subgame per workgroup, `iters` iterations per launch
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(SS_BLOCK) k_batched(SlotDesc *__restrict__ slots, int nslots, int iters) {
+    __shared__ TermShared sh;
+    TermBuf<float> b = lds_buf(sh);
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int si = blockIdx.x; si < nslots; si += gridDim.x) {
+        SlotDesc *s = slots + si;
+        if (!s->active) continue;
+        int it = s->iter;
+        const int end = s->iter_end, NP = s->NP, L = s->L;
+        for (int rep = 0; rep < iters && it < end; ++rep) {
+            ++it;
+            for (int trav = 0; trav < NP; ++trav) {
+                if (trav == NP - 1 && (it == s->final_iter || (s->snap_every > 0 && it % s->snap_every == 0))) {
+                    root_sigma_items(s, tid, nt);
+                    __syncthreads();
+                }
+                for (int l = 0; l < L; ++l) {
+                    forward_items(s, l, trav, tid, nt);
+                    __syncthreads();
+                }
+                for (int j = 0; j < s->T; ++j) {
+                    int n = s->term[j];
+                    terminal_node<float>(s, n, trav, s->reach, s->cfv + (size_t)n * s->Mp, b, TK_NODE, nullptr,
+                                         tid, nt);
+                    __syncthreads();
+                }
+                for (int l = L - 1; l >= 0; --l) {
+                    backward_items(s, l, trav, it, tid, nt);
+                    __syncthreads();
+                }
+            }
+            if (tid == 0) s->iter = it;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 best response / exploitability
+// ---------------------------------------------------------------------------
+
+// average strategy, normalised as the host does in street_solve.cu:1597-1610
+__device__ __forceinline__ void avg_items(const SlotDesc *__restrict__ s, int begin, int stride) {
+    const int Mp = s->Mp;
+    const int total = s->D * Mp;
+    for (int it = begin; it < total; it += stride) {
+        int di = it / Mp, h = it - di * Mp;
+        int n = s->dec_lvl[di];
+        const DevNode &nd = s->nodes[n];
+        int na = nd.nchild;
+        float sum = 0.f;
+        for (int a = 0; a < na; ++a) {
+            float v = s->SS[(size_t)(nd.act_base + a) * Mp + h];
+            sum += v > 0.f ? v : 0.f;
+        }
+        float inv = sum > 0.f ? 1.0f / sum : 1.0f / (float)na;
+        for (int a = 0; a < na; ++a) {
+            float v = s->SS[(size_t)(nd.act_base + a) * Mp + h];
+            v = v > 0.f ? v : 0.f;
+            s->avg[(size_t)(nd.act_base + a) * Mp + h] = sum > 0.f ? v * inv : inv;
+        }
+    }
+}
+
+// reach of every player under the average strategy
+__device__ __forceinline__ void xforward_items(const SlotDesc *__restrict__ s, int level, int begin, int stride) {
+    const int Mp = s->Mp, N = s->N, NP = s->NP;
+    const int lo = s->dec_off[level];
+    const int total = (s->dec_off[level + 1] - lo) * Mp;
+    for (int it = begin; it < total; it += stride) {
+        int ni = it / Mp, h = it - ni * Mp;
+        int n = s->dec_lvl[lo + ni];
+        const DevNode &nd = s->nodes[n];
+        for (int q = 0; q < NP; ++q) {
+            if (h >= s->nh[q]) continue;
+            double *rr = s->xreach + (size_t)q * N * Mp;
+            double pr = rr[(size_t)n * Mp + h];
+            for (int a = 0; a < nd.nchild; ++a) {
+                int c = s->children[nd.first_child + a];
+                rr[(size_t)c * Mp + h] =
+                    q == nd.player ? pr * (double)s->avg[(size_t)(nd.act_base + a) * Mp + h] : pr;
+            }
+        }
+    }
+}
+
+// best-response (max) and profile-value (avg-weighted) backup for player p
+__device__ __forceinline__ void xbackward_items(const SlotDesc *__restrict__ s, int level, int p, int begin,
+                                                int stride) {
+    const int Mp = s->Mp;
+    const int lo = s->dec_off[level];
+    const int total = (s->dec_off[level + 1] - lo) * Mp;
+    for (int it = begin; it < total; it += stride) {
+        int ni = it / Mp, h = it - ni * Mp;
+        if (h >= s->nh[p]) continue;
+        int n = s->dec_lvl[lo + ni];
+        const DevNode &nd = s->nodes[n];
+        double e = 0, b = 0;
+        for (int a = 0; a < nd.nchild; ++a) {
+            int c = s->children[nd.first_child + a];
+            double ce = s->xev[(size_t)c * Mp + h], cb = s->xbr[(size_t)c * Mp + h];
+            if (nd.player == p) {
+                e += (double)s->avg[(size_t)(nd.act_base + a) * Mp + h] * ce;
+                b = (a == 0 || cb > b) ? cb : b;
+            } else {
+                e += ce;
+                b += cb;
+            }
+        }
+        s->xev[(size_t)n * Mp + h] = e;
+        s->xbr[(size_t)n * Mp + h] = b;
+    }
+}
+
+// sum_h w_p(h) * (br - ev) and sum_h w_p(h) * ev at the root
+__device__ void xreduce_block(const SlotDesc *__restrict__ s, int p, double *red, int tid, int nt) {
+    const int Mp = s->Mp;
+    double g = 0, v = 0;
+    for (int h = tid; h < s->nh[p]; h += nt) {
+        double w = (double)s->weights[(size_t)p * Mp + h];
+        g += w * (s->xbr[h] - s->xev[h]);
+        v += w * s->xev[h];
+    }
+    red[tid] = g;
+    red[nt + tid] = v;
+    __syncthreads();
+    for (int off = nt / 2; off > 0; off >>= 1) {
+        if (tid < off) {
+            red[tid] += red[tid + off];
+            red[nt + tid] += red[nt + tid + off];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        s->xout[p] = red[0];
+        s->xout[s->NP + p] = red[nt];
+    }
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(SS_BLOCK) k_avg(const SlotDesc *__restrict__ s) {
+    avg_items(s, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+}
+
+__global__ void __launch_bounds__(SS_BLOCK) k_xforward(const SlotDesc *__restrict__ s, int level) {
+    xforward_items(s, level, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+}
+
+__global__ void __launch_bounds__(SS_BLOCK) k_xterminals(const SlotDesc *__restrict__ s, int p, double *scratch) {
+    TermBuf<double> b = global_buf(scratch + (size_t)blockIdx.x * XSCRATCH_DOUBLES);
+    for (int j = blockIdx.x; j < s->T; j += gridDim.x) {
+        int n = s->term[j];
+        terminal_node<double>(s, n, p, s->xreach, s->xev + (size_t)n * s->Mp, b, TK_NODE, nullptr, threadIdx.x,
+                              blockDim.x);
+        for (int h = threadIdx.x; h < s->nh[p]; h += blockDim.x)
+            s->xbr[(size_t)n * s->Mp + h] = s->xev[(size_t)n * s->Mp + h];
+    }
+}
+
+__global__ void __launch_bounds__(SS_BLOCK) k_xbackward(const SlotDesc *__restrict__ s, int level, int p) {
+    xbackward_items(s, level, p, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+}
+
+__global__ void __launch_bounds__(SS_BLOCK) k_xreduce(const SlotDesc *__restrict__ s, int p) {
+    __shared__ double red[2 * SS_BLOCK];
+    xreduce_block(s, p, red, threadIdx.x, blockDim.x);
+}
+
+// joint deal mass at the root with raw weights: sum_h w_0(h) * mass_0(h)
+__global__ void __launch_bounds__(SS_BLOCK) k_xjoint(const SlotDesc *__restrict__ s, double *scratch) {
+    __shared__ double red[SS_BLOCK];
+    TermBuf<double> b = global_buf(scratch);
+    // xreach rows of node 0 hold the raw weights; the per-hand mass lands in xev row 0
+    // with scale = weights of player 0 restricted to live hands
+    terminal_node<double>(s, 0, 0, s->xreach, s->xev, b, TK_MASS, s->weights, threadIdx.x, blockDim.x);
+    double acc = 0;
+    for (int h = threadIdx.x; h < s->nh[0]; h += blockDim.x)
+        if (s->hinfo[h] & SS_LIVE_BIT) acc += s->xev[h];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) s->xout[2 * s->NP] = red[0];
+}
+
+// whole exploitability evaluation of one slot by one workgroup (batched checks)
+__global__ void __launch_bounds__(SS_BLOCK) k_batched_expl(SlotDesc *__restrict__ slots, int nslots,
+                                                           double *scratch, const int *want) {
+    __shared__ double red[2 * SS_BLOCK];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int si = blockIdx.x; si < nslots; si += gridDim.x) {
+        SlotDesc *s = slots + si;
+        if (!s->active || !want[si]) continue;
+        TermBuf<double> b = global_buf(scratch + (size_t)blockIdx.x * XSCRATCH_DOUBLES);
+        avg_items(s, tid, nt);
+        __syncthreads();
+        for (int l = 0; l < s->L; ++l) {
+            xforward_items(s, l, tid, nt);
+            __syncthreads();
+        }
+        for (int p = 0; p < s->NP; ++p) {
+            for (int j = 0; j < s->T; ++j) {
+                int n = s->term[j];
+                terminal_node<double>(s, n, p, s->xreach, s->xev + (size_t)n * s->Mp, b, TK_NODE, nullptr, tid, nt);
+                for (int h = tid; h < s->nh[p]; h += nt) s->xbr[(size_t)n * s->Mp + h] = s->xev[(size_t)n * s->Mp + h];
+                __syncthreads();
+            }
+            for (int l = s->L - 1; l >= 0; --l) {
+                xbackward_items(s, l, p, tid, nt);
+                __syncthreads();
+            }
+            xreduce_block(s, p, red, tid, nt);
+        }
+        terminal_node<double>(s, 0, 0, s->xreach, s->xev, b, TK_MASS, s->weights, tid, nt);
+        double acc = 0;
+        for (int h = tid; h < s->nh[0]; h += nt)
+            if (s->hinfo[h] & SS_LIVE_BIT) acc += s->xev[h];
+        red[tid] = acc;
+        __syncthreads();
+        for (int off = nt / 2; off > 0; off >>= 1) {
+            if (tid < off) red[tid] += red[tid + off];
+            __syncthreads();
+        }
+        if (tid == 0) s->xout[2 * s->NP] = red[0];
+        __syncthreads();
+    }
+}
+
+}  // namespace ss
+
+// ===========================================================================
+// host launchers
+// ===========================================================================
+#include "ss_internal.h"
+
+namespace ss {
+
+static inline int grid_for(int64_t items, int cap = 8192) {
+    int64_t g = (items + SS_BLOCK - 1) / SS_BLOCK;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(g, cap));
+}
+
+int launch_combo_ranks(const int *d_board, uint32_t *d_out, hipStream_t st) {
+    hipLaunchKernelGGL(k_combo_ranks, dim3(6), dim3(SS_BLOCK), 0, st, d_board, d_out);
+    SS_HIP(hipGetLastError());
+    return SS_OK;
+}
+
+int plain_run(Slot &s, int iters, int final_iter, hipStream_t st, double *kernel_ms) {
+    const Plan &pl = s.plan;
+    SlotDesc *d = s.d_desc;
+    hipEvent_t e0, e1;
+    SS_HIP(hipEventCreate(&e0));
+    SS_HIP(hipEventCreate(&e1));
+    SS_HIP(hipEventRecord(e0, st));
+    for (int k = 0; k < iters; ++k) {
+        int it = s.desc.iter + 1;
+        for (int t = 0; t < pl.NP; ++t) {
+            if (it == final_iter && t == pl.NP - 1)
+                hipLaunchKernelGGL(k_root_sigma, dim3(grid_for(pl.Mp)), dim3(SS_BLOCK), 0, st, d);
+            for (int l = 0; l < pl.L; ++l) {
+                int cnt = pl.dec_off[l + 1] - pl.dec_off[l];
+                if (cnt > 0)
+                    hipLaunchKernelGGL(k_forward, dim3(grid_for((int64_t)cnt * pl.Mp)), dim3(SS_BLOCK), 0, st, d, l, t);
+            }
+            if (pl.T > 0) hipLaunchKernelGGL(k_terminals, dim3(pl.T), dim3(SS_BLOCK), 0, st, d, t);
+            for (int l = pl.L - 1; l >= 0; --l) {
+                int cnt = pl.dec_off[l + 1] - pl.dec_off[l];
+                if (cnt > 0)
+                    hipLaunchKernelGGL(k_backward, dim3(grid_for((int64_t)cnt * pl.Mp)), dim3(SS_BLOCK), 0, st, d, l, t);
+            }
+        }
+        hipLaunchKernelGGL(k_bump, dim3(1), dim3(1), 0, st, d);
+        s.desc.iter = it;
+    }
+    SS_HIP(hipGetLastError());
+    SS_HIP(hipEventRecord(e1, st));
+    SS_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    SS_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (kernel_ms) *kernel_ms += ms;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return SS_OK;
+}
+
+int plain_exploitability(Slot &s, hipStream_t st, double *expl_pct, double *gains) {
+    const Plan &pl = s.plan;
+    int blocks = std::max(1, std::min(pl.T, 64));
+    int rc = s.alloc_expl(st, blocks);
+    if (rc) return rc;
+    SlotDesc *d = s.d_desc;
+    hipLaunchKernelGGL(k_avg, dim3(grid_for((int64_t)pl.D * pl.Mp)), dim3(SS_BLOCK), 0, st, d);
+    for (int l = 0; l < pl.L; ++l) {
+        int cnt = pl.dec_off[l + 1] - pl.dec_off[l];
+        if (cnt > 0) hipLaunchKernelGGL(k_xforward, dim3(grid_for((int64_t)cnt * pl.Mp)), dim3(SS_BLOCK), 0, st, d, l);
+    }
+    for (int p = 0; p < pl.NP; ++p) {
+        if (pl.T > 0) hipLaunchKernelGGL(k_xterminals, dim3(blocks), dim3(SS_BLOCK), 0, st, d, p, s.xscratch);
+        for (int l = pl.L - 1; l >= 0; --l) {
+            int cnt = pl.dec_off[l + 1] - pl.dec_off[l];
+            if (cnt > 0)
+                hipLaunchKernelGGL(k_xbackward, dim3(grid_for((int64_t)cnt * pl.Mp)), dim3(SS_BLOCK), 0, st, d, l, p);
+        }
+        hipLaunchKernelGGL(k_xreduce, dim3(1), dim3(SS_BLOCK), 0, st, d, p);
+    }
+    hipLaunchKernelGGL(k_xjoint, dim3(1), dim3(SS_BLOCK), 0, st, d, s.xscratch);
+    SS_HIP(hipGetLastError());
+    double xout[2 * SS_MAX_PLAYERS + 1];
+    SS_HIP(hipMemcpyAsync(xout, s.desc.xout, sizeof(double) * (2 * pl.NP + 1), hipMemcpyDeviceToHost, st));
+    SS_HIP(hipStreamSynchronize(st));
+    *expl_pct = expl_from_xout(xout, pl.NP, pl.starting_pot, gains);
+    return SS_OK;
+}
+
+int launch_batched(SlotDesc *d_slots, int nslots, int iters, int grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_batched, dim3(grid), dim3(SS_BLOCK), 0, st, d_slots, nslots, iters);
+    SS_HIP(hipGetLastError());
+    return SS_OK;
+}
+
+int launch_batched_expl(SlotDesc *d_slots, int nslots, double *scratch, const int *d_want, int grid,
+                        hipStream_t st) {
+    hipLaunchKernelGGL(k_batched_expl, dim3(grid), dim3(SS_BLOCK), 0, st, d_slots, nslots, scratch, d_want);
+    SS_HIP(hipGetLastError());
+    return SS_OK;
+}
+
+// ---- test entry points -----------------------------------------------------
+int dbg_eval7_device(const int *cards, int n, uint32_t *out) {
+    int *dc = nullptr;
+    uint32_t *dv = nullptr;
+    SS_HIP(hipMalloc(&dc, sizeof(int) * 7 * std::max(1, n)));
+    SS_HIP(hipMalloc(&dv, sizeof(uint32_t) * std::max(1, n)));
+    SS_HIP(hipMemcpy(dc, cards, sizeof(int) * 7 * n, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_eval7, dim3(grid_for(n, 1024)), dim3(SS_BLOCK), 0, 0, dc, n, dv);
+    SS_HIP(hipGetLastError());
+    SS_HIP(hipMemcpy(out, dv, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+    (void)hipFree(dc);
+    (void)hipFree(dv);
+    return SS_OK;
+}
+
+int dbg_combo_ranks_device(const int *board, uint32_t *out) {
+    int *db = nullptr;
+    uint32_t *dv = nullptr;
+    SS_HIP(hipMalloc(&db, sizeof(int) * 5));
+    SS_HIP(hipMalloc(&dv, sizeof(uint32_t) * SS_NCOMBO));
+    SS_HIP(hipMemcpy(db, board, sizeof(int) * 5, hipMemcpyHostToDevice));
+    int rc = launch_combo_ranks(db, dv, 0);
+    if (rc) return rc;
+    SS_HIP(hipMemcpy(out, dv, sizeof(uint32_t) * SS_NCOMBO, hipMemcpyDeviceToHost));
+    (void)hipFree(db);
+    (void)hipFree(dv);
+    return SS_OK;
+}
+
+int dbg_eval_census_device(unsigned long long counts[10], unsigned long long *sum) {
+    unsigned long long *dc = nullptr;
+    SS_HIP(hipMalloc(&dc, sizeof(unsigned long long) * 11));
+    SS_HIP(hipMemset(dc, 0, sizeof(unsigned long long) * 11));
+    // one block per (c0, c1) pair: 1326 blocks
+    hipLaunchKernelGGL(k_eval_census, dim3(SS_NCOMBO), dim3(64), 0, 0, dc, dc + 10);
+    SS_HIP(hipGetLastError());
+    unsigned long long h[11];
+    SS_HIP(hipMemcpy(h, dc, sizeof(h), hipMemcpyDeviceToHost));
+    for (int i = 0; i < 10; ++i) counts[i] = h[i];
+    *sum = h[10];
+    (void)hipFree(dc);
+    return SS_OK;
+}
+
+}  // namespace ss

@@ -1,0 +1,552 @@
+// ss_runtime.hip -- host runtime: problem intake, device plan, slot arenas,
+// the plain driver, exploitability and output extraction.
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <numeric>
+
+#include "ss_internal.h"
+
+namespace ss {
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local std::string g_err;
+
+int set_error(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    if (log_level() > 0) fprintf(stderr, "[street_solve] error %d: %s\n", code, buf);
+    return code;
+}
+
+void clear_error() { g_err.clear(); }
+const char *last_error_str() { return g_err.c_str(); }
+
+int log_level() {
+    static int lvl = [] {
+        const char *e = getenv("SS_LOG");
+        return e ? atoi(e) : 0;
+    }();
+    return lvl;
+}
+
+// ---------------------------------------------------------------------------
+// problem intake
+// ---------------------------------------------------------------------------
+HostProblem from_legacy(const SSTreeData *td) {
+    HostProblem hp;
+    hp.tree = td;
+    hp.np = td->num_players;
+    for (int p = 0; p < hp.np && p < SS_MAX_PLAYERS; ++p) {
+        hp.nh[p] = td->num_hands[p];
+        hp.hands[p] = &td->hands[p][0][0];
+        hp.weights[p] = &td->weights[p][0];
+    }
+    hp.leaf_values = td->leaf_values;
+    hp.frozen = td->frozen_action;
+    return hp;
+}
+
+int from_ex(const SSProblem *p, HostProblem *out) {
+    if (!p || !p->tree) return set_error(SS_ERR_ARG, "problem or problem->tree is NULL");
+    HostProblem hp;
+    hp.tree = p->tree;
+    hp.np = p->tree->num_players;
+    if (hp.np < 2 || hp.np > SS_MAX_PLAYERS) return set_error(SS_ERR_ARG, "tree has %d players", hp.np);
+    for (int q = 0; q < hp.np; ++q) {
+        hp.nh[q] = p->num_hands[q];
+        hp.hands[q] = p->hands[q];
+        hp.weights[q] = p->weights[q];
+        if (hp.nh[q] < 0 || hp.nh[q] > SS_EX_MAX_HANDS)
+            return set_error(SS_ERR_ARG, "player %d has %d hands (max %d)", q, hp.nh[q], SS_EX_MAX_HANDS);
+        if (hp.nh[q] > 0 && (!hp.hands[q] || !hp.weights[q]))
+            return set_error(SS_ERR_ARG, "player %d hands/weights NULL", q);
+    }
+    hp.leaf_values = p->leaf_values;
+    hp.frozen = p->frozen_action;
+    *out = hp;
+    return SS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// plan
+// ---------------------------------------------------------------------------
+static inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
+
+int make_plan(const HostProblem &hp, Plan &pl) {
+    const SSTreeData *td = hp.tree;
+    if (!td || !td->nodes || !td->children || td->num_nodes < 1) return set_error(SS_ERR_ARG, "empty tree");
+    const int N = td->num_nodes, NP = hp.np;
+    if (NP < 2 || NP > SS_MAX_PLAYERS) return set_error(SS_ERR_ARG, "num_players %d", NP);
+    int maxh = 0;
+    for (int p = 0; p < NP; ++p) maxh = std::max(maxh, hp.nh[p]);
+    if (maxh == 0) return set_error(SS_ERR_GENERIC, "no hands for any player");   // street_solve.cu:1306
+    pl = Plan();
+    pl.N = N;
+    pl.NP = NP;
+    pl.maxh = maxh;
+    pl.Mp = round_up(maxh, 64);
+    pl.is_river = td->is_river ? 1 : 0;
+    pl.num_leaves = td->num_leaves;
+    pl.num_board = td->num_board;
+    for (int i = 0; i < 5; ++i) pl.board[i] = td->board[i];
+    pl.starting_pot = td->starting_pot;
+    for (int p = 0; p < NP; ++p) pl.nh[p] = hp.nh[p];
+
+    // nodes, depths
+    std::vector<int> depth(N, 0), bfs;
+    bfs.reserve(N);
+    bfs.push_back(0);
+    for (size_t i = 0; i < bfs.size(); ++i) {
+        const SSNode &nd = td->nodes[bfs[i]];
+        if (nd.num_children < 0 || nd.first_child < 0 ||
+            nd.first_child + nd.num_children > td->num_children_total)
+            return set_error(SS_ERR_ARG, "node %d has a bad child range", bfs[i]);
+        for (int a = 0; a < nd.num_children; ++a) {
+            int c = td->children[nd.first_child + a];
+            if (c <= 0 || c >= N) return set_error(SS_ERR_ARG, "child id %d out of range", c);
+            depth[c] = depth[bfs[i]] + 1;
+            bfs.push_back(c);
+            if ((int)bfs.size() > N) return set_error(SS_ERR_ARG, "tree is not a tree (cycle or shared child)");
+        }
+    }
+    if ((int)bfs.size() != N) return set_error(SS_ERR_ARG, "unreachable nodes in tree");
+    int maxd = 0;
+    for (int i = 0; i < N; ++i) maxd = std::max(maxd, depth[i]);
+
+    pl.nodes.resize(N);
+    int A = 0, D = 0;
+    pl.dec_index.assign(N, -1);
+    for (int i = 0; i < N; ++i) {
+        const SSNode &s = td->nodes[i];
+        DevNode &d = pl.nodes[i];
+        std::memset(&d, 0, sizeof(d));
+        d.type = s.type;
+        d.player = s.player;
+        d.nchild = s.num_children;
+        d.first_child = s.first_child;
+        d.pot = s.pot;
+        d.leaf_idx = s.leaf_idx;
+        d.act_base = -1;
+        d.depth = depth[i];
+        for (int q = 0; q < SS_MAX_PLAYERS; ++q) {
+            d.bets[q] = s.bets[q];
+            if (q < NP && s.active_players[q]) d.active_mask |= 1 << q;
+        }
+        if (s.type == SS_NODE_DECISION) {
+            if (s.num_children < 1 || s.num_children > SS_MAX_ACTIONS)
+                return set_error(SS_ERR_UNSUPPORTED, "decision node %d has %d children (max %d)", i, s.num_children,
+                                 SS_MAX_ACTIONS);
+            d.act_base = A;
+            A += s.num_children;
+            pl.dec_index[i] = D++;
+        } else if (s.num_children != 0) {
+            return set_error(SS_ERR_ARG, "terminal node %d has children", i);
+        }
+        if (s.type == SS_NODE_LEAF && s.leaf_idx >= td->num_leaves)
+            return set_error(SS_ERR_ARG, "leaf_idx %d >= num_leaves %d", s.leaf_idx, td->num_leaves);
+    }
+    pl.A = A;
+    pl.D = D;
+    pl.root_player = td->nodes[0].player;
+    pl.children.assign(td->children, td->children + td->num_children_total);
+
+    // decision nodes per level, terminals
+    pl.L = maxd + 1;
+    pl.dec_off.assign(pl.L + 1, 0);
+    for (int i = 0; i < N; ++i)
+        if (pl.nodes[i].type == SS_NODE_DECISION) pl.dec_off[depth[i] + 1]++;
+    for (int l = 0; l < pl.L; ++l) pl.dec_off[l + 1] += pl.dec_off[l];
+    pl.dec_lvl.assign(std::max(D, 1), 0);
+    {
+        std::vector<int> fill(pl.dec_off.begin(), pl.dec_off.end() - 1);
+        for (int i = 0; i < N; ++i)
+            if (pl.nodes[i].type == SS_NODE_DECISION) pl.dec_lvl[fill[depth[i]]++] = i;
+    }
+    for (int i = 0; i < N; ++i)
+        if (pl.nodes[i].type != SS_NODE_DECISION) pl.term.push_back(i);
+    pl.T = (int)pl.term.size();
+    if (pl.term.empty()) pl.term.push_back(0);
+
+    // hands
+    const int Mp = pl.Mp;
+    pl.hinfo.assign((size_t)NP * Mp, 0u);
+    pl.weights.assign((size_t)NP * Mp, 0.f);
+    bool board_mask[52] = {false};
+    for (int b = 0; b < td->num_board; ++b) {
+        if (td->board[b] < 0 || td->board[b] > 51) return set_error(SS_ERR_ARG, "board card out of range");
+        board_mask[td->board[b]] = true;
+    }
+    for (int p = 0; p < NP; ++p)
+        for (int h = 0; h < hp.nh[p]; ++h) {
+            int a = hp.hands[p][2 * h], b = hp.hands[p][2 * h + 1];
+            if (a < 0 || a > 51 || b < 0 || b > 51 || a == b)
+                return set_error(SS_ERR_ARG, "player %d hand %d has bad cards (%d, %d)", p, h, a, b);
+            uint32_t live = (!board_mask[a] && !board_mask[b]) ? SS_LIVE_BIT : 0u;
+            pl.hinfo[(size_t)p * Mp + h] = (uint32_t)a | ((uint32_t)b << 6) | ((uint32_t)combo_id(a, b) << 12) | live;
+            pl.weights[(size_t)p * Mp + h] = hp.weights[p][h];
+        }
+
+    // terminal opponent counts: 3+ opponents take the odometer path
+    int64_t odo_work = 0;
+    for (int j = 0; j < pl.T; ++j) {
+        const DevNode &nd = pl.nodes[pl.term[j]];
+        for (int t = 0; t < NP; ++t) {
+            int k = 0;
+            int64_t w = 1;
+            for (int q = 0; q < NP; ++q) {
+                if (q == t) continue;
+                if (nd.type != SS_NODE_FOLD && !((nd.active_mask >> q) & 1)) continue;
+                ++k;
+                w *= std::max(1, hp.nh[q]);
+            }
+            if (k >= 3) {
+                pl.odometer = 1;
+                odo_work = std::max(odo_work, w);
+            }
+        }
+    }
+    if (pl.odometer && odo_work > (int64_t)1 << 24)
+        return set_error(SS_ERR_UNSUPPORTED,
+                         "terminals with 3+ opponents need the exact odometer (%lld hand tuples per hand); "
+                         "ranges this large are supported for up to 2 opponents per terminal",
+                         (long long)odo_work);
+
+    // leaves
+    pl.has_leaf = (td->num_leaf_nodes > 0 || pl.num_leaves > 0) && hp.leaf_values ? 1 : 0;
+    if (pl.has_leaf) {
+        pl.leaf_src = hp.leaf_values;
+        pl.leaf_count = (size_t)pl.num_leaves * NP * maxh;
+    }
+    pl.frozen_src = hp.frozen;
+    if (pl.frozen_src)
+        for (int i = 0; i < N; ++i) {
+            int fa = pl.frozen_src[i];
+            if (fa >= 0 && (pl.nodes[i].type != SS_NODE_DECISION || fa >= pl.nodes[i].nchild))
+                return set_error(SS_ERR_ARG, "frozen_action[%d] = %d is not an action of that node", i, fa);
+        }
+
+    // algorithmic bytes of one iteration (4-byte state, M = max hands)
+    int64_t Ap[6] = {0, 0, 0, 0, 0, 0};
+    for (int i = 0; i < N; ++i)
+        if (pl.nodes[i].type == SS_NODE_DECISION && pl.nodes[i].player >= 0 && pl.nodes[i].player < NP)
+            Ap[pl.nodes[i].player] += pl.nodes[i].nchild;
+    int64_t nleaf = 0;
+    for (int i = 0; i < N; ++i) nleaf += pl.nodes[i].type == SS_NODE_LEAF;
+    int64_t rows = 0;
+    for (int t = 0; t < NP; ++t) {
+        rows += 2 * A;                                    // regret match: read R, write sigma
+        for (int rp = 0; rp < NP; ++rp)
+            if (rp != t) rows += D + A + Ap[rp];          // reach: parent, children, sigma of rp
+        rows += (int64_t)(NP - 1) * pl.T + pl.T + (pl.has_leaf ? nleaf : 0);   // terminals
+        rows += A + D + 5 * Ap[t];                        // backup
+    }
+    pl.algo_bytes_iter = rows * 4 * (int64_t)maxh;
+    return SS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// river tables from combo ranks
+// ---------------------------------------------------------------------------
+void build_river_tables(const uint32_t *cstr, RiverTables &rt) {
+    rt.cstr.assign(cstr, cstr + SS_NCOMBO);
+    rt.glt.assign(SS_NCOMBO, 0);
+    rt.gle.assign(SS_NCOMBO, 0);
+    rt.clist.assign(52 * 52, 0);
+    rt.clen.assign(52, 0);
+    rt.plt.assign(52 * SS_NCOMBO, 0);
+    rt.ple.assign(52 * SS_NCOMBO, 0);
+    std::vector<int> live;
+    for (int k = 0; k < SS_NCOMBO; ++k)
+        if (cstr[k]) live.push_back(k);
+    std::stable_sort(live.begin(), live.end(), [&](int a, int b) { return cstr[a] < cstr[b]; });
+    rt.ng = (int)live.size();
+    rt.gsort.assign(std::max(rt.ng, 1), 0);
+    std::vector<uint32_t> sorted(rt.ng);
+    for (int i = 0; i < rt.ng; ++i) {
+        rt.gsort[i] = (int16_t)live[i];
+        sorted[i] = cstr[live[i]];
+    }
+    for (int k = 0; k < SS_NCOMBO; ++k) {
+        uint32_t s = cstr[k];
+        rt.glt[k] = (int16_t)(std::lower_bound(sorted.begin(), sorted.end(), s) - sorted.begin());
+        rt.gle[k] = (int16_t)(std::upper_bound(sorted.begin(), sorted.end(), s) - sorted.begin());
+    }
+    for (int c = 0; c < 52; ++c) {
+        std::vector<int> part;
+        for (int d = 0; d < 52; ++d)
+            if (d != c && cstr[combo_id(c, d)]) part.push_back(d);
+        std::stable_sort(part.begin(), part.end(),
+                         [&](int a, int b) { return cstr[combo_id(c, a)] < cstr[combo_id(c, b)]; });
+        rt.clen[c] = (uint8_t)part.size();
+        std::vector<uint32_t> ps(part.size());
+        for (size_t j = 0; j < part.size(); ++j) {
+            rt.clist[c * 52 + j] = (uint8_t)part[j];
+            ps[j] = cstr[combo_id(c, part[j])];
+        }
+        for (int k = 0; k < SS_NCOMBO; ++k) {
+            uint32_t s = cstr[k];
+            rt.plt[c * SS_NCOMBO + k] = (uint8_t)(std::lower_bound(ps.begin(), ps.end(), s) - ps.begin());
+            rt.ple[c * SS_NCOMBO + k] = (uint8_t)(std::upper_bound(ps.begin(), ps.end(), s) - ps.begin());
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// slot arena
+// ---------------------------------------------------------------------------
+struct ArenaLayout {
+    size_t off = 0;
+    size_t take(size_t bytes) {
+        size_t o = off;
+        off += (bytes + 255) / 256 * 256;
+        return o;
+    }
+};
+
+int launch_combo_ranks(const int *d_board, uint32_t *d_out, hipStream_t st);
+
+Slot::~Slot() {
+    if (arena) (void)hipFree(arena);
+    if (xarena) (void)hipFree(xarena);
+    if (d_desc) (void)hipFree(d_desc);
+}
+
+int64_t estimate_footprint(const HostProblem &hp) {
+    Plan pl;
+    if (make_plan(hp, pl) != SS_OK) return -1;
+    size_t Mp = pl.Mp;
+    int64_t b = 0;
+    b += (int64_t)pl.N * sizeof(DevNode) + pl.children.size() * 4 + (pl.D + pl.L + 1 + pl.T) * 4;
+    b += (int64_t)pl.NP * Mp * 8 + pl.leaf_count * 4 + pl.N * 4;
+    b += (int64_t)2 * pl.A * Mp * 4 + (int64_t)pl.N * Mp * 4 + (int64_t)pl.NP * pl.N * Mp * 4 + 8 * Mp * 4;
+    b += pl.is_river ? (SS_NCOMBO * 4 + 3 * SS_NCOMBO * 2 + 52 * 53 + 2 * 52 * SS_NCOMBO) : 0;
+    // exploitability buffers
+    b += (int64_t)pl.A * Mp * 4 + (int64_t)(pl.NP + 2) * pl.N * Mp * 8;
+    return b + 64 * 1024;
+}
+
+int Slot::setup(const HostProblem &hp, hipStream_t st) {
+    int rc = make_plan(hp, plan);
+    if (rc) return rc;
+    Plan &pl = plan;
+    const size_t Mp = pl.Mp;
+    ArenaLayout L;
+    size_t o_nodes = L.take(sizeof(DevNode) * pl.N);
+    size_t o_child = L.take(sizeof(int) * std::max<size_t>(1, pl.children.size()));
+    size_t o_declvl = L.take(sizeof(int) * pl.dec_lvl.size());
+    size_t o_decoff = L.take(sizeof(int) * pl.dec_off.size());
+    size_t o_term = L.take(sizeof(int) * pl.term.size());
+    size_t o_hinfo = L.take(sizeof(uint32_t) * pl.NP * Mp);
+    size_t o_w = L.take(sizeof(float) * pl.NP * Mp);
+    size_t o_leaf = L.take(sizeof(float) * std::max<size_t>(1, pl.leaf_count));
+    size_t o_frozen = L.take(sizeof(int) * pl.N);
+    size_t o_R = L.take(sizeof(float) * (size_t)pl.A * Mp);
+    size_t o_SS = L.take(sizeof(float) * (size_t)pl.A * Mp);
+    size_t o_cfv = L.take(sizeof(float) * (size_t)pl.N * Mp);
+    size_t o_reach = L.take(sizeof(float) * (size_t)pl.NP * pl.N * Mp);
+    size_t o_rsig = L.take(sizeof(float) * 8 * Mp);
+    size_t o_cstr = L.take(sizeof(uint32_t) * SS_NCOMBO);
+    size_t o_gsort = L.take(sizeof(int16_t) * SS_NCOMBO);
+    size_t o_glt = L.take(sizeof(int16_t) * SS_NCOMBO);
+    size_t o_gle = L.take(sizeof(int16_t) * SS_NCOMBO);
+    size_t o_clist = L.take(52 * 52);
+    size_t o_clen = L.take(52);
+    size_t o_plt = pl.is_river ? L.take(52 * SS_NCOMBO) : 0;
+    size_t o_ple = pl.is_river ? L.take(52 * SS_NCOMBO) : 0;
+    size_t o_board = L.take(sizeof(int) * 8);
+    arena_bytes = L.off;
+    SS_HIP(hipMalloc(&arena, arena_bytes));
+    SS_HIP(hipMalloc(&d_desc, sizeof(SlotDesc)));
+    char *a = arena;
+    SS_HIP(hipMemsetAsync(arena, 0, arena_bytes, st));
+    SS_HIP(hipMemcpyAsync(a + o_nodes, pl.nodes.data(), sizeof(DevNode) * pl.N, hipMemcpyHostToDevice, st));
+    if (!pl.children.empty())
+        SS_HIP(hipMemcpyAsync(a + o_child, pl.children.data(), sizeof(int) * pl.children.size(),
+                              hipMemcpyHostToDevice, st));
+    SS_HIP(hipMemcpyAsync(a + o_declvl, pl.dec_lvl.data(), sizeof(int) * pl.dec_lvl.size(), hipMemcpyHostToDevice, st));
+    SS_HIP(hipMemcpyAsync(a + o_decoff, pl.dec_off.data(), sizeof(int) * pl.dec_off.size(), hipMemcpyHostToDevice, st));
+    SS_HIP(hipMemcpyAsync(a + o_term, pl.term.data(), sizeof(int) * pl.term.size(), hipMemcpyHostToDevice, st));
+    SS_HIP(hipMemcpyAsync(a + o_hinfo, pl.hinfo.data(), sizeof(uint32_t) * pl.NP * Mp, hipMemcpyHostToDevice, st));
+    SS_HIP(hipMemcpyAsync(a + o_w, pl.weights.data(), sizeof(float) * pl.NP * Mp, hipMemcpyHostToDevice, st));
+    if (pl.has_leaf)
+        SS_HIP(hipMemcpyAsync(a + o_leaf, pl.leaf_src, sizeof(float) * pl.leaf_count, hipMemcpyHostToDevice, st));
+    if (pl.frozen_src)
+        SS_HIP(hipMemcpyAsync(a + o_frozen, pl.frozen_src, sizeof(int) * pl.N, hipMemcpyHostToDevice, st));
+    // root reach = raw weights (never overwritten: the forward sweep writes children only)
+    for (int p = 0; p < pl.NP; ++p)
+        SS_HIP(hipMemcpyAsync(a + o_reach + sizeof(float) * (size_t)p * pl.N * Mp, pl.weights.data() + (size_t)p * Mp,
+                              sizeof(float) * Mp, hipMemcpyHostToDevice, st));
+
+    SlotDesc &d = desc;
+    std::memset(&d, 0, sizeof(d));
+    d.N = pl.N; d.NP = pl.NP; d.Mp = pl.Mp; d.maxh = pl.maxh;
+    d.A = pl.A; d.L = pl.L; d.T = pl.T; d.D = pl.D;
+    d.is_river = pl.is_river; d.has_leaf = pl.has_leaf; d.num_leaves = pl.num_leaves;
+    d.root_player = pl.root_player;
+    for (int p = 0; p < 6; ++p) d.nh[p] = p < pl.NP ? pl.nh[p] : 0;
+    d.odometer = pl.odometer;
+    d.nodes = (const DevNode *)(a + o_nodes);
+    d.children = (const int *)(a + o_child);
+    d.dec_lvl = (const int *)(a + o_declvl);
+    d.dec_off = (const int *)(a + o_decoff);
+    d.term = (const int *)(a + o_term);
+    d.hinfo = (const uint32_t *)(a + o_hinfo);
+    d.weights = (const float *)(a + o_w);
+    d.leaf_values = pl.has_leaf ? (const float *)(a + o_leaf) : nullptr;
+    d.frozen = pl.frozen_src ? (const int *)(a + o_frozen) : nullptr;
+    d.R = (float *)(a + o_R);
+    d.SS = (float *)(a + o_SS);
+    d.cfv = (float *)(a + o_cfv);
+    d.reach = (float *)(a + o_reach);
+    d.rsig = (float *)(a + o_rsig);
+    d.cstr = (const uint32_t *)(a + o_cstr);
+    d.gsort = (const int16_t *)(a + o_gsort);
+    d.glt = (const int16_t *)(a + o_glt);
+    d.gle = (const int16_t *)(a + o_gle);
+    d.clist = (const uint8_t *)(a + o_clist);
+    d.clen = (const uint8_t *)(a + o_clen);
+    d.plt = pl.is_river ? (const uint8_t *)(a + o_plt) : nullptr;
+    d.ple = pl.is_river ? (const uint8_t *)(a + o_ple) : nullptr;
+    d.active = 1;
+
+    if (pl.is_river && pl.num_board == 5) {
+        // ranks of all combos on this board, evaluated on the device
+        int *dboard = (int *)(a + o_board);
+        SS_HIP(hipMemcpyAsync(dboard, pl.board, sizeof(int) * 5, hipMemcpyHostToDevice, st));
+        int rc2 = launch_combo_ranks(dboard, (uint32_t *)(a + o_cstr), st);
+        if (rc2) return rc2;
+        std::vector<uint32_t> cstr(SS_NCOMBO);
+        SS_HIP(hipMemcpyAsync(cstr.data(), a + o_cstr, sizeof(uint32_t) * SS_NCOMBO, hipMemcpyDeviceToHost, st));
+        SS_HIP(hipStreamSynchronize(st));
+        RiverTables rt;
+        build_river_tables(cstr.data(), rt);
+        d.ng = rt.ng;
+        SS_HIP(hipMemcpyAsync(a + o_gsort, rt.gsort.data(), sizeof(int16_t) * rt.gsort.size(), hipMemcpyHostToDevice, st));
+        SS_HIP(hipMemcpyAsync(a + o_glt, rt.glt.data(), sizeof(int16_t) * SS_NCOMBO, hipMemcpyHostToDevice, st));
+        SS_HIP(hipMemcpyAsync(a + o_gle, rt.gle.data(), sizeof(int16_t) * SS_NCOMBO, hipMemcpyHostToDevice, st));
+        SS_HIP(hipMemcpyAsync(a + o_clist, rt.clist.data(), 52 * 52, hipMemcpyHostToDevice, st));
+        SS_HIP(hipMemcpyAsync(a + o_clen, rt.clen.data(), 52, hipMemcpyHostToDevice, st));
+        SS_HIP(hipMemcpyAsync(a + o_plt, rt.plt.data(), 52 * SS_NCOMBO, hipMemcpyHostToDevice, st));
+        SS_HIP(hipMemcpyAsync(a + o_ple, rt.ple.data(), 52 * SS_NCOMBO, hipMemcpyHostToDevice, st));
+        SS_HIP(hipStreamSynchronize(st));   // host tables go out of scope
+    } else {
+        d.is_river = 0;   // showdowns off the river carry no ranks: value 0, as street_solve.cu:1520
+    }
+    SS_HIP(hipMemcpyAsync(d_desc, &d, sizeof(d), hipMemcpyHostToDevice, st));
+    return SS_OK;
+}
+
+int Slot::alloc_expl(hipStream_t st, int blocks) {
+    if (xarena && xscratch_blocks >= blocks) return SS_OK;
+    if (xarena) { SS_HIP(hipFree(xarena)); xarena = nullptr; }
+    const size_t Mp = plan.Mp;
+    ArenaLayout L;
+    size_t o_avg = L.take(sizeof(float) * (size_t)plan.A * Mp);
+    size_t o_xr = L.take(sizeof(double) * (size_t)plan.NP * plan.N * Mp);
+    size_t o_xev = L.take(sizeof(double) * (size_t)plan.N * Mp);
+    size_t o_xbr = L.take(sizeof(double) * (size_t)plan.N * Mp);
+    size_t o_xout = L.take(sizeof(double) * 32);
+    size_t o_scr = L.take(sizeof(double) * (size_t)blocks * (2 * SS_NCOMBO + 3 * 2704 + 1328 + 3 * 52 + 256));
+    xarena_bytes = L.off;
+    SS_HIP(hipMalloc(&xarena, xarena_bytes));
+    SS_HIP(hipMemsetAsync(xarena, 0, xarena_bytes, st));
+    desc.avg = (float *)(xarena + o_avg);
+    desc.xreach = (double *)(xarena + o_xr);
+    desc.xev = (double *)(xarena + o_xev);
+    desc.xbr = (double *)(xarena + o_xbr);
+    desc.xout = (double *)(xarena + o_xout);
+    xscratch = (double *)(xarena + o_scr);
+    xscratch_blocks = blocks;
+    // root rows of xreach = raw weights, as doubles
+    std::vector<double> w(Mp);
+    for (int p = 0; p < plan.NP; ++p) {
+        for (size_t h = 0; h < Mp; ++h) w[h] = plan.weights[p * Mp + h];
+        SS_HIP(hipMemcpyAsync(desc.xreach + (size_t)p * plan.N * Mp, w.data(), sizeof(double) * Mp,
+                              hipMemcpyHostToDevice, st));
+        SS_HIP(hipStreamSynchronize(st));
+    }
+    // refresh the device copy of the descriptor, keeping device-side progress
+    SS_HIP(hipMemcpyAsync((char *)d_desc + offsetof(SlotDesc, avg), &desc.avg,
+                          sizeof(SlotDesc) - offsetof(SlotDesc, avg), hipMemcpyHostToDevice, st));
+    return SS_OK;
+}
+
+double expl_from_xout(const double *xout, int np, int starting_pot, double *gains) {
+    double total = 0;
+    for (int p = 0; p < np; ++p) {
+        total += xout[p];
+        if (gains) gains[p] = xout[p];
+    }
+    double J = xout[2 * np];
+    if (J <= 0 || starting_pot <= 0) return -1;
+    return 100.0 * (total / np) / J / (double)starting_pot;
+}
+
+// ---------------------------------------------------------------------------
+// outputs in the reference layout (street_solve.cu:1556-1612)
+// ---------------------------------------------------------------------------
+int read_output(Slot &s, const SlotDesc &dnow, hipStream_t st, SSOutput *out, int) {
+    const Plan &pl = s.plan;
+    const size_t Mp = pl.Mp;
+    const int maxh = pl.maxh, NP = pl.NP;
+    std::vector<float> SS((size_t)pl.A * Mp), rsig(8 * Mp), cfv0(Mp);
+    SS_HIP(hipMemcpyAsync(SS.data(), s.desc.SS, sizeof(float) * SS.size(), hipMemcpyDeviceToHost, st));
+    SS_HIP(hipMemcpyAsync(rsig.data(), s.desc.rsig, sizeof(float) * rsig.size(), hipMemcpyDeviceToHost, st));
+    SS_HIP(hipMemcpyAsync(cfv0.data(), s.desc.cfv, sizeof(float) * Mp, hipMemcpyDeviceToHost, st));
+    SS_HIP(hipStreamSynchronize(st));
+    (void)dnow;
+    std::memset(out, 0, sizeof(*out));
+    const DevNode &root = pl.nodes[0];
+    int root_player = root.player;
+    int root_na = root.nchild;
+    int root_nh = (root_player >= 0 && root_player < NP) ? pl.nh[root_player] : 0;
+    out->root_num_actions = root_na;
+    out->root_player = root_player;
+    out->root_strategy = (float *)std::malloc(sizeof(float) * std::max(1, root_na * root_nh));
+    for (int a = 0; a < root_na; ++a)
+        for (int h = 0; h < root_nh; ++h) out->root_strategy[a * root_nh + h] = rsig[a * Mp + h];
+    out->root_ev = (float *)std::calloc((size_t)NP * maxh, sizeof(float));
+    const int last_nh = pl.nh[NP - 1];
+    for (int h = 0; h < maxh; ++h) out->root_ev[h] = h < last_nh ? cfv0[h] : 0.f;
+    out->num_players = NP;
+    out->max_hands = maxh;
+    out->num_avg_nodes = pl.D;
+    out->avg_strategy_node_ids = (int *)std::malloc(sizeof(int) * std::max(1, pl.D));
+    out->avg_strategies = (float *)std::calloc((size_t)std::max(1, pl.D) * SS_MAX_ACTIONS * maxh, sizeof(float));
+    int di = 0;
+    for (int n = 0; n < pl.N; ++n) {
+        const DevNode &nd = pl.nodes[n];
+        if (nd.type != SS_NODE_DECISION) continue;
+        out->avg_strategy_node_ids[di] = n;
+        int q = nd.player;
+        if (q >= 0 && q < NP) {
+            int na = nd.nchild;
+            for (int h = 0; h < pl.nh[q]; ++h) {
+                float sum = 0.f;
+                for (int a = 0; a < na; ++a) {
+                    float v = SS[(size_t)(nd.act_base + a) * Mp + h];
+                    sum += v > 0.f ? v : 0.f;
+                }
+                float inv = sum > 0.f ? 1.0f / sum : 1.0f / (float)na;
+                for (int a = 0; a < na; ++a) {
+                    float v = SS[(size_t)(nd.act_base + a) * Mp + h];
+                    if (v < 0.f) v = 0.f;
+                    out->avg_strategies[((size_t)di * SS_MAX_ACTIONS + a) * maxh + h] = sum > 0.f ? v * inv : inv;
+                }
+            }
+        }
+        ++di;
+    }
+    return SS_OK;
+}
+
+}  // namespace ss

@@ -1,0 +1,466 @@
+// ss_server.hip -- served form: a continuous-batching request loop on one GPU.
+//
+// Requests are deep-copied at submit and queued. A worker thread owns the
+// device: between graph launches it admits queued requests into free slots
+// while the resident state fits the HBM budget, advances every resident
+// subgame by `iters_per_step` iterations with one captured graph (one
+// persistent launch per iteration, inactive slots skip), evaluates the FP64
+// exploitability of subgames that reached a check point, and retires those at
+// their target (or at their iteration cap) into completed results. Each slot
+// is computed by its own workgroup with no cross-slot data, so a subgame's
+// result does not depend on what else is resident.
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "ss_internal.h"
+
+namespace ss {
+const char *last_error_str();
+
+namespace {
+
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// Owned copy of everything a problem points at.
+struct OwnedProblem {
+    SSTreeData tree;
+    std::vector<SSNode> nodes;
+    std::vector<int> children;
+    std::vector<int> hands[SS_MAX_PLAYERS];
+    std::vector<float> weights[SS_MAX_PLAYERS];
+    std::vector<float> leaf;
+    std::vector<int> frozen;
+    HostProblem hp;
+
+    int copy_from(const SSProblem *p) {
+        HostProblem src;
+        int rc = from_ex(p, &src);
+        if (rc) return rc;
+        const SSTreeData *t = src.tree;
+        std::memset(&tree, 0, sizeof(tree));
+        tree = *t;   // scalars; pointer fields replaced below
+        nodes.assign(t->nodes, t->nodes + t->num_nodes);
+        children.assign(t->children, t->children + t->num_children_total);
+        tree.nodes = nodes.data();
+        tree.children = children.data();
+        tree.level_order = tree.node_depth = nullptr;
+        tree.decision_node_indices = tree.showdown_node_indices = nullptr;
+        tree.leaf_node_indices = tree.fold_node_indices = nullptr;
+        tree.leaf_values = nullptr;
+        tree.frozen_action = nullptr;
+        hp = HostProblem();
+        hp.tree = &tree;
+        hp.np = src.np;
+        int maxh = 0;
+        for (int q = 0; q < src.np; ++q) {
+            hands[q].assign(src.hands[q], src.hands[q] + 2 * src.nh[q]);
+            weights[q].assign(src.weights[q], src.weights[q] + src.nh[q]);
+            hp.nh[q] = src.nh[q];
+            hp.hands[q] = hands[q].data();
+            hp.weights[q] = weights[q].data();
+            maxh = std::max(maxh, src.nh[q]);
+        }
+        if (src.leaf_values && t->num_leaves > 0) {
+            leaf.assign(src.leaf_values, src.leaf_values + (size_t)t->num_leaves * src.np * maxh);
+            hp.leaf_values = leaf.data();
+        }
+        if (src.frozen) {
+            frozen.assign(src.frozen, src.frozen + t->num_nodes);
+            hp.frozen = frozen.data();
+        }
+        return SS_OK;
+    }
+};
+
+enum ReqState { RQ_QUEUED = 0, RQ_RUNNING = 1, RQ_DONE = 2, RQ_ERROR = 3, RQ_CANCELLED = 4 };
+
+struct Request {
+    int64_t ticket = 0;
+    OwnedProblem prob;
+    SSSolveConfig cfg{};
+    int64_t footprint = 0;
+    int state = RQ_QUEUED;
+    int err = 0;
+    std::string errmsg;
+    bool cancel = false;
+    int slot = -1;
+    std::unique_ptr<Slot> dev;
+    int iters = 0;
+    int next_check = 0;
+    double expl = -1;
+    double gains[SS_MAX_PLAYERS] = {0, 0, 0, 0, 0, 0};
+    int expl_at = -1;
+    double t_submit = 0, t_admit = 0, t_done = 0, kernel_ms = 0;
+    SSOutput out{};
+    SSReport rep{};
+};
+
+}  // namespace
+}  // namespace ss
+
+struct SSServer {
+    SSServerConfig cfg{};
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::thread worker;
+    bool stop = false;
+    int64_t next_ticket = 1;
+    std::map<int64_t, std::shared_ptr<ss::Request>> reqs;
+    std::deque<int64_t> queue;
+    // device side (worker thread only)
+    hipStream_t st = nullptr;
+    ss::SlotDesc *d_slots = nullptr;
+    int *d_want = nullptr;
+    double *scratch = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<std::shared_ptr<ss::Request>> resident;   // by slot index
+    int64_t used_bytes = 0;
+    int fatal = 0;
+    std::string fatal_msg;
+
+    int init();
+    void run();
+    int admit_locked(std::unique_lock<std::mutex> &lk);
+    int step();
+    int check_and_retire();
+    void finish(const std::shared_ptr<ss::Request> &r, int state);
+    ~SSServer();
+};
+
+using namespace ss;
+
+int SSServer::init() {
+    SS_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    const int n = cfg.max_slots;
+    SS_HIP(hipMalloc(&d_slots, sizeof(SlotDesc) * n));
+    SS_HIP(hipMemset(d_slots, 0, sizeof(SlotDesc) * n));   // active = 0 everywhere
+    SS_HIP(hipMalloc(&d_want, sizeof(int) * n));
+    SS_HIP(hipMalloc(&scratch, sizeof(double) * (size_t)n * (2 * SS_NCOMBO + 3 * 2704 + 1328 + 3 * 52 + 256)));
+    resident.assign(n, nullptr);
+    SS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < cfg.iters_per_step; ++k) {
+        int rc = launch_batched(d_slots, n, 1, n, st);
+        if (rc) {
+            (void)hipStreamEndCapture(st, &graph);
+            return rc;
+        }
+    }
+    SS_HIP(hipStreamEndCapture(st, &graph));
+    SS_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    return SS_OK;
+}
+
+SSServer::~SSServer() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (auto &r : resident) r.reset();
+    if (d_slots) (void)hipFree(d_slots);
+    if (d_want) (void)hipFree(d_want);
+    if (scratch) (void)hipFree(scratch);
+    if (st) (void)hipStreamDestroy(st);
+}
+
+void SSServer::finish(const std::shared_ptr<Request> &r, int state) {
+    r->state = state;
+    r->t_done = now_ms();
+    if (r->slot >= 0) {
+        int zero = 0;
+        (void)hipMemcpyAsync(&d_slots[r->slot].active, &zero, sizeof(int), hipMemcpyHostToDevice, st);
+        (void)hipStreamSynchronize(st);
+        resident[r->slot].reset();
+        r->slot = -1;
+    }
+    if (r->dev) {
+        used_bytes -= r->footprint;
+        r->dev.reset();
+    }
+}
+
+// Admit queued requests while a slot is free and the footprint fits.
+int SSServer::admit_locked(std::unique_lock<std::mutex> &lk) {
+    while (!queue.empty()) {
+        auto it = reqs.find(queue.front());
+        if (it == reqs.end() || it->second->cancel) {
+            if (it != reqs.end()) it->second->state = RQ_CANCELLED;
+            queue.pop_front();
+            cv_done.notify_all();
+            continue;
+        }
+        std::shared_ptr<Request> r = it->second;
+        int free_slot = -1;
+        for (int i = 0; i < cfg.max_slots; ++i)
+            if (!resident[i]) { free_slot = i; break; }
+        if (free_slot < 0 || used_bytes + r->footprint > cfg.hbm_budget_bytes) break;
+        queue.pop_front();
+        lk.unlock();
+        r->dev.reset(new Slot());
+        int rc = r->dev->setup(r->prob.hp, st);
+        if (!rc) rc = r->dev->alloc_expl(st, 1);
+        if (!rc) {
+            SlotDesc d = r->dev->desc;
+            d.iter = 0;
+            d.iter_end = std::max(0, r->cfg.max_iterations);
+            d.final_iter = d.iter_end;
+            d.snap_every = r->cfg.check_every;
+            d.active = 1;
+            if (hipMemcpyAsync(&d_slots[free_slot], &d, sizeof(d), hipMemcpyHostToDevice, st) != hipSuccess ||
+                hipStreamSynchronize(st) != hipSuccess)
+                rc = set_error(SS_ERR_HIP, "slot upload failed");
+        }
+        lk.lock();
+        if (rc) {
+            r->err = rc;
+            r->errmsg = last_error_str();
+            r->dev.reset();
+            r->state = RQ_ERROR;
+            r->t_done = now_ms();
+            cv_done.notify_all();
+            continue;
+        }
+        r->slot = free_slot;
+        r->state = RQ_RUNNING;
+        r->t_admit = now_ms();
+        r->next_check = r->cfg.check_every;
+        resident[free_slot] = r;
+        used_bytes += r->footprint;
+    }
+    return SS_OK;
+}
+
+int SSServer::step() {
+    hipEvent_t e0, e1;
+    SS_HIP(hipEventCreate(&e0));
+    SS_HIP(hipEventCreate(&e1));
+    SS_HIP(hipEventRecord(e0, st));
+    SS_HIP(hipGraphLaunch(exec, st));
+    SS_HIP(hipEventRecord(e1, st));
+    SS_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    SS_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    for (auto &r : resident)
+        if (r) {
+            r->iters = std::min(r->iters + cfg.iters_per_step, std::max(0, r->cfg.max_iterations));
+            r->kernel_ms += ms;
+        }
+    return SS_OK;
+}
+
+int SSServer::check_and_retire() {
+    const int n = cfg.max_slots;
+    std::vector<int> want(n, 0);
+    int any = 0;
+    for (int i = 0; i < n; ++i) {
+        auto &r = resident[i];
+        if (!r) continue;
+        bool at_cap = r->iters >= r->cfg.max_iterations;
+        bool at_check = r->cfg.target_exploitability > 0 && r->iters >= r->next_check;
+        bool final_eval = at_cap && !(r->cfg.flags & SS_FLAG_NO_EXPL);
+        if ((at_check || final_eval) && r->expl_at != r->iters) { want[i] = 1; any = 1; }
+    }
+    if (any) {
+        SS_HIP(hipMemcpyAsync(d_want, want.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+        int rc = launch_batched_expl(d_slots, n, scratch, d_want, n, st);
+        if (rc) return rc;
+        SS_HIP(hipStreamSynchronize(st));
+        for (int i = 0; i < n; ++i) {
+            if (!want[i]) continue;
+            auto &r = resident[i];
+            double xo[2 * SS_MAX_PLAYERS + 1];
+            SS_HIP(hipMemcpy(xo, r->dev->desc.xout, sizeof(double) * (2 * r->dev->plan.NP + 1), hipMemcpyDeviceToHost));
+            r->expl = expl_from_xout(xo, r->dev->plan.NP, r->dev->plan.starting_pot, r->gains);
+            r->expl_at = r->iters;
+            while (r->next_check <= r->iters && r->cfg.check_every > 0) r->next_check += r->cfg.check_every;
+        }
+    }
+    for (int i = 0; i < n; ++i) {
+        std::shared_ptr<Request> r = resident[i];
+        if (!r) continue;
+        bool conv = r->cfg.target_exploitability > 0 && r->expl_at == r->iters && r->expl >= 0 &&
+                    r->expl <= r->cfg.target_exploitability;
+        bool cap = r->iters >= r->cfg.max_iterations;
+        bool cancel;
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            cancel = r->cancel;
+        }
+        if (!(conv || cap || cancel)) continue;
+        int rc = SS_OK;
+        if (!cancel) rc = read_output(*r->dev, r->dev->desc, st, &r->out, 0);
+        std::lock_guard<std::mutex> lk(mu);
+        std::memset(&r->rep, 0, sizeof(r->rep));
+        r->rep.iterations = r->iters;
+        r->rep.converged = conv ? 1 : 0;
+        r->rep.exploitability = r->expl_at == r->iters ? r->expl : -1;
+        for (int p = 0; p < SS_MAX_PLAYERS; ++p) r->rep.nashconv[p] = r->gains[p];
+        r->rep.kernel_ms = r->kernel_ms;
+        r->rep.algo_bytes = r->dev->plan.algo_bytes_iter * r->iters;
+        if (rc) { r->err = rc; r->errmsg = last_error_str(); }
+        finish(r, cancel ? RQ_CANCELLED : (rc ? RQ_ERROR : RQ_DONE));
+        r->rep.elapsed_ms = r->t_done - r->t_submit;
+        cv_done.notify_all();
+    }
+    return SS_OK;
+}
+
+void SSServer::run() {
+    std::unique_lock<std::mutex> lk(mu);
+    for (;;) {
+        admit_locked(lk);
+        bool busy = false;
+        for (auto &r : resident)
+            if (r) busy = true;
+        if (!busy) {
+            cv_done.notify_all();
+            if (stop) break;
+            cv_work.wait(lk, [&] { return stop || !queue.empty(); });
+            continue;
+        }
+        lk.unlock();
+        int rc = step();
+        if (!rc) rc = check_and_retire();
+        lk.lock();
+        if (rc) {
+            fatal = rc;
+            fatal_msg = last_error_str();
+            for (auto &r : resident)
+                if (r) {
+                    r->err = rc;
+                    r->errmsg = fatal_msg;
+                    finish(r, RQ_ERROR);
+                }
+            cv_done.notify_all();
+        }
+    }
+}
+
+extern "C" {
+
+SSServer *ss_server_create(const SSServerConfig *config) {
+    clear_error();
+    SSServerConfig c;
+    std::memset(&c, 0, sizeof(c));
+    if (config) c = *config;
+    if (c.max_slots <= 0) c.max_slots = 256;
+    if (c.iters_per_step <= 0) c.iters_per_step = 10;
+    if (c.check_every <= 0) c.check_every = 50;
+    if (c.hbm_budget_bytes <= 0) {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+            set_error(SS_ERR_HIP, "hipMemGetInfo failed");
+            return nullptr;
+        }
+        c.hbm_budget_bytes = (int64_t)(fr * 0.8);
+    }
+    SSServer *s = new SSServer();
+    s->cfg = c;
+    if (s->init() != SS_OK) {
+        delete s;
+        return nullptr;
+    }
+    s->worker = std::thread([s] { s->run(); });
+    return s;
+}
+
+int64_t ss_server_submit(SSServer *s, const SSProblem *problem, const SSSolveConfig *config) {
+    clear_error();
+    if (!s || !config) return set_error(SS_ERR_ARG, "NULL server or config");
+    auto r = std::make_shared<Request>();
+    int rc = r->prob.copy_from(problem);
+    if (rc) return rc;
+    r->cfg = *config;
+    if (r->cfg.check_every <= 0) r->cfg.check_every = s->cfg.check_every;
+    r->footprint = estimate_footprint(r->prob.hp);
+    if (r->footprint < 0) return set_error(SS_ERR_ARG, "%s", last_error_str());
+    if (r->footprint > s->cfg.hbm_budget_bytes)
+        return set_error(SS_ERR_NOMEM, "request needs %lld bytes of device state, budget is %lld",
+                         (long long)r->footprint, (long long)s->cfg.hbm_budget_bytes);
+    r->t_submit = now_ms();
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->fatal) return set_error(s->fatal, "server failed: %s", s->fatal_msg.c_str());
+    r->ticket = s->next_ticket++;
+    s->reqs[r->ticket] = r;
+    s->queue.push_back(r->ticket);
+    s->cv_work.notify_all();
+    return r->ticket;
+}
+
+int ss_server_poll(SSServer *s, int64_t ticket, SSOutput *output, SSReport *report) {
+    clear_error();
+    if (!s) return set_error(SS_ERR_ARG, "NULL server");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end()) return set_error(SS_ERR_NOTFOUND, "unknown ticket %lld", (long long)ticket);
+    auto r = it->second;
+    if (r->state == RQ_QUEUED || r->state == RQ_RUNNING) return 0;
+    int rc = 1;
+    if (r->state == RQ_ERROR) rc = set_error(r->err ? r->err : SS_ERR_GENERIC, "%s", r->errmsg.c_str());
+    else if (r->state == RQ_CANCELLED) rc = set_error(SS_ERR_NOTFOUND, "ticket %lld was cancelled", (long long)ticket);
+    if (rc == 1) {
+        if (output) { *output = r->out; std::memset(&r->out, 0, sizeof(r->out)); }
+        else {
+            std::free(r->out.root_strategy); std::free(r->out.root_ev);
+            std::free(r->out.avg_strategies); std::free(r->out.avg_strategy_node_ids);
+        }
+        if (report) *report = r->rep;
+    }
+    s->reqs.erase(it);
+    return rc;
+}
+
+int ss_server_cancel(SSServer *s, int64_t ticket) {
+    clear_error();
+    if (!s) return set_error(SS_ERR_ARG, "NULL server");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end()) return set_error(SS_ERR_NOTFOUND, "unknown ticket %lld", (long long)ticket);
+    it->second->cancel = true;
+    if (it->second->state == RQ_QUEUED) it->second->state = RQ_CANCELLED;
+    s->cv_work.notify_all();
+    return SS_OK;
+}
+
+int ss_server_drain(SSServer *s) {
+    clear_error();
+    if (!s) return set_error(SS_ERR_ARG, "NULL server");
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv_done.wait(lk, [&] {
+        if (s->fatal) return true;
+        if (!s->queue.empty()) return false;
+        for (auto &r : s->resident)
+            if (r) return false;
+        return true;
+    });
+    return s->fatal ? set_error(s->fatal, "server failed: %s", s->fatal_msg.c_str()) : SS_OK;
+}
+
+void ss_server_destroy(SSServer *s) {
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+        for (auto &kv : s->reqs) kv.second->cancel = true;
+        s->queue.clear();
+        s->cv_work.notify_all();
+    }
+    if (s->worker.joinable()) s->worker.join();
+    for (auto &kv : s->reqs) {
+        SSOutput &o = kv.second->out;
+        std::free(o.root_strategy); std::free(o.root_ev);
+        std::free(o.avg_strategies); std::free(o.avg_strategy_node_ids);
+    }
+    delete s;
+}
+
+}  // extern "C"

@@ -1,0 +1,170 @@
+"""GPU parity: the HIP library against the CPU oracle and the golden vectors.
+
+Tolerances. Integer work (hand ranks, trees) is bit-exact. FP32 CFR state is
+compared after a few iterations, before rounding differences can flip the
+sign of a near-zero regret (regret matching turns such flips into O(1)
+strategy changes, and FP32-vs-FP64 runs of the same algorithm separate the
+same way): regrets / strategy sums within 2e-5 relative to the array's scale,
+average strategies within 1e-4 absolute. Long runs are compared through the
+FP64 exploitability of the solution, within 1% relative.
+"""
+
+import ctypes
+import gzip
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+import ss_abi
+import ss_api
+import workloads as wl
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _spot_problem(spot, use_cont=None):
+    prob = ss_api.Problem.from_spot(spot, use_cont=use_cont)
+    oprob = ob.Problem(prob.tree.td, spot.board, spot.hands, spot.weights, leaf_values=prob.leaf_values)
+    return prob, oprob
+
+
+def _close_state(gpu, ref, rel=2e-5):
+    scale = max(1e-30, float(np.abs(ref).max()))
+    err = float(np.abs(gpu.astype(np.float64) - ref).max())
+    assert err <= rel * scale + 1e-30, f"max err {err} vs scale {scale}"
+
+
+# ---------------------------------------------------------------- evaluator
+def test_device_combo_ranks_golden(lib):
+    E = json.load(open(os.path.join(HERE, "golden", "eval_vectors.json")))
+    lib.ssdbg_combo_ranks.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_uint32)]
+    for rb in E["river_boards"]:
+        bd = rb["board"]
+        out = np.zeros(1326, np.uint32)
+        assert lib.ssdbg_combo_ranks((ctypes.c_int * 5)(*bd), out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))) == 0
+        live = [ss for ss in range(1326)]
+        got = []
+        k = 0
+        for a in range(52):
+            for b in range(a + 1, 52):
+                if a not in bd and b not in bd:
+                    got.append(int(out[k]))
+                else:
+                    assert out[k] == 0
+                k += 1
+        assert got == rb["values"]
+
+
+def test_device_eval7_golden(lib):
+    E = json.load(open(os.path.join(HERE, "golden", "eval_vectors.json")))
+    h = np.array(E["hands7"], dtype=np.int32).ravel()
+    n = len(E["values"])
+    out = np.zeros(n, np.uint32)
+    lib.ssdbg_eval7.argtypes = [ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.POINTER(ctypes.c_uint32), ctypes.c_int]
+    assert lib.ssdbg_eval7(h.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), n,
+                           out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), 1) == 0
+    assert (out == np.array(E["values"], dtype=np.uint32)).all()
+
+
+def test_device_eval_census(lib):
+    C = json.load(open(os.path.join(HERE, "golden", "eval_census.json")))
+    counts = (ctypes.c_ulonglong * 10)()
+    s = ctypes.c_ulonglong()
+    lib.ssdbg_eval_census.argtypes = [ctypes.POINTER(ctypes.c_ulonglong), ctypes.POINTER(ctypes.c_ulonglong)]
+    assert lib.ssdbg_eval_census(counts, ctypes.byref(s)) == 0
+    assert list(counts) == C["category_counts"]
+    assert s.value == C["rank_sum_mod_2_64"]
+
+
+# ---------------------------------------------------------------- CFR state after k iterations
+SMALL = [
+    # (seed, players, hands per player, board cards, leaves)
+    (0, 2, 12, 5, False), (1, 2, 9, 5, False), (2, 3, 6, 5, False), (3, 3, 7, 5, False),
+    (4, 4, 4, 5, False), (5, 2, 10, 4, True), (6, 3, 5, 4, True),
+]
+
+
+@pytest.mark.parametrize("seed,npl,hp,nb,leaves", SMALL)
+@pytest.mark.parametrize("form", [0, 1])
+def test_state_small_vs_brute_oracle(seed, npl, hp, nb, leaves, form):
+    print(f"case seed={seed} np={npl} hands={hp} board={nb} leaves={leaves} form={form}", flush=True)
+    sp = wl.small_spot(seed, num_players=npl, hands_per_player=hp, board_cards=nb, use_leaves=leaves,
+                       bet_sizes=(0.5, 1.0), max_raises=2)
+    prob, oprob = _spot_problem(sp)
+    for k in (1, 2, 3):
+        R, S, ev = ss_api.solve_state(prob, k, form)
+        ref = ob.solve(oprob, k, fp64=True, mode=ob.TERM_BRUTE, want_state=True)
+        _close_state(R, ref["regrets"])
+        _close_state(S, ref["strat_sum"])
+
+
+def test_batched_equals_plain_bitwise():
+    sp = wl.small_spot(11, num_players=3, hands_per_player=8)
+    prob, _ = _spot_problem(sp)
+    a = ss_api.solve_state(prob, 5, 0)
+    b = ss_api.solve_state(prob, 5, 1)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+
+
+def test_state_cfg1_shape_full_ranges():
+    sp = wl.river_2p()
+    prob, oprob = _spot_problem(sp)
+    R, S, ev = ss_api.solve_state(prob, 3, 0)
+    ref = ob.solve(oprob, 3, fp64=True, mode=ob.TERM_FAST, want_state=True)
+    _close_state(R, ref["regrets"], rel=5e-5)
+    _close_state(S, ref["strat_sum"], rel=5e-5)
+
+
+def test_state_cfg2_shape_full_ranges():
+    sp = wl.river_3p()
+    prob, oprob = _spot_problem(sp)
+    R, S, ev = ss_api.solve_state(prob, 1, 0)
+    ref = ob.solve(oprob, 1, fp64=True, mode=ob.TERM_FAST, want_state=True)
+    _close_state(R, ref["regrets"], rel=5e-5)
+    _close_state(S, ref["strat_sum"], rel=5e-5)
+
+
+# ---------------------------------------------------------------- outputs and exploitability
+def test_plain_avg_and_expl_vs_oracle_small():
+    sp = wl.small_spot(21, num_players=2, hands_per_player=10)
+    prob, oprob = _spot_problem(sp)
+    res = ss_api.solve_plain(prob, 3)
+    ref = ob.solve(oprob, 3, fp64=True, mode=ob.TERM_BRUTE)
+    avg = res.avg_full(prob.tree.num_nodes)
+    assert np.abs(avg - ref["avg"]).max() < 1e-4
+    e_or, _ = ob.exploitability(oprob, avg, mode=ob.TERM_BRUTE)
+    assert abs(res.exploitability - e_or) <= 1e-6 * max(1.0, abs(e_or))
+
+
+@pytest.mark.parametrize("npl", [2, 3])
+def test_gpu_exploitability_matches_oracle_on_gpu_solution(npl):
+    sp = wl.river_2p() if npl == 2 else wl.river_3p()
+    prob, oprob = _spot_problem(sp)
+    res = ss_api.solve_plain(prob, 20)
+    e_or, _ = ob.exploitability(oprob, res.avg_full(prob.tree.num_nodes), mode=ob.TERM_FAST)
+    assert abs(res.exploitability - e_or) <= 1e-6 * max(1.0, abs(e_or)), (res.exploitability, e_or)
+
+
+def test_long_run_exploitability_cfg1_vs_fp64_oracle():
+    sp = wl.river_2p()
+    prob, oprob = _spot_problem(sp)
+    res = ss_api.solve_plain(prob, 200)
+    ref = ob.solve(oprob, 200, fp64=True, mode=ob.TERM_FAST)
+    e_ref, _ = ob.exploitability(oprob, ref["avg"], mode=ob.TERM_FAST)
+    assert abs(res.exploitability - e_ref) <= 0.01 * e_ref, (res.exploitability, e_ref)
+
+
+def test_legacy_abi_reference_spot(lib):
+    # the spot of the reference's own hot-path test (tests/test_street_solve.py:178-190)
+    from street_solver_gpu import StreetSolverGPU
+    s = StreetSolverGPU(board=["Qs", "As", "2d", "7h", "4c"], oop_range="AhKh,QhQc,JhTh,6h5h,3c3d",
+                        ip_range="AcKc,8c8d,Tc9c,KhJh,2h2c", pot_bb=10.0, stack_bb=50.0, bet_sizes=[0.33, 0.75])
+    s.solve(iterations=200)
+    strat = s.get_strategy("oop", hand_str="AhKh")
+    assert len(strat) > 0 and abs(sum(strat.values()) - 1.0) < 0.05
+    s.free()
