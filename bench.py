@@ -1,0 +1,252 @@
+#!/usr/bin/env python3
+"""Subgame solves/s at <= 0.5% pot exploitability on one MI355X.
+
+One step = one batched solve of a fixed batch of synthetic river subgames:
+every subgame starts from zero regrets and is retired when its FP64
+exploitability is <= target (% of its starting pot), checked every
+`check_every` iterations, or at the iteration cap. All inputs are uploaded
+to HBM before the timed region (ss_batch_create); each step resets the state
+on the device and solves the whole batch (ss_batch_run).
+
+Printed JSON line (rank 0, one GPU): value = subgames solved per second of
+wall time over the timed steps. Also reported: the plain form (one subgame
+per launch sequence, ss_solve_gpu_ex) on a sample of the same subgames, the
+batched/plain ratio, the roofline of the persistent CFR kernel (algorithmic
+bytes per launch / event-timed launch duration), and CPU baselines timed on
+this host: the CPU oracle of the same semantics on a bounded sample, and
+(when built) the reference's CPU solver src/solver_v2.c on the 2-player
+river shape.
+"""
+
+import argparse
+import json
+import multiprocessing as mp
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "poker-solver_amd"))
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_spots(workload, n, seed):
+    import workloads as wl
+    if workload == "batch_mix":
+        return wl.batch_mix(n, seed=seed, max_players=3)
+    if workload == "river_2p":
+        return [wl.river_2p(seed=seed + i) for i in range(n)]
+    if workload == "river_3p":
+        return [wl.river_3p(seed=seed + i) for i in range(n)]
+    raise SystemExit(f"unknown workload {workload}")
+
+
+# ---------------------------------------------------------------- CPU baselines
+def _oracle_iter_time(args):
+    """Seconds per CFR iteration of the CPU oracle (FP32, closed-form terminals)
+    on one subgame, averaged over `iters` iterations."""
+    workload, seed, idx, iters = args
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import oracle_bind as ob
+    import workloads as wl
+    spot = make_spots(workload, idx + 1, seed)[idx]
+    td = ob.build_tree(spot.board, spot.pot, spot.stack, spot.bet_sizes, spot.num_players,
+                       spot.acting_order, max_raises=spot.max_raises)
+    prob = ob.Problem(td, spot.board, spot.hands, spot.weights)
+    t0 = time.perf_counter()
+    ob.solve(prob, iters, fp64=False, mode=ob.TERM_FAST)
+    return (time.perf_counter() - t0) / iters
+
+
+def cpu_oracle_baseline(workload, seed, idxs, gpu_iters, cores, iters=2):
+    with mp.get_context("spawn").Pool(cores) as pool:
+        t_iter = pool.map(_oracle_iter_time, [(workload, seed, i, iters) for i in idxs])
+    per_solve = [t * max(1, gpu_iters[i]) for t, i in zip(t_iter, idxs)]
+    return cores / float(np.mean(per_solve)), float(np.mean(t_iter))
+
+
+def _sv2_iter_time(seed):
+    """Seconds for one iteration of the reference CPU solver (src/solver_v2.c,
+    compiled in place into oracle/_ref/solver_v2.so) on a 2-player river with
+    1081 combos per player, pot 100 BB, stacks 400 BB, sizes {0.5, 1.0}."""
+    import ctypes
+    import workloads as wl
+    lib = ctypes.CDLL(os.path.join(REPO, "oracle", "_ref", "solver_v2.so"))
+    spot = wl.river_2p(seed=seed)
+    buf = ctypes.create_string_buffer(1 << 20)   # >= sizeof(SolverV2)
+    h = [np.ascontiguousarray(x, dtype=np.int32) for x in spot.hands]
+    w = [np.ascontiguousarray(x, dtype=np.float32) for x in spot.weights]
+    b = (ctypes.c_int * 5)(*spot.board)
+    sizes = (ctypes.c_float * 2)(0.5, 1.0)
+    P = ctypes.c_void_p
+    rc = lib.sv2_init(buf, b, 5, h[0].ctypes.data_as(P), w[0].ctypes.data_as(P), len(h[0]),
+                      h[1].ctypes.data_as(P), w[1].ctypes.data_as(P), len(h[1]),
+                      spot.pot, spot.stack, sizes, 2)
+    if rc != 0:
+        return None
+    lib.sv2_solve.restype = ctypes.c_float
+    t0 = time.perf_counter()
+    lib.sv2_solve(buf, 1, ctypes.c_float(0.0))
+    dt = time.perf_counter() - t0
+    lib.sv2_free(buf)
+    return dt
+
+
+def cpu_reference_baseline(cores, iters_per_solve=500):
+    path = os.path.join(REPO, "oracle", "_ref", "solver_v2.so")
+    if not os.path.exists(path):
+        return None
+    with mp.get_context("spawn").Pool(cores) as pool:
+        ts = [t for t in pool.map(_sv2_iter_time, [20261203 + 1000 + i for i in range(cores)]) if t]
+    if not ts:
+        return None
+    t_iter = float(np.mean(ts))
+    return {"value": cores / (t_iter * iters_per_solve), "unit": "solves/s", "cores": cores,
+            "kind": "reference",
+            "sample": f"src/solver_v2.c (compiled in place), 2-player river 1081 combos/player, "
+                      f"pot 100 BB, stacks 400 BB, sizes {{0.5,1.0}}: 1 iteration per process on {cores} "
+                      f"processes, {t_iter:.1f} s/iteration, extrapolated to {iters_per_solve} iterations "
+                      f"(its cost per iteration is constant)",
+            "seconds_per_iteration": t_iter}
+
+
+# ---------------------------------------------------------------- main
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="batch_mix", choices=["batch_mix", "river_2p", "river_3p"])
+    ap.add_argument("--subgames", type=int, default=256)
+    ap.add_argument("--seed", type=int, default=20261203 + 4000)
+    ap.add_argument("--target", type=float, default=0.5, help="exploitability target, %% of pot")
+    ap.add_argument("--check-every", type=int, default=25)
+    ap.add_argument("--max-iters", type=int, default=2000)
+    ap.add_argument("--plain-sample", type=int, default=8)
+    ap.add_argument("--cpu-cores", type=int, default=0, help="0 = min(16, host cores)")
+    ap.add_argument("--cpu-sample", type=int, default=16, help="subgames timed by the CPU oracle")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu-ref", action="store_true")
+    a = ap.parse_args()
+
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: subgames are independent and shard across GPUs by request "
+              "routing with no collectives, so no multi-GPU figure is produced (run with --gpus 1).",
+              file=sys.stderr)
+        return 2
+
+    import ss_api
+    spots = make_spots(a.workload, a.subgames, a.seed)
+    log(f"[bench] building {len(spots)} subgames ({a.workload})")
+    problems = [ss_api.Problem.from_spot(s) for s in spots]
+    nodes = [p.tree.num_nodes for p in problems]
+    players = [p.num_players for p in problems]
+    batch = ss_api.Batch(problems, a.max_iters, target=a.target, check_every=a.check_every)
+    log(f"[bench] uploaded; nodes mean {np.mean(nodes):.0f} max {max(nodes)}; players {np.bincount(players)[2:].tolist()}")
+
+    for w in range(a.warmup):
+        batch.run()
+        log(f"[bench] warmup {w + 1}/{a.warmup} done")
+    reps_all, stats_all = [], []
+    t0 = time.perf_counter()
+    for k in range(a.steps):
+        reps, st = batch.run()
+        reps_all.append(reps)
+        stats_all.append(st)
+        log(f"[bench] step {k + 1}/{a.steps}: {st.run_ms:.1f} ms, max iters {st.max_iterations}")
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1000.0 / a.steps
+    value = len(problems) * a.steps / (t1 - t0)
+
+    reps = reps_all[-1]
+    iters = np.array([r.iterations for r in reps])
+    expl = np.array([r.exploitability for r in reps])
+    conv = np.array([r.converged for r in reps])
+    cfr_ms = sum(s.cfr_ms for s in stats_all)
+    check_ms = sum(s.check_ms for s in stats_all)
+    algo = sum(s.algo_bytes for s in stats_all)
+    launches = sum(s.launches for s in stats_all)
+    achieved = algo / (cfr_ms / 1000.0) / 1e9 if cfr_ms > 0 else 0.0
+
+    # plain form on a sample of the same subgames
+    plain_ms = []
+    for i in range(min(a.plain_sample, len(problems))):
+        r = ss_api.solve_plain(problems[i], a.max_iters, target=a.target, check_every=a.check_every)
+        plain_ms.append(r.kernel_ms)
+    plain_sps = 1000.0 / float(np.mean(plain_ms)) if plain_ms else None
+    log(f"[bench] plain form: {np.mean(plain_ms):.1f} ms per solve on {len(plain_ms)} subgames")
+
+    line = {
+        "metric": "subgame solves/s at <=0.5% pot exploitability",
+        "value": round(value, 3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{a.workload}: {len(problems)} river subgames per batched solve"
+                        + (", players U{2,3}, 1-4 sizes from {0.25,0.5,0.75,1,1.5} pot + all-in, range density "
+                           "U[0.1,1], pot U[50,300] BB, stacks U[200,1000] BB, max 2 raises"
+                           if a.workload == "batch_mix" else ""),
+            "subgames": len(problems),
+            "target_exploitability_pct_pot": a.target,
+            "check_every": a.check_every,
+            "max_iterations": a.max_iters,
+            "form": "batched: one persistent kernel launch per CFR iteration, hipGraph-captured",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5),
+            "traffic": None,
+            "kernel": "k_batched",
+            "launches": launches,
+            "avg_launch_us": round(cfr_ms * 1000.0 / max(1, launches), 3),
+            "algo_bytes_per_launch": int(algo / max(1, launches)),
+        },
+        "plain_solves_per_s": round(plain_sps, 3) if plain_sps else None,
+        "batched_vs_plain": round(value / plain_sps, 2) if plain_sps else None,
+        "iterations_to_target": {"mean": float(iters.mean()), "p50": float(np.percentile(iters, 50)),
+                                 "p99": float(np.percentile(iters, 99)), "max": int(iters.max())},
+        "converged_fraction": float(conv.mean()),
+        "exploitability_pct_pot": {"mean": float(expl.mean()), "max": float(expl.max())},
+        "device_ms_per_step": {"cfr": round(cfr_ms / a.steps, 3), "checks": round(check_ms / a.steps, 3)},
+    }
+
+    if not a.no_cpu:
+        cores = a.cpu_cores or min(16, os.cpu_count() or 1)
+        idxs = list(range(min(a.cpu_sample, len(problems))))
+        log(f"[bench] CPU oracle baseline on {len(idxs)} subgames, {cores} processes")
+        sps, t_iter = cpu_oracle_baseline(a.workload, a.seed, idxs, iters, cores)
+        line["cpu_baseline"] = {
+            "value": round(sps, 5), "unit": "solves/s", "cores": cores, "kind": "port",
+            "sample": f"CPU oracle (same semantics, FP32, closed-form terminals) on the first {len(idxs)} "
+                      f"subgames of the batch, 2 iterations each ({t_iter * 1000:.0f} ms/iteration mean), "
+                      f"extrapolated to each subgame's GPU iteration count to target"}
+        if not a.no_cpu_ref:
+            log("[bench] reference CPU solver (solver_v2) baseline")
+            ref = cpu_reference_baseline(cores)
+            if ref:
+                line["cpu_reference_solver_v2"] = ref
+    print(json.dumps(line), flush=True)
+    batch.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

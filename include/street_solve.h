@@ -140,16 +140,16 @@ typedef struct {
 } SSSolveConfig;
 
 #define SS_FLAG_NO_EXPL     1    /* skip the final exploitability evaluation */
-#define SS_FLAG_KEEP_STATE  2    /* keep final regrets / strategy sums for ss_read_state */
 
 typedef struct {
     int iterations;              /* iterations actually run */
     int converged;               /* 1 if target_exploitability was reached */
     double exploitability;       /* % of starting pot (FP64 best response); -1 if not evaluated */
     double nashconv[SS_MAX_PLAYERS]; /* per-player BR gain, chips per unit deal mass */
-    double elapsed_ms;           /* wall time of the solve on the device */
-    double kernel_ms;            /* summed HIP-event time of the CFR sweeps */
-    int64_t algo_bytes;          /* algorithmic HBM bytes of the CFR sweeps */
+    double elapsed_ms;           /* wall time of the call (served form: arrival to retirement) */
+    double kernel_ms;            /* solve loop on the device: CFR iterations and exploitability
+                                    checks, setup and output transfers excluded */
+    int64_t algo_bytes;          /* algorithmic HBM bytes of the CFR iterations */
 } SSReport;
 
 #ifdef __cplusplus
@@ -194,6 +194,24 @@ int ss_solve_gpu_ex(const SSProblem *problem, const SSSolveConfig *config,
  * iteration loop captured in a hipGraph. outputs/reports have n entries. */
 int ss_solve_batch_ex(const SSProblem *problems, int n, const SSSolveConfig *config,
                       SSOutput *outputs, SSReport *reports);
+
+/* Batched form as a session: ss_batch_create uploads every subgame and captures
+ * the iteration graph; each ss_batch_run resets regrets and strategy sums and
+ * solves all subgames again (inputs stay resident in HBM between runs). */
+typedef struct SSBatch SSBatch;
+typedef struct {
+    double run_ms;               /* wall time of the run (device work + host checks) */
+    double cfr_ms;               /* HIP-event time of the CFR graph replays */
+    double check_ms;             /* HIP-event time of the FP64 exploitability checks */
+    int64_t algo_bytes;          /* algorithmic bytes of all CFR iterations run */
+    int launches;                /* persistent-kernel launches (one per iteration) */
+    int max_iterations;          /* iterations of the slowest subgame */
+} SSBatchStats;
+
+SSBatch *ss_batch_create(const SSProblem *problems, int n, const SSSolveConfig *config);
+int ss_batch_run(SSBatch *batch, SSReport *reports, SSBatchStats *stats);
+int ss_batch_output(SSBatch *batch, int index, SSOutput *output);
+void ss_batch_destroy(SSBatch *batch);
 
 /* Served form: a continuous-batching request loop on one GPU. */
 typedef struct SSServer SSServer;

@@ -41,6 +41,7 @@ static int solve_plain(const HostProblem &hp, const SSSolveConfig &cfg, SSOutput
     const int chunk = target ? std::max(1, cfg.check_every > 0 ? cfg.check_every : 50) : std::max(1, T);
     double kms = 0, expl = -1, gains[SS_MAX_PLAYERS] = {0, 0, 0, 0, 0, 0};
     int done = 0, converged = 0, expl_at = -1;
+    const double t_loop = now_ms();
     while (done < T) {
         int k = std::min(chunk, T - done);
         rc = plain_run(s, k, done + k, g.st, &kms);
@@ -60,6 +61,8 @@ static int solve_plain(const HostProblem &hp, const SSSolveConfig &cfg, SSOutput
         rc = plain_exploitability(s, g.st, &expl, gains);
         if (rc) return rc;
     }
+    const double loop_ms = now_ms() - t_loop;   // every step above ends in a stream sync
+    (void)kms;
     if (out) {
         rc = read_output(s, s.desc, g.st, out, 0);
         if (rc) return rc;
@@ -71,7 +74,7 @@ static int solve_plain(const HostProblem &hp, const SSSolveConfig &cfg, SSOutput
         rep->exploitability = expl;
         for (int p = 0; p < SS_MAX_PLAYERS; ++p) rep->nashconv[p] = gains[p];
         rep->elapsed_ms = now_ms() - t0;
-        rep->kernel_ms = kms;
+        rep->kernel_ms = loop_ms;
         rep->algo_bytes = s.plan.algo_bytes_iter * done;
     }
     return SS_OK;
@@ -79,156 +82,204 @@ static int solve_plain(const HostProblem &hp, const SSSolveConfig &cfg, SSOutput
 
 // Batched form: all subgames advance together, one persistent launch per
 // iteration; `chunk` launches are captured once in a hipGraph and replayed.
-int solve_batch(const HostProblem *hps, int n, const SSSolveConfig &cfg, SSOutput *outs, SSReport *reps,
-                double *wall_ms_out) {
-    StreamGuard g;
-    if (g.rc) return g.rc;
-    double t0 = now_ms();
-    std::vector<std::unique_ptr<Slot>> slots(n);
-    const int T = std::max(0, cfg.max_iterations);
-    const bool target = cfg.target_exploitability > 0;
-    const bool want_final = !(cfg.flags & SS_FLAG_NO_EXPL);
-    const int chunk = target ? std::max(1, cfg.check_every > 0 ? cfg.check_every : 50) : std::max(1, std::min(T, 50));
-    for (int i = 0; i < n; ++i) {
-        slots[i].reset(new Slot());
-        int rc = slots[i]->setup(hps[i], g.st);
-        if (rc) return set_error(rc, "subgame %d: %s", i, last_error_str());
-        if (target || want_final) {
-            rc = slots[i]->alloc_expl(g.st, 1);
-            if (rc) return rc;
-        }
-    }
-    std::vector<SlotDesc> descs(n);
-    for (int i = 0; i < n; ++i) {
-        descs[i] = slots[i]->desc;
-        descs[i].iter = 0;
-        descs[i].iter_end = T;
-        descs[i].final_iter = T;
-        descs[i].snap_every = target ? chunk : 0;
-        descs[i].active = 1;
-    }
+struct BatchImpl {
+    SSSolveConfig cfg{};
+    int n = 0, chunk = 1;
+    bool target = false, want_final = false;
+    std::vector<std::unique_ptr<Slot>> slots;
+    std::vector<SlotDesc> templ;
     SlotDesc *d_slots = nullptr;
     int *d_want = nullptr;
     double *scratch = nullptr;
-    struct Free {
-        void *p[3];
-        ~Free() {
-            for (void *q : p)
-                if (q) (void)hipFree(q);
-        }
-    } fr{{nullptr, nullptr, nullptr}};
-    SS_HIP(hipMalloc(&d_slots, sizeof(SlotDesc) * std::max(1, n)));
-    fr.p[0] = d_slots;
-    SS_HIP(hipMalloc(&d_want, sizeof(int) * std::max(1, n)));
-    fr.p[1] = d_want;
-    if (target || want_final) {
-        SS_HIP(hipMalloc(&scratch, sizeof(double) * (size_t)std::max(1, n) *
-                                       (2 * SS_NCOMBO + 3 * 2704 + 1328 + 3 * 52 + 256)));
-        fr.p[2] = scratch;
-    }
-    SS_HIP(hipMemcpyAsync(d_slots, descs.data(), sizeof(SlotDesc) * n, hipMemcpyHostToDevice, g.st));
-    SS_HIP(hipStreamSynchronize(g.st));
-
-    // capture `chunk` iterations
+    hipStream_t st = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    SS_HIP(hipStreamBeginCapture(g.st, hipStreamCaptureModeThreadLocal));
-    for (int k = 0; k < chunk; ++k) {
-        int rc = launch_batched(d_slots, n, 1, n, g.st);
-        if (rc) {
-            (void)hipStreamEndCapture(g.st, &graph);
-            return rc;
-        }
-    }
-    SS_HIP(hipStreamEndCapture(g.st, &graph));
-    SS_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    // results of the last run
+    std::vector<int> iters, conv, expl_at;
+    std::vector<double> expl, gains;
 
-    std::vector<int> iters(n, 0), conv(n, 0), live(n, 1), want(n, 0);
-    std::vector<double> expl(n, -1.0), gains((size_t)n * SS_MAX_PLAYERS, 0.0);
-    std::vector<int> expl_at(n, -1);
-    hipEvent_t e0, e1;
-    SS_HIP(hipEventCreate(&e0));
-    SS_HIP(hipEventCreate(&e1));
-    double kms = 0;
-    auto evaluate = [&](const std::vector<int> &sel, int at) -> int {
-        SS_HIP(hipMemcpyAsync(d_want, sel.data(), sizeof(int) * n, hipMemcpyHostToDevice, g.st));
-        int rc = launch_batched_expl(d_slots, n, scratch, d_want, n, g.st);
+    ~BatchImpl() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (auto &e : ev)
+            if (e) (void)hipEventDestroy(e);
+        slots.clear();
+        if (d_slots) (void)hipFree(d_slots);
+        if (d_want) (void)hipFree(d_want);
+        if (scratch) (void)hipFree(scratch);
+        if (st) (void)hipStreamDestroy(st);
+    }
+
+    int create(const HostProblem *hps, int count, const SSSolveConfig &c) {
+        cfg = c;
+        n = count;
+        const int T = std::max(0, cfg.max_iterations);
+        target = cfg.target_exploitability > 0;
+        want_final = !(cfg.flags & SS_FLAG_NO_EXPL);
+        chunk = target ? std::max(1, cfg.check_every > 0 ? cfg.check_every : 50) : std::max(1, std::min(T, 50));
+        SS_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        for (auto &e : ev) SS_HIP(hipEventCreate(&e));
+        slots.resize(n);
+        for (int i = 0; i < n; ++i) {
+            slots[i].reset(new Slot());
+            int rc = slots[i]->setup(hps[i], st);
+            if (rc) return set_error(rc, "subgame %d: %s", i, last_error_str());
+            if (target || want_final) {
+                rc = slots[i]->alloc_expl(st, 1);
+                if (rc) return rc;
+            }
+        }
+        templ.resize(n);
+        for (int i = 0; i < n; ++i) {
+            templ[i] = slots[i]->desc;
+            templ[i].iter = 0;
+            templ[i].iter_end = T;
+            templ[i].final_iter = T;
+            templ[i].snap_every = target ? chunk : 0;
+            templ[i].active = 1;
+        }
+        SS_HIP(hipMalloc(&d_slots, sizeof(SlotDesc) * std::max(1, n)));
+        SS_HIP(hipMalloc(&d_want, sizeof(int) * std::max(1, n)));
+        if (target || want_final)
+            SS_HIP(hipMalloc(&scratch, sizeof(double) * (size_t)std::max(1, n) *
+                                           (2 * SS_NCOMBO + 3 * 2704 + 1328 + 3 * 52 + 256)));
+        SS_HIP(hipMemcpyAsync(d_slots, templ.data(), sizeof(SlotDesc) * n, hipMemcpyHostToDevice, st));
+        SS_HIP(hipStreamSynchronize(st));
+        SS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        for (int k = 0; k < chunk; ++k) {
+            int rc = launch_batched(d_slots, n, 1, n, st);
+            if (rc) {
+                (void)hipStreamEndCapture(st, &graph);
+                return rc;
+            }
+        }
+        SS_HIP(hipStreamEndCapture(st, &graph));
+        SS_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        return SS_OK;
+    }
+
+    int evaluate(const std::vector<int> &sel, int at, double *check_ms) {
+        SS_HIP(hipEventRecord(ev[2], st));
+        SS_HIP(hipMemcpyAsync(d_want, sel.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+        int rc = launch_batched_expl(d_slots, n, scratch, d_want, n, st);
         if (rc) return rc;
+        SS_HIP(hipEventRecord(ev[3], st));
+        SS_HIP(hipEventSynchronize(ev[3]));
+        float ms = 0;
+        SS_HIP(hipEventElapsedTime(&ms, ev[2], ev[3]));
+        *check_ms += ms;
         std::vector<double> xo(2 * SS_MAX_PLAYERS + 1);
         for (int i = 0; i < n; ++i) {
             if (!sel[i]) continue;
-            SS_HIP(hipMemcpyAsync(xo.data(), slots[i]->desc.xout, sizeof(double) * (2 * slots[i]->plan.NP + 1),
-                                  hipMemcpyDeviceToHost, g.st));
-            SS_HIP(hipStreamSynchronize(g.st));
+            SS_HIP(hipMemcpy(xo.data(), slots[i]->desc.xout, sizeof(double) * (2 * slots[i]->plan.NP + 1),
+                             hipMemcpyDeviceToHost));
             expl[i] = expl_from_xout(xo.data(), slots[i]->plan.NP, slots[i]->plan.starting_pot,
                                      &gains[(size_t)i * SS_MAX_PLAYERS]);
             expl_at[i] = at;
         }
         return SS_OK;
-    };
+    }
 
-    int done = 0;
-    while (done < T) {
-        SS_HIP(hipEventRecord(e0, g.st));
-        SS_HIP(hipGraphLaunch(exec, g.st));
-        SS_HIP(hipEventRecord(e1, g.st));
-        SS_HIP(hipEventSynchronize(e1));
-        float ms = 0;
-        SS_HIP(hipEventElapsedTime(&ms, e0, e1));
-        kms += ms;
-        int step = std::min(chunk, T - done);
-        done += step;
-        for (int i = 0; i < n; ++i)
-            if (live[i]) iters[i] = done;
-        if (target) {
-            for (int i = 0; i < n; ++i) want[i] = live[i];
-            int rc = evaluate(want, done);
-            if (rc) return rc;
-            int nlive = 0;
-            for (int i = 0; i < n; ++i) {
-                if (!live[i]) continue;
-                if (expl[i] >= 0 && expl[i] <= cfg.target_exploitability) {
-                    conv[i] = 1;
-                    live[i] = 0;
-                    static const int kZero = 0;
-                    SS_HIP(hipMemcpyAsync(&d_slots[i].active, &kZero, sizeof(int), hipMemcpyHostToDevice, g.st));
-                } else {
-                    ++nlive;
+    int run(SSReport *reps, SSBatchStats *stats) {
+        const int T = std::max(0, cfg.max_iterations);
+        double t0 = now_ms();
+        // reset state: regrets, strategy sums, root snapshot; fresh descriptors
+        for (auto &s : slots) {
+            SS_HIP(hipMemsetAsync(s->desc.R, 0, sizeof(float) * (size_t)s->plan.A * s->plan.Mp, st));
+            SS_HIP(hipMemsetAsync(s->desc.SS, 0, sizeof(float) * (size_t)s->plan.A * s->plan.Mp, st));
+            SS_HIP(hipMemsetAsync(s->desc.rsig, 0, sizeof(float) * 8 * (size_t)s->plan.Mp, st));
+        }
+        SS_HIP(hipMemcpyAsync(d_slots, templ.data(), sizeof(SlotDesc) * n, hipMemcpyHostToDevice, st));
+        iters.assign(n, 0);
+        conv.assign(n, 0);
+        expl_at.assign(n, -1);
+        expl.assign(n, -1.0);
+        gains.assign((size_t)n * SS_MAX_PLAYERS, 0.0);
+        std::vector<int> live(n, 1), want(n, 0);
+        double cfr_ms = 0, check_ms = 0;
+        int launches = 0, done = 0;
+        int64_t bytes = 0;
+        int64_t live_bytes = 0;
+        for (int i = 0; i < n; ++i) live_bytes += slots[i]->plan.algo_bytes_iter;
+        while (done < T) {
+            SS_HIP(hipEventRecord(ev[0], st));
+            SS_HIP(hipGraphLaunch(exec, st));
+            SS_HIP(hipEventRecord(ev[1], st));
+            SS_HIP(hipEventSynchronize(ev[1]));
+            float ms = 0;
+            SS_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+            cfr_ms += ms;
+            int step = std::min(chunk, T - done);
+            done += step;
+            launches += chunk;
+            bytes += live_bytes * step;
+            for (int i = 0; i < n; ++i)
+                if (live[i]) iters[i] = done;
+            if (target) {
+                for (int i = 0; i < n; ++i) want[i] = live[i];
+                int rc = evaluate(want, done, &check_ms);
+                if (rc) return rc;
+                int nlive = 0;
+                static const int kZero = 0;
+                for (int i = 0; i < n; ++i) {
+                    if (!live[i]) continue;
+                    if (expl[i] >= 0 && expl[i] <= cfg.target_exploitability) {
+                        conv[i] = 1;
+                        live[i] = 0;
+                        live_bytes -= slots[i]->plan.algo_bytes_iter;
+                        SS_HIP(hipMemcpyAsync(&d_slots[i].active, &kZero, sizeof(int), hipMemcpyHostToDevice, st));
+                    } else {
+                        ++nlive;
+                    }
                 }
+                SS_HIP(hipStreamSynchronize(st));
+                if (nlive == 0) break;
             }
-            SS_HIP(hipStreamSynchronize(g.st));
-            if (nlive == 0) break;
         }
-    }
-    if (want_final) {
-        for (int i = 0; i < n; ++i) want[i] = expl_at[i] != iters[i];
-        // re-activate retired slots for the evaluation only if they still need one (none do)
-        int rc = evaluate(want, done);
-        if (rc) return rc;
-    }
-    double wall = now_ms() - t0;
-    for (int i = 0; i < n; ++i) {
-        if (outs) {
-            int rc = read_output(*slots[i], slots[i]->desc, g.st, &outs[i], 0);
+        if (want_final) {
+            for (int i = 0; i < n; ++i) want[i] = expl_at[i] != iters[i];
+            int rc = evaluate(want, done, &check_ms);
             if (rc) return rc;
         }
-        if (reps) {
-            SSReport &r = reps[i];
-            std::memset(&r, 0, sizeof(r));
-            r.iterations = iters[i];
-            r.converged = conv[i];
-            r.exploitability = expl[i];
-            for (int p = 0; p < SS_MAX_PLAYERS; ++p) r.nashconv[p] = gains[(size_t)i * SS_MAX_PLAYERS + p];
-            r.elapsed_ms = wall;
-            r.kernel_ms = kms;
-            r.algo_bytes = slots[i]->plan.algo_bytes_iter * iters[i];
+        double wall = now_ms() - t0;
+        int maxit = 0;
+        for (int i = 0; i < n; ++i) maxit = std::max(maxit, iters[i]);
+        if (reps)
+            for (int i = 0; i < n; ++i) {
+                SSReport &r = reps[i];
+                std::memset(&r, 0, sizeof(r));
+                r.iterations = iters[i];
+                r.converged = conv[i];
+                r.exploitability = expl[i];
+                for (int p = 0; p < SS_MAX_PLAYERS; ++p) r.nashconv[p] = gains[(size_t)i * SS_MAX_PLAYERS + p];
+                r.elapsed_ms = wall;
+                r.kernel_ms = cfr_ms + check_ms;
+                r.algo_bytes = slots[i]->plan.algo_bytes_iter * iters[i];
+            }
+        if (stats) {
+            stats->run_ms = wall;
+            stats->cfr_ms = cfr_ms;
+            stats->check_ms = check_ms;
+            stats->algo_bytes = bytes;
+            stats->launches = launches;
+            stats->max_iterations = maxit;
         }
+        return SS_OK;
     }
-    if (wall_ms_out) *wall_ms_out = wall;
-    (void)hipGraphExecDestroy(exec);
-    (void)hipGraphDestroy(graph);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+};
+
+int solve_batch(const HostProblem *hps, int n, const SSSolveConfig &cfg, SSOutput *outs, SSReport *reps) {
+    BatchImpl b;
+    int rc = b.create(hps, n, cfg);
+    if (rc) return rc;
+    rc = b.run(reps, nullptr);
+    if (rc) return rc;
+    if (outs)
+        for (int i = 0; i < n; ++i) {
+            rc = read_output(*b.slots[i], b.slots[i]->desc, b.st, &outs[i], 0);
+            if (rc) return rc;
+        }
     return SS_OK;
 }
 
@@ -308,8 +359,47 @@ int ss_solve_batch_ex(const SSProblem *problems, int n, const SSSolveConfig *con
         int rc = from_ex(&problems[i], &hps[i]);
         if (rc) return rc;
     }
-    return solve_batch(hps.data(), n, *config, outputs, reports, nullptr);
+    return solve_batch(hps.data(), n, *config, outputs, reports);
 }
+
+}  // extern "C"
+
+struct SSBatch {
+    ss::BatchImpl impl;
+};
+
+extern "C" {
+
+SSBatch *ss_batch_create(const SSProblem *problems, int n, const SSSolveConfig *config) {
+    clear_error();
+    if (!config || n <= 0 || !problems) {
+        set_error(SS_ERR_ARG, "bad batch arguments");
+        return nullptr;
+    }
+    std::vector<HostProblem> hps(n);
+    for (int i = 0; i < n; ++i)
+        if (from_ex(&problems[i], &hps[i])) return nullptr;
+    SSBatch *b = new SSBatch();
+    if (b->impl.create(hps.data(), n, *config) != SS_OK) {
+        delete b;
+        return nullptr;
+    }
+    return b;
+}
+
+int ss_batch_run(SSBatch *batch, SSReport *reports, SSBatchStats *stats) {
+    clear_error();
+    if (!batch) return set_error(SS_ERR_ARG, "NULL batch");
+    return batch->impl.run(reports, stats);
+}
+
+int ss_batch_output(SSBatch *batch, int index, SSOutput *output) {
+    clear_error();
+    if (!batch || !output || index < 0 || index >= batch->impl.n) return set_error(SS_ERR_ARG, "bad arguments");
+    return read_output(*batch->impl.slots[index], batch->impl.slots[index]->desc, batch->impl.st, output, 0);
+}
+
+void ss_batch_destroy(SSBatch *batch) { delete batch; }
 
 int64_t ss_problem_footprint(const SSProblem *problem) {
     clear_error();

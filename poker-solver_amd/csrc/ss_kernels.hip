@@ -32,6 +32,7 @@
 namespace ss {
 
 #define SS_BLOCK 256
+#define SS_BATCH_BLOCK 512
 #define NT_DEC 8
 
 // ---------------------------------------------------------------------------
@@ -228,7 +229,19 @@ struct TermBuf {
     T *PG;               // [1328] prefix over strength-sorted combos
     T *C1, *C2, *CP;     // [52] per-card totals
     T *red;              // [SS_BLOCK] scan scratch
+    const uint8_t *clist;  // [52*52] per-card partner lists (LDS copy when available)
+    const uint8_t *clen;   // [52]
+    const uint32_t *rank;  // [1326] combo ranks
 };
+
+// A card-removal mass is a sum of non-negative reach products; evaluated by
+// inclusion-exclusion it carries rounding noise of order eps * (scale of the
+// terms). Values below this floor are set to exactly 0, as an empty sum is in
+// the reference's enumeration (an info set with no live opponent hands must
+// keep exactly zero regrets).
+template <typename T> __device__ __forceinline__ T mass_floor();
+template <> __device__ __forceinline__ float mass_floor<float>() { return 1e-5f; }
+template <> __device__ __forceinline__ double mass_floor<double>() { return 1e-12; }
 
 template <typename T>
 __device__ __forceinline__ T tsum52(const T *C) {
@@ -237,28 +250,37 @@ __device__ __forceinline__ T tsum52(const T *C) {
     return s;
 }
 
-// exclusive block scan of M[gsort[j]] into PG[0..ng]
+// inclusive prefix sum across the 64 lanes of a wavefront
+template <typename T>
+__device__ __forceinline__ T wave_incl_scan(T x, int lane) {
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        T y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    return x;
+}
+
+// exclusive block scan of M[gsort[j]] into PG[0..ng]: per-thread chunk sums,
+// a wave64 scan of the chunk sums, wave totals through LDS
 template <typename T>
 __device__ void scan_sorted(const SlotDesc *__restrict__ s, const T *M, T *PG, T *red, int tid, int nt) {
     const int ng = s->ng;
     const int chunk = (ng + nt - 1) / nt;
-    const int j0 = tid * chunk, j1 = min(ng, j0 + chunk);
+    const int j0 = min(ng, tid * chunk), j1 = min(ng, j0 + chunk);
     T part = 0;
     for (int j = j0; j < j1; ++j) part += M[s->gsort[j]];
-    red[tid] = part;
+    const int lane = tid & 63, w = tid >> 6;
+    T incl = wave_incl_scan(part, lane);
+    if (lane == 63) red[w] = incl;
     __syncthreads();
-    for (int off = 1; off < nt; off <<= 1) {
-        T v = tid >= off ? red[tid - off] : (T)0;
-        __syncthreads();
-        red[tid] += v;
-        __syncthreads();
-    }
-    T run = tid > 0 ? red[tid - 1] : (T)0;
+    T run = incl - part;
+    for (int i = 0; i < w; ++i) run += red[i];
     for (int j = j0; j < j1; ++j) {
         PG[j] = run;
         run += M[s->gsort[j]];
     }
-    if (tid == nt - 1) PG[ng] = red[nt - 1];
+    if (tid == nt - 1) PG[ng] = run;
     __syncthreads();
 }
 
@@ -320,51 +342,60 @@ __device__ __forceinline__ void scatter(const SlotDesc *__restrict__ s, int q, i
     }
 }
 
-// per-card totals (non-showdown) of M: C[c] = sum_d M[c,d]; optionally of M1*M2
+// per-card totals (non-showdown) of M: C[c] = sum_d M[c,d]; optionally of M1*M2.
+// C arrays must be zero on entry (atomics in LDS); caller barriers after.
 template <typename T>
 __device__ __forceinline__ void card_totals(const T *M1, const T *M2, T *C1, T *C2, T *CP, int tid, int nt) {
-    for (int c = tid; c < 52; c += nt) {
-        T a = 0, b = 0, p = 0;
-        for (int d = 0; d < 52; ++d) {
-            if (d == c) continue;
-            int k = combo_id(c, d);
-            T m1 = M1[k];
-            a += m1;
-            if (M2) {
-                T m2 = M2[k];
-                b += m2;
-                p += m1 * m2;
-            }
+    for (int i = tid; i < 52 * 52; i += nt) {
+        int c = i / 52, d = i - c * 52;
+        if (c == d) continue;
+        int k = combo_id(c, d);
+        T m1 = M1[k];
+        if (m1 != (T)0) atomicAdd(&C1[c], m1);
+        if (M2) {
+            T m2 = M2[k];
+            if (m2 != (T)0) atomicAdd(&C2[c], m2);
+            if (m1 * m2 != (T)0) atomicAdd(&CP[c], m1 * m2);
         }
-        C1[c] = a;
-        if (M2) { C2[c] = b; CP[c] = p; }
     }
 }
 
-// per-card prefix sums over strength-sorted partner lists (showdown)
+// per-card prefix sums over strength-sorted partner lists (showdown):
+// PC[c*52 + j] = sum of the first j partners of card c. Gather in list order,
+// then one wave64 scan per card.
 template <typename T>
-__device__ __forceinline__ void card_prefix(const SlotDesc *__restrict__ s, const T *M1, const T *M2, T *PC1,
-                                            T *PC2, T *PCP, T *C1, T *C2, T *CP, int tid, int nt) {
-    for (int c = tid; c < 52; c += nt) {
-        int len = s->clen[c];
-        T a = 0, b = 0, p = 0;
-        PC1[c * 52] = 0;
-        if (M2) { PC2[c * 52] = 0; PCP[c * 52] = 0; }
-        for (int j = 0; j < len; ++j) {
-            int k = combo_id(c, s->clist[c * 52 + j]);
-            T m1 = M1[k];
-            a += m1;
-            PC1[c * 52 + j + 1] = a;
-            if (M2) {
-                T m2 = M2[k];
-                b += m2;
-                p += m1 * m2;
-                PC2[c * 52 + j + 1] = b;
-                PCP[c * 52 + j + 1] = p;
-            }
+__device__ __forceinline__ void card_prefix(const SlotDesc *__restrict__ s, const uint8_t *clist,
+                                            const uint8_t *clen, const T *M1, const T *M2, T *PC1, T *PC2,
+                                            T *PCP, T *C1, T *C2, T *CP, int tid, int nt) {
+    for (int i = tid; i < 52 * 52; i += nt) {
+        int c = i / 52, j = i - c * 52;
+        T a = 0, b = 0;
+        if (j > 0 && j - 1 < clen[c]) {
+            int k = combo_id(c, clist[c * 52 + j - 1]);
+            a = M1[k];
+            if (M2) b = M2[k];
         }
-        C1[c] = a;
-        if (M2) { C2[c] = b; CP[c] = p; }
+        PC1[i] = a;
+        if (M2) { PC2[i] = b; PCP[i] = a * b; }
+    }
+    __syncthreads();
+    const int lane = tid & 63, nw = nt >> 6;
+    for (int c = tid >> 6; c < 52; c += nw) {
+        const int at = c * 52 + 1 + lane;
+        T a = lane < 51 ? PC1[at] : (T)0;
+        a = wave_incl_scan(a, lane);
+        if (lane < 51) PC1[at] = a;
+        T ta = __shfl(a, 50, 64);
+        if (lane == 0) C1[c] = ta;
+        if (M2) {
+            T b = lane < 51 ? PC2[at] : (T)0;
+            T p = lane < 51 ? PCP[at] : (T)0;
+            b = wave_incl_scan(b, lane);
+            p = wave_incl_scan(p, lane);
+            if (lane < 51) { PC2[at] = b; PCP[at] = p; }
+            T tb = __shfl(b, 50, 64), tp = __shfl(p, 50, 64);
+            if (lane == 0) { C2[c] = tb; CP[c] = tp; }
+        }
     }
 }
 
@@ -416,11 +447,15 @@ __device__ void terminal_node(const SlotDesc *__restrict__ s, int n, int trav, c
         B.M1[i] = 0;
         if (k == 2) B.M2[i] = 0;
     }
+    if (!sd)
+        for (int i = tid; i < 52; i += nt) { B.C1[i] = 0; B.C2[i] = 0; B.CP[i] = 0; }
     __syncthreads();
     if (k >= 1) scatter<T>(s, opp[0], n, reach, B.M1, tid, nt);
     if (k == 2) scatter<T>(s, opp[1], n, reach, B.M2, tid, nt);
     __syncthreads();
-    if (sd) card_prefix<T>(s, B.M1, k == 2 ? B.M2 : nullptr, B.PC1, B.PC2, B.PCP, B.C1, B.C2, B.CP, tid, nt);
+    if (sd && k >= 1)
+        card_prefix<T>(s, B.clist, B.clen, B.M1, k == 2 ? B.M2 : nullptr, B.PC1, B.PC2, B.PCP, B.C1, B.C2, B.CP,
+                       tid, nt);
     else if (k >= 1) card_totals<T>(B.M1, k == 2 ? B.M2 : nullptr, B.C1, B.C2, B.CP, tid, nt);
     __syncthreads();
     if (sd && k == 1) scan_sorted<T>(s, B.M1, B.PG, B.red, tid, nt);
@@ -439,7 +474,7 @@ __device__ void terminal_node(const SlotDesc *__restrict__ s, int n, int trav, c
             if (k == 0) m = 1;
             else {
                 T z1 = S1 - B.C1[x] - B.C1[y] + B.M1[kk];
-                if (k == 1) m = z1;
+                if (k == 1) m = z1 < mass_floor<T>() * S1 ? (T)0 : z1;
                 else {
                     T z2 = S2 - B.C2[x] - B.C2[y] + B.M2[kk];
                     T zp = SP - B.CP[x] - B.CP[y] + B.M1[kk] * B.M2[kk];
@@ -450,6 +485,7 @@ __device__ void terminal_node(const SlotDesc *__restrict__ s, int n, int trav, c
                         X += (B.C1[c] - B.M1[kx] - B.M1[ky]) * (B.C2[c] - B.M2[kx] - B.M2[ky]);
                     }
                     m = z1 * z2 - X + zp;
+                    if (m < mass_floor<T>() * S1 * S2) m = 0;
                 }
             }
             T f;
@@ -469,14 +505,14 @@ __device__ void terminal_node(const SlotDesc *__restrict__ s, int n, int trav, c
                 T lt = B.PG[s->glt[kk]] - B.PC1[x * 52 + lx] - B.PC1[y * 52 + ly];
                 T le = B.PG[s->gle[kk]] - B.PC1[x * 52 + ex] - B.PC1[y * 52 + ey] + B.M1[kk];
                 T z = S1 - B.C1[x] - B.C1[y] + B.M1[kk];
-                v = pot * lt + pot * (T)0.5 * (le - lt) - bet * z;
+                v = z < mass_floor<T>() * S1 ? (T)0 : pot * lt + pot * (T)0.5 * (le - lt) - bet * z;
             } else {                                                           // :936-1004, n_opp = 2
                 T h1l = 0, h1e = 0, h2l = 0, h2e = 0, hpl = 0, hpe = 0;
                 T xll = 0, xle = 0, xel = 0, xee = 0, xaa = 0;
                 T a_lx = 0, a_ex = 0, a_ly = 0, a_ey = 0, b_lx = 0, b_ex = 0, b_ly = 0, b_ey = 0;
                 T p_lx = 0, p_ex = 0, p_ly = 0, p_ey = 0;
                 for (int c = 0; c < 52; ++c) {
-                    if (s->clen[c] == 0) continue;
+                    if (B.clen[c] == 0) continue;
                     int l = s->plt[c * SS_NCOMBO + kk], e = s->ple[c * SS_NCOMBO + kk];
                     T al = B.PC1[c * 52 + l], ae = B.PC1[c * 52 + e] - al;
                     T bl = B.PC2[c * 52 + l], be = B.PC2[c * 52 + e] - bl;
@@ -486,7 +522,7 @@ __device__ void terminal_node(const SlotDesc *__restrict__ s, int n, int trav, c
                     if (c == y) { a_ly = al; a_ey = ae; b_ly = bl; b_ey = be; p_ly = pl; p_ey = pe; continue; }
                     T aa = B.C1[c], ba = B.C2[c];
                     int kx = combo_id(c, x), ky = combo_id(c, y);
-                    uint32_t rx = s->cstr[kx], ry = s->cstr[ky];
+                    uint32_t rx = B.rank[kx], ry = B.rank[ky];
                     T m1x = B.M1[kx], m1y = B.M1[ky], m2x = B.M2[kx], m2y = B.M2[ky];
                     if (rx < hs) { al -= m1x; bl -= m2x; } else if (rx == hs) { ae -= m1x; be -= m2x; }
                     if (ry < hs) { al -= m1y; bl -= m2y; } else if (ry == hs) { ae -= m1y; be -= m2y; }
@@ -505,7 +541,8 @@ __device__ void terminal_node(const SlotDesc *__restrict__ s, int n, int trav, c
                 T wel = Ae * Bl - xel;
                 T wee = Ae * Be - xee + Pe;
                 T z = Aa * Ba - xaa + Pa;
-                v = pot * (wll + (wle + wel) * (T)0.5 + wee / (T)3) - bet * z;
+                v = z < mass_floor<T>() * S1 * S2 ? (T)0
+                                                  : pot * (wll + (wle + wel) * (T)0.5 + wee / (T)3) - bet * z;
             }
         }
         out[h] = v;
@@ -514,25 +551,42 @@ __device__ void terminal_node(const SlotDesc *__restrict__ s, int n, int trav, c
 }
 
 // LDS terminal buffers for the FP32 sweep (55 KB)
+// LDS of the FP32 terminal sweep (~58 KB): reach in combo space, prefix
+// sums, and a copy of the subgame's per-card lists and combo ranks
 struct TermShared {
     float M1[SS_NCOMBO], M2[SS_NCOMBO];
     float PC1[52 * 52], PC2[52 * 52], PCP[52 * 52];
     float PG[1328];
     float C1[52], C2[52], CP[52];
-    float red[SS_BLOCK];
+    float red[64];
+    uint32_t rank[SS_NCOMBO];
+    uint8_t clist[52 * 52];
+    uint8_t clen[52];
 };
 
 __device__ __forceinline__ TermBuf<float> lds_buf(TermShared &t) {
     TermBuf<float> b;
     b.M1 = t.M1; b.M2 = t.M2; b.PC1 = t.PC1; b.PC2 = t.PC2; b.PCP = t.PCP;
     b.PG = t.PG; b.C1 = t.C1; b.C2 = t.C2; b.CP = t.CP; b.red = t.red;
+    b.clist = t.clist; b.clen = t.clen; b.rank = t.rank;
     return b;
+}
+
+// copy a river subgame's per-card lists and ranks into LDS (caller barriers)
+__device__ __forceinline__ void stage_tables(const SlotDesc *__restrict__ s, TermShared &t, int tid, int nt) {
+    if (!s->is_river) return;
+    for (int i = tid; i < 52 * 52; i += nt) t.clist[i] = s->clist[i];
+    for (int i = tid; i < 52; i += nt) t.clen[i] = s->clen[i];
+    for (int i = tid; i < SS_NCOMBO; i += nt) t.rank[i] = s->cstr[i];
 }
 
 // FP64 buffers live in global scratch: one slice per workgroup
 #define XSCRATCH_DOUBLES (2 * SS_NCOMBO + 3 * 2704 + 1328 + 3 * 52 + SS_BLOCK)
-__device__ __forceinline__ TermBuf<double> global_buf(double *base) {
+__device__ __forceinline__ TermBuf<double> global_buf(double *base, const SlotDesc *__restrict__ s) {
     TermBuf<double> b;
+    b.clist = s->clist;
+    b.clen = s->clen;
+    b.rank = s->cstr;
     b.M1 = base; b.M2 = b.M1 + SS_NCOMBO;
     b.PC1 = b.M2 + SS_NCOMBO; b.PC2 = b.PC1 + 2704; b.PCP = b.PC2 + 2704;
     b.PG = b.PCP + 2704;
@@ -551,6 +605,8 @@ __global__ void __launch_bounds__(SS_BLOCK) k_forward(const SlotDesc *__restrict
 __global__ void __launch_bounds__(SS_BLOCK) k_terminals(const SlotDesc *__restrict__ s, int trav) {
     __shared__ TermShared sh;
     TermBuf<float> b = lds_buf(sh);
+    stage_tables(s, sh, threadIdx.x, blockDim.x);
+    __syncthreads();
     for (int j = blockIdx.x; j < s->T; j += gridDim.x) {
         int n = s->term[j];
         terminal_node<float>(s, n, trav, s->reach, s->cfv + (size_t)n * s->Mp, b, TK_NODE, nullptr,
@@ -574,13 +630,15 @@ __global__ void k_bump(SlotDesc *s) {
 // ---------------------------------------------------------------------------
 // batched form: one subgame per workgroup, `iters` iterations per launch
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(SS_BLOCK) k_batched(SlotDesc *__restrict__ slots, int nslots, int iters) {
+__global__ void __launch_bounds__(SS_BATCH_BLOCK) k_batched(SlotDesc *__restrict__ slots, int nslots, int iters) {
     __shared__ TermShared sh;
     TermBuf<float> b = lds_buf(sh);
     const int tid = threadIdx.x, nt = blockDim.x;
     for (int si = blockIdx.x; si < nslots; si += gridDim.x) {
         SlotDesc *s = slots + si;
         if (!s->active) continue;
+        stage_tables(s, sh, tid, nt);
+        __syncthreads();
         int it = s->iter;
         const int end = s->iter_end, NP = s->NP, L = s->L;
         for (int rep = 0; rep < iters && it < end; ++rep) {
@@ -722,7 +780,7 @@ __global__ void __launch_bounds__(SS_BLOCK) k_xforward(const SlotDesc *__restric
 }
 
 __global__ void __launch_bounds__(SS_BLOCK) k_xterminals(const SlotDesc *__restrict__ s, int p, double *scratch) {
-    TermBuf<double> b = global_buf(scratch + (size_t)blockIdx.x * XSCRATCH_DOUBLES);
+    TermBuf<double> b = global_buf(scratch + (size_t)blockIdx.x * XSCRATCH_DOUBLES, s);
     for (int j = blockIdx.x; j < s->T; j += gridDim.x) {
         int n = s->term[j];
         terminal_node<double>(s, n, p, s->xreach, s->xev + (size_t)n * s->Mp, b, TK_NODE, nullptr, threadIdx.x,
@@ -744,7 +802,7 @@ __global__ void __launch_bounds__(SS_BLOCK) k_xreduce(const SlotDesc *__restrict
 // joint deal mass at the root with raw weights: sum_h w_0(h) * mass_0(h)
 __global__ void __launch_bounds__(SS_BLOCK) k_xjoint(const SlotDesc *__restrict__ s, double *scratch) {
     __shared__ double red[SS_BLOCK];
-    TermBuf<double> b = global_buf(scratch);
+    TermBuf<double> b = global_buf(scratch, s);
     // xreach rows of node 0 hold the raw weights; the per-hand mass lands in xev row 0
     // with scale = weights of player 0 restricted to live hands
     terminal_node<double>(s, 0, 0, s->xreach, s->xev, b, TK_MASS, s->weights, threadIdx.x, blockDim.x);
@@ -768,7 +826,7 @@ __global__ void __launch_bounds__(SS_BLOCK) k_batched_expl(SlotDesc *__restrict_
     for (int si = blockIdx.x; si < nslots; si += gridDim.x) {
         SlotDesc *s = slots + si;
         if (!s->active || !want[si]) continue;
-        TermBuf<double> b = global_buf(scratch + (size_t)blockIdx.x * XSCRATCH_DOUBLES);
+        TermBuf<double> b = global_buf(scratch + (size_t)blockIdx.x * XSCRATCH_DOUBLES, s);
         avg_items(s, tid, nt);
         __syncthreads();
         for (int l = 0; l < s->L; ++l) {
@@ -891,7 +949,7 @@ int plain_exploitability(Slot &s, hipStream_t st, double *expl_pct, double *gain
 }
 
 int launch_batched(SlotDesc *d_slots, int nslots, int iters, int grid, hipStream_t st) {
-    hipLaunchKernelGGL(k_batched, dim3(grid), dim3(SS_BLOCK), 0, st, d_slots, nslots, iters);
+    hipLaunchKernelGGL(k_batched, dim3(grid), dim3(SS_BATCH_BLOCK), 0, st, d_slots, nslots, iters);
     SS_HIP(hipGetLastError());
     return SS_OK;
 }

@@ -25,7 +25,6 @@ SS_PREC_FP32 = 0
 SS_PREC_BF16X2 = 1
 
 SS_FLAG_NO_EXPL = 1
-SS_FLAG_KEEP_STATE = 2
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG_DIR, "build", "street_solve.so")
@@ -127,6 +126,17 @@ class SSReport(ctypes.Structure):
     ]
 
 
+class SSBatchStats(ctypes.Structure):
+    _fields_ = [
+        ("run_ms", ctypes.c_double),
+        ("cfr_ms", ctypes.c_double),
+        ("check_ms", ctypes.c_double),
+        ("algo_bytes", ctypes.c_int64),
+        ("launches", ctypes.c_int),
+        ("max_iterations", ctypes.c_int),
+    ]
+
+
 class SSServerConfig(ctypes.Structure):
     _fields_ = [
         ("hbm_budget_bytes", ctypes.c_int64),
@@ -153,6 +163,10 @@ SIGNATURES = {
     "ss_solve_gpu_ex": (ctypes.c_int, [_P(SSProblem), _P(SSSolveConfig), _P(SSOutput), _P(SSReport)]),
     "ss_solve_batch_ex": (ctypes.c_int, [_P(SSProblem), ctypes.c_int, _P(SSSolveConfig),
                                          _P(SSOutput), _P(SSReport)]),
+    "ss_batch_create": (ctypes.c_void_p, [_P(SSProblem), ctypes.c_int, _P(SSSolveConfig)]),
+    "ss_batch_run": (ctypes.c_int, [ctypes.c_void_p, _P(SSReport), _P(SSBatchStats)]),
+    "ss_batch_output": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _P(SSOutput)]),
+    "ss_batch_destroy": (None, [ctypes.c_void_p]),
     "ss_server_create": (ctypes.c_void_p, [_P(SSServerConfig)]),
     "ss_server_submit": (ctypes.c_int64, [ctypes.c_void_p, _P(SSProblem), _P(SSSolveConfig)]),
     "ss_server_poll": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, _P(SSOutput), _P(SSReport)]),

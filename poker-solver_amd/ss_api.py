@@ -165,6 +165,43 @@ def solve_batch(problems, iterations, target=0.0, check_every=0, flags=0):
     return [Result(outs[i], reps[i], problems[i]) for i in range(n)]
 
 
+class Batch:
+    """Batched form as a session: upload once (ss_batch_create), then solve all
+    subgames from scratch on every run() with inputs resident in HBM."""
+
+    def __init__(self, problems, iterations, target=0.0, check_every=0, flags=0):
+        self.lib = ss_abi.load_library()
+        self.problems = list(problems)
+        self.n = len(self.problems)
+        self._structs = (SSProblem * self.n)(*[p.struct() for p in self.problems])
+        cfg = config(iterations, target, check_every, flags=flags)
+        self.h = self.lib.ss_batch_create(self._structs, self.n, ctypes.byref(cfg))
+        if not self.h:
+            raise SolveError(f"ss_batch_create failed: {ss_abi.last_error()}")
+
+    def run(self):
+        reps = (SSReport * self.n)()
+        stats = ss_abi.SSBatchStats()
+        _check(self.lib.ss_batch_run(self.h, reps, ctypes.byref(stats)), "ss_batch_run")
+        return [reps[i] for i in range(self.n)], stats
+
+    def output(self, i, report=None):
+        out = SSOutput()
+        _check(self.lib.ss_batch_output(self.h, int(i), ctypes.byref(out)), "ss_batch_output")
+        return Result(out, report, self.problems[i])
+
+    def close(self):
+        if self.h:
+            self.lib.ss_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def solve_state(problem, iterations, form=0):
     """Raw regrets / strategy sums [N][8][max_h] and root cfv after k iterations."""
     lib = ss_abi.load_library()
