@@ -206,6 +206,8 @@ typedef struct {
     int64_t algo_bytes;          /* algorithmic bytes of all CFR iterations run */
     int launches;                /* persistent-kernel launches (one per iteration) */
     int max_iterations;          /* iterations of the slowest subgame */
+    int mode;                    /* strategy the autotuner chose: 0 persistent kernel with one
+                                    workgroup per subgame, 1 phased launches over all subgames */
 } SSBatchStats;
 
 SSBatch *ss_batch_create(const SSProblem *problems, int n, const SSSolveConfig *config);

@@ -1,6 +1,7 @@
 // ss_capi.hip -- extern "C" entry points of libstreet_solve.so (include/street_solve.h).
 #include <chrono>
 #include <cstdlib>
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <vector>
@@ -95,6 +96,9 @@ struct BatchImpl {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    PhasedTasks pt;
+    int mode = 1;          // 0: persistent kernel, one workgroup per subgame; 1: phased launches
+    int xblocks = 512;     // workgroups (and FP64 scratch slices) of the exploitability sweeps
     // results of the last run
     std::vector<int> iters, conv, expl_at;
     std::vector<double> expl, gains;
@@ -141,14 +145,20 @@ struct BatchImpl {
         }
         SS_HIP(hipMalloc(&d_slots, sizeof(SlotDesc) * std::max(1, n)));
         SS_HIP(hipMalloc(&d_want, sizeof(int) * std::max(1, n)));
+        std::vector<Slot *> raw;
+        for (auto &sl : slots) raw.push_back(sl.get());
+        int rc = build_phased_tasks(raw, pt, st);
+        if (rc) return rc;
+        xblocks = std::max(1, std::min(512, std::max(n, pt.nterm)));
         if (target || want_final)
-            SS_HIP(hipMalloc(&scratch, sizeof(double) * (size_t)std::max(1, n) *
-                                           (2 * SS_NCOMBO + 3 * 2704 + 1328 + 3 * 52 + 256)));
+            SS_HIP(hipMalloc(&scratch, sizeof(double) * (size_t)xblocks * SS_XSCRATCH_DOUBLES));
+        rc = choose_mode();
+        if (rc) return rc;
         SS_HIP(hipMemcpyAsync(d_slots, templ.data(), sizeof(SlotDesc) * n, hipMemcpyHostToDevice, st));
         SS_HIP(hipStreamSynchronize(st));
         SS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         for (int k = 0; k < chunk; ++k) {
-            int rc = launch_batched(d_slots, n, 1, n, st);
+            rc = launch_iteration(mode);
             if (rc) {
                 (void)hipStreamEndCapture(st, &graph);
                 return rc;
@@ -159,10 +169,53 @@ struct BatchImpl {
         return SS_OK;
     }
 
+    int launch_iteration(int m) {
+        return m == 0 ? launch_batched(d_slots, n, 1, n, st) : launch_phased_iteration(d_slots, pt, st);
+    }
+
+    int reset_state() {
+        for (auto &sl : slots) {
+            SS_HIP(hipMemsetAsync(sl->desc.R, 0, sizeof(float) * (size_t)sl->plan.A * sl->plan.Mp, st));
+            SS_HIP(hipMemsetAsync(sl->desc.SS, 0, sizeof(float) * (size_t)sl->plan.A * sl->plan.Mp, st));
+            SS_HIP(hipMemsetAsync(sl->desc.rsig, 0, sizeof(float) * 8 * (size_t)sl->plan.Mp, st));
+        }
+        SS_HIP(hipMemcpyAsync(d_slots, templ.data(), sizeof(SlotDesc) * n, hipMemcpyHostToDevice, st));
+        return SS_OK;
+    }
+
+    // Autotune the batched strategy on this batch: time two iterations of each
+    // (after one untimed iteration) and keep the faster. SS_BATCH_MODE=
+    // persistent|phased overrides.
+    int choose_mode() {
+        const char *env = getenv("SS_BATCH_MODE");
+        if (env && !strcmp(env, "persistent")) { mode = 0; return SS_OK; }
+        if (env && !strcmp(env, "phased")) { mode = 1; return SS_OK; }
+        if (cfg.max_iterations < 3) { mode = 1; return SS_OK; }
+        float best = 1e30f;
+        for (int m = 0; m < 2; ++m) {
+            int rc = reset_state();
+            if (rc) return rc;
+            rc = launch_iteration(m);
+            if (rc) return rc;
+            SS_HIP(hipEventRecord(ev[0], st));
+            for (int k = 0; k < 2; ++k) {
+                rc = launch_iteration(m);
+                if (rc) return rc;
+            }
+            SS_HIP(hipEventRecord(ev[1], st));
+            SS_HIP(hipEventSynchronize(ev[1]));
+            float ms = 0;
+            SS_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+            if (log_level() > 0) fprintf(stderr, "[street_solve] batch mode %d: %.3f ms / 2 iterations\n", m, ms);
+            if (ms < best) { best = ms; mode = m; }
+        }
+        return reset_state();
+    }
+
     int evaluate(const std::vector<int> &sel, int at, double *check_ms) {
         SS_HIP(hipEventRecord(ev[2], st));
         SS_HIP(hipMemcpyAsync(d_want, sel.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
-        int rc = launch_batched_expl(d_slots, n, scratch, d_want, n, st);
+        int rc = launch_phased_expl(d_slots, pt, d_want, scratch, xblocks, st);
         if (rc) return rc;
         SS_HIP(hipEventRecord(ev[3], st));
         SS_HIP(hipEventSynchronize(ev[3]));
@@ -184,13 +237,9 @@ struct BatchImpl {
     int run(SSReport *reps, SSBatchStats *stats) {
         const int T = std::max(0, cfg.max_iterations);
         double t0 = now_ms();
-        // reset state: regrets, strategy sums, root snapshot; fresh descriptors
-        for (auto &s : slots) {
-            SS_HIP(hipMemsetAsync(s->desc.R, 0, sizeof(float) * (size_t)s->plan.A * s->plan.Mp, st));
-            SS_HIP(hipMemsetAsync(s->desc.SS, 0, sizeof(float) * (size_t)s->plan.A * s->plan.Mp, st));
-            SS_HIP(hipMemsetAsync(s->desc.rsig, 0, sizeof(float) * 8 * (size_t)s->plan.Mp, st));
-        }
-        SS_HIP(hipMemcpyAsync(d_slots, templ.data(), sizeof(SlotDesc) * n, hipMemcpyHostToDevice, st));
+        // fresh regrets, strategy sums, root snapshot and descriptors
+        int rc0 = reset_state();
+        if (rc0) return rc0;
         iters.assign(n, 0);
         conv.assign(n, 0);
         expl_at.assign(n, -1);
@@ -264,6 +313,7 @@ struct BatchImpl {
             stats->algo_bytes = bytes;
             stats->launches = launches;
             stats->max_iterations = maxit;
+            stats->mode = mode;
         }
         return SS_OK;
     }

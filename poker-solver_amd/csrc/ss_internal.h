@@ -102,6 +102,20 @@ int launch_batched_expl(SlotDesc *d_slots, int nslots, double *scratch, const in
                         hipStream_t st);
 double expl_from_xout(const double *xout, int np, int starting_pot, double *gains);
 
+// phased batched form: (slot << 20 | node) task lists on the device
+struct PhasedTasks {
+    int nslots = 0, NPmax = 0, L = 0;
+    std::vector<int> lvl_off;      // level l tasks: [lvl_off[l], lvl_off[l+1]); all decision tasks [0, lvl_off[L])
+    int term_off = 0, nterm = 0;   // terminal tasks, most expensive first
+    uint32_t *d_tasks = nullptr;
+    ~PhasedTasks();
+};
+int build_phased_tasks(const std::vector<Slot *> &slots, PhasedTasks &pt, hipStream_t st);
+int launch_phased_iteration(SlotDesc *d_slots, const PhasedTasks &pt, hipStream_t st);
+int launch_phased_expl(SlotDesc *d_slots, const PhasedTasks &pt, const int *d_want, double *scratch,
+                       int scratch_blocks, hipStream_t st);
+#define SS_XSCRATCH_DOUBLES (2 * SS_NCOMBO + 3 * 2704 + 1328 + 3 * 52 + 256)
+
 // debug / test entry points implemented in the kernels unit
 int dbg_eval7_device(const int *cards, int n, uint32_t *out);
 int dbg_combo_ranks_device(const int *board, uint32_t *out);

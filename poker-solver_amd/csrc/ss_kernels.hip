@@ -139,70 +139,76 @@ __device__ __forceinline__ void regret_match(const float *__restrict__ R, int ro
 }
 
 // ---------------------------------------------------------------------------
-// forward: reach of every non-traverser through one level (items = node x hand)
+// forward: reach of every non-traverser through decision node n, hand h
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void forward_items(const SlotDesc *__restrict__ s, int level, int trav, int begin,
-                                              int stride) {
+__device__ __forceinline__ void forward_one(const SlotDesc *__restrict__ s, int n, int h, int trav) {
     const int Mp = s->Mp, N = s->N, NP = s->NP;
-    const int lo = s->dec_off[level];
-    const int total = (s->dec_off[level + 1] - lo) * Mp;
-    for (int it = begin; it < total; it += stride) {
-        int ni = it / Mp, h = it - ni * Mp;
-        int n = s->dec_lvl[lo + ni];
-        const DevNode &nd = s->nodes[n];
-        int actor = nd.player, na = nd.nchild, fc = nd.first_child;
-        float sig[NT_DEC];
-        if (actor != trav && actor >= 0 && actor < NP && h < s->nh[actor])
-            regret_match(s->R, nd.act_base, na, Mp, h, s->frozen, n, sig);
-        for (int rp = 0; rp < NP; ++rp) {
-            if (rp == trav || h >= s->nh[rp]) continue;
-            float *rr = s->reach + (size_t)rp * N * Mp;
-            float pr = rr[(size_t)n * Mp + h];
-            for (int a = 0; a < na; ++a) {
-                int c = s->children[fc + a];
-                rr[(size_t)c * Mp + h] = rp == actor ? pr * sig[a] : pr;
-            }
+    const DevNode &nd = s->nodes[n];
+    int actor = nd.player, na = nd.nchild, fc = nd.first_child;
+    float sig[NT_DEC];
+    if (actor != trav && actor >= 0 && actor < NP && h < s->nh[actor])
+        regret_match(s->R, nd.act_base, na, Mp, h, s->frozen, n, sig);
+    for (int rp = 0; rp < NP; ++rp) {
+        if (rp == trav || h >= s->nh[rp]) continue;
+        float *rr = s->reach + (size_t)rp * N * Mp;
+        float pr = rr[(size_t)n * Mp + h];
+        for (int a = 0; a < na; ++a) {
+            int c = s->children[fc + a];
+            rr[(size_t)c * Mp + h] = rp == actor ? pr * sig[a] : pr;
         }
     }
 }
 
+__device__ __forceinline__ void forward_items(const SlotDesc *__restrict__ s, int level, int trav, int begin,
+                                              int stride) {
+    const int Mp = s->Mp;
+    const int lo = s->dec_off[level];
+    const int total = (s->dec_off[level + 1] - lo) * Mp;
+    for (int it = begin; it < total; it += stride) {
+        int ni = it / Mp, h = it - ni * Mp;
+        forward_one(s, s->dec_lvl[lo + ni], h, trav);
+    }
+}
+
 // ---------------------------------------------------------------------------
-// backward: value + regret / strategy-sum update through one level
+// backward: value + regret / strategy-sum update at decision node n, hand h
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ void backward_one(const SlotDesc *__restrict__ s, int n, int h, int trav, int iter) {
+    const int Mp = s->Mp;
+    if (h >= s->nh[trav]) return;
+    const DevNode &nd = s->nodes[n];
+    int na = nd.nchild, fc = nd.first_child;
+    float v = 0.f;
+    if (nd.player == trav) {
+        const float d = (float)iter / ((float)iter + 1.0f);
+        float sig[NT_DEC], cv[NT_DEC];
+        regret_match(s->R, nd.act_base, na, Mp, h, s->frozen, n, sig);
+        for (int a = 0; a < na; ++a) {
+            cv[a] = s->cfv[(size_t)s->children[fc + a] * Mp + h];
+            v += sig[a] * cv[a];
+        }
+        for (int a = 0; a < na; ++a) {
+            size_t row = (size_t)(nd.act_base + a) * Mp + h;
+            float r = s->R[row];
+            r += cv[a] - v;
+            r *= d;
+            s->R[row] = r;
+            s->SS[row] += (float)iter * sig[a];
+        }
+    } else {
+        for (int a = 0; a < na; ++a) v += s->cfv[(size_t)s->children[fc + a] * Mp + h];
+    }
+    s->cfv[(size_t)n * Mp + h] = v;
+}
+
 __device__ __forceinline__ void backward_items(const SlotDesc *__restrict__ s, int level, int trav, int iter,
                                                int begin, int stride) {
     const int Mp = s->Mp;
     const int lo = s->dec_off[level];
     const int total = (s->dec_off[level + 1] - lo) * Mp;
-    const int nh = s->nh[trav];
-    const float d = (float)iter / ((float)iter + 1.0f);
-    const float fit = (float)iter;
     for (int it = begin; it < total; it += stride) {
         int ni = it / Mp, h = it - ni * Mp;
-        if (h >= nh) continue;
-        int n = s->dec_lvl[lo + ni];
-        const DevNode &nd = s->nodes[n];
-        int na = nd.nchild, fc = nd.first_child;
-        float v = 0.f;
-        if (nd.player == trav) {
-            float sig[NT_DEC], cv[NT_DEC];
-            regret_match(s->R, nd.act_base, na, Mp, h, s->frozen, n, sig);
-            for (int a = 0; a < na; ++a) {
-                cv[a] = s->cfv[(size_t)s->children[fc + a] * Mp + h];
-                v += sig[a] * cv[a];
-            }
-            for (int a = 0; a < na; ++a) {
-                size_t row = (size_t)(nd.act_base + a) * Mp + h;
-                float r = s->R[row];
-                r += cv[a] - v;
-                r *= d;
-                s->R[row] = r;
-                s->SS[row] += fit * sig[a];
-            }
-        } else {
-            for (int a = 0; a < na; ++a) v += s->cfv[(size_t)s->children[fc + a] * Mp + h];
-        }
-        s->cfv[(size_t)n * Mp + h] = v;
+        backward_one(s, s->dec_lvl[lo + ni], h, trav, iter);
     }
 }
 
@@ -261,26 +267,31 @@ __device__ __forceinline__ T wave_incl_scan(T x, int lane) {
     return x;
 }
 
-// exclusive block scan of M[gsort[j]] into PG[0..ng]: per-thread chunk sums,
-// a wave64 scan of the chunk sums, wave totals through LDS
+// exclusive scan of M[gsort[j]] into PG[0..ng]. The decomposition is fixed
+// (256 chunks, a wave64 scan of chunk sums, 4 wave totals) whatever the
+// block size, so the rounding is the same in every form of the solver.
+#define SCAN_LANES 256
 template <typename T>
 __device__ void scan_sorted(const SlotDesc *__restrict__ s, const T *M, T *PG, T *red, int tid, int nt) {
     const int ng = s->ng;
-    const int chunk = (ng + nt - 1) / nt;
-    const int j0 = min(ng, tid * chunk), j1 = min(ng, j0 + chunk);
+    const int chunk = (ng + SCAN_LANES - 1) / SCAN_LANES;
+    const bool on = tid < SCAN_LANES;
+    const int j0 = on ? min(ng, tid * chunk) : 0, j1 = on ? min(ng, j0 + chunk) : 0;
     T part = 0;
     for (int j = j0; j < j1; ++j) part += M[s->gsort[j]];
     const int lane = tid & 63, w = tid >> 6;
     T incl = wave_incl_scan(part, lane);
-    if (lane == 63) red[w] = incl;
+    if (on && lane == 63) red[w] = incl;
     __syncthreads();
-    T run = incl - part;
-    for (int i = 0; i < w; ++i) run += red[i];
-    for (int j = j0; j < j1; ++j) {
-        PG[j] = run;
-        run += M[s->gsort[j]];
+    if (on) {
+        T run = incl - part;
+        for (int i = 0; i < w; ++i) run += red[i];
+        for (int j = j0; j < j1; ++j) {
+            PG[j] = run;
+            run += M[s->gsort[j]];
+        }
+        if (tid == SCAN_LANES - 1) PG[ng] = run;
     }
-    if (tid == nt - 1) PG[ng] = run;
     __syncthreads();
 }
 
@@ -342,20 +353,36 @@ __device__ __forceinline__ void scatter(const SlotDesc *__restrict__ s, int q, i
     }
 }
 
-// per-card totals (non-showdown) of M: C[c] = sum_d M[c,d]; optionally of M1*M2.
-// C arrays must be zero on entry (atomics in LDS); caller barriers after.
+// sum across the 64 lanes of a wavefront, fixed butterfly order
+template <typename T>
+__device__ __forceinline__ T wave_sum(T x) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
+}
+
+// per-card totals (non-showdown) of M: C[c] = sum_d M[c,d]; optionally of
+// M1*M2. One wavefront per card, lane = partner card: a fixed summation
+// order, so results do not depend on the block size or on scheduling.
 template <typename T>
 __device__ __forceinline__ void card_totals(const T *M1, const T *M2, T *C1, T *C2, T *CP, int tid, int nt) {
-    for (int i = tid; i < 52 * 52; i += nt) {
-        int c = i / 52, d = i - c * 52;
-        if (c == d) continue;
-        int k = combo_id(c, d);
-        T m1 = M1[k];
-        if (m1 != (T)0) atomicAdd(&C1[c], m1);
+    const int lane = tid & 63, nw = nt >> 6;
+    for (int c = tid >> 6; c < 52; c += nw) {
+        T a = 0, b = 0, p = 0;
+        if (lane < 52 && lane != c) {
+            int k = combo_id(c, lane);
+            a = M1[k];
+            if (M2) {
+                b = M2[k];
+                p = a * b;
+            }
+        }
+        a = wave_sum(a);
+        if (lane == 0) C1[c] = a;
         if (M2) {
-            T m2 = M2[k];
-            if (m2 != (T)0) atomicAdd(&C2[c], m2);
-            if (m1 * m2 != (T)0) atomicAdd(&CP[c], m1 * m2);
+            b = wave_sum(b);
+            p = wave_sum(p);
+            if (lane == 0) { C2[c] = b; CP[c] = p; }
         }
     }
 }
@@ -447,8 +474,6 @@ __device__ void terminal_node(const SlotDesc *__restrict__ s, int n, int trav, c
         B.M1[i] = 0;
         if (k == 2) B.M2[i] = 0;
     }
-    if (!sd)
-        for (int i = tid; i < 52; i += nt) { B.C1[i] = 0; B.C2[i] = 0; B.CP[i] = 0; }
     __syncthreads();
     if (k >= 1) scatter<T>(s, opp[0], n, reach, B.M1, tid, nt);
     if (k == 2) scatter<T>(s, opp[1], n, reach, B.M2, tid, nt);
@@ -672,52 +697,80 @@ __global__ void __launch_bounds__(SS_BATCH_BLOCK) k_batched(SlotDesc *__restrict
 // FP64 best response / exploitability
 // ---------------------------------------------------------------------------
 
-// average strategy, normalised as the host does in street_solve.cu:1597-1610
+// average strategy at decision node n, hand h, normalised as the host does
+// in street_solve.cu:1597-1610
+__device__ __forceinline__ void avg_one(const SlotDesc *__restrict__ s, int n, int h) {
+    const int Mp = s->Mp;
+    const DevNode &nd = s->nodes[n];
+    int na = nd.nchild;
+    float sum = 0.f;
+    for (int a = 0; a < na; ++a) {
+        float v = s->SS[(size_t)(nd.act_base + a) * Mp + h];
+        sum += v > 0.f ? v : 0.f;
+    }
+    float inv = sum > 0.f ? 1.0f / sum : 1.0f / (float)na;
+    for (int a = 0; a < na; ++a) {
+        float v = s->SS[(size_t)(nd.act_base + a) * Mp + h];
+        v = v > 0.f ? v : 0.f;
+        s->avg[(size_t)(nd.act_base + a) * Mp + h] = sum > 0.f ? v * inv : inv;
+    }
+}
+
 __device__ __forceinline__ void avg_items(const SlotDesc *__restrict__ s, int begin, int stride) {
     const int Mp = s->Mp;
     const int total = s->D * Mp;
     for (int it = begin; it < total; it += stride) {
         int di = it / Mp, h = it - di * Mp;
-        int n = s->dec_lvl[di];
-        const DevNode &nd = s->nodes[n];
-        int na = nd.nchild;
-        float sum = 0.f;
-        for (int a = 0; a < na; ++a) {
-            float v = s->SS[(size_t)(nd.act_base + a) * Mp + h];
-            sum += v > 0.f ? v : 0.f;
-        }
-        float inv = sum > 0.f ? 1.0f / sum : 1.0f / (float)na;
-        for (int a = 0; a < na; ++a) {
-            float v = s->SS[(size_t)(nd.act_base + a) * Mp + h];
-            v = v > 0.f ? v : 0.f;
-            s->avg[(size_t)(nd.act_base + a) * Mp + h] = sum > 0.f ? v * inv : inv;
+        avg_one(s, s->dec_lvl[di], h);
+    }
+}
+
+// reach of every player under the average strategy through node n, hand h
+__device__ __forceinline__ void xforward_one(const SlotDesc *__restrict__ s, int n, int h) {
+    const int Mp = s->Mp, N = s->N, NP = s->NP;
+    const DevNode &nd = s->nodes[n];
+    for (int q = 0; q < NP; ++q) {
+        if (h >= s->nh[q]) continue;
+        double *rr = s->xreach + (size_t)q * N * Mp;
+        double pr = rr[(size_t)n * Mp + h];
+        for (int a = 0; a < nd.nchild; ++a) {
+            int c = s->children[nd.first_child + a];
+            rr[(size_t)c * Mp + h] = q == nd.player ? pr * (double)s->avg[(size_t)(nd.act_base + a) * Mp + h] : pr;
         }
     }
 }
 
-// reach of every player under the average strategy
 __device__ __forceinline__ void xforward_items(const SlotDesc *__restrict__ s, int level, int begin, int stride) {
-    const int Mp = s->Mp, N = s->N, NP = s->NP;
+    const int Mp = s->Mp;
     const int lo = s->dec_off[level];
     const int total = (s->dec_off[level + 1] - lo) * Mp;
     for (int it = begin; it < total; it += stride) {
         int ni = it / Mp, h = it - ni * Mp;
-        int n = s->dec_lvl[lo + ni];
-        const DevNode &nd = s->nodes[n];
-        for (int q = 0; q < NP; ++q) {
-            if (h >= s->nh[q]) continue;
-            double *rr = s->xreach + (size_t)q * N * Mp;
-            double pr = rr[(size_t)n * Mp + h];
-            for (int a = 0; a < nd.nchild; ++a) {
-                int c = s->children[nd.first_child + a];
-                rr[(size_t)c * Mp + h] =
-                    q == nd.player ? pr * (double)s->avg[(size_t)(nd.act_base + a) * Mp + h] : pr;
-            }
-        }
+        xforward_one(s, s->dec_lvl[lo + ni], h);
     }
 }
 
 // best-response (max) and profile-value (avg-weighted) backup for player p
+__device__ __forceinline__ void xbackward_one(const SlotDesc *__restrict__ s, int n, int h, int p) {
+    const int Mp = s->Mp;
+    if (h >= s->nh[p]) return;
+    const DevNode &nd = s->nodes[n];
+    double e = 0, b = 0;
+    for (int a = 0; a < nd.nchild; ++a) {
+        int c = s->children[nd.first_child + a];
+        double ce = s->xev[(size_t)c * Mp + h], cb = s->xbr[(size_t)c * Mp + h];
+        if (nd.player == p) {
+            e += (double)s->avg[(size_t)(nd.act_base + a) * Mp + h] * ce;
+            b = (a == 0 || cb > b) ? cb : b;
+        } else {
+            e += ce;
+            b += cb;
+        }
+    }
+    s->xev[(size_t)n * Mp + h] = e;
+    s->xbr[(size_t)n * Mp + h] = b;
+}
+
 __device__ __forceinline__ void xbackward_items(const SlotDesc *__restrict__ s, int level, int p, int begin,
                                                 int stride) {
     const int Mp = s->Mp;
@@ -725,23 +778,7 @@ __device__ __forceinline__ void xbackward_items(const SlotDesc *__restrict__ s, 
     const int total = (s->dec_off[level + 1] - lo) * Mp;
     for (int it = begin; it < total; it += stride) {
         int ni = it / Mp, h = it - ni * Mp;
-        if (h >= s->nh[p]) continue;
-        int n = s->dec_lvl[lo + ni];
-        const DevNode &nd = s->nodes[n];
-        double e = 0, b = 0;
-        for (int a = 0; a < nd.nchild; ++a) {
-            int c = s->children[nd.first_child + a];
-            double ce = s->xev[(size_t)c * Mp + h], cb = s->xbr[(size_t)c * Mp + h];
-            if (nd.player == p) {
-                e += (double)s->avg[(size_t)(nd.act_base + a) * Mp + h] * ce;
-                b = (a == 0 || cb > b) ? cb : b;
-            } else {
-                e += ce;
-                b += cb;
-            }
-        }
-        s->xev[(size_t)n * Mp + h] = e;
-        s->xbr[(size_t)n * Mp + h] = b;
+        xbackward_one(s, s->dec_lvl[lo + ni], h, p);
     }
 }
 
@@ -863,6 +900,162 @@ __global__ void __launch_bounds__(SS_BLOCK) k_batched_expl(SlotDesc *__restrict_
 
 }  // namespace ss
 
+namespace ss {
+
+// ---------------------------------------------------------------------------
+// batched form, phased: every launch covers one phase of one traverser pass
+// for ALL resident subgames. Tasks are (slot << 20 | node); one workgroup per
+// task, so a subgame's nodes spread over every CU. Slots that are retired,
+// at their iteration cap, or have fewer players than the pass index skip.
+// ---------------------------------------------------------------------------
+#define TASK_SLOT(t) ((int)((t) >> 20))
+#define TASK_NODE(t) ((int)((t) & 0xFFFFF))
+
+__device__ __forceinline__ bool slot_runs(const SlotDesc *__restrict__ s, int trav) {
+    return s->active && s->iter < s->iter_end && trav < s->NP;
+}
+
+__global__ void __launch_bounds__(SS_BLOCK) k_fwd_phased(SlotDesc *__restrict__ slots, const uint32_t *tasks,
+                                                         int ntasks, int trav) {
+    for (int j = blockIdx.x; j < ntasks; j += gridDim.x) {
+        uint32_t tk = tasks[j];
+        const SlotDesc *s = slots + TASK_SLOT(tk);
+        if (!slot_runs(s, trav)) continue;
+        for (int h = threadIdx.x; h < s->Mp; h += blockDim.x) forward_one(s, TASK_NODE(tk), h, trav);
+    }
+}
+
+__global__ void __launch_bounds__(SS_BLOCK) k_term_phased(SlotDesc *__restrict__ slots, const uint32_t *tasks,
+                                                          int ntasks, int trav) {
+    __shared__ TermShared sh;
+    TermBuf<float> b = lds_buf(sh);
+    for (int j = blockIdx.x; j < ntasks; j += gridDim.x) {
+        uint32_t tk = tasks[j];
+        const SlotDesc *s = slots + TASK_SLOT(tk);
+        if (!slot_runs(s, trav)) continue;
+        // tables of this subgame, read through the cache
+        b.clist = s->clist;
+        b.clen = s->clen;
+        b.rank = s->cstr;
+        int n = TASK_NODE(tk);
+        terminal_node<float>(s, n, trav, s->reach, s->cfv + (size_t)n * s->Mp, b, TK_NODE, nullptr, threadIdx.x,
+                             blockDim.x);
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(SS_BLOCK) k_bwd_phased(SlotDesc *__restrict__ slots, const uint32_t *tasks,
+                                                         int ntasks, int trav) {
+    for (int j = blockIdx.x; j < ntasks; j += gridDim.x) {
+        uint32_t tk = tasks[j];
+        const SlotDesc *s = slots + TASK_SLOT(tk);
+        if (!slot_runs(s, trav)) continue;
+        int iter = s->iter + 1;
+        for (int h = threadIdx.x; h < s->Mp; h += blockDim.x) backward_one(s, TASK_NODE(tk), h, trav, iter);
+    }
+}
+
+// root strategy snapshot before the last pass of a final / check iteration
+__global__ void __launch_bounds__(SS_BLOCK) k_rsig_phased(SlotDesc *__restrict__ slots, int nslots, int trav) {
+    for (int si = blockIdx.x; si < nslots; si += gridDim.x) {
+        const SlotDesc *s = slots + si;
+        if (!slot_runs(s, trav) || trav != s->NP - 1) continue;
+        int it = s->iter + 1;
+        if (it == s->final_iter || (s->snap_every > 0 && it % s->snap_every == 0))
+            root_sigma_items(s, threadIdx.x, blockDim.x);
+    }
+}
+
+__global__ void k_bump_phased(SlotDesc *__restrict__ slots, int nslots) {
+    int si = blockIdx.x * blockDim.x + threadIdx.x;
+    if (si < nslots && slots[si].active && slots[si].iter < slots[si].iter_end) slots[si].iter += 1;
+}
+
+// FP64 exploitability, phased over every slot with want[slot] != 0
+__device__ __forceinline__ bool slot_wanted(const SlotDesc *__restrict__ s, const int *want, int si) {
+    return s->active && want[si];
+}
+
+__global__ void __launch_bounds__(SS_BLOCK) k_xavg_phased(SlotDesc *__restrict__ slots, const uint32_t *tasks,
+                                                          int ntasks, const int *want) {
+    for (int j = blockIdx.x; j < ntasks; j += gridDim.x) {
+        uint32_t tk = tasks[j];
+        const SlotDesc *s = slots + TASK_SLOT(tk);
+        if (!slot_wanted(s, want, TASK_SLOT(tk))) continue;
+        for (int h = threadIdx.x; h < s->Mp; h += blockDim.x) avg_one(s, TASK_NODE(tk), h);
+    }
+}
+
+__global__ void __launch_bounds__(SS_BLOCK) k_xfwd_phased(SlotDesc *__restrict__ slots, const uint32_t *tasks,
+                                                          int ntasks, const int *want) {
+    for (int j = blockIdx.x; j < ntasks; j += gridDim.x) {
+        uint32_t tk = tasks[j];
+        const SlotDesc *s = slots + TASK_SLOT(tk);
+        if (!slot_wanted(s, want, TASK_SLOT(tk))) continue;
+        for (int h = threadIdx.x; h < s->Mp; h += blockDim.x) xforward_one(s, TASK_NODE(tk), h);
+    }
+}
+
+__global__ void __launch_bounds__(SS_BLOCK) k_xterm_phased(SlotDesc *__restrict__ slots, const uint32_t *tasks,
+                                                           int ntasks, int p, const int *want, double *scratch) {
+    for (int j = blockIdx.x; j < ntasks; j += gridDim.x) {
+        uint32_t tk = tasks[j];
+        const SlotDesc *s = slots + TASK_SLOT(tk);
+        if (!slot_wanted(s, want, TASK_SLOT(tk)) || p >= s->NP) continue;
+        TermBuf<double> b = global_buf(scratch + (size_t)blockIdx.x * XSCRATCH_DOUBLES, s);
+        int n = TASK_NODE(tk);
+        terminal_node<double>(s, n, p, s->xreach, s->xev + (size_t)n * s->Mp, b, TK_NODE, nullptr, threadIdx.x,
+                              blockDim.x);
+        for (int h = threadIdx.x; h < s->nh[p]; h += blockDim.x)
+            s->xbr[(size_t)n * s->Mp + h] = s->xev[(size_t)n * s->Mp + h];
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(SS_BLOCK) k_xbwd_phased(SlotDesc *__restrict__ slots, const uint32_t *tasks,
+                                                          int ntasks, int p, const int *want) {
+    for (int j = blockIdx.x; j < ntasks; j += gridDim.x) {
+        uint32_t tk = tasks[j];
+        const SlotDesc *s = slots + TASK_SLOT(tk);
+        if (!slot_wanted(s, want, TASK_SLOT(tk)) || p >= s->NP) continue;
+        for (int h = threadIdx.x; h < s->Mp; h += blockDim.x) xbackward_one(s, TASK_NODE(tk), h, p);
+    }
+}
+
+__global__ void __launch_bounds__(SS_BLOCK) k_xreduce_phased(SlotDesc *__restrict__ slots, int nslots, int p,
+                                                             const int *want) {
+    __shared__ double red[2 * SS_BLOCK];
+    for (int si = blockIdx.x; si < nslots; si += gridDim.x) {
+        const SlotDesc *s = slots + si;
+        if (!slot_wanted(s, want, si) || p >= s->NP) continue;
+        xreduce_block(s, p, red, threadIdx.x, blockDim.x);
+    }
+}
+
+__global__ void __launch_bounds__(SS_BLOCK) k_xjoint_phased(SlotDesc *__restrict__ slots, int nslots,
+                                                            const int *want, double *scratch) {
+    __shared__ double red[SS_BLOCK];
+    for (int si = blockIdx.x; si < nslots; si += gridDim.x) {
+        const SlotDesc *s = slots + si;
+        if (!slot_wanted(s, want, si)) continue;
+        TermBuf<double> b = global_buf(scratch + (size_t)blockIdx.x * XSCRATCH_DOUBLES, s);
+        terminal_node<double>(s, 0, 0, s->xreach, s->xev, b, TK_MASS, s->weights, threadIdx.x, blockDim.x);
+        double acc = 0;
+        for (int h = threadIdx.x; h < s->nh[0]; h += blockDim.x)
+            if (s->hinfo[h] & SS_LIVE_BIT) acc += s->xev[h];
+        red[threadIdx.x] = acc;
+        __syncthreads();
+        for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+            if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) s->xout[2 * s->NP] = red[0];
+        __syncthreads();
+    }
+}
+
+}  // namespace ss
+
 // ===========================================================================
 // host launchers
 // ===========================================================================
@@ -957,6 +1150,119 @@ int launch_batched(SlotDesc *d_slots, int nslots, int iters, int grid, hipStream
 int launch_batched_expl(SlotDesc *d_slots, int nslots, double *scratch, const int *d_want, int grid,
                         hipStream_t st) {
     hipLaunchKernelGGL(k_batched_expl, dim3(grid), dim3(SS_BLOCK), 0, st, d_slots, nslots, scratch, d_want);
+    SS_HIP(hipGetLastError());
+    return SS_OK;
+}
+
+// ---- phased batched form ---------------------------------------------------
+PhasedTasks::~PhasedTasks() {
+    if (d_tasks) (void)hipFree(d_tasks);
+}
+
+// terminal cost class for scheduling: closed forms with 2 opponents at a
+// showdown are the heaviest (a 52-card loop per hand), odometers heavier still
+static int term_cost(const Plan &pl, int n) {
+    const DevNode &nd = pl.nodes[n];
+    int kmax = 0;
+    for (int t = 0; t < pl.NP; ++t) {
+        int k = 0;
+        for (int q = 0; q < pl.NP; ++q)
+            if (q != t && (nd.type == SS_NODE_FOLD || ((nd.active_mask >> q) & 1))) ++k;
+        kmax = std::max(kmax, k);
+    }
+    if (kmax >= 3) return 1000;
+    if (kmax == 2) return nd.type == SS_NODE_SHOWDOWN ? 100 : 20;
+    return 5;
+}
+
+int build_phased_tasks(const std::vector<Slot *> &slots, PhasedTasks &pt, hipStream_t st) {
+    pt.nslots = (int)slots.size();
+    pt.NPmax = 0;
+    pt.L = 0;
+    for (Slot *s : slots) {
+        pt.NPmax = std::max(pt.NPmax, s->plan.NP);
+        pt.L = std::max(pt.L, s->plan.L);
+        if (s->plan.N >= (1 << 20)) return set_error(SS_ERR_UNSUPPORTED, "tree too large for the batched form");
+    }
+    if (pt.nslots >= (1 << 11)) return set_error(SS_ERR_UNSUPPORTED, "at most 2047 subgames per batch");
+    std::vector<uint32_t> tasks;
+    pt.lvl_off.assign(pt.L + 1, 0);
+    for (int l = 0; l < pt.L; ++l) {
+        pt.lvl_off[l] = (int)tasks.size();
+        for (int si = 0; si < pt.nslots; ++si) {
+            const Plan &pl = slots[si]->plan;
+            if (l >= pl.L) continue;
+            for (int i = pl.dec_off[l]; i < pl.dec_off[l + 1]; ++i)
+                tasks.push_back(((uint32_t)si << 20) | (uint32_t)pl.dec_lvl[i]);
+        }
+    }
+    pt.lvl_off[pt.L] = (int)tasks.size();
+    std::vector<std::pair<int, uint32_t>> term;
+    for (int si = 0; si < pt.nslots; ++si) {
+        const Plan &pl = slots[si]->plan;
+        for (int j = 0; j < pl.T; ++j) term.push_back({term_cost(pl, pl.term[j]), ((uint32_t)si << 20) | (uint32_t)pl.term[j]});
+    }
+    std::stable_sort(term.begin(), term.end(), [](const std::pair<int, uint32_t> &a, const std::pair<int, uint32_t> &b) {
+        return a.first > b.first;
+    });
+    pt.term_off = (int)tasks.size();
+    pt.nterm = (int)term.size();
+    for (auto &t : term) tasks.push_back(t.second);
+    if (pt.d_tasks) { SS_HIP(hipFree(pt.d_tasks)); pt.d_tasks = nullptr; }
+    SS_HIP(hipMalloc(&pt.d_tasks, sizeof(uint32_t) * std::max<size_t>(1, tasks.size())));
+    SS_HIP(hipMemcpyAsync(pt.d_tasks, tasks.data(), sizeof(uint32_t) * tasks.size(), hipMemcpyHostToDevice, st));
+    SS_HIP(hipStreamSynchronize(st));
+    return SS_OK;
+}
+
+int launch_phased_iteration(SlotDesc *d_slots, const PhasedTasks &pt, hipStream_t st) {
+    for (int t = 0; t < pt.NPmax; ++t) {
+        hipLaunchKernelGGL(k_rsig_phased, dim3(pt.nslots), dim3(SS_BLOCK), 0, st, d_slots, pt.nslots, t);
+        for (int l = 0; l < pt.L; ++l) {
+            int cnt = pt.lvl_off[l + 1] - pt.lvl_off[l];
+            if (cnt > 0)
+                hipLaunchKernelGGL(k_fwd_phased, dim3(cnt), dim3(SS_BLOCK), 0, st, d_slots, pt.d_tasks + pt.lvl_off[l],
+                                   cnt, t);
+        }
+        if (pt.nterm > 0)
+            hipLaunchKernelGGL(k_term_phased, dim3(pt.nterm), dim3(SS_BLOCK), 0, st, d_slots, pt.d_tasks + pt.term_off,
+                               pt.nterm, t);
+        for (int l = pt.L - 1; l >= 0; --l) {
+            int cnt = pt.lvl_off[l + 1] - pt.lvl_off[l];
+            if (cnt > 0)
+                hipLaunchKernelGGL(k_bwd_phased, dim3(cnt), dim3(SS_BLOCK), 0, st, d_slots, pt.d_tasks + pt.lvl_off[l],
+                                   cnt, t);
+        }
+    }
+    hipLaunchKernelGGL(k_bump_phased, dim3((pt.nslots + 255) / 256), dim3(256), 0, st, d_slots, pt.nslots);
+    SS_HIP(hipGetLastError());
+    return SS_OK;
+}
+
+int launch_phased_expl(SlotDesc *d_slots, const PhasedTasks &pt, const int *d_want, double *scratch,
+                       int scratch_blocks, hipStream_t st) {
+    const int ndec = pt.lvl_off[pt.L];
+    if (ndec > 0) hipLaunchKernelGGL(k_xavg_phased, dim3(ndec), dim3(SS_BLOCK), 0, st, d_slots, pt.d_tasks, ndec, d_want);
+    for (int l = 0; l < pt.L; ++l) {
+        int cnt = pt.lvl_off[l + 1] - pt.lvl_off[l];
+        if (cnt > 0)
+            hipLaunchKernelGGL(k_xfwd_phased, dim3(cnt), dim3(SS_BLOCK), 0, st, d_slots, pt.d_tasks + pt.lvl_off[l], cnt,
+                               d_want);
+    }
+    for (int p = 0; p < pt.NPmax; ++p) {
+        if (pt.nterm > 0)
+            hipLaunchKernelGGL(k_xterm_phased, dim3(std::min(pt.nterm, scratch_blocks)), dim3(SS_BLOCK), 0, st, d_slots,
+                               pt.d_tasks + pt.term_off, pt.nterm, p, d_want, scratch);
+        for (int l = pt.L - 1; l >= 0; --l) {
+            int cnt = pt.lvl_off[l + 1] - pt.lvl_off[l];
+            if (cnt > 0)
+                hipLaunchKernelGGL(k_xbwd_phased, dim3(cnt), dim3(SS_BLOCK), 0, st, d_slots, pt.d_tasks + pt.lvl_off[l],
+                                   cnt, p, d_want);
+        }
+        hipLaunchKernelGGL(k_xreduce_phased, dim3(pt.nslots), dim3(SS_BLOCK), 0, st, d_slots, pt.nslots, p, d_want);
+    }
+    hipLaunchKernelGGL(k_xjoint_phased, dim3(std::min(pt.nslots, scratch_blocks)), dim3(SS_BLOCK), 0, st, d_slots,
+                       pt.nslots, d_want, scratch);
     SS_HIP(hipGetLastError());
     return SS_OK;
 }

@@ -134,6 +134,7 @@ class SSBatchStats(ctypes.Structure):
         ("algo_bytes", ctypes.c_int64),
         ("launches", ctypes.c_int),
         ("max_iterations", ctypes.c_int),
+        ("mode", ctypes.c_int),
     ]
 
 
