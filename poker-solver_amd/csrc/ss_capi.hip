@@ -98,6 +98,7 @@ struct BatchImpl {
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     PhasedTasks pt;
     int mode = 1;          // 0: persistent kernel, one workgroup per subgame; 1: phased launches
+    bool use_graph = true; // SS_NO_GRAPH=1 issues the same launches directly (profiling)
     int xblocks = 512;     // workgroups (and FP64 scratch slices) of the exploitability sweeps
     // results of the last run
     std::vector<int> iters, conv, expl_at;
@@ -154,6 +155,9 @@ struct BatchImpl {
             SS_HIP(hipMalloc(&scratch, sizeof(double) * (size_t)xblocks * SS_XSCRATCH_DOUBLES));
         rc = choose_mode();
         if (rc) return rc;
+        const char *ng = getenv("SS_NO_GRAPH");
+        use_graph = !(ng && atoi(ng) != 0);
+        if (!use_graph) return SS_OK;
         SS_HIP(hipMemcpyAsync(d_slots, templ.data(), sizeof(SlotDesc) * n, hipMemcpyHostToDevice, st));
         SS_HIP(hipStreamSynchronize(st));
         SS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -253,7 +257,14 @@ struct BatchImpl {
         for (int i = 0; i < n; ++i) live_bytes += slots[i]->plan.algo_bytes_iter;
         while (done < T) {
             SS_HIP(hipEventRecord(ev[0], st));
-            SS_HIP(hipGraphLaunch(exec, st));
+            if (use_graph) {
+                SS_HIP(hipGraphLaunch(exec, st));
+            } else {
+                for (int k = 0; k < chunk; ++k) {
+                    int rc = launch_iteration(mode);
+                    if (rc) return rc;
+                }
+            }
             SS_HIP(hipEventRecord(ev[1], st));
             SS_HIP(hipEventSynchronize(ev[1]));
             float ms = 0;
