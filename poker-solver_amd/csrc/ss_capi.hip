@@ -98,6 +98,7 @@ struct BatchImpl {
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     PhasedTasks pt;
     int mode = 1;          // 0: persistent kernel, one workgroup per subgame; 1: phased launches
+    int full_tasks = 0;
     bool use_graph = true; // SS_NO_GRAPH=1 issues the same launches directly (profiling)
     int xblocks = 512;     // workgroups (and FP64 scratch slices) of the exploitability sweeps
     // results of the last run
@@ -157,12 +158,20 @@ struct BatchImpl {
         if (rc) return rc;
         const char *ng = getenv("SS_NO_GRAPH");
         use_graph = !(ng && atoi(ng) != 0);
-        if (!use_graph) return SS_OK;
         SS_HIP(hipMemcpyAsync(d_slots, templ.data(), sizeof(SlotDesc) * n, hipMemcpyHostToDevice, st));
         SS_HIP(hipStreamSynchronize(st));
+        full_tasks = pt.lvl_off.empty() ? 0 : pt.lvl_off[pt.L] + pt.nterm;
+        return capture();
+    }
+
+    // capture `chunk` iterations of the current task lists into the graph
+    int capture() {
+        if (!use_graph) return SS_OK;
+        if (exec) { (void)hipGraphExecDestroy(exec); exec = nullptr; }
+        if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
         SS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         for (int k = 0; k < chunk; ++k) {
-            rc = launch_iteration(mode);
+            int rc = launch_iteration(mode);
             if (rc) {
                 (void)hipStreamEndCapture(st, &graph);
                 return rc;
@@ -171,6 +180,23 @@ struct BatchImpl {
         SS_HIP(hipStreamEndCapture(st, &graph));
         SS_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
         return SS_OK;
+    }
+
+    // Phased launches are sized by their task lists: when retired subgames
+    // hold a third or more of the tasks, rebuild the lists over the live ones
+    // and recapture, so the tail of a batch does not dispatch empty workgroups.
+    int compact(const std::vector<int> &live) {
+        if (mode != 1) return SS_OK;
+        int64_t live_tasks = 0;
+        for (int i = 0; i < n; ++i)
+            if (live[i]) live_tasks += slots[i]->plan.D + slots[i]->plan.T;
+        int cur = pt.lvl_off.empty() ? 0 : pt.lvl_off[pt.L] + pt.nterm;
+        if (live_tasks == 0 || live_tasks * 3 > (int64_t)cur * 2) return SS_OK;
+        std::vector<Slot *> raw;
+        for (auto &sl : slots) raw.push_back(sl.get());
+        int rc = build_phased_tasks(raw, pt, st, &live);
+        if (rc) return rc;
+        return capture();
     }
 
     int launch_iteration(int m) {
@@ -241,9 +267,16 @@ struct BatchImpl {
     int run(SSReport *reps, SSBatchStats *stats) {
         const int T = std::max(0, cfg.max_iterations);
         double t0 = now_ms();
-        // fresh regrets, strategy sums, root snapshot and descriptors
+        // fresh regrets, strategy sums, root snapshot and descriptors; full task lists
         int rc0 = reset_state();
         if (rc0) return rc0;
+        if (mode == 1 && pt.lvl_off[pt.L] + pt.nterm != full_tasks) {
+            std::vector<Slot *> raw;
+            for (auto &sl : slots) raw.push_back(sl.get());
+            rc0 = build_phased_tasks(raw, pt, st);
+            if (!rc0) rc0 = capture();
+            if (rc0) return rc0;
+        }
         iters.assign(n, 0);
         conv.assign(n, 0);
         expl_at.assign(n, -1);
@@ -295,6 +328,8 @@ struct BatchImpl {
                 }
                 SS_HIP(hipStreamSynchronize(st));
                 if (nlive == 0) break;
+                int rc2 = compact(live);
+                if (rc2) return rc2;
             }
         }
         if (want_final) {

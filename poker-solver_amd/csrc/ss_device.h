@@ -18,12 +18,16 @@
 //   rsig      float[8][Mp]          strategy at node 0 taken before the last pass
 //   river tables (showdown closed forms, see ss_kernels.hip):
 //   cstr      uint32[1326]          7-card rank of each combo (0 = blocked)
-//   gsort     int16[ng]             live combos by ascending rank
+//   gsort     uint16[ng]            live combos by ascending rank (cards a | b << 8)
 //   glt, gle  int16[1326]           # live combos ranked <, <= the combo
 //   clist     uint8[52][52]         per card: partner cards by ascending rank
 //   clen      uint8[52]
 //   plt, ple  uint8[52][1326]       per card c and combo k: # entries of c's
 //                                   list ranked <, <= rank(k)
+//   cpos      uint8[52][52]         position of partner d in card c's list
+//   hsort     uint16[NP][Mp]        each player's hands by ascending strength: the
+//                                   showdown sweeps walk hands in this order so a
+//                                   wavefront shares strength-class boundaries
 #pragma once
 #include <stdint.h>
 
@@ -68,13 +72,15 @@ struct SlotDesc {
     float *reach;
     float *rsig;
     const uint32_t *cstr;
-    const int16_t *gsort;
+    const uint16_t *gsort;
     const int16_t *glt;
     const int16_t *gle;
     const uint8_t *clist;
     const uint8_t *clen;
     const uint8_t *plt;
     const uint8_t *ple;
+    const uint8_t *cpos;
+    const uint16_t *hsort;     // [NP][Mp] hand indices by ascending strength (river), identity otherwise
     // exploitability scratch (FP64), allocated on demand
     float *avg;                // [A][Mp]
     double *xreach;            // [NP][N][Mp]

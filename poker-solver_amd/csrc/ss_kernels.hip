@@ -227,17 +227,25 @@ __device__ __forceinline__ void root_sigma_items(const SlotDesc *__restrict__ s,
 
 // ---------------------------------------------------------------------------
 // terminal values
+//
+// An opponent's reach at a node is scattered into a dense symmetric 52x52
+// card-pair matrix M (row c, column d = reach of the hand {c,d}; board cards
+// give zero rows), with row sums C[c] and total S = sum(C)/2.
 // ---------------------------------------------------------------------------
+#define DN 52
+#define DN2 (52 * 52)
+
 template <typename T>
 struct TermBuf {
-    T *M1, *M2;          // [1326] opponent reach in combo space
-    T *PC1, *PC2, *PCP;  // [52*52] per-card prefix over strength-sorted partners
+    T *M1, *M2;          // [52*52] dense opponent reach
+    T *PC1, *PC2, *PCP;  // [52*52] per-card prefix over strength-sorted partners; PC1 holds G for 2-opponent folds
     T *PG;               // [1328] prefix over strength-sorted combos
-    T *C1, *C2, *CP;     // [52] per-card totals
-    T *red;              // [SS_BLOCK] scan scratch
-    const uint8_t *clist;  // [52*52] per-card partner lists (LDS copy when available)
+    T *C1, *C2, *CP;     // [52] per-card totals (CP of M1 .* M2)
+    T *U1, *U2;          // [52] C1 * M2 and C2 * M1
+    T *red;              // [64] wave partials
+    const uint8_t *clist;  // [52*52] per-card partner lists, ascending strength
     const uint8_t *clen;   // [52]
-    const uint32_t *rank;  // [1326] combo ranks
+    const uint8_t *cpos;   // [52*52] position of partner d in card c's list
 };
 
 // A card-removal mass is a sum of non-negative reach products; evaluated by
@@ -267,18 +275,29 @@ __device__ __forceinline__ T wave_incl_scan(T x, int lane) {
     return x;
 }
 
-// exclusive scan of M[gsort[j]] into PG[0..ng]. The decomposition is fixed
-// (256 chunks, a wave64 scan of chunk sums, 4 wave totals) whatever the
-// block size, so the rounding is the same in every form of the solver.
+// sum across the 64 lanes of a wavefront, fixed butterfly order
+template <typename T>
+__device__ __forceinline__ T wave_sum(T x) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
+}
+
+// exclusive scan of M over the strength-sorted live combos into PG[0..ng].
+// The decomposition is fixed (256 chunks, a wave64 scan of chunk sums, 4
+// wave totals) whatever the block size, so every form rounds the same way.
 #define SCAN_LANES 256
 template <typename T>
-__device__ void scan_sorted(const SlotDesc *__restrict__ s, const T *M, T *PG, T *red, int tid, int nt) {
+__device__ void scan_sorted(const SlotDesc *__restrict__ s, const T *M, T *PG, T *red, int tid) {
     const int ng = s->ng;
     const int chunk = (ng + SCAN_LANES - 1) / SCAN_LANES;
     const bool on = tid < SCAN_LANES;
     const int j0 = on ? min(ng, tid * chunk) : 0, j1 = on ? min(ng, j0 + chunk) : 0;
     T part = 0;
-    for (int j = j0; j < j1; ++j) part += M[s->gsort[j]];
+    for (int j = j0; j < j1; ++j) {
+        uint32_t g = s->gsort[j];
+        part += M[(g & 0xFF) * DN + (g >> 8)];
+    }
     const int lane = tid & 63, w = tid >> 6;
     T incl = wave_incl_scan(part, lane);
     if (on && lane == 63) red[w] = incl;
@@ -288,7 +307,8 @@ __device__ void scan_sorted(const SlotDesc *__restrict__ s, const T *M, T *PG, T
         for (int i = 0; i < w; ++i) run += red[i];
         for (int j = j0; j < j1; ++j) {
             PG[j] = run;
-            run += M[s->gsort[j]];
+            uint32_t g = s->gsort[j];
+            run += M[(g & 0xFF) * DN + (g >> 8)];
         }
         if (tid == SCAN_LANES - 1) PG[ng] = run;
     }
@@ -342,38 +362,33 @@ __device__ T odometer(const SlotDesc *__restrict__ s, int n, int trav, int h, co
     return acc;
 }
 
-// Scatter opponent q's reach at node n into combo space.
+// Scatter opponent q's reach at node n into the dense card-pair matrix.
+// Each combo owns two cells, so with unique hands every cell gets one add.
 template <typename T>
 __device__ __forceinline__ void scatter(const SlotDesc *__restrict__ s, int q, int n, const T *reach, T *M,
                                         int tid, int nt) {
     const int Mp = s->Mp;
     for (int o = tid; o < s->nh[q]; o += nt) {
         uint32_t oi = s->hinfo[(size_t)q * Mp + o];
-        if (oi & SS_LIVE_BIT) atomicAdd(&M[(oi >> 12) & 2047], reach[((size_t)q * s->N + n) * Mp + o]);
+        if (!(oi & SS_LIVE_BIT)) continue;
+        int a = oi & 63, b = (oi >> 6) & 63;
+        T r = reach[((size_t)q * s->N + n) * Mp + o];
+        atomicAdd(&M[a * DN + b], r);
+        atomicAdd(&M[b * DN + a], r);
     }
 }
 
-// sum across the 64 lanes of a wavefront, fixed butterfly order
-template <typename T>
-__device__ __forceinline__ T wave_sum(T x) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
-    return x;
-}
-
-// per-card totals (non-showdown) of M: C[c] = sum_d M[c,d]; optionally of
-// M1*M2. One wavefront per card, lane = partner card: a fixed summation
-// order, so results do not depend on the block size or on scheduling.
+// per-card totals of M1 (and M2, M1.*M2): one wavefront per card, lane =
+// partner card, fixed butterfly order
 template <typename T>
 __device__ __forceinline__ void card_totals(const T *M1, const T *M2, T *C1, T *C2, T *CP, int tid, int nt) {
     const int lane = tid & 63, nw = nt >> 6;
     for (int c = tid >> 6; c < 52; c += nw) {
         T a = 0, b = 0, p = 0;
-        if (lane < 52 && lane != c) {
-            int k = combo_id(c, lane);
-            a = M1[k];
+        if (lane < 52) {
+            a = M1[c * DN + lane];
             if (M2) {
-                b = M2[k];
+                b = M2[c * DN + lane];
                 p = a * b;
             }
         }
@@ -388,19 +403,17 @@ __device__ __forceinline__ void card_totals(const T *M1, const T *M2, T *C1, T *
 }
 
 // per-card prefix sums over strength-sorted partner lists (showdown):
-// PC[c*52 + j] = sum of the first j partners of card c. Gather in list order,
-// then one wave64 scan per card.
+// PC[c*52 + j] = sum of the first j partners of card c; C = list totals.
 template <typename T>
-__device__ __forceinline__ void card_prefix(const SlotDesc *__restrict__ s, const uint8_t *clist,
-                                            const uint8_t *clen, const T *M1, const T *M2, T *PC1, T *PC2,
-                                            T *PCP, T *C1, T *C2, T *CP, int tid, int nt) {
-    for (int i = tid; i < 52 * 52; i += nt) {
-        int c = i / 52, j = i - c * 52;
+__device__ __forceinline__ void card_prefix(const uint8_t *clist, const uint8_t *clen, const T *M1, const T *M2,
+                                            T *PC1, T *PC2, T *PCP, T *C1, T *C2, T *CP, int tid, int nt) {
+    for (int i = tid; i < DN2; i += nt) {
+        int c = i / DN, j = i - c * DN;
         T a = 0, b = 0;
         if (j > 0 && j - 1 < clen[c]) {
-            int k = combo_id(c, clist[c * 52 + j - 1]);
-            a = M1[k];
-            if (M2) b = M2[k];
+            int d = clist[c * DN + j - 1];
+            a = M1[c * DN + d];
+            if (M2) b = M2[c * DN + d];
         }
         PC1[i] = a;
         if (M2) { PC2[i] = b; PCP[i] = a * b; }
@@ -408,7 +421,7 @@ __device__ __forceinline__ void card_prefix(const SlotDesc *__restrict__ s, cons
     __syncthreads();
     const int lane = tid & 63, nw = nt >> 6;
     for (int c = tid >> 6; c < 52; c += nw) {
-        const int at = c * 52 + 1 + lane;
+        const int at = c * DN + 1 + lane;
         T a = lane < 51 ? PC1[at] : (T)0;
         a = wave_incl_scan(a, lane);
         if (lane < 51) PC1[at] = a;
@@ -426,13 +439,38 @@ __device__ __forceinline__ void card_prefix(const SlotDesc *__restrict__ s, cons
     }
 }
 
+// Two-opponent fold/leaf precompute: G[p][q] = sum_c M1[c][p] M2[c][q],
+// U1 = C1 * M2, U2 = C2 * M1. Makes the per-hand 2-opponent mass O(1):
+//   X(x,y) = T - C1[x]C2[x] - C1[y]C2[y] - U1[x] - U1[y] + C1[y]M2[y][x] + C1[x]M2[x][y]
+//            - U2[x] - U2[y] + C2[y]M1[y][x] + C2[x]M1[x][y]
+//            + CP[x] + CP[y] - 2 M1[x][y]M2[x][y] + G[x][y] + G[y][x]
+template <typename T>
+__device__ __forceinline__ void pair_products(const T *M1, const T *M2, const T *C1, const T *C2, T *G, T *U1,
+                                              T *U2, int tid, int nt) {
+    for (int i = tid; i < DN2; i += nt) {
+        int p = i / DN, q = i - p * DN;
+        T g = 0;
+        for (int c = 0; c < 52; ++c) g += M1[c * DN + p] * M2[c * DN + q];
+        G[i] = g;
+    }
+    for (int z = tid; z < 2 * DN; z += nt) {
+        int col = z < DN ? z : z - DN;
+        const T *Cv = z < DN ? C1 : C2;
+        const T *Mv = z < DN ? M2 : M1;
+        T u = 0;
+        for (int c = 0; c < 52; ++c) u += Cv[c] * Mv[c * DN + col];
+        if (z < DN) U1[col] = u; else U2[col] = u;
+    }
+}
+
 // Kinds of terminal evaluation
 #define TK_NODE 0      // the node's own type (fold / showdown / leaf)
 #define TK_MASS 1      // opponent mass only, all non-traversers, times `scale`
 
-// Writes out[h] for h < nh[trav] of node n for traverser trav.
-// kind TK_MASS: out[h] = scale[h] * mass(all other players at node n).
-template <typename T>
+// Writes out[h] for h < nh[trav] of node n for traverser trav. HEAVY = false
+// compiles only the 0/1-opponent closed forms (callers guarantee no node with
+// more opponents reaches it).
+template <typename T, bool HEAVY>
 __device__ void terminal_node(const SlotDesc *__restrict__ s, int n, int trav, const T *reach, T *out,
                               TermBuf<T> B, int kind, const float *scale, int tid, int nt) {
     const DevNode &nd = s->nodes[n];
@@ -451,7 +489,13 @@ __device__ void terminal_node(const SlotDesc *__restrict__ s, int n, int trav, c
         return;
     }
     const T pot = (T)nd.pot, bet = (T)nd.bets[trav];
-    if (k >= 3) {
+    auto factor = [&](int h) -> T {
+        if (kind == TK_MASS) return (T)scale[h];
+        if (type == SS_NODE_FOLD)                                              // :726-733
+            return trav == nd.player ? (T)(nd.pot - nd.bets[trav]) : -(T)nd.bets[trav];
+        return (T)s->leaf_values[((size_t)nd.leaf_idx * NP + trav) * s->maxh + h];
+    };
+    if (HEAVY && k >= 3) {
         for (int h = tid; h < nh; h += nt) {
             T v;
             if (type == SS_NODE_SHOWDOWN) {
@@ -459,10 +503,7 @@ __device__ void terminal_node(const SlotDesc *__restrict__ s, int n, int trav, c
                 v = (hi & SS_LIVE_BIT) && s->cstr[(hi >> 12) & 2047]
                         ? odometer<T>(s, n, trav, h, opp, k, reach, true, pot, bet) : (T)0;
             } else {
-                T m = odometer<T>(s, n, trav, h, opp, k, reach, false, pot, bet);
-                v = m * (kind == TK_MASS ? (T)scale[h]
-                         : type == SS_NODE_FOLD ? (trav == nd.player ? (T)(nd.pot - nd.bets[trav]) : -(T)nd.bets[trav])
-                         : (T)s->leaf_values[((size_t)nd.leaf_idx * NP + trav) * s->maxh + h]);
+                v = odometer<T>(s, n, trav, h, opp, k, reach, false, pot, bet) * factor(h);
             }
             out[h] = v;
         }
@@ -470,66 +511,73 @@ __device__ void terminal_node(const SlotDesc *__restrict__ s, int n, int trav, c
     }
     // ---- closed forms, k <= 2 ----
     const bool sd = type == SS_NODE_SHOWDOWN;
-    for (int i = tid; i < SS_NCOMBO; i += nt) {
+    const bool two = HEAVY && k == 2;
+    for (int i = tid; i < DN2; i += nt) {
         B.M1[i] = 0;
-        if (k == 2) B.M2[i] = 0;
+        if (two) B.M2[i] = 0;
     }
     __syncthreads();
     if (k >= 1) scatter<T>(s, opp[0], n, reach, B.M1, tid, nt);
-    if (k == 2) scatter<T>(s, opp[1], n, reach, B.M2, tid, nt);
+    if (two) scatter<T>(s, opp[1], n, reach, B.M2, tid, nt);
     __syncthreads();
     if (sd && k >= 1)
-        card_prefix<T>(s, B.clist, B.clen, B.M1, k == 2 ? B.M2 : nullptr, B.PC1, B.PC2, B.PCP, B.C1, B.C2, B.CP,
-                       tid, nt);
-    else if (k >= 1) card_totals<T>(B.M1, k == 2 ? B.M2 : nullptr, B.C1, B.C2, B.CP, tid, nt);
+        card_prefix<T>(B.clist, B.clen, B.M1, two ? B.M2 : nullptr, B.PC1, B.PC2, B.PCP, B.C1, B.C2, B.CP, tid, nt);
+    else if (k >= 1)
+        card_totals<T>(B.M1, two ? B.M2 : nullptr, B.C1, B.C2, B.CP, tid, nt);
     __syncthreads();
-    if (sd && k == 1) scan_sorted<T>(s, B.M1, B.PG, B.red, tid, nt);
+    if (sd && k == 1) scan_sorted<T>(s, B.M1, B.PG, B.red, tid);
+    if (two && !sd) {
+        pair_products<T>(B.M1, B.M2, B.C1, B.C2, B.PC1, B.U1, B.U2, tid, nt);
+        __syncthreads();
+    }
 
-    T S1 = 0, S2 = 0, SP = 0;
+    T S1 = 0, S2 = 0, SP = 0, TT = 0;
     if (k >= 1) S1 = (T)0.5 * tsum52(B.C1);
-    if (k == 2) { S2 = (T)0.5 * tsum52(B.C2); SP = (T)0.5 * tsum52(B.CP); }
+    if (two) {
+        S2 = (T)0.5 * tsum52(B.C2);
+        SP = (T)0.5 * tsum52(B.CP);
+        if (!sd)
+            for (int c = 0; c < 52; ++c) TT += B.C1[c] * B.C2[c];
+    }
 
-    for (int h = tid; h < nh; h += nt) {
+    for (int i = tid; i < nh; i += nt) {
+        // showdowns walk the traverser's hands by strength (shared class
+        // boundaries within a wavefront turn prefix reads into broadcasts)
+        const int h = sd ? (int)s->hsort[(size_t)trav * Mp + i] : i;
         uint32_t hi = s->hinfo[(size_t)trav * Mp + h];
         int x = hi & 63, y = (hi >> 6) & 63;
-        int kk = combo_id(x, y);
+        int kk = (hi >> 12) & 2047;
+        const int xy = x * DN + y;
         T v;
         if (!sd) {
             T m;
             if (k == 0) m = 1;
             else {
-                T z1 = S1 - B.C1[x] - B.C1[y] + B.M1[kk];
-                if (k == 1) m = z1 < mass_floor<T>() * S1 ? (T)0 : z1;
+                T z1 = S1 - B.C1[x] - B.C1[y] + B.M1[xy];
+                if (!two) m = z1 < mass_floor<T>() * S1 ? (T)0 : z1;
                 else {
-                    T z2 = S2 - B.C2[x] - B.C2[y] + B.M2[kk];
-                    T zp = SP - B.CP[x] - B.CP[y] + B.M1[kk] * B.M2[kk];
-                    T X = 0;
-                    for (int c = 0; c < 52; ++c) {
-                        if (c == x || c == y) continue;
-                        int kx = combo_id(c, x), ky = combo_id(c, y);
-                        X += (B.C1[c] - B.M1[kx] - B.M1[ky]) * (B.C2[c] - B.M2[kx] - B.M2[ky]);
-                    }
+                    T m1 = B.M1[xy], m2 = B.M2[xy];
+                    T z2 = S2 - B.C2[x] - B.C2[y] + m2;
+                    T zp = SP - B.CP[x] - B.CP[y] + m1 * m2;
+                    T X = TT - B.C1[x] * B.C2[x] - B.C1[y] * B.C2[y] - B.U1[x] - B.U1[y] + B.C1[y] * m2 +
+                          B.C1[x] * m2 - B.U2[x] - B.U2[y] + B.C2[y] * m1 + B.C2[x] * m1 + B.CP[x] + B.CP[y] -
+                          (T)2 * m1 * m2 + B.PC1[xy] + B.PC1[y * DN + x];
                     m = z1 * z2 - X + zp;
                     if (m < mass_floor<T>() * S1 * S2) m = 0;
                 }
             }
-            T f;
-            if (kind == TK_MASS) f = (T)scale[h];
-            else if (type == SS_NODE_FOLD)                                   // :726-733
-                f = trav == nd.player ? (T)(nd.pot - nd.bets[trav]) : -(T)nd.bets[trav];
-            else f = (T)s->leaf_values[((size_t)nd.leaf_idx * NP + trav) * s->maxh + h];
-            v = f * m;
+            v = factor(h) * m;
         } else {
             uint32_t hs = (hi & SS_LIVE_BIT) ? s->cstr[kk] : 0u;
             if (hs == 0) { out[h] = 0; continue; }                            // :861
             if (k == 0) {
                 v = pot - bet;
-            } else if (k == 1) {                                               // :886-903
+            } else if (!two) {                                                 // :886-903
                 int lx = s->plt[x * SS_NCOMBO + kk], ly = s->plt[y * SS_NCOMBO + kk];
                 int ex = s->ple[x * SS_NCOMBO + kk], ey = s->ple[y * SS_NCOMBO + kk];
-                T lt = B.PG[s->glt[kk]] - B.PC1[x * 52 + lx] - B.PC1[y * 52 + ly];
-                T le = B.PG[s->gle[kk]] - B.PC1[x * 52 + ex] - B.PC1[y * 52 + ey] + B.M1[kk];
-                T z = S1 - B.C1[x] - B.C1[y] + B.M1[kk];
+                T lt = B.PG[s->glt[kk]] - B.PC1[x * DN + lx] - B.PC1[y * DN + ly];
+                T le = B.PG[s->gle[kk]] - B.PC1[x * DN + ex] - B.PC1[y * DN + ey] + B.M1[xy];
+                T z = S1 - B.C1[x] - B.C1[y] + B.M1[xy];
                 v = z < mass_floor<T>() * S1 ? (T)0 : pot * lt + pot * (T)0.5 * (le - lt) - bet * z;
             } else {                                                           // :936-1004, n_opp = 2
                 T h1l = 0, h1e = 0, h2l = 0, h2e = 0, hpl = 0, hpe = 0;
@@ -539,23 +587,25 @@ __device__ void terminal_node(const SlotDesc *__restrict__ s, int n, int trav, c
                 for (int c = 0; c < 52; ++c) {
                     if (B.clen[c] == 0) continue;
                     int l = s->plt[c * SS_NCOMBO + kk], e = s->ple[c * SS_NCOMBO + kk];
-                    T al = B.PC1[c * 52 + l], ae = B.PC1[c * 52 + e] - al;
-                    T bl = B.PC2[c * 52 + l], be = B.PC2[c * 52 + e] - bl;
-                    T pl = B.PCP[c * 52 + l], pe = B.PCP[c * 52 + e] - pl;
+                    T al = B.PC1[c * DN + l], ae = B.PC1[c * DN + e] - al;
+                    T bl = B.PC2[c * DN + l], be = B.PC2[c * DN + e] - bl;
+                    T pl = B.PCP[c * DN + l], pe = B.PCP[c * DN + e] - pl;
                     h1l += al; h1e += ae; h2l += bl; h2e += be; hpl += pl; hpe += pe;
                     if (c == x) { a_lx = al; a_ex = ae; b_lx = bl; b_ex = be; p_lx = pl; p_ex = pe; continue; }
                     if (c == y) { a_ly = al; a_ey = ae; b_ly = bl; b_ey = be; p_ly = pl; p_ey = pe; continue; }
                     T aa = B.C1[c], ba = B.C2[c];
-                    int kx = combo_id(c, x), ky = combo_id(c, y);
-                    uint32_t rx = B.rank[kx], ry = B.rank[ky];
-                    T m1x = B.M1[kx], m1y = B.M1[ky], m2x = B.M2[kx], m2y = B.M2[ky];
-                    if (rx < hs) { al -= m1x; bl -= m2x; } else if (rx == hs) { ae -= m1x; be -= m2x; }
-                    if (ry < hs) { al -= m1y; bl -= m2y; } else if (ry == hs) { ae -= m1y; be -= m2y; }
+                    // class of the hands {c,x}, {c,y} relative to the traverser's
+                    // strength, from their positions in c's sorted partner list
+                    int px = B.cpos[c * DN + x], py = B.cpos[c * DN + y];
+                    T m1x = B.M1[c * DN + x], m1y = B.M1[c * DN + y];
+                    T m2x = B.M2[c * DN + x], m2y = B.M2[c * DN + y];
+                    if (px < l) { al -= m1x; bl -= m2x; } else if (px < e) { ae -= m1x; be -= m2x; }
+                    if (py < l) { al -= m1y; bl -= m2y; } else if (py < e) { ae -= m1y; be -= m2y; }
                     aa -= m1x + m1y;
                     ba -= m2x + m2y;
                     xll += al * bl; xle += al * be; xel += ae * bl; xee += ae * be; xaa += aa * ba;
                 }
-                T m1k = B.M1[kk], m2k = B.M2[kk];
+                T m1k = B.M1[xy], m2k = B.M2[xy];
                 T Al = (T)0.5 * h1l - a_lx - a_ly, Ae = (T)0.5 * h1e - a_ex - a_ey + m1k;
                 T Bl = (T)0.5 * h2l - b_lx - b_ly, Be = (T)0.5 * h2e - b_ex - b_ey + m2k;
                 T Pl = (T)0.5 * hpl - p_lx - p_ly, Pe = (T)0.5 * hpe - p_ex - p_ey + m1k * m2k;
@@ -575,48 +625,67 @@ __device__ void terminal_node(const SlotDesc *__restrict__ s, int n, int trav, c
     __syncthreads();
 }
 
-// LDS terminal buffers for the FP32 sweep (55 KB)
-// LDS of the FP32 terminal sweep (~58 KB): reach in combo space, prefix
-// sums, and a copy of the subgame's per-card lists and combo ranks
-struct TermShared {
-    float M1[SS_NCOMBO], M2[SS_NCOMBO];
-    float PC1[52 * 52], PC2[52 * 52], PCP[52 * 52];
+// LDS of the FP32 terminal sweeps. Heavy (2-opponent nodes, ~66 KB, gfx950
+// allows 160 KB per workgroup): both opponents' matrices, three prefix
+// tables, the subgame's per-card lists. Light (<= 1 opponent, ~30 KB): one
+// matrix, one prefix table; five workgroups fit a CU.
+struct TermHeavy {
+    float M1[DN2], M2[DN2];
+    float PC1[DN2], PC2[DN2], PCP[DN2];
     float PG[1328];
-    float C1[52], C2[52], CP[52];
+    float C1[52], C2[52], CP[52], U1[52], U2[52];
     float red[64];
-    uint32_t rank[SS_NCOMBO];
-    uint8_t clist[52 * 52];
+    uint8_t clist[DN2];
+    uint8_t cpos[DN2];
     uint8_t clen[52];
 };
 
-__device__ __forceinline__ TermBuf<float> lds_buf(TermShared &t) {
+struct TermLight {
+    float M1[DN2];
+    float PC1[DN2];
+    float PG[1328];
+    float C1[52];
+    float red[64];
+};
+
+__device__ __forceinline__ TermBuf<float> heavy_buf(TermHeavy &t) {
     TermBuf<float> b;
     b.M1 = t.M1; b.M2 = t.M2; b.PC1 = t.PC1; b.PC2 = t.PC2; b.PCP = t.PCP;
-    b.PG = t.PG; b.C1 = t.C1; b.C2 = t.C2; b.CP = t.CP; b.red = t.red;
-    b.clist = t.clist; b.clen = t.clen; b.rank = t.rank;
+    b.PG = t.PG; b.C1 = t.C1; b.C2 = t.C2; b.CP = t.CP; b.U1 = t.U1; b.U2 = t.U2; b.red = t.red;
+    b.clist = t.clist; b.clen = t.clen; b.cpos = t.cpos;
     return b;
 }
 
-// copy a river subgame's per-card lists and ranks into LDS (caller barriers)
-__device__ __forceinline__ void stage_tables(const SlotDesc *__restrict__ s, TermShared &t, int tid, int nt) {
+// copy a river subgame's per-card lists and positions into LDS (caller barriers)
+__device__ __forceinline__ void stage_tables(const SlotDesc *__restrict__ s, TermHeavy &t, int tid, int nt) {
     if (!s->is_river) return;
-    for (int i = tid; i < 52 * 52; i += nt) t.clist[i] = s->clist[i];
+    for (int i = tid; i < DN2; i += nt) {
+        t.clist[i] = s->clist[i];
+        t.cpos[i] = s->cpos[i];
+    }
     for (int i = tid; i < 52; i += nt) t.clen[i] = s->clen[i];
-    for (int i = tid; i < SS_NCOMBO; i += nt) t.rank[i] = s->cstr[i];
+}
+
+__device__ __forceinline__ TermBuf<float> light_buf(TermLight &t, const SlotDesc *__restrict__ s) {
+    TermBuf<float> b;
+    b.M1 = t.M1; b.M2 = nullptr; b.PC1 = t.PC1; b.PC2 = nullptr; b.PCP = nullptr;
+    b.PG = t.PG; b.C1 = t.C1; b.C2 = nullptr; b.CP = nullptr; b.U1 = nullptr; b.U2 = nullptr; b.red = t.red;
+    b.clist = s->clist; b.clen = s->clen; b.cpos = s->cpos;
+    return b;
 }
 
 // FP64 buffers live in global scratch: one slice per workgroup
-#define XSCRATCH_DOUBLES (2 * SS_NCOMBO + 3 * 2704 + 1328 + 3 * 52 + SS_BLOCK)
+#define XSCRATCH_DOUBLES (5 * DN2 + 1328 + 5 * 52 + 64)
 __device__ __forceinline__ TermBuf<double> global_buf(double *base, const SlotDesc *__restrict__ s) {
     TermBuf<double> b;
+    b.M1 = base; b.M2 = b.M1 + DN2;
+    b.PC1 = b.M2 + DN2; b.PC2 = b.PC1 + DN2; b.PCP = b.PC2 + DN2;
+    b.PG = b.PCP + DN2;
+    b.C1 = b.PG + 1328; b.C2 = b.C1 + 52; b.CP = b.C2 + 52; b.U1 = b.CP + 52; b.U2 = b.U1 + 52;
+    b.red = b.U2 + 52;
     b.clist = s->clist;
     b.clen = s->clen;
-    b.rank = s->cstr;
-    b.M1 = base; b.M2 = b.M1 + SS_NCOMBO;
-    b.PC1 = b.M2 + SS_NCOMBO; b.PC2 = b.PC1 + 2704; b.PCP = b.PC2 + 2704;
-    b.PG = b.PCP + 2704;
-    b.C1 = b.PG + 1328; b.C2 = b.C1 + 52; b.CP = b.C2 + 52;
-    b.red = b.CP + 52;
+    b.cpos = s->cpos;
     return b;
 }
 
@@ -628,13 +697,13 @@ __global__ void __launch_bounds__(SS_BLOCK) k_forward(const SlotDesc *__restrict
 }
 
 __global__ void __launch_bounds__(SS_BLOCK) k_terminals(const SlotDesc *__restrict__ s, int trav) {
-    __shared__ TermShared sh;
-    TermBuf<float> b = lds_buf(sh);
+    __shared__ TermHeavy sh;
+    TermBuf<float> b = heavy_buf(sh);
     stage_tables(s, sh, threadIdx.x, blockDim.x);
     __syncthreads();
     for (int j = blockIdx.x; j < s->T; j += gridDim.x) {
         int n = s->term[j];
-        terminal_node<float>(s, n, trav, s->reach, s->cfv + (size_t)n * s->Mp, b, TK_NODE, nullptr,
+        terminal_node<float, true>(s, n, trav, s->reach, s->cfv + (size_t)n * s->Mp, b, TK_NODE, nullptr,
                              threadIdx.x, blockDim.x);
     }
 }
@@ -656,12 +725,13 @@ __global__ void k_bump(SlotDesc *s) {
 // batched form: one subgame per workgroup, `iters` iterations per launch
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(SS_BATCH_BLOCK) k_batched(SlotDesc *__restrict__ slots, int nslots, int iters) {
-    __shared__ TermShared sh;
-    TermBuf<float> b = lds_buf(sh);
+    __shared__ TermHeavy sh;
+    TermBuf<float> b = heavy_buf(sh);
     const int tid = threadIdx.x, nt = blockDim.x;
     for (int si = blockIdx.x; si < nslots; si += gridDim.x) {
         SlotDesc *s = slots + si;
         if (!s->active) continue;
+        __syncthreads();
         stage_tables(s, sh, tid, nt);
         __syncthreads();
         int it = s->iter;
@@ -679,7 +749,7 @@ __global__ void __launch_bounds__(SS_BATCH_BLOCK) k_batched(SlotDesc *__restrict
                 }
                 for (int j = 0; j < s->T; ++j) {
                     int n = s->term[j];
-                    terminal_node<float>(s, n, trav, s->reach, s->cfv + (size_t)n * s->Mp, b, TK_NODE, nullptr,
+                    terminal_node<float, true>(s, n, trav, s->reach, s->cfv + (size_t)n * s->Mp, b, TK_NODE, nullptr,
                                          tid, nt);
                     __syncthreads();
                 }
@@ -820,7 +890,7 @@ __global__ void __launch_bounds__(SS_BLOCK) k_xterminals(const SlotDesc *__restr
     TermBuf<double> b = global_buf(scratch + (size_t)blockIdx.x * XSCRATCH_DOUBLES, s);
     for (int j = blockIdx.x; j < s->T; j += gridDim.x) {
         int n = s->term[j];
-        terminal_node<double>(s, n, p, s->xreach, s->xev + (size_t)n * s->Mp, b, TK_NODE, nullptr, threadIdx.x,
+        terminal_node<double, true>(s, n, p, s->xreach, s->xev + (size_t)n * s->Mp, b, TK_NODE, nullptr, threadIdx.x,
                               blockDim.x);
         for (int h = threadIdx.x; h < s->nh[p]; h += blockDim.x)
             s->xbr[(size_t)n * s->Mp + h] = s->xev[(size_t)n * s->Mp + h];
@@ -842,7 +912,7 @@ __global__ void __launch_bounds__(SS_BLOCK) k_xjoint(const SlotDesc *__restrict_
     TermBuf<double> b = global_buf(scratch, s);
     // xreach rows of node 0 hold the raw weights; the per-hand mass lands in xev row 0
     // with scale = weights of player 0 restricted to live hands
-    terminal_node<double>(s, 0, 0, s->xreach, s->xev, b, TK_MASS, s->weights, threadIdx.x, blockDim.x);
+    terminal_node<double, true>(s, 0, 0, s->xreach, s->xev, b, TK_MASS, s->weights, threadIdx.x, blockDim.x);
     double acc = 0;
     for (int h = threadIdx.x; h < s->nh[0]; h += blockDim.x)
         if (s->hinfo[h] & SS_LIVE_BIT) acc += s->xev[h];
@@ -873,7 +943,7 @@ __global__ void __launch_bounds__(SS_BLOCK) k_batched_expl(SlotDesc *__restrict_
         for (int p = 0; p < s->NP; ++p) {
             for (int j = 0; j < s->T; ++j) {
                 int n = s->term[j];
-                terminal_node<double>(s, n, p, s->xreach, s->xev + (size_t)n * s->Mp, b, TK_NODE, nullptr, tid, nt);
+                terminal_node<double, true>(s, n, p, s->xreach, s->xev + (size_t)n * s->Mp, b, TK_NODE, nullptr, tid, nt);
                 for (int h = tid; h < s->nh[p]; h += nt) s->xbr[(size_t)n * s->Mp + h] = s->xev[(size_t)n * s->Mp + h];
                 __syncthreads();
             }
@@ -883,7 +953,7 @@ __global__ void __launch_bounds__(SS_BLOCK) k_batched_expl(SlotDesc *__restrict_
             }
             xreduce_block(s, p, red, tid, nt);
         }
-        terminal_node<double>(s, 0, 0, s->xreach, s->xev, b, TK_MASS, s->weights, tid, nt);
+        terminal_node<double, true>(s, 0, 0, s->xreach, s->xev, b, TK_MASS, s->weights, tid, nt);
         double acc = 0;
         for (int h = tid; h < s->nh[0]; h += nt)
             if (s->hinfo[h] & SS_LIVE_BIT) acc += s->xev[h];
@@ -925,21 +995,36 @@ __global__ void __launch_bounds__(SS_BLOCK) k_fwd_phased(SlotDesc *__restrict__ 
     }
 }
 
-__global__ void __launch_bounds__(SS_BLOCK) k_term_phased(SlotDesc *__restrict__ slots, const uint32_t *tasks,
-                                                          int ntasks, int trav) {
-    __shared__ TermShared sh;
-    TermBuf<float> b = lds_buf(sh);
+// terminal tasks with at most one opponent for every traverser
+__global__ void __launch_bounds__(SS_BLOCK) k_term_light(SlotDesc *__restrict__ slots, const uint32_t *tasks,
+                                                         int ntasks, int trav) {
+    __shared__ TermLight sh;
     for (int j = blockIdx.x; j < ntasks; j += gridDim.x) {
         uint32_t tk = tasks[j];
         const SlotDesc *s = slots + TASK_SLOT(tk);
         if (!slot_runs(s, trav)) continue;
-        // tables of this subgame, read through the cache
-        b.clist = s->clist;
-        b.clen = s->clen;
-        b.rank = s->cstr;
+        TermBuf<float> b = light_buf(sh, s);
         int n = TASK_NODE(tk);
-        terminal_node<float>(s, n, trav, s->reach, s->cfv + (size_t)n * s->Mp, b, TK_NODE, nullptr, threadIdx.x,
-                             blockDim.x);
+        terminal_node<float, false>(s, n, trav, s->reach, s->cfv + (size_t)n * s->Mp, b, TK_NODE, nullptr,
+                                    threadIdx.x, blockDim.x);
+        __syncthreads();
+    }
+}
+
+// terminal tasks with two (or more) opponents for some traverser
+__global__ void __launch_bounds__(SS_BLOCK) k_term_heavy(SlotDesc *__restrict__ slots, const uint32_t *tasks,
+                                                         int ntasks, int trav) {
+    __shared__ TermHeavy sh;
+    TermBuf<float> b = heavy_buf(sh);
+    for (int j = blockIdx.x; j < ntasks; j += gridDim.x) {
+        uint32_t tk = tasks[j];
+        const SlotDesc *s = slots + TASK_SLOT(tk);
+        if (!slot_runs(s, trav)) continue;
+        stage_tables(s, sh, threadIdx.x, blockDim.x);
+        __syncthreads();
+        int n = TASK_NODE(tk);
+        terminal_node<float, true>(s, n, trav, s->reach, s->cfv + (size_t)n * s->Mp, b, TK_NODE, nullptr,
+                                   threadIdx.x, blockDim.x);
         __syncthreads();
     }
 }
@@ -1004,7 +1089,7 @@ __global__ void __launch_bounds__(SS_BLOCK) k_xterm_phased(SlotDesc *__restrict_
         if (!slot_wanted(s, want, TASK_SLOT(tk)) || p >= s->NP) continue;
         TermBuf<double> b = global_buf(scratch + (size_t)blockIdx.x * XSCRATCH_DOUBLES, s);
         int n = TASK_NODE(tk);
-        terminal_node<double>(s, n, p, s->xreach, s->xev + (size_t)n * s->Mp, b, TK_NODE, nullptr, threadIdx.x,
+        terminal_node<double, true>(s, n, p, s->xreach, s->xev + (size_t)n * s->Mp, b, TK_NODE, nullptr, threadIdx.x,
                               blockDim.x);
         for (int h = threadIdx.x; h < s->nh[p]; h += blockDim.x)
             s->xbr[(size_t)n * s->Mp + h] = s->xev[(size_t)n * s->Mp + h];
@@ -1039,7 +1124,7 @@ __global__ void __launch_bounds__(SS_BLOCK) k_xjoint_phased(SlotDesc *__restrict
         const SlotDesc *s = slots + si;
         if (!slot_wanted(s, want, si)) continue;
         TermBuf<double> b = global_buf(scratch + (size_t)blockIdx.x * XSCRATCH_DOUBLES, s);
-        terminal_node<double>(s, 0, 0, s->xreach, s->xev, b, TK_MASS, s->weights, threadIdx.x, blockDim.x);
+        terminal_node<double, true>(s, 0, 0, s->xreach, s->xev, b, TK_MASS, s->weights, threadIdx.x, blockDim.x);
         double acc = 0;
         for (int h = threadIdx.x; h < s->nh[0]; h += blockDim.x)
             if (s->hinfo[h] & SS_LIVE_BIT) acc += s->xev[h];
@@ -1175,14 +1260,16 @@ static int term_cost(const Plan &pl, int n) {
     return 5;
 }
 
-int build_phased_tasks(const std::vector<Slot *> &slots, PhasedTasks &pt, hipStream_t st) {
+int build_phased_tasks(const std::vector<Slot *> &slots, PhasedTasks &pt, hipStream_t st, const std::vector<int> *live) {
     pt.nslots = (int)slots.size();
     pt.NPmax = 0;
     pt.L = 0;
-    for (Slot *s : slots) {
+    for (int si = 0; si < pt.nslots; ++si) {
+        const Slot *s = slots[si];
+        if (s->plan.N >= (1 << 20)) return set_error(SS_ERR_UNSUPPORTED, "tree too large for the batched form");
+        if (live && !(*live)[si]) continue;
         pt.NPmax = std::max(pt.NPmax, s->plan.NP);
         pt.L = std::max(pt.L, s->plan.L);
-        if (s->plan.N >= (1 << 20)) return set_error(SS_ERR_UNSUPPORTED, "tree too large for the batched form");
     }
     if (pt.nslots >= (1 << 11)) return set_error(SS_ERR_UNSUPPORTED, "at most 2047 subgames per batch");
     std::vector<uint32_t> tasks;
@@ -1191,7 +1278,7 @@ int build_phased_tasks(const std::vector<Slot *> &slots, PhasedTasks &pt, hipStr
         pt.lvl_off[l] = (int)tasks.size();
         for (int si = 0; si < pt.nslots; ++si) {
             const Plan &pl = slots[si]->plan;
-            if (l >= pl.L) continue;
+            if (l >= pl.L || (live && !(*live)[si])) continue;
             for (int i = pl.dec_off[l]; i < pl.dec_off[l + 1]; ++i)
                 tasks.push_back(((uint32_t)si << 20) | (uint32_t)pl.dec_lvl[i]);
         }
@@ -1200,6 +1287,7 @@ int build_phased_tasks(const std::vector<Slot *> &slots, PhasedTasks &pt, hipStr
     std::vector<std::pair<int, uint32_t>> term;
     for (int si = 0; si < pt.nslots; ++si) {
         const Plan &pl = slots[si]->plan;
+        if (live && !(*live)[si]) continue;
         for (int j = 0; j < pl.T; ++j) term.push_back({term_cost(pl, pl.term[j]), ((uint32_t)si << 20) | (uint32_t)pl.term[j]});
     }
     std::stable_sort(term.begin(), term.end(), [](const std::pair<int, uint32_t> &a, const std::pair<int, uint32_t> &b) {
@@ -1207,7 +1295,11 @@ int build_phased_tasks(const std::vector<Slot *> &slots, PhasedTasks &pt, hipStr
     });
     pt.term_off = (int)tasks.size();
     pt.nterm = (int)term.size();
-    for (auto &t : term) tasks.push_back(t.second);
+    pt.nheavy = 0;
+    for (auto &t : term) {
+        tasks.push_back(t.second);
+        if (t.first >= 20) ++pt.nheavy;
+    }
     if (pt.d_tasks) { SS_HIP(hipFree(pt.d_tasks)); pt.d_tasks = nullptr; }
     SS_HIP(hipMalloc(&pt.d_tasks, sizeof(uint32_t) * std::max<size_t>(1, tasks.size())));
     SS_HIP(hipMemcpyAsync(pt.d_tasks, tasks.data(), sizeof(uint32_t) * tasks.size(), hipMemcpyHostToDevice, st));
@@ -1224,9 +1316,12 @@ int launch_phased_iteration(SlotDesc *d_slots, const PhasedTasks &pt, hipStream_
                 hipLaunchKernelGGL(k_fwd_phased, dim3(cnt), dim3(SS_BLOCK), 0, st, d_slots, pt.d_tasks + pt.lvl_off[l],
                                    cnt, t);
         }
-        if (pt.nterm > 0)
-            hipLaunchKernelGGL(k_term_phased, dim3(pt.nterm), dim3(SS_BLOCK), 0, st, d_slots, pt.d_tasks + pt.term_off,
-                               pt.nterm, t);
+        if (pt.nheavy > 0)
+            hipLaunchKernelGGL(k_term_heavy, dim3(pt.nheavy), dim3(SS_BLOCK), 0, st, d_slots, pt.d_tasks + pt.term_off,
+                               pt.nheavy, t);
+        if (pt.nterm > pt.nheavy)
+            hipLaunchKernelGGL(k_term_light, dim3(pt.nterm - pt.nheavy), dim3(SS_BLOCK), 0, st, d_slots,
+                               pt.d_tasks + pt.term_off + pt.nheavy, pt.nterm - pt.nheavy, t);
         for (int l = pt.L - 1; l >= 0; --l) {
             int cnt = pt.lvl_off[l + 1] - pt.lvl_off[l];
             if (cnt > 0)

@@ -261,6 +261,7 @@ void build_river_tables(const uint32_t *cstr, RiverTables &rt) {
     rt.glt.assign(SS_NCOMBO, 0);
     rt.gle.assign(SS_NCOMBO, 0);
     rt.clist.assign(52 * 52, 0);
+    rt.cpos.assign(52 * 52, 255);
     rt.clen.assign(52, 0);
     rt.plt.assign(52 * SS_NCOMBO, 0);
     rt.ple.assign(52 * SS_NCOMBO, 0);
@@ -272,7 +273,9 @@ void build_river_tables(const uint32_t *cstr, RiverTables &rt) {
     rt.gsort.assign(std::max(rt.ng, 1), 0);
     std::vector<uint32_t> sorted(rt.ng);
     for (int i = 0; i < rt.ng; ++i) {
-        rt.gsort[i] = (int16_t)live[i];
+        int a = 0, rem = live[i];
+        while (rem >= 51 - a) { rem -= 51 - a; ++a; }
+        rt.gsort[i] = (uint16_t)(a | ((a + 1 + rem) << 8));
         sorted[i] = cstr[live[i]];
     }
     for (int k = 0; k < SS_NCOMBO; ++k) {
@@ -290,6 +293,7 @@ void build_river_tables(const uint32_t *cstr, RiverTables &rt) {
         std::vector<uint32_t> ps(part.size());
         for (size_t j = 0; j < part.size(); ++j) {
             rt.clist[c * 52 + j] = (uint8_t)part[j];
+            rt.cpos[c * 52 + part[j]] = (uint8_t)j;
             ps[j] = cstr[combo_id(c, part[j])];
         }
         for (int k = 0; k < SS_NCOMBO; ++k) {
@@ -355,13 +359,15 @@ int Slot::setup(const HostProblem &hp, hipStream_t st) {
     size_t o_reach = L.take(sizeof(float) * (size_t)pl.NP * pl.N * Mp);
     size_t o_rsig = L.take(sizeof(float) * 8 * Mp);
     size_t o_cstr = L.take(sizeof(uint32_t) * SS_NCOMBO);
-    size_t o_gsort = L.take(sizeof(int16_t) * SS_NCOMBO);
+    size_t o_gsort = L.take(sizeof(uint16_t) * SS_NCOMBO);
     size_t o_glt = L.take(sizeof(int16_t) * SS_NCOMBO);
     size_t o_gle = L.take(sizeof(int16_t) * SS_NCOMBO);
     size_t o_clist = L.take(52 * 52);
     size_t o_clen = L.take(52);
     size_t o_plt = pl.is_river ? L.take(52 * SS_NCOMBO) : 0;
     size_t o_ple = pl.is_river ? L.take(52 * SS_NCOMBO) : 0;
+    size_t o_cpos = L.take(52 * 52);
+    size_t o_hsort = L.take(sizeof(uint16_t) * pl.NP * Mp);
     size_t o_board = L.take(sizeof(int) * 8);
     arena_bytes = L.off;
     SS_HIP(hipMalloc(&arena, arena_bytes));
@@ -409,13 +415,15 @@ int Slot::setup(const HostProblem &hp, hipStream_t st) {
     d.reach = (float *)(a + o_reach);
     d.rsig = (float *)(a + o_rsig);
     d.cstr = (const uint32_t *)(a + o_cstr);
-    d.gsort = (const int16_t *)(a + o_gsort);
+    d.gsort = (const uint16_t *)(a + o_gsort);
     d.glt = (const int16_t *)(a + o_glt);
     d.gle = (const int16_t *)(a + o_gle);
     d.clist = (const uint8_t *)(a + o_clist);
     d.clen = (const uint8_t *)(a + o_clen);
     d.plt = pl.is_river ? (const uint8_t *)(a + o_plt) : nullptr;
     d.ple = pl.is_river ? (const uint8_t *)(a + o_ple) : nullptr;
+    d.cpos = (const uint8_t *)(a + o_cpos);
+    d.hsort = (const uint16_t *)(a + o_hsort);
     d.active = 1;
 
     if (pl.is_river && pl.num_board == 5) {
@@ -430,7 +438,21 @@ int Slot::setup(const HostProblem &hp, hipStream_t st) {
         RiverTables rt;
         build_river_tables(cstr.data(), rt);
         d.ng = rt.ng;
-        SS_HIP(hipMemcpyAsync(a + o_gsort, rt.gsort.data(), sizeof(int16_t) * rt.gsort.size(), hipMemcpyHostToDevice, st));
+        SS_HIP(hipMemcpyAsync(a + o_gsort, rt.gsort.data(), sizeof(uint16_t) * rt.gsort.size(), hipMemcpyHostToDevice, st));
+        SS_HIP(hipMemcpyAsync(a + o_cpos, rt.cpos.data(), 52 * 52, hipMemcpyHostToDevice, st));
+        // hands of every player by ascending strength (dead hands last)
+        std::vector<uint16_t> hs((size_t)pl.NP * Mp);
+        for (int p = 0; p < pl.NP; ++p) {
+            std::vector<int> idx(pl.nh[p]);
+            std::iota(idx.begin(), idx.end(), 0);
+            auto rank = [&](int h) -> uint32_t {
+                uint32_t hi = pl.hinfo[(size_t)p * Mp + h];
+                return (hi & SS_LIVE_BIT) ? rt.cstr[(hi >> 12) & 2047] : 0xFFFFFFFFu;
+            };
+            std::stable_sort(idx.begin(), idx.end(), [&](int a2, int b2) { return rank(a2) < rank(b2); });
+            for (size_t i = 0; i < idx.size(); ++i) hs[(size_t)p * Mp + i] = (uint16_t)idx[i];
+        }
+        SS_HIP(hipMemcpyAsync(a + o_hsort, hs.data(), sizeof(uint16_t) * hs.size(), hipMemcpyHostToDevice, st));
         SS_HIP(hipMemcpyAsync(a + o_glt, rt.glt.data(), sizeof(int16_t) * SS_NCOMBO, hipMemcpyHostToDevice, st));
         SS_HIP(hipMemcpyAsync(a + o_gle, rt.gle.data(), sizeof(int16_t) * SS_NCOMBO, hipMemcpyHostToDevice, st));
         SS_HIP(hipMemcpyAsync(a + o_clist, rt.clist.data(), 52 * 52, hipMemcpyHostToDevice, st));
@@ -438,8 +460,14 @@ int Slot::setup(const HostProblem &hp, hipStream_t st) {
         SS_HIP(hipMemcpyAsync(a + o_plt, rt.plt.data(), 52 * SS_NCOMBO, hipMemcpyHostToDevice, st));
         SS_HIP(hipMemcpyAsync(a + o_ple, rt.ple.data(), 52 * SS_NCOMBO, hipMemcpyHostToDevice, st));
         SS_HIP(hipStreamSynchronize(st));   // host tables go out of scope
+        SS_HIP(hipStreamSynchronize(st));   // hs goes out of scope
     } else {
         d.is_river = 0;   // showdowns off the river carry no ranks: value 0, as street_solve.cu:1520
+        std::vector<uint16_t> hs((size_t)pl.NP * Mp);
+        for (int p = 0; p < pl.NP; ++p)
+            for (int i = 0; i < Mp; ++i) hs[(size_t)p * Mp + i] = (uint16_t)i;
+        SS_HIP(hipMemcpyAsync(a + o_hsort, hs.data(), sizeof(uint16_t) * hs.size(), hipMemcpyHostToDevice, st));
+        SS_HIP(hipStreamSynchronize(st));
     }
     SS_HIP(hipMemcpyAsync(d_desc, &d, sizeof(d), hipMemcpyHostToDevice, st));
     return SS_OK;
@@ -455,7 +483,7 @@ int Slot::alloc_expl(hipStream_t st, int blocks) {
     size_t o_xev = L.take(sizeof(double) * (size_t)plan.N * Mp);
     size_t o_xbr = L.take(sizeof(double) * (size_t)plan.N * Mp);
     size_t o_xout = L.take(sizeof(double) * 32);
-    size_t o_scr = L.take(sizeof(double) * (size_t)blocks * (2 * SS_NCOMBO + 3 * 2704 + 1328 + 3 * 52 + 256));
+    size_t o_scr = L.take(sizeof(double) * (size_t)blocks * SS_XSCRATCH_DOUBLES);
     xarena_bytes = L.off;
     SS_HIP(hipMalloc(&xarena, xarena_bytes));
     SS_HIP(hipMemsetAsync(xarena, 0, xarena_bytes, st));

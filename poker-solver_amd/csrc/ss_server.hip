@@ -145,7 +145,7 @@ int SSServer::init() {
     SS_HIP(hipMalloc(&d_slots, sizeof(SlotDesc) * n));
     SS_HIP(hipMemset(d_slots, 0, sizeof(SlotDesc) * n));   // active = 0 everywhere
     SS_HIP(hipMalloc(&d_want, sizeof(int) * n));
-    SS_HIP(hipMalloc(&scratch, sizeof(double) * (size_t)n * (2 * SS_NCOMBO + 3 * 2704 + 1328 + 3 * 52 + 256)));
+    SS_HIP(hipMalloc(&scratch, sizeof(double) * (size_t)n * SS_XSCRATCH_DOUBLES));
     resident.assign(n, nullptr);
     SS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < cfg.iters_per_step; ++k) {
