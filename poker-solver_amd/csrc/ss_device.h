@@ -19,19 +19,26 @@
 //   river tables (showdown closed forms, see ss_kernels.hip):
 //   cstr      uint32[1326]          7-card rank of each combo (0 = blocked)
 //   gsort     uint16[ng]            live combos by ascending rank (cards a | b << 8)
-//   glt, gle  int16[1326]           # live combos ranked <, <= the combo
 //   clist     uint8[52][52]         per card: partner cards by ascending rank
 //   clen      uint8[52]
-//   plt, ple  uint8[52][1326]       per card c and combo k: # entries of c's
-//                                   list ranked <, <= rank(k)
-//   cpos      uint8[52][52]         position of partner d in card c's list
-//   hsort     uint16[NP][Mp]        each player's hands by ascending strength: the
-//                                   showdown sweeps walk hands in this order so a
-//                                   wavefront shares strength-class boundaries
+//   bnd       uint8[ncls+1][52]     row t, card c: 4 x # entries of c's list ranked
+//                                   below class t (row ncls: the list length), so
+//                                   for combo k the entries ranked <, <= rank(k)
+//                                   are rows kcls[k], kcls[k]+1 (contiguous)
+//   cpos      uint8[52][52]         4 x position of partner d in card c's list (255: none)
+//   hmeta     uint4[NP][Mp]         each player's hands by ascending strength (dead
+//                                   last), one 16-byte record per hand:
+//                                   x = h | c0 << 11 | c1 << 17 | ranked << 23,
+//                                   y = strength class t (distinct live ranks,
+//                                   ascending), z, w = # live combos ranked <, <=.
+//                                   Showdown sweeps walk hands in this order so
+//                                   a wavefront shares class boundaries
 #pragma once
+#include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #define SS_NCOMBO 1326
+#define SS_BND_BYTES (1082 * 52)   // <= 1081 live combos on a 5-card board, + 1 row
 #define SS_LIVE_BIT (1u << 23)
 
 namespace ss {
@@ -73,14 +80,11 @@ struct SlotDesc {
     float *rsig;
     const uint32_t *cstr;
     const uint16_t *gsort;
-    const int16_t *glt;
-    const int16_t *gle;
     const uint8_t *clist;
     const uint8_t *clen;
-    const uint8_t *plt;
-    const uint8_t *ple;
+    const uint8_t *bnd;
     const uint8_t *cpos;
-    const uint16_t *hsort;     // [NP][Mp] hand indices by ascending strength (river), identity otherwise
+    const uint4 *hmeta;        // [NP][Mp] river only
     // exploitability scratch (FP64), allocated on demand
     float *avg;                // [A][Mp]
     double *xreach;            // [NP][N][Mp]

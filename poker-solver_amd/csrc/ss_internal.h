@@ -67,8 +67,10 @@ struct RiverTables {
     std::vector<uint32_t> cstr;   // [1326]
     std::vector<uint16_t> gsort;          // live combos by ascending rank, packed a | b << 8
     std::vector<int16_t> glt, gle;
-    std::vector<uint8_t> clist, clen, plt, ple, cpos;
-    int ng = 0;
+    std::vector<uint8_t> clist, clen, cpos;
+    std::vector<uint16_t> kcls;   // [1326] strength class
+    std::vector<uint8_t> bnd;     // [(ncls+1)*52] class boundaries per card
+    int ng = 0, ncls = 0;
 };
 void build_river_tables(const uint32_t *cstr, RiverTables &rt);
 
@@ -108,7 +110,8 @@ struct PhasedTasks {
     int nslots = 0, NPmax = 0, L = 0;
     std::vector<int> lvl_off;      // level l tasks: [lvl_off[l], lvl_off[l+1]); all decision tasks [0, lvl_off[L])
     int term_off = 0, nterm = 0;   // terminal tasks, most expensive first
-    int nheavy = 0;                // the first nheavy have two or more opponents for some traverser
+    int nodo = 0;                  // terminal tasks [0, nodo): three or more opponents for some traverser
+    int npair = 0;                 // [nodo, nodo + npair): two opponents; the rest at most one
     uint32_t *d_tasks = nullptr;
     ~PhasedTasks();
 };
@@ -118,7 +121,7 @@ int build_phased_tasks(const std::vector<Slot *> &slots, PhasedTasks &pt, hipStr
 int launch_phased_iteration(SlotDesc *d_slots, const PhasedTasks &pt, hipStream_t st);
 int launch_phased_expl(SlotDesc *d_slots, const PhasedTasks &pt, const int *d_want, double *scratch,
                        int scratch_blocks, hipStream_t st);
-#define SS_XSCRATCH_DOUBLES (5 * 2704 + 1328 + 5 * 52 + 64)
+#define SS_XSCRATCH_DOUBLES (5 * 2704 + 3 * 1328 + 5 * 52 + 64)
 
 // debug / test entry points implemented in the kernels unit
 int dbg_eval7_device(const int *cards, int n, uint32_t *out);

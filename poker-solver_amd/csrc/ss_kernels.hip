@@ -235,18 +235,36 @@ __device__ __forceinline__ void root_sigma_items(const SlotDesc *__restrict__ s,
 #define DN 52
 #define DN2 (52 * 52)
 
+// load through the global address space (the pointers come out of a
+// SlotDesc, which the compiler would otherwise treat as generic)
+template <typename P>
+__device__ __forceinline__ P gld(const P *p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return *(const __attribute__((address_space(1))) P *)p;
+#else
+    return *p;   // host pass of the device code: never executed
+#endif
+}
+
+// element of a row addressed by 4 x its index (the byte tables store 4 x index)
+template <typename T>
+__device__ __forceinline__ T at4(const T *row, int i4) {
+    return *(const T *)((const char *)row + (size_t)i4 * (sizeof(T) / 4));
+}
+
 template <typename T>
 struct TermBuf {
     T *M1, *M2;          // [52*52] dense opponent reach
     T *PC1, *PC2, *PCP;  // [52*52] per-card prefix over strength-sorted partners; PC1 holds G for 2-opponent folds
-    T *PG;               // [1328] prefix over strength-sorted combos
+    T *PG;               // [3][PGS] prefix over strength-sorted combos of M1, M2, M1.*M2
     T *C1, *C2, *CP;     // [52] per-card totals (CP of M1 .* M2)
     T *U1, *U2;          // [52] C1 * M2 and C2 * M1
     T *red;              // [64] wave partials
     const uint8_t *clist;  // [52*52] per-card partner lists, ascending strength
     const uint8_t *clen;   // [52]
-    const uint8_t *cpos;   // [52*52] position of partner d in card c's list
+    const uint8_t *cpos;   // [52*52] 4 x position of partner d in card c's list (255: none)
 };
+#define PGS 1328   // stride of the PG channels
 
 // A card-removal mass is a sum of non-negative reach products; evaluated by
 // inclusion-exclusion it carries rounding noise of order eps * (scale of the
@@ -283,34 +301,72 @@ __device__ __forceinline__ T wave_sum(T x) {
     return x;
 }
 
-// exclusive scan of M over the strength-sorted live combos into PG[0..ng].
+// exclusive scans over the strength-sorted live combos: PG[0..ng] of M1 and,
+// with NCH = 3, PG[PGS + ..] of M2 and PG[2 PGS + ..] of M1 .* M2.
 // The decomposition is fixed (256 chunks, a wave64 scan of chunk sums, 4
 // wave totals) whatever the block size, so every form rounds the same way.
 #define SCAN_LANES 256
-template <typename T>
-__device__ void scan_sorted(const SlotDesc *__restrict__ s, const T *M, T *PG, T *red, int tid) {
+#define SCAN_CHUNK 5   // ceil(1081 / 256): a river has at most 1081 live combos
+template <typename T, int NCH>
+__device__ void scan_sorted(const SlotDesc *__restrict__ s, const T *M1, const T *M2, T *PG, T *red, int tid) {
     const int ng = s->ng;
     const int chunk = (ng + SCAN_LANES - 1) / SCAN_LANES;
     const bool on = tid < SCAN_LANES;
     const int j0 = on ? min(ng, tid * chunk) : 0, j1 = on ? min(ng, j0 + chunk) : 0;
-    T part = 0;
-    for (int j = j0; j < j1; ++j) {
-        uint32_t g = s->gsort[j];
-        part += M[(g & 0xFF) * DN + (g >> 8)];
+    int at[SCAN_CHUNK];
+#pragma unroll
+    for (int u = 0; u < SCAN_CHUNK; ++u) {
+        const uint32_t g = j0 + u < j1 ? gld(s->gsort + j0 + u) : 0u;
+        at[u] = (g & 0xFF) * DN + (g >> 8);
+    }
+    T part[NCH];
+#pragma unroll
+    for (int q = 0; q < NCH; ++q) part[q] = 0;
+#pragma unroll
+    for (int u = 0; u < SCAN_CHUNK; ++u) {
+        if (j0 + u >= j1) break;
+        const T a = M1[at[u]];
+        part[0] += a;
+        if (NCH == 3) {
+            const T b = M2[at[u]];
+            part[1] += b;
+            part[2] += a * b;
+        }
     }
     const int lane = tid & 63, w = tid >> 6;
-    T incl = wave_incl_scan(part, lane);
-    if (on && lane == 63) red[w] = incl;
+    T incl[NCH];
+#pragma unroll
+    for (int q = 0; q < NCH; ++q) {
+        incl[q] = wave_incl_scan(part[q], lane);
+        if (on && lane == 63) red[q * 4 + w] = incl[q];
+    }
     __syncthreads();
     if (on) {
-        T run = incl - part;
-        for (int i = 0; i < w; ++i) run += red[i];
-        for (int j = j0; j < j1; ++j) {
-            PG[j] = run;
-            uint32_t g = s->gsort[j];
-            run += M[(g & 0xFF) * DN + (g >> 8)];
+        T run[NCH];
+#pragma unroll
+        for (int q = 0; q < NCH; ++q) {
+            run[q] = incl[q] - part[q];
+            for (int i = 0; i < w; ++i) run[q] += red[q * 4 + i];
         }
-        if (tid == SCAN_LANES - 1) PG[ng] = run;
+#pragma unroll
+        for (int u = 0; u < SCAN_CHUNK; ++u) {
+            if (j0 + u >= j1) break;
+            const int j = j0 + u;
+            const T a = M1[at[u]];
+            PG[j] = run[0];
+            run[0] += a;
+            if (NCH == 3) {
+                const T b = M2[at[u]];
+                PG[PGS + j] = run[1];
+                PG[2 * PGS + j] = run[2];
+                run[1] += b;
+                run[2] += a * b;
+            }
+        }
+        if (tid == SCAN_LANES - 1) {
+#pragma unroll
+            for (int q = 0; q < NCH; ++q) PG[q * PGS + ng] = run[q];
+        }
     }
     __syncthreads();
 }
@@ -364,17 +420,29 @@ __device__ T odometer(const SlotDesc *__restrict__ s, int n, int trav, int h, co
 
 // Scatter opponent q's reach at node n into the dense card-pair matrix.
 // Each combo owns two cells, so with unique hands every cell gets one add.
+// Loads are issued four hands at a time, ahead of the adds.
 template <typename T>
 __device__ __forceinline__ void scatter(const SlotDesc *__restrict__ s, int q, int n, const T *reach, T *M,
                                         int tid, int nt) {
-    const int Mp = s->Mp;
-    for (int o = tid; o < s->nh[q]; o += nt) {
-        uint32_t oi = s->hinfo[(size_t)q * Mp + o];
-        if (!(oi & SS_LIVE_BIT)) continue;
-        int a = oi & 63, b = (oi >> 6) & 63;
-        T r = reach[((size_t)q * s->N + n) * Mp + o];
-        atomicAdd(&M[a * DN + b], r);
-        atomicAdd(&M[b * DN + a], r);
+    const int Mp = s->Mp, nq = s->nh[q];
+    const uint32_t *hq = s->hinfo + (size_t)q * Mp;
+    const T *rq = reach + ((size_t)q * s->N + n) * Mp;
+    for (int o0 = tid; o0 < nq; o0 += 4 * nt) {
+        uint32_t oi[4];
+        T r[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int o = o0 + u * nt;
+            oi[u] = o < nq ? gld(hq + o) : 0u;
+            r[u] = o < nq ? gld(rq + o) : (T)0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (!(oi[u] & SS_LIVE_BIT)) continue;
+            const int a = oi[u] & 63, b = (oi[u] >> 6) & 63;
+            atomicAdd(&M[a * DN + b], r[u]);
+            atomicAdd(&M[b * DN + a], r[u]);
+        }
     }
 }
 
@@ -467,10 +535,16 @@ __device__ __forceinline__ void pair_products(const T *M1, const T *M2, const T 
 #define TK_NODE 0      // the node's own type (fold / showdown / leaf)
 #define TK_MASS 1      // opponent mass only, all non-traversers, times `scale`
 
-// Writes out[h] for h < nh[trav] of node n for traverser trav. HEAVY = false
-// compiles only the 0/1-opponent closed forms (callers guarantee no node with
-// more opponents reaches it).
-template <typename T, bool HEAVY>
+// Closed-form coverage of a terminal sweep: TM_LIGHT handles nodes with at
+// most one opponent for the traverser, TM_PAIR at most two, TM_ALL any count
+// (three or more by the reference's enumeration). Callers guarantee that no
+// node beyond the mode's coverage reaches it.
+#define TM_LIGHT 0
+#define TM_PAIR 1
+#define TM_ALL 2
+
+// Writes out[h] for h < nh[trav] of node n for traverser trav.
+template <typename T, int MODE>
 __device__ void terminal_node(const SlotDesc *__restrict__ s, int n, int trav, const T *reach, T *out,
                               TermBuf<T> B, int kind, const float *scale, int tid, int nt) {
     const DevNode &nd = s->nodes[n];
@@ -495,7 +569,7 @@ __device__ void terminal_node(const SlotDesc *__restrict__ s, int n, int trav, c
             return trav == nd.player ? (T)(nd.pot - nd.bets[trav]) : -(T)nd.bets[trav];
         return (T)s->leaf_values[((size_t)nd.leaf_idx * NP + trav) * s->maxh + h];
     };
-    if (HEAVY && k >= 3) {
+    if (MODE == TM_ALL && k >= 3) {
         for (int h = tid; h < nh; h += nt) {
             T v;
             if (type == SS_NODE_SHOWDOWN) {
@@ -511,10 +585,14 @@ __device__ void terminal_node(const SlotDesc *__restrict__ s, int n, int trav, c
     }
     // ---- closed forms, k <= 2 ----
     const bool sd = type == SS_NODE_SHOWDOWN;
-    const bool two = HEAVY && k == 2;
-    for (int i = tid; i < DN2; i += nt) {
-        B.M1[i] = 0;
-        if (two) B.M2[i] = 0;
+    const bool two = MODE != TM_LIGHT && k == 2;
+    {
+        const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+        float4 *m1 = (float4 *)B.M1, *m2 = (float4 *)B.M2;
+        for (int i = tid; i < DN2 * (int)sizeof(T) / 16; i += nt) {
+            m1[i] = z4;
+            if (two) m2[i] = z4;
+        }
     }
     __syncthreads();
     if (k >= 1) scatter<T>(s, opp[0], n, reach, B.M1, tid, nt);
@@ -525,7 +603,8 @@ __device__ void terminal_node(const SlotDesc *__restrict__ s, int n, int trav, c
     else if (k >= 1)
         card_totals<T>(B.M1, two ? B.M2 : nullptr, B.C1, B.C2, B.CP, tid, nt);
     __syncthreads();
-    if (sd && k == 1) scan_sorted<T>(s, B.M1, B.PG, B.red, tid);
+    if (sd && k == 1) scan_sorted<T, 1>(s, B.M1, nullptr, B.PG, B.red, tid);
+    if (sd && two) scan_sorted<T, 3>(s, B.M1, B.M2, B.PG, B.red, tid);
     if (two && !sd) {
         pair_products<T>(B.M1, B.M2, B.C1, B.C2, B.PC1, B.U1, B.U2, tid, nt);
         __syncthreads();
@@ -540,13 +619,26 @@ __device__ void terminal_node(const SlotDesc *__restrict__ s, int n, int trav, c
             for (int c = 0; c < 52; ++c) TT += B.C1[c] * B.C2[c];
     }
 
+    // Showdowns walk the traverser's hands by strength (shared class
+    // boundaries within a wavefront turn prefix reads into broadcasts), one
+    // 16-byte record per hand, fetched one hand ahead.
+    const uint4 *meta = sd ? s->hmeta + (size_t)trav * Mp : nullptr;
+    uint4 mnext = make_uint4(0u, 0u, 0u, 0u);
+    if (sd && tid < nh) mnext = gld(meta + tid);
     for (int i = tid; i < nh; i += nt) {
-        // showdowns walk the traverser's hands by strength (shared class
-        // boundaries within a wavefront turn prefix reads into broadcasts)
-        const int h = sd ? (int)s->hsort[(size_t)trav * Mp + i] : i;
-        uint32_t hi = s->hinfo[(size_t)trav * Mp + h];
-        int x = hi & 63, y = (hi >> 6) & 63;
-        int kk = (hi >> 12) & 2047;
+        const uint4 mh = mnext;
+        if (sd && i + nt < nh) mnext = gld(meta + i + nt);
+        int h, x, y;
+        if (sd) {
+            h = mh.x & 2047;
+            x = (mh.x >> 11) & 63;
+            y = (mh.x >> 17) & 63;
+        } else {
+            h = i;
+            const uint32_t hi = s->hinfo[(size_t)trav * Mp + h];
+            x = hi & 63;
+            y = (hi >> 6) & 63;
+        }
         const int xy = x * DN + y;
         T v;
         if (!sd) {
@@ -568,56 +660,81 @@ __device__ void terminal_node(const SlotDesc *__restrict__ s, int n, int trav, c
             }
             v = factor(h) * m;
         } else {
-            uint32_t hs = (hi & SS_LIVE_BIT) ? s->cstr[kk] : 0u;
-            if (hs == 0) { out[h] = 0; continue; }                            // :861
+            if (!((mh.x >> 23) & 1)) { out[h] = 0; continue; }              // hs = 0, :861
+            const int cls = mh.y, jl = mh.z, je = mh.w;
             if (k == 0) {
                 v = pot - bet;
             } else if (!two) {                                                 // :886-903
-                int lx = s->plt[x * SS_NCOMBO + kk], ly = s->plt[y * SS_NCOMBO + kk];
-                int ex = s->ple[x * SS_NCOMBO + kk], ey = s->ple[y * SS_NCOMBO + kk];
-                T lt = B.PG[s->glt[kk]] - B.PC1[x * DN + lx] - B.PC1[y * DN + ly];
-                T le = B.PG[s->gle[kk]] - B.PC1[x * DN + ex] - B.PC1[y * DN + ey] + B.M1[xy];
+                const uint8_t *rl = s->bnd + (size_t)cls * DN, *re = rl + DN;
+                int lx = gld(rl + x) >> 2, ly = gld(rl + y) >> 2, ex = gld(re + x) >> 2, ey = gld(re + y) >> 2;
+                T lt = B.PG[jl] - B.PC1[x * DN + lx] - B.PC1[y * DN + ly];
+                T le = B.PG[je] - B.PC1[x * DN + ex] - B.PC1[y * DN + ey] + B.M1[xy];
                 T z = S1 - B.C1[x] - B.C1[y] + B.M1[xy];
                 v = z < mass_floor<T>() * S1 ? (T)0 : pot * lt + pot * (T)0.5 * (le - lt) - bet * z;
             } else {                                                           // :936-1004, n_opp = 2
-                T h1l = 0, h1e = 0, h2l = 0, h2e = 0, hpl = 0, hpe = 0;
-                T xll = 0, xle = 0, xel = 0, xee = 0, xaa = 0;
-                T a_lx = 0, a_ex = 0, a_ly = 0, a_ey = 0, b_lx = 0, b_ex = 0, b_ly = 0, b_ey = 0;
-                T p_lx = 0, p_ex = 0, p_ly = 0, p_ey = 0;
-                for (int c = 0; c < 52; ++c) {
-                    if (B.clen[c] == 0) continue;
-                    int l = s->plt[c * SS_NCOMBO + kk], e = s->ple[c * SS_NCOMBO + kk];
-                    T al = B.PC1[c * DN + l], ae = B.PC1[c * DN + e] - al;
-                    T bl = B.PC2[c * DN + l], be = B.PC2[c * DN + e] - bl;
-                    T pl = B.PCP[c * DN + l], pe = B.PCP[c * DN + e] - pl;
-                    h1l += al; h1e += ae; h2l += bl; h2e += be; hpl += pl; hpe += pe;
-                    if (c == x) { a_lx = al; a_ex = ae; b_lx = bl; b_ex = be; p_lx = pl; p_ex = pe; continue; }
-                    if (c == y) { a_ly = al; a_ey = ae; b_ly = bl; b_ey = be; p_ly = pl; p_ey = pe; continue; }
-                    T aa = B.C1[c], ba = B.C2[c];
-                    // class of the hands {c,x}, {c,y} relative to the traverser's
-                    // strength, from their positions in c's sorted partner list
-                    int px = B.cpos[c * DN + x], py = B.cpos[c * DN + y];
-                    T m1x = B.M1[c * DN + x], m1y = B.M1[c * DN + y];
-                    T m2x = B.M2[c * DN + x], m2y = B.M2[c * DN + y];
-                    if (px < l) { al -= m1x; bl -= m2x; } else if (px < e) { ae -= m1x; be -= m2x; }
-                    if (py < l) { al -= m1y; bl -= m2y; } else if (py < e) { ae -= m1y; be -= m2y; }
-                    aa -= m1x + m1y;
-                    ba -= m2x + m2y;
-                    xll += al * bl; xle += al * be; xel += ae * bl; xee += ae * be; xaa += aa * ba;
+                // Classes relative to the traverser's strength: "l" (weaker) and
+                // "le" (weaker or tied). With A = opponent-1 mass, B = opponent 2,
+                // the value is pot * (wll + (wle + wel) / 2 + wee / 3) - bet * z,
+                // and wll + (wle + wel)/2 + wee/3 = [Al(2Bl + Ble) + Ale(Bl + 2Ble)] / 6
+                // for each bilinear part (pairs sharing a card are removed per card c).
+                // Class boundaries of every card: two contiguous 52-byte rows
+                // (row kcls: entries ranked <, row kcls+1: ranked <=; stored x4),
+                // read a word (4 cards) ahead of use.
+                const uint32_t *rowl = (const uint32_t *)(s->bnd + (size_t)cls * DN);
+                const uint8_t *rl = (const uint8_t *)rowl;
+                const int lx4 = gld(rl + x), ly4 = gld(rl + y), ex4 = gld(rl + DN + x), ey4 = gld(rl + DN + y);
+                uint32_t lnext = gld(rowl), enext = gld(rowl + 13);
+                T xw = 0, xaa = 0;
+#pragma unroll 1
+                for (int g = 0; g < 13; ++g) {
+                    const uint32_t lw = lnext, ew = enext;
+                    if (g < 12) {
+                        lnext = gld(rowl + g + 1);
+                        enext = gld(rowl + 14 + g);
+                    }
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) {
+                        // board cards have empty lists: every term below is an exact 0
+                        const int c = 4 * g + b;
+                        const int l4 = (lw >> (8 * b)) & 0xFF, e4 = (ew >> (8 * b)) & 0xFF;
+                        const T P1l = at4(B.PC1 + c * DN, l4), P2l = at4(B.PC2 + c * DN, l4);
+                        const T P1e = at4(B.PC1 + c * DN, e4), P2e = at4(B.PC2 + c * DN, e4);
+                        // {c,x}, {c,y} are not opponent hands: remove them from the
+                        // classes they fall in (by their position in c's sorted list)
+                        const int px4 = B.cpos[c * DN + x], py4 = B.cpos[c * DN + y];
+                        const T m1x = B.M1[c * DN + x], m1y = B.M1[c * DN + y];
+                        const T m2x = B.M2[c * DN + x], m2y = B.M2[c * DN + y];
+                        const T fxl = px4 < l4 ? (T)1 : (T)0, fxe = px4 < e4 ? (T)1 : (T)0;
+                        const T fyl = py4 < l4 ? (T)1 : (T)0, fye = py4 < e4 ? (T)1 : (T)0;
+                        // cards of the traverser's hand carry no opponent pairs
+                        const bool own = (c == x) | (c == y);
+                        const T Al = own ? (T)0 : P1l - fxl * m1x - fyl * m1y;
+                        const T Ale = own ? (T)0 : P1e - fxe * m1x - fye * m1y;
+                        const T Bl = P2l - fxl * m2x - fyl * m2y;
+                        const T Ble = P2e - fxe * m2x - fye * m2y;
+                        xw += Al * ((T)2 * Bl + Ble);
+                        xw += Ale * (Bl + (T)2 * Ble);
+                        const T c1 = B.C1[c], c2 = B.C2[c];   // unconditional loads: no branch
+                        const T aa = c1 - m1x - m1y;
+                        xaa += (own ? (T)0 : aa) * (c2 - m2x - m2y);
+                    }
                 }
-                T m1k = B.M1[xy], m2k = B.M2[xy];
-                T Al = (T)0.5 * h1l - a_lx - a_ly, Ae = (T)0.5 * h1e - a_ex - a_ey + m1k;
-                T Bl = (T)0.5 * h2l - b_lx - b_ly, Be = (T)0.5 * h2e - b_ex - b_ey + m2k;
-                T Pl = (T)0.5 * hpl - p_lx - p_ly, Pe = (T)0.5 * hpe - p_ex - p_ey + m1k * m2k;
-                T Aa = S1 - B.C1[x] - B.C1[y] + m1k, Ba = S2 - B.C2[x] - B.C2[y] + m2k;
-                T Pa = SP - B.CP[x] - B.CP[y] + m1k * m2k;
-                T wll = Al * Bl - xll + Pl;
-                T wle = Al * Be - xle;
-                T wel = Ae * Bl - xel;
-                T wee = Ae * Be - xee + Pe;
-                T z = Aa * Ba - xaa + Pa;
-                v = z < mass_floor<T>() * S1 * S2 ? (T)0
-                                                  : pot * (wll + (wle + wel) * (T)0.5 + wee / (T)3) - bet * z;
+                const int lx = lx4 >> 2, ly = ly4 >> 2, ex = ex4 >> 2, ey = ey4 >> 2;
+                const T m1k = B.M1[xy], m2k = B.M2[xy];
+                // class totals over opponent hands disjoint from {x,y}; the
+                // traverser's own combo is tied with itself (class "le" only)
+                const T Al = B.PG[jl] - B.PC1[x * DN + lx] - B.PC1[y * DN + ly];
+                const T Ale = B.PG[je] - B.PC1[x * DN + ex] - B.PC1[y * DN + ey] + m1k;
+                const T Bl = B.PG[PGS + jl] - B.PC2[x * DN + lx] - B.PC2[y * DN + ly];
+                const T Ble = B.PG[PGS + je] - B.PC2[x * DN + ex] - B.PC2[y * DN + ey] + m2k;
+                const T Pl = B.PG[2 * PGS + jl] - B.PCP[x * DN + lx] - B.PCP[y * DN + ly];
+                const T Ple = B.PG[2 * PGS + je] - B.PCP[x * DN + ex] - B.PCP[y * DN + ey] + m1k * m2k;
+                const T Aa = S1 - B.C1[x] - B.C1[y] + m1k, Ba = S2 - B.C2[x] - B.C2[y] + m2k;
+                const T Pa = SP - B.CP[x] - B.CP[y] + m1k * m2k;
+                const T W = (Al * ((T)2 * Bl + Ble) + Ale * (Bl + (T)2 * Ble) - xw) * ((T)1 / (T)6) +
+                            Pl * ((T)2 / (T)3) + Ple * ((T)1 / (T)3);
+                const T z = Aa * Ba - xaa + Pa;
+                v = z < mass_floor<T>() * S1 * S2 ? (T)0 : pot * W - bet * z;
             }
         }
         out[h] = v;
@@ -629,14 +746,17 @@ __device__ void terminal_node(const SlotDesc *__restrict__ s, int n, int trav, c
 // allows 160 KB per workgroup): both opponents' matrices, three prefix
 // tables, the subgame's per-card lists. Light (<= 1 opponent, ~30 KB): one
 // matrix, one prefix table; five workgroups fit a CU.
+// Rows of 2752 = 43 x 64 floats: the same cell of M1 and M2 (PC1 and PC2)
+// is one ds_read2st64 away.
+#define DNP 2752
 struct TermHeavy {
-    float M1[DN2], M2[DN2];
-    float PC1[DN2], PC2[DN2], PCP[DN2];
-    float PG[1328];
+    float M1[DNP], M2[DNP];
+    float PC1[DNP], PC2[DNP], PCP[DNP];
+    float PG[3 * PGS];
     float C1[52], C2[52], CP[52], U1[52], U2[52];
     float red[64];
-    uint8_t clist[DN2];
-    uint8_t cpos[DN2];
+    alignas(16) uint8_t clist[DN2];
+    alignas(16) uint8_t cpos[DN2];
     uint8_t clen[52];
 };
 
@@ -659,9 +779,11 @@ __device__ __forceinline__ TermBuf<float> heavy_buf(TermHeavy &t) {
 // copy a river subgame's per-card lists and positions into LDS (caller barriers)
 __device__ __forceinline__ void stage_tables(const SlotDesc *__restrict__ s, TermHeavy &t, int tid, int nt) {
     if (!s->is_river) return;
-    for (int i = tid; i < DN2; i += nt) {
-        t.clist[i] = s->clist[i];
-        t.cpos[i] = s->cpos[i];
+    const uint32_t *cl = (const uint32_t *)s->clist, *cp = (const uint32_t *)s->cpos;
+    uint32_t *tl = (uint32_t *)t.clist, *tp = (uint32_t *)t.cpos;
+    for (int i = tid; i < DN2 / 4; i += nt) {
+        tl[i] = gld(cl + i);
+        tp[i] = gld(cp + i);
     }
     for (int i = tid; i < 52; i += nt) t.clen[i] = s->clen[i];
 }
@@ -675,13 +797,13 @@ __device__ __forceinline__ TermBuf<float> light_buf(TermLight &t, const SlotDesc
 }
 
 // FP64 buffers live in global scratch: one slice per workgroup
-#define XSCRATCH_DOUBLES (5 * DN2 + 1328 + 5 * 52 + 64)
+#define XSCRATCH_DOUBLES (5 * DN2 + 3 * PGS + 5 * 52 + 64)
 __device__ __forceinline__ TermBuf<double> global_buf(double *base, const SlotDesc *__restrict__ s) {
     TermBuf<double> b;
     b.M1 = base; b.M2 = b.M1 + DN2;
     b.PC1 = b.M2 + DN2; b.PC2 = b.PC1 + DN2; b.PCP = b.PC2 + DN2;
     b.PG = b.PCP + DN2;
-    b.C1 = b.PG + 1328; b.C2 = b.C1 + 52; b.CP = b.C2 + 52; b.U1 = b.CP + 52; b.U2 = b.U1 + 52;
+    b.C1 = b.PG + 3 * PGS; b.C2 = b.C1 + 52; b.CP = b.C2 + 52; b.U1 = b.CP + 52; b.U2 = b.U1 + 52;
     b.red = b.U2 + 52;
     b.clist = s->clist;
     b.clen = s->clen;
@@ -703,7 +825,7 @@ __global__ void __launch_bounds__(SS_BLOCK) k_terminals(const SlotDesc *__restri
     __syncthreads();
     for (int j = blockIdx.x; j < s->T; j += gridDim.x) {
         int n = s->term[j];
-        terminal_node<float, true>(s, n, trav, s->reach, s->cfv + (size_t)n * s->Mp, b, TK_NODE, nullptr,
+        terminal_node<float, TM_ALL>(s, n, trav, s->reach, s->cfv + (size_t)n * s->Mp, b, TK_NODE, nullptr,
                              threadIdx.x, blockDim.x);
     }
 }
@@ -749,7 +871,7 @@ __global__ void __launch_bounds__(SS_BATCH_BLOCK) k_batched(SlotDesc *__restrict
                 }
                 for (int j = 0; j < s->T; ++j) {
                     int n = s->term[j];
-                    terminal_node<float, true>(s, n, trav, s->reach, s->cfv + (size_t)n * s->Mp, b, TK_NODE, nullptr,
+                    terminal_node<float, TM_ALL>(s, n, trav, s->reach, s->cfv + (size_t)n * s->Mp, b, TK_NODE, nullptr,
                                          tid, nt);
                     __syncthreads();
                 }
@@ -890,7 +1012,7 @@ __global__ void __launch_bounds__(SS_BLOCK) k_xterminals(const SlotDesc *__restr
     TermBuf<double> b = global_buf(scratch + (size_t)blockIdx.x * XSCRATCH_DOUBLES, s);
     for (int j = blockIdx.x; j < s->T; j += gridDim.x) {
         int n = s->term[j];
-        terminal_node<double, true>(s, n, p, s->xreach, s->xev + (size_t)n * s->Mp, b, TK_NODE, nullptr, threadIdx.x,
+        terminal_node<double, TM_ALL>(s, n, p, s->xreach, s->xev + (size_t)n * s->Mp, b, TK_NODE, nullptr, threadIdx.x,
                               blockDim.x);
         for (int h = threadIdx.x; h < s->nh[p]; h += blockDim.x)
             s->xbr[(size_t)n * s->Mp + h] = s->xev[(size_t)n * s->Mp + h];
@@ -912,7 +1034,7 @@ __global__ void __launch_bounds__(SS_BLOCK) k_xjoint(const SlotDesc *__restrict_
     TermBuf<double> b = global_buf(scratch, s);
     // xreach rows of node 0 hold the raw weights; the per-hand mass lands in xev row 0
     // with scale = weights of player 0 restricted to live hands
-    terminal_node<double, true>(s, 0, 0, s->xreach, s->xev, b, TK_MASS, s->weights, threadIdx.x, blockDim.x);
+    terminal_node<double, TM_ALL>(s, 0, 0, s->xreach, s->xev, b, TK_MASS, s->weights, threadIdx.x, blockDim.x);
     double acc = 0;
     for (int h = threadIdx.x; h < s->nh[0]; h += blockDim.x)
         if (s->hinfo[h] & SS_LIVE_BIT) acc += s->xev[h];
@@ -943,7 +1065,7 @@ __global__ void __launch_bounds__(SS_BLOCK) k_batched_expl(SlotDesc *__restrict_
         for (int p = 0; p < s->NP; ++p) {
             for (int j = 0; j < s->T; ++j) {
                 int n = s->term[j];
-                terminal_node<double, true>(s, n, p, s->xreach, s->xev + (size_t)n * s->Mp, b, TK_NODE, nullptr, tid, nt);
+                terminal_node<double, TM_ALL>(s, n, p, s->xreach, s->xev + (size_t)n * s->Mp, b, TK_NODE, nullptr, tid, nt);
                 for (int h = tid; h < s->nh[p]; h += nt) s->xbr[(size_t)n * s->Mp + h] = s->xev[(size_t)n * s->Mp + h];
                 __syncthreads();
             }
@@ -953,7 +1075,7 @@ __global__ void __launch_bounds__(SS_BLOCK) k_batched_expl(SlotDesc *__restrict_
             }
             xreduce_block(s, p, red, tid, nt);
         }
-        terminal_node<double, true>(s, 0, 0, s->xreach, s->xev, b, TK_MASS, s->weights, tid, nt);
+        terminal_node<double, TM_ALL>(s, 0, 0, s->xreach, s->xev, b, TK_MASS, s->weights, tid, nt);
         double acc = 0;
         for (int h = tid; h < s->nh[0]; h += nt)
             if (s->hinfo[h] & SS_LIVE_BIT) acc += s->xev[h];
@@ -1005,15 +1127,17 @@ __global__ void __launch_bounds__(SS_BLOCK) k_term_light(SlotDesc *__restrict__ 
         if (!slot_runs(s, trav)) continue;
         TermBuf<float> b = light_buf(sh, s);
         int n = TASK_NODE(tk);
-        terminal_node<float, false>(s, n, trav, s->reach, s->cfv + (size_t)n * s->Mp, b, TK_NODE, nullptr,
+        terminal_node<float, TM_LIGHT>(s, n, trav, s->reach, s->cfv + (size_t)n * s->Mp, b, TK_NODE, nullptr,
                                     threadIdx.x, blockDim.x);
         __syncthreads();
     }
 }
 
-// terminal tasks with two (or more) opponents for some traverser
-__global__ void __launch_bounds__(SS_BLOCK) k_term_heavy(SlotDesc *__restrict__ slots, const uint32_t *tasks,
-                                                         int ntasks, int trav) {
+// terminal tasks with exactly two opponents for some traverser: 512 threads,
+// at most 128 VGPRs, so two workgroups (LDS-bound) give four waves per SIMD
+#define SS_PAIR_BLOCK 512
+__global__ void __launch_bounds__(SS_PAIR_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
+k_term_pair(SlotDesc *__restrict__ slots, const uint32_t *tasks, int ntasks, int trav) {
     __shared__ TermHeavy sh;
     TermBuf<float> b = heavy_buf(sh);
     for (int j = blockIdx.x; j < ntasks; j += gridDim.x) {
@@ -1023,8 +1147,26 @@ __global__ void __launch_bounds__(SS_BLOCK) k_term_heavy(SlotDesc *__restrict__ 
         stage_tables(s, sh, threadIdx.x, blockDim.x);
         __syncthreads();
         int n = TASK_NODE(tk);
-        terminal_node<float, true>(s, n, trav, s->reach, s->cfv + (size_t)n * s->Mp, b, TK_NODE, nullptr,
-                                   threadIdx.x, blockDim.x);
+        terminal_node<float, TM_PAIR>(s, n, trav, s->reach, s->cfv + (size_t)n * s->Mp, b, TK_NODE, nullptr,
+                                      threadIdx.x, blockDim.x);
+        __syncthreads();
+    }
+}
+
+// terminal tasks with three or more opponents for some traverser
+__global__ void __launch_bounds__(SS_BLOCK) k_term_odo(SlotDesc *__restrict__ slots, const uint32_t *tasks,
+                                                       int ntasks, int trav) {
+    __shared__ TermHeavy sh;
+    TermBuf<float> b = heavy_buf(sh);
+    for (int j = blockIdx.x; j < ntasks; j += gridDim.x) {
+        uint32_t tk = tasks[j];
+        const SlotDesc *s = slots + TASK_SLOT(tk);
+        if (!slot_runs(s, trav)) continue;
+        stage_tables(s, sh, threadIdx.x, blockDim.x);
+        __syncthreads();
+        int n = TASK_NODE(tk);
+        terminal_node<float, TM_ALL>(s, n, trav, s->reach, s->cfv + (size_t)n * s->Mp, b, TK_NODE, nullptr,
+                                     threadIdx.x, blockDim.x);
         __syncthreads();
     }
 }
@@ -1089,7 +1231,7 @@ __global__ void __launch_bounds__(SS_BLOCK) k_xterm_phased(SlotDesc *__restrict_
         if (!slot_wanted(s, want, TASK_SLOT(tk)) || p >= s->NP) continue;
         TermBuf<double> b = global_buf(scratch + (size_t)blockIdx.x * XSCRATCH_DOUBLES, s);
         int n = TASK_NODE(tk);
-        terminal_node<double, true>(s, n, p, s->xreach, s->xev + (size_t)n * s->Mp, b, TK_NODE, nullptr, threadIdx.x,
+        terminal_node<double, TM_ALL>(s, n, p, s->xreach, s->xev + (size_t)n * s->Mp, b, TK_NODE, nullptr, threadIdx.x,
                               blockDim.x);
         for (int h = threadIdx.x; h < s->nh[p]; h += blockDim.x)
             s->xbr[(size_t)n * s->Mp + h] = s->xev[(size_t)n * s->Mp + h];
@@ -1124,7 +1266,7 @@ __global__ void __launch_bounds__(SS_BLOCK) k_xjoint_phased(SlotDesc *__restrict
         const SlotDesc *s = slots + si;
         if (!slot_wanted(s, want, si)) continue;
         TermBuf<double> b = global_buf(scratch + (size_t)blockIdx.x * XSCRATCH_DOUBLES, s);
-        terminal_node<double, true>(s, 0, 0, s->xreach, s->xev, b, TK_MASS, s->weights, threadIdx.x, blockDim.x);
+        terminal_node<double, TM_ALL>(s, 0, 0, s->xreach, s->xev, b, TK_MASS, s->weights, threadIdx.x, blockDim.x);
         double acc = 0;
         for (int h = threadIdx.x; h < s->nh[0]; h += blockDim.x)
             if (s->hinfo[h] & SS_LIVE_BIT) acc += s->xev[h];
@@ -1295,10 +1437,11 @@ int build_phased_tasks(const std::vector<Slot *> &slots, PhasedTasks &pt, hipStr
     });
     pt.term_off = (int)tasks.size();
     pt.nterm = (int)term.size();
-    pt.nheavy = 0;
+    pt.nodo = pt.npair = 0;
     for (auto &t : term) {
         tasks.push_back(t.second);
-        if (t.first >= 20) ++pt.nheavy;
+        if (t.first >= 1000) ++pt.nodo;
+        else if (t.first >= 20) ++pt.npair;
     }
     if (pt.d_tasks) { SS_HIP(hipFree(pt.d_tasks)); pt.d_tasks = nullptr; }
     SS_HIP(hipMalloc(&pt.d_tasks, sizeof(uint32_t) * std::max<size_t>(1, tasks.size())));
@@ -1316,12 +1459,16 @@ int launch_phased_iteration(SlotDesc *d_slots, const PhasedTasks &pt, hipStream_
                 hipLaunchKernelGGL(k_fwd_phased, dim3(cnt), dim3(SS_BLOCK), 0, st, d_slots, pt.d_tasks + pt.lvl_off[l],
                                    cnt, t);
         }
-        if (pt.nheavy > 0)
-            hipLaunchKernelGGL(k_term_heavy, dim3(pt.nheavy), dim3(SS_BLOCK), 0, st, d_slots, pt.d_tasks + pt.term_off,
-                               pt.nheavy, t);
-        if (pt.nterm > pt.nheavy)
-            hipLaunchKernelGGL(k_term_light, dim3(pt.nterm - pt.nheavy), dim3(SS_BLOCK), 0, st, d_slots,
-                               pt.d_tasks + pt.term_off + pt.nheavy, pt.nterm - pt.nheavy, t);
+        const int nheavy = pt.nodo + pt.npair;
+        if (pt.nodo > 0)
+            hipLaunchKernelGGL(k_term_odo, dim3(pt.nodo), dim3(SS_BLOCK), 0, st, d_slots, pt.d_tasks + pt.term_off,
+                               pt.nodo, t);
+        if (pt.npair > 0)
+            hipLaunchKernelGGL(k_term_pair, dim3(pt.npair), dim3(SS_PAIR_BLOCK), 0, st, d_slots,
+                               pt.d_tasks + pt.term_off + pt.nodo, pt.npair, t);
+        if (pt.nterm > nheavy)
+            hipLaunchKernelGGL(k_term_light, dim3(pt.nterm - nheavy), dim3(SS_BLOCK), 0, st, d_slots,
+                               pt.d_tasks + pt.term_off + nheavy, pt.nterm - nheavy, t);
         for (int l = pt.L - 1; l >= 0; --l) {
             int cnt = pt.lvl_off[l + 1] - pt.lvl_off[l];
             if (cnt > 0)

@@ -263,8 +263,6 @@ void build_river_tables(const uint32_t *cstr, RiverTables &rt) {
     rt.clist.assign(52 * 52, 0);
     rt.cpos.assign(52 * 52, 255);
     rt.clen.assign(52, 0);
-    rt.plt.assign(52 * SS_NCOMBO, 0);
-    rt.ple.assign(52 * SS_NCOMBO, 0);
     std::vector<int> live;
     for (int k = 0; k < SS_NCOMBO; ++k)
         if (cstr[k]) live.push_back(k);
@@ -278,6 +276,14 @@ void build_river_tables(const uint32_t *cstr, RiverTables &rt) {
         rt.gsort[i] = (uint16_t)(a | ((a + 1 + rem) << 8));
         sorted[i] = cstr[live[i]];
     }
+    // strength classes: distinct live ranks, ascending
+    std::vector<uint32_t> cls(sorted);
+    cls.erase(std::unique(cls.begin(), cls.end()), cls.end());
+    rt.ncls = (int)cls.size();
+    rt.kcls.assign(SS_NCOMBO, 0);
+    rt.bnd.assign(SS_BND_BYTES, 0);
+    for (int k = 0; k < SS_NCOMBO; ++k)
+        if (cstr[k]) rt.kcls[k] = (uint16_t)(std::lower_bound(cls.begin(), cls.end(), cstr[k]) - cls.begin());
     for (int k = 0; k < SS_NCOMBO; ++k) {
         uint32_t s = cstr[k];
         rt.glt[k] = (int16_t)(std::lower_bound(sorted.begin(), sorted.end(), s) - sorted.begin());
@@ -293,14 +299,12 @@ void build_river_tables(const uint32_t *cstr, RiverTables &rt) {
         std::vector<uint32_t> ps(part.size());
         for (size_t j = 0; j < part.size(); ++j) {
             rt.clist[c * 52 + j] = (uint8_t)part[j];
-            rt.cpos[c * 52 + part[j]] = (uint8_t)j;
+            rt.cpos[c * 52 + part[j]] = (uint8_t)(4 * j);
             ps[j] = cstr[combo_id(c, part[j])];
         }
-        for (int k = 0; k < SS_NCOMBO; ++k) {
-            uint32_t s = cstr[k];
-            rt.plt[c * SS_NCOMBO + k] = (uint8_t)(std::lower_bound(ps.begin(), ps.end(), s) - ps.begin());
-            rt.ple[c * SS_NCOMBO + k] = (uint8_t)(std::upper_bound(ps.begin(), ps.end(), s) - ps.begin());
-        }
+        for (size_t t = 0; t < cls.size(); ++t)
+            rt.bnd[t * 52 + c] = (uint8_t)(4 * (std::lower_bound(ps.begin(), ps.end(), cls[t]) - ps.begin()));
+        rt.bnd[cls.size() * 52 + c] = (uint8_t)(4 * ps.size());
     }
 }
 
@@ -332,7 +336,7 @@ int64_t estimate_footprint(const HostProblem &hp) {
     b += (int64_t)pl.N * sizeof(DevNode) + pl.children.size() * 4 + (pl.D + pl.L + 1 + pl.T) * 4;
     b += (int64_t)pl.NP * Mp * 8 + pl.leaf_count * 4 + pl.N * 4;
     b += (int64_t)2 * pl.A * Mp * 4 + (int64_t)pl.N * Mp * 4 + (int64_t)pl.NP * pl.N * Mp * 4 + 8 * Mp * 4;
-    b += pl.is_river ? (SS_NCOMBO * 4 + 3 * SS_NCOMBO * 2 + 52 * 53 + 2 * 52 * SS_NCOMBO) : 0;
+    b += pl.is_river ? (SS_NCOMBO * 4 + SS_NCOMBO * 2 + 52 * 53 * 2 + SS_BND_BYTES + (int64_t)pl.NP * Mp * 16) : 0;
     // exploitability buffers
     b += (int64_t)pl.A * Mp * 4 + (int64_t)(pl.NP + 2) * pl.N * Mp * 8;
     return b + 64 * 1024;
@@ -360,14 +364,11 @@ int Slot::setup(const HostProblem &hp, hipStream_t st) {
     size_t o_rsig = L.take(sizeof(float) * 8 * Mp);
     size_t o_cstr = L.take(sizeof(uint32_t) * SS_NCOMBO);
     size_t o_gsort = L.take(sizeof(uint16_t) * SS_NCOMBO);
-    size_t o_glt = L.take(sizeof(int16_t) * SS_NCOMBO);
-    size_t o_gle = L.take(sizeof(int16_t) * SS_NCOMBO);
     size_t o_clist = L.take(52 * 52);
     size_t o_clen = L.take(52);
-    size_t o_plt = pl.is_river ? L.take(52 * SS_NCOMBO) : 0;
-    size_t o_ple = pl.is_river ? L.take(52 * SS_NCOMBO) : 0;
+    size_t o_bnd = pl.is_river ? L.take(SS_BND_BYTES) : 0;
     size_t o_cpos = L.take(52 * 52);
-    size_t o_hsort = L.take(sizeof(uint16_t) * pl.NP * Mp);
+    size_t o_hmeta = pl.is_river ? L.take(sizeof(uint4) * pl.NP * Mp) : 0;
     size_t o_board = L.take(sizeof(int) * 8);
     arena_bytes = L.off;
     SS_HIP(hipMalloc(&arena, arena_bytes));
@@ -416,14 +417,11 @@ int Slot::setup(const HostProblem &hp, hipStream_t st) {
     d.rsig = (float *)(a + o_rsig);
     d.cstr = (const uint32_t *)(a + o_cstr);
     d.gsort = (const uint16_t *)(a + o_gsort);
-    d.glt = (const int16_t *)(a + o_glt);
-    d.gle = (const int16_t *)(a + o_gle);
     d.clist = (const uint8_t *)(a + o_clist);
     d.clen = (const uint8_t *)(a + o_clen);
-    d.plt = pl.is_river ? (const uint8_t *)(a + o_plt) : nullptr;
-    d.ple = pl.is_river ? (const uint8_t *)(a + o_ple) : nullptr;
+    d.bnd = pl.is_river ? (const uint8_t *)(a + o_bnd) : nullptr;
     d.cpos = (const uint8_t *)(a + o_cpos);
-    d.hsort = (const uint16_t *)(a + o_hsort);
+    d.hmeta = pl.is_river ? (const uint4 *)(a + o_hmeta) : nullptr;
     d.active = 1;
 
     if (pl.is_river && pl.num_board == 5) {
@@ -440,34 +438,38 @@ int Slot::setup(const HostProblem &hp, hipStream_t st) {
         d.ng = rt.ng;
         SS_HIP(hipMemcpyAsync(a + o_gsort, rt.gsort.data(), sizeof(uint16_t) * rt.gsort.size(), hipMemcpyHostToDevice, st));
         SS_HIP(hipMemcpyAsync(a + o_cpos, rt.cpos.data(), 52 * 52, hipMemcpyHostToDevice, st));
-        // hands of every player by ascending strength (dead hands last)
-        std::vector<uint16_t> hs((size_t)pl.NP * Mp);
+        // hands of every player by ascending strength (dead hands last), one
+        // record per hand with everything a showdown sweep needs
+        std::vector<uint4> hm((size_t)pl.NP * Mp);
         for (int p = 0; p < pl.NP; ++p) {
             std::vector<int> idx(pl.nh[p]);
             std::iota(idx.begin(), idx.end(), 0);
             auto rank = [&](int h) -> uint32_t {
                 uint32_t hi = pl.hinfo[(size_t)p * Mp + h];
-                return (hi & SS_LIVE_BIT) ? rt.cstr[(hi >> 12) & 2047] : 0xFFFFFFFFu;
+                return (hi & SS_LIVE_BIT) ? rt.cstr[(hi >> 12) & 2047] : 0u;
             };
-            std::stable_sort(idx.begin(), idx.end(), [&](int a2, int b2) { return rank(a2) < rank(b2); });
-            for (size_t i = 0; i < idx.size(); ++i) hs[(size_t)p * Mp + i] = (uint16_t)idx[i];
+            auto key = [&](int h) -> uint32_t { uint32_t r = rank(h); return r ? r : 0xFFFFFFFFu; };
+            std::stable_sort(idx.begin(), idx.end(), [&](int a2, int b2) { return key(a2) < key(b2); });
+            for (size_t i = 0; i < idx.size(); ++i) {
+                const int h = idx[i];
+                const uint32_t hi = pl.hinfo[(size_t)p * Mp + h];
+                const int kk = (hi >> 12) & 2047;
+                const uint32_t ranked = rank(h) ? 1u : 0u;
+                uint4 r;
+                r.x = (uint32_t)h | ((hi & 63) << 11) | (((hi >> 6) & 63) << 17) | (ranked << 23);
+                r.y = ranked ? rt.kcls[kk] : 0u;
+                r.z = ranked ? (uint32_t)rt.glt[kk] : 0u;
+                r.w = ranked ? (uint32_t)rt.gle[kk] : 0u;
+                hm[(size_t)p * Mp + i] = r;
+            }
         }
-        SS_HIP(hipMemcpyAsync(a + o_hsort, hs.data(), sizeof(uint16_t) * hs.size(), hipMemcpyHostToDevice, st));
-        SS_HIP(hipMemcpyAsync(a + o_glt, rt.glt.data(), sizeof(int16_t) * SS_NCOMBO, hipMemcpyHostToDevice, st));
-        SS_HIP(hipMemcpyAsync(a + o_gle, rt.gle.data(), sizeof(int16_t) * SS_NCOMBO, hipMemcpyHostToDevice, st));
+        SS_HIP(hipMemcpyAsync(a + o_hmeta, hm.data(), sizeof(uint4) * hm.size(), hipMemcpyHostToDevice, st));
         SS_HIP(hipMemcpyAsync(a + o_clist, rt.clist.data(), 52 * 52, hipMemcpyHostToDevice, st));
         SS_HIP(hipMemcpyAsync(a + o_clen, rt.clen.data(), 52, hipMemcpyHostToDevice, st));
-        SS_HIP(hipMemcpyAsync(a + o_plt, rt.plt.data(), 52 * SS_NCOMBO, hipMemcpyHostToDevice, st));
-        SS_HIP(hipMemcpyAsync(a + o_ple, rt.ple.data(), 52 * SS_NCOMBO, hipMemcpyHostToDevice, st));
+        SS_HIP(hipMemcpyAsync(a + o_bnd, rt.bnd.data(), SS_BND_BYTES, hipMemcpyHostToDevice, st));
         SS_HIP(hipStreamSynchronize(st));   // host tables go out of scope
-        SS_HIP(hipStreamSynchronize(st));   // hs goes out of scope
     } else {
         d.is_river = 0;   // showdowns off the river carry no ranks: value 0, as street_solve.cu:1520
-        std::vector<uint16_t> hs((size_t)pl.NP * Mp);
-        for (int p = 0; p < pl.NP; ++p)
-            for (int i = 0; i < Mp; ++i) hs[(size_t)p * Mp + i] = (uint16_t)i;
-        SS_HIP(hipMemcpyAsync(a + o_hsort, hs.data(), sizeof(uint16_t) * hs.size(), hipMemcpyHostToDevice, st));
-        SS_HIP(hipStreamSynchronize(st));
     }
     SS_HIP(hipMemcpyAsync(d_desc, &d, sizeof(d), hipMemcpyHostToDevice, st));
     return SS_OK;
