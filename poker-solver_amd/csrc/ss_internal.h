@@ -111,7 +111,8 @@ struct PhasedTasks {
     std::vector<int> lvl_off;      // level l tasks: [lvl_off[l], lvl_off[l+1]); all decision tasks [0, lvl_off[L])
     int term_off = 0, nterm = 0;   // terminal tasks, most expensive first
     int nodo = 0;                  // terminal tasks [0, nodo): three or more opponents for some traverser
-    int npair = 0;                 // [nodo, nodo + npair): two opponents; the rest at most one
+    int npair = 0;                 // [nodo, nodo + npair): two opponents (showdowns first); the rest at most one
+    int npair_sd = 0;              // showdowns among the npair
     uint32_t *d_tasks = nullptr;
     ~PhasedTasks();
 };

@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "../../include/street_solve.h"
 #include "ss_device.h"
@@ -543,8 +544,10 @@ __device__ __forceinline__ void pair_products(const T *M1, const T *M2, const T 
 #define TM_PAIR 1
 #define TM_ALL 2
 
-// Writes out[h] for h < nh[trav] of node n for traverser trav.
-template <typename T, int MODE>
+// Writes out[h] for h < nh[trav] of node n for traverser trav. SDK = 1 / 0
+// compiles only the showdown / non-showdown sweep (callers guarantee the node
+// type), -1 both.
+template <typename T, int MODE, int SDK = -1>
 __device__ void terminal_node(const SlotDesc *__restrict__ s, int n, int trav, const T *reach, T *out,
                               TermBuf<T> B, int kind, const float *scale, int tid, int nt) {
     const DevNode &nd = s->nodes[n];
@@ -584,7 +587,7 @@ __device__ void terminal_node(const SlotDesc *__restrict__ s, int n, int trav, c
         return;
     }
     // ---- closed forms, k <= 2 ----
-    const bool sd = type == SS_NODE_SHOWDOWN;
+    const bool sd = SDK < 0 ? type == SS_NODE_SHOWDOWN : SDK == 1;
     const bool two = MODE != TM_LIGHT && k == 2;
     {
         const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -684,6 +687,28 @@ __device__ void terminal_node(const SlotDesc *__restrict__ s, int n, int trav, c
                 const uint8_t *rl = (const uint8_t *)rowl;
                 const int lx4 = gld(rl + x), ly4 = gld(rl + y), ex4 = gld(rl + DN + x), ey4 = gld(rl + DN + y);
                 uint32_t lnext = gld(rowl), enext = gld(rowl + 13);
+                // terms of card c: opponent pairs sharing c, with the hands
+                // {c,x}, {c,y} (not opponent hands) removed from the classes
+                // they fall in, by their position in c's sorted list
+                auto card = [&](int c, int l4, int e4, T &xw, T &xaa) {
+                    const T P1l = at4(B.PC1 + c * DN, l4), P2l = at4(B.PC2 + c * DN, l4);
+                    const T P1e = at4(B.PC1 + c * DN, e4), P2e = at4(B.PC2 + c * DN, e4);
+                    const int px4 = B.cpos[c * DN + x], py4 = B.cpos[c * DN + y];
+                    const T m1x = B.M1[c * DN + x], m1y = B.M1[c * DN + y];
+                    const T m2x = B.M2[c * DN + x], m2y = B.M2[c * DN + y];
+                    const T fxl = px4 < l4 ? (T)1 : (T)0, fxe = px4 < e4 ? (T)1 : (T)0;
+                    const T fyl = py4 < l4 ? (T)1 : (T)0, fye = py4 < e4 ? (T)1 : (T)0;
+                    const T Al = P1l - fxl * m1x - fyl * m1y;
+                    const T Ale = P1e - fxe * m1x - fye * m1y;
+                    const T Bl = P2l - fxl * m2x - fyl * m2y;
+                    const T Ble = P2e - fxe * m2x - fye * m2y;
+                    xw += Al * ((T)2 * Bl + Ble);
+                    xw += Ale * (Bl + (T)2 * Ble);
+                    xaa += (B.C1[c] - m1x - m1y) * (B.C2[c] - m2x - m2y);
+                };
+                // every card (board cards have empty lists: exact zeros), then
+                // the traverser's own two cards, which carry no opponent pairs,
+                // taken back out
                 T xw = 0, xaa = 0;
 #pragma unroll 1
                 for (int g = 0; g < 13; ++g) {
@@ -693,31 +718,15 @@ __device__ void terminal_node(const SlotDesc *__restrict__ s, int n, int trav, c
                         enext = gld(rowl + 14 + g);
                     }
 #pragma unroll
-                    for (int b = 0; b < 4; ++b) {
-                        // board cards have empty lists: every term below is an exact 0
-                        const int c = 4 * g + b;
-                        const int l4 = (lw >> (8 * b)) & 0xFF, e4 = (ew >> (8 * b)) & 0xFF;
-                        const T P1l = at4(B.PC1 + c * DN, l4), P2l = at4(B.PC2 + c * DN, l4);
-                        const T P1e = at4(B.PC1 + c * DN, e4), P2e = at4(B.PC2 + c * DN, e4);
-                        // {c,x}, {c,y} are not opponent hands: remove them from the
-                        // classes they fall in (by their position in c's sorted list)
-                        const int px4 = B.cpos[c * DN + x], py4 = B.cpos[c * DN + y];
-                        const T m1x = B.M1[c * DN + x], m1y = B.M1[c * DN + y];
-                        const T m2x = B.M2[c * DN + x], m2y = B.M2[c * DN + y];
-                        const T fxl = px4 < l4 ? (T)1 : (T)0, fxe = px4 < e4 ? (T)1 : (T)0;
-                        const T fyl = py4 < l4 ? (T)1 : (T)0, fye = py4 < e4 ? (T)1 : (T)0;
-                        // cards of the traverser's hand carry no opponent pairs
-                        const bool own = (c == x) | (c == y);
-                        const T Al = own ? (T)0 : P1l - fxl * m1x - fyl * m1y;
-                        const T Ale = own ? (T)0 : P1e - fxe * m1x - fye * m1y;
-                        const T Bl = P2l - fxl * m2x - fyl * m2y;
-                        const T Ble = P2e - fxe * m2x - fye * m2y;
-                        xw += Al * ((T)2 * Bl + Ble);
-                        xw += Ale * (Bl + (T)2 * Ble);
-                        const T c1 = B.C1[c], c2 = B.C2[c];   // unconditional loads: no branch
-                        const T aa = c1 - m1x - m1y;
-                        xaa += (own ? (T)0 : aa) * (c2 - m2x - m2y);
-                    }
+                    for (int b = 0; b < 4; ++b)
+                        card(4 * g + b, (lw >> (8 * b)) & 0xFF, (ew >> (8 * b)) & 0xFF, xw, xaa);
+                }
+                {
+                    T ow = 0, oa = 0;
+                    card(x, lx4, ex4, ow, oa);
+                    card(y, ly4, ey4, ow, oa);
+                    xw -= ow;
+                    xaa -= oa;
                 }
                 const int lx = lx4 >> 2, ly = ly4 >> 2, ex = ex4 >> 2, ey = ey4 >> 2;
                 const T m1k = B.M1[xy], m2k = B.M2[xy];
@@ -747,25 +756,19 @@ __device__ void terminal_node(const SlotDesc *__restrict__ s, int n, int trav, c
 // tables, the subgame's per-card lists. Light (<= 1 opponent, ~30 KB): one
 // matrix, one prefix table; five workgroups fit a CU.
 // Rows of 2752 = 43 x 64 floats: the same cell of M1 and M2 (PC1 and PC2)
-// is one ds_read2st64 away.
+// is one ds_read2st64 away. The byte tables and short arrays come first so
+// their addresses fit the 16-bit immediate offsets of LDS instructions.
 #define DNP 2752
 struct TermHeavy {
-    float M1[DNP], M2[DNP];
-    float PC1[DNP], PC2[DNP], PCP[DNP];
-    float PG[3 * PGS];
-    float C1[52], C2[52], CP[52], U1[52], U2[52];
-    float red[64];
     alignas(16) uint8_t clist[DN2];
     alignas(16) uint8_t cpos[DN2];
-    uint8_t clen[52];
-};
-
-struct TermLight {
-    float M1[DN2];
-    float PC1[DN2];
-    float PG[1328];
-    float C1[52];
+    alignas(16) uint8_t clen[64];
+    float C1[52], C2[52], CP[52], U1[52], U2[52];
     float red[64];
+    float PG[3 * PGS];
+    alignas(256) float M1[DNP];
+    float M2[DNP];
+    float PC1[DNP], PC2[DNP], PCP[DNP];
 };
 
 __device__ __forceinline__ TermBuf<float> heavy_buf(TermHeavy &t) {
@@ -775,6 +778,30 @@ __device__ __forceinline__ TermBuf<float> heavy_buf(TermHeavy &t) {
     b.clist = t.clist; b.clen = t.clen; b.cpos = t.cpos;
     return b;
 }
+
+// Two-opponent fold and leaf nodes (~33 KB): both matrices and G = M1^T M2.
+struct TermFold {
+    float M1[DNP], M2[DNP], G[DNP];
+    float C1[52], C2[52], CP[52], U1[52], U2[52];
+    float red[64];
+};
+
+__device__ __forceinline__ TermBuf<float> fold_buf(TermFold &t) {
+    TermBuf<float> b;
+    b.M1 = t.M1; b.M2 = t.M2; b.PC1 = t.G; b.PC2 = nullptr; b.PCP = nullptr;
+    b.PG = nullptr; b.C1 = t.C1; b.C2 = t.C2; b.CP = t.CP; b.U1 = t.U1; b.U2 = t.U2; b.red = t.red;
+    b.clist = nullptr; b.clen = nullptr; b.cpos = nullptr;
+    return b;
+}
+
+struct TermLight {
+    float M1[DN2];
+    float PC1[DN2];
+    float PG[1328];
+    float C1[52];
+    float red[64];
+};
+
 
 // copy a river subgame's per-card lists and positions into LDS (caller barriers)
 __device__ __forceinline__ void stage_tables(const SlotDesc *__restrict__ s, TermHeavy &t, int tid, int nt) {
@@ -1133,22 +1160,28 @@ __global__ void __launch_bounds__(SS_BLOCK) k_term_light(SlotDesc *__restrict__ 
     }
 }
 
-// terminal tasks with exactly two opponents for some traverser: 512 threads,
-// at most 128 VGPRs, so two workgroups (LDS-bound) give four waves per SIMD
+// Terminal tasks with exactly two opponents for some traverser, at most 128
+// VGPRs (four waves per SIMD). Showdowns (SDK = 1): 512 threads, two
+// workgroups per CU by LDS. Folds and leaves (SDK = 0): 256 threads, four
+// workgroups per CU.
 #define SS_PAIR_BLOCK 512
-__global__ void __launch_bounds__(SS_PAIR_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
+template <int SDK>
+__global__ void __launch_bounds__(SDK ? SS_PAIR_BLOCK : SS_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
 k_term_pair(SlotDesc *__restrict__ slots, const uint32_t *tasks, int ntasks, int trav) {
-    __shared__ TermHeavy sh;
-    TermBuf<float> b = heavy_buf(sh);
+    using Buf = typename std::conditional<SDK == 1, TermHeavy, TermFold>::type;
+    __shared__ Buf sh;
+    TermBuf<float> b;
+    if constexpr (SDK == 1) b = heavy_buf(sh);
+    else b = fold_buf(sh);
     for (int j = blockIdx.x; j < ntasks; j += gridDim.x) {
         uint32_t tk = tasks[j];
         const SlotDesc *s = slots + TASK_SLOT(tk);
         if (!slot_runs(s, trav)) continue;
-        stage_tables(s, sh, threadIdx.x, blockDim.x);
+        if constexpr (SDK == 1) stage_tables(s, sh, threadIdx.x, blockDim.x);
         __syncthreads();
         int n = TASK_NODE(tk);
-        terminal_node<float, TM_PAIR>(s, n, trav, s->reach, s->cfv + (size_t)n * s->Mp, b, TK_NODE, nullptr,
-                                      threadIdx.x, blockDim.x);
+        terminal_node<float, TM_PAIR, SDK>(s, n, trav, s->reach, s->cfv + (size_t)n * s->Mp, b, TK_NODE, nullptr,
+                                           threadIdx.x, blockDim.x);
         __syncthreads();
     }
 }
@@ -1398,8 +1431,29 @@ static int term_cost(const Plan &pl, int n) {
         kmax = std::max(kmax, k);
     }
     if (kmax >= 3) return 1000;
-    if (kmax == 2) return nd.type == SS_NODE_SHOWDOWN ? 100 : 20;
+    if (kmax == 2) return nd.type == SS_NODE_SHOWDOWN ? 100 : 20;   // k_term_pair<1> / <0>
     return 5;
+}
+
+// Workgroups of a launch are dealt to the XCDs round-robin (MI355X: 8 XCDs,
+// each with its own L2). Every phased launch runs task j on workgroup j, so
+// ordering a segment to put subgame s at positions j = s (mod 8) keeps each
+// subgame's tables and state in one XCD's L2. Queues that run dry are
+// backfilled from the longest remaining one, so the order stays a permutation.
+#define SS_NUM_XCD 8
+static void xcd_interleave(std::vector<uint32_t> &v, size_t b, size_t e) {
+    std::vector<std::vector<uint32_t>> q(SS_NUM_XCD);
+    for (size_t i = b; i < e; ++i) q[(v[i] >> 20) % SS_NUM_XCD].push_back(v[i]);
+    std::vector<size_t> at(SS_NUM_XCD, 0);
+    for (size_t j = b; j < e; ++j) {
+        int x = (int)((j - b) % SS_NUM_XCD);
+        if (at[x] == q[x].size()) {
+            size_t best = 0;
+            for (int y = 0; y < SS_NUM_XCD; ++y)
+                if (q[y].size() - at[y] > best) { best = q[y].size() - at[y]; x = y; }
+        }
+        v[j] = q[x][at[x]++];
+    }
 }
 
 int build_phased_tasks(const std::vector<Slot *> &slots, PhasedTasks &pt, hipStream_t st, const std::vector<int> *live) {
@@ -1424,6 +1478,7 @@ int build_phased_tasks(const std::vector<Slot *> &slots, PhasedTasks &pt, hipStr
             for (int i = pl.dec_off[l]; i < pl.dec_off[l + 1]; ++i)
                 tasks.push_back(((uint32_t)si << 20) | (uint32_t)pl.dec_lvl[i]);
         }
+        xcd_interleave(tasks, pt.lvl_off[l], tasks.size());
     }
     pt.lvl_off[pt.L] = (int)tasks.size();
     std::vector<std::pair<int, uint32_t>> term;
@@ -1437,12 +1492,19 @@ int build_phased_tasks(const std::vector<Slot *> &slots, PhasedTasks &pt, hipStr
     });
     pt.term_off = (int)tasks.size();
     pt.nterm = (int)term.size();
-    pt.nodo = pt.npair = 0;
+    pt.nodo = pt.npair_sd = pt.npair = 0;
     for (auto &t : term) {
         tasks.push_back(t.second);
         if (t.first >= 1000) ++pt.nodo;
         else if (t.first >= 20) ++pt.npair;
+        if (t.first == 100) ++pt.npair_sd;
     }
+    // each kernel's segment in XCD order (equal cost within a segment)
+    const size_t t0 = pt.term_off, t1 = t0 + pt.nodo, t2 = t1 + pt.npair_sd, t3 = t0 + pt.nodo + pt.npair;
+    xcd_interleave(tasks, t0, t1);
+    xcd_interleave(tasks, t1, t2);
+    xcd_interleave(tasks, t2, t3);
+    xcd_interleave(tasks, t3, tasks.size());
     if (pt.d_tasks) { SS_HIP(hipFree(pt.d_tasks)); pt.d_tasks = nullptr; }
     SS_HIP(hipMalloc(&pt.d_tasks, sizeof(uint32_t) * std::max<size_t>(1, tasks.size())));
     SS_HIP(hipMemcpyAsync(pt.d_tasks, tasks.data(), sizeof(uint32_t) * tasks.size(), hipMemcpyHostToDevice, st));
@@ -1463,9 +1525,12 @@ int launch_phased_iteration(SlotDesc *d_slots, const PhasedTasks &pt, hipStream_
         if (pt.nodo > 0)
             hipLaunchKernelGGL(k_term_odo, dim3(pt.nodo), dim3(SS_BLOCK), 0, st, d_slots, pt.d_tasks + pt.term_off,
                                pt.nodo, t);
-        if (pt.npair > 0)
-            hipLaunchKernelGGL(k_term_pair, dim3(pt.npair), dim3(SS_PAIR_BLOCK), 0, st, d_slots,
-                               pt.d_tasks + pt.term_off + pt.nodo, pt.npair, t);
+        if (pt.npair_sd > 0)
+            hipLaunchKernelGGL(k_term_pair<1>, dim3(pt.npair_sd), dim3(SS_PAIR_BLOCK), 0, st, d_slots,
+                               pt.d_tasks + pt.term_off + pt.nodo, pt.npair_sd, t);
+        if (pt.npair > pt.npair_sd)
+            hipLaunchKernelGGL(k_term_pair<0>, dim3(pt.npair - pt.npair_sd), dim3(SS_BLOCK), 0, st, d_slots,
+                               pt.d_tasks + pt.term_off + pt.nodo + pt.npair_sd, pt.npair - pt.npair_sd, t);
         if (pt.nterm > nheavy)
             hipLaunchKernelGGL(k_term_light, dim3(pt.nterm - nheavy), dim3(SS_BLOCK), 0, st, d_slots,
                                pt.d_tasks + pt.term_off + nheavy, pt.nterm - nheavy, t);
