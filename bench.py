@@ -117,6 +117,70 @@ def cpu_reference_baseline(cores, iters_per_solve=500):
             "seconds_per_iteration": t_iter}
 
 
+# ---------------------------------------------------------------- served form
+def served_bench(n, rate, seed, target, check_every, max_iters, iters_per_step, log_every=50):
+    """Poisson stream through ss_server_*: 70% rivers (2-3 players, as the
+    batch mix) and 30% turns (2-3 players, continuation leaves N(0, 0.3 pot)).
+    Requests are built before the clock starts and submitted at their arrival
+    times from a separate thread; latency = arrival to retirement (measured
+    inside the server), throughput = completed / (last retirement - first
+    arrival)."""
+    import threading
+    import ss_api
+    import workloads as wl
+    stream = wl.served_stream(n, rate, seed=seed, river_max_players=3, turn_max_players=3)
+    probs = [ss_api.Problem.from_spot(sp) for _, sp in stream]
+    kinds = [sp.kind for _, sp in stream]
+    srv = ss_api.Server(max_slots=256, iters_per_step=iters_per_step, check_every=check_every)
+    tickets = [None] * n
+    t_start = time.perf_counter()
+
+    def submitter():
+        for i, (t_arr, _) in enumerate(stream):
+            d = t_start + t_arr - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            tickets[i] = srv.submit(probs[i], max_iters, target=target, check_every=check_every)
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    done = {}
+    t_last = t_start
+    while len(done) < n:
+        for i in range(n):
+            if i in done or tickets[i] is None:
+                continue
+            r = srv.poll(tickets[i])
+            if r is not None:
+                done[i] = r
+                t_last = time.perf_counter()
+                if len(done) % log_every == 0:
+                    log(f"[bench] served {len(done)}/{n}")
+        time.sleep(0.001)
+    th.join()
+    srv.close()
+    lat = np.array([done[i].elapsed_ms for i in range(n)])
+    its = np.array([done[i].iterations for i in range(n)])
+    conv = np.array([done[i].converged for i in range(n)])
+    first_arrival = stream[0][0]
+    span = (t_last - t_start) - first_arrival
+    return {
+        "workload": f"Poisson arrivals at {rate}/s, {n} requests: 70% river (players U{{2,3}}, batch-mix "
+                    f"shapes), 30% turn (4-card board, 1128 combos, players U{{2,3}}, 1-2 sizes, "
+                    f"continuation leaves N(0, 0.3 pot)); retire at <= {target}% pot checked every "
+                    f"{check_every} iterations, cap {max_iters}",
+        "rate_per_s": rate, "requests": n,
+        "river": kinds.count("river"), "turn": kinds.count("turn"),
+        "solves_per_s": round(n / span, 3),
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2),
+                       "p99": round(float(np.percentile(lat, 99)), 2),
+                       "max": round(float(lat.max()), 2)},
+        "iterations": {"mean": float(its.mean()), "max": int(its.max())},
+        "converged_fraction": float(conv.mean()),
+        "iters_per_step": iters_per_step,
+    }
+
+
 # ---------------------------------------------------------------- main
 def main():
     ap = argparse.ArgumentParser()
@@ -134,6 +198,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=16, help="subgames timed by the CPU oracle")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cpu-ref", action="store_true")
+    ap.add_argument("--served-requests", type=int, default=200, help="0 skips the served-form run")
+    ap.add_argument("--served-rate", type=float, default=40.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--served-iters-per-step", type=int, default=5)
     a = ap.parse_args()
 
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -205,7 +272,11 @@ def main():
             "target_exploitability_pct_pot": a.target,
             "check_every": a.check_every,
             "max_iterations": a.max_iters,
-            "form": "batched: one persistent kernel launch per CFR iteration, hipGraph-captured",
+            "form": ("batched, phased: per traverser pass one launch per tree level / terminal class over the "
+                     "(subgame, node) tasks of every live subgame, check_every iterations per hipGraph replay"
+                     if stats_all[-1].mode == 1 else
+                     "batched, persistent: one workgroup per subgame, one launch per CFR iteration, "
+                     "hipGraph-captured"),
         },
         "roofline": {
             "bound": "hbm",
@@ -214,7 +285,7 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
             "traffic": None,
-            "kernel": "k_batched",
+            "kernel": "cfr_iteration" if stats_all[-1].mode == 1 else "k_batched",
             "launches": launches,
             "avg_launch_us": round(cfr_ms * 1000.0 / max(1, launches), 3),
             "algo_bytes_per_launch": int(algo / max(1, launches)),
@@ -227,6 +298,13 @@ def main():
         "exploitability_pct_pot": {"mean": float(expl.mean()), "max": float(expl.max())},
         "device_ms_per_step": {"cfr": round(cfr_ms / a.steps, 3), "checks": round(check_ms / a.steps, 3)},
     }
+
+    if a.served_requests > 0:
+        log(f"[bench] served form: {a.served_requests} requests at {a.served_rate}/s")
+        line["served"] = served_bench(a.served_requests, a.served_rate, 20261203 + 5000, a.target,
+                                      a.check_every, a.max_iters, a.served_iters_per_step)
+        log(f"[bench] served: {line['served']['solves_per_s']} solves/s, "
+            f"p50 {line['served']['latency_ms']['p50']} ms, p99 {line['served']['latency_ms']['p99']} ms")
 
     if not a.no_cpu:
         cores = a.cpu_cores or min(16, os.cpu_count() or 1)

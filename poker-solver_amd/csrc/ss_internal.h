@@ -116,7 +116,7 @@ struct PhasedTasks {
     uint32_t *d_tasks = nullptr;
     ~PhasedTasks();
 };
-// tasks of the slots with live[slot] != 0 (all slots when live is null)
+// tasks of the non-null slots with live[slot] != 0 (all of them when live is null)
 int build_phased_tasks(const std::vector<Slot *> &slots, PhasedTasks &pt, hipStream_t st,
                        const std::vector<int> *live = nullptr);
 int launch_phased_iteration(SlotDesc *d_slots, const PhasedTasks &pt, hipStream_t st);

@@ -1460,10 +1460,11 @@ int build_phased_tasks(const std::vector<Slot *> &slots, PhasedTasks &pt, hipStr
     pt.nslots = (int)slots.size();
     pt.NPmax = 0;
     pt.L = 0;
+    auto on = [&](int si) { return slots[si] && (!live || (*live)[si]); };
     for (int si = 0; si < pt.nslots; ++si) {
         const Slot *s = slots[si];
+        if (!on(si)) continue;
         if (s->plan.N >= (1 << 20)) return set_error(SS_ERR_UNSUPPORTED, "tree too large for the batched form");
-        if (live && !(*live)[si]) continue;
         pt.NPmax = std::max(pt.NPmax, s->plan.NP);
         pt.L = std::max(pt.L, s->plan.L);
     }
@@ -1473,8 +1474,9 @@ int build_phased_tasks(const std::vector<Slot *> &slots, PhasedTasks &pt, hipStr
     for (int l = 0; l < pt.L; ++l) {
         pt.lvl_off[l] = (int)tasks.size();
         for (int si = 0; si < pt.nslots; ++si) {
+            if (!on(si)) continue;
             const Plan &pl = slots[si]->plan;
-            if (l >= pl.L || (live && !(*live)[si])) continue;
+            if (l >= pl.L) continue;
             for (int i = pl.dec_off[l]; i < pl.dec_off[l + 1]; ++i)
                 tasks.push_back(((uint32_t)si << 20) | (uint32_t)pl.dec_lvl[i]);
         }
@@ -1483,8 +1485,8 @@ int build_phased_tasks(const std::vector<Slot *> &slots, PhasedTasks &pt, hipStr
     pt.lvl_off[pt.L] = (int)tasks.size();
     std::vector<std::pair<int, uint32_t>> term;
     for (int si = 0; si < pt.nslots; ++si) {
+        if (!on(si)) continue;
         const Plan &pl = slots[si]->plan;
-        if (live && !(*live)[si]) continue;
         for (int j = 0; j < pl.T; ++j) term.push_back({term_cost(pl, pl.term[j]), ((uint32_t)si << 20) | (uint32_t)pl.term[j]});
     }
     std::stable_sort(term.begin(), term.end(), [](const std::pair<int, uint32_t> &a, const std::pair<int, uint32_t> &b) {

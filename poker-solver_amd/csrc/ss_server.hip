@@ -1,16 +1,19 @@
 // ss_server.hip -- served form: a continuous-batching request loop on one GPU.
 //
 // Requests are deep-copied at submit and queued. A worker thread owns the
-// device: between graph launches it admits queued requests into free slots
-// while the resident state fits the HBM budget, advances every resident
-// subgame by `iters_per_step` iterations with one captured graph (one
-// persistent launch per iteration, inactive slots skip), evaluates the FP64
-// exploitability of subgames that reached a check point, and retires those at
-// their target (or at their iteration cap) into completed results. Each slot
-// is computed by its own workgroup with no cross-slot data, so a subgame's
-// result does not depend on what else is resident.
+// device: between steps it admits queued requests into free slots while the
+// resident state fits the HBM budget, advances every resident subgame by
+// `iters_per_step` iterations of the phased batched form (one launch per phase
+// of a traverser pass, over the (subgame, node) tasks of every resident
+// subgame), evaluates the FP64 exploitability of subgames that reached a check
+// point, and retires those at their target (or at their iteration cap) into
+// completed results. The task list is rebuilt when the resident set changes;
+// a step over an unchanged set replays a captured graph. Every task is one
+// workgroup on one subgame's own state and every reduction has a fixed order,
+// so a subgame's result does not depend on what else is resident.
 #include <chrono>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -123,6 +126,10 @@ struct SSServer {
     double *scratch = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    ss::PhasedTasks pt;
+    bool dirty = true;        // resident set changed since the tasks were built
+    bool have_graph = false;  // exec replays the current tasks
+    bool no_graph = false;    // SS_NO_GRAPH set: always launch directly (profilers)
     std::vector<std::shared_ptr<ss::Request>> resident;   // by slot index
     int64_t used_bytes = 0;
     int fatal = 0;
@@ -131,6 +138,8 @@ struct SSServer {
     int init();
     void run();
     int admit_locked(std::unique_lock<std::mutex> &lk);
+    int refresh_tasks();
+    void drop_graph();
     int step();
     int check_and_retire();
     void finish(const std::shared_ptr<ss::Request> &r, int state);
@@ -147,22 +156,32 @@ int SSServer::init() {
     SS_HIP(hipMalloc(&d_want, sizeof(int) * n));
     SS_HIP(hipMalloc(&scratch, sizeof(double) * (size_t)n * SS_XSCRATCH_DOUBLES));
     resident.assign(n, nullptr);
-    SS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    for (int k = 0; k < cfg.iters_per_step; ++k) {
-        int rc = launch_batched(d_slots, n, 1, n, st);
-        if (rc) {
-            (void)hipStreamEndCapture(st, &graph);
-            return rc;
-        }
-    }
-    SS_HIP(hipStreamEndCapture(st, &graph));
-    SS_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    no_graph = std::getenv("SS_NO_GRAPH") != nullptr;
+    return SS_OK;
+}
+
+void SSServer::drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    have_graph = false;
+}
+
+// rebuild the task list over the resident slots
+int SSServer::refresh_tasks() {
+    drop_graph();
+    std::vector<Slot *> sl(cfg.max_slots, nullptr);
+    for (int i = 0; i < cfg.max_slots; ++i)
+        if (resident[i]) sl[i] = resident[i]->dev.get();
+    int rc = build_phased_tasks(sl, pt, st);
+    if (rc) return rc;
+    dirty = false;
     return SS_OK;
 }
 
 SSServer::~SSServer() {
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
+    drop_graph();
     for (auto &r : resident) r.reset();
     if (d_slots) (void)hipFree(d_slots);
     if (d_want) (void)hipFree(d_want);
@@ -179,6 +198,7 @@ void SSServer::finish(const std::shared_ptr<Request> &r, int state) {
         (void)hipStreamSynchronize(st);
         resident[r->slot].reset();
         r->slot = -1;
+        dirty = true;
     }
     if (r->dev) {
         used_bytes -= r->footprint;
@@ -233,16 +253,44 @@ int SSServer::admit_locked(std::unique_lock<std::mutex> &lk) {
         r->next_check = r->cfg.check_every;
         resident[free_slot] = r;
         used_bytes += r->footprint;
+        dirty = true;
     }
     return SS_OK;
 }
 
+// Advance every resident subgame by iters_per_step iterations: direct launches
+// right after the resident set changed, a captured graph of the same launches
+// once it has stayed the same for a step.
 int SSServer::step() {
+    bool capture = false;
+    if (dirty) {
+        int rc = refresh_tasks();
+        if (rc) return rc;
+    } else if (!have_graph) {
+        capture = !no_graph;
+    }
     hipEvent_t e0, e1;
     SS_HIP(hipEventCreate(&e0));
     SS_HIP(hipEventCreate(&e1));
+    if (capture) {
+        SS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        int rc = SS_OK;
+        for (int k = 0; k < cfg.iters_per_step && !rc; ++k) rc = launch_phased_iteration(d_slots, pt, st);
+        hipError_t e = hipStreamEndCapture(st, &graph);
+        if (rc) return rc;
+        SS_HIP(e);
+        SS_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        have_graph = true;
+    }
     SS_HIP(hipEventRecord(e0, st));
-    SS_HIP(hipGraphLaunch(exec, st));
+    if (have_graph) {
+        SS_HIP(hipGraphLaunch(exec, st));
+    } else {
+        for (int k = 0; k < cfg.iters_per_step; ++k) {
+            int rc = launch_phased_iteration(d_slots, pt, st);
+            if (rc) return rc;
+        }
+    }
     SS_HIP(hipEventRecord(e1, st));
     SS_HIP(hipEventSynchronize(e1));
     float ms = 0;
@@ -271,7 +319,7 @@ int SSServer::check_and_retire() {
     }
     if (any) {
         SS_HIP(hipMemcpyAsync(d_want, want.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
-        int rc = launch_batched_expl(d_slots, n, scratch, d_want, n, st);
+        int rc = launch_phased_expl(d_slots, pt, d_want, scratch, n, st);
         if (rc) return rc;
         SS_HIP(hipStreamSynchronize(st));
         for (int i = 0; i < n; ++i) {

@@ -1,0 +1,121 @@
+"""Served form (ss_server_*): continuous batching on the GPU.
+
+Every request's subgame is computed by its own workgroups with fixed-order
+reductions, so its result must not depend on what else is resident: served
+results equal the plain form's bit for bit, and a subgame solved alone equals
+the same subgame solved inside a mix that arrives around it.
+"""
+
+import time
+
+import numpy as np
+import pytest
+
+import ss_api
+import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+
+def _wait(srv, tickets, timeout_s=120.0):
+    out = {}
+    t0 = time.time()
+    while len(out) < len(tickets):
+        for t in tickets:
+            if t in out:
+                continue
+            r = srv.poll(t)
+            if r is not None:
+                out[t] = r
+        assert time.time() - t0 < timeout_s, "served requests did not finish"
+        time.sleep(0.002)
+    return [out[t] for t in tickets]
+
+
+def _same(a, b):
+    assert np.array_equal(a.avg, b.avg)
+    assert np.array_equal(a.root_strategy, b.root_strategy)
+    assert np.array_equal(a.root_ev, b.root_ev)
+
+
+def _mix_spots():
+    spots = [wl.small_spot(60 + i, num_players=2 + (i % 2), hands_per_player=9) for i in range(5)]
+    spots.append(wl.river_2p(seed=91))
+    spots.append(wl.mix_river(wl.BASE_SEED + 4000 + 3, max_players=3))
+    spots.append(wl.mix_turn(wl.BASE_SEED + 200000 + 1, max_players=3))
+    return spots
+
+
+def test_served_equals_plain():
+    spots = _mix_spots()
+    probs = [ss_api.Problem.from_spot(s) for s in spots]
+    srv = ss_api.Server(max_slots=16, iters_per_step=5, check_every=10)
+    try:
+        tickets = [srv.submit(p, 20, target=0.0) for p in probs]
+        got = _wait(srv, tickets)
+    finally:
+        srv.close()
+    for p, g in zip(probs, got):
+        want = ss_api.solve_plain(p, 20)
+        assert g.iterations == 20
+        _same(g, want)
+        assert g.exploitability == pytest.approx(want.exploitability, rel=1e-12)
+
+
+def test_served_alone_equals_in_mix():
+    """Early stopping included: checks every 10 iterations against 0.5% pot."""
+    target = wl.mix_river(wl.BASE_SEED + 4000 + 7, max_players=3)
+    p = ss_api.Problem.from_spot(target)
+    srv = ss_api.Server(max_slots=8, iters_per_step=10, check_every=10)
+    try:
+        alone = _wait(srv, [srv.submit(p, 300, target=0.5, check_every=10)])[0]
+    finally:
+        srv.close()
+
+    others = [ss_api.Problem.from_spot(s) for s in _mix_spots()]
+    srv = ss_api.Server(max_slots=8, iters_per_step=10, check_every=10)
+    try:
+        first = [srv.submit(q, 120, target=0.5, check_every=10) for q in others[:4]]
+        mine = srv.submit(p, 300, target=0.5, check_every=10)
+        time.sleep(0.05)
+        later = [srv.submit(q, 120, target=0.5, check_every=10) for q in others[4:]]
+        res = _wait(srv, first + [mine] + later)
+    finally:
+        srv.close()
+    mixed = res[len(first)]
+    assert mixed.iterations == alone.iterations
+    assert mixed.converged == alone.converged
+    assert mixed.exploitability == alone.exploitability
+    _same(mixed, alone)
+
+
+def test_served_admission_by_budget():
+    """A budget that fits one subgame at a time: requests queue, then run."""
+    spots = [wl.small_spot(80 + i, num_players=2, hands_per_player=8) for i in range(4)]
+    probs = [ss_api.Problem.from_spot(s) for s in spots]
+    fp = max(ss_api.plan_bytes(q)[1] for q in probs)
+    srv = ss_api.Server(hbm_budget_bytes=int(fp * 1.5), max_slots=4, iters_per_step=5)
+    try:
+        tickets = [srv.submit(q, 15, target=0.0) for q in probs]
+        got = _wait(srv, tickets)
+    finally:
+        srv.close()
+    for q, g in zip(probs, got):
+        _same(g, ss_api.solve_plain(q, 15))
+
+
+def test_served_cancel_queued():
+    spots = [wl.small_spot(90 + i, num_players=2, hands_per_player=8) for i in range(3)]
+    probs = [ss_api.Problem.from_spot(s) for s in spots]
+    fp = max(ss_api.plan_bytes(q)[1] for q in probs)
+    srv = ss_api.Server(hbm_budget_bytes=int(fp * 1.5), max_slots=4, iters_per_step=5)
+    try:
+        t0 = srv.submit(probs[0], 400, target=0.0)
+        t1 = srv.submit(probs[1], 10, target=0.0)
+        srv.cancel(t1)
+        with pytest.raises(ss_api.SolveError):
+            _wait(srv, [t1])
+        r0 = _wait(srv, [t0])[0]
+        assert r0.iterations == 400
+    finally:
+        srv.close()
