@@ -27,7 +27,8 @@ int ssdbg_combo_ranks(const int *board5, uint32_t *out1326);
 /* Device census over all C(52,7) hands: counts per category 0..9, rank sum. */
 int ssdbg_eval_census(unsigned long long counts[10], unsigned long long *rank_sum);
 
-/* Raw state after `iterations` CFR iterations (form 0 plain, 1 batched kernel),
+/* Raw state after `iterations` CFR iterations (form 0 plain, 1 batched kernel,
+ * 2 plain with the BF16 precision showdown of 2-player rivers),
  * in the reference layout [num_nodes][8][max_hands]; root_ev[max_hands]. */
 int ssdbg_solve_state(const SSProblem *problem, int iterations, int form, float *regrets, float *strat_sum,
                       float *root_ev);

@@ -22,7 +22,7 @@ BUILD = os.path.join(PKG, "build")
 OBJ = os.path.join(BUILD, "obj")
 LIB = os.path.join(BUILD, "street_solve.so")
 
-SOURCES = ["ss_tree.cpp", "ss_runtime.hip", "ss_kernels.hip", "ss_capi.hip", "ss_server.hip"]
+SOURCES = ["ss_tree.cpp", "ss_runtime.hip", "ss_kernels.hip", "ss_precision.hip", "ss_capi.hip", "ss_server.hip"]
 
 ARCH = os.environ.get("SS_OFFLOAD_ARCH", "gfx950")
 

@@ -37,6 +37,9 @@ static int solve_plain(const HostProblem &hp, const SSSolveConfig &cfg, SSOutput
     Slot s;
     int rc = s.setup(hp, g.st);
     if (rc) return rc;
+    if (cfg.precision == SS_PREC_BF16X2) rc = prec_setup(s, g.st);
+    else if (cfg.precision != SS_PREC_FP32) rc = set_error(SS_ERR_ARG, "unknown precision %d", cfg.precision);
+    if (rc) return rc;
     const int T = std::max(0, cfg.max_iterations);
     const bool target = cfg.target_exploitability > 0;
     const int chunk = target ? std::max(1, cfg.check_every > 0 ? cfg.check_every : 50) : std::max(1, T);
@@ -118,6 +121,8 @@ struct BatchImpl {
     }
 
     int create(const HostProblem *hps, int count, const SSSolveConfig &c) {
+        if (c.precision != SS_PREC_FP32)
+            return set_error(SS_ERR_UNSUPPORTED, "the batched form runs FP32 (precision form: ss_solve_gpu_ex)");
         cfg = c;
         n = count;
         const int T = std::max(0, cfg.max_iterations);
@@ -547,7 +552,11 @@ int ssdbg_solve_state(const SSProblem *problem, int iterations, int form, float 
     Slot s;
     rc = s.setup(hp, g.st);
     if (rc) return rc;
-    if (form == 0) {
+    if (form == 2) {   // plain form with the BF16 precision showdown
+        rc = prec_setup(s, g.st);
+        if (rc) return rc;
+    }
+    if (form == 0 || form == 2) {
         double ms = 0;
         rc = plain_run(s, iterations, iterations, g.st, &ms);
         if (rc) return rc;

@@ -38,6 +38,7 @@
 #include <stdint.h>
 
 #define SS_NCOMBO 1326
+#define SS_BLOCK 256          // threads per workgroup of the (node x hand) sweeps
 #define SS_BND_BYTES (1082 * 52)   // <= 1081 live combos on a 5-card board, + 1 row
 #define SS_LIVE_BIT (1u << 23)
 

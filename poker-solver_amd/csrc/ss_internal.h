@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -74,8 +75,20 @@ struct RiverTables {
 };
 void build_river_tables(const uint32_t *cstr, RiverTables &rt);
 
+// Precision-form buffers of a 2-player river subgame (ss_precision.hip).
+struct PrecState {
+    int S = 0, S16 = 0;                   // showdown nodes, rounded up to 16
+    int *sd_nodes = nullptr;              // [S]
+    uint16_t *W[2] = {nullptr, nullptr};  // per traverser [Mp][Mp] bf16 (+1 / -1 / 0)
+    uint16_t *Bt = nullptr;               // [2 S16][Mp] bf16 hi / lo opponent reach
+    float *Cn = nullptr;                  // [S][64] per-card opponent totals
+    int *same[2] = {nullptr, nullptr};    // [Mp] opponent hand with the same cards, or -1
+    ~PrecState();
+};
+
 // A subgame resident on the device.
 struct Slot {
+    std::unique_ptr<PrecState> prec;   // precision form only
     Plan plan;
     SlotDesc desc{};              // host copy (pointers are device pointers)
     SlotDesc *d_desc = nullptr;   // device copy (plain form); batched forms copy desc into their array
@@ -117,6 +130,10 @@ struct PhasedTasks {
     ~PhasedTasks();
 };
 // tasks of the non-null slots with live[slot] != 0 (all of them when live is null)
+int prec_setup(Slot &s, hipStream_t st);
+int prec_showdowns(Slot &s, int t, hipStream_t st);
+int64_t prec_flops_per_pass(const Slot &s);
+
 int build_phased_tasks(const std::vector<Slot *> &slots, PhasedTasks &pt, hipStream_t st,
                        const std::vector<int> *live = nullptr);
 int launch_phased_iteration(SlotDesc *d_slots, const PhasedTasks &pt, hipStream_t st);

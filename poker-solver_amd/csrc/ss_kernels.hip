@@ -32,7 +32,6 @@
 
 namespace ss {
 
-#define SS_BLOCK 256
 #define SS_BATCH_BLOCK 512
 #define NT_DEC 8
 
@@ -845,13 +844,15 @@ __global__ void __launch_bounds__(SS_BLOCK) k_forward(const SlotDesc *__restrict
     forward_items(s, level, trav, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 }
 
-__global__ void __launch_bounds__(SS_BLOCK) k_terminals(const SlotDesc *__restrict__ s, int trav) {
+// skip_sd: showdown values come from the precision form's contraction
+__global__ void __launch_bounds__(SS_BLOCK) k_terminals(const SlotDesc *__restrict__ s, int trav, int skip_sd) {
     __shared__ TermHeavy sh;
     TermBuf<float> b = heavy_buf(sh);
     stage_tables(s, sh, threadIdx.x, blockDim.x);
     __syncthreads();
     for (int j = blockIdx.x; j < s->T; j += gridDim.x) {
         int n = s->term[j];
+        if (skip_sd && s->nodes[n].type == SS_NODE_SHOWDOWN) continue;
         terminal_node<float, TM_ALL>(s, n, trav, s->reach, s->cfv + (size_t)n * s->Mp, b, TK_NODE, nullptr,
                              threadIdx.x, blockDim.x);
     }
@@ -1351,7 +1352,11 @@ int plain_run(Slot &s, int iters, int final_iter, hipStream_t st, double *kernel
                 if (cnt > 0)
                     hipLaunchKernelGGL(k_forward, dim3(grid_for((int64_t)cnt * pl.Mp)), dim3(SS_BLOCK), 0, st, d, l, t);
             }
-            if (pl.T > 0) hipLaunchKernelGGL(k_terminals, dim3(pl.T), dim3(SS_BLOCK), 0, st, d, t);
+            if (s.prec) {
+                int rc = prec_showdowns(s, t, st);
+                if (rc) return rc;
+            }
+            if (pl.T > 0) hipLaunchKernelGGL(k_terminals, dim3(pl.T), dim3(SS_BLOCK), 0, st, d, t, s.prec ? 1 : 0);
             for (int l = pl.L - 1; l >= 0; --l) {
                 int cnt = pl.dec_off[l + 1] - pl.dec_off[l];
                 if (cnt > 0)

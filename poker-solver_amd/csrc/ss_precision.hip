@@ -1,0 +1,231 @@
+// ss_precision.hip -- precision form: the 2-player river showdown as a BF16
+// hi/lo MFMA contraction with FP32 accumulation.
+//
+// For traverser t, opponent o and showdown node n (pot P, traverser bet b):
+//     v(i, n) = P/2 * sum_j W(i, j) r_o(n, j) + (P/2 - b) * Z(i, n)
+// with W(i, j) = sign(rank_i - rank_j) for card-disjoint live hands (else 0)
+// and Z the opponent reach disjoint from hand i (the card-removal mass, O(1)
+// per hand from per-card totals). This is the reference's 2-player showdown
+// (street_solve.cu:886-903: pot * (P_win + P_tie / 2) - bet * Z) rewritten with
+// P_win + P_tie / 2 = (Z + W r) / 2.
+//
+// The contraction over opponent hands runs on v_mfma_f32_16x16x32_bf16 for
+// all S showdown nodes at once: Y[Mp x 2 S16] = W[Mp x Mp] * [R_hi | R_lo],
+// where R_hi = bf16(r) and R_lo = bf16(r - R_hi), so the reach carries 16
+// significant bits and W (entries -1, 0, +1) is exact. Z, the mass floor and
+// the epilogue are FP32 as in the closed-form path.
+#include "ss_internal.h"
+
+namespace ss {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
+
+#define PREC_MAX_TILES 8   // S16 / 16 <= 8: at most 128 showdown nodes
+#define DN2_P (52 * 52)
+
+struct PrecArgs {
+    int S, S16, NT;           // showdown nodes, S rounded up to 16, S16 / 16
+    const int *sd_nodes;      // [S]
+    const uint16_t *W;        // [Mp][Mp] bf16 bits, rows = traverser hands
+    uint16_t *Bt;             // [2 S16][Mp] bf16 bits: rows b < S16 hi, S16 + b lo
+    float *Cn;                // [S][64]: per-card opponent totals, [52] = total
+    const int *same;          // [Mp] opponent hand with the traverser hand's cards, or -1
+};
+
+// round-to-nearest-even float -> bf16 bits (finite inputs)
+__device__ __forceinline__ uint16_t bf16_rne(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf16_to_f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// W for traverser t: +1 / -1 / 0 by rank, 0 for blocked or card-sharing pairs
+__global__ void __launch_bounds__(SS_BLOCK) k_prec_w(const SlotDesc *__restrict__ s, int t, uint16_t *W) {
+    const int Mp = s->Mp, o = 1 - t;
+    const size_t total = (size_t)Mp * Mp;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(q / Mp), j = (int)(q % Mp);
+        uint16_t w = 0;
+        if (i < s->nh[t] && j < s->nh[o]) {
+            const uint32_t hi = s->hinfo[(size_t)t * Mp + i], hj = s->hinfo[(size_t)o * Mp + j];
+            if ((hi & SS_LIVE_BIT) && (hj & SS_LIVE_BIT)) {
+                const uint64_t mi = (1ull << (hi & 63)) | (1ull << ((hi >> 6) & 63));
+                const uint64_t mj = (1ull << (hj & 63)) | (1ull << ((hj >> 6) & 63));
+                const uint32_t si = s->cstr[(hi >> 12) & 2047], sj = s->cstr[(hj >> 12) & 2047];
+                if (!(mi & mj) && si && sj) w = si > sj ? 0x3F80u : (si < sj ? 0xBF80u : 0u);
+            }
+        }
+        W[q] = w;
+    }
+}
+
+// One workgroup per showdown node: the opponent's reach as bf16 hi / lo rows
+// of Bt, and its per-card totals (dense scatter + wave64 sums, fixed order).
+__global__ void __launch_bounds__(SS_BLOCK) k_prec_split(const SlotDesc *__restrict__ s, PrecArgs a, int t) {
+    __shared__ float M[DN2_P];
+    __shared__ float C[64];
+    const int b = blockIdx.x, n = a.sd_nodes[b], o = 1 - t, Mp = s->Mp;
+    const float *r = s->reach + ((size_t)o * s->N + n) * Mp;
+    for (int j = threadIdx.x; j < Mp; j += blockDim.x) {
+        const bool ok = j < s->nh[o] && (s->hinfo[(size_t)o * Mp + j] & SS_LIVE_BIT);
+        const float x = ok ? r[j] : 0.f;
+        const uint16_t h = bf16_rne(x);
+        a.Bt[(size_t)b * Mp + j] = h;
+        a.Bt[(size_t)(a.S16 + b) * Mp + j] = bf16_rne(x - bf16_to_f(h));
+    }
+    for (int i = threadIdx.x; i < DN2_P; i += blockDim.x) M[i] = 0.f;
+    __syncthreads();
+    for (int j = threadIdx.x; j < s->nh[o]; j += blockDim.x) {
+        const uint32_t hj = s->hinfo[(size_t)o * Mp + j];
+        if (!(hj & SS_LIVE_BIT)) continue;
+        const int c0 = hj & 63, c1 = (hj >> 6) & 63;
+        atomicAdd(&M[c0 * 52 + c1], r[j]);   // one add per cell with unique hands
+        atomicAdd(&M[c1 * 52 + c0], r[j]);
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    for (int c = threadIdx.x >> 6; c < 52; c += blockDim.x >> 6) {
+        float v = lane < 52 ? M[c * 52 + lane] : 0.f;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if (lane == 0) C[c] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 52) a.Cn[(size_t)b * 64 + threadIdx.x] = C[threadIdx.x];
+    if (threadIdx.x == 0) {
+        float tot = 0.f;
+        for (int c = 0; c < 52; ++c) tot += C[c];
+        a.Cn[(size_t)b * 64 + 52] = 0.5f * tot;
+    }
+}
+
+// One wavefront per 16 traverser hands: Y = W * [R_hi | R_lo] over all
+// showdown columns, then the showdown values written into cfv.
+// MFMA 16x16x32 lane layout: lane l holds A[l % 16][8 (l / 16) + 0..7],
+// B[8 (l / 16) + 0..7][l % 16], D[4 (l / 16) + 0..3][l % 16].
+__global__ void __launch_bounds__(64) k_prec_gemm(const SlotDesc *__restrict__ s, PrecArgs a, int t) {
+    const int l = threadIdx.x, Mp = s->Mp, o = 1 - t;
+    const int row0 = blockIdx.x * 16;
+    floatx4 acc[2 * PREC_MAX_TILES];
+#pragma unroll
+    for (int q = 0; q < 2 * PREC_MAX_TILES; ++q) acc[q] = floatx4{0.f, 0.f, 0.f, 0.f};
+    const uint16_t *wrow = a.W + (size_t)(row0 + (l & 15)) * Mp + 8 * (l >> 4);
+    const uint16_t *bcol = a.Bt + (size_t)(l & 15) * Mp + 8 * (l >> 4);
+    for (int k0 = 0; k0 < Mp; k0 += 32) {
+        const bf16x8 va = __builtin_bit_cast(bf16x8, *(const uint4 *)(wrow + k0));
+#pragma unroll
+        for (int q = 0; q < PREC_MAX_TILES; ++q) {
+            if (q >= a.NT) break;
+            const bf16x8 vh = __builtin_bit_cast(bf16x8, *(const uint4 *)(bcol + (size_t)(16 * q) * Mp + k0));
+            const bf16x8 vl =
+                __builtin_bit_cast(bf16x8, *(const uint4 *)(bcol + (size_t)(a.S16 + 16 * q) * Mp + k0));
+            acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va, vh, acc[q], 0, 0, 0);
+            acc[PREC_MAX_TILES + q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va, vl, acc[PREC_MAX_TILES + q], 0, 0, 0);
+        }
+    }
+    const float *ro = s->reach + (size_t)o * s->N * Mp;
+#pragma unroll
+    for (int q = 0; q < PREC_MAX_TILES; ++q) {
+        if (q >= a.NT) break;
+        const int b = 16 * q + (l & 15);
+        if (b >= a.S) continue;
+        const int n = a.sd_nodes[b];
+        const DevNode &nd = s->nodes[n];
+        const float half = 0.5f * (float)nd.pot, bet = (float)nd.bets[t];
+        const float *Cb = a.Cn + (size_t)b * 64;
+        const float Sb = Cb[52];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int i = row0 + 4 * (l >> 4) + e;
+            if (i >= s->nh[t]) continue;
+            const uint32_t hi = s->hinfo[(size_t)t * Mp + i];
+            float v = 0.f;
+            if ((hi & SS_LIVE_BIT) && s->cstr[(hi >> 12) & 2047]) {                 // hs = 0 -> 0, :861
+                const int x = hi & 63, y = (hi >> 6) & 63, sm = a.same[i];
+                const float z = Sb - Cb[x] - Cb[y] + (sm >= 0 ? ro[(size_t)n * Mp + sm] : 0.f);
+                const float Y = acc[q][e] + acc[PREC_MAX_TILES + q][e];
+                v = z < 1e-5f * Sb ? 0.f : half * Y + (half - bet) * z;
+            }
+            s->cfv[(size_t)n * Mp + i] = v;
+        }
+    }
+}
+
+// ---- host ------------------------------------------------------------------
+PrecState::~PrecState() {
+    for (void *p : {(void *)W[0], (void *)W[1], (void *)Bt, (void *)Cn, (void *)same[0], (void *)same[1],
+                    (void *)sd_nodes})
+        if (p) (void)hipFree(p);
+}
+
+int prec_setup(Slot &s, hipStream_t st) {
+    const Plan &pl = s.plan;
+    if (pl.NP != 2 || !s.desc.is_river)
+        return set_error(SS_ERR_UNSUPPORTED, "the BF16 precision form covers 2-player river subgames");
+    std::vector<int> sd;
+    for (int j = 0; j < pl.T; ++j)
+        if (pl.nodes[pl.term[j]].type == SS_NODE_SHOWDOWN) sd.push_back(pl.term[j]);
+    if (sd.empty() || sd.size() > 16 * PREC_MAX_TILES)
+        return set_error(SS_ERR_UNSUPPORTED, "precision form: %zu showdown nodes (1..%d supported)", sd.size(),
+                         16 * PREC_MAX_TILES);
+    std::unique_ptr<PrecState> p(new PrecState());
+    const int Mp = pl.Mp;
+    p->S = (int)sd.size();
+    p->S16 = (p->S + 15) / 16 * 16;
+    SS_HIP(hipMalloc(&p->sd_nodes, sizeof(int) * p->S));
+    SS_HIP(hipMemcpyAsync(p->sd_nodes, sd.data(), sizeof(int) * p->S, hipMemcpyHostToDevice, st));
+    SS_HIP(hipMalloc(&p->Bt, sizeof(uint16_t) * 2 * p->S16 * Mp));
+    SS_HIP(hipMemsetAsync(p->Bt, 0, sizeof(uint16_t) * 2 * p->S16 * Mp, st));   // padding columns stay 0
+    SS_HIP(hipMalloc(&p->Cn, sizeof(float) * 64 * p->S));
+    // opponent hand holding the same two cards as each traverser hand
+    std::vector<int> same(Mp);
+    for (int t = 0; t < 2; ++t) {
+        const int o = 1 - t;
+        std::vector<int> at(SS_NCOMBO, -1);
+        for (int j = 0; j < pl.nh[o]; ++j) {
+            const uint32_t h = pl.hinfo[(size_t)o * Mp + j];
+            if (h & SS_LIVE_BIT) at[(h >> 12) & 2047] = j;
+        }
+        for (int i = 0; i < Mp; ++i) {
+            const uint32_t h = i < pl.nh[t] ? pl.hinfo[(size_t)t * Mp + i] : 0u;
+            same[i] = (h & SS_LIVE_BIT) ? at[(h >> 12) & 2047] : -1;
+        }
+        SS_HIP(hipMalloc(&p->same[t], sizeof(int) * Mp));
+        SS_HIP(hipMemcpyAsync(p->same[t], same.data(), sizeof(int) * Mp, hipMemcpyHostToDevice, st));
+        SS_HIP(hipStreamSynchronize(st));   // `same` is reused
+        SS_HIP(hipMalloc(&p->W[t], sizeof(uint16_t) * (size_t)Mp * Mp));
+        hipLaunchKernelGGL(k_prec_w, dim3(std::min<int64_t>(4096, ((int64_t)Mp * Mp + SS_BLOCK - 1) / SS_BLOCK)),
+                           dim3(SS_BLOCK), 0, st, s.d_desc, t, p->W[t]);
+    }
+    SS_HIP(hipGetLastError());
+    SS_HIP(hipStreamSynchronize(st));
+    s.prec = std::move(p);
+    return SS_OK;
+}
+
+// the showdown nodes of traverser t's pass (after its forward sweep)
+int prec_showdowns(Slot &s, int t, hipStream_t st) {
+    PrecState &p = *s.prec;
+    PrecArgs a;
+    a.S = p.S;
+    a.S16 = p.S16;
+    a.NT = p.S16 / 16;
+    a.sd_nodes = p.sd_nodes;
+    a.W = p.W[t];
+    a.Bt = p.Bt;
+    a.Cn = p.Cn;
+    a.same = p.same[t];
+    hipLaunchKernelGGL(k_prec_split, dim3(p.S), dim3(SS_BLOCK), 0, st, s.d_desc, a, t);
+    hipLaunchKernelGGL(k_prec_gemm, dim3(s.plan.Mp / 16), dim3(64), 0, st, s.d_desc, a, t);
+    SS_HIP(hipGetLastError());
+    return SS_OK;
+}
+
+int64_t prec_flops_per_pass(const Slot &s) {
+    if (!s.prec) return 0;
+    return 2LL * s.plan.Mp * s.plan.Mp * 2 * s.prec->S16;
+}
+
+}  // namespace ss

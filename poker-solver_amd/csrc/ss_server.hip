@@ -428,6 +428,8 @@ int64_t ss_server_submit(SSServer *s, const SSProblem *problem, const SSSolveCon
     int rc = r->prob.copy_from(problem);
     if (rc) return rc;
     r->cfg = *config;
+    if (r->cfg.precision != SS_PREC_FP32)
+        return set_error(SS_ERR_UNSUPPORTED, "the served form runs FP32 (precision form: ss_solve_gpu_ex)");
     if (r->cfg.check_every <= 0) r->cfg.check_every = s->cfg.check_every;
     r->footprint = estimate_footprint(r->prob.hp);
     if (r->footprint < 0) return set_error(SS_ERR_ARG, "%s", last_error_str());

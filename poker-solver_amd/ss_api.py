@@ -144,11 +144,11 @@ def config(max_iterations, target=0.0, check_every=0, precision=ss_abi.SS_PREC_F
     return c
 
 
-def solve_plain(problem, iterations, target=0.0, check_every=0, flags=0):
+def solve_plain(problem, iterations, target=0.0, check_every=0, flags=0, precision=ss_abi.SS_PREC_FP32):
     lib = ss_abi.load_library()
     out, rep = SSOutput(), SSReport()
     st = problem.struct()
-    cfg = config(iterations, target, check_every, flags=flags)
+    cfg = config(iterations, target, check_every, precision=precision, flags=flags)
     _check(lib.ss_solve_gpu_ex(ctypes.byref(st), ctypes.byref(cfg), ctypes.byref(out), ctypes.byref(rep)),
            "ss_solve_gpu_ex")
     return Result(out, rep, problem)
