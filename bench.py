@@ -181,6 +181,85 @@ def served_bench(n, rate, seed, target, check_every, max_iters, iters_per_step, 
     }
 
 
+# ---------------------------------------------------------------- precision form
+def _fp64_reference(args):
+    """FP64 oracle solve of one config-1-shaped board: average strategy and
+    root values after `iters` iterations."""
+    seed, iters = args
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import oracle_bind as ob
+    import workloads as wl
+    spot = wl.river_2p(seed=seed)
+    td = ob.build_tree(spot.board, spot.pot, spot.stack, spot.bet_sizes, spot.num_players,
+                       spot.acting_order, max_raises=spot.max_raises)
+    prob = ob.Problem(td, spot.board, spot.hands, spot.weights)
+    r = ob.solve(prob, iters, fp64=True, mode=ob.TERM_FAST)
+    return np.asarray(r["avg"]), np.asarray(r["root_ev"])
+
+
+def precision_bench(boards, iters, ref_boards, cores):
+    """Config 3: `boards` 2-player river subgames (config-1 shape, seeds
+    base + b), `iters` iterations each in the precision form (BF16 hi/lo MFMA
+    showdown, FP32 accumulation). Errors of the average strategy and of the
+    root values against the FP64 oracle on the first `ref_boards` boards, with
+    the FP32 closed form's errors on the same boards beside them."""
+    import ss_abi
+    import ss_api
+    import workloads as wl
+    seeds = [20261203 + 1000 + b for b in range(boards)]
+    probs = [ss_api.Problem.from_spot(wl.river_2p(seed=sd)) for sd in seeds]
+    batch = ss_api.Batch(probs, iters, precision=ss_abi.SS_PREC_BF16X2)
+    batch.run()   # warm-up
+    t0 = time.perf_counter()
+    reps, st = batch.run()
+    wall = time.perf_counter() - t0
+    res = [batch.output(i, reps[i]) for i in range(boards)]
+    batch.close()
+    t1 = time.perf_counter()
+    plain = ss_api.solve_plain(probs[0], iters, precision=ss_abi.SS_PREC_BF16X2)
+    plain_s = time.perf_counter() - t1
+    # MFMA work: per traverser pass 2 Mp^2 (2 S16) flops (hi and lo columns)
+    S = [sum(1 for n in range(p.tree.num_nodes) if p.tree.td.nodes[n].type == 2) for p in probs]
+    mpad = [(len(p.hands[0]) + 63) // 64 * 64 for p in probs]
+    flops = sum(2.0 * m * m * 2 * ((s + 15) // 16 * 16) * 2 * iters for m, s in zip(mpad, S))
+    out = {
+        "workload": f"{boards} 2-player rivers (1081 combos each, pot 100 BB, stacks 400 BB, sizes {{0.5,1}}, "
+                    f"max 2 raises), {iters} iterations each, one batched (phased) solve with the BF16 hi/lo "
+                    f"MFMA showdown",
+        "boards": boards, "iterations": iters,
+        "solves_per_s": round(boards / wall, 3),
+        "plain_form_solves_per_s": round(1.0 / plain_s, 3),
+        "device_ms": {"cfr": round(st.cfr_ms, 2), "checks": round(st.check_ms, 2)},
+        "mfma_tflop": round(flops / 1e12, 3),
+        "mfma_tflops_over_cfr_time": round(flops / (st.cfr_ms / 1000.0) / 1e12, 3),
+        "exploitability_pct_pot": {"mean": float(np.mean([r.exploitability for r in reps])),
+                                   "max": float(np.max([r.exploitability for r in reps]))},
+    }
+    if ref_boards > 0:
+        nref = min(ref_boards, boards)
+        with mp.get_context("spawn").Pool(min(cores, nref)) as pool:
+            refs = pool.map(_fp64_reference, [(sd, iters) for sd in seeds[:nref]])
+        f32 = [ss_api.solve_plain(p, iters) for p in probs[:nref]]
+
+        def errs(rs):
+            sa, sm, ra, rm = [], [], [], []
+            for r, (avg, rev), p in zip(rs, refs, probs):
+                g = r.avg_full(p.tree.num_nodes).astype(np.float64)
+                live = np.zeros_like(avg, dtype=bool)
+                live[r.node_ids] = True
+                d = np.abs(g - avg)[live]
+                sa.append(float(d.max()))
+                sm.append(float(d.mean()))
+                pot = float(p.tree.td.starting_pot)
+                e = np.abs(r.root_ev[:len(rev)].astype(np.float64) - rev) / pot   # node-0 cfv, last traverser
+                ra.append(float(e.max()))
+                rm.append(float(e.mean()))
+            return {"avg_strategy_abs": {"max": max(sa), "mean": float(np.mean(sm))},
+                    "root_value_abs_per_pot": {"max": max(ra), "mean": float(np.mean(rm))}}
+        out["error_vs_fp64"] = {"boards": nref, "bf16x2": errs(res[:nref]), "fp32_closed_form": errs(f32)}
+    return out
+
+
 # ---------------------------------------------------------------- main
 def main():
     ap = argparse.ArgumentParser()
@@ -201,6 +280,9 @@ def main():
     ap.add_argument("--served-requests", type=int, default=200, help="0 skips the served-form run")
     ap.add_argument("--served-rate", type=float, default=40.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--served-iters-per-step", type=int, default=5)
+    ap.add_argument("--precision-boards", type=int, default=64, help="0 skips the precision-form run")
+    ap.add_argument("--precision-iters", type=int, default=1000)
+    ap.add_argument("--precision-ref-boards", type=int, default=8, help="boards compared with the FP64 oracle")
     a = ap.parse_args()
 
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -305,6 +387,12 @@ def main():
                                       a.check_every, a.max_iters, a.served_iters_per_step)
         log(f"[bench] served: {line['served']['solves_per_s']} solves/s, "
             f"p50 {line['served']['latency_ms']['p50']} ms, p99 {line['served']['latency_ms']['p99']} ms")
+
+    if a.precision_boards > 0:
+        log(f"[bench] precision form: {a.precision_boards} boards x {a.precision_iters} iterations")
+        line["precision_form"] = precision_bench(a.precision_boards, a.precision_iters, a.precision_ref_boards,
+                                                 a.cpu_cores or min(16, os.cpu_count() or 1))
+        log(f"[bench] precision: {line['precision_form']['solves_per_s']} solves/s")
 
     if not a.no_cpu:
         cores = a.cpu_cores or min(16, os.cpu_count() or 1)

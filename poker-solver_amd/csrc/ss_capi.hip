@@ -121,9 +121,10 @@ struct BatchImpl {
     }
 
     int create(const HostProblem *hps, int count, const SSSolveConfig &c) {
-        if (c.precision != SS_PREC_FP32)
-            return set_error(SS_ERR_UNSUPPORTED, "the batched form runs FP32 (precision form: ss_solve_gpu_ex)");
+        if (c.precision != SS_PREC_FP32 && c.precision != SS_PREC_BF16X2)
+            return set_error(SS_ERR_ARG, "unknown precision %d", c.precision);
         cfg = c;
+        const bool prec = c.precision == SS_PREC_BF16X2;
         n = count;
         const int T = std::max(0, cfg.max_iterations);
         target = cfg.target_exploitability > 0;
@@ -135,6 +136,7 @@ struct BatchImpl {
         for (int i = 0; i < n; ++i) {
             slots[i].reset(new Slot());
             int rc = slots[i]->setup(hps[i], st);
+            if (!rc && prec) rc = prec_setup(*slots[i], st);
             if (rc) return set_error(rc, "subgame %d: %s", i, last_error_str());
             if (target || want_final) {
                 rc = slots[i]->alloc_expl(st, 1);
@@ -225,6 +227,7 @@ struct BatchImpl {
         const char *env = getenv("SS_BATCH_MODE");
         if (env && !strcmp(env, "persistent")) { mode = 0; return SS_OK; }
         if (env && !strcmp(env, "phased")) { mode = 1; return SS_OK; }
+        if (cfg.precision == SS_PREC_BF16X2) { mode = 1; return SS_OK; }   // the contraction is phased only
         if (cfg.max_iterations < 3) { mode = 1; return SS_OK; }
         float best = 1e30f;
         for (int m = 0; m < 2; ++m) {

@@ -86,6 +86,13 @@ struct SlotDesc {
     const uint8_t *bnd;
     const uint8_t *cpos;
     const uint4 *hmeta;        // [NP][Mp] river only
+    // precision form (2-player rivers under SS_PREC_BF16X2), zero otherwise
+    int prec_S, prec_S16;      // showdown nodes, rounded up to 16
+    const int *prec_sd;        // [prec_S] showdown node ids
+    const uint16_t *prec_W[2]; // per traverser [Mp][Mp] bf16 (+1 / -1 / 0)
+    uint16_t *prec_Bt;         // [2 prec_S16][Mp] bf16 hi / lo opponent reach
+    float *prec_Cn;            // [prec_S][64] per-card opponent totals
+    const int *prec_same[2];   // per traverser [Mp] opponent hand with the same cards, or -1
     // exploitability scratch (FP64), allocated on demand
     float *avg;                // [A][Mp]
     double *xreach;            // [NP][N][Mp]

@@ -126,12 +126,19 @@ struct PhasedTasks {
     int nodo = 0;                  // terminal tasks [0, nodo): three or more opponents for some traverser
     int npair = 0;                 // [nodo, nodo + npair): two opponents (showdowns first); the rest at most one
     int npair_sd = 0;              // showdowns among the npair
+    int nprec = 0;                 // the last nprec terminal tasks: showdowns of precision-form slots
+                                   // (FP64 checks only; the CFR pass takes them from the contraction)
+    int psplit_off = 0, npsplit = 0;   // precision form: (slot, showdown index) tasks
+    std::vector<std::pair<int, int>> pgemm;   // (column tiles NT, task count) groups of (slot, row tile) tasks
+    int pgemm_off = 0;
     uint32_t *d_tasks = nullptr;
     ~PhasedTasks();
 };
 // tasks of the non-null slots with live[slot] != 0 (all of them when live is null)
 int prec_setup(Slot &s, hipStream_t st);
 int prec_showdowns(Slot &s, int t, hipStream_t st);
+struct PhasedTasks;
+int launch_prec_phased(SlotDesc *d_slots, const PhasedTasks &pt, int t, hipStream_t st);
 int64_t prec_flops_per_pass(const Slot &s);
 
 int build_phased_tasks(const std::vector<Slot *> &slots, PhasedTasks &pt, hipStream_t st,

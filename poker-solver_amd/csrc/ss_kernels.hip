@@ -1426,8 +1426,9 @@ PhasedTasks::~PhasedTasks() {
 
 // terminal cost class for scheduling: closed forms with 2 opponents at a
 // showdown are the heaviest (a 52-card loop per hand), odometers heavier still
-static int term_cost(const Plan &pl, int n) {
+static int term_cost(const Plan &pl, int n, bool prec) {
     const DevNode &nd = pl.nodes[n];
+    if (prec && nd.type == SS_NODE_SHOWDOWN) return 1;   // precision form: the contraction, FP64 checks only
     int kmax = 0;
     for (int t = 0; t < pl.NP; ++t) {
         int k = 0;
@@ -1492,26 +1493,52 @@ int build_phased_tasks(const std::vector<Slot *> &slots, PhasedTasks &pt, hipStr
     for (int si = 0; si < pt.nslots; ++si) {
         if (!on(si)) continue;
         const Plan &pl = slots[si]->plan;
-        for (int j = 0; j < pl.T; ++j) term.push_back({term_cost(pl, pl.term[j]), ((uint32_t)si << 20) | (uint32_t)pl.term[j]});
+        const bool prec = slots[si]->prec != nullptr;
+        for (int j = 0; j < pl.T; ++j)
+            term.push_back({term_cost(pl, pl.term[j], prec), ((uint32_t)si << 20) | (uint32_t)pl.term[j]});
     }
     std::stable_sort(term.begin(), term.end(), [](const std::pair<int, uint32_t> &a, const std::pair<int, uint32_t> &b) {
         return a.first > b.first;
     });
     pt.term_off = (int)tasks.size();
     pt.nterm = (int)term.size();
-    pt.nodo = pt.npair_sd = pt.npair = 0;
+    pt.nodo = pt.npair_sd = pt.npair = pt.nprec = 0;
     for (auto &t : term) {
         tasks.push_back(t.second);
         if (t.first >= 1000) ++pt.nodo;
         else if (t.first >= 20) ++pt.npair;
         if (t.first == 100) ++pt.npair_sd;
+        if (t.first == 1) ++pt.nprec;
     }
     // each kernel's segment in XCD order (equal cost within a segment)
     const size_t t0 = pt.term_off, t1 = t0 + pt.nodo, t2 = t1 + pt.npair_sd, t3 = t0 + pt.nodo + pt.npair;
     xcd_interleave(tasks, t0, t1);
     xcd_interleave(tasks, t1, t2);
+    const size_t t4 = tasks.size() - pt.nprec;
     xcd_interleave(tasks, t2, t3);
-    xcd_interleave(tasks, t3, tasks.size());
+    xcd_interleave(tasks, t3, t4);
+    xcd_interleave(tasks, t4, tasks.size());
+    // precision form: split tasks (slot, showdown index), then GEMM tasks
+    // (slot, 16-row tile) grouped by column-tile count
+    pt.psplit_off = (int)tasks.size();
+    for (int si = 0; si < pt.nslots; ++si)
+        if (on(si) && slots[si]->prec)
+            for (int b = 0; b < slots[si]->prec->S; ++b) tasks.push_back(((uint32_t)si << 20) | (uint32_t)b);
+    pt.npsplit = (int)tasks.size() - pt.psplit_off;
+    xcd_interleave(tasks, pt.psplit_off, tasks.size());
+    pt.pgemm_off = (int)tasks.size();
+    pt.pgemm.clear();
+    for (int nt = 1; nt <= 8; ++nt) {
+        const size_t g0 = tasks.size();
+        for (int si = 0; si < pt.nslots; ++si)
+            if (on(si) && slots[si]->prec && slots[si]->prec->S16 / 16 == nt)
+                for (int r = 0; r < (slots[si]->plan.Mp + 127) / 128; ++r)   // 128-row blocks (k_prec_gemm_phased)
+                    tasks.push_back(((uint32_t)si << 20) | (uint32_t)r);
+        if (tasks.size() > g0) {
+            xcd_interleave(tasks, g0, tasks.size());
+            pt.pgemm.push_back({nt, (int)(tasks.size() - g0)});
+        }
+    }
     if (pt.d_tasks) { SS_HIP(hipFree(pt.d_tasks)); pt.d_tasks = nullptr; }
     SS_HIP(hipMalloc(&pt.d_tasks, sizeof(uint32_t) * std::max<size_t>(1, tasks.size())));
     SS_HIP(hipMemcpyAsync(pt.d_tasks, tasks.data(), sizeof(uint32_t) * tasks.size(), hipMemcpyHostToDevice, st));
@@ -1528,7 +1555,11 @@ int launch_phased_iteration(SlotDesc *d_slots, const PhasedTasks &pt, hipStream_
                 hipLaunchKernelGGL(k_fwd_phased, dim3(cnt), dim3(SS_BLOCK), 0, st, d_slots, pt.d_tasks + pt.lvl_off[l],
                                    cnt, t);
         }
-        const int nheavy = pt.nodo + pt.npair;
+        if (pt.npsplit > 0) {
+            int rc = launch_prec_phased(d_slots, pt, t, st);
+            if (rc) return rc;
+        }
+        const int nheavy = pt.nodo + pt.npair, nlight = pt.nterm - nheavy - pt.nprec;
         if (pt.nodo > 0)
             hipLaunchKernelGGL(k_term_odo, dim3(pt.nodo), dim3(SS_BLOCK), 0, st, d_slots, pt.d_tasks + pt.term_off,
                                pt.nodo, t);
@@ -1538,9 +1569,9 @@ int launch_phased_iteration(SlotDesc *d_slots, const PhasedTasks &pt, hipStream_
         if (pt.npair > pt.npair_sd)
             hipLaunchKernelGGL(k_term_pair<0>, dim3(pt.npair - pt.npair_sd), dim3(SS_BLOCK), 0, st, d_slots,
                                pt.d_tasks + pt.term_off + pt.nodo + pt.npair_sd, pt.npair - pt.npair_sd, t);
-        if (pt.nterm > nheavy)
-            hipLaunchKernelGGL(k_term_light, dim3(pt.nterm - nheavy), dim3(SS_BLOCK), 0, st, d_slots,
-                               pt.d_tasks + pt.term_off + nheavy, pt.nterm - nheavy, t);
+        if (nlight > 0)
+            hipLaunchKernelGGL(k_term_light, dim3(nlight), dim3(SS_BLOCK), 0, st, d_slots,
+                               pt.d_tasks + pt.term_off + nheavy, nlight, t);
         for (int l = pt.L - 1; l >= 0; --l) {
             int cnt = pt.lvl_off[l + 1] - pt.lvl_off[l];
             if (cnt > 0)

@@ -23,6 +23,7 @@ typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
 
 #define PREC_MAX_TILES 8   // S16 / 16 <= 8: at most 128 showdown nodes
 #define DN2_P (52 * 52)
+#define PREC_WAVES 4       // wavefronts per GEMM workgroup
 
 struct PrecArgs {
     int S, S16, NT;           // showdown nodes, S rounded up to 16, S16 / 16
@@ -61,12 +62,25 @@ __global__ void __launch_bounds__(SS_BLOCK) k_prec_w(const SlotDesc *__restrict_
     }
 }
 
-// One workgroup per showdown node: the opponent's reach as bf16 hi / lo rows
-// of Bt, and its per-card totals (dense scatter + wave64 sums, fixed order).
-__global__ void __launch_bounds__(SS_BLOCK) k_prec_split(const SlotDesc *__restrict__ s, PrecArgs a, int t) {
-    __shared__ float M[DN2_P];
-    __shared__ float C[64];
-    const int b = blockIdx.x, n = a.sd_nodes[b], o = 1 - t, Mp = s->Mp;
+__device__ __forceinline__ PrecArgs prec_args(const SlotDesc *__restrict__ s, int t) {
+    PrecArgs a;
+    a.S = s->prec_S;
+    a.S16 = s->prec_S16;
+    a.NT = s->prec_S16 / 16;
+    a.sd_nodes = s->prec_sd;
+    a.W = s->prec_W[t];
+    a.Bt = s->prec_Bt;
+    a.Cn = s->prec_Cn;
+    a.same = s->prec_same[t];
+    return a;
+}
+
+// Showdown node b of traverser t's pass: the opponent's reach as bf16 hi / lo
+// rows of Bt, and its per-card totals (dense scatter + wave64 sums, fixed
+// order). One workgroup; M and C are LDS.
+__device__ void prec_split_node(const SlotDesc *__restrict__ s, const PrecArgs &a, int t, int b, float *M,
+                                float *C) {
+    const int n = a.sd_nodes[b], o = 1 - t, Mp = s->Mp;
     const float *r = s->reach + ((size_t)o * s->N + n) * Mp;
     for (int j = threadIdx.x; j < Mp; j += blockDim.x) {
         const bool ok = j < s->nh[o] && (s->hinfo[(size_t)o * Mp + j] & SS_LIVE_BIT);
@@ -99,36 +113,78 @@ __global__ void __launch_bounds__(SS_BLOCK) k_prec_split(const SlotDesc *__restr
         for (int c = 0; c < 52; ++c) tot += C[c];
         a.Cn[(size_t)b * 64 + 52] = 0.5f * tot;
     }
+    __syncthreads();
 }
 
-// One wavefront per 16 traverser hands: Y = W * [R_hi | R_lo] over all
-// showdown columns, then the showdown values written into cfv.
+__global__ void __launch_bounds__(SS_BLOCK) k_prec_split(const SlotDesc *__restrict__ s, int t) {
+    __shared__ float M[DN2_P];
+    __shared__ float C[64];
+    prec_split_node(s, prec_args(s, t), t, blockIdx.x, M, C);
+}
+
+// phased (batched) forms: tasks are slot << 20 | showdown index / row block
+__global__ void __launch_bounds__(SS_BLOCK) k_prec_split_phased(SlotDesc *__restrict__ slots, const uint32_t *tasks,
+                                                                int ntasks, int t) {
+    __shared__ float M[DN2_P];
+    __shared__ float C[64];
+    for (int j = blockIdx.x; j < ntasks; j += gridDim.x) {
+        const uint32_t tk = tasks[j];
+        const SlotDesc *s = slots + (tk >> 20);
+        if (!(s->active && s->iter < s->iter_end && t < s->NP)) continue;
+        prec_split_node(s, prec_args(s, t), t, (int)(tk & 0xFFFFF), M, C);
+    }
+}
+
+// Y = W * [R_hi | R_lo] for PREC_RT row tiles of 16 traverser hands and all
+// column tiles, then the showdown values written into cfv. Each wavefront
+// reuses every B fragment across its row tiles and loads the next K step's
+// fragments before this step's MFMAs; the 4 wavefronts of a workgroup take
+// adjacent row blocks and read the same B fragments at the same K step
+// (served from L1). K runs in order, so every form rounds alike.
 // MFMA 16x16x32 lane layout: lane l holds A[l % 16][8 (l / 16) + 0..7],
 // B[8 (l / 16) + 0..7][l % 16], D[4 (l / 16) + 0..3][l % 16].
-__global__ void __launch_bounds__(64) k_prec_gemm(const SlotDesc *__restrict__ s, PrecArgs a, int t) {
-    const int l = threadIdx.x, Mp = s->Mp, o = 1 - t;
-    const int row0 = blockIdx.x * 16;
-    floatx4 acc[2 * PREC_MAX_TILES];
+#define PREC_RT 2
+template <int NT>
+__device__ void prec_gemm_rows(const SlotDesc *__restrict__ s, const PrecArgs &a, int t, int row0) {
+    const int l = threadIdx.x & 63, Mp = s->Mp, o = 1 - t;
+    if (row0 >= Mp) return;
+    floatx4 acc[PREC_RT][2 * NT];
 #pragma unroll
-    for (int q = 0; q < 2 * PREC_MAX_TILES; ++q) acc[q] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int r = 0; r < PREC_RT; ++r)
+#pragma unroll
+        for (int q = 0; q < 2 * NT; ++q) acc[r][q] = floatx4{0.f, 0.f, 0.f, 0.f};
+    const int nrt = min(PREC_RT, (Mp - row0) / 16);
     const uint16_t *wrow = a.W + (size_t)(row0 + (l & 15)) * Mp + 8 * (l >> 4);
     const uint16_t *bcol = a.Bt + (size_t)(l & 15) * Mp + 8 * (l >> 4);
-    for (int k0 = 0; k0 < Mp; k0 += 32) {
-        const bf16x8 va = __builtin_bit_cast(bf16x8, *(const uint4 *)(wrow + k0));
+    uint4 fa[PREC_RT], fb[2 * NT];
+    auto load = [&](int k0) {
 #pragma unroll
-        for (int q = 0; q < PREC_MAX_TILES; ++q) {
-            if (q >= a.NT) break;
-            const bf16x8 vh = __builtin_bit_cast(bf16x8, *(const uint4 *)(bcol + (size_t)(16 * q) * Mp + k0));
-            const bf16x8 vl =
-                __builtin_bit_cast(bf16x8, *(const uint4 *)(bcol + (size_t)(a.S16 + 16 * q) * Mp + k0));
-            acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va, vh, acc[q], 0, 0, 0);
-            acc[PREC_MAX_TILES + q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va, vl, acc[PREC_MAX_TILES + q], 0, 0, 0);
+        for (int r = 0; r < PREC_RT; ++r)
+            fa[r] = r < nrt ? *(const uint4 *)(wrow + (size_t)(16 * r) * Mp + k0) : make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+        for (int q = 0; q < NT; ++q) {
+            fb[q] = *(const uint4 *)(bcol + (size_t)(16 * q) * Mp + k0);
+            fb[NT + q] = *(const uint4 *)(bcol + (size_t)(a.S16 + 16 * q) * Mp + k0);
         }
+    };
+    const int ksteps = Mp / 32;
+    load(0);
+    for (int ks = 0; ks < ksteps; ++ks) {
+        bf16x8 va[PREC_RT], vb[2 * NT];
+#pragma unroll
+        for (int r = 0; r < PREC_RT; ++r) va[r] = __builtin_bit_cast(bf16x8, fa[r]);
+#pragma unroll
+        for (int q = 0; q < 2 * NT; ++q) vb[q] = __builtin_bit_cast(bf16x8, fb[q]);
+        if (ks + 1 < ksteps) load(32 * (ks + 1));
+#pragma unroll
+        for (int r = 0; r < PREC_RT; ++r)
+#pragma unroll
+            for (int q = 0; q < 2 * NT; ++q)
+                acc[r][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va[r], vb[q], acc[r][q], 0, 0, 0);
     }
     const float *ro = s->reach + (size_t)o * s->N * Mp;
 #pragma unroll
-    for (int q = 0; q < PREC_MAX_TILES; ++q) {
-        if (q >= a.NT) break;
+    for (int q = 0; q < NT; ++q) {
         const int b = 16 * q + (l & 15);
         if (b >= a.S) continue;
         const int n = a.sd_nodes[b];
@@ -137,19 +193,69 @@ __global__ void __launch_bounds__(64) k_prec_gemm(const SlotDesc *__restrict__ s
         const float *Cb = a.Cn + (size_t)b * 64;
         const float Sb = Cb[52];
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int i = row0 + 4 * (l >> 4) + e;
-            if (i >= s->nh[t]) continue;
-            const uint32_t hi = s->hinfo[(size_t)t * Mp + i];
-            float v = 0.f;
-            if ((hi & SS_LIVE_BIT) && s->cstr[(hi >> 12) & 2047]) {                 // hs = 0 -> 0, :861
-                const int x = hi & 63, y = (hi >> 6) & 63, sm = a.same[i];
-                const float z = Sb - Cb[x] - Cb[y] + (sm >= 0 ? ro[(size_t)n * Mp + sm] : 0.f);
-                const float Y = acc[q][e] + acc[PREC_MAX_TILES + q][e];
-                v = z < 1e-5f * Sb ? 0.f : half * Y + (half - bet) * z;
+        for (int r = 0; r < PREC_RT; ++r) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int i = row0 + 16 * r + 4 * (l >> 4) + e;
+                if (r >= nrt || i >= s->nh[t]) continue;
+                const uint32_t hi = s->hinfo[(size_t)t * Mp + i];
+                float v = 0.f;
+                if ((hi & SS_LIVE_BIT) && s->cstr[(hi >> 12) & 2047]) {              // hs = 0 -> 0, :861
+                    const int x = hi & 63, y = (hi >> 6) & 63, sm = a.same[i];
+                    const float z = Sb - Cb[x] - Cb[y] + (sm >= 0 ? ro[(size_t)n * Mp + sm] : 0.f);
+                    const float Y = acc[r][q][e] + acc[r][NT + q][e];
+                    v = z < 1e-5f * Sb ? 0.f : half * Y + (half - bet) * z;
+                }
+                s->cfv[(size_t)n * Mp + i] = v;
             }
-            s->cfv[(size_t)n * Mp + i] = v;
         }
+    }
+}
+
+// plain form: workgroup b covers rows [128 b, 128 b + 128)
+template <int NT>
+__global__ void __launch_bounds__(64 * PREC_WAVES) k_prec_gemm(const SlotDesc *__restrict__ s, int t) {
+    const int row0 = (64 * PREC_RT) * blockIdx.x + 16 * PREC_RT * (threadIdx.x >> 6);
+    prec_gemm_rows<NT>(s, prec_args(s, t), t, row0);
+}
+
+template <int NT>
+__global__ void __launch_bounds__(64 * PREC_WAVES) k_prec_gemm_phased(SlotDesc *__restrict__ slots,
+                                                                      const uint32_t *tasks, int ntasks, int t) {
+    for (int j = blockIdx.x; j < ntasks; j += gridDim.x) {
+        const uint32_t tk = tasks[j];
+        const SlotDesc *s = slots + (tk >> 20);
+        if (!(s->active && s->iter < s->iter_end && t < s->NP)) continue;
+        const int row0 = (64 * PREC_RT) * (int)(tk & 0xFFFFF) + 16 * PREC_RT * (threadIdx.x >> 6);
+        prec_gemm_rows<NT>(s, prec_args(s, t), t, row0);
+    }
+}
+
+typedef void (*PrecGemmKernel)(const SlotDesc *, int);
+static PrecGemmKernel prec_gemm_kernel(int nt) {
+    switch (nt) {
+    case 1: return k_prec_gemm<1>;
+    case 2: return k_prec_gemm<2>;
+    case 3: return k_prec_gemm<3>;
+    case 4: return k_prec_gemm<4>;
+    case 5: return k_prec_gemm<5>;
+    case 6: return k_prec_gemm<6>;
+    case 7: return k_prec_gemm<7>;
+    default: return k_prec_gemm<8>;
+    }
+}
+
+typedef void (*PrecGemmPhased)(SlotDesc *, const uint32_t *, int, int);
+static PrecGemmPhased prec_gemm_phased_kernel(int nt) {
+    switch (nt) {
+    case 1: return k_prec_gemm_phased<1>;
+    case 2: return k_prec_gemm_phased<2>;
+    case 3: return k_prec_gemm_phased<3>;
+    case 4: return k_prec_gemm_phased<4>;
+    case 5: return k_prec_gemm_phased<5>;
+    case 6: return k_prec_gemm_phased<6>;
+    case 7: return k_prec_gemm_phased<7>;
+    default: return k_prec_gemm_phased<8>;
     }
 }
 
@@ -200,6 +306,17 @@ int prec_setup(Slot &s, hipStream_t st) {
                            dim3(SS_BLOCK), 0, st, s.d_desc, t, p->W[t]);
     }
     SS_HIP(hipGetLastError());
+    SlotDesc &d = s.desc;
+    d.prec_S = p->S;
+    d.prec_S16 = p->S16;
+    d.prec_sd = p->sd_nodes;
+    d.prec_W[0] = p->W[0];
+    d.prec_W[1] = p->W[1];
+    d.prec_Bt = p->Bt;
+    d.prec_Cn = p->Cn;
+    d.prec_same[0] = p->same[0];
+    d.prec_same[1] = p->same[1];
+    SS_HIP(hipMemcpyAsync(s.d_desc, &d, sizeof(d), hipMemcpyHostToDevice, st));
     SS_HIP(hipStreamSynchronize(st));
     s.prec = std::move(p);
     return SS_OK;
@@ -207,18 +324,24 @@ int prec_setup(Slot &s, hipStream_t st) {
 
 // the showdown nodes of traverser t's pass (after its forward sweep)
 int prec_showdowns(Slot &s, int t, hipStream_t st) {
-    PrecState &p = *s.prec;
-    PrecArgs a;
-    a.S = p.S;
-    a.S16 = p.S16;
-    a.NT = p.S16 / 16;
-    a.sd_nodes = p.sd_nodes;
-    a.W = p.W[t];
-    a.Bt = p.Bt;
-    a.Cn = p.Cn;
-    a.same = p.same[t];
-    hipLaunchKernelGGL(k_prec_split, dim3(p.S), dim3(SS_BLOCK), 0, st, s.d_desc, a, t);
-    hipLaunchKernelGGL(k_prec_gemm, dim3(s.plan.Mp / 16), dim3(64), 0, st, s.d_desc, a, t);
+    const PrecState &p = *s.prec;
+    hipLaunchKernelGGL(k_prec_split, dim3(p.S), dim3(SS_BLOCK), 0, st, s.d_desc, t);
+    hipLaunchKernelGGL(prec_gemm_kernel(p.S16 / 16), dim3((s.plan.Mp + 64 * PREC_RT - 1) / (64 * PREC_RT)),
+                       dim3(64 * PREC_WAVES), 0, st, s.d_desc, t);
+    SS_HIP(hipGetLastError());
+    return SS_OK;
+}
+
+int launch_prec_phased(SlotDesc *d_slots, const PhasedTasks &pt, int t, hipStream_t st) {
+    if (pt.npsplit == 0) return SS_OK;
+    hipLaunchKernelGGL(k_prec_split_phased, dim3(pt.npsplit), dim3(SS_BLOCK), 0, st, d_slots,
+                       pt.d_tasks + pt.psplit_off, pt.npsplit, t);
+    int off = pt.pgemm_off;
+    for (const auto &g : pt.pgemm) {
+        hipLaunchKernelGGL(prec_gemm_phased_kernel(g.first), dim3(g.second), dim3(64 * PREC_WAVES), 0, st, d_slots,
+                           pt.d_tasks + off, g.second, t);
+        off += g.second;
+    }
     SS_HIP(hipGetLastError());
     return SS_OK;
 }

@@ -169,12 +169,12 @@ class Batch:
     """Batched form as a session: upload once (ss_batch_create), then solve all
     subgames from scratch on every run() with inputs resident in HBM."""
 
-    def __init__(self, problems, iterations, target=0.0, check_every=0, flags=0):
+    def __init__(self, problems, iterations, target=0.0, check_every=0, flags=0, precision=ss_abi.SS_PREC_FP32):
         self.lib = ss_abi.load_library()
         self.problems = list(problems)
         self.n = len(self.problems)
         self._structs = (SSProblem * self.n)(*[p.struct() for p in self.problems])
-        cfg = config(iterations, target, check_every, flags=flags)
+        cfg = config(iterations, target, check_every, precision=precision, flags=flags)
         self.h = self.lib.ss_batch_create(self._structs, self.n, ctypes.byref(cfg))
         if not self.h:
             raise SolveError(f"ss_batch_create failed: {ss_abi.last_error()}")

@@ -58,9 +58,38 @@ def test_precision_exploitability_cfg1_vs_fp64_oracle():
     assert abs(res.exploitability - e_ref) <= 0.01 * e_ref, (res.exploitability, e_ref)
 
 
+def test_precision_batched_equals_plain():
+    """The batched (phased) precision form gives the plain precision form's
+    results bit for bit, for every board of a batch."""
+    probs = [ss_api.Problem.from_spot(wl.river_2p(seed=20261203 + 1000 + b)) for b in range(3)]
+    probs.append(ss_api.Problem.from_spot(wl.small_spot(12, num_players=2, hands_per_player=20)))
+    b = ss_api.Batch(probs, 15, flags=1, precision=ss_abi.SS_PREC_BF16X2)
+    b.run()
+    for i, p in enumerate(probs):
+        got = b.output(i)
+        want = ss_api.solve_plain(p, 15, flags=1, precision=ss_abi.SS_PREC_BF16X2)
+        assert np.array_equal(got.avg, want.avg), i
+        assert np.array_equal(got.root_strategy, want.root_strategy), i
+    b.close()
+
+
+def test_precision_batched_exploitability_checks():
+    """FP64 checks of a precision-form batch see every showdown: the reported
+    exploitability equals the plain precision form's."""
+    probs = [ss_api.Problem.from_spot(wl.river_2p(seed=20261203 + 1100 + b)) for b in range(2)]
+    b = ss_api.Batch(probs, 40, precision=ss_abi.SS_PREC_BF16X2)
+    reps, _ = b.run()
+    for i, p in enumerate(probs):
+        want = ss_api.solve_plain(p, 40, precision=ss_abi.SS_PREC_BF16X2)
+        assert reps[i].exploitability == pytest.approx(want.exploitability, rel=1e-12), i
+    b.close()
+
+
 def test_precision_form_scope():
-    """3+ players, and the batched / served forms, are FP32 only."""
+    """3+ players are FP32 only; the served form is FP32 only."""
     sp = wl.small_spot(5, num_players=3, hands_per_player=6)
     prob = ss_api.Problem.from_spot(sp)
     with pytest.raises(ss_api.SolveError):
         ss_api.solve_plain(prob, 2, precision=ss_abi.SS_PREC_BF16X2)
+    with pytest.raises(ss_api.SolveError):
+        ss_api.Batch([prob], 2, precision=ss_abi.SS_PREC_BF16X2)
