@@ -58,6 +58,7 @@ struct SlotDesc {
     int is_river, has_leaf, num_leaves, root_player;
     int nh[6];
     int ng, odometer;          // odometer: k>=3 terminals present (slow path)
+    int uniq;                  // live combos unique per player: scatters are plain stores
     // progress (device-updated; iteration-dependent data never travels as a kernel argument)
     int iter;                  // last completed iteration
     int iter_end;              // the slot stops advancing at this iteration

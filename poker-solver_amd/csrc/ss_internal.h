@@ -46,7 +46,7 @@ int from_ex(const SSProblem *p, HostProblem *out);
 // Host-side plan of a slot: every device array, filled before upload.
 struct Plan {
     int N = 0, NP = 0, Mp = 0, maxh = 0, A = 0, L = 0, T = 0, D = 0;
-    int is_river = 0, has_leaf = 0, num_leaves = 0, root_player = 0, ng = 0, odometer = 0;
+    int is_river = 0, has_leaf = 0, num_leaves = 0, root_player = 0, ng = 0, odometer = 0, uniq = 1;
     int nh[6] = {0, 0, 0, 0, 0, 0};
     int board[5] = {0, 0, 0, 0, 0};
     int num_board = 0;

@@ -419,7 +419,8 @@ __device__ T odometer(const SlotDesc *__restrict__ s, int n, int trav, int h, co
 }
 
 // Scatter opponent q's reach at node n into the dense card-pair matrix.
-// Each combo owns two cells, so with unique hands every cell gets one add.
+// Each combo owns two cells: with unique hands every cell gets one store
+// (bitwise the same as an add onto the zeroed cell); duplicates add.
 // Loads are issued four hands at a time, ahead of the adds.
 template <typename T>
 __device__ __forceinline__ void scatter(const SlotDesc *__restrict__ s, int q, int n, const T *reach, T *M,
@@ -440,8 +441,13 @@ __device__ __forceinline__ void scatter(const SlotDesc *__restrict__ s, int q, i
         for (int u = 0; u < 4; ++u) {
             if (!(oi[u] & SS_LIVE_BIT)) continue;
             const int a = oi[u] & 63, b = (oi[u] >> 6) & 63;
-            atomicAdd(&M[a * DN + b], r[u]);
-            atomicAdd(&M[b * DN + a], r[u]);
+            if (s->uniq) {   // one writer per cell
+                M[a * DN + b] = r[u];
+                M[b * DN + a] = r[u];
+            } else {
+                atomicAdd(&M[a * DN + b], r[u]);
+                atomicAdd(&M[b * DN + a], r[u]);
+            }
         }
     }
 }
@@ -531,6 +537,47 @@ __device__ __forceinline__ void pair_products(const T *M1, const T *M2, const T 
     }
 }
 
+// FP32 forms: G on the matrix cores, v_mfma_f32_16x16x4_f32 over 16x16 tiles
+// of the 64x64-padded product (K = 52 cards in a fixed order). The padding
+// carries the vector products too: row 52 of A = M1^T is C1 and column 52 of
+// B = M2 is C2, so G'[52][q] = U1[q], G'[p][52] = U2[p], G'[52][52] = C1.C2.
+// Lane layout: lane l holds A[l % 16][l / 16], B[l / 16][l % 16],
+// D[4 (l / 16) + 0..3][l % 16].
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+__device__ __forceinline__ void pair_products_mfma(const float *M1, const float *M2, const float *C1,
+                                                   const float *C2, float *G, float *U1, float *U2, float *TT,
+                                                   int tid, int nt) {
+    const int l = tid & 63, nw = nt >> 6;
+    for (int P = tid >> 6; P < 4; P += nw) {
+        f32x4 acc[4];
+#pragma unroll
+        for (int Q = 0; Q < 4; ++Q) acc[Q] = f32x4{0.f, 0.f, 0.f, 0.f};
+        const int p = 16 * P + (l & 15);
+#pragma unroll 1
+        for (int k0 = 0; k0 < 52; k0 += 4) {
+            const int c = k0 + (l >> 4);
+            const float a = p < 52 ? M1[c * DN + p] : (p == 52 ? C1[c] : 0.f);
+#pragma unroll
+            for (int Q = 0; Q < 4; ++Q) {
+                const int q = 16 * Q + (l & 15);
+                const float b = q < 52 ? M2[c * DN + q] : (q == 52 ? C2[c] : 0.f);
+                acc[Q] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[Q], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int Q = 0; Q < 4; ++Q)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int row = 16 * P + 4 * (l >> 4) + i, col = 16 * Q + (l & 15);
+                const float g = acc[Q][i];
+                if (row < 52 && col < 52) G[row * DN + col] = g;
+                else if (row == 52 && col < 52) U1[col] = g;
+                else if (col == 52 && row < 52) U2[row] = g;
+                else if (row == 52 && col == 52) *TT = g;
+            }
+    }
+}
+
 // Kinds of terminal evaluation
 #define TK_NODE 0      // the node's own type (fold / showdown / leaf)
 #define TK_MASS 1      // opponent mass only, all non-traversers, times `scale`
@@ -608,7 +655,10 @@ __device__ void terminal_node(const SlotDesc *__restrict__ s, int n, int trav, c
     if (sd && k == 1) scan_sorted<T, 1>(s, B.M1, nullptr, B.PG, B.red, tid);
     if (sd && two) scan_sorted<T, 3>(s, B.M1, B.M2, B.PG, B.red, tid);
     if (two && !sd) {
-        pair_products<T>(B.M1, B.M2, B.C1, B.C2, B.PC1, B.U1, B.U2, tid, nt);
+        if constexpr (std::is_same<T, float>::value)
+            pair_products_mfma(B.M1, B.M2, B.C1, B.C2, B.PC1, B.U1, B.U2, B.red, tid, nt);
+        else
+            pair_products<T>(B.M1, B.M2, B.C1, B.C2, B.PC1, B.U1, B.U2, tid, nt);
         __syncthreads();
     }
 
@@ -617,8 +667,11 @@ __device__ void terminal_node(const SlotDesc *__restrict__ s, int n, int trav, c
     if (two) {
         S2 = (T)0.5 * tsum52(B.C2);
         SP = (T)0.5 * tsum52(B.CP);
-        if (!sd)
-            for (int c = 0; c < 52; ++c) TT += B.C1[c] * B.C2[c];
+        if (!sd) {
+            if constexpr (std::is_same<T, float>::value) TT = B.red[0];   // from the MFMA padding
+            else
+                for (int c = 0; c < 52; ++c) TT += B.C1[c] * B.C2[c];
+        }
     }
 
     // Showdowns walk the traverser's hands by strength (shared class

@@ -95,8 +95,13 @@ __device__ void prec_split_node(const SlotDesc *__restrict__ s, const PrecArgs &
         const uint32_t hj = s->hinfo[(size_t)o * Mp + j];
         if (!(hj & SS_LIVE_BIT)) continue;
         const int c0 = hj & 63, c1 = (hj >> 6) & 63;
-        atomicAdd(&M[c0 * 52 + c1], r[j]);   // one add per cell with unique hands
-        atomicAdd(&M[c1 * 52 + c0], r[j]);
+        if (s->uniq) {   // one writer per cell
+            M[c0 * 52 + c1] = r[j];
+            M[c1 * 52 + c0] = r[j];
+        } else {
+            atomicAdd(&M[c0 * 52 + c1], r[j]);
+            atomicAdd(&M[c1 * 52 + c0], r[j]);
+        }
     }
     __syncthreads();
     const int lane = threadIdx.x & 63;

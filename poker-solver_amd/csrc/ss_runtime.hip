@@ -194,6 +194,19 @@ int make_plan(const HostProblem &hp, Plan &pl) {
             pl.hinfo[(size_t)p * Mp + h] = (uint32_t)a | ((uint32_t)b << 6) | ((uint32_t)combo_id(a, b) << 12) | live;
             pl.weights[(size_t)p * Mp + h] = hp.weights[p][h];
         }
+    // live combos unique within every player's list: card-pair scatters are
+    // then single writes per cell (duplicates fall back to atomic adds)
+    pl.uniq = 1;
+    for (int p = 0; p < NP && pl.uniq; ++p) {
+        std::vector<char> seen(SS_NCOMBO, 0);
+        for (int h = 0; h < hp.nh[p]; ++h) {
+            const uint32_t hi = pl.hinfo[(size_t)p * Mp + h];
+            if (!(hi & SS_LIVE_BIT)) continue;
+            char &f = seen[(hi >> 12) & 2047];
+            if (f) { pl.uniq = 0; break; }
+            f = 1;
+        }
+    }
 
     // terminal opponent counts: 3+ opponents take the odometer path
     int64_t odo_work = 0;
@@ -401,6 +414,7 @@ int Slot::setup(const HostProblem &hp, hipStream_t st) {
     d.root_player = pl.root_player;
     for (int p = 0; p < 6; ++p) d.nh[p] = p < pl.NP ? pl.nh[p] : 0;
     d.odometer = pl.odometer;
+    d.uniq = pl.uniq;
     d.nodes = (const DevNode *)(a + o_nodes);
     d.children = (const int *)(a + o_child);
     d.dec_lvl = (const int *)(a + o_declvl);
