@@ -260,6 +260,20 @@ def precision_bench(boards, iters, ref_boards, cores):
     return out
 
 
+# ---------------------------------------------------------------- roofline traffic
+def traffic_per_launch(algo_per_launch, workload):
+    """HBM bytes per CFR iteration from the committed counter profile
+    (profiles/cfr_iteration_traffic.md: calibrated FETCH_SIZE + WRITE_SIZE of a
+    full-batch iteration), carried over to this run's mean iteration by the
+    measured traffic / algorithmic-bytes ratio. None when the profile is absent
+    or was taken on another workload."""
+    path = os.path.join(REPO, "profiles", "cfr_iteration_traffic.json")
+    if workload != "batch_mix" or not os.path.exists(path):
+        return None
+    d = json.load(open(path))
+    return int(d["traffic_over_algo"] * algo_per_launch)
+
+
 # ---------------------------------------------------------------- main
 def main():
     ap = argparse.ArgumentParser()
@@ -366,7 +380,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": None,
+            "traffic": traffic_per_launch(algo / max(1, launches), a.workload),
+            "traffic_source": "profiles/cfr_iteration_traffic.md (calibrated FETCH_SIZE + WRITE_SIZE per "
+                              "full-batch iteration) x this run's algorithmic bytes per iteration",
             "kernel": "cfr_iteration" if stats_all[-1].mode == 1 else "k_batched",
             "launches": launches,
             "avg_launch_us": round(cfr_ms * 1000.0 / max(1, launches), 3),

@@ -36,6 +36,12 @@ int ssdbg_solve_state(const SSProblem *problem, int iterations, int form, float 
 /* Algorithmic bytes of one CFR iteration and the device footprint. */
 int ssdbg_plan_bytes(const SSProblem *problem, int64_t *algo_bytes_per_iter, int64_t *footprint);
 
+/* Device copy of `bytes` bytes (multiple of 16) between two fresh buffers,
+ * `repeats` times, by a plain streaming kernel (k_copy): the known-size
+ * dispatch that calibrates HBM byte counters. Returns the mean ms per copy in
+ * *ms when ms is not NULL. */
+int ssdbg_copy(int64_t bytes, int repeats, float *ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -609,4 +609,9 @@ int ssdbg_plan_bytes(const SSProblem *problem, int64_t *algo_bytes_per_iter, int
     return SS_OK;
 }
 
+int ssdbg_copy(int64_t bytes, int repeats, float *ms) {
+    clear_error();
+    return dbg_copy(bytes, repeats, ms);
+}
+
 }  // extern "C"

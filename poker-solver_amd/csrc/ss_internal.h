@@ -135,6 +135,7 @@ struct PhasedTasks {
     ~PhasedTasks();
 };
 // tasks of the non-null slots with live[slot] != 0 (all of them when live is null)
+int dbg_copy(int64_t bytes, int repeats, float *ms);
 int prec_setup(Slot &s, hipStream_t st);
 int prec_showdowns(Slot &s, int t, hipStream_t st);
 struct PhasedTasks;

@@ -1665,6 +1665,38 @@ int launch_phased_expl(SlotDesc *d_slots, const PhasedTasks &pt, const int *d_wa
     return SS_OK;
 }
 
+// plain streaming copy (HBM counter calibration)
+__global__ void __launch_bounds__(SS_BLOCK) k_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+int dbg_copy(int64_t bytes, int repeats, float *ms) {
+    if (bytes <= 0 || bytes % 16 || repeats <= 0) return set_error(SS_ERR_ARG, "bytes must be a positive multiple of 16");
+    uint4 *a = nullptr, *b = nullptr;
+    SS_HIP(hipMalloc(&a, bytes));
+    SS_HIP(hipMalloc(&b, bytes));
+    SS_HIP(hipMemset(a, 1, bytes));
+    hipEvent_t e0, e1;
+    SS_HIP(hipEventCreate(&e0));
+    SS_HIP(hipEventCreate(&e1));
+    const size_t n = (size_t)bytes / 16;
+    SS_HIP(hipEventRecord(e0, 0));
+    for (int r = 0; r < repeats; ++r)
+        hipLaunchKernelGGL(k_copy, dim3(2048), dim3(SS_BLOCK), 0, 0, (const uint4 *)a, b, n);
+    SS_HIP(hipGetLastError());
+    SS_HIP(hipEventRecord(e1, 0));
+    SS_HIP(hipEventSynchronize(e1));
+    float t = 0.f;
+    SS_HIP(hipEventElapsedTime(&t, e0, e1));
+    if (ms) *ms = t / repeats;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return SS_OK;
+}
+
 // ---- test entry points -----------------------------------------------------
 int dbg_eval7_device(const int *cards, int n, uint32_t *out) {
     int *dc = nullptr;
