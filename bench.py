@@ -292,7 +292,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cpu-ref", action="store_true")
     ap.add_argument("--served-requests", type=int, default=200, help="0 skips the served-form run")
-    ap.add_argument("--served-rate", type=float, default=40.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--served-rate", type=float, default=25.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--served-iters-per-step", type=int, default=5)
     ap.add_argument("--precision-boards", type=int, default=64, help="0 skips the precision-form run")
     ap.add_argument("--precision-iters", type=int, default=1000)
@@ -403,6 +403,12 @@ def main():
                                       a.check_every, a.max_iters, a.served_iters_per_step)
         log(f"[bench] served: {line['served']['solves_per_s']} solves/s, "
             f"p50 {line['served']['latency_ms']['p50']} ms, p99 {line['served']['latency_ms']['p99']} ms")
+        log(f"[bench] served form capacity: the same {a.served_requests} requests submitted at once")
+        cap = served_bench(a.served_requests, 1e9, 20261203 + 5000, a.target, a.check_every, a.max_iters,
+                           a.served_iters_per_step)
+        line["served"]["capacity_solves_per_s"] = cap["solves_per_s"]
+        line["served"]["capacity_latency_ms"] = cap["latency_ms"]
+        log(f"[bench] served capacity: {cap['solves_per_s']} solves/s")
 
     if a.precision_boards > 0:
         log(f"[bench] precision form: {a.precision_boards} boards x {a.precision_iters} iterations")
