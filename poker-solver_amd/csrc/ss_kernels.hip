@@ -1188,8 +1188,11 @@ __device__ __forceinline__ bool slot_runs(const SlotDesc *__restrict__ s, int tr
     return s->active && s->iter < s->iter_end && trav < s->NP;
 }
 
-__global__ void __launch_bounds__(SS_BLOCK) k_fwd_phased(SlotDesc *__restrict__ slots, const uint32_t *tasks,
-                                                         int ntasks, int trav) {
+// Sweeps: one workgroup of SWEEP_BLOCK threads per (subgame, node) task, so a
+// 1081-combo node (Mp = 1088) takes two nearly full rounds of hands.
+#define SWEEP_BLOCK 576
+__global__ void __launch_bounds__(SWEEP_BLOCK) k_fwd_phased(SlotDesc *__restrict__ slots, const uint32_t *tasks,
+                                                            int ntasks, int trav) {
     for (int j = blockIdx.x; j < ntasks; j += gridDim.x) {
         uint32_t tk = tasks[j];
         const SlotDesc *s = slots + TASK_SLOT(tk);
@@ -1258,8 +1261,8 @@ __global__ void __launch_bounds__(SS_BLOCK) k_term_odo(SlotDesc *__restrict__ sl
     }
 }
 
-__global__ void __launch_bounds__(SS_BLOCK) k_bwd_phased(SlotDesc *__restrict__ slots, const uint32_t *tasks,
-                                                         int ntasks, int trav) {
+__global__ void __launch_bounds__(SWEEP_BLOCK) k_bwd_phased(SlotDesc *__restrict__ slots, const uint32_t *tasks,
+                                                            int ntasks, int trav) {
     for (int j = blockIdx.x; j < ntasks; j += gridDim.x) {
         uint32_t tk = tasks[j];
         const SlotDesc *s = slots + TASK_SLOT(tk);
@@ -1605,7 +1608,7 @@ int launch_phased_iteration(SlotDesc *d_slots, const PhasedTasks &pt, hipStream_
         for (int l = 0; l < pt.L; ++l) {
             int cnt = pt.lvl_off[l + 1] - pt.lvl_off[l];
             if (cnt > 0)
-                hipLaunchKernelGGL(k_fwd_phased, dim3(cnt), dim3(SS_BLOCK), 0, st, d_slots, pt.d_tasks + pt.lvl_off[l],
+                hipLaunchKernelGGL(k_fwd_phased, dim3(cnt), dim3(SWEEP_BLOCK), 0, st, d_slots, pt.d_tasks + pt.lvl_off[l],
                                    cnt, t);
         }
         if (pt.npsplit > 0) {
@@ -1628,7 +1631,7 @@ int launch_phased_iteration(SlotDesc *d_slots, const PhasedTasks &pt, hipStream_
         for (int l = pt.L - 1; l >= 0; --l) {
             int cnt = pt.lvl_off[l + 1] - pt.lvl_off[l];
             if (cnt > 0)
-                hipLaunchKernelGGL(k_bwd_phased, dim3(cnt), dim3(SS_BLOCK), 0, st, d_slots, pt.d_tasks + pt.lvl_off[l],
+                hipLaunchKernelGGL(k_bwd_phased, dim3(cnt), dim3(SWEEP_BLOCK), 0, st, d_slots, pt.d_tasks + pt.lvl_off[l],
                                    cnt, t);
         }
     }
