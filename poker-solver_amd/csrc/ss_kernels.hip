@@ -524,6 +524,7 @@ __device__ __forceinline__ void pair_products(const T *M1, const T *M2, const T 
     for (int i = tid; i < DN2; i += nt) {
         int p = i / DN, q = i - p * DN;
         T g = 0;
+#pragma unroll 4
         for (int c = 0; c < 52; ++c) g += M1[c * DN + p] * M2[c * DN + q];
         G[i] = g;
     }
@@ -811,52 +812,60 @@ __device__ void terminal_node(const SlotDesc *__restrict__ s, int n, int trav, c
 // is one ds_read2st64 away. The byte tables and short arrays come first so
 // their addresses fit the 16-bit immediate offsets of LDS instructions.
 #define DNP 2752
-struct TermHeavy {
+template <typename T>
+struct TermHeavyT {
     alignas(16) uint8_t clist[DN2];
     alignas(16) uint8_t cpos[DN2];
     alignas(16) uint8_t clen[64];
-    float C1[52], C2[52], CP[52], U1[52], U2[52];
-    float red[64];
-    float PG[3 * PGS];
-    alignas(256) float M1[DNP];
-    float M2[DNP];
-    float PC1[DNP], PC2[DNP], PCP[DNP];
+    T C1[52], C2[52], CP[52], U1[52], U2[52];
+    T red[64];
+    T PG[3 * PGS];
+    alignas(256) T M1[DNP];
+    T M2[DNP];
+    T PC1[DNP], PC2[DNP], PCP[DNP];
 };
+using TermHeavy = TermHeavyT<float>;   // 76 KB; the FP64 one (150 KB) serves exploitability checks
 
-__device__ __forceinline__ TermBuf<float> heavy_buf(TermHeavy &t) {
-    TermBuf<float> b;
+template <typename T>
+__device__ __forceinline__ TermBuf<T> heavy_buf(TermHeavyT<T> &t) {
+    TermBuf<T> b;
     b.M1 = t.M1; b.M2 = t.M2; b.PC1 = t.PC1; b.PC2 = t.PC2; b.PCP = t.PCP;
     b.PG = t.PG; b.C1 = t.C1; b.C2 = t.C2; b.CP = t.CP; b.U1 = t.U1; b.U2 = t.U2; b.red = t.red;
     b.clist = t.clist; b.clen = t.clen; b.cpos = t.cpos;
     return b;
 }
 
-// Two-opponent fold and leaf nodes (~33 KB): both matrices and G = M1^T M2.
-struct TermFold {
-    float M1[DNP], M2[DNP], G[DNP];
-    float C1[52], C2[52], CP[52], U1[52], U2[52];
-    float red[64];
+// Two-opponent fold and leaf nodes (~33 KB in FP32): both matrices and G = M1^T M2.
+template <typename T>
+struct TermFoldT {
+    T M1[DNP], M2[DNP], G[DNP];
+    T C1[52], C2[52], CP[52], U1[52], U2[52];
+    T red[64];
 };
+using TermFold = TermFoldT<float>;
 
-__device__ __forceinline__ TermBuf<float> fold_buf(TermFold &t) {
-    TermBuf<float> b;
+template <typename T>
+__device__ __forceinline__ TermBuf<T> fold_buf(TermFoldT<T> &t) {
+    TermBuf<T> b;
     b.M1 = t.M1; b.M2 = t.M2; b.PC1 = t.G; b.PC2 = nullptr; b.PCP = nullptr;
     b.PG = nullptr; b.C1 = t.C1; b.C2 = t.C2; b.CP = t.CP; b.U1 = t.U1; b.U2 = t.U2; b.red = t.red;
     b.clist = nullptr; b.clen = nullptr; b.cpos = nullptr;
     return b;
 }
 
-struct TermLight {
-    float M1[DN2];
-    float PC1[DN2];
-    float PG[1328];
-    float C1[52];
-    float red[64];
+template <typename T>
+struct TermLightT {
+    T M1[DN2];
+    T PC1[DN2];
+    T PG[1328];
+    T C1[52];
+    T red[64];
 };
-
+using TermLight = TermLightT<float>;
 
 // copy a river subgame's per-card lists and positions into LDS (caller barriers)
-__device__ __forceinline__ void stage_tables(const SlotDesc *__restrict__ s, TermHeavy &t, int tid, int nt) {
+template <typename T>
+__device__ __forceinline__ void stage_tables(const SlotDesc *__restrict__ s, TermHeavyT<T> &t, int tid, int nt) {
     if (!s->is_river) return;
     const uint32_t *cl = (const uint32_t *)s->clist, *cp = (const uint32_t *)s->cpos;
     uint32_t *tl = (uint32_t *)t.clist, *tp = (uint32_t *)t.cpos;
@@ -867,8 +876,9 @@ __device__ __forceinline__ void stage_tables(const SlotDesc *__restrict__ s, Ter
     for (int i = tid; i < 52; i += nt) t.clen[i] = s->clen[i];
 }
 
-__device__ __forceinline__ TermBuf<float> light_buf(TermLight &t, const SlotDesc *__restrict__ s) {
-    TermBuf<float> b;
+template <typename T>
+__device__ __forceinline__ TermBuf<T> light_buf(TermLightT<T> &t, const SlotDesc *__restrict__ s) {
+    TermBuf<T> b;
     b.M1 = t.M1; b.M2 = nullptr; b.PC1 = t.PC1; b.PC2 = nullptr; b.PCP = nullptr;
     b.PG = t.PG; b.C1 = t.C1; b.C2 = nullptr; b.CP = nullptr; b.U1 = nullptr; b.U2 = nullptr; b.red = t.red;
     b.clist = s->clist; b.clen = s->clen; b.cpos = s->cpos;
@@ -1329,6 +1339,40 @@ __global__ void __launch_bounds__(SS_BLOCK) k_xterm_phased(SlotDesc *__restrict_
     }
 }
 
+// FP64 terminals of the closed-form classes with LDS buffers, split as the
+// FP32 ones: KIND 0 at most one opponent, 1 two-opponent showdowns (150 KB,
+// one workgroup per CU), 2 two-opponent folds and leaves. k_xterm_phased
+// (global scratch) keeps the 3+ opponent tasks.
+template <int KIND>
+__global__ void __launch_bounds__(KIND == 1 ? 512 : SS_BLOCK) __attribute__((amdgpu_waves_per_eu(2)))
+k_xterm_cls(SlotDesc *__restrict__ slots, const uint32_t *tasks, int ntasks, int p, const int *want) {
+    using Buf = typename std::conditional<KIND == 0, TermLightT<double>,
+                                          typename std::conditional<KIND == 1, TermHeavyT<double>,
+                                                                    TermFoldT<double>>::type>::type;
+    __shared__ Buf sh;
+    for (int j = blockIdx.x; j < ntasks; j += gridDim.x) {
+        uint32_t tk = tasks[j];
+        const SlotDesc *s = slots + TASK_SLOT(tk);
+        if (!slot_wanted(s, want, TASK_SLOT(tk)) || p >= s->NP) continue;
+        const int n = TASK_NODE(tk);
+        double *ev = s->xev + (size_t)n * s->Mp;
+        if constexpr (KIND == 0) {
+            terminal_node<double, TM_LIGHT>(s, n, p, s->xreach, ev, light_buf(sh, s), TK_NODE, nullptr, threadIdx.x,
+                                            blockDim.x);
+        } else if constexpr (KIND == 1) {
+            stage_tables(s, sh, threadIdx.x, blockDim.x);
+            __syncthreads();
+            terminal_node<double, TM_PAIR, 1>(s, n, p, s->xreach, ev, heavy_buf(sh), TK_NODE, nullptr, threadIdx.x,
+                                              blockDim.x);
+        } else {
+            terminal_node<double, TM_PAIR, 0>(s, n, p, s->xreach, ev, fold_buf(sh), TK_NODE, nullptr, threadIdx.x,
+                                              blockDim.x);
+        }
+        for (int h = threadIdx.x; h < s->nh[p]; h += blockDim.x) s->xbr[(size_t)n * s->Mp + h] = ev[h];
+        __syncthreads();
+    }
+}
+
 __global__ void __launch_bounds__(SS_BLOCK) k_xbwd_phased(SlotDesc *__restrict__ slots, const uint32_t *tasks,
                                                           int ntasks, int p, const int *want) {
     for (int j = blockIdx.x; j < ntasks; j += gridDim.x) {
@@ -1651,9 +1695,19 @@ int launch_phased_expl(SlotDesc *d_slots, const PhasedTasks &pt, const int *d_wa
                                d_want);
     }
     for (int p = 0; p < pt.NPmax; ++p) {
-        if (pt.nterm > 0)
-            hipLaunchKernelGGL(k_xterm_phased, dim3(std::min(pt.nterm, scratch_blocks)), dim3(SS_BLOCK), 0, st, d_slots,
-                               pt.d_tasks + pt.term_off, pt.nterm, p, d_want, scratch);
+        const uint32_t *tt = pt.d_tasks + pt.term_off;
+        const int nsd = pt.npair_sd, nfold = pt.npair - pt.npair_sd, nlight = pt.nterm - pt.nodo - pt.npair;
+        if (pt.nodo > 0)
+            hipLaunchKernelGGL(k_xterm_phased, dim3(std::min(pt.nodo, scratch_blocks)), dim3(SS_BLOCK), 0, st, d_slots,
+                               tt, pt.nodo, p, d_want, scratch);
+        if (nsd > 0)
+            hipLaunchKernelGGL(k_xterm_cls<1>, dim3(nsd), dim3(512), 0, st, d_slots, tt + pt.nodo, nsd, p, d_want);
+        if (nfold > 0)
+            hipLaunchKernelGGL(k_xterm_cls<2>, dim3(nfold), dim3(SS_BLOCK), 0, st, d_slots, tt + pt.nodo + nsd, nfold, p,
+                               d_want);
+        if (nlight > 0)
+            hipLaunchKernelGGL(k_xterm_cls<0>, dim3(nlight), dim3(SS_BLOCK), 0, st, d_slots, tt + pt.nodo + pt.npair,
+                               nlight, p, d_want);
         for (int l = pt.L - 1; l >= 0; --l) {
             int cnt = pt.lvl_off[l + 1] - pt.lvl_off[l];
             if (cnt > 0)
