@@ -142,6 +142,30 @@ def test_batch_session_modes_equal_plain(mode, monkeypatch):
     b.close()
 
 
+def test_phased_batch_with_three_or_more_opponents_equals_plain(monkeypatch):
+    """4- and 5-player spots take the odometer terminals (k_term_odo) in the
+    phased form; results equal the plain form bit for bit and the state after
+    one iteration matches the brute-force oracle."""
+    monkeypatch.setenv("SS_BATCH_MODE", "phased")
+    spots = [wl.small_spot(50, num_players=4, hands_per_player=5), wl.small_spot(51, num_players=5, hands_per_player=4),
+             wl.small_spot(52, num_players=3, hands_per_player=6)]
+    probs = [ss_api.Problem.from_spot(s) for s in spots]
+    b = ss_api.Batch(probs, 6, flags=1)
+    reps, st = b.run()
+    assert st.mode == 1
+    for i, p in enumerate(probs):
+        got = b.output(i)
+        want = ss_api.solve_plain(p, 6, flags=1)
+        assert np.array_equal(got.avg, want.avg), i
+        assert np.array_equal(got.root_strategy, want.root_strategy), i
+    b.close()
+    prob, oprob = _spot_problem(spots[1])
+    R, S, ev = ss_api.solve_state(prob, 1, 0)
+    ref = ob.solve(oprob, 1, fp64=True, mode=ob.TERM_BRUTE, want_state=True)
+    _close_state(R, ref["regrets"])
+    _close_state(S, ref["strat_sum"])
+
+
 def test_state_cfg1_shape_full_ranges():
     sp = wl.river_2p()
     prob, oprob = _spot_problem(sp)
