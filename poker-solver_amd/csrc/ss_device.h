@@ -90,7 +90,7 @@ struct SlotDesc {
     // precision form (2-player rivers under SS_PREC_BF16X2), zero otherwise
     int prec_S, prec_S16;      // showdown nodes, rounded up to 16
     const int *prec_sd;        // [prec_S] showdown node ids
-    const uint16_t *prec_W[2]; // per traverser [Mp][Mp] bf16 (+1 / -1 / 0)
+    const uint2 *prec_W[2];    // per traverser [Mp][Mp / 32] sign masks {+1 bits, -1 bits} of W
     uint16_t *prec_Bt;         // [2 prec_S16][Mp] bf16 hi / lo opponent reach
     float *prec_Cn;            // [prec_S][64] per-card opponent totals
     const int *prec_same[2];   // per traverser [Mp] opponent hand with the same cards, or -1

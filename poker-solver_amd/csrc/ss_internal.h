@@ -79,7 +79,7 @@ void build_river_tables(const uint32_t *cstr, RiverTables &rt);
 struct PrecState {
     int S = 0, S16 = 0;                   // showdown nodes, rounded up to 16
     int *sd_nodes = nullptr;              // [S]
-    uint16_t *W[2] = {nullptr, nullptr};  // per traverser [Mp][Mp] bf16 (+1 / -1 / 0)
+    uint2 *W[2] = {nullptr, nullptr};     // per traverser [Mp][Mp / 32] sign masks of W (+1 / -1 / 0)
     uint16_t *Bt = nullptr;               // [2 S16][Mp] bf16 hi / lo opponent reach
     float *Cn = nullptr;                  // [S][64] per-card opponent totals
     int *same[2] = {nullptr, nullptr};    // [Mp] opponent hand with the same cards, or -1

@@ -12,8 +12,10 @@
 // The contraction over opponent hands runs on v_mfma_f32_16x16x32_bf16 for
 // all S showdown nodes at once: Y[Mp x 2 S16] = W[Mp x Mp] * [R_hi | R_lo],
 // where R_hi = bf16(r) and R_lo = bf16(r - R_hi), so the reach carries 16
-// significant bits and W (entries -1, 0, +1) is exact. Z, the mass floor and
-// the epilogue are FP32 as in the closed-form path.
+// significant bits and W (entries -1, 0, +1) is exact. W is stored as two sign
+// masks per row and 32-column chunk (2 bits per entry, 8x fewer bytes than
+// bf16) and expanded to bf16 in registers through a 256-entry LDS table. Z,
+// the mass floor and the epilogue are FP32 as in the closed-form path.
 #include "ss_internal.h"
 
 namespace ss {
@@ -28,7 +30,7 @@ typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
 struct PrecArgs {
     int S, S16, NT;           // showdown nodes, S rounded up to 16, S16 / 16
     const int *sd_nodes;      // [S]
-    const uint16_t *W;        // [Mp][Mp] bf16 bits, rows = traverser hands
+    const uint2 *W;           // [Mp][Mp / 32] sign masks {+1 bits, -1 bits}, rows = traverser hands
     uint16_t *Bt;             // [2 S16][Mp] bf16 bits: rows b < S16 hi, S16 + b lo
     float *Cn;                // [S][64]: per-card opponent totals, [52] = total
     const int *same;          // [Mp] opponent hand with the traverser hand's cards, or -1
@@ -42,23 +44,31 @@ __device__ __forceinline__ uint16_t bf16_rne(float f) {
 }
 __device__ __forceinline__ float bf16_to_f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-// W for traverser t: +1 / -1 / 0 by rank, 0 for blocked or card-sharing pairs
-__global__ void __launch_bounds__(SS_BLOCK) k_prec_w(const SlotDesc *__restrict__ s, int t, uint16_t *W) {
-    const int Mp = s->Mp, o = 1 - t;
-    const size_t total = (size_t)Mp * Mp;
-    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (size_t)gridDim.x * blockDim.x) {
-        const int i = (int)(q / Mp), j = (int)(q % Mp);
-        uint16_t w = 0;
-        if (i < s->nh[t] && j < s->nh[o]) {
-            const uint32_t hi = s->hinfo[(size_t)t * Mp + i], hj = s->hinfo[(size_t)o * Mp + j];
-            if ((hi & SS_LIVE_BIT) && (hj & SS_LIVE_BIT)) {
-                const uint64_t mi = (1ull << (hi & 63)) | (1ull << ((hi >> 6) & 63));
+// W for traverser t: +1 / -1 / 0 by rank, 0 for blocked or card-sharing
+// pairs, as masks: W[i][q] = {bits j with W(i, 32 q + j) = +1, ... = -1}
+__global__ void __launch_bounds__(SS_BLOCK) k_prec_w(const SlotDesc *__restrict__ s, int t, uint2 *W) {
+    const int Mp = s->Mp, o = 1 - t, nq = Mp / 32;
+    const size_t total = (size_t)Mp * nq;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(e / nq), q = (int)(e % nq);
+        uint32_t pos = 0u, neg = 0u;
+        const uint32_t hi = i < s->nh[t] ? s->hinfo[(size_t)t * Mp + i] : 0u;
+        if (hi & SS_LIVE_BIT) {
+            const uint64_t mi = (1ull << (hi & 63)) | (1ull << ((hi >> 6) & 63));
+            const uint32_t si = s->cstr[(hi >> 12) & 2047];
+            for (int jb = 0; jb < 32; ++jb) {
+                const int j = 32 * q + jb;
+                if (j >= s->nh[o] || !si) continue;
+                const uint32_t hj = s->hinfo[(size_t)o * Mp + j];
+                if (!(hj & SS_LIVE_BIT)) continue;
                 const uint64_t mj = (1ull << (hj & 63)) | (1ull << ((hj >> 6) & 63));
-                const uint32_t si = s->cstr[(hi >> 12) & 2047], sj = s->cstr[(hj >> 12) & 2047];
-                if (!(mi & mj) && si && sj) w = si > sj ? 0x3F80u : (si < sj ? 0xBF80u : 0u);
+                const uint32_t sj = s->cstr[(hj >> 12) & 2047];
+                if ((mi & mj) || !sj) continue;
+                if (si > sj) pos |= 1u << jb;
+                else if (si < sj) neg |= 1u << jb;
             }
         }
-        W[q] = w;
+        W[e] = make_uint2(pos, neg);
     }
 }
 
@@ -149,8 +159,25 @@ __global__ void __launch_bounds__(SS_BLOCK) k_prec_split_phased(SlotDesc *__rest
 // MFMA 16x16x32 lane layout: lane l holds A[l % 16][8 (l / 16) + 0..7],
 // B[8 (l / 16) + 0..7][l % 16], D[4 (l / 16) + 0..3][l % 16].
 #define PREC_RT 2
+// lut_one[m]: bf16 1.0 in each of 8 slots whose bit is set in m; lut_sign[m]:
+// the sign bit in those slots
+__device__ __forceinline__ void prec_fill_luts(uint4 *lut_one, uint4 *lut_sign) {
+    for (int m = threadIdx.x; m < 256; m += blockDim.x) {
+        uint32_t w1[4], ws[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t b0 = (m >> (2 * k)) & 1u, b1 = (m >> (2 * k + 1)) & 1u;
+            w1[k] = b0 * 0x3F80u | (b1 * 0x3F80u) << 16;
+            ws[k] = b0 * 0x8000u | (b1 * 0x8000u) << 16;
+        }
+        lut_one[m] = make_uint4(w1[0], w1[1], w1[2], w1[3]);
+        lut_sign[m] = make_uint4(ws[0], ws[1], ws[2], ws[3]);
+    }
+}
+
 template <int NT>
-__device__ void prec_gemm_rows(const SlotDesc *__restrict__ s, const PrecArgs &a, int t, int row0) {
+__device__ void prec_gemm_rows(const SlotDesc *__restrict__ s, const PrecArgs &a, int t, int row0,
+                               const uint4 *lut_one, const uint4 *lut_sign) {
     const int l = threadIdx.x & 63, Mp = s->Mp, o = 1 - t;
     if (row0 >= Mp) return;
     floatx4 acc[PREC_RT][2 * NT];
@@ -158,14 +185,15 @@ __device__ void prec_gemm_rows(const SlotDesc *__restrict__ s, const PrecArgs &a
     for (int r = 0; r < PREC_RT; ++r)
 #pragma unroll
         for (int q = 0; q < 2 * NT; ++q) acc[r][q] = floatx4{0.f, 0.f, 0.f, 0.f};
-    const int nrt = min(PREC_RT, (Mp - row0) / 16);
-    const uint16_t *wrow = a.W + (size_t)(row0 + (l & 15)) * Mp + 8 * (l >> 4);
+    const int nrt = min(PREC_RT, (Mp - row0) / 16), nq = Mp / 32, sh = 8 * (l >> 4);
+    const uint2 *wrow = a.W + (size_t)(row0 + (l & 15)) * nq;
     const uint16_t *bcol = a.Bt + (size_t)(l & 15) * Mp + 8 * (l >> 4);
-    uint4 fa[PREC_RT], fb[2 * NT];
+    uint2 fa[PREC_RT];
+    uint4 fb[2 * NT];
     auto load = [&](int k0) {
 #pragma unroll
         for (int r = 0; r < PREC_RT; ++r)
-            fa[r] = r < nrt ? *(const uint4 *)(wrow + (size_t)(16 * r) * Mp + k0) : make_uint4(0u, 0u, 0u, 0u);
+            fa[r] = r < nrt ? wrow[(size_t)(16 * r) * nq + (k0 >> 5)] : make_uint2(0u, 0u);
 #pragma unroll
         for (int q = 0; q < NT; ++q) {
             fb[q] = *(const uint4 *)(bcol + (size_t)(16 * q) * Mp + k0);
@@ -177,7 +205,12 @@ __device__ void prec_gemm_rows(const SlotDesc *__restrict__ s, const PrecArgs &a
     for (int ks = 0; ks < ksteps; ++ks) {
         bf16x8 va[PREC_RT], vb[2 * NT];
 #pragma unroll
-        for (int r = 0; r < PREC_RT; ++r) va[r] = __builtin_bit_cast(bf16x8, fa[r]);
+        for (int r = 0; r < PREC_RT; ++r) {
+            // lane's 8 entries: bits sh .. sh + 7 of the chunk's masks
+            const uint32_t p8 = (fa[r].x >> sh) & 0xFFu, n8 = (fa[r].y >> sh) & 0xFFu;
+            const uint4 mag = lut_one[p8 | n8], sgn = lut_sign[n8];
+            va[r] = __builtin_bit_cast(bf16x8, make_uint4(mag.x | sgn.x, mag.y | sgn.y, mag.z | sgn.z, mag.w | sgn.w));
+        }
 #pragma unroll
         for (int q = 0; q < 2 * NT; ++q) vb[q] = __builtin_bit_cast(bf16x8, fb[q]);
         if (ks + 1 < ksteps) load(32 * (ks + 1));
@@ -220,19 +253,25 @@ __device__ void prec_gemm_rows(const SlotDesc *__restrict__ s, const PrecArgs &a
 // plain form: workgroup b covers rows [128 b, 128 b + 128)
 template <int NT>
 __global__ void __launch_bounds__(64 * PREC_WAVES) k_prec_gemm(const SlotDesc *__restrict__ s, int t) {
+    __shared__ uint4 lut_one[256], lut_sign[256];
+    prec_fill_luts(lut_one, lut_sign);
+    __syncthreads();
     const int row0 = (64 * PREC_RT) * blockIdx.x + 16 * PREC_RT * (threadIdx.x >> 6);
-    prec_gemm_rows<NT>(s, prec_args(s, t), t, row0);
+    prec_gemm_rows<NT>(s, prec_args(s, t), t, row0, lut_one, lut_sign);
 }
 
 template <int NT>
 __global__ void __launch_bounds__(64 * PREC_WAVES) k_prec_gemm_phased(SlotDesc *__restrict__ slots,
                                                                       const uint32_t *tasks, int ntasks, int t) {
+    __shared__ uint4 lut_one[256], lut_sign[256];
+    prec_fill_luts(lut_one, lut_sign);
+    __syncthreads();
     for (int j = blockIdx.x; j < ntasks; j += gridDim.x) {
         const uint32_t tk = tasks[j];
         const SlotDesc *s = slots + (tk >> 20);
         if (!(s->active && s->iter < s->iter_end && t < s->NP)) continue;
         const int row0 = (64 * PREC_RT) * (int)(tk & 0xFFFFF) + 16 * PREC_RT * (threadIdx.x >> 6);
-        prec_gemm_rows<NT>(s, prec_args(s, t), t, row0);
+        prec_gemm_rows<NT>(s, prec_args(s, t), t, row0, lut_one, lut_sign);
     }
 }
 
@@ -306,8 +345,8 @@ int prec_setup(Slot &s, hipStream_t st) {
         SS_HIP(hipMalloc(&p->same[t], sizeof(int) * Mp));
         SS_HIP(hipMemcpyAsync(p->same[t], same.data(), sizeof(int) * Mp, hipMemcpyHostToDevice, st));
         SS_HIP(hipStreamSynchronize(st));   // `same` is reused
-        SS_HIP(hipMalloc(&p->W[t], sizeof(uint16_t) * (size_t)Mp * Mp));
-        hipLaunchKernelGGL(k_prec_w, dim3(std::min<int64_t>(4096, ((int64_t)Mp * Mp + SS_BLOCK - 1) / SS_BLOCK)),
+        SS_HIP(hipMalloc(&p->W[t], sizeof(uint2) * (size_t)Mp * (Mp / 32)));
+        hipLaunchKernelGGL(k_prec_w, dim3(std::min<int64_t>(4096, ((int64_t)Mp * (Mp / 32) + SS_BLOCK - 1) / SS_BLOCK)),
                            dim3(SS_BLOCK), 0, st, s.d_desc, t, p->W[t]);
     }
     SS_HIP(hipGetLastError());
