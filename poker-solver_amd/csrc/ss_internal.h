@@ -141,6 +141,7 @@ int prec_showdowns(Slot &s, int t, hipStream_t st);
 struct PhasedTasks;
 int launch_prec_phased(SlotDesc *d_slots, const PhasedTasks &pt, int t, hipStream_t st);
 int64_t prec_flops_per_pass(const Slot &s);
+int prec_rows_per_task(int nt);   // rows of one precision GEMM task for nt column tiles
 
 int build_phased_tasks(const std::vector<Slot *> &slots, PhasedTasks &pt, hipStream_t st,
                        const std::vector<int> *live = nullptr);

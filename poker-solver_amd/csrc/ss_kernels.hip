@@ -1632,8 +1632,8 @@ int build_phased_tasks(const std::vector<Slot *> &slots, PhasedTasks &pt, hipStr
         const size_t g0 = tasks.size();
         for (int si = 0; si < pt.nslots; ++si)
             if (on(si) && slots[si]->prec && slots[si]->prec->S16 / 16 == nt)
-                for (int r = 0; r < (slots[si]->plan.Mp + 127) / 128; ++r)   // 128-row blocks (k_prec_gemm_phased)
-                    tasks.push_back(((uint32_t)si << 20) | (uint32_t)r);
+                for (int r = 0, rows = prec_rows_per_task(nt); r < (slots[si]->plan.Mp + rows - 1) / rows; ++r)
+                    tasks.push_back(((uint32_t)si << 20) | (uint32_t)r);   // row blocks of k_prec_gemm_phased
         if (tasks.size() > g0) {
             xcd_interleave(tasks, g0, tasks.size());
             pt.pgemm.push_back({nt, (int)(tasks.size() - g0)});

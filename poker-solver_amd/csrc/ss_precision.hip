@@ -150,15 +150,26 @@ __global__ void __launch_bounds__(SS_BLOCK) k_prec_split_phased(SlotDesc *__rest
     }
 }
 
-// Y = W * [R_hi | R_lo] for PREC_RT row tiles of 16 traverser hands and all
-// column tiles, then the showdown values written into cfv. Each wavefront
-// reuses every B fragment across its row tiles and loads the next K step's
-// fragments before this step's MFMAs; the 4 wavefronts of a workgroup take
-// adjacent row blocks and read the same B fragments at the same K step
-// (served from L1). K runs in order, so every form rounds alike.
+// Y = W * [R_hi | R_lo] for a block of rows and all column tiles, then the
+// showdown values written into cfv. A workgroup of 4 wavefronts owns 64 RT
+// rows (each wavefront RT row tiles of 16 hands); K (opponent hands) is walked
+// in chunks: the chunk of packed reach (all columns) and of W's sign masks
+// (all rows) is staged in LDS with coalesced loads, then every wavefront reads
+// its fragments from LDS, so each reach fragment fetched from L2 feeds 4 RT
+// MFMAs. K runs in order within a wavefront, so every form rounds alike.
 // MFMA 16x16x32 lane layout: lane l holds A[l % 16][8 (l / 16) + 0..7],
 // B[8 (l / 16) + 0..7][l % 16], D[4 (l / 16) + 0..3][l % 16].
-#define PREC_RT 2
+template <int NT, int RT_> struct PrecShape {
+    static constexpr int RT = RT_;                                // row tiles per wavefront
+    static constexpr int ROWS = 64 * RT;                          // rows per workgroup / task
+    static constexpr int KCH = (512 / NT) / 64 * 64;              // K per staged chunk (multiple of 64)
+    static constexpr int BPAD = KCH + 8;                          // LDS row of one column, padded a bank
+};
+// row tiles per wavefront in the batched form: the more column tiles, the
+// fewer rows, so the accumulators stay within the register file; the plain
+// form uses one row tile for more workgroups over a single subgame. The K
+// order is the same for any row tiling, so both forms round alike.
+template <int NT> constexpr int prec_rt() { return NT <= 2 ? 4 : (NT <= 4 ? 2 : 1); }
 // lut_one[m]: bf16 1.0 in each of 8 slots whose bit is set in m; lut_sign[m]:
 // the sign bit in those slots
 __device__ __forceinline__ void prec_fill_luts(uint4 *lut_one, uint4 *lut_sign) {
@@ -175,50 +186,63 @@ __device__ __forceinline__ void prec_fill_luts(uint4 *lut_one, uint4 *lut_sign) 
     }
 }
 
-template <int NT>
-__device__ void prec_gemm_rows(const SlotDesc *__restrict__ s, const PrecArgs &a, int t, int row0,
-                               const uint4 *lut_one, const uint4 *lut_sign) {
-    const int l = threadIdx.x & 63, Mp = s->Mp, o = 1 - t;
-    if (row0 >= Mp) return;
-    floatx4 acc[PREC_RT][2 * NT];
+template <int NT, int RT>
+struct PrecLds {
+    using SH = PrecShape<NT, RT>;
+    uint4 lut_one[256], lut_sign[256];
+    alignas(16) uint16_t Bs[32 * NT][SH::BPAD];              // reach chunk: [column][k]
+    uint2 Ws[SH::ROWS][SH::KCH / 32 + 1];                    // W mask chunk: [row][32-k step], padded a bank
+};
+
+template <int NT, int RT>
+__device__ void prec_gemm_block(const SlotDesc *__restrict__ s, const PrecArgs &a, int t, int row0,
+                                PrecLds<NT, RT> &L) {
+    using SH = PrecShape<NT, RT>;
+    constexpr int KCH = SH::KCH, NCOL = 32 * NT;
+    const int tid = threadIdx.x, nt = blockDim.x, l = tid & 63, w = tid >> 6, Mp = s->Mp, o = 1 - t;
+    const int nq = Mp / 32, sh = 8 * (l >> 4), wrow0 = 16 * RT * w;   // this wavefront's first row in the block
+    floatx4 acc[RT][2 * NT];
 #pragma unroll
-    for (int r = 0; r < PREC_RT; ++r)
+    for (int r = 0; r < RT; ++r)
 #pragma unroll
         for (int q = 0; q < 2 * NT; ++q) acc[r][q] = floatx4{0.f, 0.f, 0.f, 0.f};
-    const int nrt = min(PREC_RT, (Mp - row0) / 16), nq = Mp / 32, sh = 8 * (l >> 4);
-    const uint2 *wrow = a.W + (size_t)(row0 + (l & 15)) * nq;
-    const uint16_t *bcol = a.Bt + (size_t)(l & 15) * Mp + 8 * (l >> 4);
-    uint2 fa[PREC_RT];
-    uint4 fb[2 * NT];
-    auto load = [&](int k0) {
-#pragma unroll
-        for (int r = 0; r < PREC_RT; ++r)
-            fa[r] = r < nrt ? wrow[(size_t)(16 * r) * nq + (k0 >> 5)] : make_uint2(0u, 0u);
-#pragma unroll
-        for (int q = 0; q < NT; ++q) {
-            fb[q] = *(const uint4 *)(bcol + (size_t)(16 * q) * Mp + k0);
-            fb[NT + q] = *(const uint4 *)(bcol + (size_t)(a.S16 + 16 * q) * Mp + k0);
+    for (int k0 = 0; k0 < Mp; k0 += KCH) {
+        const int kc = min(KCH, Mp - k0);   // a multiple of 32
+        // stage: the chunk of packed reach for every column (hi columns are
+        // Bt rows 0 .. 16 NT - 1, lo columns Bt rows S16 ..) and the W masks
+        // of the block's rows, coalesced along k
+        const int k8 = kc / 8, k32 = kc / 32;
+        for (int e = tid; e < NCOL * k8; e += nt) {
+            const int c = e / k8, j = e - c * k8;
+            const int brow = c < 16 * NT ? c : a.S16 + (c - 16 * NT);
+            *(uint4 *)&L.Bs[c][8 * j] = *(const uint4 *)(a.Bt + (size_t)brow * Mp + k0 + 8 * j);
         }
-    };
-    const int ksteps = Mp / 32;
-    load(0);
-    for (int ks = 0; ks < ksteps; ++ks) {
-        bf16x8 va[PREC_RT], vb[2 * NT];
-#pragma unroll
-        for (int r = 0; r < PREC_RT; ++r) {
-            // lane's 8 entries: bits sh .. sh + 7 of the chunk's masks
-            const uint32_t p8 = (fa[r].x >> sh) & 0xFFu, n8 = (fa[r].y >> sh) & 0xFFu;
-            const uint4 mag = lut_one[p8 | n8], sgn = lut_sign[n8];
-            va[r] = __builtin_bit_cast(bf16x8, make_uint4(mag.x | sgn.x, mag.y | sgn.y, mag.z | sgn.z, mag.w | sgn.w));
+        for (int e = tid; e < SH::ROWS * k32; e += nt) {
+            const int r = e / k32, j = e - r * k32, row = row0 + r;
+            L.Ws[r][j] = row < Mp ? a.W[(size_t)row * nq + (k0 >> 5) + j] : make_uint2(0u, 0u);
         }
-#pragma unroll
-        for (int q = 0; q < 2 * NT; ++q) vb[q] = __builtin_bit_cast(bf16x8, fb[q]);
-        if (ks + 1 < ksteps) load(32 * (ks + 1));
-#pragma unroll
-        for (int r = 0; r < PREC_RT; ++r)
+        __syncthreads();
+        for (int ks = 0; ks < k32; ++ks) {
+            bf16x8 va[RT], vb[2 * NT];
 #pragma unroll
             for (int q = 0; q < 2 * NT; ++q)
-                acc[r][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va[r], vb[q], acc[r][q], 0, 0, 0);
+                vb[q] = __builtin_bit_cast(bf16x8, *(const uint4 *)&L.Bs[16 * q + (l & 15)][32 * ks + sh]);
+#pragma unroll
+            for (int r = 0; r < RT; ++r) {
+                // lane's 8 entries: bits sh .. sh + 7 of the step's masks
+                const uint2 m = L.Ws[wrow0 + 16 * r + (l & 15)][ks];
+                const uint32_t p8 = (m.x >> sh) & 0xFFu, n8 = (m.y >> sh) & 0xFFu;
+                const uint4 mag = L.lut_one[p8 | n8], sgn = L.lut_sign[n8];
+                va[r] = __builtin_bit_cast(bf16x8,
+                                           make_uint4(mag.x | sgn.x, mag.y | sgn.y, mag.z | sgn.z, mag.w | sgn.w));
+            }
+#pragma unroll
+            for (int r = 0; r < RT; ++r)
+#pragma unroll
+                for (int q = 0; q < 2 * NT; ++q)
+                    acc[r][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va[r], vb[q], acc[r][q], 0, 0, 0);
+        }
+        __syncthreads();
     }
     const float *ro = s->reach + (size_t)o * s->N * Mp;
 #pragma unroll
@@ -231,11 +255,11 @@ __device__ void prec_gemm_rows(const SlotDesc *__restrict__ s, const PrecArgs &a
         const float *Cb = a.Cn + (size_t)b * 64;
         const float Sb = Cb[52];
 #pragma unroll
-        for (int r = 0; r < PREC_RT; ++r) {
+        for (int r = 0; r < RT; ++r) {
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                const int i = row0 + 16 * r + 4 * (l >> 4) + e;
-                if (r >= nrt || i >= s->nh[t]) continue;
+                const int i = row0 + wrow0 + 16 * r + 4 * (l >> 4) + e;
+                if (i >= s->nh[t]) continue;
                 const uint32_t hi = s->hinfo[(size_t)t * Mp + i];
                 float v = 0.f;
                 if ((hi & SS_LIVE_BIT) && s->cstr[(hi >> 12) & 2047]) {              // hs = 0 -> 0, :861
@@ -250,28 +274,40 @@ __device__ void prec_gemm_rows(const SlotDesc *__restrict__ s, const PrecArgs &a
     }
 }
 
-// plain form: workgroup b covers rows [128 b, 128 b + 128)
+// plain form: workgroup b covers rows [ROWS b, ROWS (b + 1))
 template <int NT>
 __global__ void __launch_bounds__(64 * PREC_WAVES) k_prec_gemm(const SlotDesc *__restrict__ s, int t) {
-    __shared__ uint4 lut_one[256], lut_sign[256];
-    prec_fill_luts(lut_one, lut_sign);
+    __shared__ PrecLds<NT, 1> L;
+    prec_fill_luts(L.lut_one, L.lut_sign);
     __syncthreads();
-    const int row0 = (64 * PREC_RT) * blockIdx.x + 16 * PREC_RT * (threadIdx.x >> 6);
-    prec_gemm_rows<NT>(s, prec_args(s, t), t, row0, lut_one, lut_sign);
+    prec_gemm_block<NT, 1>(s, prec_args(s, t), t, PrecShape<NT, 1>::ROWS * blockIdx.x, L);
 }
 
 template <int NT>
 __global__ void __launch_bounds__(64 * PREC_WAVES) k_prec_gemm_phased(SlotDesc *__restrict__ slots,
                                                                       const uint32_t *tasks, int ntasks, int t) {
-    __shared__ uint4 lut_one[256], lut_sign[256];
-    prec_fill_luts(lut_one, lut_sign);
+    constexpr int RT = prec_rt<NT>();
+    __shared__ PrecLds<NT, RT> L;
+    prec_fill_luts(L.lut_one, L.lut_sign);
     __syncthreads();
     for (int j = blockIdx.x; j < ntasks; j += gridDim.x) {
         const uint32_t tk = tasks[j];
         const SlotDesc *s = slots + (tk >> 20);
         if (!(s->active && s->iter < s->iter_end && t < s->NP)) continue;
-        const int row0 = (64 * PREC_RT) * (int)(tk & 0xFFFFF) + 16 * PREC_RT * (threadIdx.x >> 6);
-        prec_gemm_rows<NT>(s, prec_args(s, t), t, row0, lut_one, lut_sign);
+        prec_gemm_block<NT, RT>(s, prec_args(s, t), t, PrecShape<NT, RT>::ROWS * (int)(tk & 0xFFFFF), L);
+    }
+}
+
+int prec_rows_per_task(int nt) {
+    switch (nt) {
+    case 1: return 64 * prec_rt<1>();
+    case 2: return 64 * prec_rt<2>();
+    case 3: return 64 * prec_rt<3>();
+    case 4: return 64 * prec_rt<4>();
+    case 5: return 64 * prec_rt<5>();
+    case 6: return 64 * prec_rt<6>();
+    case 7: return 64 * prec_rt<7>();
+    default: return 64 * prec_rt<8>();
     }
 }
 
@@ -370,8 +406,8 @@ int prec_setup(Slot &s, hipStream_t st) {
 int prec_showdowns(Slot &s, int t, hipStream_t st) {
     const PrecState &p = *s.prec;
     hipLaunchKernelGGL(k_prec_split, dim3(p.S), dim3(SS_BLOCK), 0, st, s.d_desc, t);
-    hipLaunchKernelGGL(prec_gemm_kernel(p.S16 / 16), dim3((s.plan.Mp + 64 * PREC_RT - 1) / (64 * PREC_RT)),
-                       dim3(64 * PREC_WAVES), 0, st, s.d_desc, t);
+    hipLaunchKernelGGL(prec_gemm_kernel(p.S16 / 16), dim3((s.plan.Mp + 63) / 64), dim3(64 * PREC_WAVES), 0,
+                       st, s.d_desc, t);
     SS_HIP(hipGetLastError());
     return SS_OK;
 }
