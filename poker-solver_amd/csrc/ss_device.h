@@ -107,4 +107,15 @@ __host__ __device__ inline int combo_id(int a, int b) {
     return 51 * a - (a * (a - 1)) / 2 + (b - a - 1);
 }
 
+// load through the global address space (the pointers come out of a
+// SlotDesc, which the compiler would otherwise treat as generic)
+template <typename P>
+__device__ __forceinline__ P gld(const P *p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return *(const __attribute__((address_space(1))) P *)p;
+#else
+    return *p;   // host pass of the device code: never executed
+#endif
+}
+
 }  // namespace ss

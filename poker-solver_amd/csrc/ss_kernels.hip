@@ -235,17 +235,6 @@ __device__ __forceinline__ void root_sigma_items(const SlotDesc *__restrict__ s,
 #define DN 52
 #define DN2 (52 * 52)
 
-// load through the global address space (the pointers come out of a
-// SlotDesc, which the compiler would otherwise treat as generic)
-template <typename P>
-__device__ __forceinline__ P gld(const P *p) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    return *(const __attribute__((address_space(1))) P *)p;
-#else
-    return *p;   // host pass of the device code: never executed
-#endif
-}
-
 // element of a row addressed by 4 x its index (the byte tables store 4 x index)
 template <typename T>
 __device__ __forceinline__ T at4(const T *row, int i4) {

@@ -22,6 +22,8 @@ namespace ss {
 
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
+typedef __attribute__((__vector_size__(4 * sizeof(uint32_t)))) uint32_t u32x4;
+typedef __attribute__((__vector_size__(2 * sizeof(uint32_t)))) uint32_t u32x2;
 
 #define PREC_MAX_TILES 8   // S16 / 16 <= 8: at most 128 showdown nodes
 #define DN2_P (52 * 52)
@@ -90,27 +92,26 @@ __device__ __forceinline__ PrecArgs prec_args(const SlotDesc *__restrict__ s, in
 // order). One workgroup; M and C are LDS.
 __device__ void prec_split_node(const SlotDesc *__restrict__ s, const PrecArgs &a, int t, int b, float *M,
                                 float *C) {
-    const int n = a.sd_nodes[b], o = 1 - t, Mp = s->Mp;
+    const int n = a.sd_nodes[b], o = 1 - t, Mp = s->Mp, nh = s->nh[o];
     const float *r = s->reach + ((size_t)o * s->N + n) * Mp;
+    for (int i = threadIdx.x; i < DN2_P; i += blockDim.x) M[i] = 0.f;
+    __syncthreads();
+    // one pass over the opponent's hands: hi / lo split and the card-pair scatter
     for (int j = threadIdx.x; j < Mp; j += blockDim.x) {
-        const bool ok = j < s->nh[o] && (s->hinfo[(size_t)o * Mp + j] & SS_LIVE_BIT);
-        const float x = ok ? r[j] : 0.f;
+        const uint32_t hj = j < nh ? gld(s->hinfo + (size_t)o * Mp + j) : 0u;
+        const bool ok = hj & SS_LIVE_BIT;
+        const float x = ok ? gld(r + j) : 0.f;
         const uint16_t h = bf16_rne(x);
         a.Bt[(size_t)b * Mp + j] = h;
         a.Bt[(size_t)(a.S16 + b) * Mp + j] = bf16_rne(x - bf16_to_f(h));
-    }
-    for (int i = threadIdx.x; i < DN2_P; i += blockDim.x) M[i] = 0.f;
-    __syncthreads();
-    for (int j = threadIdx.x; j < s->nh[o]; j += blockDim.x) {
-        const uint32_t hj = s->hinfo[(size_t)o * Mp + j];
-        if (!(hj & SS_LIVE_BIT)) continue;
+        if (!ok) continue;
         const int c0 = hj & 63, c1 = (hj >> 6) & 63;
         if (s->uniq) {   // one writer per cell
-            M[c0 * 52 + c1] = r[j];
-            M[c1 * 52 + c0] = r[j];
+            M[c0 * 52 + c1] = x;
+            M[c1 * 52 + c0] = x;
         } else {
-            atomicAdd(&M[c0 * 52 + c1], r[j]);
-            atomicAdd(&M[c1 * 52 + c0], r[j]);
+            atomicAdd(&M[c0 * 52 + c1], x);
+            atomicAdd(&M[c1 * 52 + c0], x);
         }
     }
     __syncthreads();
@@ -122,11 +123,13 @@ __device__ void prec_split_node(const SlotDesc *__restrict__ s, const PrecArgs &
         if (lane == 0) C[c] = v;
     }
     __syncthreads();
-    if (threadIdx.x < 52) a.Cn[(size_t)b * 64 + threadIdx.x] = C[threadIdx.x];
-    if (threadIdx.x == 0) {
-        float tot = 0.f;
-        for (int c = 0; c < 52; ++c) tot += C[c];
-        a.Cn[(size_t)b * 64 + 52] = 0.5f * tot;
+    if (threadIdx.x < 64) {   // wavefront 0: the card totals and their half-sum
+        const float v = lane < 52 ? C[lane] : 0.f;
+        if (lane < 52) a.Cn[(size_t)b * 64 + lane] = v;
+        float tot = v;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off, 64);
+        if (lane == 0) a.Cn[(size_t)b * 64 + 52] = 0.5f * tot;
     }
     __syncthreads();
 }
@@ -199,29 +202,60 @@ __device__ void prec_gemm_block(const SlotDesc *__restrict__ s, const PrecArgs &
                                 PrecLds<NT, RT> &L) {
     using SH = PrecShape<NT, RT>;
     constexpr int KCH = SH::KCH, NCOL = 32 * NT;
-    const int tid = threadIdx.x, nt = blockDim.x, l = tid & 63, w = tid >> 6, Mp = s->Mp, o = 1 - t;
+    const int tid = threadIdx.x, l = tid & 63, w = tid >> 6, Mp = s->Mp, o = 1 - t;
     const int nq = Mp / 32, sh = 8 * (l >> 4), wrow0 = 16 * RT * w;   // this wavefront's first row in the block
     floatx4 acc[RT][2 * NT];
 #pragma unroll
     for (int r = 0; r < RT; ++r)
 #pragma unroll
         for (int q = 0; q < 2 * NT; ++q) acc[r][q] = floatx4{0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < Mp; k0 += KCH) {
-        const int kc = min(KCH, Mp - k0);   // a multiple of 32
-        // stage: the chunk of packed reach for every column (hi columns are
-        // Bt rows 0 .. 16 NT - 1, lo columns Bt rows S16 ..) and the W masks
-        // of the block's rows, coalesced along k
-        const int k8 = kc / 8, k32 = kc / 32;
-        for (int e = tid; e < NCOL * k8; e += nt) {
-            const int c = e / k8, j = e - c * k8;
-            const int brow = c < 16 * NT ? c : a.S16 + (c - 16 * NT);
-            *(uint4 *)&L.Bs[c][8 * j] = *(const uint4 *)(a.Bt + (size_t)brow * Mp + k0 + 8 * j);
+    // staging: the chunk of packed reach for every column (hi columns are Bt
+    // rows 0 .. 16 NT - 1, lo columns Bt rows S16 ..) and the W masks of the
+    // block's rows, coalesced along k. Each thread holds a fixed number of
+    // loads in registers; the next chunk's loads are issued before the
+    // current chunk's MFMAs so that their latency is hidden. Addresses are
+    // clamped into the arrays instead of branched on: a clamped element lands
+    // in an LDS column past the chunk's end (never read) or in a row past Mp
+    // (never written back).
+    constexpr int NTHR = 64 * PREC_WAVES;
+    constexpr int BE = NCOL * KCH / 8, BU = BE / NTHR;                 // uint4 of reach per chunk
+    constexpr int WE = SH::ROWS * KCH / 32, WU = (WE + NTHR - 1) / NTHR;   // uint2 of masks per chunk
+    static_assert(BE % NTHR == 0, "reach chunk must split evenly over the workgroup");
+    u32x4 rb[BU];   // native vectors: kept in registers across the MFMA loop
+    u32x2 rw[WU];
+    // pass k0 = -KCH only fetches chunk 0; pass k0 stores chunk k0 into LDS,
+    // fetches chunk k0 + KCH, then runs chunk k0's MFMAs
+    for (int k0 = -KCH; k0 < Mp; k0 += KCH) {
+        if (k0 >= 0) {
+#pragma unroll
+            for (int u = 0; u < BU; ++u) {
+                const int e = u * NTHR + tid;
+                *(u32x4 *)&L.Bs[e / (KCH / 8)][8 * (e % (KCH / 8))] = rb[u];
+            }
+#pragma unroll
+            for (int u = 0; u < WU; ++u) {
+                const int e = u * NTHR + tid;
+                if (WE % NTHR == 0 || e < WE) *(u32x2 *)&L.Ws[e / (KCH / 32)][e % (KCH / 32)] = rw[u];
+            }
+            __syncthreads();
         }
-        for (int e = tid; e < SH::ROWS * k32; e += nt) {
-            const int r = e / k32, j = e - r * k32, row = row0 + r;
-            L.Ws[r][j] = row < Mp ? a.W[(size_t)row * nq + (k0 >> 5) + j] : make_uint2(0u, 0u);
+        const int kf = k0 + KCH;
+        if (kf < Mp) {
+            const int k8 = min(KCH, Mp - kf) / 8, k32f = k8 / 4;
+#pragma unroll
+            for (int u = 0; u < BU; ++u) {
+                const int e = u * NTHR + tid, c = e / (KCH / 8), j = min(e % (KCH / 8), k8 - 1);
+                const int brow = c < 16 * NT ? c : a.S16 + (c - 16 * NT);
+                rb[u] = gld((const u32x4 *)(a.Bt + (size_t)brow * Mp + kf + 8 * j));
+            }
+#pragma unroll
+            for (int u = 0; u < WU; ++u) {
+                const int e = min(u * NTHR + tid, WE - 1), r = e / (KCH / 32), j = min(e % (KCH / 32), k32f - 1);
+                rw[u] = gld((const u32x2 *)&a.W[(size_t)min(row0 + r, Mp - 1) * nq + (kf >> 5) + j]);
+            }
         }
-        __syncthreads();
+        if (k0 < 0) continue;
+        const int k32 = min(KCH, Mp - k0) / 32;   // the chunk's 32-k steps
         for (int ks = 0; ks < k32; ++ks) {
             bf16x8 va[RT], vb[2 * NT];
 #pragma unroll
@@ -244,31 +278,65 @@ __device__ void prec_gemm_block(const SlotDesc *__restrict__ s, const PrecArgs &
         }
         __syncthreads();
     }
+    // epilogue. Per-column constants go to LDS, over the reach buffer (free
+    // after the last chunk's barrier): Cn's 53 values, then half the pot, the
+    // traverser's bet and the node. Per-row inputs are loaded as one batch,
+    // then per column the opponent reach of the hand holding the same cards,
+    // so a thread's loads overlap instead of waiting one after another.
+    constexpr int CSTRIDE = 65;   // padded a bank: 16 columns read the same card
+    float *Cs = reinterpret_cast<float *>(&L.Bs[0][0]);
+    static_assert(sizeof(L.Bs) >= sizeof(float) * 16 * NT * CSTRIDE, "column constants must fit the reach buffer");
+    for (int e = tid; e < 16 * NT * 64; e += NTHR) {
+        const int b = e >> 6, c = e & 63;
+        float v = 0.f;   // columns past S: zeros, node 0
+        if (b < a.S && c <= 52) v = gld(a.Cn + (size_t)b * 64 + c);
+        else if (b < a.S && c <= 55) {
+            const int n = gld(a.sd_nodes + b);
+            const DevNode &nd = s->nodes[n];
+            v = c == 53 ? 0.5f * (float)nd.pot : c == 54 ? (float)nd.bets[t] : __int_as_float(n);
+        }
+        Cs[b * CSTRIDE + c] = v;
+    }
+    __syncthreads();
+    const int nh = s->nh[t], rbase = row0 + wrow0 + 4 * (l >> 4);
     const float *ro = s->reach + (size_t)o * s->N * Mp;
 #pragma unroll
-    for (int q = 0; q < NT; ++q) {
-        const int b = 16 * q + (l & 15);
-        if (b >= a.S) continue;
-        const int n = a.sd_nodes[b];
-        const DevNode &nd = s->nodes[n];
-        const float half = 0.5f * (float)nd.pot, bet = (float)nd.bets[t];
-        const float *Cb = a.Cn + (size_t)b * 64;
-        const float Sb = Cb[52];
+    for (int r = 0; r < RT; ++r) {
+        uint32_t hv[4], cs[4];
+        int sv[4];
 #pragma unroll
-        for (int r = 0; r < RT; ++r) {
+        for (int e = 0; e < 4; ++e) {
+            const int ic = min(rbase + 16 * r + e, Mp - 1);
+            hv[e] = gld(s->hinfo + (size_t)t * Mp + ic);
+            sv[e] = gld(a.same + ic);
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) cs[e] = gld(s->cstr + min((int)((hv[e] >> 12) & 2047), 1325));
+#pragma unroll
+        for (int q = 0; q < NT; ++q) {
+            const int b = 16 * q + (l & 15);
+            const float *Cb = Cs + b * CSTRIDE;
+            const float Sb = Cb[52], half = Cb[53], bet = Cb[54];
+            const int n = __float_as_int(Cb[55]);
+            float rv[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) rv[e] = gld(ro + (size_t)n * Mp + max(sv[e], 0));
+            // the values are needed here: keeps the compiler from sinking each
+            // load into the guarded store below, one wait per load
+#pragma unroll
+            for (int e = 0; e < 4; ++e) asm volatile("" ::"v"(rv[e]));
+            if (b >= a.S) continue;
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                const int i = row0 + wrow0 + 16 * r + 4 * (l >> 4) + e;
-                if (i >= s->nh[t]) continue;
-                const uint32_t hi = s->hinfo[(size_t)t * Mp + i];
-                float v = 0.f;
-                if ((hi & SS_LIVE_BIT) && s->cstr[(hi >> 12) & 2047]) {              // hs = 0 -> 0, :861
-                    const int x = hi & 63, y = (hi >> 6) & 63, sm = a.same[i];
-                    const float z = Sb - Cb[x] - Cb[y] + (sm >= 0 ? ro[(size_t)n * Mp + sm] : 0.f);
-                    const float Y = acc[r][q][e] + acc[r][NT + q][e];
-                    v = z < 1e-5f * Sb ? 0.f : half * Y + (half - bet) * z;
-                }
-                s->cfv[(size_t)n * Mp + i] = v;
+                const int i = rbase + 16 * r + e;
+                if (i >= nh) continue;
+                // computed for every lane and selected (no branch, so no
+                // load is sunk into one); dead or blocked hands give 0, :861
+                const int x = hv[e] & 63, y = (hv[e] >> 6) & 63, sm = sv[e];
+                const float z = Sb - Cb[x] - Cb[y] + (sm >= 0 ? rv[e] : 0.f);
+                const float Y = acc[r][q][e] + acc[r][NT + q][e];
+                const bool zero = !((hv[e] & SS_LIVE_BIT) && cs[e]) || z < 1e-5f * Sb;
+                s->cfv[(size_t)n * Mp + i] = zero ? 0.f : half * Y + (half - bet) * z;
             }
         }
     }
