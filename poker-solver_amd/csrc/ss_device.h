@@ -117,5 +117,14 @@ __device__ __forceinline__ P gld(const P *p) {
     return *p;   // host pass of the device code: never executed
 #endif
 }
+// store through the global address space (as gld)
+template <typename P>
+__device__ __forceinline__ void gst(P *p, P v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    *(__attribute__((address_space(1))) P *)p = v;
+#else
+    *p = v;   // host pass of the device code: never executed
+#endif
+}
 
 }  // namespace ss

@@ -1190,13 +1190,93 @@ __device__ __forceinline__ bool slot_runs(const SlotDesc *__restrict__ s, int tr
 // Sweeps: one workgroup of SWEEP_BLOCK threads per (subgame, node) task, so a
 // 1081-combo node (Mp = 1088) takes two nearly full rounds of hands.
 #define SWEEP_BLOCK 576
-__global__ void __launch_bounds__(SWEEP_BLOCK) k_fwd_phased(SlotDesc *__restrict__ slots, const uint32_t *tasks,
-                                                            int ntasks, int trav) {
+
+// A task's node, loaded once before its hands: every address of the hand
+// loop then depends only on these and the hand, and the hand loop's loads are
+// issued unconditionally (indices clamped into the node's rows), so that they
+// go out together instead of one dependent round trip each.
+struct SweepNode {
+    int actor, na, act_base, fa;   // fa: frozen action or -1
+    int ch[NT_DEC];                // children, the last one repeated past na
+};
+
+__device__ __forceinline__ SweepNode sweep_node(const SlotDesc *__restrict__ s, int n) {
+    SweepNode k;
+    const DevNode *nd = s->nodes + n;
+    k.actor = gld(&nd->player);
+    k.na = gld(&nd->nchild);
+    k.act_base = gld(&nd->act_base);
+    const int *ch = s->children + gld(&nd->first_child);
+#pragma unroll
+    for (int a = 0; a < NT_DEC; ++a) k.ch[a] = gld(ch + min(a, k.na - 1));
+    k.fa = s->frozen ? gld(s->frozen + n) : -1;
+    return k;
+}
+
+// regret_match over preloaded regret rows r (same arithmetic, same order)
+__device__ __forceinline__ void regret_match_vals(const float *r, int na, int fa, float *sig) {
+    float sum = 0.f;
+#pragma unroll
+    for (int a = 0; a < NT_DEC; ++a) {
+        if (a < na) {
+            sig[a] = r[a] > 0.f ? r[a] : 0.f;
+            sum += sig[a];
+        }
+    }
+    if (sum > 0.f) {
+        float inv = 1.0f / sum;
+#pragma unroll
+        for (int a = 0; a < NT_DEC; ++a)
+            if (a < na) sig[a] *= inv;
+    } else {
+        float u = 1.0f / (float)na;
+#pragma unroll
+        for (int a = 0; a < NT_DEC; ++a)
+            if (a < na) sig[a] = u;
+    }
+    if (fa >= 0) {
+#pragma unroll
+        for (int a = 0; a < NT_DEC; ++a)
+            if (a < na) sig[a] = a == fa ? 1.f : 0.f;
+    }
+}
+
+// forward_one for a whole task (same results)
+__global__ void __launch_bounds__(SWEEP_BLOCK) k_fwd_phased(SlotDesc *__restrict__ slots,
+                                                            const uint32_t *__restrict__ tasks, int ntasks,
+                                                            int trav) {
     for (int j = blockIdx.x; j < ntasks; j += gridDim.x) {
-        uint32_t tk = tasks[j];
+        const uint32_t tk = tasks[j];
         const SlotDesc *s = slots + TASK_SLOT(tk);
         if (!slot_runs(s, trav)) continue;
-        for (int h = threadIdx.x; h < s->Mp; h += blockDim.x) forward_one(s, TASK_NODE(tk), h, trav);
+        const int n = TASK_NODE(tk), Mp = s->Mp, N = s->N, NP = s->NP;
+        const SweepNode k = sweep_node(s, n);
+        const float *R = s->R;
+        float *reach = s->reach;
+        int nh[SS_MAX_PLAYERS];
+#pragma unroll
+        for (int p = 0; p < SS_MAX_PLAYERS; ++p) nh[p] = s->nh[p];
+        const bool acting = k.actor != trav && k.actor >= 0 && k.actor < NP;
+        const int nh_act = acting ? nh[k.actor] : 0;
+        for (int h = threadIdx.x; h < Mp; h += blockDim.x) {
+            float r[NT_DEC], pr[SS_MAX_PLAYERS], sig[NT_DEC];
+            if (acting) {
+#pragma unroll
+                for (int a = 0; a < NT_DEC; ++a) r[a] = gld(R + (size_t)(k.act_base + min(a, k.na - 1)) * Mp + h);
+            }
+#pragma unroll
+            for (int rp = 0; rp < SS_MAX_PLAYERS; ++rp)
+                pr[rp] = gld(reach + ((size_t)min(rp, NP - 1) * N + n) * Mp + h);
+            if (acting && h < nh_act) regret_match_vals(r, k.na, k.fa, sig);
+#pragma unroll
+            for (int rp = 0; rp < SS_MAX_PLAYERS; ++rp) {
+                if (rp >= NP || rp == trav || h >= nh[rp]) continue;
+                float *rr = reach + (size_t)rp * N * Mp;
+#pragma unroll
+                for (int a = 0; a < NT_DEC; ++a)
+                    if (a < k.na) gst(rr + (size_t)k.ch[a] * Mp + h, rp == k.actor ? pr[rp] * sig[a] : pr[rp]);
+            }
+        }
     }
 }
 
@@ -1260,14 +1340,53 @@ __global__ void __launch_bounds__(SS_BLOCK) k_term_odo(SlotDesc *__restrict__ sl
     }
 }
 
-__global__ void __launch_bounds__(SWEEP_BLOCK) k_bwd_phased(SlotDesc *__restrict__ slots, const uint32_t *tasks,
-                                                            int ntasks, int trav) {
+// backward_one for a whole task (same results)
+__global__ void __launch_bounds__(SWEEP_BLOCK) k_bwd_phased(SlotDesc *__restrict__ slots,
+                                                            const uint32_t *__restrict__ tasks, int ntasks,
+                                                            int trav) {
     for (int j = blockIdx.x; j < ntasks; j += gridDim.x) {
-        uint32_t tk = tasks[j];
+        const uint32_t tk = tasks[j];
         const SlotDesc *s = slots + TASK_SLOT(tk);
         if (!slot_runs(s, trav)) continue;
-        int iter = s->iter + 1;
-        for (int h = threadIdx.x; h < s->Mp; h += blockDim.x) backward_one(s, TASK_NODE(tk), h, trav, iter);
+        const int iter = s->iter + 1, n = TASK_NODE(tk), Mp = s->Mp, nh = s->nh[trav];
+        const SweepNode k = sweep_node(s, n);
+        float *R = s->R, *SS = s->SS, *cfv = s->cfv;
+        const bool own = k.actor == trav;
+        const float d = (float)iter / ((float)iter + 1.0f);
+        for (int h = threadIdx.x; h < nh; h += blockDim.x) {
+            float cv[NT_DEC], r[NT_DEC], sg[NT_DEC], sig[NT_DEC];
+#pragma unroll
+            for (int a = 0; a < NT_DEC; ++a) cv[a] = gld(cfv + (size_t)k.ch[a] * Mp + h);
+            float v = 0.f;
+            if (own) {
+#pragma unroll
+                for (int a = 0; a < NT_DEC; ++a) {
+                    const size_t row = (size_t)(k.act_base + min(a, k.na - 1)) * Mp + h;
+                    r[a] = gld(R + row);
+                    sg[a] = gld(SS + row);
+                }
+                regret_match_vals(r, k.na, k.fa, sig);
+#pragma unroll
+                for (int a = 0; a < NT_DEC; ++a)
+                    if (a < k.na) v += sig[a] * cv[a];
+#pragma unroll
+                for (int a = 0; a < NT_DEC; ++a) {
+                    if (a < k.na) {
+                        const size_t row = (size_t)(k.act_base + a) * Mp + h;
+                        float x = r[a];
+                        x += cv[a] - v;
+                        x *= d;
+                        gst(R + row, x);
+                        gst(SS + row, sg[a] + (float)iter * sig[a]);
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int a = 0; a < NT_DEC; ++a)
+                    if (a < k.na) v += cv[a];
+            }
+            gst(cfv + (size_t)n * Mp + h, v);
+        }
     }
 }
 
