@@ -1258,15 +1258,26 @@ __global__ void __launch_bounds__(SWEEP_BLOCK) k_fwd_phased(SlotDesc *__restrict
         for (int p = 0; p < SS_MAX_PLAYERS; ++p) nh[p] = s->nh[p];
         const bool acting = k.actor != trav && k.actor >= 0 && k.actor < NP;
         const int nh_act = acting ? nh[k.actor] : 0;
-        for (int h = threadIdx.x; h < Mp; h += blockDim.x) {
+        int hend = 0;   // hands of the non-traversers whose reach moves
+#pragma unroll
+        for (int p = 0; p < SS_MAX_PLAYERS; ++p)
+            if (p < NP && p != trav) hend = max(hend, nh[p]);
+        for (int h = threadIdx.x; h < hend; h += blockDim.x) {
             float r[NT_DEC], pr[SS_MAX_PLAYERS], sig[NT_DEC];
+            // hands past a player's count (never used) read that player's last
+            // hand: a cache line the wavefront fetches anyway
             if (acting) {
+                const int hr = max(0, min(h, nh_act - 1));
 #pragma unroll
-                for (int a = 0; a < NT_DEC; ++a) r[a] = gld(R + (size_t)(k.act_base + min(a, k.na - 1)) * Mp + h);
+                for (int a = 0; a < NT_DEC; ++a) r[a] = gld(R + (size_t)(k.act_base + min(a, k.na - 1)) * Mp + hr);
             }
+            // rows past NP and the traverser's own (never used) repeat a
+            // non-traverser row, so no extra row is fetched
 #pragma unroll
-            for (int rp = 0; rp < SS_MAX_PLAYERS; ++rp)
-                pr[rp] = gld(reach + ((size_t)min(rp, NP - 1) * N + n) * Mp + h);
+            for (int rp = 0; rp < SS_MAX_PLAYERS; ++rp) {
+                const int rl = rp == trav || rp >= NP ? (trav == 0 ? 1 : 0) : rp;
+                pr[rp] = gld(reach + ((size_t)rl * N + n) * Mp + max(0, min(h, nh[rl] - 1)));
+            }
             if (acting && h < nh_act) regret_match_vals(r, k.na, k.fa, sig);
 #pragma unroll
             for (int rp = 0; rp < SS_MAX_PLAYERS; ++rp) {
