@@ -51,6 +51,18 @@ struct DevNode {
     int depth, pad;
 };
 
+// One decision-node task of the phased sweeps, filled on the host whenever
+// the task lists are built: every per-task value a sweep workgroup needs
+// besides the slot's progress, so that the workgroup starts with one load.
+struct alignas(16) SweepRec {
+    int slot, node, Mp, N, NP, actor, na, act_base, fa;   // fa: frozen action or -1
+    int ch[8];                                            // children, the last repeated past na
+    int nh[6];
+    int pad;
+    float *R, *SS, *cfv, *reach;
+};
+static_assert(sizeof(SweepRec) == 128, "SweepRec is two 64-byte scalar loads");
+
 struct SlotDesc {
     // shape
     int N, NP, Mp, maxh;

@@ -58,6 +58,7 @@ struct Plan {
     const float *leaf_src = nullptr;
     size_t leaf_count = 0;
     const int *frozen_src = nullptr;
+    std::vector<int> frozen;       // host copy of frozen_src (empty when none)
     int64_t algo_bytes_iter = 0;   // algorithmic bytes of one iteration
 };
 
@@ -132,6 +133,7 @@ struct PhasedTasks {
     std::vector<std::pair<int, int>> pgemm;   // (column tiles NT, task count) groups of (slot, row tile) tasks
     int pgemm_off = 0;
     uint32_t *d_tasks = nullptr;
+    SweepRec *d_recs = nullptr;    // one per decision task, [0, lvl_off[L]), same order as d_tasks
     ~PhasedTasks();
 };
 // tasks of the non-null slots with live[slot] != 0 (all of them when live is null)

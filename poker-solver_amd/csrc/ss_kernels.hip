@@ -1191,28 +1191,6 @@ __device__ __forceinline__ bool slot_runs(const SlotDesc *__restrict__ s, int tr
 // 1081-combo node (Mp = 1088) takes two nearly full rounds of hands.
 #define SWEEP_BLOCK 576
 
-// A task's node, loaded once before its hands: every address of the hand
-// loop then depends only on these and the hand, and the hand loop's loads are
-// issued unconditionally (indices clamped into the node's rows), so that they
-// go out together instead of one dependent round trip each.
-struct SweepNode {
-    int actor, na, act_base, fa;   // fa: frozen action or -1
-    int ch[NT_DEC];                // children, the last one repeated past na
-};
-
-__device__ __forceinline__ SweepNode sweep_node(const SlotDesc *__restrict__ s, int n) {
-    SweepNode k;
-    const DevNode *nd = s->nodes + n;
-    k.actor = gld(&nd->player);
-    k.na = gld(&nd->nchild);
-    k.act_base = gld(&nd->act_base);
-    const int *ch = s->children + gld(&nd->first_child);
-#pragma unroll
-    for (int a = 0; a < NT_DEC; ++a) k.ch[a] = gld(ch + min(a, k.na - 1));
-    k.fa = s->frozen ? gld(s->frozen + n) : -1;
-    return k;
-}
-
 // regret_match over preloaded regret rows r (same arithmetic, same order)
 __device__ __forceinline__ void regret_match_vals(const float *r, int na, int fa, float *sig) {
     float sum = 0.f;
@@ -1241,27 +1219,43 @@ __device__ __forceinline__ void regret_match_vals(const float *r, int na, int fa
     }
 }
 
-// forward_one for a whole task (same results)
+// Pins loaded values: the compiler may not move the code after it (the
+// slot's progress test) above the loads, so the hand data and the slot's
+// progress are fetched in one round trip.
+#define SWEEP_PIN(x) asm volatile("" ::"v"(x))
+
+// forward_one for a whole task (same results). The task's record carries the
+// node: its fields are read into scalars up front (uniform address, so scalar
+// loads), and the first hands' loads go out with the read of the slot's
+// progress.
 __global__ void __launch_bounds__(SWEEP_BLOCK) k_fwd_phased(SlotDesc *__restrict__ slots,
-                                                            const uint32_t *__restrict__ tasks, int ntasks,
+                                                            const SweepRec *__restrict__ recs, int ntasks,
                                                             int trav) {
     for (int j = blockIdx.x; j < ntasks; j += gridDim.x) {
-        const uint32_t tk = tasks[j];
-        const SlotDesc *s = slots + TASK_SLOT(tk);
-        if (!slot_runs(s, trav)) continue;
-        const int n = TASK_NODE(tk), Mp = s->Mp, N = s->N, NP = s->NP;
-        const SweepNode k = sweep_node(s, n);
-        const float *R = s->R;
-        float *reach = s->reach;
-        int nh[SS_MAX_PLAYERS];
+        const SweepRec *kr = recs + j;
+        const SlotDesc *s = slots + kr->slot;
+        // the slot's progress, read up front and tested after the first hand's
+        // loads are issued
+        const int s_act = s->active, s_it = s->iter, s_end = s->iter_end, s_np = s->NP;
+        const int n = kr->node, Mp = kr->Mp, N = kr->N, NP = kr->NP, na = kr->na, actor = kr->actor;
+        const int ab = kr->act_base, fa = kr->fa;
+        int ch[NT_DEC], nhv[SS_MAX_PLAYERS];
 #pragma unroll
-        for (int p = 0; p < SS_MAX_PLAYERS; ++p) nh[p] = s->nh[p];
-        const bool acting = k.actor != trav && k.actor >= 0 && k.actor < NP;
-        const int nh_act = acting ? nh[k.actor] : 0;
-        int hend = 0;   // hands of the non-traversers whose reach moves
+        for (int a = 0; a < NT_DEC; ++a) ch[a] = kr->ch[a];
 #pragma unroll
-        for (int p = 0; p < SS_MAX_PLAYERS; ++p)
-            if (p < NP && p != trav) hend = max(hend, nh[p]);
+        for (int p = 0; p < SS_MAX_PLAYERS; ++p) nhv[p] = kr->nh[p];
+        const float *R = kr->R;
+        float *reach = kr->reach;
+        const bool acting = actor != trav && actor >= 0 && actor < NP;
+        const int other = trav == 0 ? 1 : 0;   // a non-traverser
+        int nh_act = 0, hend = 0, nh_other = 0;   // hend: hands of the non-traversers whose reach moves
+#pragma unroll
+        for (int p = 0; p < SS_MAX_PLAYERS; ++p) {
+            nh_act = acting && p == actor ? nhv[p] : nh_act;
+            nh_other = p == other ? nhv[p] : nh_other;
+            hend = max(hend, p < NP && p != trav ? nhv[p] : 0);
+        }
+        bool checked = false;
         for (int h = threadIdx.x; h < hend; h += blockDim.x) {
             float r[NT_DEC], pr[SS_MAX_PLAYERS], sig[NT_DEC];
             // hands past a player's count (never used) read that player's last
@@ -1269,23 +1263,30 @@ __global__ void __launch_bounds__(SWEEP_BLOCK) k_fwd_phased(SlotDesc *__restrict
             if (acting) {
                 const int hr = max(0, min(h, nh_act - 1));
 #pragma unroll
-                for (int a = 0; a < NT_DEC; ++a) r[a] = gld(R + (size_t)(k.act_base + min(a, k.na - 1)) * Mp + hr);
+                for (int a = 0; a < NT_DEC; ++a) r[a] = gld(R + (size_t)(ab + min(a, na - 1)) * Mp + hr);
             }
             // rows past NP and the traverser's own (never used) repeat a
             // non-traverser row, so no extra row is fetched
 #pragma unroll
             for (int rp = 0; rp < SS_MAX_PLAYERS; ++rp) {
-                const int rl = rp == trav || rp >= NP ? (trav == 0 ? 1 : 0) : rp;
-                pr[rp] = gld(reach + ((size_t)rl * N + n) * Mp + max(0, min(h, nh[rl] - 1)));
+                const bool alias = rp == trav || rp >= NP;
+                const int rl = alias ? other : rp, nl = alias ? nh_other : nhv[rp];
+                pr[rp] = gld(reach + ((size_t)rl * N + n) * Mp + max(0, min(h, nl - 1)));
             }
-            if (acting && h < nh_act) regret_match_vals(r, k.na, k.fa, sig);
+            if (!checked) {
+#pragma unroll
+                for (int rp = 0; rp < SS_MAX_PLAYERS; ++rp) SWEEP_PIN(pr[rp]);
+                if (!((s_act != 0) & (s_it < s_end) & (trav < s_np))) break;   // slot_runs
+                checked = true;
+            }
+            if (acting && h < nh_act) regret_match_vals(r, na, fa, sig);
 #pragma unroll
             for (int rp = 0; rp < SS_MAX_PLAYERS; ++rp) {
-                if (rp >= NP || rp == trav || h >= nh[rp]) continue;
+                if (rp >= NP || rp == trav || h >= nhv[rp]) continue;
                 float *rr = reach + (size_t)rp * N * Mp;
 #pragma unroll
                 for (int a = 0; a < NT_DEC; ++a)
-                    if (a < k.na) gst(rr + (size_t)k.ch[a] * Mp + h, rp == k.actor ? pr[rp] * sig[a] : pr[rp]);
+                    if (a < na) gst(rr + (size_t)ch[a] * Mp + h, rp == actor ? pr[rp] * sig[a] : pr[rp]);
             }
         }
     }
@@ -1351,39 +1352,57 @@ __global__ void __launch_bounds__(SS_BLOCK) k_term_odo(SlotDesc *__restrict__ sl
     }
 }
 
-// backward_one for a whole task (same results)
+// backward_one for a whole task (same results), from the task's record as
+// the forward sweep
 __global__ void __launch_bounds__(SWEEP_BLOCK) k_bwd_phased(SlotDesc *__restrict__ slots,
-                                                            const uint32_t *__restrict__ tasks, int ntasks,
+                                                            const SweepRec *__restrict__ recs, int ntasks,
                                                             int trav) {
     for (int j = blockIdx.x; j < ntasks; j += gridDim.x) {
-        const uint32_t tk = tasks[j];
-        const SlotDesc *s = slots + TASK_SLOT(tk);
-        if (!slot_runs(s, trav)) continue;
-        const int iter = s->iter + 1, n = TASK_NODE(tk), Mp = s->Mp, nh = s->nh[trav];
-        const SweepNode k = sweep_node(s, n);
-        float *R = s->R, *SS = s->SS, *cfv = s->cfv;
-        const bool own = k.actor == trav;
-        const float d = (float)iter / ((float)iter + 1.0f);
+        const SweepRec *kr = recs + j;
+        const SlotDesc *s = slots + kr->slot;
+        // the slot's progress, read up front and tested after the first hand's
+        // loads are issued
+        const int s_act = s->active, s_it = s->iter, s_end = s->iter_end, s_np = s->NP;
+        const int n = kr->node, Mp = kr->Mp, na = kr->na, ab = kr->act_base, fa = kr->fa;
+        int ch[NT_DEC];
+#pragma unroll
+        for (int a = 0; a < NT_DEC; ++a) ch[a] = kr->ch[a];
+        const int nh = kr->nh[trav];   // trav < NP <= 6
+        float *R = kr->R, *SS = kr->SS, *cfv = kr->cfv;
+        const bool own = kr->actor == trav;
+        bool checked = false;
+        float d = 0.f;
+        int iter = 0;
         for (int h = threadIdx.x; h < nh; h += blockDim.x) {
             float cv[NT_DEC], r[NT_DEC], sg[NT_DEC], sig[NT_DEC];
 #pragma unroll
-            for (int a = 0; a < NT_DEC; ++a) cv[a] = gld(cfv + (size_t)k.ch[a] * Mp + h);
-            float v = 0.f;
+            for (int a = 0; a < NT_DEC; ++a) cv[a] = gld(cfv + (size_t)ch[a] * Mp + h);
             if (own) {
 #pragma unroll
                 for (int a = 0; a < NT_DEC; ++a) {
-                    const size_t row = (size_t)(k.act_base + min(a, k.na - 1)) * Mp + h;
+                    const size_t row = (size_t)(ab + min(a, na - 1)) * Mp + h;
                     r[a] = gld(R + row);
                     sg[a] = gld(SS + row);
                 }
-                regret_match_vals(r, k.na, k.fa, sig);
+            }
+            if (!checked) {
+#pragma unroll
+                for (int a = 0; a < NT_DEC; ++a) SWEEP_PIN(cv[a]);
+                if (!((s_act != 0) & (s_it < s_end) & (trav < s_np))) break;   // slot_runs
+                iter = s_it + 1;
+                d = (float)iter / ((float)iter + 1.0f);
+                checked = true;
+            }
+            float v = 0.f;
+            if (own) {
+                regret_match_vals(r, na, fa, sig);
 #pragma unroll
                 for (int a = 0; a < NT_DEC; ++a)
-                    if (a < k.na) v += sig[a] * cv[a];
+                    if (a < na) v += sig[a] * cv[a];
 #pragma unroll
                 for (int a = 0; a < NT_DEC; ++a) {
-                    if (a < k.na) {
-                        const size_t row = (size_t)(k.act_base + a) * Mp + h;
+                    if (a < na) {
+                        const size_t row = (size_t)(ab + a) * Mp + h;
                         float x = r[a];
                         x += cv[a] - v;
                         x *= d;
@@ -1394,7 +1413,7 @@ __global__ void __launch_bounds__(SWEEP_BLOCK) k_bwd_phased(SlotDesc *__restrict
             } else {
 #pragma unroll
                 for (int a = 0; a < NT_DEC; ++a)
-                    if (a < k.na) v += cv[a];
+                    if (a < na) v += cv[a];
             }
             gst(cfv + (size_t)n * Mp + h, v);
         }
@@ -1641,6 +1660,7 @@ int launch_batched_expl(SlotDesc *d_slots, int nslots, double *scratch, const in
 // ---- phased batched form ---------------------------------------------------
 PhasedTasks::~PhasedTasks() {
     if (d_tasks) (void)hipFree(d_tasks);
+    if (d_recs) (void)hipFree(d_recs);
 }
 
 // terminal cost class for scheduling: closed forms with 2 opponents at a
@@ -1758,9 +1778,37 @@ int build_phased_tasks(const std::vector<Slot *> &slots, PhasedTasks &pt, hipStr
             pt.pgemm.push_back({nt, (int)(tasks.size() - g0)});
         }
     }
+    // sweep records of the decision tasks (same order)
+    const size_t ndec = (size_t)pt.lvl_off[pt.L];
+    std::vector<SweepRec> recs(ndec);
+    for (size_t j = 0; j < ndec; ++j) {
+        const int si = (int)(tasks[j] >> 20), n = (int)(tasks[j] & 0xFFFFF);
+        const Slot &S = *slots[si];
+        const Plan &pl = S.plan;
+        const DevNode &nd = pl.nodes[n];
+        SweepRec &r = recs[j];   // value-initialised: padding is zero
+        r.slot = si;
+        r.node = n;
+        r.Mp = pl.Mp;
+        r.N = pl.N;
+        r.NP = pl.NP;
+        r.actor = nd.player;
+        r.na = nd.nchild;
+        r.act_base = nd.act_base;
+        r.fa = pl.frozen.empty() ? -1 : pl.frozen[n];
+        for (int a = 0; a < 8; ++a) r.ch[a] = pl.children[nd.first_child + std::min(a, nd.nchild - 1)];
+        for (int q = 0; q < 6; ++q) r.nh[q] = S.desc.nh[q];
+        r.R = S.desc.R;
+        r.SS = S.desc.SS;
+        r.cfv = S.desc.cfv;
+        r.reach = S.desc.reach;
+    }
     if (pt.d_tasks) { SS_HIP(hipFree(pt.d_tasks)); pt.d_tasks = nullptr; }
+    if (pt.d_recs) { SS_HIP(hipFree(pt.d_recs)); pt.d_recs = nullptr; }
     SS_HIP(hipMalloc(&pt.d_tasks, sizeof(uint32_t) * std::max<size_t>(1, tasks.size())));
+    SS_HIP(hipMalloc(&pt.d_recs, sizeof(SweepRec) * std::max<size_t>(1, ndec)));
     SS_HIP(hipMemcpyAsync(pt.d_tasks, tasks.data(), sizeof(uint32_t) * tasks.size(), hipMemcpyHostToDevice, st));
+    if (ndec) SS_HIP(hipMemcpyAsync(pt.d_recs, recs.data(), sizeof(SweepRec) * ndec, hipMemcpyHostToDevice, st));
     SS_HIP(hipStreamSynchronize(st));
     return SS_OK;
 }
@@ -1771,7 +1819,7 @@ int launch_phased_iteration(SlotDesc *d_slots, const PhasedTasks &pt, hipStream_
         for (int l = 0; l < pt.L; ++l) {
             int cnt = pt.lvl_off[l + 1] - pt.lvl_off[l];
             if (cnt > 0)
-                hipLaunchKernelGGL(k_fwd_phased, dim3(cnt), dim3(SWEEP_BLOCK), 0, st, d_slots, pt.d_tasks + pt.lvl_off[l],
+                hipLaunchKernelGGL(k_fwd_phased, dim3(cnt), dim3(SWEEP_BLOCK), 0, st, d_slots, pt.d_recs + pt.lvl_off[l],
                                    cnt, t);
         }
         if (pt.npsplit > 0) {
@@ -1794,7 +1842,7 @@ int launch_phased_iteration(SlotDesc *d_slots, const PhasedTasks &pt, hipStream_
         for (int l = pt.L - 1; l >= 0; --l) {
             int cnt = pt.lvl_off[l + 1] - pt.lvl_off[l];
             if (cnt > 0)
-                hipLaunchKernelGGL(k_bwd_phased, dim3(cnt), dim3(SWEEP_BLOCK), 0, st, d_slots, pt.d_tasks + pt.lvl_off[l],
+                hipLaunchKernelGGL(k_bwd_phased, dim3(cnt), dim3(SWEEP_BLOCK), 0, st, d_slots, pt.d_recs + pt.lvl_off[l],
                                    cnt, t);
         }
     }

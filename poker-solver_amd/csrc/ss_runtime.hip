@@ -240,6 +240,7 @@ int make_plan(const HostProblem &hp, Plan &pl) {
         pl.leaf_count = (size_t)pl.num_leaves * NP * maxh;
     }
     pl.frozen_src = hp.frozen;
+    if (pl.frozen_src) pl.frozen.assign(pl.frozen_src, pl.frozen_src + N);
     if (pl.frozen_src)
         for (int i = 0; i < N; ++i) {
             int fa = pl.frozen_src[i];
