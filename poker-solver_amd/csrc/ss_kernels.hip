@@ -586,27 +586,31 @@ __device__ __forceinline__ void pair_products_mfma(const float *M1, const float 
 template <typename T, int MODE, int SDK = -1>
 __device__ void terminal_node(const SlotDesc *__restrict__ s, int n, int trav, const T *reach, T *out,
                               TermBuf<T> B, int kind, const float *scale, int tid, int nt) {
-    const DevNode &nd = s->nodes[n];
-    const int Mp = s->Mp, NP = s->NP, nh = s->nh[trav];
-    const int type = kind == TK_MASS ? SS_NODE_FOLD : nd.type;
+    // the node's fields and the slot's shape, read together up front (not
+    // one dependent round trip per field where each is first used)
+    const DevNode *ndp = s->nodes + n;
+    const int nd_type = gld(&ndp->type), nd_player = gld(&ndp->player), nd_pot = gld(&ndp->pot);
+    const int nd_leaf = gld(&ndp->leaf_idx), nd_mask = gld(&ndp->active_mask), nd_bet = gld(&ndp->bets[trav]);
+    const int Mp = s->Mp, NP = s->NP, nh = s->nh[trav], is_river = s->is_river, has_leaf = s->has_leaf;
+    const int type = kind == TK_MASS ? SS_NODE_FOLD : nd_type;
     int opp[6], k = 0;
     for (int q = 0; q < NP; ++q) {
         if (q == trav) continue;
-        if (type != SS_NODE_FOLD && !((nd.active_mask >> q) & 1)) continue;   // :866-873, :1037-1044
+        if (type != SS_NODE_FOLD && !((nd_mask >> q) & 1)) continue;   // :866-873, :1037-1044
         opp[k++] = q;
     }
-    bool zero = (type == SS_NODE_SHOWDOWN && (!s->is_river || !((nd.active_mask >> trav) & 1))) ||
-                (type == SS_NODE_LEAF && (!s->has_leaf || nd.leaf_idx < 0));
+    const bool zero = (type == SS_NODE_SHOWDOWN && (!is_river || !((nd_mask >> trav) & 1))) ||
+                      (type == SS_NODE_LEAF && (!has_leaf || nd_leaf < 0));
     if (zero) {
         for (int h = tid; h < nh; h += nt) out[h] = 0;
         return;
     }
-    const T pot = (T)nd.pot, bet = (T)nd.bets[trav];
+    const T pot = (T)nd_pot, bet = (T)nd_bet;
+    const T fold_pay = trav == nd_player ? (T)(nd_pot - nd_bet) : -(T)nd_bet;   // :726-733
     auto factor = [&](int h) -> T {
         if (kind == TK_MASS) return (T)scale[h];
-        if (type == SS_NODE_FOLD)                                              // :726-733
-            return trav == nd.player ? (T)(nd.pot - nd.bets[trav]) : -(T)nd.bets[trav];
-        return (T)s->leaf_values[((size_t)nd.leaf_idx * NP + trav) * s->maxh + h];
+        if (type == SS_NODE_FOLD) return fold_pay;
+        return (T)s->leaf_values[((size_t)nd_leaf * NP + trav) * s->maxh + h];
     };
     if (MODE == TM_ALL && k >= 3) {
         for (int h = tid; h < nh; h += nt) {
@@ -625,6 +629,16 @@ __device__ void terminal_node(const SlotDesc *__restrict__ s, int n, int trav, c
     // ---- closed forms, k <= 2 ----
     const bool sd = SDK < 0 ? type == SS_NODE_SHOWDOWN : SDK == 1;
     const bool two = MODE != TM_LIGHT && k == 2;
+    // Folds and leaves walk the traverser's hands in order: this thread's
+    // first HPT hand records are loaded together now, so their latency
+    // overlaps the table setup instead of one round trip per hand.
+    constexpr int HPT = 5;
+    uint32_t hv[HPT];
+#pragma unroll
+    for (int u = 0; u < HPT; ++u) {
+        const int i = tid + u * nt;
+        hv[u] = !sd && i < nh ? gld(s->hinfo + (size_t)trav * Mp + i) : 0u;
+    }
     {
         const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
         float4 *m1 = (float4 *)B.M1, *m2 = (float4 *)B.M2;
@@ -670,7 +684,7 @@ __device__ void terminal_node(const SlotDesc *__restrict__ s, int n, int trav, c
     const uint4 *meta = sd ? s->hmeta + (size_t)trav * Mp : nullptr;
     uint4 mnext = make_uint4(0u, 0u, 0u, 0u);
     if (sd && tid < nh) mnext = gld(meta + tid);
-    for (int i = tid; i < nh; i += nt) {
+    for (int i = tid, u = 0; i < nh; i += nt, ++u) {
         const uint4 mh = mnext;
         if (sd && i + nt < nh) mnext = gld(meta + i + nt);
         int h, x, y;
@@ -680,7 +694,10 @@ __device__ void terminal_node(const SlotDesc *__restrict__ s, int n, int trav, c
             y = (mh.x >> 17) & 63;
         } else {
             h = i;
-            const uint32_t hi = s->hinfo[(size_t)trav * Mp + h];
+            uint32_t hi = hv[0];   // u is the same in every lane: a select chain, no indexing
+#pragma unroll
+            for (int q = 1; q < HPT; ++q) hi = u == q ? hv[q] : hi;
+            if (u >= HPT) hi = s->hinfo[(size_t)trav * Mp + h];
             x = hi & 63;
             y = (hi >> 6) & 63;
         }

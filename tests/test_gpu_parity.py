@@ -166,6 +166,52 @@ def test_phased_batch_with_three_or_more_opponents_equals_plain(monkeypatch):
     _close_state(S, ref["strat_sum"])
 
 
+def _frozen_for(tree, every=3):
+    """Frozen action at every `every`-th decision node (the last action, so
+    deeper lines with bets are frozen too); -1 elsewhere."""
+    td = tree.td
+    fz = np.full(tree.num_nodes, -1, np.int32)
+    dec = [i for i in range(tree.num_nodes) if td.nodes[i].type == ss_abi.SS_NODE_DECISION]
+    for j, i in enumerate(dec):
+        if j % every == 0:
+            fz[i] = td.nodes[i].num_children - 1
+    return fz
+
+
+def test_frozen_actions_vs_oracle_and_batched(monkeypatch):
+    """A3 freezing (street_solve.cu:621-644): at a node with a frozen action
+    the acting player's strategy is one-hot while its regrets keep updating.
+    The plain form matches the FP64 oracle under the same freezing; both
+    batched strategies and the phased form equal the plain form bit for bit."""
+    sp = wl.small_spot(21, num_players=3, hands_per_player=7, bet_sizes=(0.5, 1.0), max_raises=2)
+    base = ss_api.Problem.from_spot(sp)
+    fz = _frozen_for(base.tree)
+    assert (fz >= 0).sum() >= 3
+    prob = ss_api.Problem(base.tree, sp.hands, sp.weights, leaf_values=base.leaf_values, frozen=fz)
+    oprob = ob.Problem(prob.tree.td, sp.board, sp.hands, sp.weights, leaf_values=prob.leaf_values, frozen=fz)
+    for k in (1, 2, 3):
+        R, S, ev = ss_api.solve_state(prob, k, 0)
+        ref = ob.solve(oprob, k, fp64=True, mode=ob.TERM_BRUTE, want_state=True)
+        flips = 0.0 if k == 1 else 0.02
+        _close_state(R, ref["regrets"], flip_frac=flips)
+        _close_state(S, ref["strat_sum"], flip_frac=flips)
+        # strategy sums at frozen nodes put all weight on the frozen action
+        for i in np.nonzero(fz >= 0)[0]:
+            others = np.delete(S[i, :base.tree.td.nodes[i].num_children], fz[i], axis=0)
+            assert not others.any(), i
+    want = ss_api.solve_plain(prob, 10, flags=1)
+    for mode in ("persistent", "phased"):
+        monkeypatch.setenv("SS_BATCH_MODE", mode)
+        other = ss_api.Problem.from_spot(wl.small_spot(22, num_players=2, hands_per_player=9))
+        b = ss_api.Batch([other, prob], 10, flags=1)
+        reps, st = b.run()
+        assert st.mode == (0 if mode == "persistent" else 1)
+        got = b.output(1)
+        assert np.array_equal(got.avg, want.avg), mode
+        assert np.array_equal(got.root_strategy, want.root_strategy), mode
+        b.close()
+
+
 def test_state_cfg1_shape_full_ranges():
     sp = wl.river_2p()
     prob, oprob = _spot_problem(sp)
