@@ -267,6 +267,7 @@ template <> __device__ __forceinline__ double mass_floor<double>() { return 1e-1
 template <typename T>
 __device__ __forceinline__ T tsum52(const T *C) {
     T s = 0;
+#pragma unroll 4
     for (int c = 0; c < 52; ++c) s += C[c];
     return s;
 }
@@ -674,6 +675,7 @@ __device__ void terminal_node(const SlotDesc *__restrict__ s, int n, int trav, c
         if (!sd) {
             if constexpr (std::is_same<T, float>::value) TT = B.red[0];   // from the MFMA padding
             else
+#pragma unroll 4
                 for (int c = 0; c < 52; ++c) TT += B.C1[c] * B.C2[c];
         }
     }
