@@ -272,23 +272,51 @@ __device__ __forceinline__ T tsum52(const T *C) {
     return s;
 }
 
-// inclusive prefix sum across the 64 lanes of a wavefront
+// Cross-lane moves by DPP (data-parallel primitives): a lane whose source is
+// outside the pattern, or whose row is masked off, reads 0. CTRL 0x110 + n is
+// row_shr:n (within 16-lane rows), 0x142 row_bcast:15 (lane 15 of each row
+// into the next row), 0x143 row_bcast:31 (lane 31 into rows 2 and 3).
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp0(float x) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, ROWS, 0xf, false));
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp0(double x) {
+    const long long v = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWS, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(v >> 32), CTRL, ROWS, 0xf, false);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+__device__ __forceinline__ float lane_value(float x, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+__device__ __forceinline__ double lane_value(double x, int l) {
+    const long long v = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane((int)v, l), hi = __builtin_amdgcn_readlane((int)(v >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// inclusive prefix sum across the 64 lanes of a wavefront (every lane
+// active): prefix within each 16-lane row by shifts of 1, 2, 4, 8, then the
+// row totals added into the rows above. Fixed order, so every form rounds
+// alike.
 template <typename T>
 __device__ __forceinline__ T wave_incl_scan(T x, int lane) {
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        T y = __shfl_up(x, off, 64);
-        if (lane >= off) x += y;
-    }
+    (void)lane;
+    x += dpp0<0x111, 0xf>(x);
+    x += dpp0<0x112, 0xf>(x);
+    x += dpp0<0x114, 0xf>(x);
+    x += dpp0<0x118, 0xf>(x);
+    x += dpp0<0x142, 0xa>(x);
+    x += dpp0<0x143, 0xc>(x);
     return x;
 }
 
-// sum across the 64 lanes of a wavefront, fixed butterfly order
+// sum across the 64 lanes of a wavefront (every lane active), the same value
+// in every lane: the last lane of the inclusive scan
 template <typename T>
 __device__ __forceinline__ T wave_sum(T x) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
-    return x;
+    return lane_value(wave_incl_scan(x, 0), 63);
 }
 
 // exclusive scans over the strength-sorted live combos: PG[0..ng] of M1 and,
@@ -471,16 +499,20 @@ __device__ __forceinline__ void card_totals(const T *M1, const T *M2, T *C1, T *
 template <typename T>
 __device__ __forceinline__ void card_prefix(const uint8_t *clist, const uint8_t *clen, const T *M1, const T *M2,
                                             T *PC1, T *PC2, T *PCP, T *C1, T *C2, T *CP, int tid, int nt) {
+    // gather without branches (a clamped list entry is always a card), so
+    // the unrolled iterations' LDS reads overlap
+#pragma unroll 3
     for (int i = tid; i < DN2; i += nt) {
-        int c = i / DN, j = i - c * DN;
-        T a = 0, b = 0;
-        if (j > 0 && j - 1 < clen[c]) {
-            int d = clist[c * DN + j - 1];
-            a = M1[c * DN + d];
-            if (M2) b = M2[c * DN + d];
-        }
+        const int c = i / DN, j = i - c * DN;
+        const bool ok = j > 0 && j - 1 < clen[c];
+        const int d = clist[c * DN + max(j - 1, 0)];
+        const T a = ok ? M1[c * DN + d] : (T)0;
         PC1[i] = a;
-        if (M2) { PC2[i] = b; PCP[i] = a * b; }
+        if (M2) {
+            const T b = ok ? M2[c * DN + d] : (T)0;
+            PC2[i] = b;
+            PCP[i] = a * b;
+        }
     }
     __syncthreads();
     const int lane = tid & 63, nw = nt >> 6;
@@ -489,7 +521,7 @@ __device__ __forceinline__ void card_prefix(const uint8_t *clist, const uint8_t 
         T a = lane < 51 ? PC1[at] : (T)0;
         a = wave_incl_scan(a, lane);
         if (lane < 51) PC1[at] = a;
-        T ta = __shfl(a, 50, 64);
+        T ta = lane_value(a, 50);
         if (lane == 0) C1[c] = ta;
         if (M2) {
             T b = lane < 51 ? PC2[at] : (T)0;
@@ -497,7 +529,7 @@ __device__ __forceinline__ void card_prefix(const uint8_t *clist, const uint8_t 
             b = wave_incl_scan(b, lane);
             p = wave_incl_scan(p, lane);
             if (lane < 51) { PC2[at] = b; PCP[at] = p; }
-            T tb = __shfl(b, 50, 64), tp = __shfl(p, 50, 64);
+            T tb = lane_value(b, 50), tp = lane_value(p, 50);
             if (lane == 0) { C2[c] = tb; CP[c] = tp; }
         }
     }
