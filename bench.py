@@ -183,8 +183,9 @@ def served_bench(n, rate, seed, target, check_every, max_iters, iters_per_step, 
 
 # ---------------------------------------------------------------- precision form
 def _fp64_reference(args):
-    """FP64 oracle solve of one config-1-shaped board: average strategy and
-    root values after `iters` iterations."""
+    """FP64 oracle solve of one 2-player 1081-combo river (pot 100 BB, stacks
+    400 BB, sizes 0.5 and 1 pot): average strategy and root values after
+    `iters` iterations."""
     seed, iters = args
     sys.path.insert(0, os.path.join(REPO, "tests"))
     import oracle_bind as ob
@@ -198,11 +199,12 @@ def _fp64_reference(args):
 
 
 def precision_bench(boards, iters, ref_boards, cores):
-    """Config 3: `boards` 2-player river subgames (config-1 shape, seeds
-    base + b), `iters` iterations each in the precision form (BF16 hi/lo MFMA
-    showdown, FP32 accumulation). Errors of the average strategy and of the
-    root values against the FP64 oracle on the first `ref_boards` boards, with
-    the FP32 closed form's errors on the same boards beside them."""
+    """Precision form: `boards` 2-player 1081-combo river subgames (pot 100 BB,
+    stacks 400 BB, sizes 0.5 and 1 pot, seeds base + b), `iters` iterations
+    each with the BF16 hi/lo MFMA showdown and FP32 accumulation. Errors of
+    the average strategy and of the root values against the FP64 oracle on
+    the first `ref_boards` boards, with the FP32 closed form's errors on the
+    same boards beside them."""
     import ss_abi
     import ss_api
     import workloads as wl

@@ -43,8 +43,8 @@ def card(s):
 # (name, board, pot, stack, bet_sizes, num_players, acting_order, use_cont)
 TREE_SPOTS = [
     ("ref_test_river_2p", "Qs As 2d 7h 4c", 1000, 5000, [0.33, 0.75], 2, [0, 1], 0),
-    ("cfg1_shape_river_2p", "Kd 7c 2h 9s 3d", 10000, 40000, [0.5, 1.0], 2, [0, 1], 0),
-    ("cfg2_shape_river_3p", "Kd 7c 2h 9s 3d", 15000, 50000, [0.33, 0.75], 3, [0, 1, 2], 0),
+    ("river_2p_100bb_sizes_50_100", "Kd 7c 2h 9s 3d", 10000, 40000, [0.5, 1.0], 2, [0, 1], 0),
+    ("river_3p_150bb_sizes_33_75", "Kd 7c 2h 9s 3d", 15000, 50000, [0.33, 0.75], 3, [0, 1, 2], 0),
     ("river_3p_order_201", "Ah Kh 2c 5d 9s", 3000, 9000, [0.5], 3, [2, 0, 1], 0),
     ("river_2p_order_10", "Ah Kh 2c 5d 9s", 1000, 5000, [0.75, 1.5], 2, [1, 0], 0),
     ("river_4p_one_size", "Ah Kh 2c 5d 9s", 2000, 6000, [1.0], 4, [0, 1, 2, 3], 0),

@@ -212,7 +212,7 @@ def test_frozen_actions_vs_oracle_and_batched(monkeypatch):
         b.close()
 
 
-def test_state_cfg1_shape_full_ranges():
+def test_state_river_2p_full_ranges():
     sp = wl.river_2p()
     prob, oprob = _spot_problem(sp)
     R, S, ev = ss_api.solve_state(prob, 3, 0)
@@ -221,7 +221,7 @@ def test_state_cfg1_shape_full_ranges():
     _close_state(S, ref["strat_sum"], rel=5e-5)
 
 
-def test_state_cfg2_shape_full_ranges():
+def test_state_river_3p_sparse_ranges():
     sp = wl.river_3p()
     prob, oprob = _spot_problem(sp)
     R, S, ev = ss_api.solve_state(prob, 1, 0)
@@ -252,7 +252,7 @@ def test_gpu_exploitability_matches_oracle_on_gpu_solution(npl):
 
 
 @pytest.mark.parametrize("iters", [50, 200])
-def test_long_run_exploitability_cfg1_vs_fp64_oracle(iters):
+def test_long_run_exploitability_river_2p_vs_fp64_oracle(iters):
     """Exploitability of the GPU's FP32 solve vs the FP64 oracle at equal
     iterations. At 50 iterations within 1% relative. Near convergence the FP32
     rounding of any implementation moves exploitability by more than 1% (the

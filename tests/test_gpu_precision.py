@@ -34,7 +34,7 @@ def test_precision_state_small_vs_brute_oracle(seed):
     _close_state(S, ref["strat_sum"], rel=5e-5)
 
 
-def test_precision_state_cfg1_shape_vs_fp64_oracle():
+def test_precision_state_river_2p_vs_fp64_oracle():
     sp = wl.river_2p()
     prob, oprob = _spot_problem(sp)
     R, S, ev = ss_api.solve_state(prob, 1, PREC)
@@ -47,7 +47,7 @@ def test_precision_state_cfg1_shape_vs_fp64_oracle():
     _close_state(S3, ref3["strat_sum"], rel=5e-5, flip_frac=0.02)
 
 
-def test_precision_exploitability_cfg1_vs_fp64_oracle():
+def test_precision_exploitability_river_2p_vs_fp64_oracle():
     """50 iterations: the solution's FP64 exploitability within 1% of the FP64
     oracle's at equal iterations (the FP32 closed form meets the same bound)."""
     sp = wl.river_2p()

@@ -67,7 +67,7 @@ def test_oracle_builder_matches_reference(t):
 
 def test_tree_sizes_at_two_raises():
     """Counts at max 2 raises (configs 1 and 2) from an independent count of
-    the builder rules: cfg1 51/18 decision/17 showdown/16 fold, cfg2 373/156/154/63."""
+    the builder rules: the 2-player 100 BB river with sizes 0.5 and 1 pot has 51 nodes, 18 decision, 17 showdown, 16 fold; the 3-player 150 BB river with 0.33 and 0.75 has 373 / 156 / 154 / 63."""
     for board, pot, stack, sizes, np_, want in [
         ([50, 23, 4, 30, 5], 10000, 40000, [0.5, 1.0], 2, (51, 18, 17, 16)),
         ([50, 23, 4, 30, 5], 15000, 50000, [0.33, 0.75], 3, (373, 156, 154, 63)),
