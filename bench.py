@@ -118,13 +118,18 @@ def cpu_reference_baseline(cores, iters_per_solve=500):
 
 
 # ---------------------------------------------------------------- served form
-def served_bench(n, rate, seed, target, check_every, max_iters, iters_per_step, log_every=50):
+def served_bench(n, rate, seed, target, check_every, max_iters, iters_per_step, log_every=50, plain_fifo=False):
     """Poisson stream through ss_server_*: 70% rivers (2-3 players, as the
     batch mix) and 30% turns (2-3 players, continuation leaves N(0, 0.3 pot)).
     Requests are built before the clock starts and submitted at their arrival
     times from a separate thread; latency = arrival to retirement (measured
     inside the server), throughput = completed / (last retirement - first
-    arrival)."""
+    arrival).
+
+    plain_fifo: the same stream through a plain-GPU request loop as well, one
+    request per solve in arrival order: each request's plain-form solve is
+    timed (setup, upload, solve, checks) and its latency follows the
+    single-server queue, finish_i = max(arrival_i, finish_{i-1}) + service_i."""
     import threading
     import ss_api
     import workloads as wl
@@ -164,7 +169,7 @@ def served_bench(n, rate, seed, target, check_every, max_iters, iters_per_step, 
     conv = np.array([done[i].converged for i in range(n)])
     first_arrival = stream[0][0]
     span = (t_last - t_start) - first_arrival
-    return {
+    out = {
         "workload": f"Poisson arrivals at {rate}/s, {n} requests: 70% river (players U{{2,3}}, batch-mix "
                     f"shapes), 30% turn (4-card board, 1128 combos, players U{{2,3}}, 1-2 sizes, "
                     f"continuation leaves N(0, 0.3 pot)); retire at <= {target}% pot checked every "
@@ -179,6 +184,26 @@ def served_bench(n, rate, seed, target, check_every, max_iters, iters_per_step, 
         "converged_fraction": float(conv.mean()),
         "iters_per_step": iters_per_step,
     }
+    if plain_fifo:
+        service = []
+        for i, pr in enumerate(probs):
+            r = ss_api.solve_plain(pr, max_iters, target=target, check_every=check_every)
+            service.append(r.elapsed_ms)
+            if (i + 1) % log_every == 0:
+                log(f"[bench] plain request loop {i + 1}/{n}")
+        finish, lat_p = 0.0, []
+        for (t_arr, _), sv in zip(stream, service):
+            finish = max(1000.0 * t_arr, finish) + sv
+            lat_p.append(finish - 1000.0 * t_arr)
+        lat_p = np.array(lat_p)
+        out["plain_request_loop"] = {
+            "latency_ms": {"p50": round(float(np.percentile(lat_p, 50)), 2),
+                           "p99": round(float(np.percentile(lat_p, 99)), 2),
+                           "max": round(float(lat_p.max()), 2)},
+            "solves_per_s": round(n / ((finish - 1000.0 * first_arrival) / 1000.0), 3),
+            "mean_service_ms": round(float(np.mean(service)), 2),
+        }
+    return out
 
 
 # ---------------------------------------------------------------- precision form
@@ -402,9 +427,12 @@ def main():
     if a.served_requests > 0:
         log(f"[bench] served form: {a.served_requests} requests at {a.served_rate}/s")
         line["served"] = served_bench(a.served_requests, a.served_rate, 20261203 + 5000, a.target,
-                                      a.check_every, a.max_iters, a.served_iters_per_step)
+                                      a.check_every, a.max_iters, a.served_iters_per_step, plain_fifo=True)
         log(f"[bench] served: {line['served']['solves_per_s']} solves/s, "
             f"p50 {line['served']['latency_ms']['p50']} ms, p99 {line['served']['latency_ms']['p99']} ms")
+        pl = line["served"]["plain_request_loop"]
+        log(f"[bench] plain request loop on the same stream: {pl['solves_per_s']} solves/s, "
+            f"p50 {pl['latency_ms']['p50']} ms, p99 {pl['latency_ms']['p99']} ms")
         log(f"[bench] served form capacity: the same {a.served_requests} requests submitted at once")
         cap = served_bench(a.served_requests, 1e9, 20261203 + 5000, a.target, a.check_every, a.max_iters,
                            a.served_iters_per_step)
