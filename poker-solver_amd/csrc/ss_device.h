@@ -58,7 +58,7 @@ struct alignas(16) SweepRec {
     int slot, node, Mp, N, NP, actor, na, act_base, fa;   // fa: frozen action or -1
     int ch[8];                                            // children, the last repeated past na
     int nh[6];
-    int pad;
+    int arow;                                             // row of the actor's reach at the node
     float *R, *SS, *cfv, *reach;
 };
 static_assert(sizeof(SweepRec) == 128, "SweepRec is two 64-byte scalar loads");
@@ -87,6 +87,7 @@ struct SlotDesc {
     const float *weights;
     const float *leaf_values;
     const int *frozen;
+    const int *rrow;           // [NP][N] row of player p's reach at node n (reach is not copied down the tree)
     float *R;
     float *SS;
     float *cfv;

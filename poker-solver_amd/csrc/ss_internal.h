@@ -59,6 +59,7 @@ struct Plan {
     size_t leaf_count = 0;
     const int *frozen_src = nullptr;
     std::vector<int> frozen;       // host copy of frozen_src (empty when none)
+    std::vector<int> rrow;         // [NP][N] reach row of player p at node n (see make_plan)
     int64_t algo_bytes_iter = 0;   // algorithmic bytes of one iteration
 };
 

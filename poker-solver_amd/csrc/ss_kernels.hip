@@ -141,21 +141,21 @@ __device__ __forceinline__ void regret_match(const float *__restrict__ R, int ro
 // ---------------------------------------------------------------------------
 // forward: reach of every non-traverser through decision node n, hand h
 // ---------------------------------------------------------------------------
+// The reach of a player who does not act at n passes to the children
+// unchanged and is found through the reach-row table (rrow), so only the
+// actor's child rows are written, and nothing when the traverser acts.
 __device__ __forceinline__ void forward_one(const SlotDesc *__restrict__ s, int n, int h, int trav) {
     const int Mp = s->Mp, N = s->N, NP = s->NP;
     const DevNode &nd = s->nodes[n];
     int actor = nd.player, na = nd.nchild, fc = nd.first_child;
+    if (actor == trav || actor < 0 || actor >= NP || h >= s->nh[actor]) return;
     float sig[NT_DEC];
-    if (actor != trav && actor >= 0 && actor < NP && h < s->nh[actor])
-        regret_match(s->R, nd.act_base, na, Mp, h, s->frozen, n, sig);
-    for (int rp = 0; rp < NP; ++rp) {
-        if (rp == trav || h >= s->nh[rp]) continue;
-        float *rr = s->reach + (size_t)rp * N * Mp;
-        float pr = rr[(size_t)n * Mp + h];
-        for (int a = 0; a < na; ++a) {
-            int c = s->children[fc + a];
-            rr[(size_t)c * Mp + h] = rp == actor ? pr * sig[a] : pr;
-        }
+    regret_match(s->R, nd.act_base, na, Mp, h, s->frozen, n, sig);
+    float *rr = s->reach + (size_t)actor * N * Mp;
+    const float pr = rr[(size_t)s->rrow[(size_t)actor * N + n] * Mp + h];
+    for (int a = 0; a < na; ++a) {
+        int c = s->children[fc + a];
+        rr[(size_t)c * Mp + h] = pr * sig[a];
     }
 }
 
@@ -413,7 +413,7 @@ __device__ T odometer(const SlotDesc *__restrict__ s, int n, int trav, int h, co
             uint64_t m = (1ull << (oi & 63)) | (1ull << ((oi >> 6) & 63));
             if (used & m) { ok = false; break; }
             used |= m;
-            prod *= reach[((size_t)q * N + n) * Mp + idx[i]];
+            prod *= reach[((size_t)q * N + s->rrow[(size_t)q * N + n]) * Mp + idx[i]];
             if (showdown) {
                 uint32_t os = s->cstr[(oi >> 12) & 2047];
                 if (os > best) { best = os; ties = 0; }
@@ -445,7 +445,7 @@ __device__ __forceinline__ void scatter(const SlotDesc *__restrict__ s, int q, i
                                         int tid, int nt) {
     const int Mp = s->Mp, nq = s->nh[q];
     const uint32_t *hq = s->hinfo + (size_t)q * Mp;
-    const T *rq = reach + ((size_t)q * s->N + n) * Mp;
+    const T *rq = reach + ((size_t)q * s->N + gld(s->rrow + (size_t)q * s->N + n)) * Mp;
     for (int o0 = tid; o0 < nq; o0 += 4 * nt) {
         uint32_t oi[4];
         T r[4];
@@ -1277,68 +1277,43 @@ __device__ __forceinline__ void regret_match_vals(const float *r, int na, int fa
 
 // forward_one for a whole task (same results). The task's record carries the
 // node: its fields are read into scalars up front (uniform address, so scalar
-// loads), and the first hands' loads go out with the read of the slot's
-// progress.
+// loads), and the first hand's loads go out with the read of the slot's
+// progress. Only the actor's child rows are written (reach-row table), so a
+// task whose actor is the traverser has nothing to do.
 __global__ void __launch_bounds__(SWEEP_BLOCK) k_fwd_phased(SlotDesc *__restrict__ slots,
                                                             const SweepRec *__restrict__ recs, int ntasks,
                                                             int trav) {
     for (int j = blockIdx.x; j < ntasks; j += gridDim.x) {
         const SweepRec *kr = recs + j;
+        const int actor = kr->actor, NP = kr->NP;
+        if (actor == trav || actor < 0 || actor >= NP) continue;
         const SlotDesc *s = slots + kr->slot;
         // the slot's progress, read up front and tested after the first hand's
         // loads are issued
         const int s_act = s->active, s_it = s->iter, s_end = s->iter_end, s_np = s->NP;
-        const int n = kr->node, Mp = kr->Mp, N = kr->N, NP = kr->NP, na = kr->na, actor = kr->actor;
-        const int ab = kr->act_base, fa = kr->fa;
-        int ch[NT_DEC], nhv[SS_MAX_PLAYERS];
+        const int Mp = kr->Mp, N = kr->N, na = kr->na, ab = kr->act_base, fa = kr->fa;
+        int ch[NT_DEC];
 #pragma unroll
         for (int a = 0; a < NT_DEC; ++a) ch[a] = kr->ch[a];
-#pragma unroll
-        for (int p = 0; p < SS_MAX_PLAYERS; ++p) nhv[p] = kr->nh[p];
+        const int nh_act = kr->nh[actor];
         const float *R = kr->R;
-        float *reach = kr->reach;
-        const bool acting = actor != trav && actor >= 0 && actor < NP;
-        const int other = trav == 0 ? 1 : 0;   // a non-traverser
-        int nh_act = 0, hend = 0, nh_other = 0;   // hend: hands of the non-traversers whose reach moves
-#pragma unroll
-        for (int p = 0; p < SS_MAX_PLAYERS; ++p) {
-            nh_act = acting && p == actor ? nhv[p] : nh_act;
-            nh_other = p == other ? nhv[p] : nh_other;
-            hend = max(hend, p < NP && p != trav ? nhv[p] : 0);
-        }
+        float *ra = kr->reach + (size_t)actor * N * Mp;
+        const float *par = ra + (size_t)kr->arow * Mp;
         bool checked = false;
-        for (int h = threadIdx.x; h < hend; h += blockDim.x) {
-            float r[NT_DEC], pr[SS_MAX_PLAYERS], sig[NT_DEC];
-            // hands past a player's count (never used) read that player's last
-            // hand: a cache line the wavefront fetches anyway
-            if (acting) {
-                const int hr = max(0, min(h, nh_act - 1));
+        for (int h = threadIdx.x; h < nh_act; h += blockDim.x) {
+            float r[NT_DEC], sig[NT_DEC];
 #pragma unroll
-                for (int a = 0; a < NT_DEC; ++a) r[a] = gld(R + (size_t)(ab + min(a, na - 1)) * Mp + hr);
-            }
-            // rows past NP and the traverser's own (never used) repeat a
-            // non-traverser row, so no extra row is fetched
-#pragma unroll
-            for (int rp = 0; rp < SS_MAX_PLAYERS; ++rp) {
-                const bool alias = rp == trav || rp >= NP;
-                const int rl = alias ? other : rp, nl = alias ? nh_other : nhv[rp];
-                pr[rp] = gld(reach + ((size_t)rl * N + n) * Mp + max(0, min(h, nl - 1)));
-            }
+            for (int a = 0; a < NT_DEC; ++a) r[a] = gld(R + (size_t)(ab + min(a, na - 1)) * Mp + h);
+            const float pr = gld(par + h);
             if (!checked) {
-#pragma unroll
-                for (int rp = 0; rp < SS_MAX_PLAYERS; ++rp) SWEEP_PIN(pr[rp]);
+                SWEEP_PIN(pr);
                 if (!((s_act != 0) & (s_it < s_end) & (trav < s_np))) break;   // slot_runs
                 checked = true;
             }
-            if (acting && h < nh_act) regret_match_vals(r, na, fa, sig);
+            regret_match_vals(r, na, fa, sig);
 #pragma unroll
-            for (int rp = 0; rp < SS_MAX_PLAYERS; ++rp) {
-                if (rp >= NP || rp == trav || h >= nhv[rp]) continue;
-                float *rr = reach + (size_t)rp * N * Mp;
-#pragma unroll
-                for (int a = 0; a < NT_DEC; ++a)
-                    if (a < na) gst(rr + (size_t)ch[a] * Mp + h, rp == actor ? pr[rp] * sig[a] : pr[rp]);
-            }
+            for (int a = 0; a < NT_DEC; ++a)
+                if (a < na) gst(ra + (size_t)ch[a] * Mp + h, pr * sig[a]);
         }
     }
 }
@@ -1847,6 +1822,7 @@ int build_phased_tasks(const std::vector<Slot *> &slots, PhasedTasks &pt, hipStr
         r.na = nd.nchild;
         r.act_base = nd.act_base;
         r.fa = pl.frozen.empty() ? -1 : pl.frozen[n];
+        r.arow = nd.player >= 0 && nd.player < pl.NP ? pl.rrow[(size_t)nd.player * pl.N + n] : 0;
         for (int a = 0; a < 8; ++a) r.ch[a] = pl.children[nd.first_child + std::min(a, nd.nchild - 1)];
         for (int q = 0; q < 6; ++q) r.nh[q] = S.desc.nh[q];
         r.R = S.desc.R;

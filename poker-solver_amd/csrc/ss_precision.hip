@@ -93,7 +93,7 @@ __device__ __forceinline__ PrecArgs prec_args(const SlotDesc *__restrict__ s, in
 __device__ void prec_split_node(const SlotDesc *__restrict__ s, const PrecArgs &a, int t, int b, float *M,
                                 float *C) {
     const int n = a.sd_nodes[b], o = 1 - t, Mp = s->Mp, nh = s->nh[o];
-    const float *r = s->reach + ((size_t)o * s->N + n) * Mp;
+    const float *r = s->reach + ((size_t)o * s->N + s->rrow[(size_t)o * s->N + n]) * Mp;
     for (int i = threadIdx.x; i < DN2_P; i += blockDim.x) M[i] = 0.f;
     __syncthreads();
     // one pass over the opponent's hands: hi / lo split and the card-pair scatter
@@ -280,7 +280,8 @@ __device__ void prec_gemm_block(const SlotDesc *__restrict__ s, const PrecArgs &
     }
     // epilogue. Per-column constants go to LDS, over the reach buffer (free
     // after the last chunk's barrier): Cn's 53 values, then half the pot, the
-    // traverser's bet and the node. Per-row inputs are loaded as one batch,
+    // traverser's bet, the node and the row of the opponent's reach at the
+    // node (reach-row table). Per-row inputs are loaded as one batch,
     // then per column the opponent reach of the hand holding the same cards,
     // so a thread's loads overlap instead of waiting one after another.
     constexpr int CSTRIDE = 65;   // padded a bank: 16 columns read the same card
@@ -290,10 +291,13 @@ __device__ void prec_gemm_block(const SlotDesc *__restrict__ s, const PrecArgs &
         const int b = e >> 6, c = e & 63;
         float v = 0.f;   // columns past S: zeros, node 0
         if (b < a.S && c <= 52) v = gld(a.Cn + (size_t)b * 64 + c);
-        else if (b < a.S && c <= 55) {
+        else if (b < a.S && c <= 56) {
             const int n = gld(a.sd_nodes + b);
             const DevNode &nd = s->nodes[n];
-            v = c == 53 ? 0.5f * (float)nd.pot : c == 54 ? (float)nd.bets[t] : __int_as_float(n);
+            v = c == 53   ? 0.5f * (float)nd.pot
+                : c == 54 ? (float)nd.bets[t]
+                : c == 55 ? __int_as_float(n)
+                          : __int_as_float(gld(s->rrow + (size_t)o * s->N + n));
         }
         Cs[b * CSTRIDE + c] = v;
     }
@@ -317,10 +321,10 @@ __device__ void prec_gemm_block(const SlotDesc *__restrict__ s, const PrecArgs &
             const int b = 16 * q + (l & 15);
             const float *Cb = Cs + b * CSTRIDE;
             const float Sb = Cb[52], half = Cb[53], bet = Cb[54];
-            const int n = __float_as_int(Cb[55]);
+            const int n = __float_as_int(Cb[55]), nrow = __float_as_int(Cb[56]);
             float rv[4];
 #pragma unroll
-            for (int e = 0; e < 4; ++e) rv[e] = gld(ro + (size_t)n * Mp + max(sv[e], 0));
+            for (int e = 0; e < 4; ++e) rv[e] = gld(ro + (size_t)nrow * Mp + max(sv[e], 0));
             // the values are needed here: keeps the compiler from sinking each
             // load into the guarded store below, one wait per load
 #pragma unroll

@@ -119,6 +119,21 @@ int make_plan(const HostProblem &hp, Plan &pl) {
         }
     }
     if ((int)bfs.size() != N) return set_error(SS_ERR_ARG, "unreachable nodes in tree");
+    // Reach rows: player p's reach changes only below p's own actions, so the
+    // row holding p's reach at node c is c where p acted at c's parent, else
+    // the parent's row (the root's row holds the range weights). The forward
+    // sweep writes only the acting player's child rows; every reader goes
+    // through this table, so no reach is copied down the tree.
+    pl.rrow.assign((size_t)NP * N, 0);
+    for (int i : bfs) {
+        const SSNode &nd = td->nodes[i];
+        for (int a = 0; a < nd.num_children; ++a) {
+            const int c = td->children[nd.first_child + a];
+            for (int p = 0; p < NP; ++p)
+                pl.rrow[(size_t)p * N + c] =
+                    nd.type == SS_NODE_DECISION && nd.player == p ? c : pl.rrow[(size_t)p * N + i];
+        }
+    }
     int maxd = 0;
     for (int i = 0; i < N; ++i) maxd = std::max(maxd, depth[i]);
 
@@ -348,7 +363,7 @@ int64_t estimate_footprint(const HostProblem &hp) {
     size_t Mp = pl.Mp;
     int64_t b = 0;
     b += (int64_t)pl.N * sizeof(DevNode) + pl.children.size() * 4 + (pl.D + pl.L + 1 + pl.T) * 4;
-    b += (int64_t)pl.NP * Mp * 8 + pl.leaf_count * 4 + pl.N * 4;
+    b += (int64_t)pl.NP * Mp * 8 + pl.leaf_count * 4 + pl.N * 4 + (int64_t)pl.NP * pl.N * 4;
     b += (int64_t)2 * pl.A * Mp * 4 + (int64_t)pl.N * Mp * 4 + (int64_t)pl.NP * pl.N * Mp * 4 + 8 * Mp * 4;
     b += pl.is_river ? (SS_NCOMBO * 4 + SS_NCOMBO * 2 + 52 * 53 * 2 + SS_BND_BYTES + (int64_t)pl.NP * Mp * 16) : 0;
     // exploitability buffers
@@ -371,6 +386,7 @@ int Slot::setup(const HostProblem &hp, hipStream_t st) {
     size_t o_w = L.take(sizeof(float) * pl.NP * Mp);
     size_t o_leaf = L.take(sizeof(float) * std::max<size_t>(1, pl.leaf_count));
     size_t o_frozen = L.take(sizeof(int) * pl.N);
+    size_t o_rrow = L.take(sizeof(int) * pl.rrow.size());
     size_t o_R = L.take(sizeof(float) * (size_t)pl.A * Mp);
     size_t o_SS = L.take(sizeof(float) * (size_t)pl.A * Mp);
     size_t o_cfv = L.take(sizeof(float) * (size_t)pl.N * Mp);
@@ -402,6 +418,7 @@ int Slot::setup(const HostProblem &hp, hipStream_t st) {
         SS_HIP(hipMemcpyAsync(a + o_leaf, pl.leaf_src, sizeof(float) * pl.leaf_count, hipMemcpyHostToDevice, st));
     if (pl.frozen_src)
         SS_HIP(hipMemcpyAsync(a + o_frozen, pl.frozen_src, sizeof(int) * pl.N, hipMemcpyHostToDevice, st));
+    SS_HIP(hipMemcpyAsync(a + o_rrow, pl.rrow.data(), sizeof(int) * pl.rrow.size(), hipMemcpyHostToDevice, st));
     // root reach = raw weights (never overwritten: the forward sweep writes children only)
     for (int p = 0; p < pl.NP; ++p)
         SS_HIP(hipMemcpyAsync(a + o_reach + sizeof(float) * (size_t)p * pl.N * Mp, pl.weights.data() + (size_t)p * Mp,
@@ -425,6 +442,7 @@ int Slot::setup(const HostProblem &hp, hipStream_t st) {
     d.weights = (const float *)(a + o_w);
     d.leaf_values = pl.has_leaf ? (const float *)(a + o_leaf) : nullptr;
     d.frozen = pl.frozen_src ? (const int *)(a + o_frozen) : nullptr;
+    d.rrow = (const int *)(a + o_rrow);
     d.R = (float *)(a + o_R);
     d.SS = (float *)(a + o_SS);
     d.cfv = (float *)(a + o_cfv);
