@@ -1694,8 +1694,12 @@ PhasedTasks::~PhasedTasks() {
 static int term_cost(const Plan &pl, int n, bool prec) {
     const DevNode &nd = pl.nodes[n];
     if (prec && nd.type == SS_NODE_SHOWDOWN) return 1;   // precision form: the contraction, FP64 checks only
+    // the most opponents any traverser meets here; at a showdown a traverser
+    // who has folded gets 0 with no opponent work (terminal_node's zero case),
+    // so only the active players count there
     int kmax = 0;
     for (int t = 0; t < pl.NP; ++t) {
+        if (nd.type == SS_NODE_SHOWDOWN && !((nd.active_mask >> t) & 1)) continue;
         int k = 0;
         for (int q = 0; q < pl.NP; ++q)
             if (q != t && (nd.type == SS_NODE_FOLD || ((nd.active_mask >> q) & 1))) ++k;
