@@ -133,8 +133,18 @@ struct PhasedTasks {
     int psplit_off = 0, npsplit = 0;   // precision form: (slot, showdown index) tasks
     std::vector<std::pair<int, int>> pgemm;   // (column tiles NT, task count) groups of (slot, row tile) tasks
     int pgemm_off = 0;
+    // Per traverser pass t < NPmax, only the tasks that pass has work in (slots
+    // with more than t players; the forward sweep also drops nodes where t
+    // acts, whose child rows it does not write). Sweep records: forward level
+    // l = [sw_off[(2t)(L+1) + l], sw_off[(2t)(L+1) + l + 1]), backward levels
+    // at (2t+1)(L+1). Terminal tasks: pterm[t] segments of d_tasks.
+    std::vector<int> sw_off;
+    struct PassTerm {
+        int off = 0, nodo = 0, nsd = 0, nfold = 0, nlight = 0;
+    };
+    std::vector<PassTerm> pterm;
     uint32_t *d_tasks = nullptr;
-    SweepRec *d_recs = nullptr;    // one per decision task, [0, lvl_off[L]), same order as d_tasks
+    SweepRec *d_recs = nullptr;    // the per-pass sweep records
     ~PhasedTasks();
 };
 // tasks of the non-null slots with live[slot] != 0 (all of them when live is null)

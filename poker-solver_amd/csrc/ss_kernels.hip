@@ -1808,11 +1808,47 @@ int build_phased_tasks(const std::vector<Slot *> &slots, PhasedTasks &pt, hipStr
             pt.pgemm.push_back({nt, (int)(tasks.size() - g0)});
         }
     }
-    // sweep records of the decision tasks (same order)
-    const size_t ndec = (size_t)pt.lvl_off[pt.L];
+    // per traverser pass: terminal tasks of the slots with more than t
+    // players, by class, each class in XCD order
+    pt.pterm.assign(pt.NPmax, PhasedTasks::PassTerm());
+    for (int t = 0; t < pt.NPmax; ++t) {
+        PhasedTasks::PassTerm &q = pt.pterm[t];
+        q.off = (int)tasks.size();
+        auto seg = [&](int lo, int hi) {
+            const size_t b = tasks.size();
+            for (auto &e : term)
+                if (e.first >= lo && e.first <= hi && t < slots[e.second >> 20]->plan.NP) tasks.push_back(e.second);
+            xcd_interleave(tasks, b, tasks.size());
+            return (int)(tasks.size() - b);
+        };
+        q.nodo = seg(1000, 1 << 30);
+        q.nsd = seg(100, 100);
+        q.nfold = seg(20, 20);
+        q.nlight = seg(5, 5);
+    }
+    // per traverser pass and level: forward tasks (actor not the traverser),
+    // backward tasks, of the slots with more than t players
+    std::vector<uint32_t> sw;
+    pt.sw_off.assign((size_t)std::max(1, pt.NPmax) * 2 * (pt.L + 1), 0);
+    for (int t = 0; t < pt.NPmax; ++t)
+        for (int dir = 0; dir < 2; ++dir) {
+            int *off = &pt.sw_off[((size_t)t * 2 + dir) * (pt.L + 1)];
+            for (int l = 0; l < pt.L; ++l) {
+                off[l] = (int)sw.size();
+                for (int j = pt.lvl_off[l]; j < pt.lvl_off[l + 1]; ++j) {
+                    const Plan &pl = slots[tasks[j] >> 20]->plan;
+                    const int a = pl.nodes[tasks[j] & 0xFFFFF].player;
+                    if (t >= pl.NP || (dir == 0 && (a == t || a < 0 || a >= pl.NP))) continue;
+                    sw.push_back(tasks[j]);
+                }
+                xcd_interleave(sw, off[l], sw.size());
+            }
+            off[pt.L] = (int)sw.size();
+        }
+    const size_t ndec = sw.size();
     std::vector<SweepRec> recs(ndec);
     for (size_t j = 0; j < ndec; ++j) {
-        const int si = (int)(tasks[j] >> 20), n = (int)(tasks[j] & 0xFFFFF);
+        const int si = (int)(sw[j] >> 20), n = (int)(sw[j] & 0xFFFFF);
         const Slot &S = *slots[si];
         const Plan &pl = S.plan;
         const DevNode &nd = pl.nodes[n];
@@ -1846,35 +1882,33 @@ int build_phased_tasks(const std::vector<Slot *> &slots, PhasedTasks &pt, hipStr
 
 int launch_phased_iteration(SlotDesc *d_slots, const PhasedTasks &pt, hipStream_t st) {
     for (int t = 0; t < pt.NPmax; ++t) {
+        const int *fo = &pt.sw_off[(size_t)t * 2 * (pt.L + 1)], *bo = fo + pt.L + 1;
         hipLaunchKernelGGL(k_rsig_phased, dim3(pt.nslots), dim3(SS_BLOCK), 0, st, d_slots, pt.nslots, t);
         for (int l = 0; l < pt.L; ++l) {
-            int cnt = pt.lvl_off[l + 1] - pt.lvl_off[l];
+            int cnt = fo[l + 1] - fo[l];
             if (cnt > 0)
-                hipLaunchKernelGGL(k_fwd_phased, dim3(cnt), dim3(SWEEP_BLOCK), 0, st, d_slots, pt.d_recs + pt.lvl_off[l],
-                                   cnt, t);
+                hipLaunchKernelGGL(k_fwd_phased, dim3(cnt), dim3(SWEEP_BLOCK), 0, st, d_slots, pt.d_recs + fo[l], cnt, t);
         }
         if (pt.npsplit > 0) {
             int rc = launch_prec_phased(d_slots, pt, t, st);
             if (rc) return rc;
         }
-        const int nheavy = pt.nodo + pt.npair, nlight = pt.nterm - nheavy - pt.nprec;
-        if (pt.nodo > 0)
-            hipLaunchKernelGGL(k_term_odo, dim3(pt.nodo), dim3(SS_BLOCK), 0, st, d_slots, pt.d_tasks + pt.term_off,
-                               pt.nodo, t);
-        if (pt.npair_sd > 0)
-            hipLaunchKernelGGL(k_term_pair<1>, dim3(pt.npair_sd), dim3(SS_PAIR_BLOCK), 0, st, d_slots,
-                               pt.d_tasks + pt.term_off + pt.nodo, pt.npair_sd, t);
-        if (pt.npair > pt.npair_sd)
-            hipLaunchKernelGGL(k_term_pair<0>, dim3(pt.npair - pt.npair_sd), dim3(SS_BLOCK), 0, st, d_slots,
-                               pt.d_tasks + pt.term_off + pt.nodo + pt.npair_sd, pt.npair - pt.npair_sd, t);
-        if (nlight > 0)
-            hipLaunchKernelGGL(k_term_light, dim3(nlight), dim3(SS_BLOCK), 0, st, d_slots,
-                               pt.d_tasks + pt.term_off + nheavy, nlight, t);
+        const PhasedTasks::PassTerm &q = pt.pterm[t];
+        const uint32_t *tt = pt.d_tasks + q.off;
+        if (q.nodo > 0)
+            hipLaunchKernelGGL(k_term_odo, dim3(q.nodo), dim3(SS_BLOCK), 0, st, d_slots, tt, q.nodo, t);
+        if (q.nsd > 0)
+            hipLaunchKernelGGL(k_term_pair<1>, dim3(q.nsd), dim3(SS_PAIR_BLOCK), 0, st, d_slots, tt + q.nodo, q.nsd, t);
+        if (q.nfold > 0)
+            hipLaunchKernelGGL(k_term_pair<0>, dim3(q.nfold), dim3(SS_BLOCK), 0, st, d_slots, tt + q.nodo + q.nsd,
+                               q.nfold, t);
+        if (q.nlight > 0)
+            hipLaunchKernelGGL(k_term_light, dim3(q.nlight), dim3(SS_BLOCK), 0, st, d_slots,
+                               tt + q.nodo + q.nsd + q.nfold, q.nlight, t);
         for (int l = pt.L - 1; l >= 0; --l) {
-            int cnt = pt.lvl_off[l + 1] - pt.lvl_off[l];
+            int cnt = bo[l + 1] - bo[l];
             if (cnt > 0)
-                hipLaunchKernelGGL(k_bwd_phased, dim3(cnt), dim3(SWEEP_BLOCK), 0, st, d_slots, pt.d_recs + pt.lvl_off[l],
-                                   cnt, t);
+                hipLaunchKernelGGL(k_bwd_phased, dim3(cnt), dim3(SWEEP_BLOCK), 0, st, d_slots, pt.d_recs + bo[l], cnt, t);
         }
     }
     hipLaunchKernelGGL(k_bump_phased, dim3((pt.nslots + 255) / 256), dim3(256), 0, st, d_slots, pt.nslots);
