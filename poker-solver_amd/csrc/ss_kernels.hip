@@ -1299,21 +1299,35 @@ __global__ void __launch_bounds__(SWEEP_BLOCK) k_fwd_phased(SlotDesc *__restrict
         const float *R = kr->R;
         float *ra = kr->reach + (size_t)actor * N * Mp;
         const float *par = ra + (size_t)kr->arow * Mp;
+        // two hands per thread (h and h + blockDim.x; a node has at most 1128
+        // hands), both hands' loads issued before the first wait
         bool checked = false;
-        for (int h = threadIdx.x; h < nh_act; h += blockDim.x) {
-            float r[NT_DEC], sig[NT_DEC];
+        for (int h0 = threadIdx.x; h0 < nh_act; h0 += 2 * blockDim.x) {
+            const int h1 = h0 + blockDim.x, hb = h1 < nh_act ? h1 : h0;
+            float r0[NT_DEC], r1[NT_DEC], sig[NT_DEC];
 #pragma unroll
-            for (int a = 0; a < NT_DEC; ++a) r[a] = gld(R + (size_t)(ab + min(a, na - 1)) * Mp + h);
-            const float pr = gld(par + h);
+            for (int a = 0; a < NT_DEC; ++a) {
+                const size_t row = (size_t)(ab + min(a, na - 1)) * Mp;
+                r0[a] = gld(R + row + h0);
+                r1[a] = gld(R + row + hb);
+            }
+            const float p0 = gld(par + h0), p1 = gld(par + hb);
             if (!checked) {
-                SWEEP_PIN(pr);
+                SWEEP_PIN(p0);
+                SWEEP_PIN(p1);
                 if (!((s_act != 0) & (s_it < s_end) & (trav < s_np))) break;   // slot_runs
                 checked = true;
             }
-            regret_match_vals(r, na, fa, sig);
+            regret_match_vals(r0, na, fa, sig);
 #pragma unroll
             for (int a = 0; a < NT_DEC; ++a)
-                if (a < na) gst(ra + (size_t)ch[a] * Mp + h, pr * sig[a]);
+                if (a < na) gst(ra + (size_t)ch[a] * Mp + h0, p0 * sig[a]);
+            if (h1 < nh_act) {
+                regret_match_vals(r1, na, fa, sig);
+#pragma unroll
+                for (int a = 0; a < NT_DEC; ++a)
+                    if (a < na) gst(ra + (size_t)ch[a] * Mp + h1, p1 * sig[a]);
+            }
         }
     }
 }
@@ -1380,6 +1394,37 @@ __global__ void __launch_bounds__(SS_BLOCK) k_term_odo(SlotDesc *__restrict__ sl
 
 // backward_one for a whole task (same results), from the task's record as
 // the forward sweep
+// one hand of a backward task from its loaded child values (and, at the
+// traverser's own node, regret and strategy-sum rows)
+__device__ __forceinline__ void bwd_hand(bool own, int na, int fa, int ab, int Mp, int h, float d, int iter,
+                                         const float *cv, const float *r, const float *sg, float *R, float *SS,
+                                         float *out) {
+    float v = 0.f;
+    if (own) {
+        float sig[NT_DEC];
+        regret_match_vals(r, na, fa, sig);
+#pragma unroll
+        for (int a = 0; a < NT_DEC; ++a)
+            if (a < na) v += sig[a] * cv[a];
+#pragma unroll
+        for (int a = 0; a < NT_DEC; ++a) {
+            if (a < na) {
+                const size_t row = (size_t)(ab + a) * Mp + h;
+                float x = r[a];
+                x += cv[a] - v;
+                x *= d;
+                gst(R + row, x);
+                gst(SS + row, sg[a] + (float)iter * sig[a]);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int a = 0; a < NT_DEC; ++a)
+            if (a < na) v += cv[a];
+    }
+    gst(out + h, v);
+}
+
 __global__ void __launch_bounds__(SWEEP_BLOCK) k_bwd_phased(SlotDesc *__restrict__ slots,
                                                             const SweepRec *__restrict__ recs, int ntasks,
                                                             int trav) {
@@ -1399,49 +1444,51 @@ __global__ void __launch_bounds__(SWEEP_BLOCK) k_bwd_phased(SlotDesc *__restrict
         bool checked = false;
         float d = 0.f;
         int iter = 0;
-        for (int h = threadIdx.x; h < nh; h += blockDim.x) {
-            float cv[NT_DEC], r[NT_DEC], sg[NT_DEC], sig[NT_DEC];
-#pragma unroll
-            for (int a = 0; a < NT_DEC; ++a) cv[a] = gld(cfv + (size_t)ch[a] * Mp + h);
-            if (own) {
+        if (own) {
+            // the traverser's node: child values, regret and strategy-sum rows
+            // (24 loads per hand), one hand per thread per round
+            for (int h = threadIdx.x; h < nh; h += blockDim.x) {
+                float cv[NT_DEC], r[NT_DEC], sg[NT_DEC];
 #pragma unroll
                 for (int a = 0; a < NT_DEC; ++a) {
                     const size_t row = (size_t)(ab + min(a, na - 1)) * Mp + h;
+                    cv[a] = gld(cfv + (size_t)ch[a] * Mp + h);
                     r[a] = gld(R + row);
                     sg[a] = gld(SS + row);
                 }
-            }
-            if (!checked) {
+                if (!checked) {
 #pragma unroll
-                for (int a = 0; a < NT_DEC; ++a) SWEEP_PIN(cv[a]);
-                if (!((s_act != 0) & (s_it < s_end) & (trav < s_np))) break;   // slot_runs
-                iter = s_it + 1;
-                d = (float)iter / ((float)iter + 1.0f);
-                checked = true;
+                    for (int a = 0; a < NT_DEC; ++a) SWEEP_PIN(cv[a]);
+                    if (!((s_act != 0) & (s_it < s_end) & (trav < s_np))) break;   // slot_runs
+                    iter = s_it + 1;
+                    d = (float)iter / ((float)iter + 1.0f);
+                    checked = true;
+                }
+                bwd_hand(true, na, fa, ab, Mp, h, d, iter, cv, r, sg, R, SS, cfv + (size_t)n * Mp);
             }
-            float v = 0.f;
-            if (own) {
-                regret_match_vals(r, na, fa, sig);
-#pragma unroll
-                for (int a = 0; a < NT_DEC; ++a)
-                    if (a < na) v += sig[a] * cv[a];
+        } else {
+            // another player's node: child values only, two hands per thread
+            // with both hands' loads issued before the first wait
+            for (int h0 = threadIdx.x; h0 < nh; h0 += 2 * blockDim.x) {
+                const int h1 = h0 + blockDim.x, hb = h1 < nh ? h1 : h0;
+                float cv0[NT_DEC], cv1[NT_DEC];
 #pragma unroll
                 for (int a = 0; a < NT_DEC; ++a) {
-                    if (a < na) {
-                        const size_t row = (size_t)(ab + a) * Mp + h;
-                        float x = r[a];
-                        x += cv[a] - v;
-                        x *= d;
-                        gst(R + row, x);
-                        gst(SS + row, sg[a] + (float)iter * sig[a]);
-                    }
+                    cv0[a] = gld(cfv + (size_t)ch[a] * Mp + h0);
+                    cv1[a] = gld(cfv + (size_t)ch[a] * Mp + hb);
                 }
-            } else {
+                if (!checked) {
 #pragma unroll
-                for (int a = 0; a < NT_DEC; ++a)
-                    if (a < na) v += cv[a];
+                    for (int a = 0; a < NT_DEC; ++a) {
+                        SWEEP_PIN(cv0[a]);
+                        SWEEP_PIN(cv1[a]);
+                    }
+                    if (!((s_act != 0) & (s_it < s_end) & (trav < s_np))) break;   // slot_runs
+                    checked = true;
+                }
+                bwd_hand(false, na, fa, ab, Mp, h0, d, iter, cv0, cv0, cv0, R, SS, cfv + (size_t)n * Mp);
+                if (h1 < nh) bwd_hand(false, na, fa, ab, Mp, h1, d, iter, cv1, cv1, cv1, R, SS, cfv + (size_t)n * Mp);
             }
-            gst(cfv + (size_t)n * Mp + h, v);
         }
     }
 }
