@@ -139,5 +139,26 @@ __device__ __forceinline__ void gst(P *p, P v) {
     *p = v;   // host pass of the device code: never executed
 #endif
 }
+// access at a wave-uniform base plus a 32-bit byte offset: a scalar base per
+// row and one shared offset register, instead of a 64-bit address register
+// pair per access
+template <typename P>
+__device__ __forceinline__ P gldo(const P *base, unsigned off) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const __attribute__((address_space(1))) char *b = (const __attribute__((address_space(1))) char *)base;
+    return *(const __attribute__((address_space(1))) P *)(b + off);
+#else
+    return *(const P *)((const char *)base + off);   // host pass: never executed
+#endif
+}
+template <typename P>
+__device__ __forceinline__ void gsto(P *base, unsigned off, P v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    __attribute__((address_space(1))) char *b = (__attribute__((address_space(1))) char *)base;
+    *(__attribute__((address_space(1))) P *)(b + off) = v;
+#else
+    *(P *)((char *)base + off) = v;   // host pass: never executed
+#endif
+}
 
 }  // namespace ss

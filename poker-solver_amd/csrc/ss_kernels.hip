@@ -1242,11 +1242,13 @@ __device__ __forceinline__ bool slot_runs(const SlotDesc *__restrict__ s, int tr
 // 1081-combo node (Mp = 1088) takes two nearly full rounds of hands.
 #define SWEEP_BLOCK 576
 
-// regret_match over preloaded regret rows r (same arithmetic, same order)
+// regret_match over preloaded regret rows r (same arithmetic, same order);
+// A bounds the action count (na <= A)
+template <int A>
 __device__ __forceinline__ void regret_match_vals(const float *r, int na, int fa, float *sig) {
     float sum = 0.f;
 #pragma unroll
-    for (int a = 0; a < NT_DEC; ++a) {
+    for (int a = 0; a < A; ++a) {
         if (a < na) {
             sig[a] = r[a] > 0.f ? r[a] : 0.f;
             sum += sig[a];
@@ -1255,17 +1257,17 @@ __device__ __forceinline__ void regret_match_vals(const float *r, int na, int fa
     if (sum > 0.f) {
         float inv = 1.0f / sum;
 #pragma unroll
-        for (int a = 0; a < NT_DEC; ++a)
+        for (int a = 0; a < A; ++a)
             if (a < na) sig[a] *= inv;
     } else {
         float u = 1.0f / (float)na;
 #pragma unroll
-        for (int a = 0; a < NT_DEC; ++a)
+        for (int a = 0; a < A; ++a)
             if (a < na) sig[a] = u;
     }
     if (fa >= 0) {
 #pragma unroll
-        for (int a = 0; a < NT_DEC; ++a)
+        for (int a = 0; a < A; ++a)
             if (a < na) sig[a] = a == fa ? 1.f : 0.f;
     }
 }
@@ -1277,9 +1279,51 @@ __device__ __forceinline__ void regret_match_vals(const float *r, int na, int fa
 
 // forward_one for a whole task (same results). The task's record carries the
 // node: its fields are read into scalars up front (uniform address, so scalar
-// loads), and the first hand's loads go out with the read of the slot's
-// progress. Only the actor's child rows are written (reach-row table), so a
-// task whose actor is the traverser has nothing to do.
+// loads). H hands per thread (h0 + u * blockDim.x), every load of the H hands
+// issued together with the read of the slot's progress; A bounds the action
+// count, so a node with few actions issues no repeated loads. Only the actor's
+// child rows are written (reach-row table).
+template <int A, int H>
+__device__ __forceinline__ void fwd_task(const SweepRec *__restrict__ kr, int actor, int trav, int s_act, int s_it,
+                                         int s_end, int s_np) {
+    const int Mp = kr->Mp, N = kr->N, na = kr->na, ab = kr->act_base, fa = kr->fa;
+    int ch[A];
+#pragma unroll
+    for (int a = 0; a < A; ++a) ch[a] = kr->ch[a];
+    const int nh = kr->nh[actor], nt = blockDim.x;
+    const float *R = kr->R;
+    float *ra = kr->reach + (size_t)actor * N * Mp;
+    const float *par = ra + (size_t)kr->arow * Mp;
+    bool checked = false;
+    for (int h0 = threadIdx.x; h0 < nh; h0 += H * nt) {
+        float r[H][A], pr[H];
+#pragma unroll
+        for (int u = 0; u < H; ++u) {
+            const unsigned o = 4u * (unsigned)(h0 + u * nt < nh ? h0 + u * nt : h0);
+#pragma unroll
+            for (int a = 0; a < A; ++a) r[u][a] = gldo(R + (size_t)(ab + min(a, na - 1)) * Mp, o);
+            pr[u] = gldo(par, o);
+        }
+        if (!checked) {
+#pragma unroll
+            for (int u = 0; u < H; ++u) SWEEP_PIN(pr[u]);
+            if (!((s_act != 0) & (s_it < s_end) & (trav < s_np))) return;   // slot_runs
+            checked = true;
+        }
+#pragma unroll
+        for (int u = 0; u < H; ++u) {
+            const int h = h0 + u * nt;
+            if (h < nh) {
+                float sig[A];
+                regret_match_vals<A>(r[u], na, fa, sig);
+#pragma unroll
+                for (int a = 0; a < A; ++a)
+                    if (a < na) gsto(ra + (size_t)ch[a] * Mp, 4u * (unsigned)h, pr[u] * sig[a]);
+            }
+        }
+    }
+}
+
 __global__ void __launch_bounds__(SWEEP_BLOCK) k_fwd_phased(SlotDesc *__restrict__ slots,
                                                             const SweepRec *__restrict__ recs, int ntasks,
                                                             int trav) {
@@ -1288,47 +1332,10 @@ __global__ void __launch_bounds__(SWEEP_BLOCK) k_fwd_phased(SlotDesc *__restrict
         const int actor = kr->actor, NP = kr->NP;
         if (actor == trav || actor < 0 || actor >= NP) continue;
         const SlotDesc *s = slots + kr->slot;
-        // the slot's progress, read up front and tested after the first hand's
-        // loads are issued
         const int s_act = s->active, s_it = s->iter, s_end = s->iter_end, s_np = s->NP;
-        const int Mp = kr->Mp, N = kr->N, na = kr->na, ab = kr->act_base, fa = kr->fa;
-        int ch[NT_DEC];
-#pragma unroll
-        for (int a = 0; a < NT_DEC; ++a) ch[a] = kr->ch[a];
-        const int nh_act = kr->nh[actor];
-        const float *R = kr->R;
-        float *ra = kr->reach + (size_t)actor * N * Mp;
-        const float *par = ra + (size_t)kr->arow * Mp;
-        // two hands per thread (h and h + blockDim.x; a node has at most 1128
-        // hands), both hands' loads issued before the first wait
-        bool checked = false;
-        for (int h0 = threadIdx.x; h0 < nh_act; h0 += 2 * blockDim.x) {
-            const int h1 = h0 + blockDim.x, hb = h1 < nh_act ? h1 : h0;
-            float r0[NT_DEC], r1[NT_DEC], sig[NT_DEC];
-#pragma unroll
-            for (int a = 0; a < NT_DEC; ++a) {
-                const size_t row = (size_t)(ab + min(a, na - 1)) * Mp;
-                r0[a] = gld(R + row + h0);
-                r1[a] = gld(R + row + hb);
-            }
-            const float p0 = gld(par + h0), p1 = gld(par + hb);
-            if (!checked) {
-                SWEEP_PIN(p0);
-                SWEEP_PIN(p1);
-                if (!((s_act != 0) & (s_it < s_end) & (trav < s_np))) break;   // slot_runs
-                checked = true;
-            }
-            regret_match_vals(r0, na, fa, sig);
-#pragma unroll
-            for (int a = 0; a < NT_DEC; ++a)
-                if (a < na) gst(ra + (size_t)ch[a] * Mp + h0, p0 * sig[a]);
-            if (h1 < nh_act) {
-                regret_match_vals(r1, na, fa, sig);
-#pragma unroll
-                for (int a = 0; a < NT_DEC; ++a)
-                    if (a < na) gst(ra + (size_t)ch[a] * Mp + h1, p1 * sig[a]);
-            }
-        }
+        // two hands per thread: a node has at most 1128 hands
+        if (kr->na <= 3) fwd_task<3, 2>(kr, actor, trav, s_act, s_it, s_end, s_np);
+        else fwd_task<NT_DEC, 2>(kr, actor, trav, s_act, s_it, s_end, s_np);
     }
 }
 
@@ -1396,99 +1403,143 @@ __global__ void __launch_bounds__(SS_BLOCK) k_term_odo(SlotDesc *__restrict__ sl
 // the forward sweep
 // one hand of a backward task from its loaded child values (and, at the
 // traverser's own node, regret and strategy-sum rows)
-__device__ __forceinline__ void bwd_hand(bool own, int na, int fa, int ab, int Mp, int h, float d, int iter,
-                                         const float *cv, const float *r, const float *sg, float *R, float *SS,
-                                         float *out) {
+template <int A, bool OWN>
+__device__ __forceinline__ void bwd_hand(int na, int fa, int ab, int Mp, int h, float d, int iter, const float *cv,
+                                         const float *r, const float *sg, float *R, float *SS, float *out) {
     float v = 0.f;
-    if (own) {
-        float sig[NT_DEC];
-        regret_match_vals(r, na, fa, sig);
+    if (OWN) {
+        float sig[A];
+        regret_match_vals<A>(r, na, fa, sig);
 #pragma unroll
-        for (int a = 0; a < NT_DEC; ++a)
+        for (int a = 0; a < A; ++a)
             if (a < na) v += sig[a] * cv[a];
 #pragma unroll
-        for (int a = 0; a < NT_DEC; ++a) {
+        for (int a = 0; a < A; ++a) {
             if (a < na) {
-                const size_t row = (size_t)(ab + a) * Mp + h;
+                const size_t row = (size_t)(ab + a) * Mp;
                 float x = r[a];
                 x += cv[a] - v;
                 x *= d;
-                gst(R + row, x);
-                gst(SS + row, sg[a] + (float)iter * sig[a]);
+                gsto(R + row, 4u * (unsigned)h, x);
+                gsto(SS + row, 4u * (unsigned)h, sg[a] + (float)iter * sig[a]);
             }
         }
     } else {
 #pragma unroll
-        for (int a = 0; a < NT_DEC; ++a)
+        for (int a = 0; a < A; ++a)
             if (a < na) v += cv[a];
     }
-    gst(out + h, v);
+    gsto(out, 4u * (unsigned)h, v);
 }
 
+// backward_one for a whole task (same results): H hands per thread, every
+// load of the H hands (child values; at the traverser's own node also the
+// regret and strategy-sum rows) issued together with the read of the slot's
+// progress. A bounds the action count.
+template <int A, int H, bool OWN>
+__device__ __forceinline__ void bwd_task(const SweepRec *__restrict__ kr, int trav, int s_act, int s_it, int s_end,
+                                         int s_np) {
+    const int n = kr->node, Mp = kr->Mp, na = kr->na, ab = kr->act_base, fa = kr->fa;
+    int ch[A];
+#pragma unroll
+    for (int a = 0; a < A; ++a) ch[a] = kr->ch[a];
+    const int nh = kr->nh[trav], nt = blockDim.x;   // trav < NP <= 6
+    float *R = kr->R, *SS = kr->SS, *cfv = kr->cfv;
+    float *out = cfv + (size_t)n * Mp;
+    bool checked = false;
+    float d = 0.f;
+    int iter = 0;
+    for (int h0 = threadIdx.x; h0 < nh; h0 += H * nt) {
+        float cv[H][A], r[H][A], sg[H][A];
+#pragma unroll
+        for (int u = 0; u < H; ++u) {
+            const unsigned o = 4u * (unsigned)(h0 + u * nt < nh ? h0 + u * nt : h0);
+#pragma unroll
+            for (int a = 0; a < A; ++a) {
+                cv[u][a] = gldo(cfv + (size_t)ch[a] * Mp, o);
+                if (OWN) {
+                    const size_t row = (size_t)(ab + min(a, na - 1)) * Mp;
+                    r[u][a] = gldo(R + row, o);
+                    sg[u][a] = gldo(SS + row, o);
+                } else {
+                    r[u][a] = sg[u][a] = 0.f;
+                }
+            }
+        }
+        if (!checked) {
+#pragma unroll
+            for (int u = 0; u < H; ++u)
+#pragma unroll
+                for (int a = 0; a < A; ++a) SWEEP_PIN(cv[u][a]);
+            if (!((s_act != 0) & (s_it < s_end) & (trav < s_np))) return;   // slot_runs
+            iter = s_it + 1;
+            d = (float)iter / ((float)iter + 1.0f);
+            checked = true;
+        }
+#pragma unroll
+        for (int u = 0; u < H; ++u) {
+            const int h = h0 + u * nt;
+            if (h < nh) bwd_hand<A, OWN>(na, fa, ab, Mp, h, d, iter, cv[u], r[u], sg[u], R, SS, out);
+        }
+    }
+}
+
+// A backward task at the traverser's own node: child values, regret and
+// strategy-sum rows, one hand per thread per round, the loads issued together
+// with the read of the slot's progress. (As a bwd_task instance the row
+// addresses are kept live across rounds.)
+template <int A>
+__device__ __forceinline__ void bwd_own(const SweepRec *__restrict__ kr, int trav, int s_act, int s_it, int s_end,
+                                        int s_np) {
+    const int n = kr->node, Mp = kr->Mp, na = kr->na, ab = kr->act_base, fa = kr->fa;
+    int ch[A];
+#pragma unroll
+    for (int a = 0; a < A; ++a) ch[a] = kr->ch[a];
+    const int nh = kr->nh[trav];
+    float *R = kr->R, *SS = kr->SS, *cfv = kr->cfv;
+    bool checked = false;
+    float d = 0.f;
+    int iter = 0;
+    for (int h = threadIdx.x; h < nh; h += blockDim.x) {
+        float cv[A], r[A], sg[A];
+        const unsigned o = 4u * (unsigned)h;
+#pragma unroll
+        for (int a = 0; a < A; ++a) {
+            const size_t row = (size_t)(ab + min(a, na - 1)) * Mp;
+            cv[a] = gldo(cfv + (size_t)ch[a] * Mp, o);
+            r[a] = gldo(R + row, o);
+            sg[a] = gldo(SS + row, o);
+        }
+        if (!checked) {
+#pragma unroll
+            for (int a = 0; a < A; ++a) SWEEP_PIN(cv[a]);
+            if (!((s_act != 0) & (s_it < s_end) & (trav < s_np))) break;   // slot_runs
+            iter = s_it + 1;
+            d = (float)iter / ((float)iter + 1.0f);
+            checked = true;
+        }
+        bwd_hand<A, true>(na, fa, ab, Mp, h, d, iter, cv, r, sg, R, SS, cfv + (size_t)n * Mp);
+    }
+}
+
+// Backward tasks: two hands per thread, except the traverser's own nodes with
+// more than three actions (24 loads per hand), which keep one hand per round.
+// At most 64 VGPRs (each path needs fewer; unbounded, the branches together
+// are given 77), so that three workgroups fit a CU.
 __global__ void __launch_bounds__(SWEEP_BLOCK) k_bwd_phased(SlotDesc *__restrict__ slots,
                                                             const SweepRec *__restrict__ recs, int ntasks,
                                                             int trav) {
     for (int j = blockIdx.x; j < ntasks; j += gridDim.x) {
         const SweepRec *kr = recs + j;
         const SlotDesc *s = slots + kr->slot;
-        // the slot's progress, read up front and tested after the first hand's
-        // loads are issued
         const int s_act = s->active, s_it = s->iter, s_end = s->iter_end, s_np = s->NP;
-        const int n = kr->node, Mp = kr->Mp, na = kr->na, ab = kr->act_base, fa = kr->fa;
-        int ch[NT_DEC];
-#pragma unroll
-        for (int a = 0; a < NT_DEC; ++a) ch[a] = kr->ch[a];
-        const int nh = kr->nh[trav];   // trav < NP <= 6
-        float *R = kr->R, *SS = kr->SS, *cfv = kr->cfv;
-        const bool own = kr->actor == trav;
-        bool checked = false;
-        float d = 0.f;
-        int iter = 0;
-        if (own) {
-            // the traverser's node: child values, regret and strategy-sum rows
-            // (24 loads per hand), one hand per thread per round
-            for (int h = threadIdx.x; h < nh; h += blockDim.x) {
-                float cv[NT_DEC], r[NT_DEC], sg[NT_DEC];
-#pragma unroll
-                for (int a = 0; a < NT_DEC; ++a) {
-                    const size_t row = (size_t)(ab + min(a, na - 1)) * Mp + h;
-                    cv[a] = gld(cfv + (size_t)ch[a] * Mp + h);
-                    r[a] = gld(R + row);
-                    sg[a] = gld(SS + row);
-                }
-                if (!checked) {
-#pragma unroll
-                    for (int a = 0; a < NT_DEC; ++a) SWEEP_PIN(cv[a]);
-                    if (!((s_act != 0) & (s_it < s_end) & (trav < s_np))) break;   // slot_runs
-                    iter = s_it + 1;
-                    d = (float)iter / ((float)iter + 1.0f);
-                    checked = true;
-                }
-                bwd_hand(true, na, fa, ab, Mp, h, d, iter, cv, r, sg, R, SS, cfv + (size_t)n * Mp);
-            }
+        const bool few = kr->na <= 3;
+        if (kr->actor == trav) {
+            if (few) bwd_own<3>(kr, trav, s_act, s_it, s_end, s_np);
+            else bwd_own<NT_DEC>(kr, trav, s_act, s_it, s_end, s_np);
         } else {
-            // another player's node: child values only, two hands per thread
-            // with both hands' loads issued before the first wait
-            for (int h0 = threadIdx.x; h0 < nh; h0 += 2 * blockDim.x) {
-                const int h1 = h0 + blockDim.x, hb = h1 < nh ? h1 : h0;
-                float cv0[NT_DEC], cv1[NT_DEC];
-#pragma unroll
-                for (int a = 0; a < NT_DEC; ++a) {
-                    cv0[a] = gld(cfv + (size_t)ch[a] * Mp + h0);
-                    cv1[a] = gld(cfv + (size_t)ch[a] * Mp + hb);
-                }
-                if (!checked) {
-#pragma unroll
-                    for (int a = 0; a < NT_DEC; ++a) {
-                        SWEEP_PIN(cv0[a]);
-                        SWEEP_PIN(cv1[a]);
-                    }
-                    if (!((s_act != 0) & (s_it < s_end) & (trav < s_np))) break;   // slot_runs
-                    checked = true;
-                }
-                bwd_hand(false, na, fa, ab, Mp, h0, d, iter, cv0, cv0, cv0, R, SS, cfv + (size_t)n * Mp);
-                if (h1 < nh) bwd_hand(false, na, fa, ab, Mp, h1, d, iter, cv1, cv1, cv1, R, SS, cfv + (size_t)n * Mp);
-            }
+            if (few) bwd_task<3, 2, false>(kr, trav, s_act, s_it, s_end, s_np);
+            else bwd_task<NT_DEC, 2, false>(kr, trav, s_act, s_it, s_end, s_np);
         }
     }
 }
