@@ -496,41 +496,53 @@ __device__ __forceinline__ void card_totals(const T *M1, const T *M2, T *C1, T *
 
 // per-card prefix sums over strength-sorted partner lists (showdown):
 // PC[c*52 + j] = sum of the first j partners of card c; C = list totals.
+// One wavefront per card (lane j = the card's j-th partner: gather, scan and
+// write in one pass), two cards in flight per wavefront so that their LDS
+// reads and scan steps overlap.
 template <typename T>
 __device__ __forceinline__ void card_prefix(const uint8_t *clist, const uint8_t *clen, const T *M1, const T *M2,
                                             T *PC1, T *PC2, T *PCP, T *C1, T *C2, T *CP, int tid, int nt) {
-    // gather without branches (a clamped list entry is always a card), so
-    // the unrolled iterations' LDS reads overlap
-#pragma unroll 3
-    for (int i = tid; i < DN2; i += nt) {
-        const int c = i / DN, j = i - c * DN;
-        const bool ok = j > 0 && j - 1 < clen[c];
-        const int d = clist[c * DN + max(j - 1, 0)];
-        const T a = ok ? M1[c * DN + d] : (T)0;
-        PC1[i] = a;
+    const int lane = tid & 63, nw = nt >> 6, li = min(lane, 50);
+    for (int c0 = tid >> 6; c0 < 52; c0 += 2 * nw) {
+        const int c1 = c0 + nw < 52 ? c0 + nw : c0;   // uniform: the second card, or c0 again
+        const int n0 = clen[c0], n1 = clen[c1];
+        const int d0 = clist[c0 * DN + li], d1 = clist[c1 * DN + li];
+        T a0 = lane < n0 ? M1[c0 * DN + d0] : (T)0, a1 = lane < n1 ? M1[c1 * DN + d1] : (T)0;
+        T b0 = 0, b1 = 0, p0 = 0, p1 = 0;
         if (M2) {
-            const T b = ok ? M2[c * DN + d] : (T)0;
-            PC2[i] = b;
-            PCP[i] = a * b;
+            b0 = lane < n0 ? M2[c0 * DN + d0] : (T)0;
+            b1 = lane < n1 ? M2[c1 * DN + d1] : (T)0;
+            p0 = a0 * b0;
+            p1 = a1 * b1;
         }
-    }
-    __syncthreads();
-    const int lane = tid & 63, nw = nt >> 6;
-    for (int c = tid >> 6; c < 52; c += nw) {
-        const int at = c * DN + 1 + lane;
-        T a = lane < 51 ? PC1[at] : (T)0;
-        a = wave_incl_scan(a, lane);
-        if (lane < 51) PC1[at] = a;
-        T ta = lane_value(a, 50);
-        if (lane == 0) C1[c] = ta;
+        a0 = wave_incl_scan(a0, lane);
+        a1 = wave_incl_scan(a1, lane);
         if (M2) {
-            T b = lane < 51 ? PC2[at] : (T)0;
-            T p = lane < 51 ? PCP[at] : (T)0;
-            b = wave_incl_scan(b, lane);
-            p = wave_incl_scan(p, lane);
-            if (lane < 51) { PC2[at] = b; PCP[at] = p; }
-            T tb = lane_value(b, 50), tp = lane_value(p, 50);
-            if (lane == 0) { C2[c] = tb; CP[c] = tp; }
+            b0 = wave_incl_scan(b0, lane);
+            b1 = wave_incl_scan(b1, lane);
+            p0 = wave_incl_scan(p0, lane);
+            p1 = wave_incl_scan(p1, lane);
+        }
+        const T ta0 = lane_value(a0, 50), ta1 = lane_value(a1, 50);
+        if (lane < 51) PC1[c0 * DN + 1 + lane] = a0;
+        if (lane == 63) PC1[c0 * DN] = (T)0;
+        if (lane == 0) C1[c0] = ta0;
+        if (M2) {
+            const T tb0 = lane_value(b0, 50), tp0 = lane_value(p0, 50);
+            if (lane < 51) { PC2[c0 * DN + 1 + lane] = b0; PCP[c0 * DN + 1 + lane] = p0; }
+            if (lane == 63) { PC2[c0 * DN] = (T)0; PCP[c0 * DN] = (T)0; }
+            if (lane == 0) { C2[c0] = tb0; CP[c0] = tp0; }
+        }
+        if (c1 != c0) {
+            if (lane < 51) PC1[c1 * DN + 1 + lane] = a1;
+            if (lane == 63) PC1[c1 * DN] = (T)0;
+            if (lane == 0) C1[c1] = ta1;
+            if (M2) {
+                const T tb1 = lane_value(b1, 50), tp1 = lane_value(p1, 50);
+                if (lane < 51) { PC2[c1 * DN + 1 + lane] = b1; PCP[c1 * DN + 1 + lane] = p1; }
+                if (lane == 63) { PC2[c1 * DN] = (T)0; PCP[c1 * DN] = (T)0; }
+                if (lane == 0) { C2[c1] = tb1; CP[c1] = tp1; }
+            }
         }
     }
 }
