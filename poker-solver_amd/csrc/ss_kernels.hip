@@ -325,18 +325,26 @@ __device__ __forceinline__ T wave_sum(T x) {
 // wave totals) whatever the block size, so every form rounds the same way.
 #define SCAN_LANES 256
 #define SCAN_CHUNK 5   // ceil(1081 / 256): a river has at most 1081 live combos
+// this thread's entries of the strength order (scan_sorted's chunk), loaded
+// ahead of the matrices they index
+__device__ __forceinline__ void scan_load(const SlotDesc *__restrict__ s, uint32_t *gs, int tid) {
+    const int ng = s->ng;
+    const int chunk = (ng + SCAN_LANES - 1) / SCAN_LANES;
+    const bool on = tid < SCAN_LANES;
+    const int j0 = on ? min(ng, tid * chunk) : 0, j1 = on ? min(ng, j0 + chunk) : 0;
+#pragma unroll
+    for (int u = 0; u < SCAN_CHUNK; ++u) gs[u] = j0 + u < j1 ? gld(s->gsort + j0 + u) : 0u;
+}
 template <typename T, int NCH>
-__device__ void scan_sorted(const SlotDesc *__restrict__ s, const T *M1, const T *M2, T *PG, T *red, int tid) {
+__device__ void scan_sorted(const SlotDesc *__restrict__ s, const T *M1, const T *M2, T *PG, T *red,
+                            const uint32_t *gs, int tid) {
     const int ng = s->ng;
     const int chunk = (ng + SCAN_LANES - 1) / SCAN_LANES;
     const bool on = tid < SCAN_LANES;
     const int j0 = on ? min(ng, tid * chunk) : 0, j1 = on ? min(ng, j0 + chunk) : 0;
     int at[SCAN_CHUNK];
 #pragma unroll
-    for (int u = 0; u < SCAN_CHUNK; ++u) {
-        const uint32_t g = j0 + u < j1 ? gld(s->gsort + j0 + u) : 0u;
-        at[u] = (g & 0xFF) * DN + (g >> 8);
-    }
+    for (int u = 0; u < SCAN_CHUNK; ++u) at[u] = (gs[u] & 0xFF) * DN + (gs[u] >> 8);
     T part[NCH];
 #pragma unroll
     for (int q = 0; q < NCH; ++q) part[q] = 0;
@@ -440,34 +448,44 @@ __device__ T odometer(const SlotDesc *__restrict__ s, int n, int trav, int h, co
 // Each combo owns two cells: with unique hands every cell gets one store
 // (bitwise the same as an add onto the zeroed cell); duplicates add.
 // Loads are issued four hands at a time, ahead of the adds.
-template <typename T>
-__device__ __forceinline__ void scatter(const SlotDesc *__restrict__ s, int q, int n, const T *reach, T *M,
-                                        int tid, int nt) {
+// Opponent q's hands into its 52 x 52 matrix, in two steps so that the
+// global loads can be issued before the matrix is cleared: scatter_load
+// fetches the first SCN * nt hands (hand record, reach at row rrow),
+// scatter_store writes them and any further hands.
+template <typename T, int SCN>
+__device__ __forceinline__ void scatter_load(const SlotDesc *__restrict__ s, int q, int rrow, const T *reach,
+                                             uint32_t *oi, T *r, int tid, int nt) {
     const int Mp = s->Mp, nq = s->nh[q];
     const uint32_t *hq = s->hinfo + (size_t)q * Mp;
-    const T *rq = reach + ((size_t)q * s->N + gld(s->rrow + (size_t)q * s->N + n)) * Mp;
-    for (int o0 = tid; o0 < nq; o0 += 4 * nt) {
-        uint32_t oi[4];
-        T r[4];
+    const T *rq = reach + ((size_t)q * s->N + rrow) * Mp;
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int o = o0 + u * nt;
-            oi[u] = o < nq ? gld(hq + o) : 0u;
-            r[u] = o < nq ? gld(rq + o) : (T)0;
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            if (!(oi[u] & SS_LIVE_BIT)) continue;
-            const int a = oi[u] & 63, b = (oi[u] >> 6) & 63;
-            if (s->uniq) {   // one writer per cell
-                M[a * DN + b] = r[u];
-                M[b * DN + a] = r[u];
-            } else {
-                atomicAdd(&M[a * DN + b], r[u]);
-                atomicAdd(&M[b * DN + a], r[u]);
-            }
-        }
+    for (int u = 0; u < SCN; ++u) {
+        const int o = tid + u * nt;
+        oi[u] = o < nq ? gld(hq + o) : 0u;
+        r[u] = o < nq ? gld(rq + o) : (T)0;
     }
+}
+template <typename T>
+__device__ __forceinline__ void scatter_one(const SlotDesc *__restrict__ s, T *M, uint32_t oi, T r) {
+    if (!(oi & SS_LIVE_BIT)) return;
+    const int a = oi & 63, b = (oi >> 6) & 63;
+    if (s->uniq) {   // one writer per cell
+        M[a * DN + b] = r;
+        M[b * DN + a] = r;
+    } else {
+        atomicAdd(&M[a * DN + b], r);
+        atomicAdd(&M[b * DN + a], r);
+    }
+}
+template <typename T, int SCN>
+__device__ __forceinline__ void scatter_store(const SlotDesc *__restrict__ s, int q, int rrow, const T *reach, T *M,
+                                              const uint32_t *oi, const T *r, int tid, int nt) {
+#pragma unroll
+    for (int u = 0; u < SCN; ++u) scatter_one(s, M, oi[u], r[u]);
+    const int Mp = s->Mp, nq = s->nh[q];
+    const uint32_t *hq = s->hinfo + (size_t)q * Mp;
+    const T *rq = reach + ((size_t)q * s->N + rrow) * Mp;
+    for (int o = tid + SCN * nt; o < nq; o += nt) scatter_one(s, M, gld(hq + o), gld(rq + o));
 }
 
 // per-card totals of M1 (and M2, M1.*M2): one wavefront per card, lane =
@@ -637,6 +655,10 @@ __device__ void terminal_node(const SlotDesc *__restrict__ s, int n, int trav, c
     const int nd_type = gld(&ndp->type), nd_player = gld(&ndp->player), nd_pot = gld(&ndp->pot);
     const int nd_leaf = gld(&ndp->leaf_idx), nd_mask = gld(&ndp->active_mask), nd_bet = gld(&ndp->bets[trav]);
     const int Mp = s->Mp, NP = s->NP, nh = s->nh[trav], is_river = s->is_river, has_leaf = s->has_leaf;
+    // every player's reach row at this node, read with the node's fields
+    int rrq[SS_MAX_PLAYERS];
+#pragma unroll
+    for (int q = 0; q < SS_MAX_PLAYERS; ++q) rrq[q] = q < NP ? gld(s->rrow + (size_t)q * s->N + n) : 0;
     const int type = kind == TK_MASS ? SS_NODE_FOLD : nd_type;
     int opp[6], k = 0;
     for (int q = 0; q < NP; ++q) {
@@ -674,6 +696,22 @@ __device__ void terminal_node(const SlotDesc *__restrict__ s, int n, int trav, c
     // ---- closed forms, k <= 2 ----
     const bool sd = SDK < 0 ? type == SS_NODE_SHOWDOWN : SDK == 1;
     const bool two = MODE != TM_LIGHT && k == 2;
+    // the opponents' hands and reach (and a showdown's strength-order
+    // entries) are fetched first: their latency overlaps the clearing below.
+    // 512-thread workgroups (two-opponent showdowns) need three hands per
+    // thread for a street's at most 1128, others five.
+    constexpr int SCN = (MODE == TM_PAIR && SDK == 1) ? 3 : 5;
+    uint32_t oi1[SCN], oi2[SCN], gs[SCAN_CHUNK];
+    T r1[SCN], r2[SCN];
+    int rr1 = 0, rr2 = 0;
+#pragma unroll
+    for (int q = 0; q < SS_MAX_PLAYERS; ++q) {   // static indices: rrq stays in registers
+        rr1 = k >= 1 && q == opp[0] ? rrq[q] : rr1;
+        rr2 = k >= 2 && q == opp[1] ? rrq[q] : rr2;
+    }
+    if (k >= 1) scatter_load<T, SCN>(s, opp[0], rr1, reach, oi1, r1, tid, nt);
+    if (two) scatter_load<T, SCN>(s, opp[1], rr2, reach, oi2, r2, tid, nt);
+    if (sd && k >= 1) scan_load(s, gs, tid);
     // Folds and leaves walk the traverser's hands in order: this thread's
     // first HPT hand records are loaded together now, so their latency
     // overlaps the table setup instead of one round trip per hand.
@@ -693,16 +731,16 @@ __device__ void terminal_node(const SlotDesc *__restrict__ s, int n, int trav, c
         }
     }
     __syncthreads();
-    if (k >= 1) scatter<T>(s, opp[0], n, reach, B.M1, tid, nt);
-    if (two) scatter<T>(s, opp[1], n, reach, B.M2, tid, nt);
+    if (k >= 1) scatter_store<T, SCN>(s, opp[0], rr1, reach, B.M1, oi1, r1, tid, nt);
+    if (two) scatter_store<T, SCN>(s, opp[1], rr2, reach, B.M2, oi2, r2, tid, nt);
     __syncthreads();
     if (sd && k >= 1)
         card_prefix<T>(B.clist, B.clen, B.M1, two ? B.M2 : nullptr, B.PC1, B.PC2, B.PCP, B.C1, B.C2, B.CP, tid, nt);
     else if (k >= 1)
         card_totals<T>(B.M1, two ? B.M2 : nullptr, B.C1, B.C2, B.CP, tid, nt);
     __syncthreads();
-    if (sd && k == 1) scan_sorted<T, 1>(s, B.M1, nullptr, B.PG, B.red, tid);
-    if (sd && two) scan_sorted<T, 3>(s, B.M1, B.M2, B.PG, B.red, tid);
+    if (sd && k == 1) scan_sorted<T, 1>(s, B.M1, nullptr, B.PG, B.red, gs, tid);
+    if (sd && two) scan_sorted<T, 3>(s, B.M1, B.M2, B.PG, B.red, gs, tid);
     if (two && !sd) {
         if constexpr (std::is_same<T, float>::value)
             pair_products_mfma(B.M1, B.M2, B.C1, B.C2, B.PC1, B.U1, B.U2, B.red, tid, nt);
