@@ -512,8 +512,39 @@ __device__ __forceinline__ void card_totals(const T *M1, const T *M2, T *C1, T *
     }
 }
 
-// per-card prefix sums over strength-sorted partner lists (showdown):
-// PC[c*52 + j] = sum of the first j partners of card c; C = list totals.
+// per-card prefix sums of one matrix over strength-sorted partner lists
+// (showdown, one opponent): PC1[c*52 + j] = sum of the first j partners of
+// card c, C1 = list totals. One lane per card sums its list in order (51
+// steps), the list read as 13 words at once (in LDS or global memory).
+// Entries past a card's list length are cards too (the lists are
+// zero-filled), so every read is in bounds; they add zero. With one table
+// this beats a wavefront scan per card.
+template <typename T>
+__device__ __forceinline__ void card_prefix1(const uint8_t *clist, const uint8_t *clen, const T *M1, T *PC1, T *C1,
+                                             int tid) {
+    if (tid >= 52) return;
+    const int c = tid;
+    const uint32_t *cw = (const uint32_t *)(clist + c * DN);   // 52 * c bytes: word aligned
+    uint32_t wv[13];
+#pragma unroll
+    for (int w = 0; w < 13; ++w) wv[w] = cw[w];
+    const int len = clen[c];
+    const T *A = M1 + c * DN;
+    T *P = PC1 + c * DN;
+    T run = 0;
+    P[0] = run;
+#pragma unroll
+    for (int j = 0; j < 51; ++j) {
+        const int d = (wv[j >> 2] >> (8 * (j & 3))) & 0xFF;
+        run += j < len ? A[d] : (T)0;
+        P[j + 1] = run;
+    }
+    C1[c] = run;
+}
+
+// per-card prefix sums over strength-sorted partner lists (showdown, two
+// opponents): PC[c*52 + j] = sum of the first j partners of card c; C = list
+// totals, for M1, M2 and M1.*M2.
 // One wavefront per card (lane j = the card's j-th partner: gather, scan and
 // write in one pass), two cards in flight per wavefront so that their LDS
 // reads and scan steps overlap.
@@ -734,8 +765,10 @@ __device__ void terminal_node(const SlotDesc *__restrict__ s, int n, int trav, c
     if (k >= 1) scatter_store<T, SCN>(s, opp[0], rr1, reach, B.M1, oi1, r1, tid, nt);
     if (two) scatter_store<T, SCN>(s, opp[1], rr2, reach, B.M2, oi2, r2, tid, nt);
     __syncthreads();
-    if (sd && k >= 1)
-        card_prefix<T>(B.clist, B.clen, B.M1, two ? B.M2 : nullptr, B.PC1, B.PC2, B.PCP, B.C1, B.C2, B.CP, tid, nt);
+    if (sd && two)
+        card_prefix<T>(B.clist, B.clen, B.M1, B.M2, B.PC1, B.PC2, B.PCP, B.C1, B.C2, B.CP, tid, nt);
+    else if (sd && k >= 1)
+        card_prefix1<T>(B.clist, B.clen, B.M1, B.PC1, B.C1, tid);
     else if (k >= 1)
         card_totals<T>(B.M1, two ? B.M2 : nullptr, B.C1, B.C2, B.CP, tid, nt);
     __syncthreads();
