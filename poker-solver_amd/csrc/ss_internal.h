@@ -161,7 +161,7 @@ int build_phased_tasks(const std::vector<Slot *> &slots, PhasedTasks &pt, hipStr
 int launch_phased_iteration(SlotDesc *d_slots, const PhasedTasks &pt, hipStream_t st);
 int launch_phased_expl(SlotDesc *d_slots, const PhasedTasks &pt, const int *d_want, double *scratch,
                        int scratch_blocks, hipStream_t st);
-#define SS_XSCRATCH_DOUBLES (5 * 2704 + 3 * 1328 + 5 * 52 + 64)
+#define SS_XSCRATCH_DOUBLES (5 * 2704 + 3 * 1328 + 6 * 52 + 64)
 
 // debug / test entry points implemented in the kernels unit
 int dbg_eval7_device(const int *cards, int n, uint32_t *out);

@@ -248,6 +248,7 @@ struct TermBuf {
     T *PG;               // [3][PGS] prefix over strength-sorted combos of M1, M2, M1.*M2
     T *C1, *C2, *CP;     // [52] per-card totals (CP of M1 .* M2)
     T *U1, *U2;          // [52] C1 * M2 and C2 * M1
+    T *XD;               // [52] column sums of M1 .* M2 (2-opponent showdowns)
     T *red;              // [64] wave partials
     const uint8_t *clist;  // [52*52] per-card partner lists, ascending strength
     const uint8_t *clen;   // [52]
@@ -663,6 +664,29 @@ __device__ __forceinline__ void pair_products_mfma(const float *M1, const float 
 }
 
 // Kinds of terminal evaluation
+// Two-opponent showdowns: for column x, U1[x] = sum_c C1[c] M2[c][x],
+// U2[x] = sum_c C2[c] M1[c][x], XD[x] = sum_c M1[c][x] M2[c][x]. With
+// TT = sum_c C1[c] C2[c] they turn the per-card product
+//   sum_c (C1[c] - M1[c][x] - M1[c][y]) (C2[c] - M2[c][x] - M2[c][y])
+// into TT - U1[x] - U1[y] - U2[x] - U2[y] + XD[x] + XD[y]
+//    + sum_c (M1[c][x] M2[c][y] + M1[c][y] M2[c][x]),
+// leaving only the last sum to the per-hand loop.
+template <typename T>
+__device__ __forceinline__ void showdown_pair_vectors(const T *M1, const T *M2, const T *C1, const T *C2, T *U1,
+                                                      T *U2, T *XD, int x) {
+    T u1 = 0, u2 = 0, dd = 0;
+#pragma unroll 4
+    for (int c = 0; c < 52; ++c) {
+        const T m1 = M1[c * DN + x], m2 = M2[c * DN + x];
+        u1 += C1[c] * m2;
+        u2 += C2[c] * m1;
+        dd += m1 * m2;
+    }
+    U1[x] = u1;
+    U2[x] = u2;
+    XD[x] = dd;
+}
+
 #define TK_NODE 0      // the node's own type (fold / showdown / leaf)
 #define TK_MASS 1      // opponent mass only, all non-traversers, times `scale`
 
@@ -773,7 +797,18 @@ __device__ void terminal_node(const SlotDesc *__restrict__ s, int n, int trav, c
         card_totals<T>(B.M1, two ? B.M2 : nullptr, B.C1, B.C2, B.CP, tid, nt);
     __syncthreads();
     if (sd && k == 1) scan_sorted<T, 1>(s, B.M1, nullptr, B.PG, B.red, gs, tid);
-    if (sd && two) scan_sorted<T, 3>(s, B.M1, B.M2, B.PG, B.red, gs, tid);
+    if (sd && two) {
+        // column vectors of the opponent-pair correction, by threads the
+        // strength-order scan leaves idle when there are enough
+        const bool spare = nt >= SCAN_LANES + 64;
+        if (spare && tid >= SCAN_LANES && tid < SCAN_LANES + 52)
+            showdown_pair_vectors<T>(B.M1, B.M2, B.C1, B.C2, B.U1, B.U2, B.XD, tid - SCAN_LANES);
+        scan_sorted<T, 3>(s, B.M1, B.M2, B.PG, B.red, gs, tid);
+        if (!spare) {
+            if (tid < 52) showdown_pair_vectors<T>(B.M1, B.M2, B.C1, B.C2, B.U1, B.U2, B.XD, tid);
+            __syncthreads();
+        }
+    }
     if (two && !sd) {
         if constexpr (std::is_same<T, float>::value)
             pair_products_mfma(B.M1, B.M2, B.C1, B.C2, B.PC1, B.U1, B.U2, B.red, tid, nt);
@@ -787,11 +822,11 @@ __device__ void terminal_node(const SlotDesc *__restrict__ s, int n, int trav, c
     if (two) {
         S2 = (T)0.5 * tsum52(B.C2);
         SP = (T)0.5 * tsum52(B.CP);
-        if (!sd) {
-            if constexpr (std::is_same<T, float>::value) TT = B.red[0];   // from the MFMA padding
-            else
+        if (!sd && std::is_same<T, float>::value) {
+            TT = B.red[0];   // from the MFMA padding
+        } else {
 #pragma unroll 4
-                for (int c = 0; c < 52; ++c) TT += B.C1[c] * B.C2[c];
+            for (int c = 0; c < 52; ++c) TT += B.C1[c] * B.C2[c];
         }
     }
 
@@ -866,7 +901,7 @@ __device__ void terminal_node(const SlotDesc *__restrict__ s, int n, int trav, c
                 // terms of card c: opponent pairs sharing c, with the hands
                 // {c,x}, {c,y} (not opponent hands) removed from the classes
                 // they fall in, by their position in c's sorted list
-                auto card = [&](int c, int l4, int e4, T &xw, T &xaa) {
+                auto card = [&](int c, int l4, int e4, T &xw, T &xaa, bool full) {
                     const T P1l = at4(B.PC1 + c * DN, l4), P2l = at4(B.PC2 + c * DN, l4);
                     const T P1e = at4(B.PC1 + c * DN, e4), P2e = at4(B.PC2 + c * DN, e4);
                     const int px4 = B.cpos[c * DN + x], py4 = B.cpos[c * DN + y];
@@ -880,7 +915,8 @@ __device__ void terminal_node(const SlotDesc *__restrict__ s, int n, int trav, c
                     const T Ble = P2e - fxe * m2x - fye * m2y;
                     xw += Al * ((T)2 * Bl + Ble);
                     xw += Ale * (Bl + (T)2 * Ble);
-                    xaa += (B.C1[c] - m1x - m1y) * (B.C2[c] - m2x - m2y);
+                    if (full) xaa += (B.C1[c] - m1x - m1y) * (B.C2[c] - m2x - m2y);
+                    else xaa += m1x * m2y + m1y * m2x;
                 };
                 // every card (board cards have empty lists: exact zeros), then
                 // the traverser's own two cards, which carry no opponent pairs,
@@ -895,14 +931,14 @@ __device__ void terminal_node(const SlotDesc *__restrict__ s, int n, int trav, c
                     }
 #pragma unroll
                     for (int b = 0; b < 4; ++b)
-                        card(4 * g + b, (lw >> (8 * b)) & 0xFF, (ew >> (8 * b)) & 0xFF, xw, xaa);
+                        card(4 * g + b, (lw >> (8 * b)) & 0xFF, (ew >> (8 * b)) & 0xFF, xw, xaa, false);
                 }
                 {
                     T ow = 0, oa = 0;
-                    card(x, lx4, ex4, ow, oa);
-                    card(y, ly4, ey4, ow, oa);
+                    card(x, lx4, ex4, ow, oa, true);
+                    card(y, ly4, ey4, ow, oa, true);
                     xw -= ow;
-                    xaa -= oa;
+                    xaa += TT - B.U1[x] - B.U1[y] - B.U2[x] - B.U2[y] + B.XD[x] + B.XD[y] - oa;
                 }
                 const int lx = lx4 >> 2, ly = ly4 >> 2, ex = ex4 >> 2, ey = ey4 >> 2;
                 const T m1k = B.M1[xy], m2k = B.M2[xy];
@@ -940,7 +976,7 @@ struct TermHeavyT {
     alignas(16) uint8_t clist[DN2];
     alignas(16) uint8_t cpos[DN2];
     alignas(16) uint8_t clen[64];
-    T C1[52], C2[52], CP[52], U1[52], U2[52];
+    T C1[52], C2[52], CP[52], U1[52], U2[52], XD[52];
     T red[64];
     T PG[3 * PGS];
     alignas(256) T M1[DNP];
@@ -953,7 +989,7 @@ template <typename T>
 __device__ __forceinline__ TermBuf<T> heavy_buf(TermHeavyT<T> &t) {
     TermBuf<T> b;
     b.M1 = t.M1; b.M2 = t.M2; b.PC1 = t.PC1; b.PC2 = t.PC2; b.PCP = t.PCP;
-    b.PG = t.PG; b.C1 = t.C1; b.C2 = t.C2; b.CP = t.CP; b.U1 = t.U1; b.U2 = t.U2; b.red = t.red;
+    b.PG = t.PG; b.C1 = t.C1; b.C2 = t.C2; b.CP = t.CP; b.U1 = t.U1; b.U2 = t.U2; b.XD = t.XD; b.red = t.red;
     b.clist = t.clist; b.clen = t.clen; b.cpos = t.cpos;
     return b;
 }
@@ -971,7 +1007,7 @@ template <typename T>
 __device__ __forceinline__ TermBuf<T> fold_buf(TermFoldT<T> &t) {
     TermBuf<T> b;
     b.M1 = t.M1; b.M2 = t.M2; b.PC1 = t.G; b.PC2 = nullptr; b.PCP = nullptr;
-    b.PG = nullptr; b.C1 = t.C1; b.C2 = t.C2; b.CP = t.CP; b.U1 = t.U1; b.U2 = t.U2; b.red = t.red;
+    b.PG = nullptr; b.C1 = t.C1; b.C2 = t.C2; b.CP = t.CP; b.U1 = t.U1; b.U2 = t.U2; b.XD = nullptr; b.red = t.red;
     b.clist = nullptr; b.clen = nullptr; b.cpos = nullptr;
     return b;
 }
@@ -1003,13 +1039,14 @@ template <typename T>
 __device__ __forceinline__ TermBuf<T> light_buf(TermLightT<T> &t, const SlotDesc *__restrict__ s) {
     TermBuf<T> b;
     b.M1 = t.M1; b.M2 = nullptr; b.PC1 = t.PC1; b.PC2 = nullptr; b.PCP = nullptr;
-    b.PG = t.PG; b.C1 = t.C1; b.C2 = nullptr; b.CP = nullptr; b.U1 = nullptr; b.U2 = nullptr; b.red = t.red;
+    b.PG = t.PG; b.C1 = t.C1; b.C2 = nullptr; b.CP = nullptr; b.U1 = nullptr; b.U2 = nullptr; b.XD = nullptr;
+    b.red = t.red;
     b.clist = s->clist; b.clen = s->clen; b.cpos = s->cpos;
     return b;
 }
 
 // FP64 buffers live in global scratch: one slice per workgroup
-#define XSCRATCH_DOUBLES (5 * DN2 + 3 * PGS + 5 * 52 + 64)
+#define XSCRATCH_DOUBLES (5 * DN2 + 3 * PGS + 6 * 52 + 64)
 __device__ __forceinline__ TermBuf<double> global_buf(double *base, const SlotDesc *__restrict__ s) {
     TermBuf<double> b;
     b.M1 = base; b.M2 = b.M1 + DN2;
@@ -1017,6 +1054,7 @@ __device__ __forceinline__ TermBuf<double> global_buf(double *base, const SlotDe
     b.PG = b.PCP + DN2;
     b.C1 = b.PG + 3 * PGS; b.C2 = b.C1 + 52; b.CP = b.C2 + 52; b.U1 = b.CP + 52; b.U2 = b.U1 + 52;
     b.red = b.U2 + 52;
+    b.XD = b.red + 64;
     b.clist = s->clist;
     b.clen = s->clen;
     b.cpos = s->cpos;
