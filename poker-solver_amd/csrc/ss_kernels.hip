@@ -265,14 +265,6 @@ template <typename T> __device__ __forceinline__ T mass_floor();
 template <> __device__ __forceinline__ float mass_floor<float>() { return 1e-5f; }
 template <> __device__ __forceinline__ double mass_floor<double>() { return 1e-12; }
 
-template <typename T>
-__device__ __forceinline__ T tsum52(const T *C) {
-    T s = 0;
-#pragma unroll 4
-    for (int c = 0; c < 52; ++c) s += C[c];
-    return s;
-}
-
 // Cross-lane moves by DPP (data-parallel primitives): a lane whose source is
 // outside the pattern, or whose row is masked off, reads 0. CTRL 0x110 + n is
 // row_shr:n (within 16-lane rows), 0x142 row_bcast:15 (lane 15 of each row
@@ -318,6 +310,13 @@ __device__ __forceinline__ T wave_incl_scan(T x, int lane) {
 template <typename T>
 __device__ __forceinline__ T wave_sum(T x) {
     return lane_value(wave_incl_scan(x, 0), 63);
+}
+
+// sum of a 52-card table, lane c holding card c (every lane active): the same
+// value in every wavefront, whatever the block size
+template <typename T>
+__device__ __forceinline__ T csum52(const T *C, int lane) {
+    return wave_sum(lane < 52 ? C[lane] : (T)0);
 }
 
 // exclusive scans over the strength-sorted live combos: PG[0..ng] of M1 and,
@@ -818,16 +817,15 @@ __device__ void terminal_node(const SlotDesc *__restrict__ s, int n, int trav, c
     }
 
     T S1 = 0, S2 = 0, SP = 0, TT = 0;
-    if (k >= 1) S1 = (T)0.5 * tsum52(B.C1);
+    const int lane = tid & 63;
+    if (k >= 1) S1 = (T)0.5 * csum52(B.C1, lane);
     if (two) {
-        S2 = (T)0.5 * tsum52(B.C2);
-        SP = (T)0.5 * tsum52(B.CP);
-        if (!sd && std::is_same<T, float>::value) {
+        S2 = (T)0.5 * csum52(B.C2, lane);
+        SP = (T)0.5 * csum52(B.CP, lane);
+        if (!sd && std::is_same<T, float>::value)
             TT = B.red[0];   // from the MFMA padding
-        } else {
-#pragma unroll 4
-            for (int c = 0; c < 52; ++c) TT += B.C1[c] * B.C2[c];
-        }
+        else
+            TT = wave_sum(lane < 52 ? B.C1[lane] * B.C2[lane] : (T)0);
     }
 
     // Showdowns walk the traverser's hands by strength (shared class
