@@ -914,7 +914,10 @@ __device__ void terminal_node(const SlotDesc *__restrict__ s, int n, int trav, c
                     xw += Al * ((T)2 * Bl + Ble);
                     xw += Ale * (Bl + (T)2 * Ble);
                     if (full) xaa += (B.C1[c] - m1x - m1y) * (B.C2[c] - m2x - m2y);
-                    else xaa += m1x * m2y + m1y * m2x;
+                    else {
+                        xaa = fma(m1x, m2y, xaa);
+                        xaa = fma(m1y, m2x, xaa);
+                    }
                 };
                 // every card (board cards have empty lists: exact zeros), then
                 // the traverser's own two cards, which carry no opponent pairs,
@@ -965,9 +968,13 @@ __device__ void terminal_node(const SlotDesc *__restrict__ s, int n, int trav, c
 // allows 160 KB per workgroup): both opponents' matrices, three prefix
 // tables, the subgame's per-card lists. Light (<= 1 opponent, ~30 KB): one
 // matrix, one prefix table; five workgroups fit a CU.
-// Rows of 2752 = 43 x 64 floats: the same cell of M1 and M2 (PC1 and PC2)
-// is one ds_read2st64 away. The byte tables and short arrays come first so
-// their addresses fit the 16-bit immediate offsets of LDS instructions.
+// Rows of 2752 = 43 x 64 floats: the same cell of M1 and M2 is one
+// ds_read2st64 away. PC2 sits one float past that spacing from PC1, so the
+// two per-card prefix reads of the showdown loop stay single reads sharing
+// one address register, each card's row offset in the 16-bit immediate (a
+// paired read2st64 cannot encode a 208-byte row step and costs an add per
+// card). The byte tables and short arrays come first so their addresses fit
+// the 16-bit immediate offsets of LDS instructions.
 #define DNP 2752
 template <typename T>
 struct TermHeavyT {
@@ -979,7 +986,7 @@ struct TermHeavyT {
     T PG[3 * PGS];
     alignas(256) T M1[DNP];
     T M2[DNP];
-    T PC1[DNP], PC2[DNP], PCP[DNP];
+    T PC1[DNP + 1], PC2[DNP], PCP[DN2];
 };
 using TermHeavy = TermHeavyT<float>;   // 76 KB; the FP64 one (150 KB) serves exploitability checks
 
