@@ -1403,6 +1403,49 @@ __device__ __forceinline__ void regret_match_vals(const float *r, int na, int fa
 // progress are fetched in one round trip.
 #define SWEEP_PIN(x) asm volatile("" ::"v"(x))
 
+// A sweep task's record fields, read into scalars together before anything
+// branches on them (one round trip instead of one per use).
+struct SweepTask {
+    int node, Mp, N, na, ab, fa, arow, nh, actor, NP, slot;
+    int ch[NT_DEC];
+    float *R, *SS, *cfv, *reach;
+};
+// nhp: whose hand count (the actor for the forward sweep, the traverser for
+// the backward sweep)
+__device__ __forceinline__ SweepTask sweep_task(const SweepRec *__restrict__ kr, int trav, bool fwd) {
+    SweepTask t;
+    t.node = kr->node; t.Mp = kr->Mp; t.N = kr->N; t.na = kr->na; t.ab = kr->act_base; t.fa = kr->fa;
+    t.arow = kr->arow; t.actor = kr->actor; t.NP = kr->NP; t.slot = kr->slot;
+#pragma unroll
+    for (int a = 0; a < NT_DEC; ++a) t.ch[a] = kr->ch[a];
+    int nh[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) nh[q] = kr->nh[q];
+    t.R = kr->R; t.SS = kr->SS; t.cfv = kr->cfv; t.reach = kr->reach;
+    // every field pinned here: the loads are issued together and waited once
+    asm volatile("" ::"s"(t.node), "s"(t.Mp), "s"(t.N), "s"(t.na), "s"(t.ab), "s"(t.fa), "s"(t.arow), "s"(t.actor));
+    asm volatile("" ::"s"(t.NP), "s"(t.slot), "s"(t.R), "s"(t.SS), "s"(t.cfv), "s"(t.reach));
+    asm volatile("" ::"s"(t.ch[0]), "s"(t.ch[1]), "s"(t.ch[2]), "s"(t.ch[3]), "s"(t.ch[4]), "s"(t.ch[5]),
+                 "s"(t.ch[6]), "s"(t.ch[7]));
+    asm volatile("" ::"s"(nh[0]), "s"(nh[1]), "s"(nh[2]), "s"(nh[3]), "s"(nh[4]), "s"(nh[5]));
+    const int w = fwd ? t.actor : trav;   // selects, static indices: nh stays in registers
+    int x = nh[0];
+#pragma unroll
+    for (int q = 1; q < 6; ++q) x = w == q ? nh[q] : x;
+    t.nh = x;
+    return t;
+}
+
+// A workgroup walks several tasks (grid-stride): the record of its next task
+// is touched (one word in each of its two 64-byte lines) while the current
+// task's loads are in flight, and the words are consumed only at the end of
+// the task, so the next task's record reads hit the scalar cache.
+__device__ __forceinline__ int2 sweep_warm(const SweepRec *__restrict__ recs, int jn, int ntasks) {
+    const SweepRec *r = recs + (jn < ntasks ? jn : 0);
+    return make_int2(r->slot, r->nh[0]);
+}
+#define SWEEP_USE(w) asm volatile("" ::"s"((w).x), "s"((w).y))
+
 // forward_one for a whole task (same results). The task's record carries the
 // node: its fields are read into scalars up front (uniform address, so scalar
 // loads). H hands per thread (h0 + u * blockDim.x), every load of the H hands
@@ -1410,16 +1453,16 @@ __device__ __forceinline__ void regret_match_vals(const float *r, int na, int fa
 // count, so a node with few actions issues no repeated loads. Only the actor's
 // child rows are written (reach-row table).
 template <int A, int H>
-__device__ __forceinline__ void fwd_task(const SweepRec *__restrict__ kr, int actor, int trav, int s_act, int s_it,
-                                         int s_end, int s_np) {
-    const int Mp = kr->Mp, N = kr->N, na = kr->na, ab = kr->act_base, fa = kr->fa;
+__device__ __forceinline__ void fwd_task(const SweepTask &kr, int actor, int trav, int s_act, int s_it, int s_end,
+                                         int s_np) {
+    const int Mp = kr.Mp, N = kr.N, na = kr.na, ab = kr.ab, fa = kr.fa;
     int ch[A];
 #pragma unroll
-    for (int a = 0; a < A; ++a) ch[a] = kr->ch[a];
-    const int nh = kr->nh[actor], nt = blockDim.x;
-    const float *R = kr->R;
-    float *ra = kr->reach + (size_t)actor * N * Mp;
-    const float *par = ra + (size_t)kr->arow * Mp;
+    for (int a = 0; a < A; ++a) ch[a] = kr.ch[a];
+    const int nh = kr.nh, nt = SWEEP_BLOCK;
+    const float *R = kr.R;
+    float *ra = kr.reach + (size_t)actor * N * Mp;
+    const float *par = ra + (size_t)kr.arow * Mp;
     bool checked = false;
     for (int h0 = threadIdx.x; h0 < nh; h0 += H * nt) {
         float r[H][A], pr[H];
@@ -1453,15 +1496,18 @@ __device__ __forceinline__ void fwd_task(const SweepRec *__restrict__ kr, int ac
 __global__ void __launch_bounds__(SWEEP_BLOCK) k_fwd_phased(SlotDesc *__restrict__ slots,
                                                             const SweepRec *__restrict__ recs, int ntasks,
                                                             int trav) {
-    for (int j = blockIdx.x; j < ntasks; j += gridDim.x) {
-        const SweepRec *kr = recs + j;
-        const int actor = kr->actor, NP = kr->NP;
+    const int G = gridDim.x;
+    for (int j = blockIdx.x; j < ntasks; j += G) {
+        const SweepTask kr = sweep_task(recs + j, trav, true);
+        const int2 warm = sweep_warm(recs, j + G, ntasks);
+        const int actor = kr.actor, NP = kr.NP;
         if (actor == trav || actor < 0 || actor >= NP) continue;
-        const SlotDesc *s = slots + kr->slot;
+        const SlotDesc *s = slots + kr.slot;
         const int s_act = s->active, s_it = s->iter, s_end = s->iter_end, s_np = s->NP;
         // two hands per thread: a node has at most 1128 hands
-        if (kr->na <= 3) fwd_task<3, 2>(kr, actor, trav, s_act, s_it, s_end, s_np);
+        if (kr.na <= 3) fwd_task<3, 2>(kr, actor, trav, s_act, s_it, s_end, s_np);
         else fwd_task<NT_DEC, 2>(kr, actor, trav, s_act, s_it, s_end, s_np);
+        SWEEP_USE(warm);
     }
 }
 
@@ -1563,14 +1609,13 @@ __device__ __forceinline__ void bwd_hand(int na, int fa, int ab, int Mp, int h, 
 // regret and strategy-sum rows) issued together with the read of the slot's
 // progress. A bounds the action count.
 template <int A, int H, bool OWN>
-__device__ __forceinline__ void bwd_task(const SweepRec *__restrict__ kr, int trav, int s_act, int s_it, int s_end,
-                                         int s_np) {
-    const int n = kr->node, Mp = kr->Mp, na = kr->na, ab = kr->act_base, fa = kr->fa;
+__device__ __forceinline__ void bwd_task(const SweepTask &kr, int trav, int s_act, int s_it, int s_end, int s_np) {
+    const int n = kr.node, Mp = kr.Mp, na = kr.na, ab = kr.ab, fa = kr.fa;
     int ch[A];
 #pragma unroll
-    for (int a = 0; a < A; ++a) ch[a] = kr->ch[a];
-    const int nh = kr->nh[trav], nt = blockDim.x;   // trav < NP <= 6
-    float *R = kr->R, *SS = kr->SS, *cfv = kr->cfv;
+    for (int a = 0; a < A; ++a) ch[a] = kr.ch[a];
+    const int nh = kr.nh, nt = SWEEP_BLOCK;
+    float *R = kr.R, *SS = kr.SS, *cfv = kr.cfv;
     float *out = cfv + (size_t)n * Mp;
     bool checked = false;
     float d = 0.f;
@@ -1615,18 +1660,17 @@ __device__ __forceinline__ void bwd_task(const SweepRec *__restrict__ kr, int tr
 // with the read of the slot's progress. (As a bwd_task instance the row
 // addresses are kept live across rounds.)
 template <int A>
-__device__ __forceinline__ void bwd_own(const SweepRec *__restrict__ kr, int trav, int s_act, int s_it, int s_end,
-                                        int s_np) {
-    const int n = kr->node, Mp = kr->Mp, na = kr->na, ab = kr->act_base, fa = kr->fa;
+__device__ __forceinline__ void bwd_own(const SweepTask &kr, int trav, int s_act, int s_it, int s_end, int s_np) {
+    const int n = kr.node, Mp = kr.Mp, na = kr.na, ab = kr.ab, fa = kr.fa;
     int ch[A];
 #pragma unroll
-    for (int a = 0; a < A; ++a) ch[a] = kr->ch[a];
-    const int nh = kr->nh[trav];
-    float *R = kr->R, *SS = kr->SS, *cfv = kr->cfv;
+    for (int a = 0; a < A; ++a) ch[a] = kr.ch[a];
+    const int nh = kr.nh;
+    float *R = kr.R, *SS = kr.SS, *cfv = kr.cfv;
     bool checked = false;
     float d = 0.f;
     int iter = 0;
-    for (int h = threadIdx.x; h < nh; h += blockDim.x) {
+    for (int h = threadIdx.x; h < nh; h += SWEEP_BLOCK) {
         float cv[A], r[A], sg[A];
         const unsigned o = 4u * (unsigned)h;
 #pragma unroll
@@ -1655,12 +1699,14 @@ __device__ __forceinline__ void bwd_own(const SweepRec *__restrict__ kr, int tra
 __global__ void __launch_bounds__(SWEEP_BLOCK) k_bwd_phased(SlotDesc *__restrict__ slots,
                                                             const SweepRec *__restrict__ recs, int ntasks,
                                                             int trav) {
-    for (int j = blockIdx.x; j < ntasks; j += gridDim.x) {
-        const SweepRec *kr = recs + j;
-        const SlotDesc *s = slots + kr->slot;
+    const int G = gridDim.x;
+    for (int j = blockIdx.x; j < ntasks; j += G) {
+        const SweepTask kr = sweep_task(recs + j, trav, false);
+        const int2 warm = sweep_warm(recs, j + G, ntasks);
+        const SlotDesc *s = slots + kr.slot;
         const int s_act = s->active, s_it = s->iter, s_end = s->iter_end, s_np = s->NP;
-        const bool few = kr->na <= 3;
-        if (kr->actor == trav) {
+        const bool few = kr.na <= 3;
+        if (kr.actor == trav) {
             if (few) bwd_own<3>(kr, trav, s_act, s_it, s_end, s_np);
             else bwd_own<NT_DEC>(kr, trav, s_act, s_it, s_end, s_np);
         } else {
@@ -2104,6 +2150,13 @@ int build_phased_tasks(const std::vector<Slot *> &slots, PhasedTasks &pt, hipStr
     return SS_OK;
 }
 
+// sweep launches: at most this many workgroups (six per CU), each walking
+// its share of the level's tasks (SS_SWEEP_GRID overrides)
+static const int SWEEP_GRID = [] {
+    const char *e = getenv("SS_SWEEP_GRID");
+    return e && atoi(e) > 0 ? atoi(e) : 1536;
+}();
+
 int launch_phased_iteration(SlotDesc *d_slots, const PhasedTasks &pt, hipStream_t st) {
     for (int t = 0; t < pt.NPmax; ++t) {
         const int *fo = &pt.sw_off[(size_t)t * 2 * (pt.L + 1)], *bo = fo + pt.L + 1;
@@ -2111,7 +2164,8 @@ int launch_phased_iteration(SlotDesc *d_slots, const PhasedTasks &pt, hipStream_
         for (int l = 0; l < pt.L; ++l) {
             int cnt = fo[l + 1] - fo[l];
             if (cnt > 0)
-                hipLaunchKernelGGL(k_fwd_phased, dim3(cnt), dim3(SWEEP_BLOCK), 0, st, d_slots, pt.d_recs + fo[l], cnt, t);
+                hipLaunchKernelGGL(k_fwd_phased, dim3(std::min(cnt, SWEEP_GRID)), dim3(SWEEP_BLOCK), 0, st, d_slots,
+                                   pt.d_recs + fo[l], cnt, t);
         }
         if (pt.npsplit > 0) {
             int rc = launch_prec_phased(d_slots, pt, t, st);
@@ -2132,7 +2186,8 @@ int launch_phased_iteration(SlotDesc *d_slots, const PhasedTasks &pt, hipStream_
         for (int l = pt.L - 1; l >= 0; --l) {
             int cnt = bo[l + 1] - bo[l];
             if (cnt > 0)
-                hipLaunchKernelGGL(k_bwd_phased, dim3(cnt), dim3(SWEEP_BLOCK), 0, st, d_slots, pt.d_recs + bo[l], cnt, t);
+                hipLaunchKernelGGL(k_bwd_phased, dim3(std::min(cnt, SWEEP_GRID)), dim3(SWEEP_BLOCK), 0, st, d_slots,
+                                   pt.d_recs + bo[l], cnt, t);
         }
     }
     hipLaunchKernelGGL(k_bump_phased, dim3((pt.nslots + 255) / 256), dim3(256), 0, st, d_slots, pt.nslots);
