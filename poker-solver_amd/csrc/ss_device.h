@@ -10,6 +10,9 @@
 //   dec_off   int[L+1]                level l = dec_lvl[dec_off[l] .. dec_off[l+1])
 //   term      int[T]                fold / showdown / leaf nodes
 //   hinfo     uint32[NP][Mp]        packed hand: c0 | c1<<6 | combo<<12 | live<<23
+//                                   (c0, c1 in the table numbering: cards off
+//                                   the board first, board cards last; combo
+//                                   is the real combo id)
 //   weights   float[NP][Mp]         root reach (raw input weights)
 //   R, SS     float[A][Mp]          regrets / strategy sums, one row per action
 //                                   of each decision node (rows act_base..+nchild)

@@ -50,6 +50,10 @@ struct Plan {
     int nh[6] = {0, 0, 0, 0, 0, 0};
     int board[5] = {0, 0, 0, 0, 0};
     int num_board = 0;
+    // Card numbering of the device tables: the cards off the board first
+    // (ascending), the board cards last, so loops over the cards that can be
+    // in a hand stop early. Combo ids (strength lookups) stay real.
+    int lab[52] = {0};
     int starting_pot = 0;
     std::vector<DevNode> nodes;
     std::vector<int> children, dec_lvl, dec_off, term, dec_index;
@@ -75,7 +79,7 @@ struct RiverTables {
     std::vector<uint8_t> bnd;     // [(ncls+1)*52] class boundaries per card
     int ng = 0, ncls = 0;
 };
-void build_river_tables(const uint32_t *cstr, RiverTables &rt);
+void build_river_tables(const uint32_t *cstr, const int *lab, RiverTables &rt);
 
 // Precision-form buffers of a 2-player river subgame (ss_precision.hip).
 struct PrecState {

@@ -919,14 +919,15 @@ __device__ void terminal_node(const SlotDesc *__restrict__ s, int n, int trav, c
                         xaa = fma(m1y, m2x, xaa);
                     }
                 };
-                // every card (board cards have empty lists: exact zeros), then
-                // the traverser's own two cards, which carry no opponent pairs,
-                // taken back out
+                // every card that can be in a hand (the tables number the cards
+                // off the board first: 0..46 on a river, and card 47, a board
+                // card, has empty lists: exact zeros), then the traverser's own
+                // two cards, which carry no opponent pairs, taken back out
                 T xw = 0, xaa = 0;
 #pragma unroll 1
-                for (int g = 0; g < 13; ++g) {
+                for (int g = 0; g < 12; ++g) {   // cards 0..47: every card off a river board
                     const uint32_t lw = lnext, ew = enext;
-                    if (g < 12) {
+                    if (g < 11) {
                         lnext = gld(rowl + g + 1);
                         enext = gld(rowl + 14 + g);
                     }

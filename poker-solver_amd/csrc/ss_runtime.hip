@@ -200,13 +200,21 @@ int make_plan(const HostProblem &hp, Plan &pl) {
         if (td->board[b] < 0 || td->board[b] > 51) return set_error(SS_ERR_ARG, "board card out of range");
         board_mask[td->board[b]] = true;
     }
+    {
+        int nl = 0;
+        for (int c = 0; c < 52; ++c)
+            if (!board_mask[c]) pl.lab[c] = nl++;
+        for (int c = 0; c < 52; ++c)
+            if (board_mask[c]) pl.lab[c] = nl++;
+    }
     for (int p = 0; p < NP; ++p)
         for (int h = 0; h < hp.nh[p]; ++h) {
             int a = hp.hands[p][2 * h], b = hp.hands[p][2 * h + 1];
             if (a < 0 || a > 51 || b < 0 || b > 51 || a == b)
                 return set_error(SS_ERR_ARG, "player %d hand %d has bad cards (%d, %d)", p, h, a, b);
             uint32_t live = (!board_mask[a] && !board_mask[b]) ? SS_LIVE_BIT : 0u;
-            pl.hinfo[(size_t)p * Mp + h] = (uint32_t)a | ((uint32_t)b << 6) | ((uint32_t)combo_id(a, b) << 12) | live;
+            pl.hinfo[(size_t)p * Mp + h] =
+                (uint32_t)pl.lab[a] | ((uint32_t)pl.lab[b] << 6) | ((uint32_t)combo_id(a, b) << 12) | live;
             pl.weights[(size_t)p * Mp + h] = hp.weights[p][h];
         }
     // live combos unique within every player's list: card-pair scatters are
@@ -285,7 +293,7 @@ int make_plan(const HostProblem &hp, Plan &pl) {
 // ---------------------------------------------------------------------------
 // river tables from combo ranks
 // ---------------------------------------------------------------------------
-void build_river_tables(const uint32_t *cstr, RiverTables &rt) {
+void build_river_tables(const uint32_t *cstr, const int *lab, RiverTables &rt) {
     rt.cstr.assign(cstr, cstr + SS_NCOMBO);
     rt.glt.assign(SS_NCOMBO, 0);
     rt.gle.assign(SS_NCOMBO, 0);
@@ -302,7 +310,7 @@ void build_river_tables(const uint32_t *cstr, RiverTables &rt) {
     for (int i = 0; i < rt.ng; ++i) {
         int a = 0, rem = live[i];
         while (rem >= 51 - a) { rem -= 51 - a; ++a; }
-        rt.gsort[i] = (uint16_t)(a | ((a + 1 + rem) << 8));
+        rt.gsort[i] = (uint16_t)(lab[a] | (lab[a + 1 + rem] << 8));
         sorted[i] = cstr[live[i]];
     }
     // strength classes: distinct live ranks, ascending
@@ -324,16 +332,17 @@ void build_river_tables(const uint32_t *cstr, RiverTables &rt) {
             if (d != c && cstr[combo_id(c, d)]) part.push_back(d);
         std::stable_sort(part.begin(), part.end(),
                          [&](int a, int b) { return cstr[combo_id(c, a)] < cstr[combo_id(c, b)]; });
-        rt.clen[c] = (uint8_t)part.size();
+        const int lc = lab[c];
+        rt.clen[lc] = (uint8_t)part.size();
         std::vector<uint32_t> ps(part.size());
         for (size_t j = 0; j < part.size(); ++j) {
-            rt.clist[c * 52 + j] = (uint8_t)part[j];
-            rt.cpos[c * 52 + part[j]] = (uint8_t)(4 * j);
+            rt.clist[lc * 52 + j] = (uint8_t)lab[part[j]];
+            rt.cpos[lc * 52 + lab[part[j]]] = (uint8_t)(4 * j);
             ps[j] = cstr[combo_id(c, part[j])];
         }
         for (size_t t = 0; t < cls.size(); ++t)
-            rt.bnd[t * 52 + c] = (uint8_t)(4 * (std::lower_bound(ps.begin(), ps.end(), cls[t]) - ps.begin()));
-        rt.bnd[cls.size() * 52 + c] = (uint8_t)(4 * ps.size());
+            rt.bnd[t * 52 + lc] = (uint8_t)(4 * (std::lower_bound(ps.begin(), ps.end(), cls[t]) - ps.begin()));
+        rt.bnd[cls.size() * 52 + lc] = (uint8_t)(4 * ps.size());
     }
 }
 
@@ -467,7 +476,7 @@ int Slot::setup(const HostProblem &hp, hipStream_t st) {
         SS_HIP(hipMemcpyAsync(cstr.data(), a + o_cstr, sizeof(uint32_t) * SS_NCOMBO, hipMemcpyDeviceToHost, st));
         SS_HIP(hipStreamSynchronize(st));
         RiverTables rt;
-        build_river_tables(cstr.data(), rt);
+        build_river_tables(cstr.data(), pl.lab, rt);
         d.ng = rt.ng;
         SS_HIP(hipMemcpyAsync(a + o_gsort, rt.gsort.data(), sizeof(uint16_t) * rt.gsort.size(), hipMemcpyHostToDevice, st));
         SS_HIP(hipMemcpyAsync(a + o_cpos, rt.cpos.data(), 52 * 52, hipMemcpyHostToDevice, st));
