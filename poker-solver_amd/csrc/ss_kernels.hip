@@ -488,27 +488,28 @@ __device__ __forceinline__ void scatter_store(const SlotDesc *__restrict__ s, in
     for (int o = tid + SCN * nt; o < nq; o += nt) scatter_one(s, M, gld(hq + o), gld(rq + o));
 }
 
-// per-card totals of M1 (and M2, M1.*M2): one wavefront per card, lane =
-// partner card, fixed butterfly order
+// per-card totals of M1 (and M2, M1.*M2). The matrices are symmetric, so card
+// c's total is column c's sum: lane c of the first wavefront sums its column
+// down the rows in order (consecutive lanes read consecutive words).
 template <typename T>
 __device__ __forceinline__ void card_totals(const T *M1, const T *M2, T *C1, T *C2, T *CP, int tid, int nt) {
-    const int lane = tid & 63, nw = nt >> 6;
-    for (int c = tid >> 6; c < 52; c += nw) {
-        T a = 0, b = 0, p = 0;
-        if (lane < 52) {
-            a = M1[c * DN + lane];
-            if (M2) {
-                b = M2[c * DN + lane];
-                p = a * b;
-            }
-        }
-        a = wave_sum(a);
-        if (lane == 0) C1[c] = a;
+    (void)nt;
+    if (tid >= 52) return;
+    T a = 0, b = 0, p = 0;
+#pragma unroll 4
+    for (int d = 0; d < 52; ++d) {
+        const T m1 = M1[d * DN + tid];
+        a += m1;
         if (M2) {
-            b = wave_sum(b);
-            p = wave_sum(p);
-            if (lane == 0) { C2[c] = b; CP[c] = p; }
+            const T m2 = M2[d * DN + tid];
+            b += m2;
+            p += m1 * m2;
         }
+    }
+    C1[tid] = a;
+    if (M2) {
+        C2[tid] = b;
+        CP[tid] = p;
     }
 }
 
@@ -637,8 +638,8 @@ __device__ __forceinline__ void pair_products_mfma(const float *M1, const float 
 #pragma unroll
         for (int Q = 0; Q < 4; ++Q) acc[Q] = f32x4{0.f, 0.f, 0.f, 0.f};
         const int p = 16 * P + (l & 15);
-#pragma unroll 1
-        for (int k0 = 0; k0 < 52; k0 += 4) {
+#pragma unroll
+        for (int k0 = 0; k0 < 52; k0 += 4) {   // unrolled: the next steps' loads overlap these products
             const int c = k0 + (l >> 4);
             const float a = p < 52 ? M1[c * DN + p] : (p == 52 ? C1[c] : 0.f);
 #pragma unroll
