@@ -966,10 +966,10 @@ __device__ void terminal_node(const SlotDesc *__restrict__ s, int n, int trav, c
     __syncthreads();
 }
 
-// LDS of the FP32 terminal sweeps. Heavy (2-opponent nodes, ~66 KB, gfx950
+// LDS of the FP32 terminal sweeps. Heavy (2-opponent nodes, ~76 KB, gfx950
 // allows 160 KB per workgroup): both opponents' matrices, three prefix
-// tables, the subgame's per-card lists. Light (<= 1 opponent, ~30 KB): one
-// matrix, one prefix table; five workgroups fit a CU.
+// tables, the subgame's per-card lists. Light (<= 1 opponent, ~26 KB): one
+// matrix, one prefix table; six workgroups fit a CU.
 // Rows of 2752 = 43 x 64 floats: the same cell of M1 and M2 is one
 // ds_read2st64 away. PC2 sits one float past that spacing from PC1, so the
 // two per-card prefix reads of the showdown loop stay single reads sharing
@@ -1023,7 +1023,7 @@ template <typename T>
 struct TermLightT {
     T M1[DN2];
     T PC1[DN2];
-    T PG[1328];
+    T PG[1088];   // one channel: a river's at most 1081 live combos + the total (26.4 KB: six per CU)
     T C1[52];
     T red[64];
 };
