@@ -85,7 +85,8 @@ static int solve_plain(const HostProblem &hp, const SSSolveConfig &cfg, SSOutput
 }
 
 // Batched form: all subgames advance together, one persistent launch per
-// iteration; `chunk` launches are captured once in a hipGraph and replayed.
+// iteration; `gsteps` iterations are captured in a hipGraph and replayed
+// chunk / gsteps times between checks.
 struct BatchImpl {
     SSSolveConfig cfg{};
     int n = 0, chunk = 1;
@@ -104,6 +105,9 @@ struct BatchImpl {
     int full_tasks = 0;
     bool use_graph = true; // SS_NO_GRAPH=1 issues the same launches directly (profiling)
     int xblocks = 512;     // workgroups (and FP64 scratch slices) of the exploitability sweeps
+    int gsteps = 1;        // iterations per captured graph (a divisor of chunk, at most 10)
+    double *d_xout = nullptr;     // every subgame's exploitability terms, 32 doubles each
+    std::vector<double> h_xout;   // (one read back per check)
     // results of the last run
     std::vector<int> iters, conv, expl_at;
     std::vector<double> expl, gains;
@@ -117,6 +121,7 @@ struct BatchImpl {
         if (d_slots) (void)hipFree(d_slots);
         if (d_want) (void)hipFree(d_want);
         if (scratch) (void)hipFree(scratch);
+        if (d_xout) (void)hipFree(d_xout);
         if (st) (void)hipStreamDestroy(st);
     }
 
@@ -130,6 +135,10 @@ struct BatchImpl {
         target = cfg.target_exploitability > 0;
         want_final = !(cfg.flags & SS_FLAG_NO_EXPL);
         chunk = target ? std::max(1, cfg.check_every > 0 ? cfg.check_every : 50) : std::max(1, std::min(T, 50));
+        // a graph of a few iterations replayed chunk / gsteps times: the same
+        // launches, with a capture (redone at every task compaction) a
+        // fraction of the size
+        for (gsteps = std::min(chunk, 10); chunk % gsteps; --gsteps) {}
         SS_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         for (auto &e : ev) SS_HIP(hipEventCreate(&e));
         slots.resize(n);
@@ -142,6 +151,12 @@ struct BatchImpl {
                 rc = slots[i]->alloc_expl(st, 1);
                 if (rc) return rc;
             }
+        }
+        if (target || want_final) {
+            SS_HIP(hipMalloc(&d_xout, sizeof(double) * 32 * (size_t)n));
+            SS_HIP(hipMemsetAsync(d_xout, 0, sizeof(double) * 32 * (size_t)n, st));
+            for (int i = 0; i < n; ++i) slots[i]->desc.xout = d_xout + 32 * (size_t)i;
+            h_xout.assign(32 * (size_t)n, 0.0);
         }
         templ.resize(n);
         for (int i = 0; i < n; ++i) {
@@ -171,13 +186,13 @@ struct BatchImpl {
         return capture();
     }
 
-    // capture `chunk` iterations of the current task lists into the graph
+    // capture `gsteps` iterations of the current task lists into the graph
     int capture() {
         if (!use_graph) return SS_OK;
         if (exec) { (void)hipGraphExecDestroy(exec); exec = nullptr; }
         if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
         SS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        for (int k = 0; k < chunk; ++k) {
+        for (int k = 0; k < gsteps; ++k) {
             int rc = launch_iteration(mode);
             if (rc) {
                 (void)hipStreamEndCapture(st, &graph);
@@ -192,6 +207,7 @@ struct BatchImpl {
     // Phased launches are sized by their task lists: when retired subgames
     // hold a third or more of the tasks, rebuild the lists over the live ones
     // and recapture, so the tail of a batch does not dispatch empty workgroups.
+    double host_compact_ms = 0, host_capture_ms = 0, host_eval_ms = 0;
     int compact(const std::vector<int> &live) {
         if (mode != 1) return SS_OK;
         int64_t live_tasks = 0;
@@ -201,9 +217,14 @@ struct BatchImpl {
         if (live_tasks == 0 || live_tasks * 3 > (int64_t)cur * 2) return SS_OK;
         std::vector<Slot *> raw;
         for (auto &sl : slots) raw.push_back(sl.get());
+        const double t0 = now_ms();
         int rc = build_phased_tasks(raw, pt, st, &live);
         if (rc) return rc;
-        return capture();
+        const double t1 = now_ms();
+        rc = capture();
+        host_compact_ms += t1 - t0;
+        host_capture_ms += now_ms() - t1;
+        return rc;
     }
 
     int launch_iteration(int m) {
@@ -251,6 +272,7 @@ struct BatchImpl {
     }
 
     int evaluate(const std::vector<int> &sel, int at, double *check_ms) {
+        const double h0 = now_ms();
         SS_HIP(hipEventRecord(ev[2], st));
         SS_HIP(hipMemcpyAsync(d_want, sel.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
         int rc = launch_phased_expl(d_slots, pt, d_want, scratch, xblocks, st);
@@ -260,15 +282,14 @@ struct BatchImpl {
         float ms = 0;
         SS_HIP(hipEventElapsedTime(&ms, ev[2], ev[3]));
         *check_ms += ms;
-        std::vector<double> xo(2 * SS_MAX_PLAYERS + 1);
+        SS_HIP(hipMemcpy(h_xout.data(), d_xout, sizeof(double) * h_xout.size(), hipMemcpyDeviceToHost));
         for (int i = 0; i < n; ++i) {
             if (!sel[i]) continue;
-            SS_HIP(hipMemcpy(xo.data(), slots[i]->desc.xout, sizeof(double) * (2 * slots[i]->plan.NP + 1),
-                             hipMemcpyDeviceToHost));
-            expl[i] = expl_from_xout(xo.data(), slots[i]->plan.NP, slots[i]->plan.starting_pot,
+            expl[i] = expl_from_xout(h_xout.data() + 32 * (size_t)i, slots[i]->plan.NP, slots[i]->plan.starting_pot,
                                      &gains[(size_t)i * SS_MAX_PLAYERS]);
             expl_at[i] = at;
         }
+        host_eval_ms += now_ms() - h0 - ms;
         return SS_OK;
     }
 
@@ -299,7 +320,7 @@ struct BatchImpl {
         while (done < T) {
             SS_HIP(hipEventRecord(ev[0], st));
             if (use_graph) {
-                SS_HIP(hipGraphLaunch(exec, st));
+                for (int k = 0; k < chunk / gsteps; ++k) SS_HIP(hipGraphLaunch(exec, st));
             } else {
                 for (int k = 0; k < chunk; ++k) {
                     int rc = launch_iteration(mode);
@@ -346,6 +367,9 @@ struct BatchImpl {
             if (rc) return rc;
         }
         double wall = now_ms() - t0;
+        if (log_level() > 0)
+            fprintf(stderr, "[street_solve] batch run %.1f ms: cfr %.1f, checks %.1f, host: task rebuilds %.1f, graph captures %.1f, check readback %.1f\n",
+                    wall, cfr_ms, check_ms, host_compact_ms, host_capture_ms, host_eval_ms);
         int maxit = 0;
         for (int i = 0; i < n; ++i) maxit = std::max(maxit, iters[i]);
         if (reps)
