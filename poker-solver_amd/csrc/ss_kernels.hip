@@ -1177,17 +1177,19 @@ __device__ __forceinline__ void avg_items(const SlotDesc *__restrict__ s, int be
 }
 
 // reach of every player under the average strategy through node n, hand h
+// reach under the average strategy; as in the CFR forward sweep only the
+// actor's child rows are written (the others are read at the row of their
+// last action through rrow)
 __device__ __forceinline__ void xforward_one(const SlotDesc *__restrict__ s, int n, int h) {
     const int Mp = s->Mp, N = s->N, NP = s->NP;
     const DevNode &nd = s->nodes[n];
-    for (int q = 0; q < NP; ++q) {
-        if (h >= s->nh[q]) continue;
-        double *rr = s->xreach + (size_t)q * N * Mp;
-        double pr = rr[(size_t)n * Mp + h];
-        for (int a = 0; a < nd.nchild; ++a) {
-            int c = s->children[nd.first_child + a];
-            rr[(size_t)c * Mp + h] = q == nd.player ? pr * (double)s->avg[(size_t)(nd.act_base + a) * Mp + h] : pr;
-        }
+    const int q = nd.player;
+    if (q < 0 || q >= NP || h >= s->nh[q]) return;
+    double *rr = s->xreach + (size_t)q * N * Mp;
+    const double pr = rr[(size_t)s->rrow[(size_t)q * N + n] * Mp + h];
+    for (int a = 0; a < nd.nchild; ++a) {
+        int c = s->children[nd.first_child + a];
+        rr[(size_t)c * Mp + h] = pr * (double)s->avg[(size_t)(nd.act_base + a) * Mp + h];
     }
 }
 
@@ -1749,16 +1751,31 @@ __global__ void __launch_bounds__(SS_BLOCK) k_xavg_phased(SlotDesc *__restrict__
     }
 }
 
-__global__ void __launch_bounds__(SS_BLOCK) k_xfwd_phased(SlotDesc *__restrict__ slots, const uint32_t *tasks,
-                                                          int ntasks, const int *want) {
-    for (int j = blockIdx.x; j < ntasks; j += gridDim.x) {
-        uint32_t tk = tasks[j];
-        const SlotDesc *s = slots + TASK_SLOT(tk);
-        if (!slot_wanted(s, want, TASK_SLOT(tk))) continue;
-        for (int h = threadIdx.x; h < s->Mp; h += blockDim.x) xforward_one(s, TASK_NODE(tk), h);
+// FP64 sweeps of the exploitability check over the sweep records of every
+// decision task (the first pass's backward lists): the record read into
+// scalars in one batch, grid-stride over tasks; same arithmetic as
+// xforward_one / xbackward_one.
+__global__ void __launch_bounds__(SWEEP_BLOCK) k_xfwd_phased(SlotDesc *__restrict__ slots,
+                                                             const SweepRec *__restrict__ recs, int ntasks,
+                                                             const int *want) {
+    const int G = gridDim.x;
+    for (int j = blockIdx.x; j < ntasks; j += G) {
+        const SweepTask kr = sweep_task(recs + j, 0, true);
+        const SlotDesc *s = slots + kr.slot;
+        const int q = kr.actor;
+        if (!s->active || !want[kr.slot] || q < 0 || q >= kr.NP) continue;
+        const int Mp = kr.Mp, na = kr.na, ab = kr.ab;
+        double *rr = s->xreach + (size_t)q * kr.N * Mp;
+        const double *par = rr + (size_t)kr.arow * Mp;
+        const float *avg = s->avg;
+        for (int h = threadIdx.x; h < kr.nh; h += SWEEP_BLOCK) {
+            const double pr = par[h];
+#pragma unroll
+            for (int a = 0; a < NT_DEC; ++a)
+                if (a < na) rr[(size_t)kr.ch[a] * Mp + h] = pr * (double)avg[(size_t)(ab + a) * Mp + h];
+        }
     }
 }
-
 __global__ void __launch_bounds__(SS_BLOCK) k_xterm_phased(SlotDesc *__restrict__ slots, const uint32_t *tasks,
                                                            int ntasks, int p, const int *want, double *scratch) {
     for (int j = blockIdx.x; j < ntasks; j += gridDim.x) {
@@ -1809,16 +1826,38 @@ k_xterm_cls(SlotDesc *__restrict__ slots, const uint32_t *tasks, int ntasks, int
     }
 }
 
-__global__ void __launch_bounds__(SS_BLOCK) k_xbwd_phased(SlotDesc *__restrict__ slots, const uint32_t *tasks,
-                                                          int ntasks, int p, const int *want) {
-    for (int j = blockIdx.x; j < ntasks; j += gridDim.x) {
-        uint32_t tk = tasks[j];
-        const SlotDesc *s = slots + TASK_SLOT(tk);
-        if (!slot_wanted(s, want, TASK_SLOT(tk)) || p >= s->NP) continue;
-        for (int h = threadIdx.x; h < s->Mp; h += blockDim.x) xbackward_one(s, TASK_NODE(tk), h, p);
+__global__ void __launch_bounds__(SWEEP_BLOCK) k_xbwd_phased(SlotDesc *__restrict__ slots,
+                                                             const SweepRec *__restrict__ recs, int ntasks, int p,
+                                                             const int *want) {
+    const int G = gridDim.x;
+    for (int j = blockIdx.x; j < ntasks; j += G) {
+        const SweepTask kr = sweep_task(recs + j, p, false);
+        const SlotDesc *s = slots + kr.slot;
+        if (!s->active || !want[kr.slot] || p >= kr.NP) continue;
+        const int Mp = kr.Mp, na = kr.na, ab = kr.ab;
+        const bool own = kr.actor == p;
+        double *xev = s->xev, *xbr = s->xbr;
+        const float *avg = s->avg;
+        for (int h = threadIdx.x; h < kr.nh; h += SWEEP_BLOCK) {
+            double e = 0, b = 0;
+#pragma unroll
+            for (int a = 0; a < NT_DEC; ++a) {
+                if (a < na) {
+                    const double ce = xev[(size_t)kr.ch[a] * Mp + h], cb = xbr[(size_t)kr.ch[a] * Mp + h];
+                    if (own) {
+                        e += (double)avg[(size_t)(ab + a) * Mp + h] * ce;
+                        b = (a == 0 || cb > b) ? cb : b;
+                    } else {
+                        e += ce;
+                        b += cb;
+                    }
+                }
+            }
+            xev[(size_t)kr.node * Mp + h] = e;
+            xbr[(size_t)kr.node * Mp + h] = b;
+        }
     }
 }
-
 __global__ void __launch_bounds__(SS_BLOCK) k_xreduce_phased(SlotDesc *__restrict__ slots, int nslots, int p,
                                                              const int *want) {
     __shared__ double red[2 * SS_BLOCK];
@@ -2201,11 +2240,13 @@ int launch_phased_expl(SlotDesc *d_slots, const PhasedTasks &pt, const int *d_wa
                        int scratch_blocks, hipStream_t st) {
     const int ndec = pt.lvl_off[pt.L];
     if (ndec > 0) hipLaunchKernelGGL(k_xavg_phased, dim3(ndec), dim3(SS_BLOCK), 0, st, d_slots, pt.d_tasks, ndec, d_want);
+    // every decision task: the first pass's backward sweep records
+    const int *lv = pt.NPmax > 0 ? &pt.sw_off[(size_t)(pt.L + 1)] : nullptr;
     for (int l = 0; l < pt.L; ++l) {
-        int cnt = pt.lvl_off[l + 1] - pt.lvl_off[l];
+        int cnt = lv[l + 1] - lv[l];
         if (cnt > 0)
-            hipLaunchKernelGGL(k_xfwd_phased, dim3(cnt), dim3(SS_BLOCK), 0, st, d_slots, pt.d_tasks + pt.lvl_off[l], cnt,
-                               d_want);
+            hipLaunchKernelGGL(k_xfwd_phased, dim3(std::min(cnt, SWEEP_GRID)), dim3(SWEEP_BLOCK), 0, st, d_slots,
+                               pt.d_recs + lv[l], cnt, d_want);
     }
     for (int p = 0; p < pt.NPmax; ++p) {
         const uint32_t *tt = pt.d_tasks + pt.term_off;
@@ -2222,10 +2263,10 @@ int launch_phased_expl(SlotDesc *d_slots, const PhasedTasks &pt, const int *d_wa
             hipLaunchKernelGGL(k_xterm_cls<0>, dim3(nlight), dim3(SS_BLOCK), 0, st, d_slots, tt + pt.nodo + pt.npair,
                                nlight, p, d_want);
         for (int l = pt.L - 1; l >= 0; --l) {
-            int cnt = pt.lvl_off[l + 1] - pt.lvl_off[l];
+            int cnt = lv[l + 1] - lv[l];
             if (cnt > 0)
-                hipLaunchKernelGGL(k_xbwd_phased, dim3(cnt), dim3(SS_BLOCK), 0, st, d_slots, pt.d_tasks + pt.lvl_off[l],
-                                   cnt, p, d_want);
+                hipLaunchKernelGGL(k_xbwd_phased, dim3(std::min(cnt, SWEEP_GRID)), dim3(SWEEP_BLOCK), 0, st, d_slots,
+                                   pt.d_recs + lv[l], cnt, p, d_want);
         }
         hipLaunchKernelGGL(k_xreduce_phased, dim3(pt.nslots), dim3(SS_BLOCK), 0, st, d_slots, pt.nslots, p, d_want);
     }
