@@ -108,6 +108,7 @@ struct BatchImpl {
     int gsteps = 1;        // iterations per captured graph (a divisor of chunk, at most 10)
     double *d_xout = nullptr;     // every subgame's exploitability terms, 32 doubles each
     std::vector<double> h_xout;   // (one read back per check)
+    std::vector<char> joint_done; // the joint deal mass (weights only) is in d_xout
     // results of the last run
     std::vector<int> iters, conv, expl_at;
     std::vector<double> expl, gains;
@@ -157,6 +158,7 @@ struct BatchImpl {
             SS_HIP(hipMemsetAsync(d_xout, 0, sizeof(double) * 32 * (size_t)n, st));
             for (int i = 0; i < n; ++i) slots[i]->desc.xout = d_xout + 32 * (size_t)i;
             h_xout.assign(32 * (size_t)n, 0.0);
+            joint_done.assign(n, 0);
         }
         templ.resize(n);
         for (int i = 0; i < n; ++i) {
@@ -275,7 +277,11 @@ struct BatchImpl {
         const double h0 = now_ms();
         SS_HIP(hipEventRecord(ev[2], st));
         SS_HIP(hipMemcpyAsync(d_want, sel.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
-        int rc = launch_phased_expl(d_slots, pt, d_want, scratch, xblocks, st);
+        bool joint = false;
+        for (int i = 0; i < n; ++i) joint |= sel[i] && !joint_done[i];
+        int rc = launch_phased_expl(d_slots, pt, d_want, scratch, xblocks, st, joint);
+        if (joint)
+            for (int i = 0; i < n; ++i) joint_done[i] |= (char)(sel[i] != 0);
         if (rc) return rc;
         SS_HIP(hipEventRecord(ev[3], st));
         SS_HIP(hipEventSynchronize(ev[3]));

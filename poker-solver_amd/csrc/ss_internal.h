@@ -145,6 +145,7 @@ struct PhasedTasks {
     std::vector<int> sw_off;
     struct PassTerm {
         int off = 0, nodo = 0, nsd = 0, nfold = 0, nlight = 0;
+        int sd_parts = 1;   // workgroups per two-opponent showdown task
     };
     std::vector<PassTerm> pterm;
     uint32_t *d_tasks = nullptr;
@@ -163,8 +164,10 @@ int prec_rows_per_task(int nt);   // rows of one precision GEMM task for nt colu
 int build_phased_tasks(const std::vector<Slot *> &slots, PhasedTasks &pt, hipStream_t st,
                        const std::vector<int> *live = nullptr);
 int launch_phased_iteration(SlotDesc *d_slots, const PhasedTasks &pt, hipStream_t st);
+// joint = false skips the joint deal mass (xout[2 NP]), which depends only on
+// the raw weights: a caller whose xout persists computes it once
 int launch_phased_expl(SlotDesc *d_slots, const PhasedTasks &pt, const int *d_want, double *scratch,
-                       int scratch_blocks, hipStream_t st);
+                       int scratch_blocks, hipStream_t st, bool joint = true);
 #define SS_XSCRATCH_DOUBLES (5 * 2704 + 3 * 1328 + 6 * 52 + 64)
 
 // debug / test entry points implemented in the kernels unit

@@ -703,7 +703,8 @@ __device__ __forceinline__ void showdown_pair_vectors(const T *M1, const T *M2, 
 // type), -1 both.
 template <typename T, int MODE, int SDK = -1>
 __device__ void terminal_node(const SlotDesc *__restrict__ s, int n, int trav, const T *reach, T *out,
-                              TermBuf<T> B, int kind, const float *scale, int tid, int nt) {
+                              TermBuf<T> B, int kind, const float *scale, int tid, int nt, int part = 0,
+                              int parts = 1) {
     // the node's fields and the slot's shape, read together up front (not
     // one dependent round trip per field where each is first used)
     const DevNode *ndp = s->nodes + n;
@@ -832,12 +833,16 @@ __device__ void terminal_node(const SlotDesc *__restrict__ s, int n, int trav, c
     // Showdowns walk the traverser's hands by strength (shared class
     // boundaries within a wavefront turn prefix reads into broadcasts), one
     // 16-byte record per hand, fetched one hand ahead.
+    // A showdown may be split over `parts` workgroups, each taking a range of
+    // the strength order (each repeats the setup above in its own LDS).
+    const int hlo = sd ? (int)((int64_t)nh * part / parts) : 0;
+    const int hhi = sd ? (int)((int64_t)nh * (part + 1) / parts) : nh;
     const uint4 *meta = sd ? s->hmeta + (size_t)trav * Mp : nullptr;
     uint4 mnext = make_uint4(0u, 0u, 0u, 0u);
-    if (sd && tid < nh) mnext = gld(meta + tid);
-    for (int i = tid, u = 0; i < nh; i += nt, ++u) {
+    if (sd && hlo + tid < hhi) mnext = gld(meta + hlo + tid);
+    for (int i = hlo + tid, u = 0; i < hhi; i += nt, ++u) {
         const uint4 mh = mnext;
-        if (sd && i + nt < nh) mnext = gld(meta + i + nt);
+        if (sd && i + nt < hhi) mnext = gld(meta + i + nt);
         int h, x, y;
         if (sd) {
             h = mh.x & 2047;
@@ -1538,13 +1543,16 @@ __global__ void __launch_bounds__(SS_BLOCK) k_term_light(SlotDesc *__restrict__ 
 #define SS_PAIR_BLOCK 512
 template <int SDK>
 __global__ void __launch_bounds__(SDK ? SS_PAIR_BLOCK : SS_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
-k_term_pair(SlotDesc *__restrict__ slots, const uint32_t *tasks, int ntasks, int trav) {
+k_term_pair(SlotDesc *__restrict__ slots, const uint32_t *tasks, int ntasks, int trav, int parts) {
     using Buf = typename std::conditional<SDK == 1, TermHeavy, TermFold>::type;
     __shared__ Buf sh;
     TermBuf<float> b;
     if constexpr (SDK == 1) b = heavy_buf(sh);
     else b = fold_buf(sh);
-    for (int j = blockIdx.x; j < ntasks; j += gridDim.x) {
+    // parts > 1 (showdowns, when a pass has fewer tasks than the GPU has
+    // workgroup slots): each task's hands split over `parts` workgroups
+    for (int jj = blockIdx.x; jj < ntasks * parts; jj += gridDim.x) {
+        const int j = jj / parts, part = jj - j * parts;
         uint32_t tk = tasks[j];
         const SlotDesc *s = slots + TASK_SLOT(tk);
         if (!slot_runs(s, trav)) continue;
@@ -1552,7 +1560,7 @@ k_term_pair(SlotDesc *__restrict__ slots, const uint32_t *tasks, int ntasks, int
         __syncthreads();
         int n = TASK_NODE(tk);
         terminal_node<float, TM_PAIR, SDK>(s, n, trav, s->reach, s->cfv + (size_t)n * s->Mp, b, TK_NODE, nullptr,
-                                           threadIdx.x, blockDim.x);
+                                           threadIdx.x, blockDim.x, part, parts);
         __syncthreads();
     }
 }
@@ -2136,6 +2144,17 @@ int build_phased_tasks(const std::vector<Slot *> &slots, PhasedTasks &pt, hipStr
         q.nsd = seg(100, 100);
         q.nfold = seg(20, 20);
         q.nlight = seg(5, 5);
+        // fewer two-opponent showdowns than resident workgroup slots (two per
+        // CU): split each one's hands, up to four ways, so the pass's latency
+        // is not one whole node's
+        static const int sd_slots = [] {
+            int cus = 0, dev = 0;
+            if (hipGetDevice(&dev) != hipSuccess ||
+                hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+                cus = 256;
+            return 2 * cus;
+        }();
+        q.sd_parts = q.nsd > 0 ? std::max(1, std::min(4, sd_slots / q.nsd)) : 1;
     }
     // per traverser pass and level: forward tasks (actor not the traverser),
     // backward tasks, of the slots with more than t players
@@ -2217,10 +2236,11 @@ int launch_phased_iteration(SlotDesc *d_slots, const PhasedTasks &pt, hipStream_
         if (q.nodo > 0)
             hipLaunchKernelGGL(k_term_odo, dim3(q.nodo), dim3(SS_BLOCK), 0, st, d_slots, tt, q.nodo, t);
         if (q.nsd > 0)
-            hipLaunchKernelGGL(k_term_pair<1>, dim3(q.nsd), dim3(SS_PAIR_BLOCK), 0, st, d_slots, tt + q.nodo, q.nsd, t);
+            hipLaunchKernelGGL(k_term_pair<1>, dim3(q.nsd * q.sd_parts), dim3(SS_PAIR_BLOCK), 0, st, d_slots,
+                               tt + q.nodo, q.nsd, t, q.sd_parts);
         if (q.nfold > 0)
             hipLaunchKernelGGL(k_term_pair<0>, dim3(q.nfold), dim3(SS_BLOCK), 0, st, d_slots, tt + q.nodo + q.nsd,
-                               q.nfold, t);
+                               q.nfold, t, 1);
         if (q.nlight > 0)
             hipLaunchKernelGGL(k_term_light, dim3(q.nlight), dim3(SS_BLOCK), 0, st, d_slots,
                                tt + q.nodo + q.nsd + q.nfold, q.nlight, t);
@@ -2237,7 +2257,7 @@ int launch_phased_iteration(SlotDesc *d_slots, const PhasedTasks &pt, hipStream_
 }
 
 int launch_phased_expl(SlotDesc *d_slots, const PhasedTasks &pt, const int *d_want, double *scratch,
-                       int scratch_blocks, hipStream_t st) {
+                       int scratch_blocks, hipStream_t st, bool joint) {
     const int ndec = pt.lvl_off[pt.L];
     if (ndec > 0) hipLaunchKernelGGL(k_xavg_phased, dim3(ndec), dim3(SS_BLOCK), 0, st, d_slots, pt.d_tasks, ndec, d_want);
     // every decision task: the first pass's backward sweep records
@@ -2270,8 +2290,9 @@ int launch_phased_expl(SlotDesc *d_slots, const PhasedTasks &pt, const int *d_wa
         }
         hipLaunchKernelGGL(k_xreduce_phased, dim3(pt.nslots), dim3(SS_BLOCK), 0, st, d_slots, pt.nslots, p, d_want);
     }
-    hipLaunchKernelGGL(k_xjoint_phased, dim3(std::min(pt.nslots, scratch_blocks)), dim3(SS_BLOCK), 0, st, d_slots,
-                       pt.nslots, d_want, scratch);
+    if (joint)
+        hipLaunchKernelGGL(k_xjoint_phased, dim3(std::min(pt.nslots, scratch_blocks)), dim3(SS_BLOCK), 0, st, d_slots,
+                           pt.nslots, d_want, scratch);
     SS_HIP(hipGetLastError());
     return SS_OK;
 }
