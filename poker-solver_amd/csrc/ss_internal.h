@@ -150,6 +150,7 @@ struct PhasedTasks {
     std::vector<PassTerm> pterm;
     uint32_t *d_tasks = nullptr;
     SweepRec *d_recs = nullptr;    // the per-pass sweep records
+    size_t cap_tasks = 0, cap_recs = 0;   // allocated lengths of d_tasks, d_recs
     ~PhasedTasks();
 };
 // tasks of the non-null slots with live[slot] != 0 (all of them when live is null)

@@ -2200,10 +2200,19 @@ int build_phased_tasks(const std::vector<Slot *> &slots, PhasedTasks &pt, hipStr
         r.cfv = S.desc.cfv;
         r.reach = S.desc.reach;
     }
-    if (pt.d_tasks) { SS_HIP(hipFree(pt.d_tasks)); pt.d_tasks = nullptr; }
-    if (pt.d_recs) { SS_HIP(hipFree(pt.d_recs)); pt.d_recs = nullptr; }
-    SS_HIP(hipMalloc(&pt.d_tasks, sizeof(uint32_t) * std::max<size_t>(1, tasks.size())));
-    SS_HIP(hipMalloc(&pt.d_recs, sizeof(SweepRec) * std::max<size_t>(1, ndec)));
+    // device lists: reused while large enough (a compaction only shrinks them)
+    if (!pt.d_tasks || pt.cap_tasks < tasks.size()) {
+        if (pt.d_tasks) SS_HIP(hipFree(pt.d_tasks));
+        pt.d_tasks = nullptr;
+        pt.cap_tasks = std::max<size_t>(1, tasks.size());
+        SS_HIP(hipMalloc(&pt.d_tasks, sizeof(uint32_t) * pt.cap_tasks));
+    }
+    if (!pt.d_recs || pt.cap_recs < ndec) {
+        if (pt.d_recs) SS_HIP(hipFree(pt.d_recs));
+        pt.d_recs = nullptr;
+        pt.cap_recs = std::max<size_t>(1, ndec);
+        SS_HIP(hipMalloc(&pt.d_recs, sizeof(SweepRec) * pt.cap_recs));
+    }
     SS_HIP(hipMemcpyAsync(pt.d_tasks, tasks.data(), sizeof(uint32_t) * tasks.size(), hipMemcpyHostToDevice, st));
     if (ndec) SS_HIP(hipMemcpyAsync(pt.d_recs, recs.data(), sizeof(SweepRec) * ndec, hipMemcpyHostToDevice, st));
     SS_HIP(hipStreamSynchronize(st));
