@@ -1461,19 +1461,19 @@ __device__ __forceinline__ int2 sweep_warm(const SweepRec *__restrict__ recs, in
 // issued together with the read of the slot's progress; A bounds the action
 // count, so a node with few actions issues no repeated loads. Only the actor's
 // child rows are written (reach-row table).
-template <int A, int H>
+template <int A, int H, int NT>
 __device__ __forceinline__ void fwd_task(const SweepTask &kr, int actor, int trav, int s_act, int s_it, int s_end,
-                                         int s_np) {
+                                         int s_np, int tid) {
     const int Mp = kr.Mp, N = kr.N, na = kr.na, ab = kr.ab, fa = kr.fa;
     int ch[A];
 #pragma unroll
     for (int a = 0; a < A; ++a) ch[a] = kr.ch[a];
-    const int nh = kr.nh, nt = SWEEP_BLOCK;
+    const int nh = kr.nh, nt = NT;
     const float *R = kr.R;
     float *ra = kr.reach + (size_t)actor * N * Mp;
     const float *par = ra + (size_t)kr.arow * Mp;
     bool checked = false;
-    for (int h0 = threadIdx.x; h0 < nh; h0 += H * nt) {
+    for (int h0 = tid; h0 < nh; h0 += H * nt) {
         float r[H][A], pr[H];
 #pragma unroll
         for (int u = 0; u < H; ++u) {
@@ -1502,20 +1502,49 @@ __device__ __forceinline__ void fwd_task(const SweepTask &kr, int actor, int tra
     }
 }
 
-__global__ void __launch_bounds__(SWEEP_BLOCK) k_fwd_phased(SlotDesc *__restrict__ slots,
-                                                            const SweepRec *__restrict__ recs, int ntasks,
-                                                            int trav) {
+// per = 3 (levels with enough tasks to fill the GPU three to a workgroup):
+// three tasks per workgroup, one per group of three wavefronts (each
+// wavefront reads its own task's record into scalars; nothing is shared, so
+// the groups run independently), six hands per thread, 192 x 6 = 1152 >=
+// 1128, all loads of a task in one round (nodes with more than three actions:
+// two rounds of three hands). per = 1: one task per workgroup, two hands per
+// thread (more workgroups when a level is small).
+#define SWEEP_GROUP 192
+#define SWEEP_GH 6
+__global__ void __launch_bounds__(SWEEP_BLOCK) __attribute__((amdgpu_waves_per_eu(7))) k_fwd_phased(SlotDesc *__restrict__ slots,
+                                                            const SweepRec *__restrict__ recs, int ntasks, int trav,
+                                                            int per) {
     const int G = gridDim.x;
-    for (int j = blockIdx.x; j < ntasks; j += G) {
+    if (per == 1) {
+        for (int j = blockIdx.x; j < ntasks; j += G) {
+            const SweepTask kr = sweep_task(recs + j, trav, true);
+            const int2 warm = sweep_warm(recs, j + G, ntasks);
+            const int actor = kr.actor, NP = kr.NP;
+            if (actor == trav || actor < 0 || actor >= NP) continue;
+            const SlotDesc *s = slots + kr.slot;
+            const int s_act = s->active, s_it = s->iter, s_end = s->iter_end, s_np = s->NP;
+            if (kr.na <= 3) fwd_task<3, 2, SWEEP_BLOCK>(kr, actor, trav, s_act, s_it, s_end, s_np, threadIdx.x);
+            else fwd_task<NT_DEC, 2, SWEEP_BLOCK>(kr, actor, trav, s_act, s_it, s_end, s_np, threadIdx.x);
+            SWEEP_USE(warm);
+        }
+        return;
+    }
+    const int nunits = (ntasks + 2) / 3;
+    // the group (0..2) is the same in every lane of a wavefront; readfirstlane
+    // tells the compiler, so a group's record index and fields stay scalar
+    const int sub = __builtin_amdgcn_readfirstlane((int)threadIdx.x / SWEEP_GROUP);
+    const int tg = threadIdx.x - sub * SWEEP_GROUP;
+    for (int u = blockIdx.x; u < nunits; u += G) {
+        const int j = 3 * u + sub;
+        if (j >= ntasks) continue;
         const SweepTask kr = sweep_task(recs + j, trav, true);
-        const int2 warm = sweep_warm(recs, j + G, ntasks);
+        const int2 warm = sweep_warm(recs, j + 3 * G, ntasks);
         const int actor = kr.actor, NP = kr.NP;
         if (actor == trav || actor < 0 || actor >= NP) continue;
         const SlotDesc *s = slots + kr.slot;
         const int s_act = s->active, s_it = s->iter, s_end = s->iter_end, s_np = s->NP;
-        // two hands per thread: a node has at most 1128 hands
-        if (kr.na <= 3) fwd_task<3, 2>(kr, actor, trav, s_act, s_it, s_end, s_np);
-        else fwd_task<NT_DEC, 2>(kr, actor, trav, s_act, s_it, s_end, s_np);
+        if (kr.na <= 3) fwd_task<3, SWEEP_GH, SWEEP_GROUP>(kr, actor, trav, s_act, s_it, s_end, s_np, tg);
+        else fwd_task<NT_DEC, 3, SWEEP_GROUP>(kr, actor, trav, s_act, s_it, s_end, s_np, tg);
         SWEEP_USE(warm);
     }
 }
@@ -2158,6 +2187,7 @@ int build_phased_tasks(const std::vector<Slot *> &slots, PhasedTasks &pt, hipStr
     }
     // per traverser pass and level: forward tasks (actor not the traverser),
     // backward tasks, of the slots with more than t players
+    // (backward levels: the traverser's own nodes, the heavier tasks, first)
     std::vector<uint32_t> sw;
     pt.sw_off.assign((size_t)std::max(1, pt.NPmax) * 2 * (pt.L + 1), 0);
     for (int t = 0; t < pt.NPmax; ++t)
@@ -2165,13 +2195,17 @@ int build_phased_tasks(const std::vector<Slot *> &slots, PhasedTasks &pt, hipStr
             int *off = &pt.sw_off[((size_t)t * 2 + dir) * (pt.L + 1)];
             for (int l = 0; l < pt.L; ++l) {
                 off[l] = (int)sw.size();
-                for (int j = pt.lvl_off[l]; j < pt.lvl_off[l + 1]; ++j) {
-                    const Plan &pl = slots[tasks[j] >> 20]->plan;
-                    const int a = pl.nodes[tasks[j] & 0xFFFFF].player;
-                    if (t >= pl.NP || (dir == 0 && (a == t || a < 0 || a >= pl.NP))) continue;
-                    sw.push_back(tasks[j]);
+                for (int pass = dir == 1 ? 0 : 1; pass < 2; ++pass) {
+                    const size_t b = sw.size();
+                    for (int j = pt.lvl_off[l]; j < pt.lvl_off[l + 1]; ++j) {
+                        const Plan &pl = slots[tasks[j] >> 20]->plan;
+                        const int a = pl.nodes[tasks[j] & 0xFFFFF].player;
+                        if (t >= pl.NP || (dir == 0 && (a == t || a < 0 || a >= pl.NP))) continue;
+                        if (dir == 1 && (pass == 0) != (a == t)) continue;
+                        sw.push_back(tasks[j]);
+                    }
+                    xcd_interleave(sw, b, sw.size());
                 }
-                xcd_interleave(sw, off[l], sw.size());
             }
             off[pt.L] = (int)sw.size();
         }
@@ -2226,6 +2260,19 @@ static const int SWEEP_GRID = [] {
     return e && atoi(e) > 0 ? atoi(e) : 1536;
 }();
 
+// sweep tasks per workgroup: three when they still fill three resident
+// workgroups per CU that way, else one
+static int per_of(int cnt) {
+    static const int full = [] {
+        int cus = 0, dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+        return 3 * 3 * cus;
+    }();
+    return cnt >= full ? 3 : 1;
+}
+
 int launch_phased_iteration(SlotDesc *d_slots, const PhasedTasks &pt, hipStream_t st) {
     for (int t = 0; t < pt.NPmax; ++t) {
         const int *fo = &pt.sw_off[(size_t)t * 2 * (pt.L + 1)], *bo = fo + pt.L + 1;
@@ -2233,8 +2280,8 @@ int launch_phased_iteration(SlotDesc *d_slots, const PhasedTasks &pt, hipStream_
         for (int l = 0; l < pt.L; ++l) {
             int cnt = fo[l + 1] - fo[l];
             if (cnt > 0)
-                hipLaunchKernelGGL(k_fwd_phased, dim3(std::min(cnt, SWEEP_GRID)), dim3(SWEEP_BLOCK), 0, st, d_slots,
-                                   pt.d_recs + fo[l], cnt, t);
+                hipLaunchKernelGGL(k_fwd_phased, dim3(std::min((cnt + per_of(cnt) - 1) / per_of(cnt), SWEEP_GRID)),
+                                   dim3(SWEEP_BLOCK), 0, st, d_slots, pt.d_recs + fo[l], cnt, t, per_of(cnt));
         }
         if (pt.npsplit > 0) {
             int rc = launch_prec_phased(d_slots, pt, t, st);
