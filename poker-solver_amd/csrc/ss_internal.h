@@ -146,6 +146,7 @@ struct PhasedTasks {
     struct PassTerm {
         int off = 0, nodo = 0, nsd = 0, nfold = 0, nlight = 0;
         int sd_parts = 1;   // workgroups per two-opponent showdown task
+        int small_parts = 0;   // > 0: the pass's closed-form terminals as one launch, showdowns split this many ways
     };
     std::vector<PassTerm> pterm;
     uint32_t *d_tasks = nullptr;

@@ -1594,6 +1594,52 @@ k_term_pair(SlotDesc *__restrict__ slots, const uint32_t *tasks, int ntasks, int
     }
 }
 
+// A pass whose closed-form terminal tasks fit in two rounds of resident
+// workgroups: the three classes in one launch (the workgroup index picks the
+// class; the pass's segment lists showdowns, folds, light tasks in that
+// order, so the slowest start first), so their latencies overlap instead of
+// adding up. Every class runs
+// with 512 threads in the largest LDS layout (occupancy does not matter at
+// this size); results equal the separate kernels' (no terminal reduction
+// depends on the block size).
+union TermAny {
+    TermHeavy heavy;
+    TermFold fold;
+    TermLight light;
+};
+__global__ void __launch_bounds__(SS_PAIR_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
+k_term_small(SlotDesc *__restrict__ slots, const uint32_t *tasks, int nsd, int parts, int nfold, int nlight,
+             int trav) {
+    __shared__ TermAny sh;
+    const int b = blockIdx.x, nsdw = nsd * parts;
+    int j, part = 0;
+    if (b < nsdw) {
+        j = b / parts;
+        part = b - j * parts;
+    } else if (b < nsdw + nfold + nlight) {
+        j = nsd + (b - nsdw);
+    } else {
+        return;
+    }
+    const uint32_t tk = tasks[j];
+    const SlotDesc *s = slots + TASK_SLOT(tk);
+    if (!slot_runs(s, trav)) return;   // workgroup-uniform
+    const int n = TASK_NODE(tk);
+    float *out = s->cfv + (size_t)n * s->Mp;
+    if (b < nsdw) {
+        stage_tables(s, sh.heavy, threadIdx.x, blockDim.x);
+        __syncthreads();
+        terminal_node<float, TM_PAIR, 1>(s, n, trav, s->reach, out, heavy_buf(sh.heavy), TK_NODE, nullptr,
+                                         threadIdx.x, blockDim.x, part, parts);
+    } else if (b < nsdw + nfold) {
+        terminal_node<float, TM_PAIR, 0>(s, n, trav, s->reach, out, fold_buf(sh.fold), TK_NODE, nullptr,
+                                         threadIdx.x, blockDim.x);
+    } else {
+        terminal_node<float, TM_LIGHT>(s, n, trav, s->reach, out, light_buf(sh.light, s), TK_NODE, nullptr,
+                                       threadIdx.x, blockDim.x);
+    }
+}
+
 // terminal tasks with three or more opponents for some traverser
 __global__ void __launch_bounds__(SS_BLOCK) k_term_odo(SlotDesc *__restrict__ slots, const uint32_t *tasks,
                                                        int ntasks, int trav) {
@@ -2063,6 +2109,19 @@ static int term_cost(const Plan &pl, int n, bool prec) {
 // ordering a segment to put subgame s at positions j = s (mod 8) keeps each
 // subgame's tables and state in one XCD's L2. Queues that run dry are
 // backfilled from the longest remaining one, so the order stays a permutation.
+// terminal workgroups of one pass that run as one launch (k_term_small): one
+// round of two 76 KB workgroups per CU
+static int small_term_limit() {
+    static const int lim = [] {
+        int cus = 0, dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+        return 2 * cus;
+    }();
+    return lim;
+}
+
 #define SS_NUM_XCD 8
 static void xcd_interleave(std::vector<uint32_t> &v, size_t b, size_t e) {
     std::vector<std::vector<uint32_t>> q(SS_NUM_XCD);
@@ -2184,6 +2243,14 @@ int build_phased_tasks(const std::vector<Slot *> &slots, PhasedTasks &pt, hipStr
             return 2 * cus;
         }();
         q.sd_parts = q.nsd > 0 ? std::max(1, std::min(4, sd_slots / q.nsd)) : 1;
+        // all three closed-form classes within two rounds of resident
+        // workgroups: one launch (k_term_small), the showdowns first so the
+        // lighter classes cycle through the other slots beside them; the
+        // showdowns split to fill what the others leave of one round
+        const int lim = small_term_limit(), nall = q.nsd + q.nfold + q.nlight;
+        q.small_parts = 0;
+        if (q.nodo == 0 && nall > 0 && nall <= 2 * lim)
+            q.small_parts = q.nsd > 0 ? std::max(1, std::min(4, (lim - q.nfold - q.nlight) / q.nsd)) : 1;
     }
     // per traverser pass and level: forward tasks (actor not the traverser),
     // backward tasks, of the slots with more than t players
@@ -2289,6 +2356,10 @@ int launch_phased_iteration(SlotDesc *d_slots, const PhasedTasks &pt, hipStream_
         }
         const PhasedTasks::PassTerm &q = pt.pterm[t];
         const uint32_t *tt = pt.d_tasks + q.off;
+        if (q.small_parts > 0) {
+            hipLaunchKernelGGL(k_term_small, dim3(q.nsd * q.small_parts + q.nfold + q.nlight), dim3(SS_PAIR_BLOCK), 0,
+                               st, d_slots, tt, q.nsd, q.small_parts, q.nfold, q.nlight, t);
+        } else {
         if (q.nodo > 0)
             hipLaunchKernelGGL(k_term_odo, dim3(q.nodo), dim3(SS_BLOCK), 0, st, d_slots, tt, q.nodo, t);
         if (q.nsd > 0)
@@ -2300,6 +2371,7 @@ int launch_phased_iteration(SlotDesc *d_slots, const PhasedTasks &pt, hipStream_
         if (q.nlight > 0)
             hipLaunchKernelGGL(k_term_light, dim3(q.nlight), dim3(SS_BLOCK), 0, st, d_slots,
                                tt + q.nodo + q.nsd + q.nfold, q.nlight, t);
+        }
         for (int l = pt.L - 1; l >= 0; --l) {
             int cnt = bo[l + 1] - bo[l];
             if (cnt > 0)
