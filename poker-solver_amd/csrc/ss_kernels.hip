@@ -1502,34 +1502,36 @@ __device__ __forceinline__ void fwd_task(const SweepTask &kr, int actor, int tra
     }
 }
 
-// per = 3 (levels with enough tasks to fill the GPU three to a workgroup):
-// three tasks per workgroup, one per group of three wavefronts (each
+// One task per workgroup, two hands per thread (k_fwd_phased), or, on
+// levels with enough tasks to fill the GPU that way, three tasks per
+// workgroup (k_fwd_grouped): one per group of three wavefronts (each
 // wavefront reads its own task's record into scalars; nothing is shared, so
 // the groups run independently), six hands per thread, 192 x 6 = 1152 >=
 // 1128, all loads of a task in one round (nodes with more than three actions:
-// two rounds of three hands). per = 1: one task per workgroup, two hands per
-// thread (more workgroups when a level is small).
+// two rounds of three hands).
 #define SWEEP_GROUP 192
 #define SWEEP_GH 6
-__global__ void __launch_bounds__(SWEEP_BLOCK) __attribute__((amdgpu_waves_per_eu(7))) k_fwd_phased(SlotDesc *__restrict__ slots,
-                                                            const SweepRec *__restrict__ recs, int ntasks, int trav,
-                                                            int per) {
+__global__ void __launch_bounds__(SWEEP_BLOCK) k_fwd_phased(SlotDesc *__restrict__ slots,
+                                                            const SweepRec *__restrict__ recs, int ntasks,
+                                                            int trav) {
     const int G = gridDim.x;
-    if (per == 1) {
-        for (int j = blockIdx.x; j < ntasks; j += G) {
-            const SweepTask kr = sweep_task(recs + j, trav, true);
-            const int2 warm = sweep_warm(recs, j + G, ntasks);
-            const int actor = kr.actor, NP = kr.NP;
-            if (actor == trav || actor < 0 || actor >= NP) continue;
-            const SlotDesc *s = slots + kr.slot;
-            const int s_act = s->active, s_it = s->iter, s_end = s->iter_end, s_np = s->NP;
-            if (kr.na <= 3) fwd_task<3, 2, SWEEP_BLOCK>(kr, actor, trav, s_act, s_it, s_end, s_np, threadIdx.x);
-            else fwd_task<NT_DEC, 2, SWEEP_BLOCK>(kr, actor, trav, s_act, s_it, s_end, s_np, threadIdx.x);
-            SWEEP_USE(warm);
-        }
-        return;
+    for (int j = blockIdx.x; j < ntasks; j += G) {
+        const SweepTask kr = sweep_task(recs + j, trav, true);
+        const int2 warm = sweep_warm(recs, j + G, ntasks);
+        const int actor = kr.actor, NP = kr.NP;
+        if (actor == trav || actor < 0 || actor >= NP) continue;
+        const SlotDesc *s = slots + kr.slot;
+        const int s_act = s->active, s_it = s->iter, s_end = s->iter_end, s_np = s->NP;
+        if (kr.na <= 3) fwd_task<3, 2, SWEEP_BLOCK>(kr, actor, trav, s_act, s_it, s_end, s_np, threadIdx.x);
+        else fwd_task<NT_DEC, 2, SWEEP_BLOCK>(kr, actor, trav, s_act, s_it, s_end, s_np, threadIdx.x);
+        SWEEP_USE(warm);
     }
-    const int nunits = (ntasks + 2) / 3;
+}
+// the same, three tasks per workgroup (a kernel of its own: its registers do
+// not weigh on the one-task form)
+__global__ void __launch_bounds__(SWEEP_BLOCK) __attribute__((amdgpu_waves_per_eu(7)))
+k_fwd_grouped(SlotDesc *__restrict__ slots, const SweepRec *__restrict__ recs, int ntasks, int trav) {
+    const int G = gridDim.x, nunits = (ntasks + 2) / 3;
     // the group (0..2) is the same in every lane of a wavefront; readfirstlane
     // tells the compiler, so a group's record index and fields stay scalar
     const int sub = __builtin_amdgcn_readfirstlane((int)threadIdx.x / SWEEP_GROUP);
@@ -2243,14 +2245,16 @@ int build_phased_tasks(const std::vector<Slot *> &slots, PhasedTasks &pt, hipStr
             return 2 * cus;
         }();
         q.sd_parts = q.nsd > 0 ? std::max(1, std::min(4, sd_slots / q.nsd)) : 1;
-        // all three closed-form classes within two rounds of resident
-        // workgroups: one launch (k_term_small), the showdowns first so the
-        // lighter classes cycle through the other slots beside them; the
-        // showdowns split to fill what the others leave of one round
+        // two-opponent showdowns and the other closed-form classes within two
+        // rounds of resident workgroups: one launch (k_term_small), the
+        // showdowns first so the lighter classes cycle through the other
+        // slots beside them; the showdowns split to fill what the others
+        // leave of one round (without showdowns there is nothing to overlap:
+        // the light kernel's own shape is faster)
         const int lim = small_term_limit(), nall = q.nsd + q.nfold + q.nlight;
         q.small_parts = 0;
-        if (q.nodo == 0 && nall > 0 && nall <= 2 * lim)
-            q.small_parts = q.nsd > 0 ? std::max(1, std::min(4, (lim - q.nfold - q.nlight) / q.nsd)) : 1;
+        if (q.nodo == 0 && q.nsd > 0 && nall <= 2 * lim)
+            q.small_parts = std::max(1, std::min(4, (lim - q.nfold - q.nlight) / q.nsd));
     }
     // per traverser pass and level: forward tasks (actor not the traverser),
     // backward tasks, of the slots with more than t players
@@ -2346,9 +2350,12 @@ int launch_phased_iteration(SlotDesc *d_slots, const PhasedTasks &pt, hipStream_
         hipLaunchKernelGGL(k_rsig_phased, dim3(pt.nslots), dim3(SS_BLOCK), 0, st, d_slots, pt.nslots, t);
         for (int l = 0; l < pt.L; ++l) {
             int cnt = fo[l + 1] - fo[l];
-            if (cnt > 0)
-                hipLaunchKernelGGL(k_fwd_phased, dim3(std::min((cnt + per_of(cnt) - 1) / per_of(cnt), SWEEP_GRID)),
-                                   dim3(SWEEP_BLOCK), 0, st, d_slots, pt.d_recs + fo[l], cnt, t, per_of(cnt));
+            if (cnt > 0 && per_of(cnt) == 3)
+                hipLaunchKernelGGL(k_fwd_grouped, dim3(std::min((cnt + 2) / 3, SWEEP_GRID)), dim3(SWEEP_BLOCK), 0, st,
+                                   d_slots, pt.d_recs + fo[l], cnt, t);
+            else if (cnt > 0)
+                hipLaunchKernelGGL(k_fwd_phased, dim3(std::min(cnt, SWEEP_GRID)), dim3(SWEEP_BLOCK), 0, st, d_slots,
+                                   pt.d_recs + fo[l], cnt, t);
         }
         if (pt.npsplit > 0) {
             int rc = launch_prec_phased(d_slots, pt, t, st);
