@@ -102,6 +102,7 @@ struct Request {
     double expl = -1;
     double gains[SS_MAX_PLAYERS] = {0, 0, 0, 0, 0, 0};
     int expl_at = -1;
+    bool joint_done = false;   // the joint deal mass (weights only) is in the slot's xout
     double t_submit = 0, t_admit = 0, t_done = 0, kernel_ms = 0;
     SSOutput out{};
     SSReport rep{};
@@ -319,7 +320,9 @@ int SSServer::check_and_retire() {
     }
     if (any) {
         SS_HIP(hipMemcpyAsync(d_want, want.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
-        int rc = launch_phased_expl(d_slots, pt, d_want, scratch, n, st);
+        bool joint = false;
+        for (int i = 0; i < n; ++i) joint |= want[i] && !resident[i]->joint_done;
+        int rc = launch_phased_expl(d_slots, pt, d_want, scratch, n, st, joint);
         if (rc) return rc;
         SS_HIP(hipStreamSynchronize(st));
         for (int i = 0; i < n; ++i) {
@@ -328,6 +331,7 @@ int SSServer::check_and_retire() {
             double xo[2 * SS_MAX_PLAYERS + 1];
             SS_HIP(hipMemcpy(xo, r->dev->desc.xout, sizeof(double) * (2 * r->dev->plan.NP + 1), hipMemcpyDeviceToHost));
             r->expl = expl_from_xout(xo, r->dev->plan.NP, r->dev->plan.starting_pot, r->gains);
+            r->joint_done = true;
             r->expl_at = r->iters;
             while (r->next_check <= r->iters && r->cfg.check_every > 0) r->next_check += r->cfg.check_every;
         }
