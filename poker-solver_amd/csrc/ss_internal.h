@@ -147,6 +147,7 @@ struct PhasedTasks {
         int off = 0, nodo = 0, nsd = 0, nfold = 0, nlight = 0;
         int sd_parts = 1;   // workgroups per two-opponent showdown task
         int small_parts = 0;   // > 0: the pass's closed-form terminals as one launch, showdowns split this many ways
+        bool rsig = false;     // some listed subgame has t + 1 players (its root snapshot pass)
     };
     std::vector<PassTerm> pterm;
     uint32_t *d_tasks = nullptr;

@@ -2223,6 +2223,10 @@ int build_phased_tasks(const std::vector<Slot *> &slots, PhasedTasks &pt, hipStr
     for (int t = 0; t < pt.NPmax; ++t) {
         PhasedTasks::PassTerm &q = pt.pterm[t];
         q.off = (int)tasks.size();
+        // the root snapshot is taken in a subgame's last pass: only passes
+        // that are some listed subgame's last launch it
+        q.rsig = false;
+        for (int si = 0; si < pt.nslots; ++si) q.rsig |= on(si) && slots[si]->plan.NP == t + 1;
         auto seg = [&](int lo, int hi) {
             const size_t b = tasks.size();
             for (auto &e : term)
@@ -2347,7 +2351,8 @@ static int per_of(int cnt) {
 int launch_phased_iteration(SlotDesc *d_slots, const PhasedTasks &pt, hipStream_t st) {
     for (int t = 0; t < pt.NPmax; ++t) {
         const int *fo = &pt.sw_off[(size_t)t * 2 * (pt.L + 1)], *bo = fo + pt.L + 1;
-        hipLaunchKernelGGL(k_rsig_phased, dim3(pt.nslots), dim3(SS_BLOCK), 0, st, d_slots, pt.nslots, t);
+        if (pt.pterm[t].rsig)
+            hipLaunchKernelGGL(k_rsig_phased, dim3(pt.nslots), dim3(SS_BLOCK), 0, st, d_slots, pt.nslots, t);
         for (int l = 0; l < pt.L; ++l) {
             int cnt = fo[l + 1] - fo[l];
             if (cnt > 0 && per_of(cnt) == 3)
