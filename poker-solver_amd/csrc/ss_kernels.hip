@@ -2113,16 +2113,17 @@ static int term_cost(const Plan &pl, int n, bool prec) {
 // backfilled from the longest remaining one, so the order stays a permutation.
 // terminal workgroups of one pass that run as one launch (k_term_small): one
 // round of two 76 KB workgroups per CU
-static int small_term_limit() {
-    static const int lim = [] {
-        int cus = 0, dev = 0;
+static int device_cus() {
+    static const int cus = [] {
+        int n = 0, dev = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
-            cus = 256;
-        return 2 * cus;
+            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+            n = 256;   // MI355X
+        return n;
     }();
-    return lim;
+    return cus;
 }
+static int small_term_limit() { return 2 * device_cus(); }
 
 #define SS_NUM_XCD 8
 static void xcd_interleave(std::vector<uint32_t> &v, size_t b, size_t e) {
@@ -2241,13 +2242,7 @@ int build_phased_tasks(const std::vector<Slot *> &slots, PhasedTasks &pt, hipStr
         // fewer two-opponent showdowns than resident workgroup slots (two per
         // CU): split each one's hands, up to four ways, so the pass's latency
         // is not one whole node's
-        static const int sd_slots = [] {
-            int cus = 0, dev = 0;
-            if (hipGetDevice(&dev) != hipSuccess ||
-                hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
-                cus = 256;
-            return 2 * cus;
-        }();
+        const int sd_slots = 2 * device_cus();
         q.sd_parts = q.nsd > 0 ? std::max(1, std::min(4, sd_slots / q.nsd)) : 1;
         // two-opponent showdowns and the other closed-form classes within two
         // rounds of resident workgroups: one launch (k_term_small), the
@@ -2338,14 +2333,7 @@ static const int SWEEP_GRID = [] {
 // sweep tasks per workgroup: three when they still fill three resident
 // workgroups per CU that way, else one
 static int per_of(int cnt) {
-    static const int full = [] {
-        int cus = 0, dev = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
-            cus = 256;
-        return 3 * 3 * cus;
-    }();
-    return cnt >= full ? 3 : 1;
+    return cnt >= 3 * 3 * device_cus() ? 3 : 1;
 }
 
 int launch_phased_iteration(SlotDesc *d_slots, const PhasedTasks &pt, hipStream_t st) {
