@@ -1307,7 +1307,7 @@ __global__ void __launch_bounds__(SS_BLOCK) k_xjoint(const SlotDesc *__restrict_
     red[threadIdx.x] = acc;
     __syncthreads();
     for (int off = blockDim.x / 2; off > 0; off >>= 1) {
-        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
         __syncthreads();
     }
     if (threadIdx.x == 0) s->xout[2 * s->NP] = red[0];
@@ -1781,10 +1781,10 @@ __device__ __forceinline__ void bwd_own(const SweepTask &kr, int trav, int s_act
     }
 }
 
-// Backward tasks: two hands per thread, except the traverser's own nodes with
-// more than three actions (24 loads per hand), which keep one hand per round.
-// At most 64 VGPRs (each path needs fewer; unbounded, the branches together
-// are given 77), so that three workgroups fit a CU.
+// Backward tasks: two hands per thread at other players' nodes; the
+// traverser's own nodes (child values, regret and strategy-sum rows: up to 24
+// loads per hand) one hand per thread per round. 78 VGPRs (two workgroups per
+// CU): capped at 72 for a third the paths spill and measure slower.
 __global__ void __launch_bounds__(SWEEP_BLOCK) k_bwd_phased(SlotDesc *__restrict__ slots,
                                                             const SweepRec *__restrict__ recs, int ntasks,
                                                             int trav) {
@@ -1802,6 +1802,7 @@ __global__ void __launch_bounds__(SWEEP_BLOCK) k_bwd_phased(SlotDesc *__restrict
             if (few) bwd_task<3, 2, false>(kr, trav, s_act, s_it, s_end, s_np);
             else bwd_task<NT_DEC, 2, false>(kr, trav, s_act, s_it, s_end, s_np);
         }
+        SWEEP_USE(warm);
     }
 }
 
@@ -1967,7 +1968,7 @@ __global__ void __launch_bounds__(SS_BLOCK) k_xjoint_phased(SlotDesc *__restrict
         red[threadIdx.x] = acc;
         __syncthreads();
         for (int off = blockDim.x / 2; off > 0; off >>= 1) {
-            if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+            if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
             __syncthreads();
         }
         if (threadIdx.x == 0) s->xout[2 * s->NP] = red[0];
