@@ -42,6 +42,16 @@ int ssdbg_plan_bytes(const SSProblem *problem, int64_t *algo_bytes_per_iter, int
  * *ms when ms is not NULL. */
 int ssdbg_copy(int64_t bytes, int repeats, float *ms);
 
+/* Launch shapes of the phased form, for tests of their equivalence (every
+ * shape gives the same results bit for bit); -1 restores the automatic
+ * choice. fwd_group_min: forward-sweep levels with at least this many tasks
+ * run three tasks per workgroup (0: every level). term_small: 0 never runs a
+ * pass's terminal classes as one launch, 1 when they fit (the default).
+ * sd_parts: workgroups per two-opponent showdown (1-4) when the classes run
+ * as separate launches. Applies to task lists built afterwards; not
+ * thread-safe against concurrent solves. */
+int ssdbg_set_phased_shape(int fwd_group_min, int term_small, int sd_parts);
+
 #ifdef __cplusplus
 }
 #endif

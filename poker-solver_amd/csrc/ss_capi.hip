@@ -639,6 +639,12 @@ int ssdbg_plan_bytes(const SSProblem *problem, int64_t *algo_bytes_per_iter, int
     return SS_OK;
 }
 
+int ssdbg_set_phased_shape(int fwd_group_min, int term_small, int sd_parts) {
+    clear_error();
+    set_phased_shape(fwd_group_min, term_small, sd_parts);
+    return SS_OK;
+}
+
 int ssdbg_copy(int64_t bytes, int repeats, float *ms) {
     clear_error();
     return dbg_copy(bytes, repeats, ms);

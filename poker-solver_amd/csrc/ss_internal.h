@@ -167,6 +167,8 @@ int prec_rows_per_task(int nt);   // rows of one precision GEMM task for nt colu
 int build_phased_tasks(const std::vector<Slot *> &slots, PhasedTasks &pt, hipStream_t st,
                        const std::vector<int> *live = nullptr);
 int launch_phased_iteration(SlotDesc *d_slots, const PhasedTasks &pt, hipStream_t st);
+// launch shapes of the phased form (-1: automatic); see ssdbg_set_phased_shape
+void set_phased_shape(int fwd_group_min, int term_small, int sd_parts);
 // joint = false skips the joint deal mass (xout[2 NP]), which depends only on
 // the raw weights: a caller whose xout persists computes it once
 int launch_phased_expl(SlotDesc *d_slots, const PhasedTasks &pt, const int *d_want, double *scratch,

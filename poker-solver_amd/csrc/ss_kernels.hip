@@ -2126,6 +2126,16 @@ static int device_cus() {
 }
 static int small_term_limit() { return 2 * device_cus(); }
 
+// forced launch shapes (tests); -1: automatic
+static struct {
+    int fwd_group_min = -1, term_small = -1, sd_parts = -1;
+} g_shape;
+void set_phased_shape(int fwd_group_min, int term_small, int sd_parts) {
+    g_shape.fwd_group_min = fwd_group_min;
+    g_shape.term_small = term_small;
+    g_shape.sd_parts = sd_parts;
+}
+
 #define SS_NUM_XCD 8
 static void xcd_interleave(std::vector<uint32_t> &v, size_t b, size_t e) {
     std::vector<std::vector<uint32_t>> q(SS_NUM_XCD);
@@ -2245,6 +2255,7 @@ int build_phased_tasks(const std::vector<Slot *> &slots, PhasedTasks &pt, hipStr
         // is not one whole node's
         const int sd_slots = 2 * device_cus();
         q.sd_parts = q.nsd > 0 ? std::max(1, std::min(4, sd_slots / q.nsd)) : 1;
+        if (g_shape.sd_parts >= 1) q.sd_parts = std::min(4, g_shape.sd_parts);
         // two-opponent showdowns and the other closed-form classes within two
         // rounds of resident workgroups: one launch (k_term_small), the
         // showdowns first so the lighter classes cycle through the other
@@ -2253,7 +2264,7 @@ int build_phased_tasks(const std::vector<Slot *> &slots, PhasedTasks &pt, hipStr
         // the light kernel's own shape is faster)
         const int lim = small_term_limit(), nall = q.nsd + q.nfold + q.nlight;
         q.small_parts = 0;
-        if (q.nodo == 0 && q.nsd > 0 && nall <= 2 * lim)
+        if (q.nodo == 0 && q.nsd > 0 && nall <= 2 * lim && g_shape.term_small != 0)
             q.small_parts = std::max(1, std::min(4, (lim - q.nfold - q.nlight) / q.nsd));
     }
     // per traverser pass and level: forward tasks (actor not the traverser),
@@ -2334,6 +2345,7 @@ static const int SWEEP_GRID = [] {
 // sweep tasks per workgroup: three when they still fill three resident
 // workgroups per CU that way, else one
 static int per_of(int cnt) {
+    if (g_shape.fwd_group_min >= 0) return cnt >= g_shape.fwd_group_min ? 3 : 1;
     return cnt >= 3 * 3 * device_cus() ? 3 : 1;
 }
 

@@ -142,6 +142,35 @@ def test_batch_session_modes_equal_plain(mode, monkeypatch):
     b.close()
 
 
+@pytest.mark.parametrize("shape", [(0, 1, -1), (0, 0, 1), (0, 0, 3), (1 << 30, 0, 4), (1 << 30, 0, 2)])
+def test_phased_launch_shapes_equal_plain(shape, monkeypatch):
+    """Every launch shape of the phased form gives the plain form's results
+    bit for bit: forward tasks three to a workgroup or one, a pass's terminal
+    classes as one launch or separate ones, two-opponent showdowns split over
+    one to four workgroups (the automatic choice depends on the batch size,
+    so the shapes are forced here)."""
+    monkeypatch.setenv("SS_BATCH_MODE", "phased")
+    lib = ss_abi.load_library()
+    lib.ssdbg_set_phased_shape.restype = ctypes.c_int
+    lib.ssdbg_set_phased_shape.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    assert lib.ssdbg_set_phased_shape(*shape) == 0
+    try:
+        spots = [wl.small_spot(60 + i, num_players=3, hands_per_player=12) for i in range(3)]
+        spots += [wl.small_spot(70, num_players=2, hands_per_player=12), wl.river_3p(seed=79), wl.river_2p(seed=78)]
+        probs = [ss_api.Problem.from_spot(s) for s in spots]
+        b = ss_api.Batch(probs, 8, flags=1)
+        reps, st = b.run()
+        assert st.mode == 1
+        for i, p in enumerate(probs):
+            got = b.output(i)
+            want = ss_api.solve_plain(p, 8, flags=1)
+            assert np.array_equal(got.avg, want.avg), (shape, i)
+            assert np.array_equal(got.root_strategy, want.root_strategy), (shape, i)
+        b.close()
+    finally:
+        lib.ssdbg_set_phased_shape(-1, -1, -1)
+
+
 def test_phased_batch_with_three_or_more_opponents_equals_plain(monkeypatch):
     """4- and 5-player spots take the odometer terminals (k_term_odo) in the
     phased form; results equal the plain form bit for bit and the state after
