@@ -288,6 +288,44 @@ def precision_bench(boards, iters, ref_boards, cores):
 
 
 # ---------------------------------------------------------------- roofline traffic
+def leaf_bench(reps=5):
+    """Depth-limit leaf values of a 3-player full-range turn subgame
+    (workloads.turn_leaf_workload: 1128 combos per player, 12 sampled rivers,
+    2000 sampled opponent pairs per hero and river, 28 leaves x 64 bias
+    profiles). GPU: wall time of compute_turn_leaf_values (host runout and
+    tuple lists, upload, kernels, 24 MiB read back). CPU: the oracle's
+    vectorised equity on the same inputs plus its value loop on one leaf,
+    scaled to all leaves; the oracle's equity must equal the GPU's bit for
+    bit."""
+    import leaf_values as lv
+    import ss_abi
+    import workloads
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import leaf_oracle as lo
+    lib = ss_abi.load_library()
+    board, infos, hands, mh = workloads.turn_leaf_workload(lib)
+    lv.compute_turn_leaf_values(board, infos, mh, 1500, player_hands=hands)
+    t = time.perf_counter()
+    for _ in range(reps):
+        out = lv.compute_turn_leaf_values(board, infos, mh, 1500, player_hands=hands)
+    gpu_ms = (time.perf_counter() - t) * 1000 / reps
+    nh = [len(h) for h in hands]
+    hc = [[(a, b) for a, b, _ in hs] for hs in hands]
+    t = time.perf_counter()
+    raw = lo.runout_equity(hc, nh, lo.river_runouts(board, nh), mh)
+    eq_ms = (time.perf_counter() - t) * 1000
+    t = time.perf_counter()
+    one = lo.values_from_equity(raw, nh, mh, [(0, infos[0].pot)])
+    val_ms = (time.perf_counter() - t) * 1000 * len(infos)
+    same = bool(np.array_equal(one[:64], out[:64])) if infos[0].leaf_idx == 0 else None
+    return {"workload": "3-player turn, 1128 combos per player, 12 sampled rivers, 2000 sampled opponent "
+                        f"pairs, {len(infos)} leaves x 64 bias profiles",
+            "gpu_ms_per_call": round(gpu_ms, 3), "output_mib": round(out.nbytes / 2**20, 1),
+            "cpu_oracle_ms": round(eq_ms + val_ms, 1), "cpu_oracle_equity_ms": round(eq_ms, 1),
+            "cpu_kind": "port (numpy-vectorised oracle, 1 core; value loop timed on one leaf and scaled)",
+            "speedup_vs_cpu_oracle": round((eq_ms + val_ms) / gpu_ms, 1), "leaf0_equal_oracle": same}
+
+
 def traffic_per_launch(algo_per_launch, workload):
     """HBM bytes per CFR iteration from the committed counter profile
     (profiles/cfr_iteration_traffic.md: calibrated FETCH_SIZE + WRITE_SIZE of a
@@ -324,6 +362,7 @@ def main():
     ap.add_argument("--precision-boards", type=int, default=64, help="0 skips the precision-form run")
     ap.add_argument("--precision-iters", type=int, default=1000)
     ap.add_argument("--precision-ref-boards", type=int, default=8, help="boards compared with the FP64 oracle")
+    ap.add_argument("--no-leaf", action="store_true", help="skip the turn leaf-value producer run")
     a = ap.parse_args()
 
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -445,6 +484,12 @@ def main():
         line["precision_form"] = precision_bench(a.precision_boards, a.precision_iters, a.precision_ref_boards,
                                                  a.cpu_cores or min(16, os.cpu_count() or 1))
         log(f"[bench] precision: {line['precision_form']['solves_per_s']} solves/s")
+
+    if not a.no_leaf:
+        log("[bench] leaf values: 3-player full-range turn subgame")
+        line["leaf_values"] = leaf_bench()
+        log(f"[bench] leaf values: {line['leaf_values']['gpu_ms_per_call']} ms per call on the GPU, "
+            f"{line['leaf_values']['cpu_oracle_ms']} ms CPU oracle")
 
     if not a.no_cpu:
         cores = a.cpu_cores or min(16, os.cpu_count() or 1)

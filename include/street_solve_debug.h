@@ -52,6 +52,11 @@ int ssdbg_copy(int64_t bytes, int repeats, float *ms);
  * thread-safe against concurrent solves. */
 int ssdbg_set_phased_shape(int fwd_group_min, int term_small, int sd_parts);
 
+/* Python-compatible draws: for t < count and k < num_lists, out[t*num_lists+k]
+ * = the index random.Random(seed).choice picks from a list of lens[k] items,
+ * drawn in that order (the opponent sampling of ss_leaf_equity). Host only. */
+int ssdbg_py_choice(int seed, const int *lens, int num_lists, int count, int *out);
+
 #ifdef __cplusplus
 }
 #endif

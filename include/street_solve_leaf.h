@@ -1,0 +1,65 @@
+/*
+ * street_solve_leaf.h -- depth-limit leaf values computed on the GPU.
+ *
+ * Replaces the equity and bias loops of python/leaf_values.py, which the
+ * reference runs in pure Python on the host before every turn or flop solve:
+ *   ss_leaf_equity  <- _accumulate_nplayer_equity (leaf_values.py:573-691)
+ *                      over the runout loops of compute_turn_leaf_values
+ *                      (:761-793) and compute_flop_leaf_equity (:486-523)
+ *   ss_leaf_values  <- the per-leaf bias expansion (:795-816 and :525-567)
+ *                      with biased_leaf_value (:91-122)
+ * The runout lists (numpy RandomState draws, leaf_values.py:757-763 and
+ * :477-492) and the leaf infos (extract_leaf_info_from_tree, :189-262) stay
+ * with the caller; poker-solver_amd/leaf_values.py builds both exactly as the
+ * reference does and calls these two functions.
+ *
+ * Cards are 0..51 (rank * 4 + suit). Return 0 on success, a negative
+ * SS_ERR_* code otherwise (message in ss_last_error()). Both calls are
+ * synchronous and take host buffers.
+ */
+#ifndef STREET_SOLVE_LEAF_H
+#define STREET_SOLVE_LEAF_H
+
+#include "street_solve.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SS_LEAF_MAX_HANDS 1326
+#define SS_LEAF_OPP_COMBO_LIMIT 2000 /* leaf_values.py:629 OPP_COMBO_LIMIT */
+#define SS_LEAF_SAMPLE_SEED 42       /* leaf_values.py:635 random.Random(42) */
+
+/* Per-hand showdown equity averaged over runouts.
+ *   hands       [num_players][max_hands][2] hole cards (rows >= num_hands[p]
+ *               are ignored)
+ *   runouts     [num_runouts][5] five-card boards
+ *   raw_equity  [num_players][max_hands] out: for hand h of player p, the
+ *               mean over the runouts where h and at least one opponent tuple
+ *               are unblocked of (wins + sum 1/ties) / tuples, over all
+ *               opponent tuples (one unblocked hand per other player, no
+ *               shared cards) when there are <= 2000 of them, else over 2000
+ *               tuples drawn as random.Random(42).choice per opponent; 0 for
+ *               hands never counted. Range weights play no part (as in the
+ *               reference). */
+int ss_leaf_equity(int num_players, const int *hands, const int *num_hands, int max_hands,
+                   const int *runouts, int num_runouts, double *raw_equity);
+
+/* Leaf values from equity.
+ *   leaf_idx, leaf_pot  [num_leaf_infos] original leaf index and pot (chips)
+ *   leaf_values  [num_leaf_infos * 4^num_players][num_players][max_hands]
+ *                out: row leaf_idx * 4^NP + combo holds, for h < num_hands[p],
+ *                biased_leaf_value(raw_equity[p][h], pot, s_p, s_opp) with
+ *                s_i = (combo >> 2i) & 3 and s_opp the other player's profile
+ *                (two players) or round-half-even of the mean of the others'
+ *                (3-6 players); rows no leaf info names, and hands beyond
+ *                num_hands[p], are 0. A later info with the same leaf_idx
+ *                overwrites an earlier one; rows past the array are dropped. */
+int ss_leaf_values(int num_players, const double *raw_equity, const int *num_hands, int max_hands,
+                   const int *leaf_idx, const int *leaf_pot, int num_leaf_infos, float *leaf_values);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,517 @@
+// ss_leaf.hip -- depth-limit leaf values on the GPU (turn and flop leaves).
+//
+// Reference: python/leaf_values.py
+//   _accumulate_nplayer_equity :573-691   per-runout equity of every hand of a
+//                                         hero player against the joint
+//                                         opponent hands (exact cross product
+//                                         when it has <= 2000 tuples, else 2000
+//                                         tuples drawn by random.Random(42))
+//   compute_turn_leaf_values   :694-816   runouts = river cards (12 drawn by
+//   compute_flop_leaf_equity   :408-570   numpy RandomState when any range has
+//                                         > 50 hands); equity averaged over the
+//                                         runouts where the hand was counted
+//   biased_leaf_value          :91-122    first-order bias model per
+//                                         (s_self, s_opp) profile pair, written
+//                                         for all 4^NP profiles of every leaf
+//
+// Work split:
+//   host   per (runout, hero) the opponents' unblocked hand lists and the
+//          opponent-tuple pattern: the mixed-radix product (decoded on the
+//          device from the tuple index) or the 2000 draws of a Python-
+//          compatible Mersenne twister (one pattern per distinct list-length
+//          tuple: the generator is reseeded with 42 for every hero and runout,
+//          so draws depend only on the lengths)
+//   device k_leaf_strength  eval7 of every (runout, player, hand)
+//          k_leaf_tuples    one thread per (runout, hero, tuple): the record
+//                           {card mask, best opponent rank, multiplicity of
+//                           that rank}
+//          k_leaf_equity    one 256-lane workgroup per (runout, hero, 256 hero
+//                           hands): the records copied to LDS, then every
+//                           hero hand walks them in order, summing 1 or
+//                           1/ties in float64 exactly as the reference's
+//                           loop does
+//          k_leaf_reduce    per hand, the runout average in runout order
+//          k_leaf_bias      one thread per output value (leaf, profile,
+//                           player, hand), float64 without contraction, then
+//                           rounded to float32
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <map>
+#include <vector>
+
+#include "ss_eval.h"
+#include "ss_internal.h"
+#include "../../include/street_solve_leaf.h"
+
+namespace ss {
+
+namespace {
+
+constexpr int LEAF_BLOCK = 256;
+
+// Python's random.Random: MT19937 (Matsumoto & Nishimura, mt19937ar) seeded
+// by init_by_array with the seed's 32-bit words, and _randbelow(n) drawing
+// bit_length(n) top bits of a 32-bit output until the value is below n.
+struct PyRandom {
+    uint32_t mt[624];
+    int mti = 625;
+
+    void init_genrand(uint32_t s) {
+        mt[0] = s;
+        for (mti = 1; mti < 624; ++mti) mt[mti] = 1812433253u * (mt[mti - 1] ^ (mt[mti - 1] >> 30)) + (uint32_t)mti;
+    }
+    explicit PyRandom(uint32_t seed) {
+        init_genrand(19650218u);
+        int i = 1;
+        for (int k = 624; k; --k) {  // key = {seed}, length 1
+            mt[i] = (mt[i] ^ ((mt[i - 1] ^ (mt[i - 1] >> 30)) * 1664525u)) + seed;
+            if (++i >= 624) { mt[0] = mt[623]; i = 1; }
+        }
+        for (int k = 623; k; --k) {
+            mt[i] = (mt[i] ^ ((mt[i - 1] ^ (mt[i - 1] >> 30)) * 1566083941u)) - (uint32_t)i;
+            if (++i >= 624) { mt[0] = mt[623]; i = 1; }
+        }
+        mt[0] = 0x80000000u;
+        mti = 624;
+    }
+    uint32_t next() {
+        if (mti >= 624) {
+            for (int k = 0; k < 624; ++k) {
+                uint32_t y = (mt[k] & 0x80000000u) | (mt[(k + 1) % 624] & 0x7fffffffu);
+                mt[k] = mt[(k + 397) % 624] ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
+            }
+            mti = 0;
+        }
+        uint32_t y = mt[mti++];
+        y ^= y >> 11;
+        y ^= (y << 7) & 0x9d2c5680u;
+        y ^= (y << 15) & 0xefc60000u;
+        y ^= y >> 18;
+        return y;
+    }
+    int below(int n) {
+        int k = 32 - __builtin_clz((uint32_t)n);
+        uint32_t r;
+        do r = next() >> (32 - k); while (r >= (uint32_t)n);
+        return (int)r;
+    }
+};
+
+// Per (runout, hero) tuple source.
+struct RpDesc {
+    int count;    // tuples to walk (0: hero skipped)
+    int pattern;  // offset into the draw pool (sampled), -1 for the product
+    int lens[SS_MAX_PLAYERS - 1];
+};
+
+// One opponent tuple of a (runout, hero) walk, 32 bytes.
+struct TupleRec {
+    uint32_t mlo, mhi;  // opponents' card mask; all ones when opponents share a card
+    uint32_t best;      // best opponent rank
+    uint32_t mult;      // opponents holding it
+    double tie;         // 1 / (mult + 1), rounded once as the reference's 1.0 / ties
+    double unused;
+};
+
+struct LeafDev {
+    uint16_t *hc = nullptr;      // [np][mh] c0 | c1 << 8
+    int *runouts = nullptr;      // [r][5]
+    uint32_t *str = nullptr;     // [r][np][mh], 0 = blocked / no hand
+    int16_t *valid = nullptr;    // [r][np][mh] unblocked hand indices, in order
+    RpDesc *desc = nullptr;      // [r][np]
+    int16_t *pool = nullptr;     // draws, [tuples][np-1] per pattern
+    double *eq = nullptr;        // [r][np][mh]
+    uint8_t *cnt = nullptr;      // [r][np][mh] hand counted on this runout
+    double *raw = nullptr;       // [np][mh]
+    TupleRec *recs = nullptr;    // [r][np][SS_LEAF_OPP_COMBO_LIMIT]
+    ~LeafDev() {
+        void *p[] = {hc, runouts, str, valid, desc, pool, eq, cnt, raw, recs};
+        for (void *q : p)
+            if (q) (void)hipFree(q);
+    }
+};
+
+template <class T>
+int dev_upload(T **d, const T *h, size_t n) {
+    SS_HIP(hipMalloc((void **)d, std::max<size_t>(n, 1) * sizeof(T)));
+    if (n) SS_HIP(hipMemcpy(*d, h, n * sizeof(T), hipMemcpyHostToDevice));
+    return SS_OK;
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(LEAF_BLOCK) void k_leaf_strength(const uint16_t *__restrict__ hc,
+                                                              const int *__restrict__ runouts,
+                                                              uint32_t *__restrict__ str, int np, int mh,
+                                                              int total) {
+    __shared__ uint8_t tab[8192];
+    fill_straight_table(tab, threadIdx.x, blockDim.x);
+    __syncthreads();
+    int i = blockIdx.x * LEAF_BLOCK + threadIdx.x;
+    if (i >= total) return;
+    int h = i % mh, p = (i / mh) % np, r = i / (mh * np);
+    uint32_t v = hc[p * mh + h];
+    const int *b = runouts + 5 * r;
+    int c[7] = {b[0], b[1], b[2], b[3], b[4], (int)(v & 0xff), (int)(v >> 8)};
+    bool dead = v == 0xffffu;
+    for (int k = 0; k < 5; ++k) dead |= (c[k] == c[5]) | (c[k] == c[6]);
+    str[i] = dead ? 0u : eval7_lut(c, tab);
+}
+
+// biased_leaf_value's tables (python/leaf_values.py:84-88), by bias profile
+// 0 unmodified, 1 fold, 2 call, 3 raise.
+__constant__ double c_fold_prob[4] = {0.0, 0.35, 0.0, 0.0};
+__constant__ double c_pot_factor[4] = {1.0, 0.7, 1.0, 1.4};
+
+
+// One record per (runout, hero, tuple): every tuple source resolved to
+// hand indices, the opponents' card mask, best rank and its multiplicity.
+__global__ __launch_bounds__(LEAF_BLOCK) void k_leaf_tuples(const uint16_t *__restrict__ hc,
+                                                            const uint32_t *__restrict__ str,
+                                                            const int16_t *__restrict__ valid,
+                                                            const RpDesc *__restrict__ desc,
+                                                            const int16_t *__restrict__ pool,
+                                                            TupleRec *__restrict__ recs, int np, int mh) {
+    const int rp = blockIdx.y, r = rp / np, p = rp % np;
+    const int t = blockIdx.x * LEAF_BLOCK + threadIdx.x;
+    const RpDesc d = desc[rp];
+    if (t >= d.count) return;
+    const int no = np - 1;
+    // Opponents taken last to first: the product index is mixed radix with
+    // the last opponent fastest; mask, best rank and multiplicity do not
+    // depend on the order.
+    uint64_t m = 0;
+    uint32_t best = 0, mult = 0;
+    int x = t;
+    for (int k = no - 1; k >= 0; --k) {
+        int pos;
+        if (d.pattern < 0) {
+            int len = desc[rp].lens[k];
+            pos = x % len;
+            x /= len;
+        } else {
+            pos = pool[d.pattern + t * no + k];
+        }
+        int q = k + (k >= p);
+        int hidx = valid[((size_t)r * np + q) * mh + pos];
+        uint32_t v = hc[q * mh + hidx];
+        m |= (1ull << (v & 0xff)) | (1ull << (v >> 8));
+        uint32_t s = str[((size_t)r * np + q) * mh + hidx];
+        if (s > best) { best = s; mult = 1; }
+        else if (s == best) ++mult;
+    }
+    if (__builtin_popcountll(m) != 2 * no) m = ~0ull;  // every hero hand collides with it
+    const double tie = mult == 1 ? 1.0 / 2 : mult == 2 ? 1.0 / 3 : mult == 3 ? 1.0 / 4 : mult == 4 ? 1.0 / 5 : 1.0 / 6;
+    recs[(size_t)rp * SS_LEAF_OPP_COMBO_LIMIT + t] =
+        TupleRec{(uint32_t)m, (uint32_t)(m >> 32), best, mult, tie, 0.0};
+}
+
+// One tuple of the walk: counted when the hero holds none of its cards (a
+// tuple whose opponents share a card has an all-ones mask); then 1 for a win,
+// the tuple's 1 / ties for a tie. Adding 0.0 for a loss or a skipped tuple
+// leaves acc as the reference's loop leaves it.
+__device__ __forceinline__ void tuple_step(const TupleRec &e, uint32_t hs, uint32_t hlo, uint32_t hhi,
+                                           double &acc, int &nvalid) {
+    const bool use = ((e.mlo & hlo) | (e.mhi & hhi)) == 0;
+    const double add = hs > e.best ? 1.0 : (hs == e.best ? e.tie : 0.0);
+    nvalid += use ? 1 : 0;
+    acc += use ? add : 0.0;
+}
+
+// One 256-lane workgroup per (runout, hero, 256 hero hands): the (runout,
+// hero) records are copied to LDS (64 KB), then each lane walks them in order
+// for its hand, four records per LDS wait.
+constexpr int EQ_BLOCK = 256;
+
+__global__ __launch_bounds__(EQ_BLOCK) void k_leaf_equity(const uint16_t *__restrict__ hc,
+                                                          const uint32_t *__restrict__ str,
+                                                          const RpDesc *__restrict__ desc,
+                                                          const TupleRec *__restrict__ recs,
+                                                          double *__restrict__ eq,
+                                                          uint8_t *__restrict__ cnt, int np, int mh) {
+    __shared__ TupleRec srec[SS_LEAF_OPP_COMBO_LIMIT];
+    const int rp = blockIdx.y, r = rp / np, p = rp % np;
+    const int count = desc[rp].count;
+    {
+        const uint4 *src = reinterpret_cast<const uint4 *>(recs + (size_t)rp * SS_LEAF_OPP_COMBO_LIMIT);
+        uint4 *dst = reinterpret_cast<uint4 *>(srec);
+        const int n16 = count * (int)(sizeof(TupleRec) / sizeof(uint4));
+        for (int i = threadIdx.x; i < n16; i += EQ_BLOCK) dst[i] = src[i];
+    }
+    __syncthreads();
+    const int h = blockIdx.x * EQ_BLOCK + threadIdx.x;
+    if (h >= mh) return;
+    const size_t o = ((size_t)r * np + p) * mh + h;
+    const uint32_t hs = str[o];
+    double acc = 0.0;
+    int nvalid = 0;
+    if (hs && count) {
+        const uint32_t v = hc[p * mh + h];
+        const uint64_t hm = (1ull << (v & 0xff)) | (1ull << (v >> 8));
+        const uint32_t hlo = (uint32_t)hm, hhi = (uint32_t)(hm >> 32);
+        int t = 0;
+        for (; t + 4 <= count; t += 4) {
+            const TupleRec e0 = srec[t], e1 = srec[t + 1], e2 = srec[t + 2], e3 = srec[t + 3];
+            tuple_step(e0, hs, hlo, hhi, acc, nvalid);
+            tuple_step(e1, hs, hlo, hhi, acc, nvalid);
+            tuple_step(e2, hs, hlo, hhi, acc, nvalid);
+            tuple_step(e3, hs, hlo, hhi, acc, nvalid);
+        }
+        for (; t < count; ++t) tuple_step(srec[t], hs, hlo, hhi, acc, nvalid);
+    }
+    eq[o] = nvalid ? acc / (double)nvalid : 0.0;
+    cnt[o] = nvalid ? 1 : 0;
+}
+
+__global__ __launch_bounds__(LEAF_BLOCK) void k_leaf_reduce(const double *__restrict__ eq,
+                                                            const uint8_t *__restrict__ cnt,
+                                                            double *__restrict__ raw, int nrun, int n) {
+    int i = blockIdx.x * LEAF_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0, c = 0.0;
+    for (int r = 0; r < nrun; ++r)
+        if (cnt[(size_t)r * n + i]) {
+            s += eq[(size_t)r * n + i];
+            c += 1.0;
+        }
+    raw[i] = c > 0.0 ? s / c : 0.0;
+}
+
+__device__ __forceinline__ double biased_value(double equity, double pot, int s_self, int s_opp) {
+#pragma clang fp contract(off)
+    double fp = c_fold_prob[s_opp];
+    double showdown_pot = pot * (c_pot_factor[s_self] * c_pot_factor[s_opp]);
+    double fold_payoff = 0.5 * pot;
+    double showdown_payoff = (equity - 0.5) * showdown_pot;
+    return fp * fold_payoff + (1.0 - fp) * showdown_payoff;
+}
+
+__global__ __launch_bounds__(LEAF_BLOCK) void k_leaf_bias(const double *__restrict__ raw,
+                                                          const long long *__restrict__ flat_pot,
+                                                          const int *__restrict__ nh, float *__restrict__ out,
+                                                          int np, int mh) {
+    const int j = blockIdx.y * LEAF_BLOCK + threadIdx.x;  // p * mh + h
+    if (j >= np * mh) return;
+    const long long leaf = blockIdx.x;
+    const int p = j / mh, h = j - p * mh;
+    const long long pot = flat_pot[leaf];
+    float v = 0.0f;
+    if (pot != LLONG_MIN && h < nh[p]) {
+        const int combo = (int)(leaf & ((1ll << (2 * np)) - 1));
+        int s_self = (combo >> (2 * p)) & 3, s_opp;
+        if (np > 2) {
+            int other = 0;
+            for (int q = 0; q < np; ++q)
+                if (q != p) other += (combo >> (2 * q)) & 3;
+            s_opp = (int)rint((double)other / (double)(np - 1));  // round half to even
+            s_opp = s_opp < 0 ? 0 : (s_opp > 3 ? 3 : s_opp);
+        } else {
+            s_opp = (combo >> (2 * (1 - p))) & 3;
+        }
+        v = (float)biased_value(raw[j], (double)pot, s_self, s_opp);
+    }
+    out[leaf * np * mh + j] = v;
+}
+
+namespace {
+
+int check_hands(int np, const int *hands, const int *nh, int mh) {
+    if (np < 2 || np > SS_MAX_PLAYERS) return set_error(SS_ERR_ARG, "num_players %d (2..6)", np);
+    if (mh < 1 || mh > SS_LEAF_MAX_HANDS) return set_error(SS_ERR_ARG, "max_hands %d (1..1326)", mh);
+    if (!nh) return set_error(SS_ERR_ARG, "num_hands is NULL");
+    for (int p = 0; p < np; ++p) {
+        if (nh[p] < 0 || nh[p] > mh)
+            return set_error(SS_ERR_ARG, "player %d has %d hands (max_hands %d)", p, nh[p], mh);
+        for (int h = 0; h < nh[p]; ++h) {
+            int a = hands[(p * mh + h) * 2], b = hands[(p * mh + h) * 2 + 1];
+            if (a < 0 || a > 51 || b < 0 || b > 51 || a == b)
+                return set_error(SS_ERR_ARG, "player %d hand %d has cards (%d, %d)", p, h, a, b);
+        }
+    }
+    return SS_OK;
+}
+
+}  // namespace
+
+int leaf_equity(int np, const int *hands, const int *nh, int mh, const int *runouts, int nrun, double *raw) {
+    int rc = check_hands(np, hands, nh, mh);
+    if (rc) return rc;
+    if (nrun < 0 || (nrun > 0 && !runouts)) return set_error(SS_ERR_ARG, "bad runouts");
+    if (!raw) return set_error(SS_ERR_ARG, "raw_equity is NULL");
+    for (int r = 0; r < nrun; ++r) {
+        uint64_t m = 0;
+        for (int k = 0; k < 5; ++k) {
+            int c = runouts[5 * r + k];
+            if (c < 0 || c > 51 || (m >> c & 1)) return set_error(SS_ERR_ARG, "runout %d: bad board", r);
+            m |= 1ull << c;
+        }
+    }
+    const int no = np - 1;
+    std::vector<uint16_t> hc((size_t)np * mh, 0xffffu);
+    for (int p = 0; p < np; ++p)
+        for (int h = 0; h < nh[p]; ++h)
+            hc[p * mh + h] = (uint16_t)(hands[(p * mh + h) * 2] | hands[(p * mh + h) * 2 + 1] << 8);
+
+    // Unblocked hand lists per (runout, player) and the tuple source per (runout, hero).
+    std::vector<int16_t> valid((size_t)nrun * np * mh, 0);
+    std::vector<int> vlen((size_t)nrun * np, 0);
+    std::vector<RpDesc> desc((size_t)nrun * np);
+    std::vector<int16_t> pool;
+    std::map<std::vector<int>, int> patterns;
+    for (int r = 0; r < nrun; ++r) {
+        uint64_t bm = 0;
+        for (int k = 0; k < 5; ++k) bm |= 1ull << runouts[5 * r + k];
+        for (int p = 0; p < np; ++p) {
+            int n = 0;
+            for (int h = 0; h < nh[p]; ++h) {
+                uint16_t v = hc[p * mh + h];
+                if (!((bm >> (v & 0xff)) & 1) && !((bm >> (v >> 8)) & 1)) valid[((size_t)r * np + p) * mh + n++] = (int16_t)h;
+            }
+            vlen[r * np + p] = n;
+        }
+        for (int p = 0; p < np; ++p) {
+            RpDesc &d = desc[r * np + p];
+            d = RpDesc{0, -1, {0, 0, 0, 0, 0}};
+            if (!vlen[r * np + p]) continue;
+            std::vector<int> lens;
+            long long total = 1;
+            for (int k = 0; k < no; ++k) {
+                int q = k + (k >= p);
+                d.lens[k] = vlen[r * np + q];
+                lens.push_back(d.lens[k]);
+                total = std::min<long long>(total * d.lens[k], 1ll << 40);
+            }
+            if (total == 0) continue;
+            if (total <= SS_LEAF_OPP_COMBO_LIMIT) {
+                d.count = (int)total;
+                continue;
+            }
+            auto it = patterns.find(lens);
+            if (it == patterns.end()) {
+                int off = (int)pool.size();
+                PyRandom rng(SS_LEAF_SAMPLE_SEED);
+                for (int t = 0; t < SS_LEAF_OPP_COMBO_LIMIT; ++t)
+                    for (int k = 0; k < no; ++k) pool.push_back((int16_t)rng.below(lens[k]));
+                it = patterns.emplace(lens, off).first;
+            }
+            d.count = SS_LEAF_OPP_COMBO_LIMIT;
+            d.pattern = it->second;
+        }
+    }
+
+    if (nrun == 0) {
+        std::fill(raw, raw + (size_t)np * mh, 0.0);
+        return SS_OK;
+    }
+    LeafDev D;
+    size_t nrp = (size_t)nrun * np, nall = nrp * mh;
+    if ((rc = dev_upload(&D.hc, hc.data(), hc.size()))) return rc;
+    if ((rc = dev_upload(&D.runouts, runouts, (size_t)nrun * 5))) return rc;
+    if ((rc = dev_upload(&D.valid, valid.data(), valid.size()))) return rc;
+    if ((rc = dev_upload(&D.desc, desc.data(), desc.size()))) return rc;
+    if ((rc = dev_upload(&D.pool, pool.data(), pool.size()))) return rc;
+    SS_HIP(hipMalloc((void **)&D.str, nall * sizeof(uint32_t)));
+    SS_HIP(hipMalloc((void **)&D.eq, nall * sizeof(double)));
+    SS_HIP(hipMalloc((void **)&D.cnt, nall));
+    SS_HIP(hipMalloc((void **)&D.raw, (size_t)np * mh * sizeof(double)));
+    SS_HIP(hipMalloc((void **)&D.recs, nrp * SS_LEAF_OPP_COMBO_LIMIT * sizeof(TupleRec)));
+    k_leaf_strength<<<(unsigned)((nall + LEAF_BLOCK - 1) / LEAF_BLOCK), LEAF_BLOCK>>>(D.hc, D.runouts, D.str, np, mh,
+                                                                                     (int)nall);
+    SS_HIP(hipGetLastError());
+    k_leaf_tuples<<<dim3((SS_LEAF_OPP_COMBO_LIMIT + LEAF_BLOCK - 1) / LEAF_BLOCK, (unsigned)nrp), LEAF_BLOCK>>>(
+        D.hc, D.str, D.valid, D.desc, D.pool, D.recs, np, mh);
+    SS_HIP(hipGetLastError());
+    k_leaf_equity<<<dim3((mh + EQ_BLOCK - 1) / EQ_BLOCK, (unsigned)nrp), EQ_BLOCK>>>(D.hc, D.str, D.desc, D.recs,
+                                                                                  D.eq, D.cnt, np, mh);
+    SS_HIP(hipGetLastError());
+    k_leaf_reduce<<<(np * mh + LEAF_BLOCK - 1) / LEAF_BLOCK, LEAF_BLOCK>>>(D.eq, D.cnt, D.raw, nrun, np * mh);
+    SS_HIP(hipGetLastError());
+    SS_HIP(hipMemcpy(raw, D.raw, (size_t)np * mh * sizeof(double), hipMemcpyDeviceToHost));
+    return SS_OK;
+}
+
+int leaf_values(int np, const double *raw, const int *nh, int mh, const int *leaf_idx, const int *leaf_pot,
+                int ninfo, float *out) {
+    if (np < 2 || np > SS_MAX_PLAYERS) return set_error(SS_ERR_ARG, "num_players %d (2..6)", np);
+    if (mh < 1 || mh > SS_LEAF_MAX_HANDS) return set_error(SS_ERR_ARG, "max_hands %d (1..1326)", mh);
+    if (!raw || !nh || !out || ninfo < 0 || (ninfo > 0 && (!leaf_idx || !leaf_pot)))
+        return set_error(SS_ERR_ARG, "bad arguments");
+    for (int p = 0; p < np; ++p)
+        if (nh[p] < 0 || nh[p] > mh) return set_error(SS_ERR_ARG, "player %d has %d hands (max_hands %d)", p, nh[p], mh);
+    const long long cont = 1ll << (2 * np), nleaves = (long long)ninfo * cont;
+    // Pot of each output leaf: the last leaf info naming it wins, leaves no
+    // info names stay zero (LLONG_MIN marks them).
+    std::vector<long long> flat(std::max<long long>(nleaves, 1), LLONG_MIN);
+    for (int i = 0; i < ninfo; ++i) {
+        if (leaf_idx[i] < 0) return set_error(SS_ERR_ARG, "leaf info %d has leaf_idx %d", i, leaf_idx[i]);
+        for (long long c = 0; c < cont; ++c) {
+            long long f = (long long)leaf_idx[i] * cont + c;
+            if (f < nleaves) flat[f] = leaf_pot[i];
+        }
+    }
+    const long long total = nleaves * np * mh;
+    if (total == 0) return SS_OK;
+    double *d_raw = nullptr;
+    long long *d_pot = nullptr;
+    int *d_nh = nullptr;
+    float *d_out = nullptr;
+    int rc = SS_OK;
+    auto cleanup = [&]() {
+        if (d_raw) (void)hipFree(d_raw);
+        if (d_pot) (void)hipFree(d_pot);
+        if (d_nh) (void)hipFree(d_nh);
+        if (d_out) (void)hipFree(d_out);
+    };
+    if ((rc = dev_upload(&d_raw, raw, (size_t)np * mh)) || (rc = dev_upload(&d_pot, flat.data(), (size_t)nleaves)) ||
+        (rc = dev_upload(&d_nh, nh, (size_t)np))) {
+        cleanup();
+        return rc;
+    }
+    hipError_t e = hipMalloc((void **)&d_out, (size_t)total * sizeof(float));
+    if (e == hipSuccess) {
+        k_leaf_bias<<<dim3((unsigned)nleaves, (np * mh + LEAF_BLOCK - 1) / LEAF_BLOCK), LEAF_BLOCK>>>(
+            d_raw, d_pot, d_nh, d_out, np, mh);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(out, d_out, (size_t)total * sizeof(float), hipMemcpyDeviceToHost);
+    cleanup();
+    if (e != hipSuccess) return set_error(SS_ERR_HIP, "leaf values: %s", hipGetErrorString(e));
+    return SS_OK;
+}
+
+int py_choice_draws(int seed, const int *lens, int nlists, int count, int *out) {
+    if (!lens || !out || nlists < 1 || count < 0) return set_error(SS_ERR_ARG, "bad arguments");
+    for (int k = 0; k < nlists; ++k)
+        if (lens[k] < 1) return set_error(SS_ERR_ARG, "list %d is empty", k);
+    PyRandom rng((uint32_t)seed);
+    for (int t = 0; t < count; ++t)
+        for (int k = 0; k < nlists; ++k) out[t * nlists + k] = rng.below(lens[k]);
+    return SS_OK;
+}
+
+}  // namespace ss
+
+extern "C" {
+
+int ss_leaf_equity(int num_players, const int *hands, const int *num_hands, int max_hands, const int *runouts,
+                   int num_runouts, double *raw_equity) {
+    ss::clear_error();
+    if (!hands) return ss::set_error(SS_ERR_ARG, "hands is NULL");
+    return ss::leaf_equity(num_players, hands, num_hands, max_hands, runouts, num_runouts, raw_equity);
+}
+
+int ss_leaf_values(int num_players, const double *raw_equity, const int *num_hands, int max_hands,
+                   const int *leaf_idx, const int *leaf_pot, int num_leaf_infos, float *leaf_values) {
+    ss::clear_error();
+    return ss::leaf_values(num_players, raw_equity, num_hands, max_hands, leaf_idx, leaf_pot, num_leaf_infos,
+                           leaf_values);
+}
+
+int ssdbg_py_choice(int seed, const int *lens, int num_lists, int count, int *out) {
+    ss::clear_error();
+    return ss::py_choice_draws(seed, lens, num_lists, count, out);
+}
+
+}  // extern "C"

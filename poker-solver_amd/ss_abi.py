@@ -147,7 +147,8 @@ class SSServerConfig(ctypes.Structure):
     ]
 
 
-# Every symbol include/street_solve.h declares, with its ctypes signature.
+# Every symbol include/street_solve.h and include/street_solve_leaf.h declare,
+# with its ctypes signature.
 _P = ctypes.POINTER
 SIGNATURES = {
     "ss_build_tree": (ctypes.c_int, [_P(ctypes.c_int), ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -175,6 +176,10 @@ SIGNATURES = {
     "ss_server_drain": (ctypes.c_int, [ctypes.c_void_p]),
     "ss_server_destroy": (None, [ctypes.c_void_p]),
     "ss_problem_footprint": (ctypes.c_int64, [_P(SSProblem)]),
+    "ss_leaf_equity": (ctypes.c_int, [ctypes.c_int, _P(ctypes.c_int), _P(ctypes.c_int), ctypes.c_int,
+                                      _P(ctypes.c_int), ctypes.c_int, _P(ctypes.c_double)]),
+    "ss_leaf_values": (ctypes.c_int, [ctypes.c_int, _P(ctypes.c_double), _P(ctypes.c_int), ctypes.c_int,
+                                      _P(ctypes.c_int), _P(ctypes.c_int), ctypes.c_int, _P(ctypes.c_float)]),
     "ss_last_error": (ctypes.c_char_p, []),
     "ss_version": (ctypes.c_char_p, []),
 }

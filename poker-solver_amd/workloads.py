@@ -159,3 +159,26 @@ def small_spot(seed, num_players=2, hands_per_player=6, board_cards=5, pot=1000,
     return Spot(board, pot, stack, list(bet_sizes), num_players, hands, weights,
                 max_raises=max_raises, leaf_sigma=(0.3 * pot if use_leaves else None), seed=seed,
                 kind="river" if board_cards == 5 else "turn")
+
+
+def turn_leaf_workload(lib):
+    """Leaf values of one 3-player turn subgame at full ranges: board Kd 7c 2h
+    9s, every player holding all 1128 unblocked combos, pot 1500, stack 5000,
+    one bet size (0.75 pot) with continuation leaves; the leaf infos come from
+    the tree the library builds (28 leaves, 64 bias profiles each). Returns
+    (board, leaf_infos, player_hands, max_hands)."""
+    import ctypes
+    import ss_abi
+    import leaf_values as lv
+    board = [45, 20, 2, 31]
+    td = ss_abi.SSTreeData()
+    rc = lib.ss_build_tree((ctypes.c_int * 4)(*board), 4, 1500, 5000, (ctypes.c_float * 1)(0.75), 1, 3,
+                           (ctypes.c_int * 3)(0, 1, 2), 0, 1, ctypes.byref(td))
+    if rc != 0:
+        raise RuntimeError(f"ss_build_tree failed: {ss_abi.last_error(lib)}")
+    try:
+        infos = lv.extract_leaf_info_from_tree(td, num_players=3)
+    finally:
+        lib.ss_free_tree(ctypes.byref(td))
+    combos = [(int(a), int(b), 1.0) for a, b in all_combos(board)]
+    return board, infos, [combos] * 3, len(combos)

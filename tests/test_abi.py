@@ -3,6 +3,7 @@ host evaluator, argument checking, and the Python drop-in wrapper's host logic.
 No GPU is touched here (tree building and the host evaluator need none)."""
 
 import ctypes
+import glob
 import gzip
 import json
 import os
@@ -23,8 +24,8 @@ TREE_KEYS = ["num_nodes", "num_children_total", "num_leaves", "max_depth", "node
 
 def declared_symbols():
     names = set()
-    for h in ("street_solve.h", "street_solve_debug.h"):
-        txt = open(os.path.join(REPO, "include", h)).read()
+    for h in sorted(glob.glob(os.path.join(REPO, "include", "*.h"))):
+        txt = open(h).read()
         txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
         for m in re.finditer(r"\b((?:ss|ssdbg)_[a-z0-9_]+)\s*\(", txt):
             names.add(m.group(1))
@@ -34,6 +35,7 @@ def declared_symbols():
 def test_library_exports_every_declared_symbol(lib):
     syms = declared_symbols()
     assert "ss_build_tree" in syms and "ss_solve_gpu" in syms and "ss_batch_run" in syms
+    assert "ss_leaf_equity" in syms and "ss_leaf_values" in syms
     for s in syms:
         assert hasattr(lib, s), s
 

@@ -1,0 +1,113 @@
+"""GPU parity of the leaf-value producers (include/street_solve_leaf.h).
+
+Every output is compared bit for bit: the float64 equity sums run in the
+reference's order, the bias formula is evaluated without contraction, and the
+float32 rounding is the reference's assignment into a float32 array. Checked
+against the reference's own outputs (tests/golden/leaf_values.npz), the
+oracle at full ranges (1128 turn combos per player for 2, 3 and 6 players;
+1326 listed combos with the board-blocked ones), hand-derived answers, and
+the edge cases (an empty range, no runouts).
+"""
+
+import ctypes
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import ss_abi
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, os.path.join(REPO, "oracle"))
+import leaf_oracle as lo  # noqa: E402
+import leaf_values as lv  # noqa: E402
+from test_leaf import GOLD, NAMES, card, cards, golden_case  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_equity(lib, hands, nh, runouts, mh):
+    npl = len(nh)
+    h = np.zeros((npl, mh, 2), dtype=np.int32)
+    for p in range(npl):
+        for i in range(nh[p]):
+            h[p, i] = hands[p][i][:2]
+    run = np.array(runouts, dtype=np.int32).reshape(-1, 5)
+    raw = np.zeros((npl, mh))
+    ip = ctypes.POINTER(ctypes.c_int)
+    rc = lib.ss_leaf_equity(npl, h.ctypes.data_as(ip), np.array(nh, dtype=np.int32).ctypes.data_as(ip), mh,
+                            run.ctypes.data_as(ip), len(run), raw.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    assert rc == 0, ss_abi.last_error(lib)
+    return raw
+
+
+def turn_hands(rng, board, nh, all_combos=False):
+    free = [c for c in range(52) if all_combos or c not in board]
+    combos = [(a, b) for i, a in enumerate(free) for b in free[i + 1:]]
+    out = []
+    for n in nh:
+        pick = rng.permutation(len(combos))[:n]
+        out.append([combos[k] for k in pick])
+    return out
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_leaf_values_equal_reference(lib, name):
+    kind, board, hands, nh, mh, infos, ref = golden_case(name)
+    print(f"leaf golden {name}", flush=True)
+    lis = [lv.LeafInfo(i, p, ()) for i, p in infos]
+    ph = [[(a, b, 1.0) for a, b in hs] for hs in hands]
+    f = lv.compute_turn_leaf_values if kind == "turn" else lv.compute_flop_leaf_equity
+    out = f(board, lis, mh, 1000, player_hands=ph)
+    assert out.shape == ref.shape
+    assert np.array_equal(out.view(np.uint32), ref.view(np.uint32))
+
+
+def test_known_answers(lib):
+    b = cards("2c 7d 9h Js")
+    raw = gpu_equity(lib, [[(card("Ah"), card("As"))], [(card("Kd"), card("Kc"))]], [1, 1],
+                     lo.river_runouts(b, [1, 1]), 1)
+    assert raw[0, 0] == 42 / 44 and raw[1, 0] == 2 / 44
+    t = cards("Ts Js Qs Ks")
+    h3 = [[(card("2c"), card("3c"))], [(card("2d"), card("3d"))], [(card("2h"), card("3h"))]]
+    raw = gpu_equity(lib, h3, [1, 1, 1], lo.river_runouts(t, [1, 1, 1]), 1)
+    assert np.array_equal(raw, lo.runout_equity(h3, [1, 1, 1], lo.river_runouts(t, [1, 1, 1]), 1))
+    assert np.allclose(raw, 1.0 / 3, rtol=0, atol=1e-14)
+    hc = [[(card("Ah"), card("As"))], [(card("Kd"), card("Kc"))], [(card("Kd"), card("Qc"))]]
+    assert np.all(gpu_equity(lib, hc, [1, 1, 1], lo.river_runouts(b, [1, 1, 1]), 1) == 0.0)
+
+
+@pytest.mark.parametrize("npl,all_combos", [(2, False), (3, False), (6, False), (2, True)])
+def test_full_ranges_equal_oracle(lib, npl, all_combos):
+    rng = np.random.default_rng(7000 + npl)
+    board = cards("Kd 7c 2h 9s")
+    nh = [1326 if all_combos else 1128] * npl
+    hands = turn_hands(rng, board, nh, all_combos)
+    runs = lv.turn_runouts(board, nh)
+    print(f"leaf full ranges np={npl} hands={nh[0]} runouts={len(runs)}", flush=True)
+    raw = gpu_equity(lib, hands, nh, runs, nh[0])
+    ref = lo.runout_equity(hands, nh, runs, nh[0])
+    assert np.array_equal(raw, ref)
+    assert raw.min() >= 0.0 and raw.max() <= 1.0
+    if npl == 3:
+        infos = [(0, 1500)]
+        out = lv._leaf_values([[(a, b, 1.0) for a, b in hs] for hs in hands], runs,
+                              [lv.LeafInfo(0, 1500, ())], nh[0], lib)
+        assert np.array_equal(out, lo.values_from_equity(ref, nh, nh[0], infos))
+
+
+def test_edge_cases(lib):
+    board = cards("Qs As 2d 7h")
+    rng = np.random.default_rng(11)
+    hands = turn_hands(rng, board, [20, 0, 15])
+    nh = [20, 0, 15]
+    runs = lo.river_runouts(board, nh)
+    raw = gpu_equity(lib, hands, nh, runs, 20)
+    assert np.array_equal(raw, lo.runout_equity(hands, nh, runs, 20))
+    assert np.all(raw == 0.0)  # an empty opponent range leaves no tuple for anyone
+    raw = gpu_equity(lib, turn_hands(rng, board, [5, 5]), [5, 5], [], 5)
+    assert np.all(raw == 0.0)
+    out = lv._leaf_values([[(a, b, 1.0) for a, b in hs] for hs in turn_hands(rng, board, [3, 2])],
+                          lo.river_runouts(board, [3, 2]), [], 4, lib)
+    assert out.shape == (0, 2, 4)
