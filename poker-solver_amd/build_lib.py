@@ -4,8 +4,9 @@
 
 Every source is compiled by hipcc for gfx950 with packed-FP32 instruction
 selection disabled, then linked into one shared library exporting the C-ABI
-of include/street_solve.h and include/street_solve_leaf.h (and the test
-entry points of include/street_solve_debug.h).
+of include/street_solve.h, include/street_solve_leaf.h and
+include/street_solve_range.h (and the test entry points of
+include/street_solve_debug.h).
 """
 
 import argparse
@@ -22,7 +23,7 @@ BUILD = os.path.join(PKG, "build")
 OBJ = os.path.join(BUILD, "obj")
 LIB = os.path.join(BUILD, "street_solve.so")
 
-SOURCES = ["ss_tree.cpp", "ss_runtime.hip", "ss_kernels.hip", "ss_precision.hip", "ss_capi.hip", "ss_server.hip", "ss_leaf.hip"]
+SOURCES = ["ss_tree.cpp", "ss_runtime.hip", "ss_kernels.hip", "ss_precision.hip", "ss_capi.hip", "ss_server.hip", "ss_leaf.hip", "ss_range.hip"]
 
 ARCH = os.environ.get("SS_OFFLOAD_ARCH", "gfx950")
 

@@ -147,8 +147,8 @@ class SSServerConfig(ctypes.Structure):
     ]
 
 
-# Every symbol include/street_solve.h and include/street_solve_leaf.h declare,
-# with its ctypes signature.
+# Every symbol include/street_solve.h, street_solve_leaf.h and
+# street_solve_range.h declare, with its ctypes signature.
 _P = ctypes.POINTER
 SIGNATURES = {
     "ss_build_tree": (ctypes.c_int, [_P(ctypes.c_int), ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -180,6 +180,8 @@ SIGNATURES = {
                                       _P(ctypes.c_int), ctypes.c_int, _P(ctypes.c_double)]),
     "ss_leaf_values": (ctypes.c_int, [ctypes.c_int, _P(ctypes.c_double), _P(ctypes.c_int), ctypes.c_int,
                                       _P(ctypes.c_int), _P(ctypes.c_int), ctypes.c_int, _P(ctypes.c_float)]),
+    "ss_range_update": (ctypes.c_int, [ctypes.c_int, _P(ctypes.c_double), _P(ctypes.c_double),
+                                       _P(ctypes.c_ubyte), ctypes.c_double, ctypes.c_double]),
     "ss_last_error": (ctypes.c_char_p, []),
     "ss_version": (ctypes.c_char_p, []),
 }
