@@ -289,7 +289,9 @@ def precision_bench(boards, iters, ref_boards, cores):
 
 # ---------------------------------------------------------------- roofline traffic
 def leaf_bench(reps=5):
-    """Depth-limit leaf values of a 3-player full-range turn subgame
+    """Depth-limit leaf values of a 3-player full-range turn subgame, and of a
+    3-player full-range flop subgame (workloads.flop_leaf_workload, record
+    "flop")
     (workloads.turn_leaf_workload: 1128 combos per player, 12 sampled rivers,
     2000 sampled opponent pairs per hero and river, 28 leaves x 64 bias
     profiles). GPU: wall time of compute_turn_leaf_values (host runout and
@@ -318,12 +320,37 @@ def leaf_bench(reps=5):
     one = lo.values_from_equity(raw, nh, mh, [(0, infos[0].pot)])
     val_ms = (time.perf_counter() - t) * 1000 * len(infos)
     same = bool(np.array_equal(one[:64], out[:64])) if infos[0].leaf_idx == 0 else None
-    return {"workload": "3-player turn, 1128 combos per player, 12 sampled rivers, 2000 sampled opponent "
-                        f"pairs, {len(infos)} leaves x 64 bias profiles",
-            "gpu_ms_per_call": round(gpu_ms, 3), "output_mib": round(out.nbytes / 2**20, 1),
-            "cpu_oracle_ms": round(eq_ms + val_ms, 1), "cpu_oracle_equity_ms": round(eq_ms, 1),
-            "cpu_kind": "port (numpy-vectorised oracle, 1 core; value loop timed on one leaf and scaled)",
-            "speedup_vs_cpu_oracle": round((eq_ms + val_ms) / gpu_ms, 1), "leaf0_equal_oracle": same}
+    rec = {"workload": "3-player turn, 1128 combos per player, 12 sampled rivers, 2000 sampled opponent "
+                       f"pairs, {len(infos)} leaves x 64 bias profiles",
+           "gpu_ms_per_call": round(gpu_ms, 3), "output_mib": round(out.nbytes / 2**20, 1),
+           "cpu_oracle_ms": round(eq_ms + val_ms, 1), "cpu_oracle_equity_ms": round(eq_ms, 1),
+           "cpu_kind": "port (numpy-vectorised oracle, 1 core; value loop timed on one leaf and scaled)",
+           "speedup_vs_cpu_oracle": round((eq_ms + val_ms) / gpu_ms, 1), "leaf0_equal_oracle": same}
+    # Flop: 12 sampled turns x 12 sampled rivers; the oracle's equity is timed
+    # on the first 12 runouts and scaled to all 144.
+    board, infos, hands, mh = workloads.flop_leaf_workload(lib)
+    lv.compute_flop_leaf_equity(board, infos, mh, 1500, player_hands=hands)
+    t = time.perf_counter()
+    for _ in range(reps):
+        out = lv.compute_flop_leaf_equity(board, infos, mh, 1500, player_hands=hands)
+    fgpu = (time.perf_counter() - t) * 1000 / reps
+    nh = [len(h) for h in hands]
+    hc = [[(a, b) for a, b, _ in hs] for hs in hands]
+    runs = lo.flop_runouts(board, nh)
+    t = time.perf_counter()
+    lo.runout_equity(hc, nh, runs[:12], mh)
+    feq = (time.perf_counter() - t) * 1000 * len(runs) / 12
+    t = time.perf_counter()
+    lo.values_from_equity(np.zeros((3, mh)), nh, mh, [(0, infos[0].pot)])
+    fval = (time.perf_counter() - t) * 1000 * len(infos)
+    rec["flop"] = {"workload": f"3-player flop, 1176 combos per player, {len(runs)} sampled runouts (12 turns x "
+                               f"12 rivers), 2000 sampled opponent pairs, {len(infos)} leaves x 64 bias profiles",
+                   "gpu_ms_per_call": round(fgpu, 3), "output_mib": round(out.nbytes / 2**20, 1),
+                   "cpu_oracle_ms": round(feq + fval, 1),
+                   "cpu_kind": "port (numpy-vectorised oracle, 1 core; equity timed on 12 of the runouts and "
+                               "the value loop on one leaf, both scaled)",
+                   "speedup_vs_cpu_oracle": round((feq + fval) / fgpu, 1)}
+    return rec
 
 
 def traffic_per_launch(algo_per_launch, workload):

@@ -182,3 +182,25 @@ def turn_leaf_workload(lib):
         lib.ss_free_tree(ctypes.byref(td))
     combos = [(int(a), int(b), 1.0) for a, b in all_combos(board)]
     return board, infos, [combos] * 3, len(combos)
+
+
+def flop_leaf_workload(lib):
+    """Leaf values of one 3-player flop subgame at full ranges: flop Kd 7c 2h,
+    every player holding all 1176 unblocked combos, pot 1500, stack 5000, one
+    bet size (0.75 pot) with continuation leaves. Returns (board, leaf_infos,
+    player_hands, max_hands)."""
+    import ctypes
+    import ss_abi
+    import leaf_values as lv
+    board = [45, 20, 2]
+    td = ss_abi.SSTreeData()
+    rc = lib.ss_build_tree((ctypes.c_int * 3)(*board), 3, 1500, 5000, (ctypes.c_float * 1)(0.75), 1, 3,
+                           (ctypes.c_int * 3)(0, 1, 2), 0, 1, ctypes.byref(td))
+    if rc != 0:
+        raise RuntimeError(f"ss_build_tree failed: {ss_abi.last_error(lib)}")
+    try:
+        infos = lv.extract_leaf_info_from_tree(td, num_players=3)
+    finally:
+        lib.ss_free_tree(ctypes.byref(td))
+    combos = [(int(a), int(b), 1.0) for a, b in all_combos(board)]
+    return board, infos, [combos] * 3, len(combos)

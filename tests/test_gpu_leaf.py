@@ -5,8 +5,9 @@ reference's order, the bias formula is evaluated without contraction, and the
 float32 rounding is the reference's assignment into a float32 array. Checked
 against the reference's own outputs (tests/golden/leaf_values.npz), the
 oracle at full ranges (1128 turn combos per player for 2, 3 and 6 players;
-1326 listed combos with the board-blocked ones), hand-derived answers, and
-the edge cases (an empty range, no runouts).
+1326 listed combos with the board-blocked ones; a 3-player flop over its 144
+sampled runouts), hand-derived answers, and the edge cases (an empty range,
+no runouts).
 """
 
 import ctypes
@@ -111,3 +112,15 @@ def test_edge_cases(lib):
     out = lv._leaf_values([[(a, b, 1.0) for a, b in hs] for hs in turn_hands(rng, board, [3, 2])],
                           lo.river_runouts(board, [3, 2]), [], 4, lib)
     assert out.shape == (0, 2, 4)
+
+
+def test_flop_sampled_runouts_equal_oracle(lib):
+    rng = np.random.default_rng(8100)
+    board = cards("Kd 7c 2h")
+    nh = [300, 300, 300]
+    hands = turn_hands(rng, board, nh)
+    runs = lv.flop_runouts(board, nh)
+    assert len(runs) == 144
+    print(f"leaf flop np=3 hands=300 runouts={len(runs)}", flush=True)
+    raw = gpu_equity(lib, hands, nh, runs, 300)
+    assert np.array_equal(raw, lo.runout_equity(hands, nh, runs, 300))
