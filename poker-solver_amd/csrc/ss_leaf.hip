@@ -39,6 +39,7 @@
 #include <algorithm>
 #include <climits>
 #include <map>
+#include <mutex>
 #include <vector>
 
 #include "ss_eval.h"
@@ -115,28 +116,48 @@ struct TupleRec {
     double unused;
 };
 
-struct LeafDev {
-    uint16_t *hc = nullptr;      // [np][mh] c0 | c1 << 8
-    int *runouts = nullptr;      // [r][5]
-    uint32_t *str = nullptr;     // [r][np][mh], 0 = blocked / no hand
-    int16_t *valid = nullptr;    // [r][np][mh] unblocked hand indices, in order
-    RpDesc *desc = nullptr;      // [r][np]
-    int16_t *pool = nullptr;     // draws, [tuples][np-1] per pattern
-    double *eq = nullptr;        // [r][np][mh]
-    uint8_t *cnt = nullptr;      // [r][np][mh] hand counted on this runout
-    double *raw = nullptr;       // [np][mh]
-    TupleRec *recs = nullptr;    // [r][np][SS_LEAF_OPP_COMBO_LIMIT]
-    ~LeafDev() {
-        void *p[] = {hc, runouts, str, valid, desc, pool, eq, cnt, raw, recs};
-        for (void *q : p)
-            if (q) (void)hipFree(q);
-    }
+// Device buffers of the leaf calls, per device: grown to the largest call so
+// far and kept until the process ends (no allocation on repeated calls of a
+// size seen before). The calls are synchronous; the mutex serialises callers.
+enum LeafSlot { L_HC, L_RUN, L_STR, L_VALID, L_DESC, L_POOL, L_EQ, L_CNT, L_RAW, L_RECS,
+                V_RAW, V_POT, V_NH, V_OUT, L_NSLOTS };
+
+struct LeafArena {
+    std::mutex mu;
+    void *p[L_NSLOTS] = {};
+    size_t cap[L_NSLOTS] = {};
 };
 
+constexpr int LEAF_MAX_DEVICES = 16;
+
+int leaf_arena(LeafArena **out) {
+    static LeafArena arenas[LEAF_MAX_DEVICES];
+    int dev = 0;
+    SS_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= LEAF_MAX_DEVICES) return set_error(SS_ERR_UNSUPPORTED, "device %d", dev);
+    *out = &arenas[dev];
+    return SS_OK;
+}
+
 template <class T>
-int dev_upload(T **d, const T *h, size_t n) {
-    SS_HIP(hipMalloc((void **)d, std::max<size_t>(n, 1) * sizeof(T)));
-    if (n) SS_HIP(hipMemcpy(*d, h, n * sizeof(T), hipMemcpyHostToDevice));
+int arena_buf(LeafArena &a, int slot, size_t n, T **out) {
+    size_t b = std::max<size_t>(n, 1) * sizeof(T);
+    if (a.cap[slot] < b) {
+        if (a.p[slot]) (void)hipFree(a.p[slot]);
+        a.p[slot] = nullptr;
+        a.cap[slot] = 0;
+        SS_HIP(hipMalloc(&a.p[slot], b));
+        a.cap[slot] = b;
+    }
+    *out = static_cast<T *>(a.p[slot]);
+    return SS_OK;
+}
+
+template <class T>
+int arena_upload(LeafArena &a, int slot, const T *h, size_t n, T **out) {
+    int rc = arena_buf(a, slot, n, out);
+    if (rc) return rc;
+    if (n) SS_HIP(hipMemcpy(*out, h, n * sizeof(T), hipMemcpyHostToDevice));
     return SS_OK;
 }
 
@@ -265,18 +286,24 @@ __global__ __launch_bounds__(EQ_BLOCK) void k_leaf_equity(const uint16_t *__rest
     cnt[o] = nvalid ? 1 : 0;
 }
 
+// Per hand, the runout average in runout order. Loads are unconditional and
+// an uncounted runout adds 0.0 (exact), so eight runouts' loads go out
+// together.
 __global__ __launch_bounds__(LEAF_BLOCK) void k_leaf_reduce(const double *__restrict__ eq,
                                                             const uint8_t *__restrict__ cnt,
                                                             double *__restrict__ raw, int nrun, int n) {
     int i = blockIdx.x * LEAF_BLOCK + threadIdx.x;
     if (i >= n) return;
-    double s = 0.0, c = 0.0;
-    for (int r = 0; r < nrun; ++r)
-        if (cnt[(size_t)r * n + i]) {
-            s += eq[(size_t)r * n + i];
-            c += 1.0;
-        }
-    raw[i] = c > 0.0 ? s / c : 0.0;
+    double s = 0.0;
+    int c = 0;
+#pragma unroll 8
+    for (int r = 0; r < nrun; ++r) {
+        const double e = eq[(size_t)r * n + i];
+        const int k = cnt[(size_t)r * n + i];
+        s += k ? e : 0.0;
+        c += k;
+    }
+    raw[i] = c > 0 ? s / (double)c : 0.0;
 }
 
 __device__ __forceinline__ double biased_value(double equity, double pot, int s_self, int s_opp) {
@@ -405,30 +432,39 @@ int leaf_equity(int np, const int *hands, const int *nh, int mh, const int *runo
         std::fill(raw, raw + (size_t)np * mh, 0.0);
         return SS_OK;
     }
-    LeafDev D;
+    LeafArena *A = nullptr;
+    if ((rc = leaf_arena(&A))) return rc;
+    std::lock_guard<std::mutex> lock(A->mu);
     size_t nrp = (size_t)nrun * np, nall = nrp * mh;
-    if ((rc = dev_upload(&D.hc, hc.data(), hc.size()))) return rc;
-    if ((rc = dev_upload(&D.runouts, runouts, (size_t)nrun * 5))) return rc;
-    if ((rc = dev_upload(&D.valid, valid.data(), valid.size()))) return rc;
-    if ((rc = dev_upload(&D.desc, desc.data(), desc.size()))) return rc;
-    if ((rc = dev_upload(&D.pool, pool.data(), pool.size()))) return rc;
-    SS_HIP(hipMalloc((void **)&D.str, nall * sizeof(uint32_t)));
-    SS_HIP(hipMalloc((void **)&D.eq, nall * sizeof(double)));
-    SS_HIP(hipMalloc((void **)&D.cnt, nall));
-    SS_HIP(hipMalloc((void **)&D.raw, (size_t)np * mh * sizeof(double)));
-    SS_HIP(hipMalloc((void **)&D.recs, nrp * SS_LEAF_OPP_COMBO_LIMIT * sizeof(TupleRec)));
-    k_leaf_strength<<<(unsigned)((nall + LEAF_BLOCK - 1) / LEAF_BLOCK), LEAF_BLOCK>>>(D.hc, D.runouts, D.str, np, mh,
+    uint16_t *d_hc;
+    int *d_run;
+    int16_t *d_valid, *d_pool;
+    RpDesc *d_desc;
+    uint32_t *d_str;
+    double *d_eq, *d_raw;
+    uint8_t *d_cnt;
+    TupleRec *d_recs;
+    if ((rc = arena_upload(*A, L_HC, hc.data(), hc.size(), &d_hc)) ||
+        (rc = arena_upload(*A, L_RUN, runouts, (size_t)nrun * 5, &d_run)) ||
+        (rc = arena_upload(*A, L_VALID, valid.data(), valid.size(), &d_valid)) ||
+        (rc = arena_upload(*A, L_DESC, desc.data(), desc.size(), &d_desc)) ||
+        (rc = arena_upload(*A, L_POOL, pool.data(), pool.size(), &d_pool)) ||
+        (rc = arena_buf(*A, L_STR, nall, &d_str)) || (rc = arena_buf(*A, L_EQ, nall, &d_eq)) ||
+        (rc = arena_buf(*A, L_CNT, nall, &d_cnt)) || (rc = arena_buf(*A, L_RAW, (size_t)np * mh, &d_raw)) ||
+        (rc = arena_buf(*A, L_RECS, nrp * SS_LEAF_OPP_COMBO_LIMIT, &d_recs)))
+        return rc;
+    k_leaf_strength<<<(unsigned)((nall + LEAF_BLOCK - 1) / LEAF_BLOCK), LEAF_BLOCK>>>(d_hc, d_run, d_str, np, mh,
                                                                                      (int)nall);
     SS_HIP(hipGetLastError());
     k_leaf_tuples<<<dim3((SS_LEAF_OPP_COMBO_LIMIT + LEAF_BLOCK - 1) / LEAF_BLOCK, (unsigned)nrp), LEAF_BLOCK>>>(
-        D.hc, D.str, D.valid, D.desc, D.pool, D.recs, np, mh);
+        d_hc, d_str, d_valid, d_desc, d_pool, d_recs, np, mh);
     SS_HIP(hipGetLastError());
-    k_leaf_equity<<<dim3((mh + EQ_BLOCK - 1) / EQ_BLOCK, (unsigned)nrp), EQ_BLOCK>>>(D.hc, D.str, D.desc, D.recs,
-                                                                                  D.eq, D.cnt, np, mh);
+    k_leaf_equity<<<dim3((mh + EQ_BLOCK - 1) / EQ_BLOCK, (unsigned)nrp), EQ_BLOCK>>>(d_hc, d_str, d_desc, d_recs,
+                                                                                  d_eq, d_cnt, np, mh);
     SS_HIP(hipGetLastError());
-    k_leaf_reduce<<<(np * mh + LEAF_BLOCK - 1) / LEAF_BLOCK, LEAF_BLOCK>>>(D.eq, D.cnt, D.raw, nrun, np * mh);
+    k_leaf_reduce<<<(np * mh + LEAF_BLOCK - 1) / LEAF_BLOCK, LEAF_BLOCK>>>(d_eq, d_cnt, d_raw, nrun, np * mh);
     SS_HIP(hipGetLastError());
-    SS_HIP(hipMemcpy(raw, D.raw, (size_t)np * mh * sizeof(double), hipMemcpyDeviceToHost));
+    SS_HIP(hipMemcpy(raw, d_raw, (size_t)np * mh * sizeof(double), hipMemcpyDeviceToHost));
     return SS_OK;
 }
 
@@ -453,30 +489,22 @@ int leaf_values(int np, const double *raw, const int *nh, int mh, const int *lea
     }
     const long long total = nleaves * np * mh;
     if (total == 0) return SS_OK;
-    double *d_raw = nullptr;
-    long long *d_pot = nullptr;
-    int *d_nh = nullptr;
-    float *d_out = nullptr;
-    int rc = SS_OK;
-    auto cleanup = [&]() {
-        if (d_raw) (void)hipFree(d_raw);
-        if (d_pot) (void)hipFree(d_pot);
-        if (d_nh) (void)hipFree(d_nh);
-        if (d_out) (void)hipFree(d_out);
-    };
-    if ((rc = dev_upload(&d_raw, raw, (size_t)np * mh)) || (rc = dev_upload(&d_pot, flat.data(), (size_t)nleaves)) ||
-        (rc = dev_upload(&d_nh, nh, (size_t)np))) {
-        cleanup();
+    LeafArena *A = nullptr;
+    int rc = leaf_arena(&A);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lock(A->mu);
+    double *d_raw;
+    long long *d_pot;
+    int *d_nh;
+    float *d_out;
+    if ((rc = arena_upload(*A, V_RAW, raw, (size_t)np * mh, &d_raw)) ||
+        (rc = arena_upload(*A, V_POT, flat.data(), (size_t)nleaves, &d_pot)) ||
+        (rc = arena_upload(*A, V_NH, nh, (size_t)np, &d_nh)) || (rc = arena_buf(*A, V_OUT, (size_t)total, &d_out)))
         return rc;
-    }
-    hipError_t e = hipMalloc((void **)&d_out, (size_t)total * sizeof(float));
-    if (e == hipSuccess) {
-        k_leaf_bias<<<dim3((unsigned)nleaves, (np * mh + LEAF_BLOCK - 1) / LEAF_BLOCK), LEAF_BLOCK>>>(
-            d_raw, d_pot, d_nh, d_out, np, mh);
-        e = hipGetLastError();
-    }
+    k_leaf_bias<<<dim3((unsigned)nleaves, (np * mh + LEAF_BLOCK - 1) / LEAF_BLOCK), LEAF_BLOCK>>>(d_raw, d_pot, d_nh,
+                                                                                               d_out, np, mh);
+    hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpy(out, d_out, (size_t)total * sizeof(float), hipMemcpyDeviceToHost);
-    cleanup();
     if (e != hipSuccess) return set_error(SS_ERR_HIP, "leaf values: %s", hipGetErrorString(e));
     return SS_OK;
 }
