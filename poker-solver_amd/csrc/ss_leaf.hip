@@ -104,6 +104,7 @@ struct PyRandom {
 struct RpDesc {
     int count;    // tuples to walk (0: hero skipped)
     int pattern;  // offset into the draw pool (sampled), -1 for the product
+    long long rec;  // first record of this (runout, hero) in the packed record array
     int lens[SS_MAX_PLAYERS - 1];
 };
 
@@ -225,7 +226,7 @@ __global__ __launch_bounds__(LEAF_BLOCK) void k_leaf_tuples(const uint16_t *__re
     }
     if (__builtin_popcountll(m) != 2 * no) m = ~0ull;  // every hero hand collides with it
     const double tie = mult == 1 ? 1.0 / 2 : mult == 2 ? 1.0 / 3 : mult == 3 ? 1.0 / 4 : mult == 4 ? 1.0 / 5 : 1.0 / 6;
-    recs[(size_t)rp * SS_LEAF_OPP_COMBO_LIMIT + t] =
+    recs[d.rec + t] =
         TupleRec{(uint32_t)m, (uint32_t)(m >> 32), best, mult, tie, 0.0};
 }
 
@@ -256,7 +257,7 @@ __global__ __launch_bounds__(EQ_BLOCK) void k_leaf_equity(const uint16_t *__rest
     const int rp = blockIdx.y, r = rp / np, p = rp % np;
     const int count = desc[rp].count;
     {
-        const uint4 *src = reinterpret_cast<const uint4 *>(recs + (size_t)rp * SS_LEAF_OPP_COMBO_LIMIT);
+        const uint4 *src = reinterpret_cast<const uint4 *>(recs + desc[rp].rec);
         uint4 *dst = reinterpret_cast<uint4 *>(srec);
         const int n16 = count * (int)(sizeof(TupleRec) / sizeof(uint4));
         for (int i = threadIdx.x; i < n16; i += EQ_BLOCK) dst[i] = src[i];
@@ -367,6 +368,8 @@ int leaf_equity(int np, const int *hands, const int *nh, int mh, const int *runo
     if (rc) return rc;
     if (nrun < 0 || (nrun > 0 && !runouts)) return set_error(SS_ERR_ARG, "bad runouts");
     if (!raw) return set_error(SS_ERR_ARG, "raw_equity is NULL");
+    if ((long long)nrun * np > 65535)
+        return set_error(SS_ERR_ARG, "%d runouts x %d players exceeds 65535 (runout, player) pairs", nrun, np);
     for (int r = 0; r < nrun; ++r) {
         uint64_t m = 0;
         for (int k = 0; k < 5; ++k) {
@@ -400,7 +403,7 @@ int leaf_equity(int np, const int *hands, const int *nh, int mh, const int *runo
         }
         for (int p = 0; p < np; ++p) {
             RpDesc &d = desc[r * np + p];
-            d = RpDesc{0, -1, {0, 0, 0, 0, 0}};
+            d = RpDesc{0, -1, 0, {0, 0, 0, 0, 0}};
             if (!vlen[r * np + p]) continue;
             std::vector<int> lens;
             long long total = 1;
@@ -428,6 +431,11 @@ int leaf_equity(int np, const int *hands, const int *nh, int mh, const int *runo
         }
     }
 
+    long long nrec = 0;
+    for (RpDesc &d : desc) {
+        d.rec = nrec;
+        nrec += d.count;
+    }
     if (nrun == 0) {
         std::fill(raw, raw + (size_t)np * mh, 0.0);
         return SS_OK;
@@ -451,7 +459,7 @@ int leaf_equity(int np, const int *hands, const int *nh, int mh, const int *runo
         (rc = arena_upload(*A, L_POOL, pool.data(), pool.size(), &d_pool)) ||
         (rc = arena_buf(*A, L_STR, nall, &d_str)) || (rc = arena_buf(*A, L_EQ, nall, &d_eq)) ||
         (rc = arena_buf(*A, L_CNT, nall, &d_cnt)) || (rc = arena_buf(*A, L_RAW, (size_t)np * mh, &d_raw)) ||
-        (rc = arena_buf(*A, L_RECS, nrp * SS_LEAF_OPP_COMBO_LIMIT, &d_recs)))
+        (rc = arena_buf(*A, L_RECS, (size_t)nrec, &d_recs)))
         return rc;
     k_leaf_strength<<<(unsigned)((nall + LEAF_BLOCK - 1) / LEAF_BLOCK), LEAF_BLOCK>>>(d_hc, d_run, d_str, np, mh,
                                                                                      (int)nall);

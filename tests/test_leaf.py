@@ -138,7 +138,6 @@ def test_runout_lists_equal_oracle():
 
 
 def test_leaf_arguments_rejected_without_gpu(lib):
-    ip, dp = ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)
     hands = (ctypes.c_int * 4)(0, 1, 2, 3)
     run = (ctypes.c_int * 5)(10, 11, 12, 13, 14)
     raw = (ctypes.c_double * 2)()
@@ -146,6 +145,9 @@ def test_leaf_arguments_rejected_without_gpu(lib):
                             (2, (1, 1), 1, (ctypes.c_int * 5)(10, 11, 12, 13, 13))]:
         nha = (ctypes.c_int * 6)(*nh)
         assert lib.ss_leaf_equity(npl, hands, nha, mh, rr, 1, raw) == -2
+    # (runout, player) pairs are a grid dimension: more than 65535 is refused up front
+    assert lib.ss_leaf_equity(2, hands, (ctypes.c_int * 2)(1, 1), 1, run, 32768, raw) == -2
+    assert "65535" in ss_abi.last_error(lib)
     bad = (ctypes.c_int * 4)(0, 0, 2, 3)
     assert lib.ss_leaf_equity(2, bad, (ctypes.c_int * 2)(1, 1), 1, run, 1, raw) == -2
     assert "cards" in ss_abi.last_error(lib)
