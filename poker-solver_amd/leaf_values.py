@@ -26,6 +26,7 @@ per-action EVs and is not provided.
 """
 
 import ctypes
+import functools
 from collections import namedtuple
 from typing import List, Optional, Tuple
 
@@ -124,10 +125,16 @@ def _players(player_hands, oop_hands, ip_hands, fname):
     return player_hands
 
 
+@functools.lru_cache(maxsize=4096)
+def _drawn(cards, seed):
+    # The draw depends only on the ordered card list and the seed.
+    return tuple(int(c) for c in sorted(np.random.RandomState(seed).choice(list(cards), MAX_RUNOUT_SAMPLES,
+                                                                            replace=False)))
+
+
 def _sample(cards, seed, large):
     if len(cards) > MAX_RUNOUT_SAMPLES and large:
-        return [int(c) for c in sorted(np.random.RandomState(seed).choice(cards, MAX_RUNOUT_SAMPLES,
-                                                                           replace=False))]
+        return list(_drawn(tuple(cards), seed))
     return list(cards)
 
 
@@ -140,7 +147,11 @@ def _leaf_values(player_hands, runouts, leaf_infos, max_hands, lib=None):
     hands = np.zeros((npl, max_hands, 2), dtype=np.int32)
     for p, hs in enumerate(player_hands):
         if hs:
-            hands[p, :len(hs)] = np.array([(hand[0], hand[1]) for hand in hs], dtype=np.int32)
+            try:
+                rows = np.asarray(hs, dtype=np.float64)[:, :2]
+            except (ValueError, TypeError):
+                rows = [(hand[0], hand[1]) for hand in hs]
+            hands[p, :len(hs)] = rows
     nh_a = np.array(nh, dtype=np.int32)
     run = np.ascontiguousarray(np.array(runouts, dtype=np.int32).reshape(-1, 5))
     raw = np.zeros((npl, max_hands), dtype=np.float64)
@@ -151,7 +162,9 @@ def _leaf_values(player_hands, runouts, leaf_infos, max_hands, lib=None):
         raise RuntimeError(f"ss_leaf_equity failed ({rc}): {ss_abi.last_error(lib)}")
     idx = np.array([li.leaf_idx for li in leaf_infos], dtype=np.int32)
     pot = np.array([li.pot for li in leaf_infos], dtype=np.int32)
-    out = np.zeros((len(leaf_infos) * 4 ** npl, npl, max_hands), dtype=np.float32)
+    # Every element is written by the library (rows no leaf info names and
+    # hands past a player's count as 0).
+    out = np.empty((len(leaf_infos) * 4 ** npl, npl, max_hands), dtype=np.float32)
     rc = lib.ss_leaf_values(npl, raw.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), nh_a.ctypes.data_as(ip),
                             max_hands, idx.ctypes.data_as(ip), pot.ctypes.data_as(ip), len(leaf_infos),
                             out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
