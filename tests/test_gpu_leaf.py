@@ -6,8 +6,9 @@ float32 rounding is the reference's assignment into a float32 array. Checked
 against the reference's own outputs (tests/golden/leaf_values.npz), the
 oracle at full ranges (1128 turn combos per player for 2, 3 and 6 players;
 1326 listed combos with the board-blocked ones; a 3-player flop over its 144
-sampled runouts), hand-derived answers, and the edge cases (an empty range,
-no runouts).
+sampled runouts), hand-derived answers, the edge cases (an empty range,
+no runouts), and 24 seeded random cases (turn 2-6 players, flop 2-3 players,
+range sizes on both sides of the sampling thresholds, random leaf infos).
 """
 
 import ctypes
@@ -124,3 +125,36 @@ def test_flop_sampled_runouts_equal_oracle(lib):
     print(f"leaf flop np=3 hands=300 runouts={len(runs)}", flush=True)
     raw = gpu_equity(lib, hands, nh, runs, 300)
     assert np.array_equal(raw, lo.runout_equity(hands, nh, runs, 300))
+
+
+def _random_case(seed):
+    rng = np.random.default_rng(9000 + seed)
+    npl = int(rng.integers(2, 7))
+    flop = bool(rng.integers(0, 2)) and npl <= 3
+    board = [int(c) for c in rng.choice(52, 3 if flop else 4, replace=False)]
+    if npl >= 5 or flop:
+        sizes = [int(rng.integers(1, 12)) for _ in range(npl)]
+        if not flop and rng.integers(0, 2):
+            sizes[0] = 51 + int(rng.integers(0, 20))  # > 50 hands: 12 sampled rivers
+    else:
+        sizes = [int(rng.integers(0, 90)) for _ in range(npl)]
+    hands = turn_hands(rng, board, sizes)
+    mh = max(max(sizes), 1) + int(rng.integers(0, 3))
+    infos = [(int(rng.integers(0, 4)), int(rng.integers(100, 5000))) for _ in range(int(rng.integers(1, 4)))]
+    return flop, board, hands, sizes, mh, infos
+
+
+@pytest.mark.parametrize("seed", range(24))
+def test_random_cases_equal_oracle(lib, seed):
+    flop, board, hands, nh, mh, infos = _random_case(seed)
+    print(f"leaf random seed={seed} flop={flop} nh={nh} max_hands={mh} infos={infos}", flush=True)
+    ph = [[(a, b, 1.0) for a, b in hs] for hs in hands]
+    lis = [lv.LeafInfo(i, p, ()) for i, p in infos]
+    if flop:
+        out = lv.compute_flop_leaf_equity(board, lis, mh, 1000, player_hands=ph)
+        ref = lo.flop_leaf_equity(board, hands, nh, mh, infos)
+    else:
+        out = lv.compute_turn_leaf_values(board, lis, mh, 1000, player_hands=ph)
+        ref = lo.turn_leaf_values(board, hands, nh, mh, infos)
+    assert out.shape == ref.shape
+    assert np.array_equal(out.view(np.uint32), ref.view(np.uint32))
