@@ -16,7 +16,8 @@
  * Cards are 0..51 (rank * 4 + suit). Return 0 on success, a negative
  * SS_ERR_* code otherwise (message in ss_last_error()). Both calls are
  * synchronous and take host buffers; calls from several threads are
- * serialised. The device buffers they use are kept per device between calls
+ * serialised. The device buffers they use, and a pinned host buffer that
+ * large results are read back through, are kept per device between calls
  * (grown to the largest call so far) until the process ends.
  */
 #ifndef STREET_SOLVE_LEAF_H

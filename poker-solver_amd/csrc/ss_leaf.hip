@@ -39,7 +39,9 @@
 #include <algorithm>
 #include <climits>
 #include <map>
+#include <cstring>
 #include <mutex>
+#include <thread>
 #include <vector>
 
 #include "ss_eval.h"
@@ -127,7 +129,38 @@ struct LeafArena {
     std::mutex mu;
     void *p[L_NSLOTS] = {};
     size_t cap[L_NSLOTS] = {};
+    void *pinned = nullptr;  // host staging for large read-backs (hipHostMalloc)
+    size_t pinned_cap = 0;
 };
+
+// Large results are read back into pinned staging at full link speed and
+// copied to the caller's (pageable) buffer by several host threads.
+constexpr size_t PINNED_MIN_BYTES = 4u << 20;
+constexpr int COPY_THREADS = 8;
+
+int read_back(LeafArena &a, void *dst, const void *src, size_t bytes) {
+    if (bytes < PINNED_MIN_BYTES) {
+        SS_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+        return SS_OK;
+    }
+    if (a.pinned_cap < bytes) {
+        if (a.pinned) (void)hipHostFree(a.pinned);
+        a.pinned = nullptr;
+        a.pinned_cap = 0;
+        SS_HIP(hipHostMalloc(&a.pinned, bytes, hipHostMallocDefault));
+        a.pinned_cap = bytes;
+    }
+    SS_HIP(hipMemcpy(a.pinned, src, bytes, hipMemcpyDeviceToHost));
+    std::vector<std::thread> th;
+    const size_t chunk = (bytes + COPY_THREADS - 1) / COPY_THREADS;
+    for (int t = 0; t < COPY_THREADS; ++t) {
+        size_t b = (size_t)t * chunk, e = std::min(bytes, b + chunk);
+        if (b >= e) break;
+        th.emplace_back([=, &a]() { std::memcpy((char *)dst + b, (const char *)a.pinned + b, e - b); });
+    }
+    for (auto &x : th) x.join();
+    return SS_OK;
+}
 
 constexpr int LEAF_MAX_DEVICES = 16;
 
@@ -512,9 +545,8 @@ int leaf_values(int np, const double *raw, const int *nh, int mh, const int *lea
     k_leaf_bias<<<dim3((unsigned)nleaves, (np * mh + LEAF_BLOCK - 1) / LEAF_BLOCK), LEAF_BLOCK>>>(d_raw, d_pot, d_nh,
                                                                                                d_out, np, mh);
     hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = hipMemcpy(out, d_out, (size_t)total * sizeof(float), hipMemcpyDeviceToHost);
     if (e != hipSuccess) return set_error(SS_ERR_HIP, "leaf values: %s", hipGetErrorString(e));
-    return SS_OK;
+    return read_back(*A, out, d_out, (size_t)total * sizeof(float));
 }
 
 int py_choice_draws(int seed, const int *lens, int nlists, int count, int *out) {
