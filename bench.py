@@ -366,6 +366,110 @@ def traffic_per_launch(algo_per_launch, workload):
     return int(d["traffic_over_algo"] * algo_per_launch)
 
 
+# ---------------------------------------------------------------- profiler workloads
+def profile_traffic(a, subgames=256, iters=20, copy_mib=256):
+    """For `rocprofv3 --pmc FETCH_SIZE` / `--pmc WRITE_SIZE`: one plain device
+    copy of a known size (ssdbg_copy, kernel k_copy: the dispatch that
+    calibrates a byte counter), then `iters` CFR iterations of the mixed batch
+    with every subgame live (no exploitability checks). HBM bytes per
+    iteration = sum over the CFR dispatches of counter (KiB) * 1024 *
+    (copy bytes / the copy dispatch's reading of the counter) / iterations."""
+    import ctypes
+    import ss_abi
+    import ss_api
+    import workloads as wl
+    lib = ss_abi.load_library()
+    lib.ssdbg_copy.restype = ctypes.c_int
+    lib.ssdbg_copy.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
+    ms = ctypes.c_float()
+    copy_bytes = copy_mib << 20
+    assert lib.ssdbg_copy(copy_bytes, 1, ctypes.byref(ms)) == 0, ss_abi.last_error()
+    probs = [ss_api.Problem.from_spot(s) for s in wl.batch_mix(subgames, max_players=3)]
+    algo = sum(ss_api.plan_bytes(p)[0] for p in probs)
+    b = ss_api.Batch(probs, iters, flags=ss_abi.SS_FLAG_NO_EXPL)
+    reps, st = b.run()
+    b.close()
+    print(json.dumps({"copy_bytes": copy_bytes, "copy_ms": ms.value, "iterations": iters,
+                      "algo_bytes_per_iteration": algo, "cfr_ms": st.cfr_ms, "mode": st.mode}), flush=True)
+    return 0
+
+
+def profile_precision(a, boards=64, iters=50):
+    """64 two-player 1081-combo rivers (pot 100 BB, stacks 400 BB, sizes
+    {0.5, 1}) for 50 iterations in one batched precision-form solve (BF16
+    hi/lo MFMA showdown), then one of them in the plain precision form."""
+    import ss_abi
+    import ss_api
+    import workloads as wl
+    probs = [ss_api.Problem.from_spot(wl.river_2p(seed=20261203 + 1000 + b)) for b in range(boards)]
+    b = ss_api.Batch(probs, iters, flags=ss_abi.SS_FLAG_NO_EXPL, precision=ss_abi.SS_PREC_BF16X2)
+    reps, st = b.run()
+    print(f"batched: {st.cfr_ms:.2f} ms for {boards} boards x {iters} iterations", flush=True)
+    b.close()
+    r = ss_api.solve_plain(probs[0], iters, precision=ss_abi.SS_PREC_BF16X2, flags=ss_abi.SS_FLAG_NO_EXPL)
+    print(f"plain: {r.kernel_ms:.2f} ms for 1 board x {iters} iterations", flush=True)
+    return 0
+
+
+def profile_served(a):
+    """The served workload of the bench (Poisson stream, 70% river / 30%
+    turn, 2-3 players), 60 requests at 40/s."""
+    r = served_bench(60, 40.0, 20261203 + 5000, 0.5, 25, 2000, 5, log_every=20)
+    print(r["solves_per_s"], r["latency_ms"], flush=True)
+    return 0
+
+
+def _profile_leaf(flop, reps=10):
+    """`reps` calls of the turn (or flop) leaf-value producer on the 3-player
+    full-range workload, then the two library calls alone on prepared
+    arrays, each with its wall time per call."""
+    import ctypes
+    import leaf_values as lv
+    import ss_abi
+    import workloads
+    lib = ss_abi.load_library()
+    if flop:
+        board, infos, hands, mh = workloads.flop_leaf_workload(lib)
+        produce, runouts = lv.compute_flop_leaf_equity, lv.flop_runouts
+    else:
+        board, infos, hands, mh = workloads.turn_leaf_workload(lib)
+        produce, runouts = lv.compute_turn_leaf_values, lv.turn_runouts
+    print(f"{'flop' if flop else 'turn'} 3p leaves: {len(infos)} leaf infos, {mh} hands per player", flush=True)
+    produce(board, infos, mh, 1500, player_hands=hands)
+    t = time.perf_counter()
+    for _ in range(reps):
+        out = produce(board, infos, mh, 1500, player_hands=hands)
+    print(f"{(time.perf_counter() - t) * 1000 / reps:.3f} ms per call, output {out.shape}", flush=True)
+    nh = np.array([len(h) for h in hands], dtype=np.int32)
+    hc = np.array([[(x, y) for x, y, _ in hs] for hs in hands], dtype=np.int32)
+    run = np.array(runouts(board, list(nh)), dtype=np.int32)
+    raw = np.zeros((len(hands), mh))
+    ip, dp = ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)
+    idx = np.array([li.leaf_idx for li in infos], dtype=np.int32)
+    pot = np.array([li.pot for li in infos], dtype=np.int32)
+    t = time.perf_counter()
+    for _ in range(reps):
+        lib.ss_leaf_equity(len(hands), hc.ctypes.data_as(ip), nh.ctypes.data_as(ip), mh, run.ctypes.data_as(ip),
+                           len(run), raw.ctypes.data_as(dp))
+    eq_ms = (time.perf_counter() - t) * 1000 / reps
+    t = time.perf_counter()
+    for _ in range(reps):
+        lib.ss_leaf_values(len(hands), raw.ctypes.data_as(dp), nh.ctypes.data_as(ip), mh, idx.ctypes.data_as(ip),
+                           pot.ctypes.data_as(ip), len(infos), out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+    val_ms = (time.perf_counter() - t) * 1000 / reps
+    print(f"ss_leaf_equity {eq_ms:.3f} ms, ss_leaf_values {val_ms:.3f} ms per call", flush=True)
+    return 0
+
+
+PROFILE_TARGETS = {
+    "traffic": profile_traffic,
+    "precision": profile_precision,
+    "served": profile_served,
+    "leaf": lambda a: _profile_leaf(False),
+    "leaf-flop": lambda a: _profile_leaf(True),
+}
+
+
 # ---------------------------------------------------------------- main
 def main():
     ap = argparse.ArgumentParser()
@@ -390,7 +494,14 @@ def main():
     ap.add_argument("--precision-iters", type=int, default=1000)
     ap.add_argument("--precision-ref-boards", type=int, default=8, help="boards compared with the FP64 oracle")
     ap.add_argument("--no-leaf", action="store_true", help="skip the turn leaf-value producer run")
+    ap.add_argument("--fresh-steps", type=int, default=2,
+                    help="steps of the setup-inclusive rate (fresh subgames each step); 0 skips it")
+    ap.add_argument("--profile-target", default=None, choices=sorted(PROFILE_TARGETS),
+                    help="run only one workload for a profiler (rocprofv3) and exit")
     a = ap.parse_args()
+
+    if a.profile_target:
+        return PROFILE_TARGETS[a.profile_target](a)
 
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: subgames are independent and shard across GPUs by request "
@@ -421,6 +532,29 @@ def main():
     ms_per_step = (t1 - t0) * 1000.0 / a.steps
     value = len(problems) * a.steps / (t1 - t0)
 
+    # Setup-inclusive rate: fresh subgames every step (other seeds), with the
+    # host tree build, upload, planning, mode choice, graph capture, the solve
+    # and the read-back of every output inside the clock.
+    setup_rec = None
+    if a.fresh_steps > 0:
+        fresh = [make_spots(a.workload, a.subgames, a.seed + 100000 * (k + 1)) for k in range(a.fresh_steps)]
+        tf0 = time.perf_counter()
+        for k, spots_k in enumerate(fresh):
+            probs_k = [ss_api.Problem.from_spot(s) for s in spots_k]
+            bk = ss_api.Batch(probs_k, a.max_iters, target=a.target, check_every=a.check_every)
+            reps_k, _ = bk.run()
+            for i in range(len(probs_k)):
+                bk.output(i, reps_k[i])
+            bk.close()
+            log(f"[bench] fresh-batch step {k + 1}/{a.fresh_steps} done")
+        tf1 = time.perf_counter()
+        setup_rec = {"value": round(len(problems) * a.fresh_steps / (tf1 - tf0), 3), "unit": "solves/s",
+                     "steps": a.fresh_steps,
+                     "ms_per_step": round((tf1 - tf0) * 1000.0 / a.fresh_steps, 3),
+                     "what": "a fresh batch of new subgames per step: host tree build, upload, planning, "
+                             "mode choice, graph capture, solve to target and read-back of every output "
+                             "inside the clock (synthetic input generation outside)"}
+
     reps = reps_all[-1]
     iters = np.array([r.iterations for r in reps])
     expl = np.array([r.exploitability for r in reps])
@@ -440,7 +574,8 @@ def main():
     log(f"[bench] plain form: {np.mean(plain_ms):.1f} ms per solve on {len(plain_ms)} subgames")
 
     line = {
-        "metric": "subgame solves/s at <=0.5% pot exploitability",
+        "metric": "subgame solves/s at <=0.5% pot exploitability; p50/p99 latency vs plain GPU "
+                  "(river subgames with players U{2,3}: 4-6 player full-range subgames are refused)",
         "value": round(value, 3),
         "unit": "solves/s",
         "n_gpus": 1,
@@ -481,6 +616,7 @@ def main():
             "avg_launch_us": round(cfr_ms * 1000.0 / max(1, launches), 3),
             "algo_bytes_per_launch": int(algo / max(1, launches)),
         },
+        "setup_inclusive": setup_rec,
         "plain_solves_per_s": round(plain_sps, 3) if plain_sps else None,
         "batched_vs_plain": round(value / plain_sps, 2) if plain_sps else None,
         "iterations_to_target": {"mean": float(iters.mean()), "p50": float(np.percentile(iters, 50)),

@@ -151,13 +151,22 @@ int read_back(LeafArena &a, void *dst, const void *src, size_t bytes) {
         a.pinned_cap = bytes;
     }
     SS_HIP(hipMemcpy(a.pinned, src, bytes, hipMemcpyDeviceToHost));
-    std::vector<std::thread> th;
+    // Chunk t goes to thread t; a chunk whose thread could not be started
+    // (std::system_error from the constructor) is copied on this thread, so
+    // no exception crosses the C-ABI and every byte is copied exactly once.
     const size_t chunk = (bytes + COPY_THREADS - 1) / COPY_THREADS;
-    for (int t = 0; t < COPY_THREADS; ++t) {
+    auto copy_chunk = [&a, dst, bytes, chunk](int t) {
         size_t b = (size_t)t * chunk, e = std::min(bytes, b + chunk);
-        if (b >= e) break;
-        th.emplace_back([=, &a]() { std::memcpy((char *)dst + b, (const char *)a.pinned + b, e - b); });
+        if (b < e) std::memcpy((char *)dst + b, (const char *)a.pinned + b, e - b);
+    };
+    std::vector<std::thread> th;
+    int started = 0;
+    try {
+        th.reserve(COPY_THREADS);
+        for (; started < COPY_THREADS; ++started) th.emplace_back(copy_chunk, started);
+    } catch (...) {
     }
+    for (int t = started; t < COPY_THREADS; ++t) copy_chunk(t);
     for (auto &x : th) x.join();
     return SS_OK;
 }

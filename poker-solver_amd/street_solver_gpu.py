@@ -1,26 +1,37 @@
 """StreetSolverGPU -- drop-in for python/street_solver_gpu.py on MI355X.
 
-Same constructor, methods, action labels and output thresholds as the
-reference class (python/street_solver_gpu.py:173-596), backed by
-libstreet_solve.so through the legacy C-ABI (ss_build_tree / ss_solve_gpu /
-ss_free_tree / ss_free_output). Behaviour kept from the reference:
-  - board-blocked combos are dropped, ranges above 200 combos are trimmed to
-    the 200 heaviest (stable by weight) (:224-229);
-  - pot / stack in BB are converted with SCALE = 100 and truncated (:263-264);
-  - continuation strategies only off the river (:272);
-  - leaf values: explicit array, else leaf_value_fn(tree, hands, max_h, pot),
-    else zeros with a warning off the river (:296-319);
-  - labels "Fold" / "Check" / "Call" / "Bet {pct}%" / "Raise {pct}%" (:128-169)
-    and the 0.001 output threshold (:418, :484).
-Difference: ss_free_tree here never frees the caller-owned leaf-value or
-frozen-action buffers (the reference frees leaf_values at
-src/cuda/street_solve.cu:1650, a buffer this class owns).
+The public surface is the reference class's (python/street_solver_gpu.py:
+173-596): constructor arguments, attribute names, methods, action labels and
+the 0.001 output threshold, because callers (hud_solver.py:834-857) parse
+them. The implementation is organised differently:
 
-extended=True lifts the 200-combo cap (up to 1326) and uses ss_solve_gpu_ex.
+  * ``_TreeView`` copies the built tree once into numpy arrays (node type,
+    actor, pot, bets, child lists); action labels and tree walks read those
+    arrays instead of ctypes structs.
+  * ``_Solution`` wraps the library's output arrays as numpy views once per
+    solve and indexes decision nodes by id and by player.
+  * One walker (``_TreeView.follow``) serves both the A3 freeze
+    (``set_frozen_actions``) and the strategy lookup (``get_strategy``).
+  * Native handles (tree and output) are owned by ``_NativeHandles`` and
+    released exactly once, by ``free()`` or by a ``weakref.finalize`` hook.
+
+Input preparation keeps the reference's behaviour: board-blocked combos are
+dropped and ranges above the cap are cut to the heaviest combos (stable by
+weight, :224-229); pot and stack in BB are converted with SCALE = 100 and
+truncated (:263-264); continuation strategies are used only off the river
+(:272); leaf values come from the explicit array, else from
+leaf_value_fn(tree, hands, max_h, pot), else zeros with a warning off the
+river (:296-319). ``ss_free_tree`` here never frees the caller-owned
+leaf-value or frozen-action buffers (the reference frees leaf_values at
+src/cuda/street_solve.cu:1650, a buffer this object owns).
+
+extended=True lifts the 200-combo cap (up to 1326) and solves through
+ss_solve_gpu_ex, which also reports the FP64 exploitability.
 """
 
 import ctypes
 import sys
+import weakref
 
 import numpy as np
 
@@ -33,38 +44,165 @@ SS_MAX_HANDS = ss_abi.SS_MAX_HANDS
 SS_MAX_ACTIONS = ss_abi.SS_MAX_ACTIONS
 SS_MAX_BOARD = ss_abi.SS_MAX_BOARD
 SCALE = 100
+OUTPUT_THRESHOLD = 0.001        # python/street_solver_gpu.py:418, :484
+
+_SEAT_NAMES = {"oop": 0, "ip": 1}
+_NODE_INTS = ctypes.sizeof(ss_abi.SSNode) // ctypes.sizeof(ctypes.c_int)   # 26
 
 
 def _get_lib():
     return ss_abi.load_library()
 
 
-def _build_labels_at_node(tree_data, node_idx):
-    """Action labels of a decision node (python/street_solver_gpu.py:128-169)."""
-    node = tree_data.nodes[node_idx]
-    acting = node.player
-    acting_bet = node.bets[acting] if acting >= 0 else 0
-    top = max(node.bets[p] for p in range(SS_MAX_PLAYERS))
-    to_call = top - acting_bet if acting >= 0 else 0
-    labels = []
-    for i in range(node.num_children):
-        child = tree_data.nodes[tree_data.children[node.first_child + i]]
-        if child.type == ss_abi.SS_NODE_FOLD:
-            labels.append("Fold")
-        elif child.type in (ss_abi.SS_NODE_SHOWDOWN, ss_abi.SS_NODE_LEAF):
-            labels.append("Call" if to_call > 0 else "Check")
-        elif child.type == ss_abi.SS_NODE_DECISION:
-            amount = (child.bets[acting] if acting >= 0 else 0) - acting_bet
-            if amount == 0:
-                labels.append("Check")
-            elif node.pot > 0:
-                pct = int(round(100 * amount / node.pot))
-                labels.append(f"Raise {pct}%" if to_call > 0 else f"Bet {pct}%")
+class _TreeView:
+    """Read-only numpy copy of the node table and child lists of a built tree."""
+
+    def __init__(self, td):
+        n = td.num_nodes
+        raw = np.ctypeslib.as_array(ctypes.cast(td.nodes, ctypes.POINTER(ctypes.c_int)),
+                                    shape=(n * _NODE_INTS,)).reshape(n, _NODE_INTS)
+        self.kind = raw[:, 0].copy()
+        self.actor = raw[:, 1].copy()
+        self.nkids = raw[:, 2].copy()
+        self.first = raw[:, 3].copy()
+        self.pot = raw[:, 4].copy()
+        self.bets = raw[:, 5:5 + SS_MAX_PLAYERS].copy()
+        m = td.num_children_total
+        self.kids = (np.ctypeslib.as_array(td.children, shape=(m,)).copy() if m > 0
+                     else np.zeros(0, np.int32))
+        self.num_nodes = n
+
+    def children(self, node):
+        f = int(self.first[node])
+        return self.kids[f:f + int(self.nkids[node])]
+
+    def labels(self, node):
+        """Action labels of a decision node (python/street_solver_gpu.py:128-169):
+        a fold child is "Fold"; a showdown or leaf child is "Call" when the
+        actor faces a bet, else "Check"; a decision child is named by how far
+        the actor's bet grew, in percent of the pot (Python round())."""
+        who = int(self.actor[node])
+        own = int(self.bets[node, who]) if who >= 0 else 0
+        facing = (int(self.bets[node].max()) - own) > 0 if who >= 0 else False
+        pot = int(self.pot[node])
+        out = []
+        for i, c in enumerate(self.children(node)):
+            k = int(self.kind[c])
+            if k == ss_abi.SS_NODE_FOLD:
+                out.append("Fold")
+            elif k in (ss_abi.SS_NODE_SHOWDOWN, ss_abi.SS_NODE_LEAF):
+                out.append("Call" if facing else "Check")
+            elif k == ss_abi.SS_NODE_DECISION:
+                grow = (int(self.bets[c, who]) if who >= 0 else 0) - own
+                verb = "Raise" if facing else "Bet"
+                if grow == 0:
+                    out.append("Check")
+                elif pot > 0:
+                    out.append(f"{verb} {int(round(100 * grow / pot))}%")
+                else:
+                    out.append(verb)
             else:
-                labels.append("Bet" if to_call == 0 else "Raise")
-        else:
-            labels.append(f"Action {i}")
-    return labels
+                out.append(f"Action {i}")
+        return out
+
+    def follow(self, actions, same):
+        """Walk from the root along action labels; yields (node, action index)
+        for every matched step and returns through StopIteration's value the
+        node reached. Stops at a non-decision node or an unmatched label.
+        ``same(tree_label, wanted)`` decides a match."""
+        node = 0
+        for want in actions:
+            if node >= self.num_nodes or int(self.kind[node]) != ss_abi.SS_NODE_DECISION:
+                break
+            names = self.labels(node)
+            hit = next((i for i, nm in enumerate(names) if same(nm, want)), None)
+            if hit is None:
+                break
+            yield node, hit
+            node = int(self.children(node)[hit])
+        return node
+
+    def walk_to(self, actions, same):
+        it = self.follow(actions, same)
+        while True:
+            try:
+                next(it)
+            except StopIteration as stop:
+                return stop.value
+
+
+def _build_labels_at_node(tree_data, node_idx):
+    """Module-level label helper with the reference's signature (:128)."""
+    return _TreeView(tree_data).labels(node_idx)
+
+
+def _exact_label(tree_label, wanted):          # set_frozen_actions (:349)
+    return tree_label.lower() == wanted.lower()
+
+
+def _loose_label(tree_label, wanted):          # get_strategy walk (:457-460)
+    return tree_label.lower().strip() == wanted.lower().strip()
+
+
+def _filter_range(rng, blocked, cap):
+    hands = parse_range_string(rng) if isinstance(rng, str) else list(rng)
+    hands = [h for h in hands if h[0] not in blocked and h[1] not in blocked]
+    if len(hands) > cap:
+        hands = sorted(hands, key=lambda h: -h[2])[:cap]    # stable: ties keep order
+    return hands
+
+
+class _NativeHandles:
+    """Owns the library-allocated tree and output; ``release`` runs once."""
+
+    def __init__(self, lib):
+        self.lib = lib
+        self.tree = SSTreeData()
+        self.output = SSOutput()
+        self.tree_built = False
+        self.output_live = False
+
+    def drop_output(self):
+        if self.output_live:
+            self.output_live = False
+            self.lib.ss_free_output(ctypes.byref(self.output))
+
+    def release(self):
+        self.drop_output()
+        if self.tree_built:
+            self.tree_built = False
+            self.lib.ss_free_tree(ctypes.byref(self.tree))
+
+
+class _Solution:
+    """numpy views of one SSOutput plus decision-node indexes."""
+
+    def __init__(self, out, view, hand_counts, max_h):
+        self.root_player = out.root_player
+        na = out.root_num_actions
+        nh = hand_counts[self.root_player] if 0 <= self.root_player < len(hand_counts) else 0
+        self.root = (np.ctypeslib.as_array(out.root_strategy, shape=(na * nh,)).reshape(na, nh)
+                     if na * nh > 0 else np.zeros((na, 0), np.float32))
+        d = out.num_avg_nodes
+        self.node_ids = (np.ctypeslib.as_array(out.avg_strategy_node_ids, shape=(d,)).copy()
+                         if d > 0 else np.zeros(0, np.int32))
+        self.avg = (np.ctypeslib.as_array(out.avg_strategies, shape=(d * SS_MAX_ACTIONS * max_h,))
+                    .reshape(d, SS_MAX_ACTIONS, max_h) if d > 0 else np.zeros((0, SS_MAX_ACTIONS, max_h)))
+        self.row_of_node = {}
+        self.first_row_of_player = {}
+        for row, nid in enumerate(self.node_ids.tolist()):
+            if int(view.kind[nid]) != ss_abi.SS_NODE_DECISION:
+                continue
+            self.row_of_node[nid] = row
+            self.first_row_of_player.setdefault(int(view.actor[nid]), row)
+
+
+def _as_dict(labels, probs):
+    """{label: p} for p above the threshold, compared in float64 as the
+    reference compares Python floats."""
+    p64 = np.asarray(probs, dtype=np.float64)
+    return {labels[a]: float(p64[a]) for a in range(min(len(labels), len(p64)))
+            if p64[a] > OUTPUT_THRESHOLD}
 
 
 class StreetSolverGPU:
@@ -75,10 +213,12 @@ class StreetSolverGPU:
                  leaf_value_fn=None, extended=False, max_raises=None):
         self.lib = _get_lib()
         self.extended = bool(extended)
-        self.board_cards = [card_to_int(c) for c in board] if isinstance(board[0], str) else list(board)
+        self._freed = False
+        self._solved = False
+        self._sol = None
+        self.board_cards = [card_to_int(c) if isinstance(c, str) else int(c) for c in board]
         self.num_board = len(self.board_cards)
         self.is_river = self.num_board == 5
-        blocked = set(self.board_cards)
         if player_ranges is not None:
             ranges = list(player_ranges)
         elif oop_range is not None and ip_range is not None:
@@ -86,117 +226,123 @@ class StreetSolverGPU:
         else:
             raise ValueError("Provide oop_range+ip_range or player_ranges")
         cap = ss_abi.SS_EX_MAX_HANDS if self.extended else SS_MAX_HANDS
-        self.num_players = len(ranges)
-        self.player_hands = []
-        for r in ranges:
-            hands = parse_range_string(r) if isinstance(r, str) else list(r)
-            hands = [(c0, c1, w) for c0, c1, w in hands if c0 not in blocked and c1 not in blocked]
-            if len(hands) > cap:
-                hands.sort(key=lambda x: -x[2])
-                hands = hands[:cap]
-            self.player_hands.append(hands)
+        blocked = frozenset(self.board_cards)
+        self.player_hands = [_filter_range(r, blocked, cap) for r in ranges]
+        self.num_players = len(self.player_hands)
         if self.num_players >= 1:
             self.oop_hands = self.player_hands[0]
         if self.num_players >= 2:
             self.ip_hands = self.player_hands[1]
         self.bet_sizes = [0.33, 0.75, 1.5] if bet_sizes is None else list(bet_sizes)
         self.acting_order = list(range(self.num_players)) if acting_order is None else list(acting_order)
-        pot, stack = int(pot_bb * SCALE), int(stack_bb * SCALE)
+        self._index_of = [None] * self.num_players
 
-        self._tree = SSTreeData()
-        use_cs = 1 if (use_cont_strats and not self.is_river) else 0
-        barr = (ctypes.c_int * self.num_board)(*self.board_cards)
-        sarr = (ctypes.c_float * max(1, len(self.bet_sizes)))(*self.bet_sizes)
-        oarr = (ctypes.c_int * len(self.acting_order))(*self.acting_order)
-        if max_raises is None:
-            err = self.lib.ss_build_tree(barr, self.num_board, pot, stack, sarr, len(self.bet_sizes),
-                                         self.num_players, oarr, 1 if self.is_river else 0, use_cs,
-                                         ctypes.byref(self._tree))
-        else:
-            err = self.lib.ss_build_tree_ex(barr, self.num_board, pot, stack, sarr, len(self.bet_sizes),
-                                            self.num_players, oarr, 1 if self.is_river else 0, use_cs,
-                                            int(max_raises), ctypes.byref(self._tree))
-        if err != 0:
-            raise RuntimeError(f"ss_build_tree failed: {ss_abi.last_error()}")
-
-        # hands and weights (legacy struct slots hold the first 200; the
-        # extended path passes arrays)
-        self._hand_arrs, self._weight_arrs = [], []
-        for p in range(self.num_players):
-            hs = self.player_hands[p]
-            self._hand_arrs.append(np.array([[c0, c1] for c0, c1, _ in hs], dtype=np.int32).reshape(-1, 2))
-            self._weight_arrs.append(np.array([w for _, _, w in hs], dtype=np.float32))
-            if not self.extended:
-                self._tree.num_hands[p] = len(hs)
-                for i, (c0, c1, w) in enumerate(hs):
-                    self._tree.hands[p][i][0] = c0
-                    self._tree.hands[p][i][1] = c1
-                    self._tree.weights[p][i] = w
-        max_h = max(len(h) for h in self.player_hands)
-        self._max_hands = max_h
-
-        if leaf_values is None and leaf_value_fn is not None and self._tree.num_leaves > 0:
-            leaf_values = leaf_value_fn(self._tree, self.player_hands, max_h, pot)
-        self._leaf_arr = None
-        if leaf_values is not None and self._tree.num_leaves > 0:
-            expected = self._tree.num_leaves * self.num_players * max_h
-            flat = np.ascontiguousarray(np.asarray(leaf_values, dtype=np.float32).ravel())
-            if len(flat) != expected:
-                raise ValueError(f"leaf_values mismatch: {len(flat)} vs {expected} "
-                                 f"(leaves={self._tree.num_leaves}, players={self.num_players}, "
-                                 f"max_hands={max_h})")
-            self._leaf_arr = flat
-        elif not self.is_river and self._tree.num_leaves > 0:
-            print("[WARNING] Zero leaf values — no blueprint data.", file=sys.stderr)
-            self._leaf_arr = np.zeros(self._tree.num_leaves * self.num_players * max_h, np.float32)
+        self._handles = _NativeHandles(self.lib)
+        self._tree = self._handles.tree
+        self._output = self._handles.output
+        self._build_tree(int(pot_bb * SCALE), int(stack_bb * SCALE), use_cont_strats, max_raises)
+        self._finalizer = weakref.finalize(self, self._handles.release)
+        self._view = _TreeView(self._tree)
+        self._install_ranges()
+        self._max_hands = max(len(h) for h in self.player_hands)
+        self._leaf_arr = self._leaf_buffer(leaf_values, leaf_value_fn, int(pot_bb * SCALE))
         if self._leaf_arr is not None:
             self._tree.leaf_values = self._leaf_arr.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
-
         self._frozen_arr = None
-        self._output = SSOutput()
         self._report = SSReport()
-        self._solved = False
-        self._freed = False
 
-    # -- A3 freezing (python/street_solver_gpu.py:327-366)
+    # -- construction helpers -------------------------------------------------
+    def _build_tree(self, pot, stack, use_cont_strats, max_raises):
+        nb = self.num_board
+        args = [(ctypes.c_int * nb)(*self.board_cards), nb, pot, stack,
+                (ctypes.c_float * max(1, len(self.bet_sizes)))(*self.bet_sizes), len(self.bet_sizes),
+                self.num_players, (ctypes.c_int * len(self.acting_order))(*self.acting_order),
+                int(self.is_river), int(bool(use_cont_strats) and not self.is_river)]
+        if max_raises is None:
+            rc = self.lib.ss_build_tree(*args, ctypes.byref(self._tree))
+        else:
+            rc = self.lib.ss_build_tree_ex(*args, int(max_raises), ctypes.byref(self._tree))
+        if rc != 0:
+            raise RuntimeError(f"ss_build_tree failed: {ss_abi.last_error()}")
+        self._handles.tree_built = True
+
+    def _install_ranges(self):
+        """Hands and weights as int32 [nh, 2] / float32 [nh] arrays; in legacy
+        mode also written into the struct's fixed 200-combo slots, as the
+        reference wrapper fills them after the build (:286-291)."""
+        self._hand_arrs, self._weight_arrs = [], []
+        for p, hs in enumerate(self.player_hands):
+            cards_arr = np.array([h[:2] for h in hs], dtype=np.int32).reshape(-1, 2)
+            w_arr = np.array([h[2] for h in hs], dtype=np.float32)
+            self._hand_arrs.append(cards_arr)
+            self._weight_arrs.append(w_arr)
+            if self.extended:
+                continue
+            n = len(hs)
+            self._tree.num_hands[p] = n
+            slot_c = np.ctypeslib.as_array(self._tree.hands[p])          # [200, 2] view
+            slot_w = np.ctypeslib.as_array(self._tree.weights[p])        # [200] view
+            slot_c[:n] = cards_arr
+            slot_w[:n] = w_arr
+
+    def _leaf_buffer(self, leaf_values, leaf_value_fn, pot):
+        L = self._tree.num_leaves
+        if L <= 0:
+            return None
+        if leaf_values is None and leaf_value_fn is not None:
+            leaf_values = leaf_value_fn(self._tree, self.player_hands, self._max_hands, pot)
+        want = L * self.num_players * self._max_hands
+        if leaf_values is not None:
+            flat = np.ascontiguousarray(np.asarray(leaf_values, dtype=np.float32).reshape(-1))
+            if flat.size != want:
+                raise ValueError(f"leaf_values mismatch: {flat.size} vs {want} "
+                                 f"(leaves={L}, players={self.num_players}, max_hands={self._max_hands})")
+            return flat
+        if not self.is_river:
+            print("[WARNING] Zero leaf values — no blueprint data.", file=sys.stderr)
+            return np.zeros(want, np.float32)
+        return None
+
+    # -- A3 freezing (python/street_solver_gpu.py:327-366) --------------------
     def set_frozen_actions(self, frozen_actions, hero_player):
-        n = self._tree.num_nodes
-        frozen = np.full(n, -1, dtype=np.int32)
-        cur = 0
-        for label in frozen_actions:
-            node = self._tree.nodes[cur]
-            if node.type != ss_abi.SS_NODE_DECISION:
-                break
-            labels = _build_labels_at_node(self._tree, cur)
-            hit = next((i for i, l in enumerate(labels) if l.lower() == label.lower()), -1)
-            if hit < 0:
-                break
-            if node.player == hero_player:
-                frozen[cur] = hit
-            cur = self._tree.children[node.first_child + hit]
+        frozen = np.full(self._view.num_nodes, -1, dtype=np.int32)
+        for node, act in self._view.follow(frozen_actions, _exact_label):
+            if int(self._view.actor[node]) == hero_player:
+                frozen[node] = act
         self._frozen_arr = frozen
         self._tree.frozen_action = frozen.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
 
+    # -- solving ----------------------------------------------------------------
+    def _problem(self):
+        prob = SSProblem()
+        prob.tree = ctypes.pointer(self._tree)
+        for p in range(self.num_players):
+            prob.num_hands[p] = len(self._hand_arrs[p])
+            prob.hands[p] = self._hand_arrs[p].ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+            prob.weights[p] = self._weight_arrs[p].ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        if self._leaf_arr is not None:
+            prob.leaf_values = self._leaf_arr.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        if self._frozen_arr is not None:
+            prob.frozen_action = self._frozen_arr.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        return prob
+
     def solve(self, iterations=200):
+        if self._freed:
+            raise RuntimeError("solver already freed")
+        self._handles.drop_output()       # a second solve replaces the first output
+        self._sol, self._solved = None, False
         if self.extended:
-            prob = SSProblem()
-            prob.tree = ctypes.pointer(self._tree)
-            for p in range(self.num_players):
-                prob.num_hands[p] = len(self._hand_arrs[p])
-                prob.hands[p] = self._hand_arrs[p].ctypes.data_as(ctypes.POINTER(ctypes.c_int))
-                prob.weights[p] = self._weight_arrs[p].ctypes.data_as(ctypes.POINTER(ctypes.c_float))
-            if self._leaf_arr is not None:
-                prob.leaf_values = self._leaf_arr.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
-            if self._frozen_arr is not None:
-                prob.frozen_action = self._frozen_arr.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
             cfg = SSSolveConfig()
             cfg.max_iterations = int(iterations)
-            err = self.lib.ss_solve_gpu_ex(ctypes.byref(prob), ctypes.byref(cfg), ctypes.byref(self._output),
-                                           ctypes.byref(self._report))
+            rc = self.lib.ss_solve_gpu_ex(ctypes.byref(self._problem()), ctypes.byref(cfg),
+                                          ctypes.byref(self._output), ctypes.byref(self._report))
         else:
-            err = self.lib.ss_solve_gpu(ctypes.byref(self._tree), int(iterations), ctypes.byref(self._output))
-        if err != 0:
+            rc = self.lib.ss_solve_gpu(ctypes.byref(self._tree), int(iterations), ctypes.byref(self._output))
+        if rc != 0:
             raise RuntimeError(f"ss_solve_gpu failed: {ss_abi.last_error()}")
+        self._handles.output_live = True
+        self._sol = _Solution(self._output, self._view, [len(h) for h in self.player_hands],
+                              self._max_hands)
         self._solved = True
 
     @property
@@ -204,141 +350,87 @@ class StreetSolverGPU:
         """FP64 exploitability in % of the starting pot (extended mode only)."""
         return self._report.exploitability if self.extended else None
 
-    # -- strategy access (python/street_solver_gpu.py:375-553)
-    def get_strategy(self, player, hand_str=None, hand_idx=None, frozen_actions=None):
+    # -- strategy access (python/street_solver_gpu.py:375-553) ----------------
+    def _require_solution(self):
         if not self._solved:
             raise RuntimeError("Must call solve() first")
+        return self._sol
+
+    def _row_strategy(self, row, hand_idx):
+        if row is None:
+            return {}
+        node = int(self._sol.node_ids[row])
+        return _as_dict(self._view.labels(node), self._sol.avg[row, :int(self._view.nkids[node]), hand_idx])
+
+    def get_strategy(self, player, hand_str=None, hand_idx=None, frozen_actions=None):
+        """Strategy {label: frequency} of one hand at the player's current
+        decision: with no action history, the root's last-iteration strategy
+        when the player acts first, else the player's first decision node's
+        average strategy; with a history, the average strategy at the node the
+        history leads to if the player acts there, else the first one."""
+        sol = self._require_solution()
         pi = self._resolve_player(player)
         if hand_idx is None:
             hand_idx = self._find_hand(pi, hand_str)
         if not frozen_actions:
-            if self._output.root_player == pi:
-                nh = len(self.player_hands[pi])
-                labels = _build_labels_at_node(self._tree, 0)
-                out = {}
-                for a in range(min(self._output.root_num_actions, len(labels))):
-                    f = self._output.root_strategy[a * nh + hand_idx]
-                    if f > 0.001:
-                        out[labels[a]] = float(f)
-                return out
-            return self._get_strategy_at_player_node(pi, hand_idx)
-        cur, k = 0, 0
-        while k < len(frozen_actions) and cur < self._tree.num_nodes:
-            node = self._tree.nodes[cur]
-            if node.type != ss_abi.SS_NODE_DECISION:
-                break
-            labels = _build_labels_at_node(self._tree, cur)
-            want = frozen_actions[k]
-            hit = next((c for c in range(node.num_children)
-                        if c < len(labels) and self._match_action_label(labels[c], want)), -1)
-            if hit < 0:
-                break
-            cur = self._tree.children[node.first_child + hit]
-            k += 1
-        node = self._tree.nodes[cur]
-        if node.type == ss_abi.SS_NODE_DECISION and node.player == pi:
-            return self._get_strategy_at_node(cur, pi, hand_idx)
-        return self._get_strategy_at_player_node(pi, hand_idx)
+            if sol.root_player == pi:
+                return _as_dict(self._view.labels(0), sol.root[:, hand_idx])
+            return self._row_strategy(sol.first_row_of_player.get(pi), hand_idx)
+        node = self._view.walk_to(frozen_actions, _loose_label)
+        if int(self._view.kind[node]) == ss_abi.SS_NODE_DECISION and int(self._view.actor[node]) == pi:
+            return self._row_strategy(sol.row_of_node.get(node), hand_idx)
+        return self._row_strategy(sol.first_row_of_player.get(pi), hand_idx)
 
     @staticmethod
     def _match_action_label(tree_label, frozen_label):
-        return tree_label.lower().strip() == frozen_label.lower().strip()
-
-    def _avg_row(self, di, na, labels, hand_idx):
-        mh = self._max_hands
-        out = {}
-        for a in range(min(na, len(labels))):
-            f = self._output.avg_strategies[di * SS_MAX_ACTIONS * mh + a * mh + hand_idx]
-            if f > 0.001:
-                out[labels[a]] = float(f)
-        return out
-
-    def _get_strategy_at_node(self, node_idx, player_idx, hand_idx):
-        for di in range(self._output.num_avg_nodes):
-            nid = self._output.avg_strategy_node_ids[di]
-            if nid != node_idx:
-                continue
-            node = self._tree.nodes[nid]
-            if node.player != player_idx or node.type != ss_abi.SS_NODE_DECISION:
-                continue
-            return self._avg_row(di, node.num_children, _build_labels_at_node(self._tree, nid), hand_idx)
-        return {}
-
-    def _get_strategy_at_player_node(self, player_idx, hand_idx):
-        for di in range(self._output.num_avg_nodes):
-            nid = self._output.avg_strategy_node_ids[di]
-            node = self._tree.nodes[nid]
-            if node.player != player_idx or node.type != ss_abi.SS_NODE_DECISION:
-                continue
-            return self._avg_row(di, node.num_children, _build_labels_at_node(self._tree, nid), hand_idx)
-        return {}
+        return _loose_label(tree_label, frozen_label)
 
     def get_all_strategies(self, player):
-        if not self._solved:
-            raise RuntimeError("Must call solve() first")
+        """{hand index: strategy} for every hand whose strategy is non-empty."""
+        self._require_solution()
         pi = self._resolve_player(player)
-        out = {}
-        for h in range(len(self.player_hands[pi])):
-            s = self.get_strategy(pi, hand_idx=h)
-            if s:
-                out[h] = s
-        return out
+        every = ((h, self.get_strategy(pi, hand_idx=h)) for h in range(len(self.player_hands[pi])))
+        return {h: s for h, s in every if s}
 
     def get_avg_strategy(self, player):
-        if not self._solved:
-            raise RuntimeError("Must call solve() first")
+        """Average strategy at the player's first decision node, keyed by the
+        (low card, high card) combo: {(c0, c1): {label: frequency}}."""
+        sol = self._require_solution()
         pi = self._resolve_player(player)
-        for di in range(self._output.num_avg_nodes):
-            nid = self._output.avg_strategy_node_ids[di]
-            node = self._tree.nodes[nid]
-            if node.type != ss_abi.SS_NODE_DECISION or node.player != pi:
-                continue
-            labels = _build_labels_at_node(self._tree, nid)
-            out = {}
-            for h, (c0, c1, _) in enumerate(self.player_hands[pi]):
-                out[(min(c0, c1), max(c0, c1))] = self._avg_row(di, node.num_children, labels, h)
-            return out
-        return {}
+        row = sol.first_row_of_player.get(pi)
+        if row is None:
+            return {}
+        return {(min(h[0], h[1]), max(h[0], h[1])): self._row_strategy(row, i)
+                for i, h in enumerate(self.player_hands[pi])}
 
     def get_avg_strategy_probs(self, player, action_label):
-        return {k: v.get(action_label, 0.0) for k, v in self.get_avg_strategy(player).items()}
+        """P(action | hand) for every hand, 0 where the action is below the
+        threshold (the input of range narrowing)."""
+        return {combo: s.get(action_label, 0.0) for combo, s in self.get_avg_strategy(player).items()}
 
     def _resolve_player(self, player):
         if isinstance(player, int):
             return player
-        if player == "oop":
-            return 0
-        if player == "ip":
-            return 1
-        return int(player)
+        return _SEAT_NAMES[player] if player in _SEAT_NAMES else int(player)
 
     def _find_hand(self, player_idx, hand_str):
-        c0, c1 = card_to_int(hand_str[:2]), card_to_int(hand_str[2:])
-        key = (min(c0, c1), max(c0, c1))
-        for i, (h0, h1, _) in enumerate(self.player_hands[player_idx]):
-            if (min(h0, h1), max(h0, h1)) == key:
-                return i
-        raise ValueError(f"Hand {hand_str} not in player {player_idx}'s range")
+        if self._index_of[player_idx] is None:
+            index = {}
+            for i, h in enumerate(self.player_hands[player_idx]):
+                index.setdefault((min(h[0], h[1]), max(h[0], h[1])), i)
+            self._index_of[player_idx] = index
+        a, b = card_to_int(hand_str[:2]), card_to_int(hand_str[2:])
+        i = self._index_of[player_idx].get((min(a, b), max(a, b)))
+        if i is None:
+            raise ValueError(f"Hand {hand_str} not in player {player_idx}'s range")
+        return i
 
+    # -- lifetime ---------------------------------------------------------------
     def free(self):
-        if getattr(self, "_freed", False):
-            return
+        """Release the native tree and output (idempotent; also runs when the
+        object is garbage-collected)."""
         self._freed = True
-        tree = getattr(self, "_tree", None)
-        if tree is not None and getattr(tree, "nodes", None):
-            try:
-                self.lib.ss_free_tree(ctypes.byref(tree))
-            except Exception:
-                pass
-        out = getattr(self, "_output", None)
-        if out is not None and getattr(self, "_solved", False):
-            try:
-                self.lib.ss_free_output(ctypes.byref(out))
-            except Exception:
-                pass
-
-    def __del__(self):
-        try:
-            self.free()
-        except Exception:
-            pass
+        self._sol = None
+        fin = self.__dict__.get("_finalizer")
+        if fin is not None:
+            fin()

@@ -158,3 +158,23 @@ def test_random_cases_equal_oracle(lib, seed):
         ref = lo.turn_leaf_values(board, hands, nh, mh, infos)
     assert out.shape == ref.shape
     assert np.array_equal(out.view(np.uint32), ref.view(np.uint32))
+
+
+def test_large_output_read_back_equals_oracle(lib):
+    """An output above the library's 4 MiB pinned read-back threshold (3
+    players x 1128 hands x 8 leaf infos x 64 profiles = 6.9 MB, split over
+    the host copy threads) equals the oracle's values on the GPU's own equity
+    in every element, bit for bit."""
+    rng = np.random.default_rng(8300)
+    board = cards("Kd 7c 2h 9s")
+    nh = [1128, 1128, 1128]
+    hands = turn_hands(rng, board, nh)
+    runs = lv.turn_runouts(board, nh)
+    infos = [(i, 700 + 150 * i) for i in range(8)]
+    print(f"leaf large read-back np=3 hands=1128 infos={len(infos)}", flush=True)
+    out = lv._leaf_values([[(a, b, 1.0) for a, b in hs] for hs in hands], runs,
+                          [lv.LeafInfo(i, p, ()) for i, p in infos], 1128, lib)
+    assert out.nbytes >= 4 << 20
+    raw = gpu_equity(lib, hands, nh, runs, 1128)
+    ref = lo.values_from_equity(raw, nh, 1128, infos)
+    assert np.array_equal(out.view(np.uint32), ref.view(np.uint32))

@@ -6,8 +6,9 @@ Error budget. The opponent reach is carried as hi = bf16(r), lo = bf16(r - hi):
 is exact, so a showdown value differs from the FP32 closed form by at most
 about 2^-16 of pot * (opponent mass) plus FP32 accumulation rounding. State
 after one iteration is checked against the FP64 oracle at 5e-5 of the array
-scale (the FP32 closed form passes at 2e-5); later iterations inherit the
-same flip caveat as every FP32 path (see test_gpu_parity.py).
+scale (the FP32 closed form passes at 2e-5); after three iterations only
+rows whose FP64 regrets sit at regret matching's discontinuity (largest
+regret within the 5e-5 budget of 0) may differ.
 """
 
 import numpy as np
@@ -18,7 +19,7 @@ import ss_abi
 import ss_api
 import workloads as wl
 
-from test_gpu_parity import _close_state, _spot_problem
+from test_gpu_parity import _close_state, _spot_problem, _tie_rows
 
 pytestmark = pytest.mark.gpu
 PREC = 2   # ssdbg_solve_state form: plain + BF16 showdown
@@ -43,8 +44,9 @@ def test_precision_state_river_2p_vs_fp64_oracle():
     _close_state(S, ref["strat_sum"], rel=5e-5)
     R3, S3, _ = ss_api.solve_state(prob, 3, PREC)
     ref3 = ob.solve(oprob, 3, fp64=True, mode=ob.TERM_FAST, want_state=True)
-    _close_state(R3, ref3["regrets"], rel=5e-5, flip_frac=0.02)
-    _close_state(S3, ref3["strat_sum"], rel=5e-5, flip_frac=0.02)
+    ties = _tie_rows(prob, oprob, 3, mode=ob.TERM_FAST, tol=5e-5)   # the BF16 error budget
+    _close_state(R3, ref3["regrets"], rel=5e-5, excused=ties)
+    _close_state(S3, ref3["strat_sum"], rel=5e-5, excused=ties)
 
 
 def test_precision_exploitability_river_2p_vs_fp64_oracle():

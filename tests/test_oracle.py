@@ -256,3 +256,202 @@ def test_exploitability_decreases():
         res = ob.solve(prob, it, fp64=True, mode=ob.TERM_FAST)
         es.append(ob.exploitability(prob, res["avg"])[0])
     assert es[0] > es[1] > es[2] >= 0
+
+
+# ---------------------------------------------------------------- hand-derived pins of the CFR loop
+# A 9-node two-player river (pot 100, stack 50, one size 0.5 pot, max 2 raises):
+#   0 P0: check -> 1, bet 50 -> 6      1 P1: check -> 2 (showdown, pot 100), bet 50 -> 3
+#   3 P0: fold -> 4 (P1 wins pot 150), call -> 5 (showdown, pot 200)
+#   6 P1: fold -> 7 (P0 wins pot 150), call -> 8 (showdown, pot 200)
+# P0 holds 6h6s (a seven-high straight), P1 holds AcAd (the wheel): P0 wins
+# every showdown. Raw weights 2.0 (P0) and 0.5 (P1), deliberately not 1 and
+# not normalised. Every number below was worked out by hand from
+# street_solve.cu:575-614 (regret matching), :650-683 (reach), :704-1005
+# (terminal values), :1155-1175 (backup: R <- (R + dR) * t/(t+1),
+# SS += t * sigma), :1471-1537 (passes P0 then P1, each re-matching every
+# node first) and :1563-1578 (root outputs).
+TINY_BOARD = "2c 3d 4h 5s 7c"
+
+
+def _tiny_2p():
+    board = _cards(TINY_BOARD)
+    td = ob.build_tree(board, 100, 50, [0.5], 2, max_raises=2)
+    assert td.num_nodes == 9
+    hands = [np.array([_cards("6h 6s")], np.int32), np.array([_cards("Ac Ad")], np.int32)]
+    assert ob.eval7(board + _cards("6h 6s")) > ob.eval7(board + _cards("Ac Ad"))
+    return ob.Problem(td, board, hands, [np.array([2.0], np.float32), np.array([0.5], np.float32)])
+
+
+# (node, action) -> literal after iteration 1 and after iteration 2
+TINY_REGRETS = {
+    1: {(0, 0): -4.6875, (0, 1): 4.6875, (3, 0): -9.375, (3, 1): 9.375,
+        (1, 0): 0.0, (1, 1): 0.0, (6, 0): 25.0, (6, 1): -25.0},
+    2: {(0, 0): 125.0 / 24.0, (0, 1): 3.125, (3, 0): -31.25, (3, 1): 6.25,
+        (1, 0): 125.0 / 6.0, (1, 1): -125.0 / 6.0, (6, 0): 50.0 / 3.0, (6, 1): -125.0 / 3.0},
+}
+TINY_STRAT_SUMS = {
+    1: {(0, 0): 0.5, (0, 1): 0.5, (3, 0): 0.5, (3, 1): 0.5, (1, 0): 0.5, (1, 1): 0.5, (6, 0): 0.5, (6, 1): 0.5},
+    2: {(0, 0): 0.5, (0, 1): 2.5, (3, 0): 0.5, (3, 1): 2.5, (1, 0): 1.5, (1, 1): 1.5, (6, 0): 2.5, (6, 1): 0.5},
+}
+TINY_ROOT = {1: ((0.0, 1.0), -50.0), 2: ((0.625, 0.375), -31.25)}
+
+
+@pytest.mark.parametrize("k", [1, 2])
+def test_cfr_state_hand_derived(k):
+    """Regrets and strategy sums after k iterations equal the hand-derived
+    literals. Mistakes this catches: the discount applied before the regret
+    add (R <- R*t/(t+1) + dR gives R[3] = (-18.75, 18.75) after one
+    iteration); a reach-weighted strategy sum (P0's weight 2 doubles SS[0]);
+    a strategy sum without the t factor (SS[0] = (0.5, 1.5) after two);
+    the traverser's own reach in its values, or normalised weights (every
+    regret scales)."""
+    prob = _tiny_2p()
+    r = ob.solve(prob, k, fp64=True, mode=ob.TERM_BRUTE, want_state=True)
+    for (n, a), want in TINY_REGRETS[k].items():
+        assert r["regrets"][n, a, 0] == pytest.approx(want, abs=1e-12), (k, n, a)
+    for (n, a), want in TINY_STRAT_SUMS[k].items():
+        assert r["strat_sum"][n, a, 0] == pytest.approx(want, abs=1e-12), (k, n, a)
+    # the fast (closed-form) terminals give the same numbers
+    f = ob.solve(prob, k, fp64=True, mode=ob.TERM_FAST, want_state=True)
+    assert np.array_equal(f["regrets"], r["regrets"]) and np.array_equal(f["strat_sum"], r["strat_sum"])
+
+
+@pytest.mark.parametrize("k", [1, 2])
+def test_root_outputs_hand_derived(k):
+    """root_strategy is the root actor's strategy from the LAST traverser
+    pass's regret matching ((0.625, 0.375) after two iterations; the first
+    pass's would be (0, 1)); root_ev is the node-0 value of the LAST traverser
+    (player NP-1 = P1: -50 after one iteration; P0's would be +53.125)."""
+    prob = _tiny_2p()
+    r = ob.solve(prob, k, fp64=True, mode=ob.TERM_BRUTE)
+    (s0, s1), ev = TINY_ROOT[k]
+    assert r["root_strategy"][0, 0] == pytest.approx(s0, abs=1e-6)
+    assert r["root_strategy"][1, 0] == pytest.approx(s1, abs=1e-6)
+    assert r["root_ev"][0] == pytest.approx(ev, abs=1e-5)
+
+
+def test_average_strategy_hand_derived():
+    """Host normalisation (street_solve.cu:1597-1610) of the strategy sums
+    after two iterations: (1/6, 5/6) at P0's nodes 0 and 3, uniform at node 1,
+    (5/6, 1/6) at node 6."""
+    r = ob.solve(_tiny_2p(), 2, fp64=True, mode=ob.TERM_BRUTE)
+    want = {0: (1 / 6, 5 / 6), 1: (0.5, 0.5), 3: (1 / 6, 5 / 6), 6: (5 / 6, 1 / 6)}
+    for n, (a0, a1) in want.items():
+        assert r["avg"][n, 0, 0] == pytest.approx(a0, abs=1e-6)
+        assert r["avg"][n, 1, 0] == pytest.approx(a1, abs=1e-6)
+
+
+# ---------------------------------------------------------------- hand-derived terminal values
+def _three_player_tree(stack=50):
+    # 25 nodes for stack 50 (pot 100, size 0.5): node 10 = showdown after
+    # bet-call-call (pot 250, bets 50 each); node 6 = fold won by P2 with
+    # P0 and P1 folded (pot 150, bets (0, 0, 50)).
+    return ob.build_tree(_cards("Kc Kd 7h 7s 2c"), 100, stack, [0.5], 3, max_raises=2)
+
+
+def _tv(td, board, hands, reach, node, trav, mode):
+    prob = ob.Problem(td, board, [np.array(h, np.int32).reshape(-1, 2) for h in hands],
+                      [np.ones(len(h), np.float32) for h in hands])
+    return ob.terminal_values(prob, node, trav, [np.array(r, np.float64) for r in reach], mode)
+
+
+@pytest.mark.parametrize("mode", [ob.TERM_BRUTE, ob.TERM_FAST])
+def test_showdown_three_way_and_two_way_ties(mode):
+    """Board Kc Kd 7h 7s 2c. P0 holds Ac3h (kings and sevens, ace kicker); P1
+    holds As4h (ties, reach 0.3) or QhJh (loses, 0.7); P2 holds Ah5c (ties,
+    0.2) or 3d4d (loses, 0.5). At node 10 (pot 250, everyone in for 50),
+    P0's value is, by tuple:
+      3-way tie   (250/3 - 50) * 0.06 = 2
+      2-way ties  (250/2 - 50) * (0.15 + 0.14) = 21.75
+      outright    (250 - 50) * 0.35 = 70
+    total 93.75 (street_solve.cu:972-990). Catches pot/2 for every tie
+    (96.25) and ties paid without subtracting the bet."""
+    td = _three_player_tree()
+    nd = td.nodes[10]
+    assert nd.type == 2 and nd.pot == 250 and list(nd.bets)[:3] == [50, 50, 50]
+    board = _cards("Kc Kd 7h 7s 2c")
+    h0, a1, b1, a2, b2 = (_cards(x) for x in ("Ac 3h", "As 4h", "Qh Jh", "Ah 5c", "3d 4d"))
+    s = lambda h: ob.eval7(board + h)
+    assert s(h0) == s(a1) == s(a2) and s(b1) < s(h0) and s(b2) < s(h0)
+    v = _tv(td, board, [[h0], [a1, b1], [a2, b2]], [[1.0], [0.3, 0.7], [0.2, 0.5]], 10, 0, mode)
+    assert v[0] == pytest.approx(93.75, abs=1e-9)
+
+
+def test_two_player_tie_pays_half_pot_minus_bet():
+    """Two players at node 8 of the tiny river (pot 200, bets 50 each): a tie
+    pays 200/2 - 50 = 50, a win 150, so opponent reach 0.4 tying and 0.6
+    losing gives 0.4 * 50 + 0.6 * 150 = 110 (street_solve.cu:886-903)."""
+    board = _cards(TINY_BOARD)
+    td = ob.build_tree(board, 100, 50, [0.5], 2, max_raises=2)
+    h0, tie, lose = _cards("6h 6s"), _cards("6c 6d"), _cards("Ac Ad")
+    assert ob.eval7(board + h0) == ob.eval7(board + tie) > ob.eval7(board + lose)
+    for mode in (ob.TERM_BRUTE, ob.TERM_FAST):
+        v = _tv(td, board, [[h0], [tie, lose]], [[1.0], [0.4, 0.6]], 8, 0, mode)
+        assert v[0] == pytest.approx(110.0, abs=1e-9)
+
+
+@pytest.mark.parametrize("mode", [ob.TERM_BRUTE, ob.TERM_FAST])
+def test_folded_traverser_at_showdown_gets_zero(mode):
+    """A player who folded earlier gets 0 at a later showdown even though
+    chips it put in are in the pot (street_solve.cu:857): the value is not
+    -bet. The node is found in a deeper 3-player tree (stack 1000) as a
+    showdown where an inactive player has a positive bet."""
+    td = _three_player_tree(stack=1000)
+    board = _cards("Kc Kd 7h 7s 2c")
+    found = None
+    for n in range(td.num_nodes):
+        nd = td.nodes[n]
+        if nd.type == 2:
+            for p in range(3):
+                if not nd.active_players[p] and nd.bets[p] > 0:
+                    found = (n, p)
+                    break
+        if found:
+            break
+    assert found is not None
+    n, p = found
+    hands = [[_cards("Ac 3h")], [_cards("Qh Jh")], [_cards("3d 4d")]]
+    v = _tv(td, board, hands, [[1.0], [1.0], [1.0]], n, p, mode)
+    assert v[0] == 0.0 and td.nodes[n].bets[p] > 0
+
+
+@pytest.mark.parametrize("mode", [ob.TERM_BRUTE, ob.TERM_FAST])
+def test_fold_counts_folded_players_reach(mode):
+    """Node 6 (pot 150, P2 wins with P0 and P1 folded): traverser P2's value is
+    (150 - 50) times the joint reach of ALL non-traversers, folded ones
+    included, over card-disjoint pairs (street_solve.cu:735-743, 764-825).
+    P0 holds x (0.4) or y (0.6); P1 holds u (0.5) or v (0.25), v sharing a
+    card with x: mass = 0.4*0.5 + 0.6*(0.5 + 0.25) = 0.65, value 65.
+    Catches 'active opponents only' (100) and a missing opponent-opponent
+    conflict check (75)."""
+    td = _three_player_tree()
+    nd = td.nodes[6]
+    assert nd.type == 1 and nd.player == 2 and nd.pot == 150 and nd.bets[2] == 50
+    assert not nd.active_players[0] and not nd.active_players[1]
+    board = _cards("Kc Kd 7h 7s 2c")
+    x, y, u, v = _cards("Ah 9d"), _cards("Qs Jd"), _cards("Tc 8c"), _cards("9d 6s")
+    h2 = _cards("4c 5c")
+    val = _tv(td, board, [[x, y], [u, v], [h2]], [[0.4, 0.6], [0.5, 0.25], [1.0]], 6, 2, mode)
+    assert val[0] == pytest.approx(65.0, abs=1e-9)
+
+
+def test_leaf_value_with_no_active_opponent_has_mass_one():
+    """A leaf where no opponent of the traverser is active takes mass 1
+    (street_solve.cu:1056-1058): value = the traverser's leaf value. The
+    builder never makes such a leaf, so a two-player turn leaf is edited to
+    leave only the traverser active."""
+    board = _cards("2c 3d 4h 5s")
+    td = ob.build_tree(board, 100, 50, [0.5], 2, use_cont=0, max_raises=2)
+    leaf = next(n for n in range(td.num_nodes) if td.nodes[n].type == 4)
+    td.nodes[leaf].active_players[1] = 0
+    hands = [np.array([_cards("Ac Ad")], np.int32), np.array([_cards("Kc Kd")], np.int32)]
+    lv = np.zeros((td.num_leaves, 2, 1), np.float32)
+    lv[td.nodes[leaf].leaf_idx, 0, 0] = 37.5
+    prob = ob.Problem(td, board, hands, [np.ones(1, np.float32)] * 2, leaf_values=lv)
+    for mode in (ob.TERM_BRUTE, ob.TERM_FAST):
+        v = ob.terminal_values(prob, leaf, 0, [np.array([1.0]), np.array([0.25])], mode)
+        assert v[0] == pytest.approx(37.5, abs=1e-9)
+    td.nodes[leaf].active_players[1] = 1
+    for mode in (ob.TERM_BRUTE, ob.TERM_FAST):
+        v = ob.terminal_values(prob, leaf, 0, [np.array([1.0]), np.array([0.25])], mode)
+        assert v[0] == pytest.approx(37.5 * 0.25, abs=1e-9)
