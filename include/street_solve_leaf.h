@@ -61,6 +61,28 @@ int ss_leaf_equity(int num_players, const int *hands, const int *num_hands, int 
 int ss_leaf_values(int num_players, const double *raw_equity, const int *num_hands, int max_hands,
                    const int *leaf_idx, const int *leaf_pot, int num_leaf_infos, float *leaf_values);
 
+/* Allocation-free variant of ss_leaf_equity (same arguments, same results
+ * bit for bit). No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code between HIP graph captures, tight
+ * serving loops, hosts that forbid hipMalloc). Every device buffer of the
+ * call (uploaded hand and runout lists, per-(runout, player) strengths,
+ * tuple records, per-runout equity) lives in one fixed array of
+ * SS_LEAF_STATIC_SCRATCH_BYTES compiled into the library, one copy per GPU,
+ * held until the process ends; the call makes no hipMalloc, hipFree,
+ * hipHostMalloc or other allocating call and reads the result back with a
+ * plain hipMemcpy. A call that would need more scratch (the need grows with
+ * runouts x players x max_hands and the 2000-tuple records) fails with
+ * SS_ERR_ARG and says how much it needs; a 3-player flop over 144 runouts at
+ * 1176 hands needs about 35 MB. All calls on one GPU share the array:
+ * they are serialised by a per-device lock here, and any other code that
+ * used the same array concurrently would race with them. The array keeps
+ * the last call's contents; every element a call reads was written earlier
+ * in that call. */
+#define SS_LEAF_STATIC_SCRATCH_BYTES (64u << 20)
+
+int ss_leaf_equity_static(int num_players, const int *hands, const int *num_hands, int max_hands,
+                          const int *runouts, int num_runouts, double *raw_equity);
+
 #ifdef __cplusplus
 }
 #endif

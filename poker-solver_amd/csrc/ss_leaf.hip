@@ -42,6 +42,7 @@
 #include <cstring>
 #include <mutex>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 #include "ss_eval.h"
@@ -405,7 +406,34 @@ int check_hands(int np, const int *hands, const int *nh, int mh) {
 
 }  // namespace
 
-int leaf_equity(int np, const int *hands, const int *nh, int mh, const int *runouts, int nrun, double *raw) {
+// Host half of an equity call: argument checks, the packed hole cards, the
+// unblocked hand list of every (runout, player), the tuple source of every
+// (runout, hero) and the record offsets.
+struct EquityPlan {
+    int np = 0, mh = 0, nrun = 0;
+    std::vector<uint16_t> hc;
+    std::vector<int16_t> valid;
+    std::vector<RpDesc> desc;
+    std::vector<int16_t> pool;
+    long long nrec = 0;
+    size_t nrp() const { return (size_t)nrun * np; }
+    size_t nall() const { return nrp() * mh; }
+};
+
+// Device buffers of one equity call.
+struct EquityBufs {
+    uint16_t *hc;
+    int *run;
+    int16_t *valid, *pool;
+    RpDesc *desc;
+    uint32_t *str;
+    double *eq, *raw;
+    uint8_t *cnt;
+    TupleRec *recs;
+};
+
+int plan_equity(int np, const int *hands, const int *nh, int mh, const int *runouts, int nrun, const double *raw,
+                EquityPlan &pl) {
     int rc = check_hands(np, hands, nh, mh);
     if (rc) return rc;
     if (nrun < 0 || (nrun > 0 && !runouts)) return set_error(SS_ERR_ARG, "bad runouts");
@@ -420,17 +448,20 @@ int leaf_equity(int np, const int *hands, const int *nh, int mh, const int *runo
             m |= 1ull << c;
         }
     }
+    pl.np = np;
+    pl.mh = mh;
+    pl.nrun = nrun;
     const int no = np - 1;
-    std::vector<uint16_t> hc((size_t)np * mh, 0xffffu);
+    pl.hc.assign((size_t)np * mh, 0xffffu);
     for (int p = 0; p < np; ++p)
         for (int h = 0; h < nh[p]; ++h)
-            hc[p * mh + h] = (uint16_t)(hands[(p * mh + h) * 2] | hands[(p * mh + h) * 2 + 1] << 8);
+            pl.hc[p * mh + h] = (uint16_t)(hands[(p * mh + h) * 2] | hands[(p * mh + h) * 2 + 1] << 8);
 
     // Unblocked hand lists per (runout, player) and the tuple source per (runout, hero).
-    std::vector<int16_t> valid((size_t)nrun * np * mh, 0);
+    pl.valid.assign((size_t)nrun * np * mh, 0);
     std::vector<int> vlen((size_t)nrun * np, 0);
-    std::vector<RpDesc> desc((size_t)nrun * np);
-    std::vector<int16_t> pool;
+    pl.desc.assign((size_t)nrun * np, RpDesc{});
+    pl.pool.clear();
     std::map<std::vector<int>, int> patterns;
     for (int r = 0; r < nrun; ++r) {
         uint64_t bm = 0;
@@ -438,13 +469,14 @@ int leaf_equity(int np, const int *hands, const int *nh, int mh, const int *runo
         for (int p = 0; p < np; ++p) {
             int n = 0;
             for (int h = 0; h < nh[p]; ++h) {
-                uint16_t v = hc[p * mh + h];
-                if (!((bm >> (v & 0xff)) & 1) && !((bm >> (v >> 8)) & 1)) valid[((size_t)r * np + p) * mh + n++] = (int16_t)h;
+                uint16_t v = pl.hc[p * mh + h];
+                if (!((bm >> (v & 0xff)) & 1) && !((bm >> (v >> 8)) & 1))
+                    pl.valid[((size_t)r * np + p) * mh + n++] = (int16_t)h;
             }
             vlen[r * np + p] = n;
         }
         for (int p = 0; p < np; ++p) {
-            RpDesc &d = desc[r * np + p];
+            RpDesc &d = pl.desc[r * np + p];
             d = RpDesc{0, -1, 0, {0, 0, 0, 0, 0}};
             if (!vlen[r * np + p]) continue;
             std::vector<int> lens;
@@ -462,22 +494,57 @@ int leaf_equity(int np, const int *hands, const int *nh, int mh, const int *runo
             }
             auto it = patterns.find(lens);
             if (it == patterns.end()) {
-                int off = (int)pool.size();
+                int off = (int)pl.pool.size();
                 PyRandom rng(SS_LEAF_SAMPLE_SEED);
                 for (int t = 0; t < SS_LEAF_OPP_COMBO_LIMIT; ++t)
-                    for (int k = 0; k < no; ++k) pool.push_back((int16_t)rng.below(lens[k]));
+                    for (int k = 0; k < no; ++k) pl.pool.push_back((int16_t)rng.below(lens[k]));
                 it = patterns.emplace(lens, off).first;
             }
             d.count = SS_LEAF_OPP_COMBO_LIMIT;
             d.pattern = it->second;
         }
     }
-
-    long long nrec = 0;
-    for (RpDesc &d : desc) {
-        d.rec = nrec;
-        nrec += d.count;
+    pl.nrec = 0;
+    for (RpDesc &d : pl.desc) {
+        d.rec = pl.nrec;
+        pl.nrec += d.count;
     }
+    return SS_OK;
+}
+
+// Device half: uploads, the four kernels, the read-back of the equity. Every
+// element a kernel reads was written earlier in the same call, so buffers
+// need no clearing (they may hold an earlier call's contents).
+int run_equity(const EquityPlan &pl, const int *runouts, const EquityBufs &b, double *raw) {
+    const int np = pl.np, mh = pl.mh, nrun = pl.nrun;
+    const size_t nrp = pl.nrp(), nall = pl.nall();
+    auto up = [](void *dst, const void *src, size_t bytes) {
+        return bytes ? hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) : hipSuccess;
+    };
+    SS_HIP(up(b.hc, pl.hc.data(), pl.hc.size() * sizeof(uint16_t)));
+    SS_HIP(up(b.run, runouts, (size_t)nrun * 5 * sizeof(int)));
+    SS_HIP(up(b.valid, pl.valid.data(), pl.valid.size() * sizeof(int16_t)));
+    SS_HIP(up(b.desc, pl.desc.data(), pl.desc.size() * sizeof(RpDesc)));
+    SS_HIP(up(b.pool, pl.pool.data(), pl.pool.size() * sizeof(int16_t)));
+    k_leaf_strength<<<(unsigned)((nall + LEAF_BLOCK - 1) / LEAF_BLOCK), LEAF_BLOCK>>>(b.hc, b.run, b.str, np, mh,
+                                                                                     (int)nall);
+    SS_HIP(hipGetLastError());
+    k_leaf_tuples<<<dim3((SS_LEAF_OPP_COMBO_LIMIT + LEAF_BLOCK - 1) / LEAF_BLOCK, (unsigned)nrp), LEAF_BLOCK>>>(
+        b.hc, b.str, b.valid, b.desc, b.pool, b.recs, np, mh);
+    SS_HIP(hipGetLastError());
+    k_leaf_equity<<<dim3((mh + EQ_BLOCK - 1) / EQ_BLOCK, (unsigned)nrp), EQ_BLOCK>>>(b.hc, b.str, b.desc, b.recs,
+                                                                                  b.eq, b.cnt, np, mh);
+    SS_HIP(hipGetLastError());
+    k_leaf_reduce<<<(np * mh + LEAF_BLOCK - 1) / LEAF_BLOCK, LEAF_BLOCK>>>(b.eq, b.cnt, b.raw, nrun, np * mh);
+    SS_HIP(hipGetLastError());
+    SS_HIP(hipMemcpy(raw, b.raw, (size_t)np * mh * sizeof(double), hipMemcpyDeviceToHost));
+    return SS_OK;
+}
+
+int leaf_equity(int np, const int *hands, const int *nh, int mh, const int *runouts, int nrun, double *raw) {
+    EquityPlan pl;
+    int rc = plan_equity(np, hands, nh, mh, runouts, nrun, raw, pl);
+    if (rc) return rc;
     if (nrun == 0) {
         std::fill(raw, raw + (size_t)np * mh, 0.0);
         return SS_OK;
@@ -485,37 +552,96 @@ int leaf_equity(int np, const int *hands, const int *nh, int mh, const int *runo
     LeafArena *A = nullptr;
     if ((rc = leaf_arena(&A))) return rc;
     std::lock_guard<std::mutex> lock(A->mu);
-    size_t nrp = (size_t)nrun * np, nall = nrp * mh;
-    uint16_t *d_hc;
-    int *d_run;
-    int16_t *d_valid, *d_pool;
-    RpDesc *d_desc;
-    uint32_t *d_str;
-    double *d_eq, *d_raw;
-    uint8_t *d_cnt;
-    TupleRec *d_recs;
-    if ((rc = arena_upload(*A, L_HC, hc.data(), hc.size(), &d_hc)) ||
-        (rc = arena_upload(*A, L_RUN, runouts, (size_t)nrun * 5, &d_run)) ||
-        (rc = arena_upload(*A, L_VALID, valid.data(), valid.size(), &d_valid)) ||
-        (rc = arena_upload(*A, L_DESC, desc.data(), desc.size(), &d_desc)) ||
-        (rc = arena_upload(*A, L_POOL, pool.data(), pool.size(), &d_pool)) ||
-        (rc = arena_buf(*A, L_STR, nall, &d_str)) || (rc = arena_buf(*A, L_EQ, nall, &d_eq)) ||
-        (rc = arena_buf(*A, L_CNT, nall, &d_cnt)) || (rc = arena_buf(*A, L_RAW, (size_t)np * mh, &d_raw)) ||
-        (rc = arena_buf(*A, L_RECS, (size_t)nrec, &d_recs)))
+    EquityBufs b;
+    if ((rc = arena_buf(*A, L_HC, pl.hc.size(), &b.hc)) || (rc = arena_buf(*A, L_RUN, (size_t)nrun * 5, &b.run)) ||
+        (rc = arena_buf(*A, L_VALID, pl.valid.size(), &b.valid)) ||
+        (rc = arena_buf(*A, L_DESC, pl.desc.size(), &b.desc)) || (rc = arena_buf(*A, L_POOL, pl.pool.size(), &b.pool)) ||
+        (rc = arena_buf(*A, L_STR, pl.nall(), &b.str)) || (rc = arena_buf(*A, L_EQ, pl.nall(), &b.eq)) ||
+        (rc = arena_buf(*A, L_CNT, pl.nall(), &b.cnt)) || (rc = arena_buf(*A, L_RAW, (size_t)np * mh, &b.raw)) ||
+        (rc = arena_buf(*A, L_RECS, (size_t)pl.nrec, &b.recs)))
         return rc;
-    k_leaf_strength<<<(unsigned)((nall + LEAF_BLOCK - 1) / LEAF_BLOCK), LEAF_BLOCK>>>(d_hc, d_run, d_str, np, mh,
-                                                                                     (int)nall);
-    SS_HIP(hipGetLastError());
-    k_leaf_tuples<<<dim3((SS_LEAF_OPP_COMBO_LIMIT + LEAF_BLOCK - 1) / LEAF_BLOCK, (unsigned)nrp), LEAF_BLOCK>>>(
-        d_hc, d_str, d_valid, d_desc, d_pool, d_recs, np, mh);
-    SS_HIP(hipGetLastError());
-    k_leaf_equity<<<dim3((mh + EQ_BLOCK - 1) / EQ_BLOCK, (unsigned)nrp), EQ_BLOCK>>>(d_hc, d_str, d_desc, d_recs,
-                                                                                  d_eq, d_cnt, np, mh);
-    SS_HIP(hipGetLastError());
-    k_leaf_reduce<<<(np * mh + LEAF_BLOCK - 1) / LEAF_BLOCK, LEAF_BLOCK>>>(d_eq, d_cnt, d_raw, nrun, np * mh);
-    SS_HIP(hipGetLastError());
-    SS_HIP(hipMemcpy(raw, d_raw, (size_t)np * mh * sizeof(double), hipMemcpyDeviceToHost));
+    return run_equity(pl, runouts, b, raw);
+}
+
+// ---------------------------------------------------------------- allocation-free equity
+// The device memory of ss_leaf_equity_static: a fixed array in the code
+// object, one copy per GPU, held until the process ends. Host code never
+// passes `scratch` itself to a kernel or copy (in host code the name is a
+// stand-in, not a device address); it uses the address hipGetSymbolAddress
+// returns for the current device.
+static __device__ __align__(256) unsigned char scratch[SS_LEAF_STATIC_SCRATCH_BYTES];
+
+struct StaticScratch {
+    std::once_flag once;
+    std::mutex mu;               // serialises calls on one device
+    unsigned char *ptr = nullptr;
+    hipError_t err = hipSuccess;
+};
+
+// Device address of this device's copy of `scratch`, resolved once per
+// device on its first call, with that device current.
+int static_scratch(StaticScratch **out) {
+    static StaticScratch per_dev[LEAF_MAX_DEVICES];
+    int dev = 0;
+    SS_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= LEAF_MAX_DEVICES) return set_error(SS_ERR_UNSUPPORTED, "device %d", dev);
+    StaticScratch &s = per_dev[dev];
+    std::call_once(s.once, [&s]() {
+        void *p = nullptr;
+        s.err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        s.ptr = s.err == hipSuccess ? static_cast<unsigned char *>(p) : nullptr;
+    });
+    if (s.err != hipSuccess || !s.ptr)
+        return set_error(SS_ERR_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(s.err));
+    *out = &s;
     return SS_OK;
+}
+
+// Carves the buffers of `pl` out of [base, base + cap) at 256-byte
+// alignment; returns the bytes needed (also when they exceed cap, with the
+// pointers then unset).
+size_t carve_static(const EquityPlan &pl, unsigned char *base, size_t cap, EquityBufs &b) {
+    size_t off = 0;
+    auto take = [&](auto *&ptr, size_t n) {
+        using T = std::remove_pointer_t<std::remove_reference_t<decltype(ptr)>>;
+        off = (off + 255) & ~(size_t)255;
+        size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+        ptr = off + bytes <= cap ? reinterpret_cast<T *>(base + off) : nullptr;
+        off += bytes;
+    };
+    take(b.hc, pl.hc.size());
+    take(b.run, (size_t)pl.nrun * 5);
+    take(b.valid, pl.valid.size());
+    take(b.desc, pl.desc.size());
+    take(b.pool, pl.pool.size());
+    take(b.str, pl.nall());
+    take(b.eq, pl.nall());
+    take(b.cnt, pl.nall());
+    take(b.raw, (size_t)pl.np * pl.mh);
+    take(b.recs, (size_t)pl.nrec);
+    return off;
+}
+
+int leaf_equity_static(int np, const int *hands, const int *nh, int mh, const int *runouts, int nrun, double *raw) {
+    EquityPlan pl;
+    int rc = plan_equity(np, hands, nh, mh, runouts, nrun, raw, pl);
+    if (rc) return rc;
+    EquityBufs b;
+    const size_t need = carve_static(pl, nullptr, 0, b);
+    if (need > SS_LEAF_STATIC_SCRATCH_BYTES)
+        return set_error(SS_ERR_ARG,
+                         "equity call needs %zu bytes of device scratch, more than the fixed %zu "
+                         "(%d runouts x %d players x %d hands, %lld tuple records); use ss_leaf_equity",
+                         need, (size_t)SS_LEAF_STATIC_SCRATCH_BYTES, nrun, np, mh, pl.nrec);
+    if (nrun == 0) {
+        std::fill(raw, raw + (size_t)np * mh, 0.0);
+        return SS_OK;
+    }
+    StaticScratch *s = nullptr;
+    if ((rc = static_scratch(&s))) return rc;
+    std::lock_guard<std::mutex> lock(s->mu);
+    carve_static(pl, s->ptr, SS_LEAF_STATIC_SCRATCH_BYTES, b);
+    return run_equity(pl, runouts, b, raw);
 }
 
 int leaf_values(int np, const double *raw, const int *nh, int mh, const int *leaf_idx, const int *leaf_pot,
@@ -584,6 +710,13 @@ int ss_leaf_values(int num_players, const double *raw_equity, const int *num_han
     ss::clear_error();
     return ss::leaf_values(num_players, raw_equity, num_hands, max_hands, leaf_idx, leaf_pot, num_leaf_infos,
                            leaf_values);
+}
+
+int ss_leaf_equity_static(int num_players, const int *hands, const int *num_hands, int max_hands,
+                          const int *runouts, int num_runouts, double *raw_equity) {
+    ss::clear_error();
+    if (!hands) return ss::set_error(SS_ERR_ARG, "hands is NULL");
+    return ss::leaf_equity_static(num_players, hands, num_hands, max_hands, runouts, num_runouts, raw_equity);
 }
 
 int ssdbg_py_choice(int seed, const int *lens, int num_lists, int count, int *out) {

@@ -178,6 +178,8 @@ SIGNATURES = {
     "ss_problem_footprint": (ctypes.c_int64, [_P(SSProblem)]),
     "ss_leaf_equity": (ctypes.c_int, [ctypes.c_int, _P(ctypes.c_int), _P(ctypes.c_int), ctypes.c_int,
                                       _P(ctypes.c_int), ctypes.c_int, _P(ctypes.c_double)]),
+    "ss_leaf_equity_static": (ctypes.c_int, [ctypes.c_int, _P(ctypes.c_int), _P(ctypes.c_int), ctypes.c_int,
+                                             _P(ctypes.c_int), ctypes.c_int, _P(ctypes.c_double)]),
     "ss_leaf_values": (ctypes.c_int, [ctypes.c_int, _P(ctypes.c_double), _P(ctypes.c_int), ctypes.c_int,
                                       _P(ctypes.c_int), _P(ctypes.c_int), ctypes.c_int, _P(ctypes.c_float)]),
     "ss_range_update": (ctypes.c_int, [ctypes.c_int, _P(ctypes.c_double), _P(ctypes.c_double),
