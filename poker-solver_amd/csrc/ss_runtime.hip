@@ -368,7 +368,8 @@ Slot::~Slot() {
 
 int64_t estimate_footprint(const HostProblem &hp) {
     Plan pl;
-    if (make_plan(hp, pl) != SS_OK) return -1;
+    int rc = make_plan(hp, pl);
+    if (rc != SS_OK) return rc;   // the planner's SS_ERR_* code (message in ss_last_error)
     size_t Mp = pl.Mp;
     int64_t b = 0;
     b += (int64_t)pl.N * sizeof(DevNode) + pl.children.size() * 4 + (pl.D + pl.L + 1 + pl.T) * 4;
