@@ -461,12 +461,165 @@ def _profile_leaf(flop, reps=10):
     return 0
 
 
+# ---------------------------------------------------------------- error vs the FP64 oracle
+def _fp64_solution(args):
+    """FP64 oracle solve of a workloads spot: (average strategy [N][8][max_h],
+    exploitability % pot). Runs in a worker process (one OpenMP thread)."""
+    kind, seed, iters = args
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import oracle_bind as ob
+    import workloads as wl
+    spot = wl.river_2p(seed=seed) if kind == "river_2p" else wl.river_3p(seed=seed)
+    td = ob.build_tree(spot.board, spot.pot, spot.stack, spot.bet_sizes, spot.num_players,
+                       spot.acting_order, max_raises=spot.max_raises)
+    prob = ob.Problem(td, spot.board, spot.hands, spot.weights)
+    r = ob.solve(prob, iters, fp64=True, mode=ob.TERM_FAST)
+    e, _ = ob.exploitability(prob, r["avg"], mode=ob.TERM_FAST)
+    return np.asarray(r["avg"]), float(e)
+
+
+def _own_reach(td, avg, weights, mh):
+    """pi[p, n, h]: player p's own reach at node n for hand h under strategy
+    avg ([N][8][mh]), starting from the raw range weights."""
+    npl = len(weights)
+    pi = np.zeros((npl, td.num_nodes, mh))
+    for p in range(npl):
+        pi[p, 0, :len(weights[p])] = weights[p]
+    for i in range(td.num_nodes):
+        n = td.level_order[i]
+        nd = td.nodes[n]
+        if nd.type != 0:
+            continue
+        for a in range(nd.num_children):
+            c = td.children[nd.first_child + a]
+            pi[:, c] = pi[:, n]
+            pi[nd.player, c] *= avg[n, a]
+    return pi
+
+
+def strategy_error(td, avg, ref, weights, hands):
+    """Average-strategy error of `avg` against `ref` (both [N][8][mh]) over
+    the info-set rows (decision node, hand of its actor): the mean of
+    max_a |avg - ref| weighted by the actor's own reach under ref, the
+    fraction of rows with positive reach whose error exceeds 1e-4, and the
+    largest error on such a row."""
+    mh = ref.shape[2]
+    pi = _own_reach(td, ref, weights, mh)
+    num = den = 0.0
+    reached = over = 0
+    worst = 0.0
+    for n in range(td.num_nodes):
+        nd = td.nodes[n]
+        if nd.type != 0:
+            continue
+        p, na, nh = nd.player, nd.num_children, len(hands[nd.player])
+        err = np.abs(avg[n, :na, :nh].astype(np.float64) - ref[n, :na, :nh]).max(axis=0)
+        w = pi[p, n, :nh]
+        live = w > 0
+        num += float((w * err).sum())
+        den += float(w.sum())
+        reached += int(live.sum())
+        over += int((err[live] > 1e-4).sum())
+        if live.any():
+            worst = max(worst, float(err[live].max()))
+    return {"reach_weighted_abs": num / den if den > 0 else 0.0, "reached_rows": reached,
+            "frac_reached_rows_above_1e-4": over / max(1, reached), "max_abs_reached": worst}
+
+
+def _dequant_reference(path, td, hands, mh):
+    z = np.load(path)
+    ref = np.zeros((td.num_nodes, 8, mh))
+    off = 0
+    q = z["avg_q16"].astype(np.float64) / 65535.0
+    for n, na in zip(z["node_ids"], z["num_actions"]):
+        nh = len(hands[td.nodes[int(n)].player])
+        ref[n, :na, :nh] = q[off:off + na * nh].reshape(na, nh)
+        off += na * nh
+    return ref, float(z["exploitability"]), int(z["iterations"])
+
+
+def error_report(a):
+    """FP32 and BF16x2 solutions against the FP64 oracle at the configs' own
+    iteration counts, per subgame: the 2-player 1081-combo river at 500
+    iterations (8 seeds), the 64 precision-form boards at 1000 iterations
+    (BF16 hi/lo MFMA showdown and the FP32 closed form), and the 3-player
+    river at 1000 iterations (FP64 reference precomputed in
+    tests/golden/river_3p_fp64_1000it.npz by tests/golden/make_fp64_reference.py).
+    Exploitability is each solution's FP64 value (% pot); relative error =
+    |e - e_fp64| / e_fp64."""
+    import ss_abi
+    import ss_api
+    import workloads as wl
+    cores = a.cpu_cores or min(16, os.cpu_count() or 1)
+    os.environ["OMP_NUM_THREADS"] = "1"
+    base = 20261203 + 1000
+    rep = {"cores": cores}
+    t0 = time.perf_counter()
+    # GPU solutions
+    seeds3 = [base + b for b in range(64)]
+    probs = [ss_api.Problem.from_spot(wl.river_2p(seed=sd)) for sd in seeds3]
+    gpu = {}
+    for name, prec in (("bf16x2", ss_abi.SS_PREC_BF16X2), ("fp32", ss_abi.SS_PREC_FP32)):
+        b = ss_api.Batch(probs, 1000, precision=prec)
+        reps, _ = b.run()
+        gpu[name] = [b.output(i, reps[i]) for i in range(len(probs))]
+        b.close()
+    gpu1 = [ss_api.solve_plain(probs[i], 500) for i in range(8)]
+    log(f"[errors] GPU solves done ({time.perf_counter() - t0:.1f} s); FP64 oracle on {cores} processes")
+    jobs = [("river_2p", sd, 1000) for sd in seeds3] + [("river_2p", seeds3[i], 500) for i in range(8)]
+    rows3, rows1 = [], []
+    with mp.get_context("spawn").Pool(cores) as pool:
+        for k, (ref, e_ref) in enumerate(pool.imap(_fp64_solution, jobs)):
+            i = k if k < 64 else k - 64
+            p = probs[i]
+            td, hands, w = p.tree.td, p.hands, p.weights
+            if k < 64:
+                row = {"board": i, "expl_fp64": e_ref}
+                for name in ("bf16x2", "fp32"):
+                    r = gpu[name][i]
+                    row[name] = {"expl": r.exploitability,
+                                 "expl_rel_err": abs(r.exploitability - e_ref) / e_ref,
+                                 **strategy_error(td, r.avg_full(td.num_nodes), ref, w, hands)}
+                rows3.append(row)
+            else:
+                r = gpu1[i]
+                rows1.append({"subgame": i, "expl_fp64": e_ref, "expl": r.exploitability,
+                              "expl_rel_err": abs(r.exploitability - e_ref) / e_ref,
+                              **strategy_error(td, r.avg_full(td.num_nodes), ref, w, hands)})
+            log(f"[errors] {'board' if k < 64 else 'river_2p 500-it subgame'} {i} done "
+                f"({time.perf_counter() - t0:.1f} s)")
+    rep["precision_boards_1000it"] = rows3
+    rep["river_2p_500it"] = rows1
+    sp3 = wl.river_3p()
+    p3 = ss_api.Problem.from_spot(sp3)
+    r3 = ss_api.solve_plain(p3, 1000)
+    ref3, e3, it3 = _dequant_reference(os.path.join(REPO, "tests", "golden", "river_3p_fp64_1000it.npz"),
+                                       p3.tree.td, p3.hands, p3.max_h)
+    rep["river_3p_1000it"] = {"iterations": it3, "expl_fp64": e3, "expl": r3.exploitability,
+                              "expl_rel_err": abs(r3.exploitability - e3) / e3,
+                              **strategy_error(p3.tree.td, r3.avg_full(p3.tree.num_nodes), ref3, p3.weights,
+                                               p3.hands)}
+
+    def summ(rows, key=None):
+        get = (lambda r: r[key]) if key else (lambda r: r)
+        out = {}
+        for f in ("expl_rel_err", "reach_weighted_abs", "frac_reached_rows_above_1e-4", "max_abs_reached"):
+            v = np.array([get(r)[f] for r in rows])
+            out[f] = {"mean": float(v.mean()), "max": float(v.max())}
+        return out
+    rep["summary"] = {"precision_boards_bf16x2": summ(rows3, "bf16x2"), "precision_boards_fp32": summ(rows3, "fp32"),
+                      "river_2p_500it": summ(rows1), "river_3p_1000it": rep["river_3p_1000it"]}
+    print(json.dumps(rep), flush=True)
+    return 0
+
+
 PROFILE_TARGETS = {
     "traffic": profile_traffic,
     "precision": profile_precision,
     "served": profile_served,
     "leaf": lambda a: _profile_leaf(False),
     "leaf-flop": lambda a: _profile_leaf(True),
+    "error-report": error_report,
 }
 
 

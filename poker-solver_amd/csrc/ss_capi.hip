@@ -9,6 +9,7 @@
 #include "../../include/street_solve_debug.h"
 #include "ss_eval.h"
 #include "ss_internal.h"
+#include "../../include/street_solve_range.h"
 
 namespace ss {
 const char *last_error_str();
@@ -534,6 +535,38 @@ int ss_batch_output(SSBatch *batch, int index, SSOutput *output) {
 }
 
 void ss_batch_destroy(SSBatch *batch) { delete batch; }
+
+int ss_batch_range_update(SSBatch *batch, int index, int node, int action, int num_hands, const int *solver_hand,
+                          double *weights, double weight_floor, double p_threshold) {
+    clear_error();
+    if (!batch || index < 0 || index >= batch->impl.n) return set_error(SS_ERR_ARG, "bad batch or subgame index");
+    if (batch->impl.iters.empty()) return set_error(SS_ERR_ARG, "the batch has not been run");
+    if (num_hands < 0 || num_hands > SS_RANGE_MAX_HANDS)
+        return set_error(SS_ERR_ARG, "num_hands %d (0..%d)", num_hands, SS_RANGE_MAX_HANDS);
+    if (num_hands > 0 && (!solver_hand || !weights)) return set_error(SS_ERR_ARG, "NULL solver_hand or weights");
+    const Slot &s = *batch->impl.slots[index];
+    const Plan &pl = s.plan;
+    if (node < 0 || node >= pl.N || pl.nodes[node].type != SS_NODE_DECISION)
+        return set_error(SS_ERR_ARG, "node %d is not a decision node", node);
+    const DevNode &nd = pl.nodes[node];
+    const int p = nd.player, nsolver = pl.nh[p];
+    if (action < -1 || action >= nd.nchild) return set_error(SS_ERR_ARG, "action %d (node has %d)", action, nd.nchild);
+    for (int h = 0; h < num_hands; ++h)
+        if (solver_hand[h] < -1 || solver_hand[h] >= nsolver)
+            return set_error(SS_ERR_ARG, "solver_hand[%d] = %d (player %d has %d hands)", h, solver_hand[h], p,
+                             nsolver);
+    // the probability dict of the host path holds one entry per combo: the
+    // solve's hand list must not repeat one
+    std::vector<char> seen(SS_NCOMBO, 0);
+    for (int h = 0; h < nsolver; ++h) {
+        char &f = seen[(pl.hinfo[(size_t)p * pl.Mp + h] >> 12) & 2047];
+        if (f) return set_error(SS_ERR_ARG, "player %d's hands repeat a combo", p);
+        f = 1;
+    }
+    if (num_hands == 0) return SS_OK;
+    return range_update_from_strategy(s.desc.SS, nd.act_base, nd.nchild, action, pl.Mp, nsolver, solver_hand,
+                                      weights, num_hands, weight_floor, p_threshold, batch->impl.st);
+}
 
 int64_t ss_problem_footprint(const SSProblem *problem) {
     clear_error();

@@ -113,6 +113,11 @@ struct Slot {
 
 int64_t estimate_footprint(const HostProblem &hp);
 
+// Range narrowing from a resident solve's strategy sums (ss_range.hip).
+int range_update_from_strategy(const float *d_SS, int row0, int na, int action, int Mp, int nsolver,
+                               const int *solver_hand, double *weights, int n, double floor_w, double threshold,
+                               hipStream_t st);
+
 // plain form
 int plain_run(Slot &s, int iters, int final_iter, hipStream_t st, double *kernel_ms);
 int plain_exploitability(Slot &s, hipStream_t st, double *expl_pct, double *gains);

@@ -68,6 +68,35 @@ class RangeNarrower:
             self._ranges[player] = [(c0, c1, float(x)) for (c0, c1, _), x in zip(hands, w)]
         self._action_log.setdefault(player, []).append(action)
 
+    def update_from_solution(self, player, action, solver, solver_player, action_label=None):
+        """update(player, action, solver.get_avg_strategy_probs(solver_player,
+        label)) with P(label | hand) read on the GPU from the solve's resident
+        strategy sums (StreetSolverGPU(..., resident=True)) instead of a host
+        dictionary: ss_batch_range_update normalises the strategy sums at the
+        solver player's first decision node as the solver's output does, cuts
+        them at 0.001, and applies this update; the weights equal the host
+        path's bit for bit. label defaults to action."""
+        return solver.narrow_range(self, player, solver_player, action if action_label is None else action_label)
+
+    def _device_update(self, player, action, batch, node, action_index, solver_keys, threshold):
+        if player not in self._ranges:
+            return
+        hands = self._ranges[player]
+        n = len(hands)
+        if n:
+            w = np.array([h[2] for h in hands], dtype=np.float64)
+            # the host path looks each hand up by its (c0, c1) key as stored
+            sh = np.array([solver_keys.get((c0, c1), -1) for c0, c1, _ in hands], dtype=np.int32)
+            lib = ss_abi.load_library()
+            rc = lib.ss_batch_range_update(batch, 0, int(node), int(action_index), n,
+                                           sh.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                           w.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                           self.WEIGHT_FLOOR, float(threshold))
+            if rc != 0:
+                raise RuntimeError(f"ss_batch_range_update failed ({rc}): {ss_abi.last_error(lib)}")
+            self._ranges[player] = [(c0, c1, float(x)) for (c0, c1, _), x in zip(hands, w)]
+        self._action_log.setdefault(player, []).append(action)
+
     def remove_folded_hands(self, player, blueprint_probs):
         """Drop hands whose P(fold | hand) is at least 0.95."""
         if player not in self._ranges:

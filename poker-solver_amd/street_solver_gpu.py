@@ -26,7 +26,9 @@ leaf-value or frozen-action buffers (the reference frees leaf_values at
 src/cuda/street_solve.cu:1650, a buffer this object owns).
 
 extended=True lifts the 200-combo cap (up to 1326) and solves through
-ss_solve_gpu_ex, which also reports the FP64 exploitability.
+ss_solve_gpu_ex, which also reports the FP64 exploitability. resident=True
+(implies extended) keeps the solve on the GPU as a batch session, so that
+range narrowing can read the average strategy there (narrow_range).
 """
 
 import ctypes
@@ -161,11 +163,15 @@ class _NativeHandles:
         self.output = SSOutput()
         self.tree_built = False
         self.output_live = False
+        self.batch = None          # resident solve (ss_batch_* session), resident=True only
 
     def drop_output(self):
         if self.output_live:
             self.output_live = False
             self.lib.ss_free_output(ctypes.byref(self.output))
+        if self.batch:
+            batch, self.batch = self.batch, None
+            self.lib.ss_batch_destroy(batch)
 
     def release(self):
         self.drop_output()
@@ -210,9 +216,10 @@ class StreetSolverGPU:
 
     def __init__(self, board, pot_bb, stack_bb, oop_range=None, ip_range=None, player_ranges=None,
                  acting_order=None, bet_sizes=None, leaf_values=None, use_cont_strats=True,
-                 leaf_value_fn=None, extended=False, max_raises=None):
+                 leaf_value_fn=None, extended=False, max_raises=None, resident=False):
         self.lib = _get_lib()
-        self.extended = bool(extended)
+        self.resident = bool(resident)
+        self.extended = bool(extended) or self.resident
         self._freed = False
         self._solved = False
         self._sol = None
@@ -331,7 +338,9 @@ class StreetSolverGPU:
             raise RuntimeError("solver already freed")
         self._handles.drop_output()       # a second solve replaces the first output
         self._sol, self._solved = None, False
-        if self.extended:
+        if self.resident:
+            rc = self._solve_resident(int(iterations))
+        elif self.extended:
             cfg = SSSolveConfig()
             cfg.max_iterations = int(iterations)
             rc = self.lib.ss_solve_gpu_ex(ctypes.byref(self._problem()), ctypes.byref(cfg),
@@ -344,6 +353,51 @@ class StreetSolverGPU:
         self._sol = _Solution(self._output, self._view, [len(h) for h in self.player_hands],
                               self._max_hands)
         self._solved = True
+
+    def _solve_resident(self, iterations):
+        """Solve as a one-subgame batch session whose state stays on the GPU
+        (ss_batch_create / ss_batch_run / ss_batch_output) until the next
+        solve or free(), so later steps can read it there."""
+        cfg = SSSolveConfig()
+        cfg.max_iterations = iterations
+        prob = self._problem()
+        batch = self.lib.ss_batch_create(ctypes.byref(prob), 1, ctypes.byref(cfg))
+        if not batch:
+            return -1
+        self._handles.batch = batch
+        rc = self.lib.ss_batch_run(batch, ctypes.byref(self._report), ctypes.byref(ss_abi.SSBatchStats()))
+        if rc == 0:
+            rc = self.lib.ss_batch_output(batch, 0, ctypes.byref(self._output))
+        return rc
+
+    def narrow_range(self, narrower, narrower_player, player, action_label, threshold=OUTPUT_THRESHOLD):
+        """RangeNarrower.update(narrower_player, action_label,
+        self.get_avg_strategy_probs(player, action_label)) computed on the GPU
+        from this solve's resident strategy sums (resident=True), with the
+        same result bit for bit; see RangeNarrower.update_from_solution."""
+        sol = self._require_solution()
+        if not self._handles.batch:
+            raise RuntimeError("narrow_range needs a resident solve (StreetSolverGPU(..., resident=True))")
+        pi = self._resolve_player(player)
+        row = sol.first_row_of_player.get(pi)
+        if row is None:
+            raise ValueError(f"player {pi} has no decision node")
+        node = int(sol.node_ids[row])
+        names = self._view.labels(node)
+        hits = [a for a, nm in enumerate(names) if nm == action_label]
+        if len(hits) > 1:
+            raise ValueError(f"label {action_label!r} names {len(hits)} actions at node {node}")
+        action = hits[0] if hits else -1
+        # hand h of the narrowed range -> index of the same (c0, c1) key in
+        # this solve's range (keys are (low, high) as get_avg_strategy builds them)
+        keys = {}
+        for i, h in enumerate(self.player_hands[pi]):
+            k = (min(h[0], h[1]), max(h[0], h[1]))
+            if k in keys:
+                raise ValueError(f"player {pi}'s range repeats combo {k}")
+            keys[k] = i
+        return narrower._device_update(narrower_player, action_label, self._handles.batch, node, action, keys,
+                                       threshold)
 
     @property
     def exploitability(self):
@@ -431,6 +485,7 @@ class StreetSolverGPU:
         object is garbage-collected)."""
         self._freed = True
         self._sol = None
+        self._solved = False
         fin = self.__dict__.get("_finalizer")
         if fin is not None:
             fin()
