@@ -478,6 +478,22 @@ def _fp64_solution(args):
     return np.asarray(r["avg"]), float(e)
 
 
+def _fp32_oracle_solution(args):
+    """FP32 oracle solve (a second FP32 implementation; mode = terminal mode)
+    of a 2-player river: (average strategy, its FP64 exploitability % pot)."""
+    seed, iters, mode = args
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import oracle_bind as ob
+    import workloads as wl
+    spot = wl.river_2p(seed=seed)
+    td = ob.build_tree(spot.board, spot.pot, spot.stack, spot.bet_sizes, spot.num_players,
+                       spot.acting_order, max_raises=spot.max_raises)
+    prob = ob.Problem(td, spot.board, spot.hands, spot.weights)
+    r = ob.solve(prob, iters, fp64=False, mode=mode)
+    e, _ = ob.exploitability(prob, r["avg"], mode=ob.TERM_FAST)
+    return np.asarray(r["avg"]), float(e)
+
+
 def _own_reach(td, avg, weights, mh):
     """pi[p, n, h]: player p's own reach at node n for hand h under strategy
     avg ([N][8][mh]), starting from the raw range weights."""
@@ -567,13 +583,14 @@ def error_report(a):
     gpu1 = [ss_api.solve_plain(probs[i], 500) for i in range(8)]
     log(f"[errors] GPU solves done ({time.perf_counter() - t0:.1f} s); FP64 oracle on {cores} processes")
     jobs = [("river_2p", sd, 1000) for sd in seeds3] + [("river_2p", seeds3[i], 500) for i in range(8)]
-    rows3, rows1 = [], []
+    rows3, rows1, refs = [], [], {}
     with mp.get_context("spawn").Pool(cores) as pool:
         for k, (ref, e_ref) in enumerate(pool.imap(_fp64_solution, jobs)):
             i = k if k < 64 else k - 64
             p = probs[i]
             td, hands, w = p.tree.td, p.hands, p.weights
             if k < 64:
+                refs[i] = (ref, e_ref)
                 row = {"board": i, "expl_fp64": e_ref}
                 for name in ("bf16x2", "fp32"):
                     r = gpu[name][i]
@@ -588,6 +605,16 @@ def error_report(a):
                               **strategy_error(td, r.avg_full(td.num_nodes), ref, w, hands)})
             log(f"[errors] {'board' if k < 64 else 'river_2p 500-it subgame'} {i} done "
                 f"({time.perf_counter() - t0:.1f} s)")
+    # the FP32 envelope: the oracle's own FP32 solve of the same boards
+    # (closed-form terminals) against the same FP64 solutions
+    with mp.get_context("spawn").Pool(cores) as pool:
+        jobs32 = [(sd, 1000, 1) for sd in seeds3]
+        for i, (avg32, e32) in enumerate(pool.imap(_fp32_oracle_solution, jobs32)):
+            ref, e_ref = refs[i]
+            p = probs[i]
+            rows3[i]["fp32_oracle_closed_form"] = {"expl": e32, "expl_rel_err": abs(e32 - e_ref) / e_ref,
+                                                   **strategy_error(p.tree.td, avg32, ref, p.weights, p.hands)}
+            log(f"[errors] FP32 oracle board {i} done ({time.perf_counter() - t0:.1f} s)")
     rep["precision_boards_1000it"] = rows3
     rep["river_2p_500it"] = rows1
     sp3 = wl.river_3p()
@@ -608,6 +635,7 @@ def error_report(a):
             out[f] = {"mean": float(v.mean()), "max": float(v.max())}
         return out
     rep["summary"] = {"precision_boards_bf16x2": summ(rows3, "bf16x2"), "precision_boards_fp32": summ(rows3, "fp32"),
+                      "precision_boards_fp32_oracle_closed_form": summ(rows3, "fp32_oracle_closed_form"),
                       "river_2p_500it": summ(rows1), "river_3p_1000it": rep["river_3p_1000it"]}
     print(json.dumps(rep), flush=True)
     return 0

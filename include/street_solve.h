@@ -208,6 +208,11 @@ typedef struct {
     int max_iterations;          /* iterations of the slowest subgame */
     int mode;                    /* strategy the autotuner chose: 0 persistent kernel with one
                                     workgroup per subgame, 1 phased launches over all subgames */
+    /* launch shape the autotuner chose for the phased form (-1 / 0: the automatic rule):
+       forward levels with >= this many tasks run three per workgroup; 0 = terminal classes
+       in separate launches; workgroups per two-opponent showdown; sweep workgroup cap */
+    int shape_fwd_group_min, shape_term_small, shape_sd_parts, shape_sweep_grid;
+    double tune_ms;              /* time ss_batch_create spent tuning the shape (0 when cached) */
 } SSBatchStats;
 
 SSBatch *ss_batch_create(const SSProblem *problems, int n, const SSSolveConfig *config);

@@ -3,7 +3,10 @@
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
+#include <array>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "../../include/street_solve_debug.h"
@@ -250,8 +253,11 @@ struct BatchImpl {
     int choose_mode() {
         const char *env = getenv("SS_BATCH_MODE");
         if (env && !strcmp(env, "persistent")) { mode = 0; return SS_OK; }
-        if (env && !strcmp(env, "phased")) { mode = 1; return SS_OK; }
-        if (cfg.precision == SS_PREC_BF16X2) { mode = 1; return SS_OK; }   // the contraction is phased only
+        if ((env && !strcmp(env, "phased")) || cfg.precision == SS_PREC_BF16X2) {   // the contraction is phased only
+            mode = 1;
+            int rc = tune_shape();
+            return rc ? rc : reset_state();
+        }
         if (cfg.max_iterations < 3) { mode = 1; return SS_OK; }
         float best = 1e30f;
         for (int m = 0; m < 2; ++m) {
@@ -271,7 +277,89 @@ struct BatchImpl {
             if (log_level() > 0) fprintf(stderr, "[street_solve] batch mode %d: %.3f ms / 2 iterations\n", m, ms);
             if (ms < best) { best = ms; mode = m; }
         }
+        if (mode == 1) {
+            int rc = tune_shape();
+            if (rc) return rc;
+        }
         return reset_state();
+    }
+
+    // Launch-shape autotuner of the phased form: coordinate descent over the
+    // forward-sweep grouping, the merged terminal launch, the two-opponent
+    // showdown split and the sweep grid, each candidate timed on this batch
+    // (one untimed and two timed iterations); a candidate replaces the best
+    // shape when it is at least 2% faster. Every shape gives the same results
+    // bit for bit, so the choice only moves time. The result is kept per
+    // batch signature (subgames, decision and terminal tasks, players) for
+    // later batches of the same make-up. SS_NO_AUTOTUNE=1 keeps the
+    // automatic rules; shapes forced through ssdbg_set_phased_shape win.
+    double tune_ms = 0;
+    int tune_shape() {
+        const char *no = getenv("SS_NO_AUTOTUNE");
+        if ((no && atoi(no) != 0) || phased_shape_forced() || cfg.max_iterations < 3) return SS_OK;
+        const double t0 = now_ms();
+        std::vector<Slot *> raw;
+        for (auto &sl : slots) raw.push_back(sl.get());
+        const int ndec = pt.lvl_off.empty() ? 0 : pt.lvl_off[pt.L];
+        const std::array<int64_t, 5> key{n, ndec, pt.nterm, pt.NPmax, (int64_t)cfg.precision};
+        static std::mutex mu;
+        static std::map<std::array<int64_t, 5>, PhasedTasks::Shape> cache;
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            auto it = cache.find(key);
+            if (it != cache.end()) {
+                pt.shape = it->second;
+                int rc = build_phased_tasks(raw, pt, st);
+                tune_ms = now_ms() - t0;
+                return rc;
+            }
+        }
+        auto time_shape = [&](const PhasedTasks::Shape &sh, float *ms) {
+            pt.shape = sh;
+            int rc = build_phased_tasks(raw, pt, st);
+            if (!rc) rc = reset_state();
+            if (!rc) rc = launch_iteration(1);
+            if (rc) return rc;
+            SS_HIP(hipEventRecord(ev[0], st));
+            for (int k = 0; k < 2 && !rc; ++k) rc = launch_iteration(1);
+            if (rc) return rc;
+            SS_HIP(hipEventRecord(ev[1], st));
+            SS_HIP(hipEventSynchronize(ev[1]));
+            SS_HIP(hipEventElapsedTime(ms, ev[0], ev[1]));
+            return SS_OK;
+        };
+        PhasedTasks::Shape best_shape;
+        float best = 0;
+        int rc = time_shape(best_shape, &best);
+        if (rc) return rc;
+        const int knobs[4][4] = {{0, 1 << 30, -2, -2},     // fwd_group_min: always three / always one
+                                 {0, -2, -2, -2},          // term_small: separate launches
+                                 {1, 2, 4, -2},            // sd_parts
+                                 {1024, 2048, -2, -2}};    // sweep_grid
+        for (int k = 0; k < 4; ++k)
+            for (int v = 0; v < 4 && knobs[k][v] != -2; ++v) {
+                PhasedTasks::Shape c = best_shape;
+                int *f = k == 0 ? &c.fwd_group_min : k == 1 ? &c.term_small : k == 2 ? &c.sd_parts : &c.sweep_grid;
+                if (*f == knobs[k][v]) continue;
+                *f = knobs[k][v];
+                float ms = 0;
+                if ((rc = time_shape(c, &ms))) return rc;
+                if (log_level() > 0)
+                    fprintf(stderr, "[street_solve] shape fwd %d small %d sd %d grid %d: %.3f ms / 2 iterations\n",
+                            c.fwd_group_min, c.term_small, c.sd_parts, c.sweep_grid, ms);
+                if (ms < 0.98f * best) {
+                    best = ms;
+                    best_shape = c;
+                }
+            }
+        pt.shape = best_shape;
+        rc = build_phased_tasks(raw, pt, st);
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            cache[key] = best_shape;
+        }
+        tune_ms = now_ms() - t0;
+        return rc;
     }
 
     int evaluate(const std::vector<int> &sel, int at, double *check_ms) {
@@ -399,6 +487,11 @@ struct BatchImpl {
             stats->launches = launches;
             stats->max_iterations = maxit;
             stats->mode = mode;
+            stats->shape_fwd_group_min = pt.shape.fwd_group_min;
+            stats->shape_term_small = pt.shape.term_small;
+            stats->shape_sd_parts = pt.shape.sd_parts;
+            stats->shape_sweep_grid = pt.shape.sweep_grid;
+            stats->tune_ms = tune_ms;
         }
         return SS_OK;
     }

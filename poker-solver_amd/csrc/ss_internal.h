@@ -155,6 +155,16 @@ struct PhasedTasks {
         bool rsig = false;     // some listed subgame has t + 1 players (its root snapshot pass)
     };
     std::vector<PassTerm> pterm;
+    // Launch shape of this task list (the batch autotuner's choice): forward
+    // levels with >= fwd_group_min tasks run three tasks per workgroup (-1:
+    // the automatic threshold), term_small 0 keeps the terminal classes in
+    // separate launches (-1: one launch when they fit), sd_parts workgroups
+    // per two-opponent showdown (-1: automatic), sweep_grid the sweep
+    // launches' workgroup cap (0: the default). Every shape gives the same
+    // results bit for bit.
+    struct Shape {
+        int fwd_group_min = -1, term_small = -1, sd_parts = -1, sweep_grid = 0;
+    } shape;
     uint32_t *d_tasks = nullptr;
     SweepRec *d_recs = nullptr;    // the per-pass sweep records
     size_t cap_tasks = 0, cap_recs = 0;   // allocated lengths of d_tasks, d_recs
@@ -172,8 +182,10 @@ int prec_rows_per_task(int nt);   // rows of one precision GEMM task for nt colu
 int build_phased_tasks(const std::vector<Slot *> &slots, PhasedTasks &pt, hipStream_t st,
                        const std::vector<int> *live = nullptr);
 int launch_phased_iteration(SlotDesc *d_slots, const PhasedTasks &pt, hipStream_t st);
-// launch shapes of the phased form (-1: automatic); see ssdbg_set_phased_shape
+// launch shapes of the phased form forced for every task list (-1: not
+// forced); see ssdbg_set_phased_shape
 void set_phased_shape(int fwd_group_min, int term_small, int sd_parts);
+bool phased_shape_forced();
 // joint = false skips the joint deal mass (xout[2 NP]), which depends only on
 // the raw weights: a caller whose xout persists computes it once
 int launch_phased_expl(SlotDesc *d_slots, const PhasedTasks &pt, const int *d_want, double *scratch,

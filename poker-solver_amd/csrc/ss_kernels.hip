@@ -2126,14 +2126,23 @@ static int device_cus() {
 }
 static int small_term_limit() { return 2 * device_cus(); }
 
-// forced launch shapes (tests); -1: automatic
-static struct {
-    int fwd_group_min = -1, term_small = -1, sd_parts = -1;
-} g_shape;
+// launch shapes forced on every task list (tests); -1: not forced
+static PhasedTasks::Shape g_shape;
 void set_phased_shape(int fwd_group_min, int term_small, int sd_parts) {
     g_shape.fwd_group_min = fwd_group_min;
     g_shape.term_small = term_small;
     g_shape.sd_parts = sd_parts;
+}
+bool phased_shape_forced() {
+    return g_shape.fwd_group_min >= 0 || g_shape.term_small >= 0 || g_shape.sd_parts >= 1;
+}
+// a task list's shape with the forced fields applied
+static PhasedTasks::Shape effective_shape(const PhasedTasks &pt) {
+    PhasedTasks::Shape s = pt.shape;
+    if (g_shape.fwd_group_min >= 0) s.fwd_group_min = g_shape.fwd_group_min;
+    if (g_shape.term_small >= 0) s.term_small = g_shape.term_small;
+    if (g_shape.sd_parts >= 1) s.sd_parts = g_shape.sd_parts;
+    return s;
 }
 
 #define SS_NUM_XCD 8
@@ -2153,6 +2162,7 @@ static void xcd_interleave(std::vector<uint32_t> &v, size_t b, size_t e) {
 }
 
 int build_phased_tasks(const std::vector<Slot *> &slots, PhasedTasks &pt, hipStream_t st, const std::vector<int> *live) {
+    const PhasedTasks::Shape shape = effective_shape(pt);
     pt.nslots = (int)slots.size();
     pt.NPmax = 0;
     pt.L = 0;
@@ -2255,7 +2265,7 @@ int build_phased_tasks(const std::vector<Slot *> &slots, PhasedTasks &pt, hipStr
         // is not one whole node's
         const int sd_slots = 2 * device_cus();
         q.sd_parts = q.nsd > 0 ? std::max(1, std::min(4, sd_slots / q.nsd)) : 1;
-        if (g_shape.sd_parts >= 1) q.sd_parts = std::min(4, g_shape.sd_parts);
+        if (shape.sd_parts >= 1) q.sd_parts = std::min(4, shape.sd_parts);
         // two-opponent showdowns and the other closed-form classes within two
         // rounds of resident workgroups: one launch (k_term_small), the
         // showdowns first so the lighter classes cycle through the other
@@ -2264,7 +2274,7 @@ int build_phased_tasks(const std::vector<Slot *> &slots, PhasedTasks &pt, hipStr
         // the light kernel's own shape is faster)
         const int lim = small_term_limit(), nall = q.nsd + q.nfold + q.nlight;
         q.small_parts = 0;
-        if (q.nodo == 0 && q.nsd > 0 && nall <= 2 * lim && g_shape.term_small != 0)
+        if (q.nodo == 0 && q.nsd > 0 && nall <= 2 * lim && shape.term_small != 0)
             q.small_parts = std::max(1, std::min(4, (lim - q.nfold - q.nlight) / q.nsd));
     }
     // per traverser pass and level: forward tasks (actor not the traverser),
@@ -2344,19 +2354,21 @@ static const int SWEEP_GRID = [] {
 
 // sweep tasks per workgroup: three when they still fill three resident
 // workgroups per CU that way, else one
-static int per_of(int cnt) {
-    if (g_shape.fwd_group_min >= 0) return cnt >= g_shape.fwd_group_min ? 3 : 1;
+static int per_of(int cnt, const PhasedTasks::Shape &s) {
+    if (s.fwd_group_min >= 0) return cnt >= s.fwd_group_min ? 3 : 1;
     return cnt >= 3 * 3 * device_cus() ? 3 : 1;
 }
 
 int launch_phased_iteration(SlotDesc *d_slots, const PhasedTasks &pt, hipStream_t st) {
+    const PhasedTasks::Shape shape = effective_shape(pt);
+    const int SWEEP_GRID = shape.sweep_grid > 0 ? shape.sweep_grid : ss::SWEEP_GRID;
     for (int t = 0; t < pt.NPmax; ++t) {
         const int *fo = &pt.sw_off[(size_t)t * 2 * (pt.L + 1)], *bo = fo + pt.L + 1;
         if (pt.pterm[t].rsig)
             hipLaunchKernelGGL(k_rsig_phased, dim3(pt.nslots), dim3(SS_BLOCK), 0, st, d_slots, pt.nslots, t);
         for (int l = 0; l < pt.L; ++l) {
             int cnt = fo[l + 1] - fo[l];
-            if (cnt > 0 && per_of(cnt) == 3)
+            if (cnt > 0 && per_of(cnt, shape) == 3)
                 hipLaunchKernelGGL(k_fwd_grouped, dim3(std::min((cnt + 2) / 3, SWEEP_GRID)), dim3(SWEEP_BLOCK), 0, st,
                                    d_slots, pt.d_recs + fo[l], cnt, t);
             else if (cnt > 0)

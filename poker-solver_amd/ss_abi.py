@@ -135,6 +135,11 @@ class SSBatchStats(ctypes.Structure):
         ("launches", ctypes.c_int),
         ("max_iterations", ctypes.c_int),
         ("mode", ctypes.c_int),
+        ("shape_fwd_group_min", ctypes.c_int),
+        ("shape_term_small", ctypes.c_int),
+        ("shape_sd_parts", ctypes.c_int),
+        ("shape_sweep_grid", ctypes.c_int),
+        ("tune_ms", ctypes.c_double),
     ]
 
 

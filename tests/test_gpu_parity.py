@@ -480,3 +480,27 @@ def test_cfr_state_hand_derived_literals(form):
         res = ss_api.solve_plain(prob, k, flags=ss_abi.SS_FLAG_NO_EXPL)
         assert res.root_strategy[:, 0] == pytest.approx([s0, s1], abs=1e-6)
         assert res.root_ev[0] == pytest.approx(root_ev, rel=1e-6)
+
+
+def test_autotuned_launch_shape_equals_plain_and_is_cached():
+    """ss_batch_create times launch shapes of the phased form on the batch and
+    keeps the fastest; a second batch of the same make-up reuses the choice
+    without tuning. Every shape gives the plain form's results bit for bit."""
+    spots = [wl.river_2p(seed=90 + i) for i in range(3)] + [wl.river_3p(seed=93)]
+    spots += [wl.small_spot(94 + i, num_players=2 + i % 2, hands_per_player=11) for i in range(4)]
+    probs = [ss_api.Problem.from_spot(s) for s in spots]
+    tuned = []
+    for rep in range(2):
+        b = ss_api.Batch(probs, 9, flags=1)
+        reps, st = b.run()
+        assert st.mode == 1
+        tuned.append((st.tune_ms, (st.shape_fwd_group_min, st.shape_term_small, st.shape_sd_parts,
+                                   st.shape_sweep_grid)))
+        print(f"autotune run {rep}: {tuned[-1]}", flush=True)
+        for i, p in enumerate(probs):
+            got = b.output(i)
+            want = ss_api.solve_plain(p, 9, flags=1)
+            assert np.array_equal(got.avg, want.avg), (rep, i)
+            assert np.array_equal(got.root_strategy, want.root_strategy), (rep, i)
+        b.close()
+    assert tuned[0][0] > 0 and tuned[1][0] < tuned[0][0] and tuned[1][1] == tuned[0][1]
