@@ -61,6 +61,29 @@ int ss_leaf_equity(int num_players, const int *hands, const int *num_hands, int 
 int ss_leaf_values(int num_players, const double *raw_equity, const int *num_hands, int max_hands,
                    const int *leaf_idx, const int *leaf_pot, int num_leaf_infos, float *leaf_values);
 
+/* Leaf values evaluated from equity straight into the device leaf buffer
+ * of subgame `index` of a batch session (ss_batch_create with any leaf
+ * values, e.g. zeros; include/street_solve.h), without the round trip of
+ * the [rows][NP][max_hands] array through the host: the next ss_batch_run
+ * solves with them. raw_equity is ss_leaf_equity's [NP][max_hands] output
+ * (max_hands = the subgame's largest range); leaf_idx / leaf_pot as for
+ * ss_leaf_values.
+ *   tree_order = 0: tree leaf l takes profile row l, as the reference passes
+ *     the producer's array to the solver (python/street_solver_gpu.py:
+ *     299-311); num_leaf_infos * 4^NP must equal the tree's leaves. The
+ *     solve equals one given ss_leaf_values' array as its leaf values.
+ *   tree_order = 1 (opt-in): tree leaf l takes row L0 * 4^NP + sum_p
+ *     s_p 4^p, where L0 is the original leaf its continuation chain
+ *     expands (numbered in node order), s_p the continuation strategy
+ *     player p picks along the chain (street_solve.cu:400-455: the first
+ *     active player's choice outermost) and 0 for players not in the
+ *     chain: the profile the leaf actually stands for, whatever the
+ *     number of active players. leaf_idx then names original leaves.
+ * No counterpart in the reference: the leaf values there go through host
+ * memory, in the producer's digit order (leaf_values.py:552, 801). */
+int ss_batch_leaf_values(SSBatch *batch, int index, const double *raw_equity, const int *leaf_idx,
+                         const int *leaf_pot, int num_leaf_infos, int tree_order);
+
 /* Allocation-free variant of ss_leaf_equity (same arguments, same results
  * bit for bit). No counterpart in the reference: it is for callers that
  * cannot allocate device memory (code between HIP graph captures, tight

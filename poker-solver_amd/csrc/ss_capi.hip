@@ -1,8 +1,10 @@
 // ss_capi.hip -- extern "C" entry points of libstreet_solve.so (include/street_solve.h).
 #include <chrono>
+#include <cmath>
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <array>
 #include <map>
 #include <memory>
@@ -300,8 +302,12 @@ struct BatchImpl {
         const double t0 = now_ms();
         std::vector<Slot *> raw;
         for (auto &sl : slots) raw.push_back(sl.get());
+        // the make-up of the batch: subgames, players, and the decision and
+        // terminal task counts in quarter-octave buckets (batches drawn from
+        // one distribution share a key)
         const int ndec = pt.lvl_off.empty() ? 0 : pt.lvl_off[pt.L];
-        const std::array<int64_t, 5> key{n, ndec, pt.nterm, pt.NPmax, (int64_t)cfg.precision};
+        auto bucket = [](int x) { return (int64_t)std::lround(4.0 * std::log2(1.0 + x)); };
+        const std::array<int64_t, 5> key{n, bucket(ndec), bucket(pt.nterm), pt.NPmax, (int64_t)cfg.precision};
         static std::mutex mu;
         static std::map<std::array<int64_t, 5>, PhasedTasks::Shape> cache;
         {
@@ -628,6 +634,75 @@ int ss_batch_output(SSBatch *batch, int index, SSOutput *output) {
 }
 
 void ss_batch_destroy(SSBatch *batch) { delete batch; }
+
+int ss_batch_leaf_values(SSBatch *batch, int index, const double *raw_equity, const int *leaf_idx,
+                         const int *leaf_pot, int num_leaf_infos, int tree_order) {
+    clear_error();
+    if (!batch || index < 0 || index >= batch->impl.n) return set_error(SS_ERR_ARG, "bad batch or subgame index");
+    if (!raw_equity || num_leaf_infos < 0 || (num_leaf_infos > 0 && (!leaf_idx || !leaf_pot)))
+        return set_error(SS_ERR_ARG, "bad leaf arguments");
+    Slot &s = *batch->impl.slots[index];
+    const Plan &pl = s.plan;
+    if (!pl.has_leaf || pl.num_leaves <= 0 || !s.desc.leaf_values)
+        return set_error(SS_ERR_ARG, "subgame %d has no leaf buffer (create the batch with leaf values)", index);
+    const int NP = pl.NP, L = pl.num_leaves;
+    const long long rows = (long long)num_leaf_infos << (2 * NP);
+    std::vector<int> row_of;
+    if (!tree_order) {
+        // the reference passes the producer's array as is: its rows must be
+        // the tree's leaves (python/street_solver_gpu.py:299-307)
+        if (rows != L)
+            return set_error(SS_ERR_ARG, "%lld profile rows for %d tree leaves (reference order needs equal counts)",
+                             rows, L);
+    } else {
+        // tree leaf l -> row L0 * 4^NP + sum_p s_p 4^p: climb the continuation
+        // chain (one level per active player, the first active player's
+        // choice at the top, street_solve.cu:400-455) to its head, the
+        // original leaf, numbered in node order; players not in the chain
+        // keep profile 0
+        const int N = pl.N;
+        std::vector<int> parent(N, -1), slot_of(N, 0);
+        for (int n = 0; n < N; ++n) {
+            const DevNode &d = pl.nodes[n];
+            if (d.type != SS_NODE_DECISION) continue;
+            for (int a = 0; a < d.nchild; ++a) {
+                parent[pl.children[d.first_child + a]] = n;
+                slot_of[pl.children[d.first_child + a]] = a;
+            }
+        }
+        std::vector<int> head_of(L, -1), combo_of(L, 0);
+        for (int n = 0; n < N; ++n) {
+            const DevNode &d = pl.nodes[n];
+            if (d.type != SS_NODE_LEAF || d.leaf_idx < 0 || d.leaf_idx >= L) continue;
+            int cur = n, combo = 0, ok = 1;
+            for (int q = NP - 1; q >= 0; --q) {
+                if (!((d.active_mask >> q) & 1)) continue;
+                const int par = parent[cur];
+                if (par < 0 || pl.nodes[par].player != q || pl.nodes[par].nchild != SS_NUM_CONT_STRATS) {
+                    ok = 0;
+                    break;
+                }
+                combo |= slot_of[cur] << (2 * q);
+                cur = par;
+            }
+            if (!ok) return set_error(SS_ERR_ARG, "leaf node %d is not at the end of a continuation chain", n);
+            head_of[d.leaf_idx] = cur;
+            combo_of[d.leaf_idx] = combo;
+        }
+        std::vector<int> heads(head_of.begin(), head_of.end());
+        std::sort(heads.begin(), heads.end());
+        heads.erase(std::unique(heads.begin(), heads.end()), heads.end());
+        row_of.resize(L);
+        for (int l = 0; l < L; ++l) {
+            if (head_of[l] < 0) return set_error(SS_ERR_ARG, "tree leaf %d has no node", l);
+            const int L0 = (int)(std::lower_bound(heads.begin(), heads.end(), head_of[l]) - heads.begin());
+            row_of[l] = (L0 << (2 * NP)) | combo_of[l];
+        }
+    }
+    return leaf_values_into(const_cast<float *>(s.desc.leaf_values), L, NP, pl.maxh, raw_equity, pl.nh,
+                            leaf_idx, leaf_pot, num_leaf_infos, tree_order ? row_of.data() : nullptr,
+                            batch->impl.st);
+}
 
 int ss_batch_range_update(SSBatch *batch, int index, int node, int action, int num_hands, const int *solver_hand,
                           double *weights, double weight_floor, double p_threshold) {

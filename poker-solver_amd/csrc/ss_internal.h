@@ -113,6 +113,12 @@ struct Slot {
 
 int64_t estimate_footprint(const HostProblem &hp);
 
+// Leaf values evaluated from equity straight into a device leaf buffer
+// [num_leaves][np][mh] (ss_leaf.hip); row_of (host, may be null) maps tree
+// leaves to profile rows of the ss_leaf_values layout.
+int leaf_values_into(float *d_leaf, int num_leaves, int np, int mh, const double *raw, const int *nh,
+                     const int *leaf_idx, const int *leaf_pot, int ninfo, const int *row_of, hipStream_t st);
+
 // Range narrowing from a resident solve's strategy sums (ss_range.hip).
 int range_update_from_strategy(const float *d_SS, int row0, int na, int action, int Mp, int nsolver,
                                const int *solver_hand, double *weights, int n, double floor_w, double threshold,

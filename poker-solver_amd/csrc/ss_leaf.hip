@@ -124,7 +124,7 @@ struct TupleRec {
 // far and kept until the process ends (no allocation on repeated calls of a
 // size seen before). The calls are synchronous; the mutex serialises callers.
 enum LeafSlot { L_HC, L_RUN, L_STR, L_VALID, L_DESC, L_POOL, L_EQ, L_CNT, L_RAW, L_RECS,
-                V_RAW, V_POT, V_NH, V_OUT, L_NSLOTS };
+                V_RAW, V_POT, V_NH, V_OUT, V_ROW, L_NSLOTS };
 
 struct LeafArena {
     std::mutex mu;
@@ -359,6 +359,27 @@ __device__ __forceinline__ double biased_value(double equity, double pot, int s_
     return fp * fold_payoff + (1.0 - fp) * showdown_payoff;
 }
 
+// Value of profile row `row` (leaf * 4^np + combo) for player p's hand h,
+// element j = p * mh + h of the equity: biased_leaf_value with the profile
+// digits of the row (player 0 least significant), 0 for rows no leaf info
+// names (pot LLONG_MIN) and hands past the player's count.
+__device__ __forceinline__ float leaf_row_value(const double *__restrict__ raw, long long pot, long long row,
+                                                int j, int p, int h, int np, const int *__restrict__ nh) {
+    if (pot == LLONG_MIN || h >= nh[p]) return 0.0f;
+    const int combo = (int)(row & ((1ll << (2 * np)) - 1));
+    int s_self = (combo >> (2 * p)) & 3, s_opp;
+    if (np > 2) {
+        int other = 0;
+        for (int q = 0; q < np; ++q)
+            if (q != p) other += (combo >> (2 * q)) & 3;
+        s_opp = (int)rint((double)other / (double)(np - 1));  // round half to even
+        s_opp = s_opp < 0 ? 0 : (s_opp > 3 ? 3 : s_opp);
+    } else {
+        s_opp = (combo >> (2 * (1 - p))) & 3;
+    }
+    return (float)biased_value(raw[j], (double)pot, s_self, s_opp);
+}
+
 __global__ __launch_bounds__(LEAF_BLOCK) void k_leaf_bias(const double *__restrict__ raw,
                                                           const long long *__restrict__ flat_pot,
                                                           const int *__restrict__ nh, float *__restrict__ out,
@@ -367,23 +388,24 @@ __global__ __launch_bounds__(LEAF_BLOCK) void k_leaf_bias(const double *__restri
     if (j >= np * mh) return;
     const long long leaf = blockIdx.x;
     const int p = j / mh, h = j - p * mh;
-    const long long pot = flat_pot[leaf];
-    float v = 0.0f;
-    if (pot != LLONG_MIN && h < nh[p]) {
-        const int combo = (int)(leaf & ((1ll << (2 * np)) - 1));
-        int s_self = (combo >> (2 * p)) & 3, s_opp;
-        if (np > 2) {
-            int other = 0;
-            for (int q = 0; q < np; ++q)
-                if (q != p) other += (combo >> (2 * q)) & 3;
-            s_opp = (int)rint((double)other / (double)(np - 1));  // round half to even
-            s_opp = s_opp < 0 ? 0 : (s_opp > 3 ? 3 : s_opp);
-        } else {
-            s_opp = (combo >> (2 * (1 - p))) & 3;
-        }
-        v = (float)biased_value(raw[j], (double)pot, s_self, s_opp);
-    }
-    out[leaf * np * mh + j] = v;
+    out[leaf * np * mh + j] = leaf_row_value(raw, flat_pot[leaf], leaf, j, p, h, np, nh);
+}
+
+// The same values gathered into a solve's device leaf buffer [num_leaves][np]
+// [mh]: tree leaf l takes profile row row_of[l] (row_of == nullptr: row l),
+// rows outside [0, rows) give 0.
+__global__ __launch_bounds__(LEAF_BLOCK) void k_leaf_bias_into(const double *__restrict__ raw,
+                                                               const long long *__restrict__ flat_pot,
+                                                               const int *__restrict__ row_of,
+                                                               const int *__restrict__ nh, float *__restrict__ out,
+                                                               int np, int mh, long long rows) {
+    const int j = blockIdx.y * LEAF_BLOCK + threadIdx.x;
+    if (j >= np * mh) return;
+    const long long l = blockIdx.x;
+    const long long row = row_of ? (long long)row_of[l] : l;
+    const int p = j / mh, h = j - p * mh;
+    const long long pot = row >= 0 && row < rows ? flat_pot[row] : LLONG_MIN;
+    out[l * np * mh + j] = leaf_row_value(raw, pot, row, j, p, h, np, nh);
 }
 
 namespace {
@@ -682,6 +704,37 @@ int leaf_values(int np, const double *raw, const int *nh, int mh, const int *lea
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return set_error(SS_ERR_HIP, "leaf values: %s", hipGetErrorString(e));
     return read_back(*A, out, d_out, (size_t)total * sizeof(float));
+}
+
+int leaf_values_into(float *d_leaf, int num_leaves, int np, int mh, const double *raw, const int *nh,
+                     const int *leaf_idx, const int *leaf_pot, int ninfo, const int *row_of, hipStream_t st) {
+    const long long cont = 1ll << (2 * np), rows = (long long)ninfo * cont;
+    std::vector<long long> flat(std::max<long long>(rows, 1), LLONG_MIN);
+    for (int i = 0; i < ninfo; ++i) {
+        if (leaf_idx[i] < 0) return set_error(SS_ERR_ARG, "leaf info %d has leaf_idx %d", i, leaf_idx[i]);
+        for (long long c = 0; c < cont; ++c) {
+            long long f = (long long)leaf_idx[i] * cont + c;
+            if (f < rows) flat[f] = leaf_pot[i];
+        }
+    }
+    if (num_leaves <= 0) return SS_OK;
+    LeafArena *A = nullptr;
+    int rc = leaf_arena(&A);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lock(A->mu);
+    double *d_raw;
+    long long *d_pot;
+    int *d_nh, *d_row = nullptr;
+    if ((rc = arena_upload(*A, V_RAW, raw, (size_t)np * mh, &d_raw)) ||
+        (rc = arena_upload(*A, V_POT, flat.data(), flat.size(), &d_pot)) ||
+        (rc = arena_upload(*A, V_NH, nh, (size_t)np, &d_nh)) ||
+        (row_of && (rc = arena_upload(*A, V_ROW, row_of, (size_t)num_leaves, &d_row))))
+        return rc;
+    k_leaf_bias_into<<<dim3((unsigned)num_leaves, (np * mh + LEAF_BLOCK - 1) / LEAF_BLOCK), LEAF_BLOCK, 0, st>>>(
+        d_raw, d_pot, d_row, d_nh, d_leaf, np, mh, rows);
+    SS_HIP(hipGetLastError());
+    SS_HIP(hipStreamSynchronize(st));
+    return SS_OK;
 }
 
 int py_choice_draws(int seed, const int *lens, int nlists, int count, int *out) {

@@ -138,8 +138,8 @@ def _sample(cards, seed, large):
     return list(cards)
 
 
-def _leaf_values(player_hands, runouts, leaf_infos, max_hands, lib=None):
-    lib = lib or ss_abi.load_library()
+def _equity(player_hands, runouts, max_hands, lib):
+    """ss_leaf_equity: float64 [NP][max_hands] runout-averaged equity."""
     npl = len(player_hands)
     nh = [len(h) for h in player_hands]
     if max(nh) > max_hands:
@@ -160,6 +160,14 @@ def _leaf_values(player_hands, runouts, leaf_infos, max_hands, lib=None):
                             run.ctypes.data_as(ip), len(run), raw.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
     if rc != 0:
         raise RuntimeError(f"ss_leaf_equity failed ({rc}): {ss_abi.last_error(lib)}")
+    return raw, nh_a
+
+
+def _leaf_values(player_hands, runouts, leaf_infos, max_hands, lib=None):
+    lib = lib or ss_abi.load_library()
+    npl = len(player_hands)
+    ip = ctypes.POINTER(ctypes.c_int)
+    raw, nh_a = _equity(player_hands, runouts, max_hands, lib)
     idx = np.array([li.leaf_idx for li in leaf_infos], dtype=np.int32)
     pot = np.array([li.pot for li in leaf_infos], dtype=np.int32)
     # Every element is written by the library (rows no leaf info names and
@@ -225,3 +233,53 @@ def compute_flop_leaf_equity(
     player_hands = _players(player_hands, oop_hands, ip_hands, "compute_flop_leaf_equity")
     nh = [len(h) for h in player_hands]
     return _leaf_values(player_hands, flop_runouts(flop_board, nh), leaf_infos, max_hands)
+
+
+def tree_leaf_infos(tree_data, num_players):
+    """One LeafInfo per original depth-limit leaf of a continuation-expanded
+    tree, numbered in node order (the leaf_idx the tree-order layout of
+    ss_batch_leaf_values uses): each expanded leaf climbs its chain (one level
+    per active player, street_solve.cu:400-455) to the head node, whose pot
+    and first num_players bets the info carries."""
+    nodes, children = tree_data.nodes, tree_data.children
+    parent = {}
+    for i in range(tree_data.num_nodes):
+        n = nodes[i]
+        if n.type == _DECISION:
+            for a in range(n.num_children):
+                parent[children[n.first_child + a]] = i
+    heads = set()
+    for i in range(tree_data.num_nodes):
+        n = nodes[i]
+        if n.type != _LEAF:
+            continue
+        cur = i
+        for _ in range(sum(1 for q in range(num_players) if n.active_players[q])):
+            cur = parent[cur]
+        heads.add(cur)
+    return [LeafInfo(leaf_idx=k, pot=nodes[h].pot, bets=tuple(nodes[h].bets[q] for q in range(num_players)))
+            for k, h in enumerate(sorted(heads))]
+
+
+def load_leaf_values_into_batch(batch, index, board, leaf_infos, player_hands, tree_order=False, lib=None):
+    """The leaf values of subgame `index` of an ss_api.Batch computed on the
+    GPU and written straight into its device leaf buffer
+    (ss_batch_leaf_values): equity over the turn's river runouts (4-card
+    board) or the flop's (turn, river) runouts, drawn as
+    compute_turn_leaf_values / compute_flop_leaf_equity draw them, then the
+    bias profiles. tree_order=False gives the solve the same leaf values as
+    passing compute_*'s array (the reference's layout); True uses the
+    tree-order profiles (tree_leaf_infos numbering)."""
+    lib = lib or ss_abi.load_library()
+    max_hands = max(len(h) for h in player_hands)
+    nh = [len(h) for h in player_hands]
+    runouts = turn_runouts(board, nh) if len(board) == 4 else flop_runouts(board, nh)
+    raw, _ = _equity(player_hands, runouts, max_hands, lib)
+    idx = np.array([li.leaf_idx for li in leaf_infos], dtype=np.int32)
+    pot = np.array([li.pot for li in leaf_infos], dtype=np.int32)
+    ip = ctypes.POINTER(ctypes.c_int)
+    rc = lib.ss_batch_leaf_values(batch.h, int(index), raw.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                  idx.ctypes.data_as(ip), pot.ctypes.data_as(ip), len(leaf_infos), int(tree_order))
+    if rc != 0:
+        raise RuntimeError(f"ss_batch_leaf_values failed ({rc}): {ss_abi.last_error(lib)}")
+

@@ -189,6 +189,8 @@ SIGNATURES = {
                                       _P(ctypes.c_int), _P(ctypes.c_int), ctypes.c_int, _P(ctypes.c_float)]),
     "ss_range_update": (ctypes.c_int, [ctypes.c_int, _P(ctypes.c_double), _P(ctypes.c_double),
                                        _P(ctypes.c_ubyte), ctypes.c_double, ctypes.c_double]),
+    "ss_batch_leaf_values": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _P(ctypes.c_double), _P(ctypes.c_int),
+                                            _P(ctypes.c_int), ctypes.c_int, ctypes.c_int]),
     "ss_batch_range_update": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                              _P(ctypes.c_int), _P(ctypes.c_double), ctypes.c_double,
                                              ctypes.c_double]),

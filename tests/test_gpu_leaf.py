@@ -178,3 +178,98 @@ def test_large_output_read_back_equals_oracle(lib):
     raw = gpu_equity(lib, hands, nh, runs, 1128)
     ref = lo.values_from_equity(raw, nh, 1128, infos)
     assert np.array_equal(out.view(np.uint32), ref.view(np.uint32))
+
+
+def _turn_problem(seed, npl, nh, zeros):
+    """A turn subgame with continuation leaves: ss_api Problem with the given
+    leaf values (or zeros), its hands as (c0, c1, w) lists, the tree."""
+    import ss_api
+    import workloads as wl
+    rng = np.random.default_rng(seed)
+    board = [int(c) for c in rng.choice(52, 4, replace=False)]
+    combos = wl.all_combos(board)
+    ph = []
+    for _ in range(npl):
+        pick = np.sort(rng.choice(len(combos), nh, replace=False))
+        ph.append([(int(combos[k][0]), int(combos[k][1]), float(rng.uniform(0.1, 1.0))) for k in pick])
+    tree = ss_api.Tree(board, 1200, 6000, [0.75], npl, use_cont=1, max_raises=1)
+    hands = [np.array([(a, b) for a, b, _ in h], np.int32) for h in ph]
+    weights = [np.array([w for _, _, w in h], np.float32) for h in ph]
+    return board, ph, tree, hands, weights
+
+
+def _solve_with(tree, hands, weights, leaf, iters=12):
+    import ss_api
+    prob = ss_api.Problem(tree, hands, weights, leaf_values=leaf)
+    b = ss_api.Batch([prob], iters, flags=1)
+    return prob, b
+
+
+def test_leaf_values_into_resident_solve_reference_order(lib):
+    """ss_batch_leaf_values (reference order) gives the solve exactly the leaf
+    values of compute_turn_leaf_values' host array: the solves are equal bit
+    for bit (2-player turn, 200 combos per player, continuation leaves)."""
+    board, ph, tree, hands, weights = _turn_problem(8400, 2, 200, False)
+    mh = 200
+    infos = lv.extract_leaf_info_from_tree(tree.td, 2)
+    host = lv.compute_turn_leaf_values(board, infos, mh, 1200, player_hands=ph)
+    assert host.shape[0] == tree.num_leaves
+    print(f"leaf into batch: turn 2p leaves={tree.num_leaves}", flush=True)
+    _, ba = _solve_with(tree, hands, weights, host.reshape(tree.num_leaves, 2, mh))
+    _, bb = _solve_with(tree, hands, weights, np.zeros((tree.num_leaves, 2, mh), np.float32))
+    lv.load_leaf_values_into_batch(bb, 0, board, infos, ph)
+    ba.run()
+    bb.run()
+    a, b = ba.output(0), bb.output(0)
+    assert np.array_equal(a.avg, b.avg) and np.array_equal(a.root_ev, b.root_ev)
+    ba.close()
+    bb.close()
+
+
+def test_leaf_values_into_resident_solve_tree_order(lib):
+    """Tree order on a 3-player turn whose leaves have 2 or 3 active players
+    (the reference layout does not fit such a tree): tree leaf l gets the
+    producer's row L0 * 64 + sum_p s_p 4^p of the strategies its chain picks
+    (players outside the chain at profile 0); the device-filled solve equals
+    the solve given that gather made on the host."""
+    board, ph, tree, hands, weights = _turn_problem(8401, 3, 60, False)
+    mh = 60
+    td = tree.td
+    infos = lv.tree_leaf_infos(td, 3)
+    prod = lv.compute_turn_leaf_values(board, infos, mh, 1200, player_hands=ph)
+    parent, pos = {}, {}
+    for i in range(td.num_nodes):
+        n = td.nodes[i]
+        if n.type == 0:
+            for a in range(n.num_children):
+                parent[td.children[n.first_child + a]] = i
+                pos[td.children[n.first_child + a]] = a
+    heads, rows, nactive = {}, {}, set()
+    for i in range(td.num_nodes):
+        n = td.nodes[i]
+        if n.type != 4:
+            continue
+        act = [q for q in range(3) if n.active_players[q]]
+        nactive.add(len(act))
+        cur, combo = i, 0
+        for q in reversed(act):
+            assert td.nodes[parent[cur]].player == q
+            combo += pos[cur] * 4 ** q
+            cur = parent[cur]
+        heads[n.leaf_idx], rows[n.leaf_idx] = cur, combo
+    assert nactive == {2, 3}
+    rank = {h: k for k, h in enumerate(sorted(set(heads.values())))}
+    gathered = np.stack([prod[rank[heads[l]] * 64 + rows[l]] for l in range(tree.num_leaves)])
+    print(f"leaf into batch: turn 3p tree order leaves={tree.num_leaves} heads={len(rank)}", flush=True)
+    _, ba = _solve_with(tree, hands, weights, gathered)
+    _, bb = _solve_with(tree, hands, weights, np.zeros_like(gathered))
+    lv.load_leaf_values_into_batch(bb, 0, board, infos, ph, tree_order=True)
+    ba.run()
+    bb.run()
+    a, b = ba.output(0), bb.output(0)
+    assert np.array_equal(a.avg, b.avg) and np.array_equal(a.root_ev, b.root_ev)
+    # the reference layout is refused for this tree
+    with pytest.raises(RuntimeError):
+        lv.load_leaf_values_into_batch(bb, 0, board, lv.extract_leaf_info_from_tree(td, 3), ph)
+    ba.close()
+    bb.close()
