@@ -245,10 +245,12 @@ def precision_bench(boards, iters, ref_boards, cores):
     t1 = time.perf_counter()
     plain = ss_api.solve_plain(probs[0], iters, precision=ss_abi.SS_PREC_BF16X2)
     plain_s = time.perf_counter() - t1
-    # MFMA work: per traverser pass 2 Mp^2 (2 S16) flops (hi and lo columns)
+    # MFMA work per traverser pass: 2 Mp^2 16 ceil(2S / 16) flops as issued (hi and lo
+    # columns packed into 16-wide tiles, hands padded to Mp); 2 nh^2 2S algorithmic
     S = [sum(1 for n in range(p.tree.num_nodes) if p.tree.td.nodes[n].type == 2) for p in probs]
     mpad = [(len(p.hands[0]) + 63) // 64 * 64 for p in probs]
-    flops = sum(2.0 * m * m * 2 * ((s + 15) // 16 * 16) * 2 * iters for m, s in zip(mpad, S))
+    flops = sum(2.0 * m * m * 16 * ((2 * s + 15) // 16) * 2 * iters for m, s in zip(mpad, S))
+    alg_flops = sum(2.0 * len(p.hands[0]) ** 2 * 2 * s * 2 * iters for p, s in zip(probs, S))
     out = {
         "workload": f"{boards} 2-player rivers (1081 combos each, pot 100 BB, stacks 400 BB, sizes {{0.5,1}}, "
                     f"max 2 raises), {iters} iterations each, one batched (phased) solve with the BF16 hi/lo "
@@ -259,6 +261,7 @@ def precision_bench(boards, iters, ref_boards, cores):
         "device_ms": {"cfr": round(st.cfr_ms, 2), "checks": round(st.check_ms, 2)},
         "mfma_tflop": round(flops / 1e12, 3),
         "mfma_tflops_over_cfr_time": round(flops / (st.cfr_ms / 1000.0) / 1e12, 3),
+        "mfma_algorithmic_tflop": round(alg_flops / 1e12, 3),
         "exploitability_pct_pot": {"mean": float(np.mean([r.exploitability for r in reps])),
                                    "max": float(np.max([r.exploitability for r in reps]))},
     }

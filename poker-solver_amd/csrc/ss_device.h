@@ -104,12 +104,14 @@ struct SlotDesc {
     const uint8_t *cpos;
     const uint4 *hmeta;        // [NP][Mp] river only
     // precision form (2-player rivers under SS_PREC_BF16X2), zero otherwise
-    int prec_S, prec_S16;      // showdown nodes, rounded up to 16
+    int prec_S, prec_NC;       // showdown nodes; 16-column tiles of the 2 S hi / lo columns
+    int prec_MpK;              // row length of prec_Bt: Mp rounded up to 256
     const int *prec_sd;        // [prec_S] showdown node ids
-    const uint2 *prec_W[2];    // per traverser [Mp][Mp / 32] sign masks {+1 bits, -1 bits} of W
-    uint16_t *prec_Bt;         // [2 prec_S16][Mp] bf16 hi / lo opponent reach
-    float *prec_Cn;            // [prec_S][64] per-card opponent totals
+    const uint32_t *prec_W[2]; // per traverser: W as 2-bit FP4 fragments, lane-ordered
+    uint16_t *prec_Bt;         // [16 prec_NC][prec_MpK] bf16 reach: column b hi, column S + b lo
+    float *prec_Z;             // [prec_S][Mp] card-removal mass z per node and traverser hand, -1 for value 0
     const int *prec_same[2];   // per traverser [Mp] opponent hand with the same cards, or -1
+    const float4 *prec_C;      // [2][prec_S] per traverser and node: P/2, traverser bet, node id (bits), 0
     // exploitability scratch (FP64), allocated on demand
     float *avg;                // [A][Mp]
     double *xreach;            // [NP][N][Mp]

@@ -83,12 +83,14 @@ void build_river_tables(const uint32_t *cstr, const int *lab, RiverTables &rt);
 
 // Precision-form buffers of a 2-player river subgame (ss_precision.hip).
 struct PrecState {
-    int S = 0, S16 = 0;                   // showdown nodes, rounded up to 16
+    int S = 0, NC = 0, MpK = 0;           // showdown nodes; column tiles ceil(2 S / 16); Mp rounded to 256
+    char *block = nullptr;                // one device allocation holding every buffer below
     int *sd_nodes = nullptr;              // [S]
-    uint2 *W[2] = {nullptr, nullptr};     // per traverser [Mp][Mp / 32] sign masks of W (+1 / -1 / 0)
-    uint16_t *Bt = nullptr;               // [2 S16][Mp] bf16 hi / lo opponent reach
-    float *Cn = nullptr;                  // [S][64] per-card opponent totals
+    uint32_t *W[2] = {nullptr, nullptr};  // per traverser: W (+1 / -1 / 0) as lane-ordered FP4 fragments
+    uint16_t *Bt = nullptr;               // [16 NC][MpK] bf16 opponent reach, hi and lo columns packed
+    float *Z = nullptr;                   // [S][Mp] card-removal mass per node and traverser hand
     int *same[2] = {nullptr, nullptr};    // [Mp] opponent hand with the same cards, or -1
+    float4 *C = nullptr;                  // [2][S] per traverser and node: P/2, traverser bet, node id
     ~PrecState();
 };
 
@@ -146,7 +148,8 @@ struct PhasedTasks {
     int nprec = 0;                 // the last nprec terminal tasks: showdowns of precision-form slots
                                    // (FP64 checks only; the CFR pass takes them from the contraction)
     int psplit_off = 0, npsplit = 0;   // precision form: (slot, showdown index) tasks
-    std::vector<std::pair<int, int>> pgemm;   // (column tiles NT, task count) groups of (slot, row tile) tasks
+    struct PrecGroup { int nc, rt, count; };   // column tiles, row tiles per wavefront, tasks
+    std::vector<PrecGroup> pgemm;              // groups of (slot, row block) precision GEMM tasks
     int pgemm_off = 0;
     // Per traverser pass t < NPmax, only the tasks that pass has work in (slots
     // with more than t players; the forward sweep also drops nodes where t
@@ -183,7 +186,10 @@ int prec_showdowns(Slot &s, int t, hipStream_t st);
 struct PhasedTasks;
 int launch_prec_phased(SlotDesc *d_slots, const PhasedTasks &pt, int t, hipStream_t st);
 int64_t prec_flops_per_pass(const Slot &s);
-int prec_rows_per_task(int nt);   // rows of one precision GEMM task for nt column tiles
+// Row tiles per wavefront of the batched precision GEMM for subgames of nc
+// column tiles with the given row tile counts (the choice that loads the
+// most loaded compute unit least); a task covers 64 rt rows.
+int prec_choose_rt(int nc, const std::vector<int> &tiles);
 
 int build_phased_tasks(const std::vector<Slot *> &slots, PhasedTasks &pt, hipStream_t st,
                        const std::vector<int> *live = nullptr);

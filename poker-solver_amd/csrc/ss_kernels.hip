@@ -2228,16 +2228,21 @@ int build_phased_tasks(const std::vector<Slot *> &slots, PhasedTasks &pt, hipStr
     xcd_interleave(tasks, pt.psplit_off, tasks.size());
     pt.pgemm_off = (int)tasks.size();
     pt.pgemm.clear();
-    for (int nt = 1; nt <= 8; ++nt) {
-        const size_t g0 = tasks.size();
+    for (int nc = 1; nc <= 16; ++nc) {
+        std::vector<int> group, tiles;
         for (int si = 0; si < pt.nslots; ++si)
-            if (on(si) && slots[si]->prec && slots[si]->prec->S16 / 16 == nt)
-                for (int r = 0, rows = prec_rows_per_task(nt); r < (slots[si]->plan.Mp + rows - 1) / rows; ++r)
-                    tasks.push_back(((uint32_t)si << 20) | (uint32_t)r);   // row blocks of k_prec_gemm_phased
-        if (tasks.size() > g0) {
-            xcd_interleave(tasks, g0, tasks.size());
-            pt.pgemm.push_back({nt, (int)(tasks.size() - g0)});
-        }
+            if (on(si) && slots[si]->prec && slots[si]->prec->NC == nc) {
+                group.push_back(si);
+                tiles.push_back(slots[si]->plan.Mp / 16);
+            }
+        if (group.empty()) continue;
+        const int rt = prec_choose_rt(nc, tiles), rows = 64 * rt;
+        const size_t g0 = tasks.size();
+        for (int si : group)
+            for (int r = 0; r < (slots[si]->plan.Mp + rows - 1) / rows; ++r)
+                tasks.push_back(((uint32_t)si << 20) | (uint32_t)r);   // row blocks of k_prec_gemm_phased
+        xcd_interleave(tasks, g0, tasks.size());
+        pt.pgemm.push_back({nc, rt, (int)(tasks.size() - g0)});
     }
     // per traverser pass: terminal tasks of the slots with more than t
     // players, by class, each class in XCD order

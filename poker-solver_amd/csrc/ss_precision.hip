@@ -10,32 +10,41 @@
 // P_win + P_tie / 2 = (Z + W r) / 2.
 //
 // The contraction over opponent hands runs on v_mfma_f32_16x16x32_bf16 for
-// all S showdown nodes at once: Y[Mp x 2 S16] = W[Mp x Mp] * [R_hi | R_lo],
-// where R_hi = bf16(r) and R_lo = bf16(r - R_hi), so the reach carries 16
-// significant bits and W (entries -1, 0, +1) is exact. W is stored as two sign
-// masks per row and 32-column chunk (2 bits per entry, 8x fewer bytes than
-// bf16) and expanded to bf16 in registers through a 256-entry LDS table. Z,
-// the mass floor and the epilogue are FP32 as in the closed-form path.
+// all S showdown nodes at once: Y[Mp x 2S] = W[Mp x Mp] * [R_hi | R_lo], where
+// R_hi = bf16(r) and R_lo = bf16(r - R_hi), so the reach carries 16
+// significant bits and W (entries -1, 0, +1) is exact. The 2S hi / lo columns
+// are packed into ceil(2S / 16) column tiles. W travels at 2 bits per entry,
+// already in the order of the MFMA's A operand: one AND (and a shift for odd
+// steps) gives a lane's 8 entries as FP4 (E2M1) nibbles, and four
+// v_cvt_scalef32_pk_bf16_fp4 turn them into the bf16 fragment. Z, the mass
+// floor and the epilogue are FP32 as in the closed-form path.
 #include "ss_internal.h"
+
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <utility>
 
 namespace ss {
 
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(2 * sizeof(__bf16)))) __bf16 bf16x2;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
 typedef __attribute__((__vector_size__(4 * sizeof(uint32_t)))) uint32_t u32x4;
 typedef __attribute__((__vector_size__(2 * sizeof(uint32_t)))) uint32_t u32x2;
 
-#define PREC_MAX_TILES 8   // S16 / 16 <= 8: at most 128 showdown nodes
+#define PREC_MAX_TILES 16   // ceil(2 S / 16) <= 16: at most 128 showdown nodes
 #define DN2_P (52 * 52)
-#define PREC_WAVES 4       // wavefronts per GEMM workgroup
+#define PREC_WAVES 4        // wavefronts per GEMM workgroup
 
 struct PrecArgs {
-    int S, S16, NT;           // showdown nodes, S rounded up to 16, S16 / 16
+    int S, NC, MpK;           // showdown nodes, column tiles, Bt row length
     const int *sd_nodes;      // [S]
-    const uint2 *W;           // [Mp][Mp / 32] sign masks {+1 bits, -1 bits}, rows = traverser hands
-    uint16_t *Bt;             // [2 S16][Mp] bf16 bits: rows b < S16 hi, S16 + b lo
-    float *Cn;                // [S][64]: per-card opponent totals, [52] = total
+    const uint32_t *W;        // FP4 fragments of W (k_prec_w)
+    uint16_t *Bt;             // [MpK / 8][16 NC][8] bf16 bits: column b hi, column S + b lo, others 0
+    float *Z;                 // [S][Mp]: card-removal mass per node and traverser hand, -1 where the value is 0
     const int *same;          // [Mp] opponent hand with the traverser hand's cards, or -1
+    const floatx4 *C;         // [S] per node: P/2, the traverser's bet, node id (bits), 0
 };
 
 // round-to-nearest-even float -> bf16 bits (finite inputs)
@@ -46,50 +55,78 @@ __device__ __forceinline__ uint16_t bf16_rne(float f) {
 }
 __device__ __forceinline__ float bf16_to_f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-// W for traverser t: +1 / -1 / 0 by rank, 0 for blocked or card-sharing
-// pairs, as masks: W[i][q] = {bits j with W(i, 32 q + j) = +1, ... = -1}
-__global__ void __launch_bounds__(SS_BLOCK) k_prec_w(const SlotDesc *__restrict__ s, int t, uint2 *W) {
-    const int Mp = s->Mp, o = 1 - t, nq = Mp / 32;
-    const size_t total = (size_t)Mp * nq;
+// W(i, j) for opponent o given hand i's card mask and strength: +1 / -1 by
+// rank, 0 for dead, blocked or card-sharing pairs and ties
+__device__ __forceinline__ int prec_w_entry(const SlotDesc *__restrict__ s, int o, uint64_t mi, uint32_t si, int j) {
+    if (j >= s->nh[o] || !si) return 0;
+    const uint32_t hj = s->hinfo[(size_t)o * s->Mp + j];
+    if (!(hj & SS_LIVE_BIT)) return 0;
+    const uint64_t mj = (1ull << (hj & 63)) | (1ull << ((hj >> 6) & 63));
+    const uint32_t sj = s->cstr[(hj >> 12) & 2047];
+    if ((mi & mj) || !sj) return 0;
+    return si > sj ? 1 : (si < sj ? -1 : 0);
+}
+
+// W for traverser t as the GEMM's A fragments. MFMA 16x16x32 lane l holds
+// A[l % 16][8 (l / 16) + 0..7]: for row tile rt and 32-k step q, lane l's 8
+// entries are W(16 rt + l % 16, 32 q + 8 (l / 16) + e), e = 0..7, as FP4
+// nibbles (+1 = 0x2, -1 = 0xA, 0 = 0x0; nibble e is element e, which is the
+// order v_cvt_scalef32_pk_bf16_fp4 reads them in). Two steps share one 32-bit
+// word: step 2m keeps bits 1 and 3 of each nibble, step 2m + 1 bits 0 and 2,
+// so `word & 0xAAAAAAAA` and `(word << 1) & 0xAAAAAAAA` are the two FP4 words.
+// Layout chunk-major, [nch][Mp / 16][64 lanes][PREC_KS / 2 words] (chunks of
+// PREC_KS steps), so the words of one chunk for consecutive row tiles are one
+// contiguous range; steps past Mp / 32 are 0, and PREC_W_PAD zero words follow
+// the last chunk (a wavefront past the last row tile reads them).
+#define PREC_KS 4                       // 32-k steps per staged chunk
+#define PREC_W_PAD (4 * 64 * PREC_KS / 2)   // words: 4 row tiles of one chunk
+__global__ void __launch_bounds__(SS_BLOCK) k_prec_w(const SlotDesc *__restrict__ s, int t, uint32_t *W) {
+    constexpr int WPC = PREC_KS / 2;
+    const int Mp = s->Mp, o = 1 - t, nq = Mp / 32, ntile = Mp / 16, nch = (nq + PREC_KS - 1) / PREC_KS;
+    const size_t total = (size_t)nch * ntile * 64;
     for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
-        const int i = (int)(e / nq), q = (int)(e % nq);
-        uint32_t pos = 0u, neg = 0u;
+        const int l = (int)(e % 64), rt = (int)((e / 64) % ntile), c = (int)(e / 64 / ntile);
+        const int i = 16 * rt + (l & 15), g = l >> 4;
+        uint32_t word[WPC] = {};
         const uint32_t hi = i < s->nh[t] ? s->hinfo[(size_t)t * Mp + i] : 0u;
         if (hi & SS_LIVE_BIT) {
             const uint64_t mi = (1ull << (hi & 63)) | (1ull << ((hi >> 6) & 63));
             const uint32_t si = s->cstr[(hi >> 12) & 2047];
-            for (int jb = 0; jb < 32; ++jb) {
-                const int j = 32 * q + jb;
-                if (j >= s->nh[o] || !si) continue;
-                const uint32_t hj = s->hinfo[(size_t)o * Mp + j];
-                if (!(hj & SS_LIVE_BIT)) continue;
-                const uint64_t mj = (1ull << (hj & 63)) | (1ull << ((hj >> 6) & 63));
-                const uint32_t sj = s->cstr[(hj >> 12) & 2047];
-                if ((mi & mj) || !sj) continue;
-                if (si > sj) pos |= 1u << jb;
-                else if (si < sj) neg |= 1u << jb;
-            }
+            for (int m = 0; m < WPC; ++m)
+                for (int odd = 0; odd < 2; ++odd) {
+                    const int q = PREC_KS * c + 2 * m + odd;
+                    if (q >= nq) continue;
+                    for (int x = 0; x < 8; ++x) {
+                        const int v = prec_w_entry(s, o, mi, si, 32 * q + 8 * g + x);
+                        if (v) word[m] |= ((v > 0 ? 0x2u : 0xAu) << (4 * x)) >> odd;
+                    }
+                }
         }
-        W[e] = make_uint2(pos, neg);
+        for (int m = 0; m < WPC; ++m) W[WPC * e + m] = word[m];
     }
 }
 
 __device__ __forceinline__ PrecArgs prec_args(const SlotDesc *__restrict__ s, int t) {
     PrecArgs a;
     a.S = s->prec_S;
-    a.S16 = s->prec_S16;
-    a.NT = s->prec_S16 / 16;
+    a.NC = s->prec_NC;
+    a.MpK = s->prec_MpK;
     a.sd_nodes = s->prec_sd;
     a.W = s->prec_W[t];
     a.Bt = s->prec_Bt;
-    a.Cn = s->prec_Cn;
+    a.Z = s->prec_Z;
     a.same = s->prec_same[t];
+    a.C = (const floatx4 *)(s->prec_C + (size_t)t * s->prec_S);
     return a;
 }
 
 // Showdown node b of traverser t's pass: the opponent's reach as bf16 hi / lo
-// rows of Bt, and its per-card totals (dense scatter + wave64 sums, fixed
-// order). One workgroup; M and C are LDS.
+// rows of Bt, its per-card totals (dense scatter + wave64 sums, fixed order)
+// and from them, per traverser hand i, the opponent reach disjoint from i:
+// z = S - C[x] - C[y] + r(same(i)) (S the total, C the per-card totals, the
+// hand holding i's cards counted twice), written as -1 where the showdown
+// value is 0 (dead or blocked hands, :861, and the mass floor). One
+// workgroup; M and C are LDS.
 __device__ void prec_split_node(const SlotDesc *__restrict__ s, const PrecArgs &a, int t, int b, float *M,
                                 float *C) {
     const int n = a.sd_nodes[b], o = 1 - t, Mp = s->Mp, nh = s->nh[o];
@@ -102,8 +139,9 @@ __device__ void prec_split_node(const SlotDesc *__restrict__ s, const PrecArgs &
         const bool ok = hj & SS_LIVE_BIT;
         const float x = ok ? gld(r + j) : 0.f;
         const uint16_t h = bf16_rne(x);
-        a.Bt[(size_t)b * Mp + j] = h;
-        a.Bt[(size_t)(a.S16 + b) * Mp + j] = bf16_rne(x - bf16_to_f(h));
+        uint16_t *piece = a.Bt + (size_t)(j >> 3) * 16 * a.NC * 8 + (j & 7);   // k group j / 8, element j % 8
+        piece[8 * b] = h;
+        piece[8 * (a.S + b)] = bf16_rne(x - bf16_to_f(h));
         if (!ok) continue;
         const int c0 = hj & 63, c1 = (hj >> 6) & 63;
         if (s->uniq) {   // one writer per cell
@@ -123,13 +161,23 @@ __device__ void prec_split_node(const SlotDesc *__restrict__ s, const PrecArgs &
         if (lane == 0) C[c] = v;
     }
     __syncthreads();
-    if (threadIdx.x < 64) {   // wavefront 0: the card totals and their half-sum
-        const float v = lane < 52 ? C[lane] : 0.f;
-        if (lane < 52) a.Cn[(size_t)b * 64 + lane] = v;
-        float tot = v;
+    if (threadIdx.x < 64) {   // wavefront 0: half the sum of the card totals (each hand holds two cards)
+        float tot = lane < 52 ? C[lane] : 0.f;
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off, 64);
-        if (lane == 0) a.Cn[(size_t)b * 64 + 52] = 0.5f * tot;
+        if (lane == 0) C[52] = 0.5f * tot;
+    }
+    __syncthreads();
+    const float Sb = C[52];
+    const int nt = s->nh[t];
+    for (int i = threadIdx.x; i < nt; i += blockDim.x) {
+        const uint32_t hv = gld(s->hinfo + (size_t)t * Mp + i);
+        const int sm = gld(a.same + i);
+        const uint32_t cs = gld(s->cstr + min((int)((hv >> 12) & 2047), 1325));
+        const float rv = sm >= 0 ? gld(r + sm) : 0.f;
+        const float z = Sb - C[hv & 63] - C[(hv >> 6) & 63] + rv;
+        const bool zero = !((hv & SS_LIVE_BIT) && cs) || z < 1e-5f * Sb;
+        a.Z[(size_t)b * Mp + i] = zero ? -1.f : z;
     }
     __syncthreads();
 }
@@ -155,267 +203,329 @@ __global__ void __launch_bounds__(SS_BLOCK) k_prec_split_phased(SlotDesc *__rest
 
 // Y = W * [R_hi | R_lo] for a block of rows and all column tiles, then the
 // showdown values written into cfv. A workgroup of 4 wavefronts owns 64 RT
-// rows (each wavefront RT row tiles of 16 hands); K (opponent hands) is walked
-// in chunks: the chunk of packed reach (all columns) and of W's sign masks
-// (all rows) is staged in LDS with coalesced loads, then every wavefront reads
-// its fragments from LDS, so each reach fragment fetched from L2 feeds 4 RT
-// MFMAs. K runs in order within a wavefront, so every form rounds alike.
-// MFMA 16x16x32 lane layout: lane l holds A[l % 16][8 (l / 16) + 0..7],
+// rows (each wavefront RT row tiles of 16 hands). K (opponent hands) is walked
+// in chunks of PREC_KS 32-k steps through a ring of NBUF LDS stages. A stage
+// holds the chunk's packed reach for all columns (shared by the workgroup) and
+// each wavefront's W words for its row tiles; both arrive by
+// global_load_lds_dwordx4 (global memory straight into LDS, no registers),
+// issued NBUF - 1 chunks ahead, so that NBUF - 1 chunks of MFMAs cover their
+// latency. Each chunk costs one barrier, after the wavefront has waited for
+// its own copies of that chunk (s_waitcnt vmcnt counts them: the loop issues
+// no other vector memory instruction). Bt is stored in 16-byte pieces of 8
+// consecutive k of one column, all columns of one 8-k group together, so a
+// chunk is one contiguous range and a lane's B fragment one piece, the 16
+// lanes of a column tile reading 256 contiguous bytes. K runs in order within
+// a wavefront whatever the row tiling, so every form rounds alike. MFMA
+// 16x16x32 lane layout: lane l holds A[l % 16][8 (l / 16) + 0..7],
 // B[8 (l / 16) + 0..7][l % 16], D[4 (l / 16) + 0..3][l % 16].
-template <int NT, int RT_> struct PrecShape {
-    static constexpr int RT = RT_;                                // row tiles per wavefront
-    static constexpr int ROWS = 64 * RT;                          // rows per workgroup / task
-    static constexpr int KCH = (512 / NT) / 64 * 64;              // K per staged chunk (multiple of 64)
-    static constexpr int BPAD = KCH + 8;                          // LDS row of one column, padded a bank
+template <int NC, int RT_> struct PrecShape {
+    static constexpr int RT = RT_;                // row tiles per wavefront
+    static constexpr int ROWS = 64 * RT;          // rows per workgroup / task
+    static constexpr int KS = PREC_KS;            // 32-k steps per chunk
+    static constexpr int WPC = KS / 2;            // W words per lane and row tile of a chunk
+    static constexpr int NCOL = 16 * NC;          // columns (hi and lo)
+    static constexpr int BP = 4 * KS * NCOL;      // 16-byte reach pieces per chunk
+    static constexpr int BPT = BP / (64 * PREC_WAVES);   // of them per thread
+    static constexpr int WQ = RT * 64 * WPC / 4;  // 16-byte W pieces per wavefront and chunk
+    static constexpr int WDMA = (WQ + 63) / 64;   // W copies per wavefront and chunk
+    static constexpr int STAGE = 16 * (BP + PREC_WAVES * WQ);   // bytes of one stage
+    static constexpr int NBUF = 4 * STAGE <= 106496 ? 4 : (3 * STAGE <= 122880 ? 3 : 2);
+    static constexpr int SMAX = 8 * NC;           // showdown nodes: 2 S <= 16 NC
+    static constexpr int RW = 16 * RT;            // rows per wavefront
+    static constexpr int YSTR = RW + 4;           // epilogue: column of accumulators, 16 lanes' 4-row writes a bank apart
+    static constexpr int ZPL = (RW * SMAX + 63) / 64;   // epilogue: (node, row) outputs per lane at most
+    static_assert(BP % (64 * PREC_WAVES) == 0, "a chunk's reach splits evenly over the workgroup");
 };
-// row tiles per wavefront in the batched form: the more column tiles, the
-// fewer rows, so the accumulators stay within the register file; the plain
-// form uses one row tile for more workgroups over a single subgame. The K
-// order is the same for any row tiling, so both forms round alike.
-template <int NT> constexpr int prec_rt() { return NT <= 2 ? 4 : (NT <= 4 ? 2 : 1); }
-// lut_one[m]: bf16 1.0 in each of 8 slots whose bit is set in m; lut_sign[m]:
-// the sign bit in those slots
-__device__ __forceinline__ void prec_fill_luts(uint4 *lut_one, uint4 *lut_sign) {
-    for (int m = threadIdx.x; m < 256; m += blockDim.x) {
-        uint32_t w1[4], ws[4];
+// row tiles per wavefront: the batched form picks one of 1 .. prec_rt_max(NC)
+// per launch (the more column tiles, the fewer rows, so the accumulators stay
+// within the register file); the plain form uses one row tile, for more
+// workgroups over a single subgame. The K order is the same for any row
+// tiling, so both forms round alike.
+constexpr int prec_rt_max(int nc) { return nc <= 4 ? 6 : (nc <= 8 ? 2 : 1); }
+
+template <int NC, int RT>
+struct PrecLds {
+    using SH = PrecShape<NC, RT>;
+    union {
+        struct {
+            u32x4 B[SH::NBUF][SH::BP];                          // reach pieces of the chunk
+            u32x4 W[SH::NBUF][PREC_WAVES][SH::WDMA * 64];       // each wavefront's W words of the chunk
+        } st;
+        struct {
+            float Ys[PREC_WAVES][SH::NCOL][SH::YSTR];           // each wavefront's accumulators, column-major
+            alignas(16) float Cs[SH::SMAX][4];                  // per node: P/2, traverser bet, node id
+        } ep;
+    } u;
+};
+
+// s_waitcnt vmcnt(N): at most N vector memory instructions of this
+// wavefront still in flight. The "memory" clobber keeps the compiler from
+// moving loads or copies across it, so N counts what the source issues after
+// the awaited copies.
+template <int N> __device__ __forceinline__ void prec_wait_vm() {
+    static_assert(N >= 0 && N < 64, "vmcnt is 6 bits");
+    asm volatile("s_waitcnt vmcnt(%0)" ::"n"(N) : "memory");
+}
+// workgroup barrier after this wavefront's LDS accesses, without the memory
+// fence of __syncthreads (which would also wait for every global-to-LDS copy
+// in flight). The "memory" clobber keeps the compiler from moving LDS
+// accesses across it.
+__device__ __forceinline__ void prec_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+// one wavefront's 1 KB (64 lanes x 16 bytes) global-to-LDS copy; lds is the
+// wavefront's destination (uniform), src the lane's 16 bytes
+__device__ __forceinline__ void prec_dma(const void *src, void *lds) {
+    __builtin_amdgcn_global_load_lds(src, lds, 16, 0, 0);
+}
+
+// the bf16 A fragment of one step from its FP4 word
+__device__ __forceinline__ bf16x8 prec_expand(uint32_t f) {
+    const bf16x2 p0 = __builtin_amdgcn_cvt_scalef32_pk_bf16_fp4(f, 1.0f, 0);
+    const bf16x2 p1 = __builtin_amdgcn_cvt_scalef32_pk_bf16_fp4(f, 1.0f, 1);
+    const bf16x2 p2 = __builtin_amdgcn_cvt_scalef32_pk_bf16_fp4(f, 1.0f, 2);
+    const bf16x2 p3 = __builtin_amdgcn_cvt_scalef32_pk_bf16_fp4(f, 1.0f, 3);
+    return __builtin_shufflevector(__builtin_shufflevector(p0, p1, 0, 1, 2, 3),
+                                   __builtin_shufflevector(p2, p3, 0, 1, 2, 3), 0, 1, 2, 3, 4, 5, 6, 7);
+}
+
+// the steps of one staged chunk (the first `steps` of them when GUARD: the
+// last chunk, whose later steps lie past the subgame's hands)
+template <int NC, int RT, bool GUARD>
+__device__ __forceinline__ void prec_chunk(const u32x4 *Bc, const uint32_t *Wc, floatx4 (&acc)[RT][NC], int l,
+                                           int steps) {
+    using SH = PrecShape<NC, RT>;
+    const int lane_piece = (l >> 4) * SH::NCOL + (l & 15);
+    // every LDS read of the chunk first (one latency per chunk, not per step)
+    uint32_t wd[RT][SH::WPC];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const uint32_t b0 = (m >> (2 * k)) & 1u, b1 = (m >> (2 * k + 1)) & 1u;
-            w1[k] = b0 * 0x3F80u | (b1 * 0x3F80u) << 16;
-            ws[k] = b0 * 0x8000u | (b1 * 0x8000u) << 16;
+    for (int r = 0; r < RT; ++r) {
+        const u32x2 v = *(const u32x2 *)(Wc + (r * 64 + l) * SH::WPC);
+        wd[r][0] = v[0], wd[r][1] = v[1];
+    }
+    bf16x8 vb[SH::KS][NC];
+#pragma unroll
+    for (int ks = 0; ks < SH::KS; ++ks)
+#pragma unroll
+        for (int q = 0; q < NC; ++q)
+            vb[ks][q] = __builtin_bit_cast(bf16x8, Bc[4 * ks * SH::NCOL + 16 * q + lane_piece]);
+#pragma unroll
+    for (int ks = 0; ks < SH::KS; ++ks) {
+        if (GUARD && ks >= steps) break;
+        bf16x8 va[RT];
+#pragma unroll
+        for (int r = 0; r < RT; ++r) {
+            const uint32_t word = wd[r][ks >> 1];
+            va[r] = prec_expand(((ks & 1) ? word << 1 : word) & 0xAAAAAAAAu);
         }
-        lut_one[m] = make_uint4(w1[0], w1[1], w1[2], w1[3]);
-        lut_sign[m] = make_uint4(ws[0], ws[1], ws[2], ws[3]);
+#pragma unroll
+        for (int r = 0; r < RT; ++r)
+#pragma unroll
+            for (int q = 0; q < NC; ++q)
+                acc[r][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va[r], vb[ks][q], acc[r][q], 0, 0, 0);
     }
 }
 
-template <int NT, int RT>
-struct PrecLds {
-    using SH = PrecShape<NT, RT>;
-    uint4 lut_one[256], lut_sign[256];
-    alignas(16) uint16_t Bs[32 * NT][SH::BPAD];              // reach chunk: [column][k]
-    uint2 Ws[SH::ROWS][SH::KCH / 32 + 1];                    // W mask chunk: [row][32-k step], padded a bank
-};
-
-template <int NT, int RT>
+template <int NC, int RT>
 __device__ void prec_gemm_block(const SlotDesc *__restrict__ s, const PrecArgs &a, int t, int row0,
-                                PrecLds<NT, RT> &L) {
-    using SH = PrecShape<NT, RT>;
-    constexpr int KCH = SH::KCH, NCOL = 32 * NT;
-    const int tid = threadIdx.x, l = tid & 63, w = tid >> 6, Mp = s->Mp, o = 1 - t;
-    const int nq = Mp / 32, sh = 8 * (l >> 4), wrow0 = 16 * RT * w;   // this wavefront's first row in the block
-    floatx4 acc[RT][2 * NT];
+                                PrecLds<NC, RT> &L) {
+    using SH = PrecShape<NC, RT>;
+    constexpr int KS = SH::KS, BP = SH::BP, BPT = SH::BPT, WPC = SH::WPC, NBUF = SH::NBUF, NTHR = 64 * PREC_WAVES;
+    constexpr int RW = SH::RW, ZPL = SH::ZPL, WQ = SH::WQ, WDMA = SH::WDMA;
+    const int tid = threadIdx.x, l = tid & 63, w = tid >> 6, g = l >> 4, Mp = s->Mp;
+    const int nq = Mp / 32, nch = (nq + KS - 1) / KS, ntile = Mp / 16;
+    const int rt0 = (row0 >> 4) + RT * w;   // this wavefront's first row tile
+    const bool live = rt0 < ntile;          // past the subgame's rows: the wavefront only stages reach
+    const u32x4 *Bg = (const u32x4 *)a.Bt;
+    const u32x4 *Wg = (const u32x4 *)a.W;
+    // chunk c into stage c % NBUF: the workgroup's reach pieces (each
+    // wavefront 1 KB blocks), then a live wavefront's W words for its row
+    // tiles (contiguous: RT row tiles of chunk c; a partly covered wavefront
+    // reads on into the next chunk or the padding, rows it never writes)
+    auto issue = [&](int c) {
+        u32x4 *Bs = L.u.st.B[c % NBUF];
+#pragma unroll
+        for (int u = 0; u < BPT; ++u) prec_dma(Bg + (size_t)c * BP + u * NTHR + tid, Bs + u * NTHR + 64 * w);
+        if (live) {
+            const u32x4 *src = Wg + ((size_t)c * ntile + rt0) * 64 * WPC / 4;
+            u32x4 *dst = L.u.st.W[c % NBUF][w];
+#pragma unroll
+            for (int u = 0; u < WDMA; ++u)
+                if (WQ % 64 == 0 || u * 64 + l < WQ) prec_dma(src + u * 64 + l, dst + u * 64);
+        }
+    };
+    // vector memory instructions of one chunk: the wait for chunk c leaves
+    // the chunks issued after it in flight
+    constexpr int D_LIVE = BPT + WDMA, D_DEAD = BPT;
+    auto wait_chunk = [&](int c) {
+        const int after = min(NBUF - 2, nch - 1 - c);   // chunks issued after chunk c
+        if (live) {
+            if (after >= 2) prec_wait_vm<2 * D_LIVE>();
+            else if (after == 1) prec_wait_vm<D_LIVE>();
+            else prec_wait_vm<0>();
+        } else {
+            if (after >= 2) prec_wait_vm<2 * D_DEAD>();
+            else if (after == 1) prec_wait_vm<D_DEAD>();
+            else prec_wait_vm<0>();
+        }
+    };
+    static_assert(NBUF <= 4 && 2 * D_LIVE < 64 && ZPL + 1 < 64, "vmcnt waits are 6 bits");
+    for (int c = 0; c < NBUF - 1 && c < nch; ++c) issue(c);
+    floatx4 acc[RT][NC];
 #pragma unroll
     for (int r = 0; r < RT; ++r)
 #pragma unroll
-        for (int q = 0; q < 2 * NT; ++q) acc[r][q] = floatx4{0.f, 0.f, 0.f, 0.f};
-    // staging: the chunk of packed reach for every column (hi columns are Bt
-    // rows 0 .. 16 NT - 1, lo columns Bt rows S16 ..) and the W masks of the
-    // block's rows, coalesced along k. Each thread holds a fixed number of
-    // loads in registers; the next chunk's loads are issued before the
-    // current chunk's MFMAs so that their latency is hidden. Addresses are
-    // clamped into the arrays instead of branched on: a clamped element lands
-    // in an LDS column past the chunk's end (never read) or in a row past Mp
-    // (never written back).
-    constexpr int NTHR = 64 * PREC_WAVES;
-    constexpr int BE = NCOL * KCH / 8, BU = BE / NTHR;                 // uint4 of reach per chunk
-    constexpr int WE = SH::ROWS * KCH / 32, WU = (WE + NTHR - 1) / NTHR;   // uint2 of masks per chunk
-    static_assert(BE % NTHR == 0, "reach chunk must split evenly over the workgroup");
-    u32x4 rb[BU];   // native vectors: kept in registers across the MFMA loop
-    u32x2 rw[WU];
-    // pass k0 = -KCH only fetches chunk 0; pass k0 stores chunk k0 into LDS,
-    // fetches chunk k0 + KCH, then runs chunk k0's MFMAs
-    for (int k0 = -KCH; k0 < Mp; k0 += KCH) {
-        if (k0 >= 0) {
-#pragma unroll
-            for (int u = 0; u < BU; ++u) {
-                const int e = u * NTHR + tid;
-                *(u32x4 *)&L.Bs[e / (KCH / 8)][8 * (e % (KCH / 8))] = rb[u];
-            }
-#pragma unroll
-            for (int u = 0; u < WU; ++u) {
-                const int e = u * NTHR + tid;
-                if (WE % NTHR == 0 || e < WE) *(u32x2 *)&L.Ws[e / (KCH / 32)][e % (KCH / 32)] = rw[u];
-            }
-            __syncthreads();
-        }
-        const int kf = k0 + KCH;
-        if (kf < Mp) {
-            const int k8 = min(KCH, Mp - kf) / 8, k32f = k8 / 4;
-#pragma unroll
-            for (int u = 0; u < BU; ++u) {
-                const int e = u * NTHR + tid, c = e / (KCH / 8), j = min(e % (KCH / 8), k8 - 1);
-                const int brow = c < 16 * NT ? c : a.S16 + (c - 16 * NT);
-                rb[u] = gld((const u32x4 *)(a.Bt + (size_t)brow * Mp + kf + 8 * j));
-            }
-#pragma unroll
-            for (int u = 0; u < WU; ++u) {
-                const int e = min(u * NTHR + tid, WE - 1), r = e / (KCH / 32), j = min(e % (KCH / 32), k32f - 1);
-                rw[u] = gld((const u32x2 *)&a.W[(size_t)min(row0 + r, Mp - 1) * nq + (kf >> 5) + j]);
-            }
-        }
-        if (k0 < 0) continue;
-        const int k32 = min(KCH, Mp - k0) / 32;   // the chunk's 32-k steps
-        for (int ks = 0; ks < k32; ++ks) {
-            bf16x8 va[RT], vb[2 * NT];
-#pragma unroll
-            for (int q = 0; q < 2 * NT; ++q)
-                vb[q] = __builtin_bit_cast(bf16x8, *(const uint4 *)&L.Bs[16 * q + (l & 15)][32 * ks + sh]);
-#pragma unroll
-            for (int r = 0; r < RT; ++r) {
-                // lane's 8 entries: bits sh .. sh + 7 of the step's masks
-                const uint2 m = L.Ws[wrow0 + 16 * r + (l & 15)][ks];
-                const uint32_t p8 = (m.x >> sh) & 0xFFu, n8 = (m.y >> sh) & 0xFFu;
-                const uint4 mag = L.lut_one[p8 | n8], sgn = L.lut_sign[n8];
-                va[r] = __builtin_bit_cast(bf16x8,
-                                           make_uint4(mag.x | sgn.x, mag.y | sgn.y, mag.z | sgn.z, mag.w | sgn.w));
-            }
-#pragma unroll
-            for (int r = 0; r < RT; ++r)
-#pragma unroll
-                for (int q = 0; q < 2 * NT; ++q)
-                    acc[r][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va[r], vb[q], acc[r][q], 0, 0, 0);
-        }
-        __syncthreads();
+        for (int q = 0; q < NC; ++q) acc[r][q] = floatx4{0.f, 0.f, 0.f, 0.f};
+    for (int c = 0; c + 1 < nch; ++c) {
+        wait_chunk(c);
+        prec_barrier();   // chunk c in LDS; chunk c - 1's stage free
+        if (c + NBUF - 1 < nch) issue(c + NBUF - 1);
+        if (live)
+            prec_chunk<NC, RT, false>(L.u.st.B[c % NBUF], (const uint32_t *)L.u.st.W[c % NBUF][w], acc, l, KS);
     }
-    // epilogue. Per-column constants go to LDS, over the reach buffer (free
-    // after the last chunk's barrier): Cn's 53 values, then half the pot, the
-    // traverser's bet, the node and the row of the opponent's reach at the
-    // node (reach-row table). Per-row inputs are loaded as one batch,
-    // then per column the opponent reach of the hand holding the same cards,
-    // so a thread's loads overlap instead of waiting one after another.
-    constexpr int CSTRIDE = 65;   // padded a bank: 16 columns read the same card
-    float *Cs = reinterpret_cast<float *>(&L.Bs[0][0]);
-    static_assert(sizeof(L.Bs) >= sizeof(float) * 16 * NT * CSTRIDE, "column constants must fit the reach buffer");
-    for (int e = tid; e < 16 * NT * 64; e += NTHR) {
-        const int b = e >> 6, c = e & 63;
-        float v = 0.f;   // columns past S: zeros, node 0
-        if (b < a.S && c <= 52) v = gld(a.Cn + (size_t)b * 64 + c);
-        else if (b < a.S && c <= 56) {
-            const int n = gld(a.sd_nodes + b);
-            const DevNode &nd = s->nodes[n];
-            v = c == 53   ? 0.5f * (float)nd.pot
-                : c == 54 ? (float)nd.bets[t]
-                : c == 55 ? __int_as_float(n)
-                          : __int_as_float(gld(s->rrow + (size_t)o * s->N + n));
+    // the epilogue's inputs, issued after every copy and left in flight by
+    // the last chunk's wait: per node its half pot, the traverser's bet and
+    // its id (one load per thread, clamped), and per (node, row) output the
+    // split kernel's card-removal mass (lane l takes outputs l + 64 u,
+    // node-major over the wavefront's rows)
+    const floatx4 cnode = gld(a.C + min(tid, a.S - 1));
+    float zr[ZPL];
+    if (live) {
+#pragma unroll
+        for (int u = 0; u < ZPL; ++u) {
+            const int o = l + 64 * u, b = min(o / RW, a.S - 1), i = min(16 * rt0 + o % RW, Mp - 1);
+            zr[u] = gld(a.Z + (size_t)b * Mp + i);
         }
-        Cs[b * CSTRIDE + c] = v;
     }
+    {
+        const int c = nch - 1;
+        if (live) prec_wait_vm<ZPL + 1>();
+        else prec_wait_vm<1>();
+        prec_barrier();
+        if (live)
+            prec_chunk<NC, RT, true>(L.u.st.B[c % NBUF], (const uint32_t *)L.u.st.W[c % NBUF][w], acc, l,
+                                     nq - c * KS);
+    }
+    prec_barrier();   // every wavefront is done with the stages
+    // epilogue: the accumulators go to LDS column-major over the stages (a
+    // lane's 4 rows in one write), then the lanes walk the (node, row)
+    // outputs node-major, so the cfv stores run along hands. Y = hi + lo in
+    // that order in every form.
+    if (live) {
+#pragma unroll
+        for (int r = 0; r < RT; ++r)
+#pragma unroll
+            for (int q = 0; q < NC; ++q)
+                *(floatx4 *)&L.u.ep.Ys[w][16 * q + (l & 15)][16 * r + 4 * g] = acc[r][q];
+    }
+    if (tid < a.S) *(floatx4 *)L.u.ep.Cs[tid] = cnode;
     __syncthreads();
-    const int nh = s->nh[t], rbase = row0 + wrow0 + 4 * (l >> 4);
-    const float *ro = s->reach + (size_t)o * s->N * Mp;
+    if (live) {
+        // every input of the lane's outputs read first (indices clamped) and
+        // held together, so the LDS reads go out back to back with one wait;
+        // then the values, and the stores of the outputs that exist
+        const int nh = s->nh[t];
+        float *cfv = s->cfv;   // read once: the stores below could otherwise alias the descriptor
+        float yh[ZPL], yl[ZPL], hf[ZPL], bt[ZPL];
+        int nd[ZPL];
 #pragma unroll
-    for (int r = 0; r < RT; ++r) {
-        uint32_t hv[4], cs[4];
-        int sv[4];
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int ic = min(rbase + 16 * r + e, Mp - 1);
-            hv[e] = gld(s->hinfo + (size_t)t * Mp + ic);
-            sv[e] = gld(a.same + ic);
+        for (int u = 0; u < ZPL; ++u) {
+            const int o = l + 64 * u, b = min(o / RW, a.S - 1), lr = o % RW;
+            const floatx4 cb = *(const floatx4 *)L.u.ep.Cs[b];
+            yh[u] = L.u.ep.Ys[w][b][lr];
+            yl[u] = L.u.ep.Ys[w][a.S + b][lr];
+            hf[u] = cb[0];
+            bt[u] = cb[1];
+            nd[u] = __float_as_int(cb[2]);
         }
 #pragma unroll
-        for (int e = 0; e < 4; ++e) cs[e] = gld(s->cstr + min((int)((hv[e] >> 12) & 2047), 1325));
+        for (int u = 0; u < ZPL; ++u) asm volatile("" ::"v"(yh[u]), "v"(yl[u]), "v"(hf[u]), "v"(bt[u]), "v"(nd[u]));
 #pragma unroll
-        for (int q = 0; q < NT; ++q) {
-            const int b = 16 * q + (l & 15);
-            const float *Cb = Cs + b * CSTRIDE;
-            const float Sb = Cb[52], half = Cb[53], bet = Cb[54];
-            const int n = __float_as_int(Cb[55]), nrow = __float_as_int(Cb[56]);
-            float rv[4];
-#pragma unroll
-            for (int e = 0; e < 4; ++e) rv[e] = gld(ro + (size_t)nrow * Mp + max(sv[e], 0));
-            // the values are needed here: keeps the compiler from sinking each
-            // load into the guarded store below, one wait per load
-#pragma unroll
-            for (int e = 0; e < 4; ++e) asm volatile("" ::"v"(rv[e]));
-            if (b >= a.S) continue;
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int i = rbase + 16 * r + e;
-                if (i >= nh) continue;
-                // computed for every lane and selected (no branch, so no
-                // load is sunk into one); dead or blocked hands give 0, :861
-                const int x = hv[e] & 63, y = (hv[e] >> 6) & 63, sm = sv[e];
-                const float z = Sb - Cb[x] - Cb[y] + (sm >= 0 ? rv[e] : 0.f);
-                const float Y = acc[r][q][e] + acc[r][NT + q][e];
-                const bool zero = !((hv[e] & SS_LIVE_BIT) && cs[e]) || z < 1e-5f * Sb;
-                s->cfv[(size_t)n * Mp + i] = zero ? 0.f : half * Y + (half - bet) * z;
-            }
+        for (int u = 0; u < ZPL; ++u) {
+            const int o = l + 64 * u, i = 16 * rt0 + o % RW;
+            const float Y = yh[u] + yl[u], z = zr[u];
+            const float val = z < 0.f ? 0.f : hf[u] * Y + (hf[u] - bt[u]) * z;
+            if (o / RW < a.S && i < nh) gst(cfv + (size_t)nd[u] * Mp + i, val);
         }
     }
+    prec_barrier();   // LDS is reused by the workgroup's next task (the stores need no wait)
 }
 
-// plain form: workgroup b covers rows [ROWS b, ROWS (b + 1))
-template <int NT>
+// plain form: workgroup b covers rows [64 b, 64 (b + 1))
+template <int NC>
 __global__ void __launch_bounds__(64 * PREC_WAVES) k_prec_gemm(const SlotDesc *__restrict__ s, int t) {
-    __shared__ PrecLds<NT, 1> L;
-    prec_fill_luts(L.lut_one, L.lut_sign);
-    __syncthreads();
-    prec_gemm_block<NT, 1>(s, prec_args(s, t), t, PrecShape<NT, 1>::ROWS * blockIdx.x, L);
+    __shared__ PrecLds<NC, 1> L;
+    prec_gemm_block<NC, 1>(s, prec_args(s, t), t, PrecShape<NC, 1>::ROWS * blockIdx.x, L);
 }
 
-template <int NT>
+template <int NC, int RT>
 __global__ void __launch_bounds__(64 * PREC_WAVES) k_prec_gemm_phased(SlotDesc *__restrict__ slots,
                                                                       const uint32_t *tasks, int ntasks, int t) {
-    constexpr int RT = prec_rt<NT>();
-    __shared__ PrecLds<NT, RT> L;
-    prec_fill_luts(L.lut_one, L.lut_sign);
-    __syncthreads();
+    __shared__ PrecLds<NC, RT> L;
     for (int j = blockIdx.x; j < ntasks; j += gridDim.x) {
         const uint32_t tk = tasks[j];
         const SlotDesc *s = slots + (tk >> 20);
         if (!(s->active && s->iter < s->iter_end && t < s->NP)) continue;
-        prec_gemm_block<NT, RT>(s, prec_args(s, t), t, PrecShape<NT, RT>::ROWS * (int)(tk & 0xFFFFF), L);
+        prec_gemm_block<NC, RT>(s, prec_args(s, t), t, PrecShape<NC, RT>::ROWS * (int)(tk & 0xFFFFF), L);
     }
 }
 
-int prec_rows_per_task(int nt) {
-    switch (nt) {
-    case 1: return 64 * prec_rt<1>();
-    case 2: return 64 * prec_rt<2>();
-    case 3: return 64 * prec_rt<3>();
-    case 4: return 64 * prec_rt<4>();
-    case 5: return 64 * prec_rt<5>();
-    case 6: return 64 * prec_rt<6>();
-    case 7: return 64 * prec_rt<7>();
-    default: return 64 * prec_rt<8>();
-    }
+typedef void (*PrecPlain)(const SlotDesc *, int);
+typedef void (*PrecPhased)(SlotDesc *, const uint32_t *, int, int);
+#define PREC_RT_SLOTS 6   // table columns: row tiles 1 .. 6
+template <int NC, int RT> static PrecPhased prec_entry() {
+    if constexpr (RT <= prec_rt_max(NC)) return k_prec_gemm_phased<NC, RT>;
+    else return nullptr;
 }
-
-typedef void (*PrecGemmKernel)(const SlotDesc *, int);
-static PrecGemmKernel prec_gemm_kernel(int nt) {
-    switch (nt) {
-    case 1: return k_prec_gemm<1>;
-    case 2: return k_prec_gemm<2>;
-    case 3: return k_prec_gemm<3>;
-    case 4: return k_prec_gemm<4>;
-    case 5: return k_prec_gemm<5>;
-    case 6: return k_prec_gemm<6>;
-    case 7: return k_prec_gemm<7>;
-    default: return k_prec_gemm<8>;
-    }
+template <int... I> static PrecPhased prec_phased(int nc, int rt, std::integer_sequence<int, I...>) {
+    static const PrecPhased k[] = {prec_entry<I / PREC_RT_SLOTS + 1, I % PREC_RT_SLOTS + 1>()...};
+    return k[(nc - 1) * PREC_RT_SLOTS + rt - 1];
 }
+static PrecPhased prec_phased(int nc, int rt) {
+    return prec_phased(nc, rt, std::make_integer_sequence<int, PREC_MAX_TILES * PREC_RT_SLOTS>());
+}
+template <int... I> static PrecPlain prec_plain(int nc, std::integer_sequence<int, I...>) {
+    static const PrecPlain k[] = {k_prec_gemm<I + 1>...};
+    return k[nc - 1];
+}
+static PrecPlain prec_plain(int nc) { return prec_plain(nc, std::make_integer_sequence<int, PREC_MAX_TILES>()); }
 
-typedef void (*PrecGemmPhased)(SlotDesc *, const uint32_t *, int, int);
-static PrecGemmPhased prec_gemm_phased_kernel(int nt) {
-    switch (nt) {
-    case 1: return k_prec_gemm_phased<1>;
-    case 2: return k_prec_gemm_phased<2>;
-    case 3: return k_prec_gemm_phased<3>;
-    case 4: return k_prec_gemm_phased<4>;
-    case 5: return k_prec_gemm_phased<5>;
-    case 6: return k_prec_gemm_phased<6>;
-    case 7: return k_prec_gemm_phased<7>;
-    default: return k_prec_gemm_phased<8>;
+int prec_choose_rt(int nc, const std::vector<int> &tiles) {
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        hipDeviceProp_t prop;
+        cus = (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+                  ? prop.multiProcessorCount
+                  : 256;
     }
+    const int rmax = prec_rt_max(nc);
+    if (const char *e = getenv("SS_PREC_RT")) {   // forced (measurement)
+        const int r = atoi(e);
+        if (r >= 1 && r <= rmax) return r;
+    }
+    // A workgroup of 4 wavefronts takes 4 rt row tiles of one subgame; as
+    // many as registers and LDS allow share a compute unit, whose SIMDs then
+    // run one wavefront per resident workgroup. Cost: rt x resident
+    // workgroups x dispatch rounds (row-tile contractions on the busiest SIMD).
+    int best = 1;
+    long best_cost = -1;
+    for (int rt = 1; rt <= rmax; ++rt) {
+        long g = 0;
+        for (int n : tiles) g += (n + 4 * rt - 1) / (4 * rt);
+        int cap = 1;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&cap, prec_phased(nc, rt), 64 * PREC_WAVES, 0) != hipSuccess)
+            cap = 1;
+        cap = std::max(1, cap);
+        const long rounds = (g + (long)cus * cap - 1) / ((long)cus * cap);
+        const long cost = (long)rt * cap * rounds;
+        if (best_cost < 0 || cost <= best_cost) best = rt, best_cost = cost;
+    }
+    return best;
 }
 
 // ---- host ------------------------------------------------------------------
 PrecState::~PrecState() {
-    for (void *p : {(void *)W[0], (void *)W[1], (void *)Bt, (void *)Cn, (void *)same[0], (void *)same[1],
-                    (void *)sd_nodes})
-        if (p) (void)hipFree(p);
+    if (block) (void)hipFree(block);
 }
 
 int prec_setup(Slot &s, hipStream_t st) {
@@ -425,18 +535,46 @@ int prec_setup(Slot &s, hipStream_t st) {
     std::vector<int> sd;
     for (int j = 0; j < pl.T; ++j)
         if (pl.nodes[pl.term[j]].type == SS_NODE_SHOWDOWN) sd.push_back(pl.term[j]);
-    if (sd.empty() || sd.size() > 16 * PREC_MAX_TILES)
+    if (sd.empty() || sd.size() > 8 * PREC_MAX_TILES)
         return set_error(SS_ERR_UNSUPPORTED, "precision form: %zu showdown nodes (1..%d supported)", sd.size(),
-                         16 * PREC_MAX_TILES);
+                         8 * PREC_MAX_TILES);
     std::unique_ptr<PrecState> p(new PrecState());
     const int Mp = pl.Mp;
     p->S = (int)sd.size();
-    p->S16 = (p->S + 15) / 16 * 16;
-    SS_HIP(hipMalloc(&p->sd_nodes, sizeof(int) * p->S));
+    p->NC = (2 * p->S + 15) / 16;
+    p->MpK = (Mp + 255) / 256 * 256;
+    // every buffer of the form in one allocation, rounded to 2 MiB, so the
+    // contraction's streams stay within a few large pages
+    const int nch = (Mp / 32 + PREC_KS - 1) / PREC_KS;
+    const size_t w_words = (size_t)nch * (Mp / 16) * 64 * (PREC_KS / 2);
+    const size_t bt_bytes = sizeof(uint16_t) * 16 * p->NC * (size_t)p->MpK;
+    const size_t w_bytes = sizeof(uint32_t) * (w_words + PREC_W_PAD);
+    auto up = [](size_t n) { return (n + 255) / 256 * 256; };
+    const size_t off_bt = 0, off_w0 = off_bt + up(bt_bytes), off_w1 = off_w0 + up(w_bytes),
+                 off_z = off_w1 + up(w_bytes), off_same = off_z + up(sizeof(float) * (size_t)p->S * Mp),
+                 off_sd = off_same + 2 * up(sizeof(int) * Mp), off_c = off_sd + up(sizeof(int) * p->S),
+                 total = off_c + up(sizeof(float4) * 2 * p->S);
+    SS_HIP(hipMalloc(&p->block, (total + (2u << 20) - 1) / (2u << 20) * (2u << 20)));
+    p->Bt = (uint16_t *)(p->block + off_bt);
+    p->W[0] = (uint32_t *)(p->block + off_w0);
+    p->W[1] = (uint32_t *)(p->block + off_w1);
+    p->Z = (float *)(p->block + off_z);
+    p->same[0] = (int *)(p->block + off_same);
+    p->same[1] = (int *)(p->block + off_same + up(sizeof(int) * Mp));
+    p->sd_nodes = (int *)(p->block + off_sd);
+    p->C = (float4 *)(p->block + off_c);
+    std::vector<float4> cn(2 * p->S);
+    for (int t = 0; t < 2; ++t)
+        for (int b = 0; b < p->S; ++b) {
+            const DevNode &nd = pl.nodes[sd[b]];
+            int bits = sd[b];
+            float id;
+            memcpy(&id, &bits, sizeof(id));
+            cn[(size_t)t * p->S + b] = make_float4(0.5f * (float)nd.pot, (float)nd.bets[t], id, 0.f);
+        }
+    SS_HIP(hipMemcpyAsync(p->C, cn.data(), sizeof(float4) * cn.size(), hipMemcpyHostToDevice, st));
     SS_HIP(hipMemcpyAsync(p->sd_nodes, sd.data(), sizeof(int) * p->S, hipMemcpyHostToDevice, st));
-    SS_HIP(hipMalloc(&p->Bt, sizeof(uint16_t) * 2 * p->S16 * Mp));
-    SS_HIP(hipMemsetAsync(p->Bt, 0, sizeof(uint16_t) * 2 * p->S16 * Mp, st));   // padding columns stay 0
-    SS_HIP(hipMalloc(&p->Cn, sizeof(float) * 64 * p->S));
+    SS_HIP(hipMemsetAsync(p->Bt, 0, bt_bytes, st));   // padding columns and k past Mp stay 0
     // opponent hand holding the same two cards as each traverser hand
     std::vector<int> same(Mp);
     for (int t = 0; t < 2; ++t) {
@@ -450,24 +588,25 @@ int prec_setup(Slot &s, hipStream_t st) {
             const uint32_t h = i < pl.nh[t] ? pl.hinfo[(size_t)t * Mp + i] : 0u;
             same[i] = (h & SS_LIVE_BIT) ? at[(h >> 12) & 2047] : -1;
         }
-        SS_HIP(hipMalloc(&p->same[t], sizeof(int) * Mp));
         SS_HIP(hipMemcpyAsync(p->same[t], same.data(), sizeof(int) * Mp, hipMemcpyHostToDevice, st));
         SS_HIP(hipStreamSynchronize(st));   // `same` is reused
-        SS_HIP(hipMalloc(&p->W[t], sizeof(uint2) * (size_t)Mp * (Mp / 32)));
-        hipLaunchKernelGGL(k_prec_w, dim3(std::min<int64_t>(4096, ((int64_t)Mp * (Mp / 32) + SS_BLOCK - 1) / SS_BLOCK)),
+        SS_HIP(hipMemsetAsync(p->W[t] + w_words, 0, sizeof(uint32_t) * PREC_W_PAD, st));
+        hipLaunchKernelGGL(k_prec_w, dim3((unsigned)std::min<size_t>(4096, (w_words / 2 + SS_BLOCK - 1) / SS_BLOCK)),
                            dim3(SS_BLOCK), 0, st, s.d_desc, t, p->W[t]);
     }
     SS_HIP(hipGetLastError());
     SlotDesc &d = s.desc;
     d.prec_S = p->S;
-    d.prec_S16 = p->S16;
+    d.prec_NC = p->NC;
+    d.prec_MpK = p->MpK;
     d.prec_sd = p->sd_nodes;
     d.prec_W[0] = p->W[0];
     d.prec_W[1] = p->W[1];
     d.prec_Bt = p->Bt;
-    d.prec_Cn = p->Cn;
+    d.prec_Z = p->Z;
     d.prec_same[0] = p->same[0];
     d.prec_same[1] = p->same[1];
+    d.prec_C = p->C;
     SS_HIP(hipMemcpyAsync(s.d_desc, &d, sizeof(d), hipMemcpyHostToDevice, st));
     SS_HIP(hipStreamSynchronize(st));
     s.prec = std::move(p);
@@ -478,8 +617,8 @@ int prec_setup(Slot &s, hipStream_t st) {
 int prec_showdowns(Slot &s, int t, hipStream_t st) {
     const PrecState &p = *s.prec;
     hipLaunchKernelGGL(k_prec_split, dim3(p.S), dim3(SS_BLOCK), 0, st, s.d_desc, t);
-    hipLaunchKernelGGL(prec_gemm_kernel(p.S16 / 16), dim3((s.plan.Mp + 63) / 64), dim3(64 * PREC_WAVES), 0,
-                       st, s.d_desc, t);
+    hipLaunchKernelGGL(prec_plain(p.NC), dim3((s.plan.Mp + 63) / 64), dim3(64 * PREC_WAVES), 0, st,
+                       s.d_desc, t);
     SS_HIP(hipGetLastError());
     return SS_OK;
 }
@@ -490,9 +629,9 @@ int launch_prec_phased(SlotDesc *d_slots, const PhasedTasks &pt, int t, hipStrea
                        pt.d_tasks + pt.psplit_off, pt.npsplit, t);
     int off = pt.pgemm_off;
     for (const auto &g : pt.pgemm) {
-        hipLaunchKernelGGL(prec_gemm_phased_kernel(g.first), dim3(g.second), dim3(64 * PREC_WAVES), 0, st, d_slots,
-                           pt.d_tasks + off, g.second, t);
-        off += g.second;
+        hipLaunchKernelGGL(prec_phased(g.nc, g.rt), dim3(g.count), dim3(64 * PREC_WAVES), 0, st, d_slots,
+                           pt.d_tasks + off, g.count, t);
+        off += g.count;
     }
     SS_HIP(hipGetLastError());
     return SS_OK;
@@ -500,7 +639,7 @@ int launch_prec_phased(SlotDesc *d_slots, const PhasedTasks &pt, int t, hipStrea
 
 int64_t prec_flops_per_pass(const Slot &s) {
     if (!s.prec) return 0;
-    return 2LL * s.plan.Mp * s.plan.Mp * 2 * s.prec->S16;
+    return 2LL * s.plan.Mp * s.plan.Mp * 16 * s.prec->NC;
 }
 
 }  // namespace ss
