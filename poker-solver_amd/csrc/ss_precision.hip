@@ -35,7 +35,6 @@ typedef __attribute__((__vector_size__(2 * sizeof(uint32_t)))) uint32_t u32x2;
 
 #define PREC_MAX_TILES 16   // ceil(2 S / 16) <= 16: at most 128 showdown nodes
 #define DN2_P (52 * 52)
-#define PREC_WAVES 4        // wavefronts per GEMM workgroup
 
 struct PrecArgs {
     int S, NC, MpK;           // showdown nodes, column tiles, Bt row length
@@ -125,8 +124,8 @@ __device__ __forceinline__ PrecArgs prec_args(const SlotDesc *__restrict__ s, in
 // and from them, per traverser hand i, the opponent reach disjoint from i:
 // z = S - C[x] - C[y] + r(same(i)) (S the total, C the per-card totals, the
 // hand holding i's cards counted twice), written as -1 where the showdown
-// value is 0 (dead or blocked hands, :861, and the mass floor). One
-// workgroup; M and C are LDS.
+// value is 0 (dead or blocked hands, :861, the mass floor, and the padding
+// hands past the traverser's count). One workgroup; M and C are LDS.
 __device__ void prec_split_node(const SlotDesc *__restrict__ s, const PrecArgs &a, int t, int b, float *M,
                                 float *C) {
     const int n = a.sd_nodes[b], o = 1 - t, Mp = s->Mp, nh = s->nh[o];
@@ -170,8 +169,8 @@ __device__ void prec_split_node(const SlotDesc *__restrict__ s, const PrecArgs &
     __syncthreads();
     const float Sb = C[52];
     const int nt = s->nh[t];
-    for (int i = threadIdx.x; i < nt; i += blockDim.x) {
-        const uint32_t hv = gld(s->hinfo + (size_t)t * Mp + i);
+    for (int i = threadIdx.x; i < Mp; i += blockDim.x) {   // padding hands (i >= nt) get -1: value 0
+        const uint32_t hv = i < nt ? gld(s->hinfo + (size_t)t * Mp + i) : 0u;
         const int sm = gld(a.same + i);
         const uint32_t cs = gld(s->cstr + min((int)((hv >> 12) & 2047), 1325));
         const float rv = sm >= 0 ? gld(r + sm) : 0.f;
@@ -202,57 +201,67 @@ __global__ void __launch_bounds__(SS_BLOCK) k_prec_split_phased(SlotDesc *__rest
 }
 
 // Y = W * [R_hi | R_lo] for a block of rows and all column tiles, then the
-// showdown values written into cfv. A workgroup of 4 wavefronts owns 64 RT
-// rows (each wavefront RT row tiles of 16 hands). K (opponent hands) is walked
-// in chunks of PREC_KS 32-k steps through a ring of NBUF LDS stages. A stage
-// holds the chunk's packed reach for all columns (shared by the workgroup) and
-// each wavefront's W words for its row tiles; both arrive by
-// global_load_lds_dwordx4 (global memory straight into LDS, no registers),
-// issued NBUF - 1 chunks ahead, so that NBUF - 1 chunks of MFMAs cover their
-// latency. Each chunk costs one barrier, after the wavefront has waited for
-// its own copies of that chunk (s_waitcnt vmcnt counts them: the loop issues
-// no other vector memory instruction). Bt is stored in 16-byte pieces of 8
-// consecutive k of one column, all columns of one 8-k group together, so a
-// chunk is one contiguous range and a lane's B fragment one piece, the 16
-// lanes of a column tile reading 256 contiguous bytes. K runs in order within
-// a wavefront whatever the row tiling, so every form rounds alike. MFMA
-// 16x16x32 lane layout: lane l holds A[l % 16][8 (l / 16) + 0..7],
+// showdown values written into cfv. A workgroup owns 64 RT rows in 4 row
+// slots of RT row tiles (16 hands each) and has KG wavefronts per slot, one
+// per K group: K (opponent hands) is walked in chunks of PREC_KS 32-k steps,
+// and K group g takes chunks g, g + KG, g + 2 KG, ... into its own
+// accumulators. Chunks travel through a ring of NBUF LDS stages, stage k
+// holding chunks KG k .. KG k + KG - 1 (one per K group): the chunk's packed
+// reach for all columns and each wavefront's W words for its row tiles, both
+// copied by global_load_lds_dwordx4 (global memory straight into LDS, no
+// registers) by the K group that uses the chunk, NBUF - 1 stages ahead. Each
+// stage costs one barrier, after each wavefront has waited for its own copies
+// of that stage (s_waitcnt vmcnt counts them: the loop issues no other vector
+// memory instruction). With two K groups each SIMD runs two wavefronts, one
+// computing while the other waits. The K groups' partial sums are added in a
+// fixed order (group 1's, then group 0's), so every form rounds alike. Bt is
+// stored in 16-byte pieces of 8 consecutive k of one column, all columns of
+// one 8-k group together, so a chunk is one contiguous range and a lane's B
+// fragment one piece, the 16 lanes of a column tile reading 256 contiguous
+// bytes. MFMA 16x16x32 lane layout: lane l holds A[l % 16][8 (l / 16) + 0..7],
 // B[8 (l / 16) + 0..7][l % 16], D[4 (l / 16) + 0..3][l % 16].
+constexpr int prec_kg(int nc) { return nc <= 4 ? 2 : 1; }   // K groups (two stages of reach fit LDS)
 template <int NC, int RT_> struct PrecShape {
     static constexpr int RT = RT_;                // row tiles per wavefront
+    static constexpr int KG = prec_kg(NC);        // K groups
+    static constexpr int WV = 4 * KG;             // wavefronts per workgroup
+    static constexpr int NTHR = 64 * WV;
     static constexpr int ROWS = 64 * RT;          // rows per workgroup / task
     static constexpr int KS = PREC_KS;            // 32-k steps per chunk
     static constexpr int WPC = KS / 2;            // W words per lane and row tile of a chunk
     static constexpr int NCOL = 16 * NC;          // columns (hi and lo)
     static constexpr int BP = 4 * KS * NCOL;      // 16-byte reach pieces per chunk
-    static constexpr int BPT = BP / (64 * PREC_WAVES);   // of them per thread
+    static constexpr int BPT = BP / 256;          // of them per thread of a K group
     static constexpr int WQ = RT * 64 * WPC / 4;  // 16-byte W pieces per wavefront and chunk
     static constexpr int WDMA = (WQ + 63) / 64;   // W copies per wavefront and chunk
-    static constexpr int STAGE = 16 * (BP + PREC_WAVES * WQ);   // bytes of one stage
-    static constexpr int NBUF = 4 * STAGE <= 106496 ? 4 : (3 * STAGE <= 122880 ? 3 : 2);
+    static constexpr int STAGE = 16 * (KG * BP + WV * WDMA * 64);   // bytes of one stage
+    static constexpr int NBUF = KG == 2 ? (4 * STAGE <= 155648 ? 4 : (3 * STAGE <= 155648 ? 3 : 2))
+                                        : (4 * STAGE <= 106496 ? 4 : (3 * STAGE <= 122880 ? 3 : 2));
     static constexpr int SMAX = 8 * NC;           // showdown nodes: 2 S <= 16 NC
-    static constexpr int RW = 16 * RT;            // rows per wavefront
+    static constexpr int RW = 16 * RT;            // rows per row slot
     static constexpr int YSTR = RW + 4;           // epilogue: column of accumulators, 16 lanes' 4-row writes a bank apart
-    static constexpr int ZPL = (RW * SMAX + 63) / 64;   // epilogue: (node, row) outputs per lane at most
-    static_assert(BP % (64 * PREC_WAVES) == 0, "a chunk's reach splits evenly over the workgroup");
+    static constexpr int RQ = RW / 4;             // epilogue: quads of 4 rows per row slot
+    static constexpr int QPL = (RQ * SMAX + 64 * KG - 1) / (64 * KG);   // epilogue: (node, quad) outputs per lane at most
+    static_assert(BP % 256 == 0, "a chunk's reach splits evenly over a K group");
 };
 // row tiles per wavefront: the batched form picks one of 1 .. prec_rt_max(NC)
-// per launch (the more column tiles, the fewer rows, so the accumulators stay
-// within the register file); the plain form uses one row tile, for more
-// workgroups over a single subgame. The K order is the same for any row
-// tiling, so both forms round alike.
-constexpr int prec_rt_max(int nc) { return nc <= 4 ? 6 : (nc <= 8 ? 2 : 1); }
+// per launch (the more column tiles, the fewer rows, so the accumulators and
+// fragments stay within the 256 registers of two wavefronts per SIMD); the
+// plain form uses one row tile, for more workgroups over a single subgame.
+// The K order is the same for any row tiling, so both forms round alike.
+constexpr int prec_rt_max(int nc) { return nc <= 2 ? 6 : (nc == 3 ? 5 : (nc == 4 ? 3 : (nc <= 8 ? 2 : 1))); }
+constexpr int prec_threads(int nc) { return 256 * prec_kg(nc); }
 
 template <int NC, int RT>
 struct PrecLds {
     using SH = PrecShape<NC, RT>;
     union {
         struct {
-            u32x4 B[SH::NBUF][SH::BP];                          // reach pieces of the chunk
-            u32x4 W[SH::NBUF][PREC_WAVES][SH::WDMA * 64];       // each wavefront's W words of the chunk
+            u32x4 B[SH::NBUF][SH::KG][SH::BP];                  // reach pieces of the stage's chunks
+            u32x4 W[SH::NBUF][SH::WV][SH::WDMA * 64];           // each wavefront's W words of its chunk
         } st;
         struct {
-            float Ys[PREC_WAVES][SH::NCOL][SH::YSTR];           // each wavefront's accumulators, column-major
+            float Ys[SH::KG][4][SH::NCOL][SH::YSTR];            // each K group's and row slot's sums, column-major
             alignas(16) float Cs[SH::SMAX][4];                  // per node: P/2, traverser bet, node id
         } ep;
     } u;
@@ -288,40 +297,45 @@ __device__ __forceinline__ bf16x8 prec_expand(uint32_t f) {
                                    __builtin_shufflevector(p2, p3, 0, 1, 2, 3), 0, 1, 2, 3, 4, 5, 6, 7);
 }
 
-// the steps of one staged chunk (the first `steps` of them when GUARD: the
-// last chunk, whose later steps lie past the subgame's hands)
-template <int NC, int RT, bool GUARD>
-__device__ __forceinline__ void prec_chunk(const u32x4 *Bc, const uint32_t *Wc, floatx4 (&acc)[RT][NC], int l,
-                                           int steps) {
+// one chunk's fragments in registers: the lane's reach fragments of every
+// step and column tile, and its W words of every row tile
+template <int NC, int RT> struct PrecFrag {
+    bf16x8 vb[PREC_KS][NC];
+    uint32_t wd[RT][PREC_KS / 2];
+};
+template <int NC, int RT>
+__device__ __forceinline__ void prec_load_frag(PrecFrag<NC, RT> &F, const u32x4 *Bc, const uint32_t *Wc, int l) {
     using SH = PrecShape<NC, RT>;
     const int lane_piece = (l >> 4) * SH::NCOL + (l & 15);
-    // every LDS read of the chunk first (one latency per chunk, not per step)
-    uint32_t wd[RT][SH::WPC];
 #pragma unroll
     for (int r = 0; r < RT; ++r) {
         const u32x2 v = *(const u32x2 *)(Wc + (r * 64 + l) * SH::WPC);
-        wd[r][0] = v[0], wd[r][1] = v[1];
+        F.wd[r][0] = v[0], F.wd[r][1] = v[1];
     }
-    bf16x8 vb[SH::KS][NC];
 #pragma unroll
     for (int ks = 0; ks < SH::KS; ++ks)
 #pragma unroll
         for (int q = 0; q < NC; ++q)
-            vb[ks][q] = __builtin_bit_cast(bf16x8, Bc[4 * ks * SH::NCOL + 16 * q + lane_piece]);
+            F.vb[ks][q] = __builtin_bit_cast(bf16x8, Bc[4 * ks * SH::NCOL + 16 * q + lane_piece]);
+}
+// the steps of one chunk (the first `steps` of them when GUARD: the last
+// chunk, whose later steps lie past the subgame's hands)
+template <int NC, int RT, bool GUARD>
+__device__ __forceinline__ void prec_chunk(const PrecFrag<NC, RT> &F, floatx4 (&acc)[RT][NC], int steps) {
 #pragma unroll
-    for (int ks = 0; ks < SH::KS; ++ks) {
+    for (int ks = 0; ks < PREC_KS; ++ks) {
         if (GUARD && ks >= steps) break;
         bf16x8 va[RT];
 #pragma unroll
         for (int r = 0; r < RT; ++r) {
-            const uint32_t word = wd[r][ks >> 1];
+            const uint32_t word = F.wd[r][ks >> 1];
             va[r] = prec_expand(((ks & 1) ? word << 1 : word) & 0xAAAAAAAAu);
         }
 #pragma unroll
         for (int r = 0; r < RT; ++r)
 #pragma unroll
             for (int q = 0; q < NC; ++q)
-                acc[r][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va[r], vb[ks][q], acc[r][q], 0, 0, 0);
+                acc[r][q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va[r], F.vb[ks][q], acc[r][q], 0, 0, 0);
     }
 }
 
@@ -329,93 +343,115 @@ template <int NC, int RT>
 __device__ void prec_gemm_block(const SlotDesc *__restrict__ s, const PrecArgs &a, int t, int row0,
                                 PrecLds<NC, RT> &L) {
     using SH = PrecShape<NC, RT>;
-    constexpr int KS = SH::KS, BP = SH::BP, BPT = SH::BPT, WPC = SH::WPC, NBUF = SH::NBUF, NTHR = 64 * PREC_WAVES;
-    constexpr int RW = SH::RW, ZPL = SH::ZPL, WQ = SH::WQ, WDMA = SH::WDMA;
+    constexpr int KS = SH::KS, KG = SH::KG, BP = SH::BP, BPT = SH::BPT, WPC = SH::WPC, NBUF = SH::NBUF;
+    constexpr int RQ = SH::RQ, QPL = SH::QPL, WQ = SH::WQ, WDMA = SH::WDMA;
     const int tid = threadIdx.x, l = tid & 63, w = tid >> 6, g = l >> 4, Mp = s->Mp;
-    const int nq = Mp / 32, nch = (nq + KS - 1) / KS, ntile = Mp / 16;
-    const int rt0 = (row0 >> 4) + RT * w;   // this wavefront's first row tile
-    const bool live = rt0 < ntile;          // past the subgame's rows: the wavefront only stages reach
+    const int slot = w & 3, kg = w >> 2;    // row slot and K group of this wavefront
+    const int nq = Mp / 32, nch = (nq + KS - 1) / KS, nst = (nch + KG - 1) / KG, ntile = Mp / 16;
+    const int rt0 = (row0 >> 4) + RT * slot;   // this wavefront's first row tile
+    const bool live = rt0 < ntile;             // past the subgame's rows: the wavefront only stages reach
     const u32x4 *Bg = (const u32x4 *)a.Bt;
     const u32x4 *Wg = (const u32x4 *)a.W;
-    // chunk c into stage c % NBUF: the workgroup's reach pieces (each
-    // wavefront 1 KB blocks), then a live wavefront's W words for its row
-    // tiles (contiguous: RT row tiles of chunk c; a partly covered wavefront
-    // reads on into the next chunk or the padding, rows it never writes)
-    auto issue = [&](int c) {
-        u32x4 *Bs = L.u.st.B[c % NBUF];
+    auto mine = [&](int k) { return KG * k + kg < nch; };   // stage k holds a chunk of this K group
+    // stage k % NBUF gets chunk KG k + kg from this wavefront's K group: the
+    // reach pieces (each wavefront of the group 1 KB blocks), then a live
+    // wavefront's W words for its row tiles (contiguous: RT row tiles of the
+    // chunk; a partly covered wavefront reads on into the next chunk or the
+    // padding, rows it never writes)
+    auto issue = [&](int k) {
+        if (!mine(k)) return;
+        const int c = KG * k + kg;
+        u32x4 *Bs = L.u.st.B[k % NBUF][kg];
 #pragma unroll
-        for (int u = 0; u < BPT; ++u) prec_dma(Bg + (size_t)c * BP + u * NTHR + tid, Bs + u * NTHR + 64 * w);
+        for (int u = 0; u < BPT; ++u) prec_dma(Bg + (size_t)c * BP + u * 256 + 64 * slot + l, Bs + u * 256 + 64 * slot);
         if (live) {
             const u32x4 *src = Wg + ((size_t)c * ntile + rt0) * 64 * WPC / 4;
-            u32x4 *dst = L.u.st.W[c % NBUF][w];
+            u32x4 *dst = L.u.st.W[k % NBUF][w];
 #pragma unroll
             for (int u = 0; u < WDMA; ++u)
                 if (WQ % 64 == 0 || u * 64 + l < WQ) prec_dma(src + u * 64 + l, dst + u * 64);
         }
     };
-    // vector memory instructions of one chunk: the wait for chunk c leaves
-    // the chunks issued after it in flight
+    // the wait for stage k leaves this wavefront's copies of the stages
+    // issued after it in flight (vector memory instructions per stage: D)
     constexpr int D_LIVE = BPT + WDMA, D_DEAD = BPT;
-    auto wait_chunk = [&](int c) {
-        const int after = min(NBUF - 2, nch - 1 - c);   // chunks issued after chunk c
+    auto wait_stage = [&](int k, int last) {   // last: the latest stage issued
+        int after = 0;
+        for (int j = k + 1; j <= last && j < nst; ++j) after += mine(j) ? 1 : 0;
         if (live) {
-            if (after >= 2) prec_wait_vm<2 * D_LIVE>();
+            if (after >= 3) prec_wait_vm<3 * D_LIVE>();
+            else if (after == 2) prec_wait_vm<2 * D_LIVE>();
             else if (after == 1) prec_wait_vm<D_LIVE>();
             else prec_wait_vm<0>();
         } else {
-            if (after >= 2) prec_wait_vm<2 * D_DEAD>();
+            if (after >= 3) prec_wait_vm<3 * D_DEAD>();
+            else if (after == 2) prec_wait_vm<2 * D_DEAD>();
             else if (after == 1) prec_wait_vm<D_DEAD>();
             else prec_wait_vm<0>();
         }
     };
-    static_assert(NBUF <= 4 && 2 * D_LIVE < 64 && ZPL + 1 < 64, "vmcnt waits are 6 bits");
-    for (int c = 0; c < NBUF - 1 && c < nch; ++c) issue(c);
+    static_assert(NBUF <= 4 && 3 * D_LIVE < 64, "vmcnt waits are 6 bits");
+    // the epilogue's inputs, issued after the last stage's wait, in flight
+    // during the last chunk: per node its half pot, the traverser's bet and
+    // its id (one load per thread, clamped), and the split kernel's
+    // card-removal masses. Outputs go by quads of 4 consecutive rows of one
+    // node (16-byte loads and stores); of a row slot's quads, the wavefront of
+    // K group kg takes quad KG 64 u + 64 kg + l, node-major.
+    floatx4 cnode, zr[QPL];
+    auto epilogue_loads = [&]() {
+        cnode = gld(a.C + min(tid, a.S - 1));
+        if (live) {
+#pragma unroll
+            for (int u = 0; u < QPL; ++u) {
+                const int o = 64 * (KG * u + kg) + l, b = min(o / RQ, a.S - 1),
+                          i = min(16 * rt0 + 4 * (o % RQ), Mp - 4);
+                zr[u] = gld((const floatx4 *)(a.Z + (size_t)b * Mp + i));
+            }
+        }
+    };
     floatx4 acc[RT][NC];
 #pragma unroll
     for (int r = 0; r < RT; ++r)
 #pragma unroll
         for (int q = 0; q < NC; ++q) acc[r][q] = floatx4{0.f, 0.f, 0.f, 0.f};
-    for (int c = 0; c + 1 < nch; ++c) {
-        wait_chunk(c);
-        prec_barrier();   // chunk c in LDS; chunk c - 1's stage free
-        if (c + NBUF - 1 < nch) issue(c + NBUF - 1);
-        if (live)
-            prec_chunk<NC, RT, false>(L.u.st.B[c % NBUF], (const uint32_t *)L.u.st.W[c % NBUF][w], acc, l, KS);
-    }
-    // the epilogue's inputs, issued after every copy and left in flight by
-    // the last chunk's wait: per node its half pot, the traverser's bet and
-    // its id (one load per thread, clamped), and per (node, row) output the
-    // split kernel's card-removal mass (lane l takes outputs l + 64 u,
-    // node-major over the wavefront's rows)
-    const floatx4 cnode = gld(a.C + min(tid, a.S - 1));
-    float zr[ZPL];
-    if (live) {
-#pragma unroll
-        for (int u = 0; u < ZPL; ++u) {
-            const int o = l + 64 * u, b = min(o / RW, a.S - 1), i = min(16 * rt0 + o % RW, Mp - 1);
-            zr[u] = gld(a.Z + (size_t)b * Mp + i);
+    // Every stage before the last holds full chunks for both K groups (the
+    // last chunk, possibly short, lies in the last stage), so the loop has one
+    // compute body and the last stage its own.
+    for (int k = 0; k < NBUF - 1 && k < nst; ++k) issue(k);
+    for (int k = 0; k + 1 < nst; ++k) {
+        wait_stage(k, k + NBUF - 2);
+        prec_barrier();   // stage k in LDS; stage k - 1 read by every wavefront
+        if (k + NBUF - 1 < nst) issue(k + NBUF - 1);
+        if (live) {
+            PrecFrag<NC, RT> F;
+            prec_load_frag(F, L.u.st.B[k % NBUF][kg], (const uint32_t *)L.u.st.W[k % NBUF][w], l);
+            prec_chunk<NC, RT, false>(F, acc, KS);
         }
     }
     {
-        const int c = nch - 1;
-        if (live) prec_wait_vm<ZPL + 1>();
-        else prec_wait_vm<1>();
+        const int k = nst - 1, c = KG * k + kg;
+        wait_stage(k, k);
         prec_barrier();
-        if (live)
-            prec_chunk<NC, RT, true>(L.u.st.B[c % NBUF], (const uint32_t *)L.u.st.W[c % NBUF][w], acc, l,
-                                     nq - c * KS);
+        epilogue_loads();
+        if (live && c < nch) {
+            PrecFrag<NC, RT> F;
+            prec_load_frag(F, L.u.st.B[k % NBUF][kg], (const uint32_t *)L.u.st.W[k % NBUF][w], l);
+            prec_chunk<NC, RT, true>(F, acc, nq - c * KS);
+        }
     }
     prec_barrier();   // every wavefront is done with the stages
-    // epilogue: the accumulators go to LDS column-major over the stages (a
-    // lane's 4 rows in one write), then the lanes walk the (node, row)
-    // outputs node-major, so the cfv stores run along hands. Y = hi + lo in
-    // that order in every form.
+    // epilogue: each K group's sums go to LDS column-major over the stages (a
+    // lane's 4 rows in one write), then the lanes walk the (node, quad)
+    // outputs node-major, so the cfv stores run along hands. The column sums
+    // are group 1's + group 0's (one group: its own), and Y = hi + lo, in that
+    // order in every form. Quads of padding hands store 0 (their z is -1);
+    // quads past Mp (clamped row tiles) store nothing.
     if (live) {
 #pragma unroll
         for (int r = 0; r < RT; ++r)
 #pragma unroll
             for (int q = 0; q < NC; ++q)
-                *(floatx4 *)&L.u.ep.Ys[w][16 * q + (l & 15)][16 * r + 4 * g] = acc[r][q];
+                *(floatx4 *)&L.u.ep.Ys[kg][slot][16 * q + (l & 15)][16 * r + 4 * g] = acc[r][q];
     }
     if (tid < a.S) *(floatx4 *)L.u.ep.Cs[tid] = cnode;
     __syncthreads();
@@ -423,28 +459,32 @@ __device__ void prec_gemm_block(const SlotDesc *__restrict__ s, const PrecArgs &
         // every input of the lane's outputs read first (indices clamped) and
         // held together, so the LDS reads go out back to back with one wait;
         // then the values, and the stores of the outputs that exist
-        const int nh = s->nh[t];
         float *cfv = s->cfv;   // read once: the stores below could otherwise alias the descriptor
-        float yh[ZPL], yl[ZPL], hf[ZPL], bt[ZPL];
-        int nd[ZPL];
+        floatx4 yh[QPL], yl[QPL], cb[QPL];
 #pragma unroll
-        for (int u = 0; u < ZPL; ++u) {
-            const int o = l + 64 * u, b = min(o / RW, a.S - 1), lr = o % RW;
-            const floatx4 cb = *(const floatx4 *)L.u.ep.Cs[b];
-            yh[u] = L.u.ep.Ys[w][b][lr];
-            yl[u] = L.u.ep.Ys[w][a.S + b][lr];
-            hf[u] = cb[0];
-            bt[u] = cb[1];
-            nd[u] = __float_as_int(cb[2]);
+        for (int u = 0; u < QPL; ++u) {
+            const int o = 64 * (KG * u + kg) + l, b = min(o / RQ, a.S - 1), r4 = 4 * (o % RQ);
+            yh[u] = *(const floatx4 *)&L.u.ep.Ys[KG - 1][slot][b][r4];
+            yl[u] = *(const floatx4 *)&L.u.ep.Ys[KG - 1][slot][a.S + b][r4];
+            if (KG == 2) {
+                yh[u] = yh[u] + *(const floatx4 *)&L.u.ep.Ys[0][slot][b][r4];
+                yl[u] = yl[u] + *(const floatx4 *)&L.u.ep.Ys[0][slot][a.S + b][r4];
+            }
+            cb[u] = *(const floatx4 *)L.u.ep.Cs[b];
         }
 #pragma unroll
-        for (int u = 0; u < ZPL; ++u) asm volatile("" ::"v"(yh[u]), "v"(yl[u]), "v"(hf[u]), "v"(bt[u]), "v"(nd[u]));
+        for (int u = 0; u < QPL; ++u) asm volatile("" ::"v"(yh[u]), "v"(yl[u]), "v"(cb[u]));
 #pragma unroll
-        for (int u = 0; u < ZPL; ++u) {
-            const int o = l + 64 * u, i = 16 * rt0 + o % RW;
-            const float Y = yh[u] + yl[u], z = zr[u];
-            const float val = z < 0.f ? 0.f : hf[u] * Y + (hf[u] - bt[u]) * z;
-            if (o / RW < a.S && i < nh) gst(cfv + (size_t)nd[u] * Mp + i, val);
+        for (int u = 0; u < QPL; ++u) {
+            const int o = 64 * (KG * u + kg) + l, i = 16 * rt0 + 4 * (o % RQ);
+            const float half = cb[u][0], bet = cb[u][1];
+            floatx4 val;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const float Y = yh[u][e] + yl[u][e], z = zr[u][e];
+                val[e] = z < 0.f ? 0.f : half * Y + (half - bet) * z;
+            }
+            if (o / RQ < a.S && i < Mp) gst((floatx4 *)(cfv + (size_t)__float_as_int(cb[u][2]) * Mp + i), val);
         }
     }
     prec_barrier();   // LDS is reused by the workgroup's next task (the stores need no wait)
@@ -452,13 +492,13 @@ __device__ void prec_gemm_block(const SlotDesc *__restrict__ s, const PrecArgs &
 
 // plain form: workgroup b covers rows [64 b, 64 (b + 1))
 template <int NC>
-__global__ void __launch_bounds__(64 * PREC_WAVES) k_prec_gemm(const SlotDesc *__restrict__ s, int t) {
+__global__ void __launch_bounds__(prec_threads(NC)) k_prec_gemm(const SlotDesc *__restrict__ s, int t) {
     __shared__ PrecLds<NC, 1> L;
     prec_gemm_block<NC, 1>(s, prec_args(s, t), t, PrecShape<NC, 1>::ROWS * blockIdx.x, L);
 }
 
 template <int NC, int RT>
-__global__ void __launch_bounds__(64 * PREC_WAVES) k_prec_gemm_phased(SlotDesc *__restrict__ slots,
+__global__ void __launch_bounds__(prec_threads(NC)) k_prec_gemm_phased(SlotDesc *__restrict__ slots,
                                                                       const uint32_t *tasks, int ntasks, int t) {
     __shared__ PrecLds<NC, RT> L;
     for (int j = blockIdx.x; j < ntasks; j += gridDim.x) {
@@ -503,17 +543,18 @@ int prec_choose_rt(int nc, const std::vector<int> &tiles) {
         const int r = atoi(e);
         if (r >= 1 && r <= rmax) return r;
     }
-    // A workgroup of 4 wavefronts takes 4 rt row tiles of one subgame; as
-    // many as registers and LDS allow share a compute unit, whose SIMDs then
-    // run one wavefront per resident workgroup. Cost: rt x resident
-    // workgroups x dispatch rounds (row-tile contractions on the busiest SIMD).
+    // A workgroup takes 4 rt row tiles of one subgame (4 row slots, each
+    // with one wavefront per K group); as many as registers and LDS allow
+    // share a compute unit, whose SIMDs then run one row slot per resident
+    // workgroup. Cost: rt x resident workgroups x dispatch rounds (row-tile
+    // contractions on the busiest SIMD).
     int best = 1;
     long best_cost = -1;
     for (int rt = 1; rt <= rmax; ++rt) {
         long g = 0;
         for (int n : tiles) g += (n + 4 * rt - 1) / (4 * rt);
         int cap = 1;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&cap, prec_phased(nc, rt), 64 * PREC_WAVES, 0) != hipSuccess)
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&cap, prec_phased(nc, rt), prec_threads(nc), 0) != hipSuccess)
             cap = 1;
         cap = std::max(1, cap);
         const long rounds = (g + (long)cus * cap - 1) / ((long)cus * cap);
@@ -617,7 +658,7 @@ int prec_setup(Slot &s, hipStream_t st) {
 int prec_showdowns(Slot &s, int t, hipStream_t st) {
     const PrecState &p = *s.prec;
     hipLaunchKernelGGL(k_prec_split, dim3(p.S), dim3(SS_BLOCK), 0, st, s.d_desc, t);
-    hipLaunchKernelGGL(prec_plain(p.NC), dim3((s.plan.Mp + 63) / 64), dim3(64 * PREC_WAVES), 0, st,
+    hipLaunchKernelGGL(prec_plain(p.NC), dim3((s.plan.Mp + 63) / 64), dim3(prec_threads(p.NC)), 0, st,
                        s.d_desc, t);
     SS_HIP(hipGetLastError());
     return SS_OK;
@@ -629,7 +670,7 @@ int launch_prec_phased(SlotDesc *d_slots, const PhasedTasks &pt, int t, hipStrea
                        pt.d_tasks + pt.psplit_off, pt.npsplit, t);
     int off = pt.pgemm_off;
     for (const auto &g : pt.pgemm) {
-        hipLaunchKernelGGL(prec_phased(g.nc, g.rt), dim3(g.count), dim3(64 * PREC_WAVES), 0, st, d_slots,
+        hipLaunchKernelGGL(prec_phased(g.nc, g.rt), dim3(g.count), dim3(prec_threads(g.nc)), 0, st, d_slots,
                            pt.d_tasks + off, g.count, t);
         off += g.count;
     }
