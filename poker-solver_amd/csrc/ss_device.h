@@ -108,10 +108,10 @@ struct SlotDesc {
     int prec_MpK;              // row length of prec_Bt: Mp rounded up to 256
     const int *prec_sd;        // [prec_S] showdown node ids
     const uint32_t *prec_W[2]; // per traverser: W as 2-bit FP4 fragments, lane-ordered
-    uint16_t *prec_Bt;         // [16 prec_NC][prec_MpK] bf16 reach: column b hi, column S + b lo
+    uint16_t *prec_Bt;         // [prec_MpK / 8][16 prec_NC][8] bf16 reach: column b hi, column S + b lo
     float *prec_Z;             // [prec_S][Mp] card-removal mass z per node and traverser hand, -1 for value 0
-    const int *prec_same[2];   // per traverser [Mp] opponent hand with the same cards, or -1
-    const float4 *prec_C;      // [2][prec_S] per traverser and node: P/2, traverser bet, node id (bits), 0
+    const int2 *prec_zi[2];    // per traverser [Mp] hand: x = c0 | c1 << 6 | valid << 12, y = opponent hand with its cards or -1
+    const float4 *prec_C;      // [2][prec_S] per traverser and node: P/2, traverser bet, node id, opponent reach row (bits)
     // exploitability scratch (FP64), allocated on demand
     float *avg;                // [A][Mp]
     double *xreach;            // [NP][N][Mp]

@@ -89,8 +89,8 @@ struct PrecState {
     uint32_t *W[2] = {nullptr, nullptr};  // per traverser: W (+1 / -1 / 0) as lane-ordered FP4 fragments
     uint16_t *Bt = nullptr;               // [16 NC][MpK] bf16 opponent reach, hi and lo columns packed
     float *Z = nullptr;                   // [S][Mp] card-removal mass per node and traverser hand
-    int *same[2] = {nullptr, nullptr};    // [Mp] opponent hand with the same cards, or -1
-    float4 *C = nullptr;                  // [2][S] per traverser and node: P/2, traverser bet, node id
+    int2 *zi[2] = {nullptr, nullptr};     // [Mp] per hand: cards and validity, opponent hand with its cards
+    float4 *C = nullptr;                  // [2][S] per traverser and node: P/2, traverser bet, node id, opponent reach row
     ~PrecState();
 };
 

@@ -42,8 +42,8 @@ struct PrecArgs {
     const uint32_t *W;        // FP4 fragments of W (k_prec_w)
     uint16_t *Bt;             // [MpK / 8][16 NC][8] bf16 bits: column b hi, column S + b lo, others 0
     float *Z;                 // [S][Mp]: card-removal mass per node and traverser hand, -1 where the value is 0
-    const int *same;          // [Mp] opponent hand with the traverser hand's cards, or -1
-    const floatx4 *C;         // [S] per node: P/2, the traverser's bet, node id (bits), 0
+    const int2 *zi;           // [Mp] traverser hand: x = c0 | c1 << 6 | valid << 12, y = same-cards opponent hand or -1
+    const floatx4 *C;         // [S] per node: P/2, the traverser's bet, node id, opponent's reach row (bits)
 };
 
 // round-to-nearest-even float -> bf16 bits (finite inputs)
@@ -114,9 +114,21 @@ __device__ __forceinline__ PrecArgs prec_args(const SlotDesc *__restrict__ s, in
     a.W = s->prec_W[t];
     a.Bt = s->prec_Bt;
     a.Z = s->prec_Z;
-    a.same = s->prec_same[t];
+    a.zi = s->prec_zi[t];
     a.C = (const floatx4 *)(s->prec_C + (size_t)t * s->prec_S);
     return a;
+}
+
+// A traverser hand's static inputs of the card-removal mass, packed at setup:
+// its cards and whether it takes part (live, not blocked), and the opponent
+// hand holding the same two cards. zi comes in holding {0, same-cards hand}.
+__global__ void __launch_bounds__(SS_BLOCK) k_prec_zinfo(const SlotDesc *__restrict__ s, int t, int2 *zi) {
+    const int Mp = s->Mp;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < Mp; i += gridDim.x * blockDim.x) {
+        const uint32_t hv = i < s->nh[t] ? s->hinfo[(size_t)t * Mp + i] : 0u;
+        const bool valid = (hv & SS_LIVE_BIT) && s->cstr[min((int)((hv >> 12) & 2047), 1325)];
+        zi[i].x = valid ? (int)((hv & 63) | ((hv >> 6) & 63) << 6 | 1u << 12) : 0;
+    }
 }
 
 // Showdown node b of traverser t's pass: the opponent's reach as bf16 hi / lo
@@ -125,35 +137,51 @@ __device__ __forceinline__ PrecArgs prec_args(const SlotDesc *__restrict__ s, in
 // z = S - C[x] - C[y] + r(same(i)) (S the total, C the per-card totals, the
 // hand holding i's cards counted twice), written as -1 where the showdown
 // value is 0 (dead or blocked hands, :861, the mass floor, and the padding
-// hands past the traverser's count). One workgroup; M and C are LDS.
+// hands past the traverser's count). One workgroup; M and C are LDS. Both
+// passes over the hands keep SPLIT_U hands' loads per thread in flight.
+#define SPLIT_U 6
 __device__ void prec_split_node(const SlotDesc *__restrict__ s, const PrecArgs &a, int t, int b, float *M,
                                 float *C) {
-    const int n = a.sd_nodes[b], o = 1 - t, Mp = s->Mp, nh = s->nh[o];
-    const float *r = s->reach + ((size_t)o * s->N + s->rrow[(size_t)o * s->N + n]) * Mp;
-    for (int i = threadIdx.x; i < DN2_P; i += blockDim.x) M[i] = 0.f;
+    const int o = 1 - t, Mp = s->Mp, nh = s->nh[o], bd = blockDim.x;
+    const int rrow = __float_as_int(gld(a.C + b)[3]);   // the opponent's reach row at the node
+    const float *r = s->reach + ((size_t)o * s->N + rrow) * Mp;
+    const uint32_t *ho = s->hinfo + (size_t)o * Mp;
+    for (int i = threadIdx.x; i < DN2_P; i += bd) M[i] = 0.f;
     __syncthreads();
     // one pass over the opponent's hands: hi / lo split and the card-pair scatter
-    for (int j = threadIdx.x; j < Mp; j += blockDim.x) {
-        const uint32_t hj = j < nh ? gld(s->hinfo + (size_t)o * Mp + j) : 0u;
-        const bool ok = hj & SS_LIVE_BIT;
-        const float x = ok ? gld(r + j) : 0.f;
-        const uint16_t h = bf16_rne(x);
-        uint16_t *piece = a.Bt + (size_t)(j >> 3) * 16 * a.NC * 8 + (j & 7);   // k group j / 8, element j % 8
-        piece[8 * b] = h;
-        piece[8 * (a.S + b)] = bf16_rne(x - bf16_to_f(h));
-        if (!ok) continue;
-        const int c0 = hj & 63, c1 = (hj >> 6) & 63;
-        if (s->uniq) {   // one writer per cell
-            M[c0 * 52 + c1] = x;
-            M[c1 * 52 + c0] = x;
-        } else {
-            atomicAdd(&M[c0 * 52 + c1], x);
-            atomicAdd(&M[c1 * 52 + c0], x);
+    for (int j0 = threadIdx.x; j0 < Mp; j0 += SPLIT_U * bd) {
+        uint32_t hj[SPLIT_U];
+        float xr[SPLIT_U];
+#pragma unroll
+        for (int u = 0; u < SPLIT_U; ++u) {
+            const int j = j0 + u * bd;
+            hj[u] = j < nh ? gld(ho + j) : 0u;
+            xr[u] = j < Mp ? gld(r + j) : 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < SPLIT_U; ++u) {
+            const int j = j0 + u * bd;
+            if (j >= Mp) break;
+            const bool ok = hj[u] & SS_LIVE_BIT;
+            const float x = ok ? xr[u] : 0.f;
+            const uint16_t h = bf16_rne(x);
+            uint16_t *piece = a.Bt + (size_t)(j >> 3) * 16 * a.NC * 8 + (j & 7);   // k group j / 8, element j % 8
+            piece[8 * b] = h;
+            piece[8 * (a.S + b)] = bf16_rne(x - bf16_to_f(h));
+            if (!ok) continue;
+            const int c0 = hj[u] & 63, c1 = (hj[u] >> 6) & 63;
+            if (s->uniq) {   // one writer per cell
+                M[c0 * 52 + c1] = x;
+                M[c1 * 52 + c0] = x;
+            } else {
+                atomicAdd(&M[c0 * 52 + c1], x);
+                atomicAdd(&M[c1 * 52 + c0], x);
+            }
         }
     }
     __syncthreads();
     const int lane = threadIdx.x & 63;
-    for (int c = threadIdx.x >> 6; c < 52; c += blockDim.x >> 6) {
+    for (int c = threadIdx.x >> 6; c < 52; c += bd >> 6) {
         float v = lane < 52 ? M[c * 52 + lane] : 0.f;
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -168,15 +196,26 @@ __device__ void prec_split_node(const SlotDesc *__restrict__ s, const PrecArgs &
     }
     __syncthreads();
     const float Sb = C[52];
-    const int nt = s->nh[t];
-    for (int i = threadIdx.x; i < Mp; i += blockDim.x) {   // padding hands (i >= nt) get -1: value 0
-        const uint32_t hv = i < nt ? gld(s->hinfo + (size_t)t * Mp + i) : 0u;
-        const int sm = gld(a.same + i);
-        const uint32_t cs = gld(s->cstr + min((int)((hv >> 12) & 2047), 1325));
-        const float rv = sm >= 0 ? gld(r + sm) : 0.f;
-        const float z = Sb - C[hv & 63] - C[(hv >> 6) & 63] + rv;
-        const bool zero = !((hv & SS_LIVE_BIT) && cs) || z < 1e-5f * Sb;
-        a.Z[(size_t)b * Mp + i] = zero ? -1.f : z;
+    float *Zb = a.Z + (size_t)b * Mp;
+    for (int i0 = threadIdx.x; i0 < Mp; i0 += SPLIT_U * bd) {   // padding hands have valid = 0: -1
+        int2 zi[SPLIT_U];
+        float rv[SPLIT_U];
+#pragma unroll
+        for (int u = 0; u < SPLIT_U; ++u) {
+            const int i = i0 + u * bd;
+            zi[u] = i < Mp ? gld(a.zi + i) : make_int2(0, -1);
+        }
+#pragma unroll
+        for (int u = 0; u < SPLIT_U; ++u) rv[u] = zi[u].y >= 0 ? gld(r + zi[u].y) : 0.f;
+#pragma unroll
+        for (int u = 0; u < SPLIT_U; ++u) {
+            const int i = i0 + u * bd;
+            if (i >= Mp) break;
+            const int x = zi[u].x & 63, y = (zi[u].x >> 6) & 63;
+            const float z = Sb - C[x] - C[y] + rv[u];
+            const bool zero = !(zi[u].x >> 12) || z < 1e-5f * Sb;
+            gst(Zb + i, zero ? -1.f : z);
+        }
     }
     __syncthreads();
 }
@@ -592,16 +631,16 @@ int prec_setup(Slot &s, hipStream_t st) {
     const size_t w_bytes = sizeof(uint32_t) * (w_words + PREC_W_PAD);
     auto up = [](size_t n) { return (n + 255) / 256 * 256; };
     const size_t off_bt = 0, off_w0 = off_bt + up(bt_bytes), off_w1 = off_w0 + up(w_bytes),
-                 off_z = off_w1 + up(w_bytes), off_same = off_z + up(sizeof(float) * (size_t)p->S * Mp),
-                 off_sd = off_same + 2 * up(sizeof(int) * Mp), off_c = off_sd + up(sizeof(int) * p->S),
+                 off_z = off_w1 + up(w_bytes), off_zi = off_z + up(sizeof(float) * (size_t)p->S * Mp),
+                 off_sd = off_zi + 2 * up(sizeof(int2) * Mp), off_c = off_sd + up(sizeof(int) * p->S),
                  total = off_c + up(sizeof(float4) * 2 * p->S);
     SS_HIP(hipMalloc(&p->block, (total + (2u << 20) - 1) / (2u << 20) * (2u << 20)));
     p->Bt = (uint16_t *)(p->block + off_bt);
     p->W[0] = (uint32_t *)(p->block + off_w0);
     p->W[1] = (uint32_t *)(p->block + off_w1);
     p->Z = (float *)(p->block + off_z);
-    p->same[0] = (int *)(p->block + off_same);
-    p->same[1] = (int *)(p->block + off_same + up(sizeof(int) * Mp));
+    p->zi[0] = (int2 *)(p->block + off_zi);
+    p->zi[1] = (int2 *)(p->block + off_zi + up(sizeof(int2) * Mp));
     p->sd_nodes = (int *)(p->block + off_sd);
     p->C = (float4 *)(p->block + off_c);
     std::vector<float4> cn(2 * p->S);
@@ -611,13 +650,16 @@ int prec_setup(Slot &s, hipStream_t st) {
             int bits = sd[b];
             float id;
             memcpy(&id, &bits, sizeof(id));
-            cn[(size_t)t * p->S + b] = make_float4(0.5f * (float)nd.pot, (float)nd.bets[t], id, 0.f);
+            int rbits = pl.rrow[(size_t)(1 - t) * pl.N + sd[b]];
+            float rr;
+            memcpy(&rr, &rbits, sizeof(rr));
+            cn[(size_t)t * p->S + b] = make_float4(0.5f * (float)nd.pot, (float)nd.bets[t], id, rr);
         }
     SS_HIP(hipMemcpyAsync(p->C, cn.data(), sizeof(float4) * cn.size(), hipMemcpyHostToDevice, st));
     SS_HIP(hipMemcpyAsync(p->sd_nodes, sd.data(), sizeof(int) * p->S, hipMemcpyHostToDevice, st));
     SS_HIP(hipMemsetAsync(p->Bt, 0, bt_bytes, st));   // padding columns and k past Mp stay 0
     // opponent hand holding the same two cards as each traverser hand
-    std::vector<int> same(Mp);
+    std::vector<int2> same(Mp);
     for (int t = 0; t < 2; ++t) {
         const int o = 1 - t;
         std::vector<int> at(SS_NCOMBO, -1);
@@ -627,9 +669,11 @@ int prec_setup(Slot &s, hipStream_t st) {
         }
         for (int i = 0; i < Mp; ++i) {
             const uint32_t h = i < pl.nh[t] ? pl.hinfo[(size_t)t * Mp + i] : 0u;
-            same[i] = (h & SS_LIVE_BIT) ? at[(h >> 12) & 2047] : -1;
+            same[i] = make_int2(0, (h & SS_LIVE_BIT) ? at[(h >> 12) & 2047] : -1);
         }
-        SS_HIP(hipMemcpyAsync(p->same[t], same.data(), sizeof(int) * Mp, hipMemcpyHostToDevice, st));
+        SS_HIP(hipMemcpyAsync(p->zi[t], same.data(), sizeof(int2) * Mp, hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_prec_zinfo, dim3((Mp + SS_BLOCK - 1) / SS_BLOCK), dim3(SS_BLOCK), 0, st, s.d_desc, t,
+                           p->zi[t]);
         SS_HIP(hipStreamSynchronize(st));   // `same` is reused
         SS_HIP(hipMemsetAsync(p->W[t] + w_words, 0, sizeof(uint32_t) * PREC_W_PAD, st));
         hipLaunchKernelGGL(k_prec_w, dim3((unsigned)std::min<size_t>(4096, (w_words / 2 + SS_BLOCK - 1) / SS_BLOCK)),
@@ -645,8 +689,8 @@ int prec_setup(Slot &s, hipStream_t st) {
     d.prec_W[1] = p->W[1];
     d.prec_Bt = p->Bt;
     d.prec_Z = p->Z;
-    d.prec_same[0] = p->same[0];
-    d.prec_same[1] = p->same[1];
+    d.prec_zi[0] = p->zi[0];
+    d.prec_zi[1] = p->zi[1];
     d.prec_C = p->C;
     SS_HIP(hipMemcpyAsync(s.d_desc, &d, sizeof(d), hipMemcpyHostToDevice, st));
     SS_HIP(hipStreamSynchronize(st));
