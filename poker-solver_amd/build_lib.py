@@ -23,7 +23,7 @@ BUILD = os.path.join(PKG, "build")
 OBJ = os.path.join(BUILD, "obj")
 LIB = os.path.join(BUILD, "street_solve.so")
 
-SOURCES = ["ss_tree.cpp", "ss_runtime.hip", "ss_kernels.hip", "ss_precision.hip", "ss_capi.hip", "ss_server.hip", "ss_leaf.hip", "ss_range.hip"]
+SOURCES = ["ss_tree.cpp", "ss_runtime.hip", "ss_kernels.hip", "ss_precision.hip", "ss_capi.hip", "ss_server.hip", "ss_leaf.hip", "ss_range.hip", "ss_flop.hip"]
 
 ARCH = os.environ.get("SS_OFFLOAD_ARCH", "gfx950")
 

@@ -188,3 +188,77 @@ def terminal_values(prob, node, trav, reach, mode):
     lib().or_terminal_values(ctypes.byref(st), node, trav, ptrs, mode,
                              out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
     return out[:len(prob.hands[trav])]
+
+
+# ---- flop-through-river solve (oracle/fs_oracle.c) ---------------------------
+
+class FsOrOutput(ctypes.Structure):
+    _fields_ = [
+        ("root_strategy", ctypes.POINTER(ctypes.c_float)),
+        ("avg", ctypes.POINTER(ctypes.c_float)),
+        ("cfv", ctypes.POINTER(ctypes.c_float)),
+        ("regrets", ctypes.POINTER(ctypes.c_double)),
+        ("strategy_sum", ctypes.POINTER(ctypes.c_double)),
+    ]
+
+
+def fs_solve(td, iterations, fp64=False, want_state=False):
+    """Oracle flop solve of an FSTreeData (hands filled). Returns root_strategy
+    [na][nh root player], avg [N][6][max_h], cfv [N][max_h] (and regrets /
+    strategy_sum when want_state), as fs_solve_gpu_extract_all lays them out."""
+    import flop_solve
+    L = lib()
+    L.fs_or_solve.restype = ctypes.c_int
+    L.fs_or_solve.argtypes = [ctypes.POINTER(flop_solve.FSTreeData), ctypes.c_int, ctypes.c_int,
+                              ctypes.POINTER(FsOrOutput)]
+    N = td.num_nodes
+    mh = max(td.num_hands[0], td.num_hands[1])
+    root = td.nodes[0]
+    na = root.num_children if root.type == flop_solve.FS_NODE_DECISION else 0
+    nh_root = td.num_hands[root.player] if na else 0
+    res = {"root_strategy": np.zeros((na, nh_root), np.float32),
+           "avg": np.zeros((N, flop_solve.FS_MAX_ACTIONS, mh), np.float32),
+           "cfv": np.zeros((N, mh), np.float32)}
+    out = FsOrOutput()
+    fp = ctypes.POINTER(ctypes.c_float)
+    out.root_strategy = res["root_strategy"].ctypes.data_as(fp)
+    out.avg = res["avg"].ctypes.data_as(fp)
+    out.cfv = res["cfv"].ctypes.data_as(fp)
+    if want_state:
+        for k in ("regrets", "strategy_sum"):
+            res[k] = np.zeros((N, flop_solve.FS_MAX_ACTIONS, mh), np.float64)
+            setattr(out, k, res[k].ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    rc = L.fs_or_solve(ctypes.byref(td), int(iterations), 1 if fp64 else 0, ctypes.byref(out))
+    assert rc == 0
+    return res
+
+
+class FsCustomTree:
+    """An FSTreeData over hand-built nodes: rows of (type, player, children,
+    pot, bets, board); children are node ids, listed per node in order."""
+
+    def __init__(self, rows, hands0, hands1, weights0=None, weights1=None):
+        import flop_solve
+        n = len(rows)
+        self.nodes = (flop_solve.FSNode * n)()
+        kids = []
+        for i, (typ, player, ch, pot, bets, board) in enumerate(rows):
+            nd = self.nodes[i]
+            nd.type, nd.player, nd.pot = typ, player, pot
+            nd.num_children, nd.first_child = len(ch), len(kids)
+            nd.bets[0], nd.bets[1] = bets
+            for b, c in enumerate(board):
+                nd.board_cards[b] = c
+            nd.num_board = len(board)
+            kids.extend(ch)
+        self.children = (ctypes.c_int * max(1, len(kids)))(*kids)
+        self.td = flop_solve.FSTreeData()
+        self.td.nodes = ctypes.cast(self.nodes, ctypes.POINTER(flop_solve.FSNode))
+        self.td.children = ctypes.cast(self.children, ctypes.POINTER(ctypes.c_int))
+        self.td.num_nodes = n
+        self.td.num_children_total = len(kids)
+        for p, (hands, w) in enumerate(((hands0, weights0), (hands1, weights1))):
+            for h, (c0, c1) in enumerate(hands):
+                self.td.hands[p][h][0], self.td.hands[p][h][1] = c0, c1
+                self.td.weights[p][h] = 1.0 if w is None else w[h]
+            self.td.num_hands[p] = len(hands)

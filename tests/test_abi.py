@@ -27,7 +27,7 @@ def declared_symbols():
     for h in sorted(glob.glob(os.path.join(REPO, "include", "*.h"))):
         txt = open(h).read()
         txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-        for m in re.finditer(r"\b((?:ss|ssdbg)_[a-z0-9_]+)\s*\(", txt):
+        for m in re.finditer(r"\b((?:ss|ssdbg|fs)_[a-z0-9_]+)\s*\(", txt):
             names.add(m.group(1))
     return sorted(names)
 
