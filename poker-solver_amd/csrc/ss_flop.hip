@@ -8,22 +8,30 @@
 //     cards are blocked only at the terminals below it;
 //   - fold: starting_pot / 2 plus the loser's bets of the street, times the
 //     opponent reach of hands disjoint from the traverser's hand and the
-//     node's board (:731-779; the traverser's own board conflicts are not
+//     node's board (:737-779; the traverser's own board conflicts are not
 //     checked);
 //   - showdown: +-pot / 2 per opponent hand ranked below / above, 0 for ties,
 //     blocked hands (rank 0) and a blocked traverser (:781-833);
-//   - each traverser pass re-matches every decision node, then propagates the
-//     opponent's reach down the levels and the values up them, updating the
-//     traverser's regrets R <- (R + dv) t / (t + 1) and strategy sums
+//   - each traverser pass regret-matches every decision node, then propagates
+//     the opponent's reach down the levels and the values up them, updating
+//     the traverser's regrets R <- (R + dv) t / (t + 1) and strategy sums
 //     SS += t sigma (:835-891, the iteration loop :1150-1228).
 //
-// The GPU form: every array is [row][Mp] (Mp = the larger hand count rounded
-// to 64, one wavefront of hands). One traverser pass is a fixed sequence of
-// launches over per-level task lists (decision and chance nodes), the fold
-// list and the showdown list; the whole iteration is captured once in a
-// hipGraph and replayed, with the iteration counter on the device. Showdown
-// ranks are computed once per showdown node at setup. Terminal workgroups
-// stage the opponent's reach, cards and ranks in LDS.
+// The GPU form. Every per-hand array is [row][Mp] (Mp = the larger hand count
+// rounded to 64, one wavefront per 64 hands). One traverser pass is: the
+// root's reach, one launch per level down, the fold and showdown launches,
+// one launch per level up; the whole iteration is captured once in a hipGraph
+// and replayed, with the iteration counter on the device. Regret matching is
+// folded into the sweeps: a decision's strategy is computed where it is used
+// (the opponent's nodes on the way down, the traverser's on the way up, where
+// its regrets are still those of the pass's start), so no strategy array is
+// stored. The terminals are (hand x opponent hand) sums in the reference's
+// order: at setup every distinct showdown board gets a table of
+// 1 + sign(rank difference) codes per (traverser hand, opponent hand), one
+// byte each, 0 / 1 card-disjointness codes serve the folds, and a terminal
+// workgroup stages its table in LDS once for many nodes, so a hand pair costs
+// a byte-to-float conversion and two fused multiply-adds whose zero terms
+// leave the sum unchanged: the results are the reference loop's bit for bit.
 #include "ss_internal.h"
 #include "ss_eval.h"
 #include "../../include/flop_solve.h"
@@ -171,72 +179,123 @@ template <typename T> T *copy_out(const std::vector<T> &v) {
 struct FsDev {
     int N, Mp, nh[2];
     int starting_pot;
+    int no16;                  // opponent hands in groups of 16: ceil(max hands / 16)
     const int4 *node;          // [N] x = type | player << 4 (player + 1), y = nchild, z = first_child, w = pot
     const int2 *bets;          // [N]
     const int *children;
     const int *abase;          // [N] first action row of a decision node, -1 otherwise
-    const uint32_t *board;     // [N] 5 cards, 6 bits each, and the count << 30
+    const uint32_t *board;     // [N] 5 card slots of 6 bits each, 63 in the slots past the node's board
     const uint32_t *hands;     // [2][Mp] c0 | c1 << 8 (0xFFFF past the hand count)
     const float *weights;      // [2][Mp]
-    const int *sd_row;         // [N] showdown rank row, -1 otherwise
-    uint32_t *rank;            // [num showdowns][2][Mp], written once at setup
-    float *R, *sigma, *SS;     // [A][Mp]
+    float *R, *SS;             // [A][Mp]
     float *reach, *cfv;        // [N][Mp]
+    float *root_sigma;         // [FS_MAX_ACTIONS][Mp] the root's strategy as last computed
     int *iter;                 // the iteration being run
 };
 
 __device__ __forceinline__ int fs_type(int4 n) { return n.x & 15; }
 __device__ __forceinline__ int fs_player(int4 n) { return (n.x >> 4) - 1; }
 __device__ __forceinline__ bool fs_on_board(uint32_t bw, int c) {
-    const int nb = (int)(bw >> 30);
-    for (int b = 0; b < nb; ++b)
-        if ((int)((bw >> (6 * b)) & 63) == c) return true;
-    return false;
+    bool on = false;
+    for (int b = 0; b < 5; ++b) on |= (int)((bw >> (6 * b)) & 63) == c;   // an empty slot (63) matches no card
+    return on;
+}
+__device__ __forceinline__ bool fs_disjoint(uint32_t a, uint32_t b) {
+    const uint32_t a0 = a & 255, a1 = (a >> 8) & 255, b0 = b & 255, b1 = (b >> 8) & 255;
+    return a0 != b0 && a0 != b1 && a1 != b0 && a1 != b1;
 }
 
-// ranks of both players' hands at every showdown node (0: blocked by the board)
-__global__ void __launch_bounds__(SS_BLOCK) k_fs_rank(FsDev d, const int *sd_nodes, int nsd) {
+// Regret matching of one hand at one node (fs_batch_regret_match,
+// flop_solve.cu:657-690): positive parts times the reciprocal of their sum,
+// uniform when none is positive.
+__device__ __forceinline__ void fs_match(const float *R, int Mp, int na, float *sg) {
+    float r[FS_MAX_ACTIONS];
+    float sum = 0.f;
+#pragma unroll
+    for (int a = 0; a < FS_MAX_ACTIONS; ++a)
+        if (a < na) {
+            r[a] = R[(size_t)a * Mp];
+            sum += r[a] > 0.f ? r[a] : 0.f;
+        }
+    if (sum > 0.f) {
+        const float inv = 1.0f / sum;
+#pragma unroll
+        for (int a = 0; a < FS_MAX_ACTIONS; ++a)
+            if (a < na) sg[a] = (r[a] > 0.f ? r[a] : 0.f) * inv;
+    } else {
+        const float u = 1.0f / (float)na;
+#pragma unroll
+        for (int a = 0; a < FS_MAX_ACTIONS; ++a)
+            if (a < na) sg[a] = u;
+    }
+}
+
+// ranks of both players' hands on every distinct showdown board (0: blocked by it)
+__global__ void __launch_bounds__(SS_BLOCK) k_fs_rank(FsDev d, const uint32_t *boards, int nboards,
+                                                      uint32_t *rank) {
     __shared__ uint8_t tab[8192];
     fill_straight_table(tab, threadIdx.x, blockDim.x);
     __syncthreads();
-    const size_t total = (size_t)nsd * 2 * d.Mp;
-    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
-        const int h = (int)(e % d.Mp), p = (int)((e / d.Mp) % 2), s = (int)(e / d.Mp / 2);
+    const int total = nboards * 2 * d.Mp;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < total; e += gridDim.x * blockDim.x) {
+        const int h = e % d.Mp, p = (e / d.Mp) % 2, b = e / d.Mp / 2;
         uint32_t r = 0;
         if (h < d.nh[p]) {
-            const uint32_t bw = d.board[sd_nodes[s]], hc = d.hands[p * d.Mp + h];
+            const uint32_t bw = boards[b], hc = d.hands[p * d.Mp + h];
             const int c0 = hc & 255, c1 = (hc >> 8) & 255;
             if (!fs_on_board(bw, c0) && !fs_on_board(bw, c1)) {
                 int c[7];
-                for (int b = 0; b < 5; ++b) c[b] = (int)((bw >> (6 * b)) & 63);
+                for (int k = 0; k < 5; ++k) c[k] = (int)((bw >> (6 * k)) & 63);
                 c[5] = c0;
                 c[6] = c1;
                 r = eval7_lut(c, tab);
             }
         }
-        d.rank[e] = r;
+        rank[e] = r;
     }
 }
 
-// regret matching at every decision node, for its actor's hands
-__global__ void __launch_bounds__(SS_BLOCK) k_fs_match(FsDev d, const int *dec, int ndec) {
-    const size_t total = (size_t)ndec * d.Mp;
-    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
-        const int h = (int)(e % d.Mp), n = dec[e / d.Mp];
-        const int4 nd = d.node[n];
-        if (h >= d.nh[fs_player(nd)]) continue;
-        const int na = nd.y;
-        const float *R = d.R + (size_t)d.abase[n] * d.Mp + h;
-        float *sg = d.sigma + (size_t)d.abase[n] * d.Mp + h;
-        float sum = 0.f;
-        for (int a = 0; a < na; ++a) sum += fmaxf(R[(size_t)a * d.Mp], 0.f);
-        if (sum > 0.f) {
-            const float inv = 1.0f / sum;
-            for (int a = 0; a < na; ++a) sg[(size_t)a * d.Mp] = fmaxf(R[(size_t)a * d.Mp], 0.f) * inv;
-        } else {
-            const float u = 1.0f / na;
-            for (int a = 0; a < na; ++a) sg[(size_t)a * d.Mp] = u;
+// Showdown codes of every board and traverser: byte k of word (c, h) holds
+// 1 + sign(rank_h - rank_o) for opponent hand o = 16 c + k, 1 (no term) for a
+// tie, a blocked hand, shared cards or a hand past the counts.
+__global__ void __launch_bounds__(SS_BLOCK) k_fs_wmat(FsDev d, const uint32_t *rank, int nboards, uint4 *W) {
+    const int per = d.no16 * d.Mp, total = nboards * 2 * per;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < total; e += gridDim.x * blockDim.x) {
+        const int h = e % d.Mp, c = (e / d.Mp) % d.no16, t = (e / per) % 2, b = e / per / 2, opp = 1 - t;
+        uint32_t w[4] = {0x01010101u, 0x01010101u, 0x01010101u, 0x01010101u};
+        if (h < d.nh[t]) {
+            const uint32_t rt = rank[((size_t)b * 2 + t) * d.Mp + h], hc = d.hands[t * d.Mp + h];
+            for (int k = 0; k < 16; ++k) {
+                const int o = 16 * c + k;
+                if (o >= d.nh[opp] || !rt) continue;
+                const uint32_t ro = rank[((size_t)b * 2 + opp) * d.Mp + o];
+                if (!ro || !fs_disjoint(hc, d.hands[opp * d.Mp + o])) continue;
+                const uint32_t code = 1u + (rt > ro) - (rt < ro);
+                w[k >> 2] = (w[k >> 2] & ~(255u << (8 * (k & 3)))) | code << (8 * (k & 3));
+            }
         }
+        W[e] = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+}
+
+// Fold codes of every distinct fold board and traverser: byte k of word (c, h)
+// is 1 when opponent hand o = 16 c + k shares no card with hand h and none
+// with the board (the traverser's own board cards are not checked), else 0.
+__global__ void __launch_bounds__(SS_BLOCK) k_fs_fmat(FsDev d, const uint32_t *boards, int nboards, uint4 *F) {
+    const int per = d.no16 * d.Mp, total = nboards * 2 * per;
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < total; e += gridDim.x * blockDim.x) {
+        const int h = e % d.Mp, c = (e / d.Mp) % d.no16, t = (e / per) % 2, b = e / per / 2, opp = 1 - t;
+        uint32_t w[4] = {0, 0, 0, 0};
+        if (h < d.nh[t])
+            for (int k = 0; k < 16; ++k) {
+                const int o = 16 * c + k;
+                if (o >= d.nh[opp]) continue;
+                const uint32_t oc = d.hands[opp * d.Mp + o];
+                if (fs_disjoint(d.hands[t * d.Mp + h], oc) && !fs_on_board(boards[b], oc & 255) &&
+                    !fs_on_board(boards[b], (oc >> 8) & 255))
+                    w[k >> 2] |= 1u << (8 * (k & 3));
+            }
+        F[e] = make_uint4(w[0], w[1], w[2], w[3]);
     }
 }
 
@@ -245,107 +304,251 @@ __global__ void __launch_bounds__(SS_BLOCK) k_fs_root(FsDev d, int opp) {
     for (int h = threadIdx.x; h < d.nh[opp]; h += blockDim.x) d.reach[h] = d.weights[opp * d.Mp + h];
 }
 
-// the opponent's reach down one level (decision and chance nodes)
-__global__ void __launch_bounds__(SS_BLOCK) k_fs_down(FsDev d, const int *lvl, int nlvl, int opp) {
-    const size_t total = (size_t)nlvl * d.Mp;
-    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
-        const int h = (int)(e % d.Mp), n = lvl[e / d.Mp];
-        if (h >= d.nh[opp]) continue;
-        const int4 nd = d.node[n];
-        const float pr = d.reach[(size_t)n * d.Mp + h];
-        const int *kid = d.children + nd.z;
-        if (fs_type(nd) == FS_NODE_DECISION && fs_player(nd) == opp) {
-            const float *sg = d.sigma + (size_t)d.abase[n] * d.Mp + h;
-            for (int a = 0; a < nd.y; ++a) d.reach[(size_t)kid[a] * d.Mp + h] = pr * sg[(size_t)a * d.Mp];
+// A sweep record: everything one (node, hand segment) item of a level sweep
+// needs, in one 64-byte scalar load: node, type | (player + 1) << 4 |
+// children << 8, the first action row (decision) or the first child slot
+// (chance), and up to six children inline (decision).
+constexpr int FS_REC = 16;
+
+// Regret matching from regrets already in registers (the arithmetic of
+// fs_match).
+__device__ __forceinline__ void fs_match_regs(const float *r, int na, float *sg) {
+    float sum = 0.f;
+#pragma unroll
+    for (int a = 0; a < FS_MAX_ACTIONS; ++a)
+        if (a < na) sum += r[a] > 0.f ? r[a] : 0.f;
+    if (sum > 0.f) {
+        const float inv = 1.0f / sum;
+#pragma unroll
+        for (int a = 0; a < FS_MAX_ACTIONS; ++a)
+            if (a < na) sg[a] = (r[a] > 0.f ? r[a] : 0.f) * inv;
+    } else {
+        const float u = 1.0f / (float)na;
+#pragma unroll
+        for (int a = 0; a < FS_MAX_ACTIONS; ++a)
+            if (a < na) sg[a] = u;
+    }
+}
+
+// The opponent's reach down one level (fs_batch_propagate_reach,
+// flop_solve.cu:693-735): one wavefront per node, S segments of 64 hands per
+// lane (their loads in flight together); the opponent's own decisions scale
+// by its current strategy, regret-matched here.
+template <int S>
+__global__ void __launch_bounds__(SS_BLOCK) k_fs_down(FsDev d, const int *__restrict__ recs, int nrec, int opp,
+                                                      const float *__restrict__ R, float *__restrict__ reach,
+                                                      float *__restrict__ root_sigma) {
+    const int lane = threadIdx.x & 63, nh = d.nh[opp], Mp = d.Mp;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (int i = blockIdx.x * 4 + w; i < nrec; i += gridDim.x * 4) {
+        const int *rc = recs + (size_t)i * FS_REC;
+        const int n = rc[0], meta = rc[1], na = meta >> 8;
+        float pr[S];
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+            if (s * 64 + lane < nh) pr[s] = reach[(size_t)n * Mp + s * 64 + lane];
+        if ((meta & 15) == FS_NODE_DECISION) {
+            if (((meta >> 4) & 15) - 1 == opp) {
+                float r[S][FS_MAX_ACTIONS];
+#pragma unroll
+                for (int s = 0; s < S; ++s)
+#pragma unroll
+                    for (int a = 0; a < FS_MAX_ACTIONS; ++a)
+                        if (a < na && s * 64 + lane < nh) r[s][a] = R[(size_t)(rc[2] + a) * Mp + s * 64 + lane];
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    const int h = s * 64 + lane;
+                    if (h >= nh) continue;
+                    float sg[FS_MAX_ACTIONS];
+                    fs_match_regs(r[s], na, sg);
+#pragma unroll
+                    for (int a = 0; a < FS_MAX_ACTIONS; ++a)
+                        if (a < na) {
+                            reach[(size_t)rc[3 + a] * Mp + h] = pr[s] * sg[a];
+                            if (n == 0) root_sigma[a * Mp + h] = sg[a];
+                        }
+                }
+            } else {
+#pragma unroll
+                for (int s = 0; s < S; ++s)
+#pragma unroll
+                    for (int a = 0; a < FS_MAX_ACTIONS; ++a)
+                        if (a < na && s * 64 + lane < nh) reach[(size_t)rc[3 + a] * Mp + s * 64 + lane] = pr[s];
+            }
         } else {
-            for (int a = 0; a < nd.y; ++a) d.reach[(size_t)kid[a] * d.Mp + h] = pr;
+            const int *kid = d.children + rc[2];
+            for (int a = 0; a < na; ++a) {
+                const size_t row = (size_t)kid[a] * Mp;
+#pragma unroll
+                for (int s = 0; s < S; ++s)
+                    if (s * 64 + lane < nh) reach[row + s * 64 + lane] = pr[s];
+            }
         }
     }
 }
 
-// fold and showdown values for the traverser's hands; one workgroup per node,
-// the opponent's reach, cards and ranks staged in LDS
-__global__ void __launch_bounds__(SS_BLOCK) k_fs_terminal(FsDev d, const int *term, int nterm, int trav) {
-    __shared__ float s_reach[256];
-    __shared__ uint32_t s_cards[256], s_rank[256];
-    const int opp = 1 - trav, n_opp = d.nh[opp], n_trav = d.nh[trav];
-    for (int j = blockIdx.x; j < nterm; j += gridDim.x) {
-        const int n = term[j];
-        const int4 nd = d.node[n];
-        const bool fold = fs_type(nd) == FS_NODE_FOLD;
-        const int sr = d.sd_row[n];
-        const uint32_t bw = d.board[n];
-        __syncthreads();   // the previous node's staging is read
-        for (int o = threadIdx.x; o < n_opp; o += blockDim.x) {
-            const uint32_t oc = d.hands[opp * d.Mp + o];
-            s_reach[o] = d.reach[(size_t)n * d.Mp + o];
-            s_cards[o] = oc;
-            // fold: an opponent hand on the node's board is left out (rank 0 marks it)
-            s_rank[o] = fold ? ((fs_on_board(bw, oc & 255) || fs_on_board(bw, (oc >> 8) & 255)) ? 0u : 1u)
-                             : d.rank[((size_t)sr * 2 + opp) * d.Mp + o];
-        }
-        __syncthreads();
-        for (int h = threadIdx.x; h < n_trav; h += blockDim.x) {
-            const uint32_t hc = d.hands[trav * d.Mp + h];
-            const uint32_t a0 = hc & 255, a1 = (hc >> 8) & 255;
-            float val = 0.f;
-            if (fold) {
-                const int winner = fs_player(nd), loser = 1 - winner;
-                const float half_start = d.starting_pot * 0.5f;
-                const float payoff = trav == winner ? half_start + (float)(loser == 0 ? d.bets[n].x : d.bets[n].y)
-                                                    : -(half_start + (float)(trav == 0 ? d.bets[n].x : d.bets[n].y));
-                float opp_sum = 0.f;
-                for (int o = 0; o < n_opp; ++o) {
-                    const uint32_t oc = s_cards[o], b0 = oc & 255, b1 = (oc >> 8) & 255;
-                    if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1 || !s_rank[o]) continue;
-                    opp_sum += s_reach[o];
+// Values up one level (fs_batch_propagate_cfv, flop_solve.cu:835-891): at
+// the traverser's decisions the strategy of this pass (its regrets are still
+// those of the pass's start), the value, the regret and strategy-sum updates;
+// chance nodes average their children (their loads eight children at a time,
+// the sum in child order).
+template <int S>
+__global__ void __launch_bounds__(SS_BLOCK) k_fs_up(FsDev d, const int *__restrict__ recs, int nrec, int trav,
+                                                    float *__restrict__ R, float *__restrict__ SS,
+                                                    float *__restrict__ cfv, float *__restrict__ root_sigma) {
+    const int lane = threadIdx.x & 63, nh = d.nh[trav], Mp = d.Mp;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int t = *d.iter;
+    const float disc = (float)t / ((float)t + 1.0f), tw = (float)t;
+    for (int i = blockIdx.x * 4 + w; i < nrec; i += gridDim.x * 4) {
+        const int *rc = recs + (size_t)i * FS_REC;
+        const int n = rc[0], meta = rc[1], na = meta >> 8;
+        if ((meta & 15) == FS_NODE_DECISION) {
+            const bool mine = ((meta >> 4) & 15) - 1 == trav;
+            float cv[S][FS_MAX_ACTIONS], r[S][FS_MAX_ACTIONS], ss[S][FS_MAX_ACTIONS];
+#pragma unroll
+            for (int s = 0; s < S; ++s)
+#pragma unroll
+                for (int a = 0; a < FS_MAX_ACTIONS; ++a)
+                    if (a < na && s * 64 + lane < nh) {
+                        cv[s][a] = cfv[(size_t)rc[3 + a] * Mp + s * 64 + lane];
+                        if (mine) {
+                            r[s][a] = R[(size_t)(rc[2] + a) * Mp + s * 64 + lane];
+                            ss[s][a] = SS[(size_t)(rc[2] + a) * Mp + s * 64 + lane];
+                        }
+                    }
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                const int h = s * 64 + lane;
+                if (h >= nh) continue;
+                float val = 0.f;
+                if (mine) {
+                    float sg[FS_MAX_ACTIONS];
+                    fs_match_regs(r[s], na, sg);
+#pragma unroll
+                    for (int a = 0; a < FS_MAX_ACTIONS; ++a)
+                        if (a < na) val = fmaf(sg[a], cv[s][a], val);
+#pragma unroll
+                    for (int a = 0; a < FS_MAX_ACTIONS; ++a)
+                        if (a < na) {
+                            const size_t row = (size_t)(rc[2] + a) * Mp + h;
+                            R[row] = (r[s][a] + (cv[s][a] - val)) * disc;
+                            SS[row] = fmaf(tw, sg[a], ss[s][a]);
+                            if (n == 0) root_sigma[a * Mp + h] = sg[a];
+                        }
+                } else {
+#pragma unroll
+                    for (int a = 0; a < FS_MAX_ACTIONS; ++a)
+                        if (a < na) val += cv[s][a];
                 }
-                val = opp_sum * payoff;
-            } else {
-                const uint32_t hs = d.rank[((size_t)sr * 2 + trav) * d.Mp + h];
-                if (hs) {
-                    const float half_pot = nd.w * 0.5f;
-                    for (int o = 0; o < n_opp; ++o) {
-                        const uint32_t oc = s_cards[o], b0 = oc & 255, b1 = (oc >> 8) & 255;
-                        if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) continue;
-                        const uint32_t os = s_rank[o];
-                        if (!os) continue;
-                        const float w = s_reach[o];
-                        if (hs > os) val += w * half_pot;
-                        else if (hs < os) val -= w * half_pot;
+                cfv[(size_t)n * Mp + h] = val;
+            }
+        } else {
+            const int *kid = d.children + rc[2];
+            float acc[S];
+#pragma unroll
+            for (int s = 0; s < S; ++s) acc[s] = 0.f;
+            for (int a0 = 0; a0 < na; a0 += 8) {
+                float v[8][S];
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                    if (a0 + k < na) {
+                        const size_t row = (size_t)kid[a0 + k] * Mp;
+#pragma unroll
+                        for (int s = 0; s < S; ++s)
+                            if (s * 64 + lane < nh) v[k][s] = cfv[row + s * 64 + lane];
+                    }
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                    if (a0 + k < na)
+#pragma unroll
+                        for (int s = 0; s < S; ++s) acc[s] += v[k][s];
+            }
+#pragma unroll
+            for (int s = 0; s < S; ++s)
+                if (s * 64 + lane < nh) cfv[(size_t)n * Mp + s * 64 + lane] = acc[s] / (float)na;
+        }
+    }
+}
+
+// Terminal values, fold (FOLD) or showdown, for one traverser: a task is the
+// set of terminal nodes on one board (tasks[j] = table, first, count into
+// `nodes`); its workgroup stages the board's codes for this traverser
+// ([no16][Mp] uint4, 16 opponent hands per lane word) in LDS once, then the
+// nodes' opponent reach rows 16 at a time, and computes four nodes at a time,
+// each lane one traverser hand (the reach values are LDS broadcasts). Per opponent
+// hand o, in the reference's order:
+//   fold      acc = fma(code, r_o, acc)                  (code 1 / 0:
+//             flop_solve.cu:737-779 adds r_o or skips it)
+//   showdown  acc = fma(fma(code, r_o, -r_o), pot/2, acc) (code 2 / 0 / 1:
+//             +-r_o pot/2 or nothing, flop_solve.cu:781-833)
+// A zero term leaves acc unchanged, so the sums are the reference's bit for
+// bit; the fold value is acc times the payoff.
+constexpr int FS_TASK_NODES = 16;
+constexpr int FS_ROW16 = 16 * ((FS_MAX_HANDS + 15) / 16);   // staged reach row: opponent hands padded to 16
+
+template <bool FOLD>
+__global__ void __launch_bounds__(SS_BLOCK) k_fs_term(FsDev d, const int4 *__restrict__ tasks, int ntasks,
+                                                      const int *__restrict__ nodes, const uint4 *__restrict__ tables,
+                                                      const float *__restrict__ reach, float *__restrict__ cfv,
+                                                      int trav) {
+    extern __shared__ uint4 s_tab[];
+    __shared__ float s_rch[FS_TASK_NODES][FS_ROW16];
+    const int n_trav = d.nh[trav], Mp = d.Mp, no16 = d.no16, rw = 16 * no16;
+    const int h = threadIdx.x;
+    const float half_start = d.starting_pot * 0.5f;
+    for (int j = blockIdx.x; j < ntasks; j += gridDim.x) {
+        const int4 tk = tasks[j];
+        __syncthreads();   // the previous task's readers are done
+        const uint4 *src = tables + ((size_t)tk.x * 2 + trav) * no16 * Mp;
+        for (int i = threadIdx.x; i < no16 * Mp; i += blockDim.x) s_tab[i] = src[i];
+        for (int f = 0; f < tk.z; f += FS_TASK_NODES) {   // the board's nodes, 16 rows staged at a time
+          const int cnt = min(FS_TASK_NODES, tk.z - f), first = tk.y + f;
+          if (f) __syncthreads();   // the previous rows' readers are done
+          for (int i = threadIdx.x; i < cnt * rw; i += blockDim.x) {
+              const int m = i / rw, o = i - m * rw;   // o < rw <= Mp: inside the row
+              s_rch[m][o] = reach[(size_t)nodes[first + m] * Mp + o];
+          }
+          __syncthreads();
+          if (h >= n_trav) continue;
+          for (int g = 0; g < cnt; g += 4) {
+            int nn[4], mm[4];
+            float scale[4];   // showdown: pot / 2; fold: the payoff
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                mm[m] = min(g + m, cnt - 1);   // a short group repeats its last node
+                nn[m] = nodes[first + mm[m]];
+                const int4 nd = d.node[nn[m]];
+                if (FOLD) {
+                    const int winner = (nd.x >> 4) - 1, loser = 1 - winner;
+                    const int2 bt = d.bets[nn[m]];
+                    scale[m] = trav == winner ? half_start + (float)(loser == 0 ? bt.x : bt.y)
+                                              : -(half_start + (float)(trav == 0 ? bt.x : bt.y));
+                } else {
+                    scale[m] = nd.w * 0.5f;
+                }
+            }
+            float acc[4] = {0.f, 0.f, 0.f, 0.f};
+            for (int c = 0; c < no16; ++c) {
+                const uint4 q = s_tab[c * Mp + h];
+                const uint32_t qw[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+                for (int k = 0; k < 16; ++k) {
+                    const float code = (float)((qw[k >> 2] >> (8 * (k & 3))) & 255u);
+#pragma unroll
+                    for (int m = 0; m < 4; ++m) {
+                        const float r = s_rch[mm[m]][16 * c + k];
+                        if (FOLD)
+                            acc[m] = fmaf(code, r, acc[m]);
+                        else
+                            acc[m] = fmaf(fmaf(code, r, -r), scale[m], acc[m]);
                     }
                 }
             }
-            d.cfv[(size_t)n * d.Mp + h] = val;
-        }
-    }
-}
-
-// values up one level; the traverser's regrets and strategy sums at its nodes
-__global__ void __launch_bounds__(SS_BLOCK) k_fs_up(FsDev d, const int *lvl, int nlvl, int trav) {
-    const int t = *d.iter;
-    const float disc = (float)t / ((float)t + 1.0f), tw = (float)t;
-    const size_t total = (size_t)nlvl * d.Mp;
-    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
-        const int h = (int)(e % d.Mp), n = lvl[e / d.Mp];
-        if (h >= d.nh[trav]) continue;
-        const int4 nd = d.node[n];
-        const int *kid = d.children + nd.z;
-        const int na = nd.y;
-        float val = 0.f;
-        if (fs_type(nd) == FS_NODE_DECISION && fs_player(nd) == trav) {
-            const size_t row = (size_t)d.abase[n] * d.Mp + h;
-            for (int a = 0; a < na; ++a) val += d.sigma[row + (size_t)a * d.Mp] * d.cfv[(size_t)kid[a] * d.Mp + h];
-            d.cfv[(size_t)n * d.Mp + h] = val;
-            for (int a = 0; a < na; ++a) {
-                float *r = d.R + row + (size_t)a * d.Mp;
-                *r = (*r + (d.cfv[(size_t)kid[a] * d.Mp + h] - val)) * disc;
-                d.SS[row + (size_t)a * d.Mp] += tw * d.sigma[row + (size_t)a * d.Mp];
-            }
-        } else {
-            for (int a = 0; a < na; ++a) val += d.cfv[(size_t)kid[a] * d.Mp + h];
-            if (fs_type(nd) == FS_NODE_CHANCE) val = val / (float)na;
-            d.cfv[(size_t)n * d.Mp + h] = val;
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+                if (g + m < cnt) cfv[(size_t)nn[m] * Mp + h] = FOLD ? acc[m] * scale[m] : acc[m];
+          }
         }
     }
 }
@@ -361,10 +564,36 @@ struct FsRun {
     std::vector<void *> bufs;
     FsDev d{};
     int A = 0;
-    std::vector<int> dec, fold_sd, sd;
+    std::vector<int> dec;
+    // Fold or showdown nodes grouped by board: distinct boards, the nodes
+    // sorted by board, one task per board (board, first, count), and on the
+    // device the same plus the per-board code tables.
+    struct TermSet {
+        std::vector<uint32_t> boards;
+        std::vector<int> sorted;
+        std::vector<int4> tasks;
+        uint32_t *d_boards = nullptr;
+        int *d_nodes = nullptr;
+        int4 *d_tasks = nullptr;
+        uint4 *d_tab = nullptr;
+        void group(const std::vector<int> &nodes, const std::vector<uint32_t> &board) {
+            std::vector<std::pair<uint32_t, int>> keyed;
+            for (int n : nodes) keyed.push_back({board[n], n});
+            std::sort(keyed.begin(), keyed.end());
+            for (size_t i = 0; i < keyed.size();) {
+                size_t e = i;
+                while (e < keyed.size() && keyed[e].first == keyed[i].first) ++e;
+                tasks.push_back(make_int4((int)boards.size(), (int)i, (int)(e - i), 0));
+                boards.push_back(keyed[i].first);
+                i = e;
+            }
+            for (auto &k : keyed) sorted.push_back(k.second);
+        }
+    } sdset, foldset;
     std::vector<std::vector<int>> lvl;   // decision and chance nodes per depth
     std::vector<int *> d_lvl;
-    int *d_dec = nullptr, *d_term = nullptr, *d_sd = nullptr;
+    int cus = 256;
+    size_t tab_bytes = 0;                // LDS of one staged table
     std::vector<int> abase;
     hipStream_t st = nullptr;
     hipGraph_t graph = nullptr;
@@ -394,13 +623,16 @@ struct FsRun {
         const int maxh = std::max(td->num_hands[0], td->num_hands[1]);
         const int Mp = (maxh + 63) / 64 * 64;
         SS_HIP(hipStreamCreate(&st));
+        int dev = 0;
+        SS_HIP(hipGetDevice(&dev));
+        SS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
         // host tables
         std::vector<int4> node(N);
         std::vector<int2> bets(N);
         std::vector<uint32_t> board(N);
-        std::vector<int> sd_row(N, -1), depth(N, -1);
+        std::vector<int> depth(N, -1), sd, fold;
         abase.assign(N, -1);
-        int total_kids = td->num_children_total;
+        const int total_kids = td->num_children_total;
         for (int n = 0; n < N; ++n) {
             const FSNode &f = td->nodes[n];
             if (f.type < 0 || f.type > 3 || f.num_children < 0 || (f.num_children > 0 && (f.first_child < 0 ||
@@ -412,17 +644,19 @@ struct FsRun {
                                  f.num_children, f.player);
             if (f.type == FS_NODE_CHANCE && f.num_children < 1)
                 return set_error(SS_ERR_ARG, "flop solve: chance node %d has no children", n);
+            if ((f.type == FS_NODE_FOLD || f.type == FS_NODE_SHOWDOWN) && f.num_children != 0)
+                return set_error(SS_ERR_ARG, "flop solve: terminal node %d has children", n);
             if (f.type == FS_NODE_FOLD && (f.player < 0 || f.player > 1))
                 return set_error(SS_ERR_ARG, "flop solve: fold node %d has winner %d", n, f.player);
             if (f.type == FS_NODE_SHOWDOWN && f.num_board != 5)
                 return set_error(SS_ERR_ARG, "flop solve: showdown node %d has %d board cards", n, f.num_board);
             node[n] = make_int4(f.type | (f.player + 1) << 4, f.num_children, f.first_child, f.pot);
             bets[n] = make_int2(f.bets[0], f.bets[1]);
-            uint32_t bw = (uint32_t)f.num_board << 30;
-            for (int b = 0; b < f.num_board; ++b) {
-                if (f.board_cards[b] < 0 || f.board_cards[b] > 51)
+            uint32_t bw = 0;
+            for (int b = 0; b < 5; ++b) {
+                if (b < f.num_board && (f.board_cards[b] < 0 || f.board_cards[b] > 51))
                     return set_error(SS_ERR_ARG, "flop solve: node %d board card %d", n, f.board_cards[b]);
-                bw |= (uint32_t)f.board_cards[b] << (6 * b);
+                bw |= (uint32_t)(b < f.num_board ? f.board_cards[b] : 63) << (6 * b);
             }
             board[n] = bw;
             if (f.type == FS_NODE_DECISION) {
@@ -430,11 +664,8 @@ struct FsRun {
                 A += f.num_children;
                 dec.push_back(n);
             }
-            if (f.type == FS_NODE_SHOWDOWN) {
-                sd_row[n] = (int)sd.size();
-                sd.push_back(n);
-            }
-            if (f.type == FS_NODE_FOLD || f.type == FS_NODE_SHOWDOWN) fold_sd.push_back(n);
+            if (f.type == FS_NODE_SHOWDOWN) sd.push_back(n);
+            if (f.type == FS_NODE_FOLD) fold.push_back(n);
         }
         // depths from the root (breadth first); every node must hang off it
         std::vector<int> queue{0};
@@ -450,13 +681,18 @@ struct FsRun {
                 queue.push_back(k);
             }
         }
-        if ((int)queue.size() != N) return set_error(SS_ERR_ARG, "flop solve: %d nodes off the root's tree", N - (int)queue.size());
+        if ((int)queue.size() != N)
+            return set_error(SS_ERR_ARG, "flop solve: %d nodes off the root's tree", N - (int)queue.size());
         int L = 0;
         for (int n = 0; n < N; ++n) L = std::max(L, depth[n] + 1);
         lvl.assign(L, {});
         for (int n : queue)
             if (td->nodes[n].type == FS_NODE_DECISION || td->nodes[n].type == FS_NODE_CHANCE)
                 lvl[depth[n]].push_back(n);
+        // terminals grouped by board (the nodes of one river betting tree
+        // share it)
+        sdset.group(sd, board);
+        foldset.group(fold, board);
         std::vector<uint32_t> hands(2 * Mp, 0xFFFFu);
         std::vector<float> w(2 * Mp, 0.f);
         for (int p = 0; p < 2; ++p)
@@ -472,22 +708,30 @@ struct FsRun {
         d.Mp = Mp;
         d.nh[0] = td->num_hands[0];
         d.nh[1] = td->num_hands[1];
+        d.no16 = (maxh + 15) / 16;
         d.starting_pot = td->nodes[0].pot;
+        tab_bytes = sizeof(uint4) * (size_t)d.no16 * Mp;
         int4 *p_node;
         int2 *p_bets;
-        int *p_kids, *p_abase, *p_sdrow, *p_iter;
+        int *p_kids, *p_abase, *p_iter;
         uint32_t *p_board, *p_hands, *p_rank;
         float *p_w;
+        const size_t tab = 2 * (size_t)d.no16 * Mp;   // uint4 per board: both traversers
         int rc;
         if ((rc = alloc(&p_node, N, node.data())) || (rc = alloc(&p_bets, N, bets.data())) ||
             (rc = alloc(&p_kids, (size_t)total_kids, td->children)) || (rc = alloc(&p_abase, N, abase.data())) ||
             (rc = alloc(&p_board, N, board.data())) || (rc = alloc(&p_hands, hands.size(), hands.data())) ||
-            (rc = alloc(&p_w, w.size(), w.data())) || (rc = alloc(&p_sdrow, N, sd_row.data())) ||
-            (rc = alloc(&p_rank, sd.size() * 2 * Mp)) || (rc = alloc(&d.R, (size_t)A * Mp)) ||
-            (rc = alloc(&d.sigma, (size_t)A * Mp)) || (rc = alloc(&d.SS, (size_t)A * Mp)) ||
+            (rc = alloc(&p_w, w.size(), w.data())) || (rc = alloc(&p_rank, sdset.boards.size() * 2 * Mp)) ||
+            (rc = alloc(&d.R, (size_t)A * Mp)) || (rc = alloc(&d.SS, (size_t)A * Mp)) ||
             (rc = alloc(&d.reach, (size_t)N * Mp)) || (rc = alloc(&d.cfv, (size_t)N * Mp)) ||
-            (rc = alloc(&p_iter, 1)))
+            (rc = alloc(&d.root_sigma, (size_t)FS_MAX_ACTIONS * Mp)) || (rc = alloc(&p_iter, 1)))
             return rc;
+        for (TermSet *ts : {&sdset, &foldset})
+            if ((rc = alloc(&ts->d_boards, ts->boards.size(), ts->boards.data())) ||
+                (rc = alloc(&ts->d_nodes, ts->sorted.size(), ts->sorted.data())) ||
+                (rc = alloc(&ts->d_tasks, ts->tasks.size(), ts->tasks.data())) ||
+                (rc = alloc(&ts->d_tab, ts->boards.size() * tab)))
+                return rc;
         d.node = p_node;
         d.bets = p_bets;
         d.children = p_kids;
@@ -495,18 +739,36 @@ struct FsRun {
         d.board = p_board;
         d.hands = p_hands;
         d.weights = p_w;
-        d.sd_row = p_sdrow;
-        d.rank = p_rank;
         d.iter = p_iter;
-        if ((rc = alloc(&d_dec, dec.size(), dec.data())) || (rc = alloc(&d_term, fold_sd.size(), fold_sd.data())) ||
-            (rc = alloc(&d_sd, sd.size(), sd.data())))
-            return rc;
+        // sweep records per level
         d_lvl.assign(L, nullptr);
-        for (int l = 0; l < L; ++l)
-            if (!lvl[l].empty() && (rc = alloc(&d_lvl[l], lvl[l].size(), lvl[l].data()))) return rc;
-        if (!sd.empty())
-            hipLaunchKernelGGL(k_fs_rank, dim3(grid_for((size_t)sd.size() * 2 * Mp)), dim3(SS_BLOCK), 0, st, d, d_sd,
-                               (int)sd.size());
+        for (int l = 0; l < L; ++l) {
+            if (lvl[l].empty()) continue;
+            std::vector<int> rec(lvl[l].size() * FS_REC, 0);
+            for (size_t i = 0; i < lvl[l].size(); ++i) {
+                const int n = lvl[l][i];
+                const FSNode &f = td->nodes[n];
+                int *r = rec.data() + i * FS_REC;
+                r[0] = n;
+                r[1] = f.type | (f.player + 1) << 4 | f.num_children << 8;
+                if (f.type == FS_NODE_DECISION) {
+                    r[2] = abase[n];
+                    for (int a = 0; a < f.num_children; ++a) r[3 + a] = td->children[f.first_child + a];
+                } else {
+                    r[2] = f.first_child;
+                }
+            }
+            if ((rc = alloc(&d_lvl[l], rec.size(), rec.data()))) return rc;
+        }
+        if (const int nb = (int)sdset.boards.size()) {
+            hipLaunchKernelGGL(k_fs_rank, dim3(grid_for((size_t)nb * 2 * Mp)), dim3(SS_BLOCK), 0, st, d,
+                               sdset.d_boards, nb, p_rank);
+            hipLaunchKernelGGL(k_fs_wmat, dim3(grid_for((size_t)nb * tab)), dim3(SS_BLOCK), 0, st, d, p_rank, nb,
+                               sdset.d_tab);
+        }
+        if (const int nb = (int)foldset.boards.size())
+            hipLaunchKernelGGL(k_fs_fmat, dim3(grid_for((size_t)nb * tab)), dim3(SS_BLOCK), 0, st, d,
+                               foldset.d_boards, nb, foldset.d_tab);
         SS_HIP(hipGetLastError());
         SS_HIP(hipStreamSynchronize(st));
         return SS_OK;
@@ -515,26 +777,54 @@ struct FsRun {
     static unsigned grid_for(size_t work) {
         return (unsigned)std::max<size_t>(1, std::min<size_t>(8192, (work + SS_BLOCK - 1) / SS_BLOCK));
     }
+    // level sweeps: one node per wavefront, four per workgroup, every hand
+    // segment of the node in the same lane
+    static dim3 sweep_grid(size_t nodes) {
+        return dim3((unsigned)std::max<size_t>(1, std::min<size_t>(16384, (nodes + 3) / 4)));
+    }
+    template <int S> void down_s(size_t l, int opp) {
+        hipLaunchKernelGGL(k_fs_down<S>, sweep_grid(lvl[l].size()), dim3(SS_BLOCK), 0, st, d, d_lvl[l],
+                           (int)lvl[l].size(), opp, d.R, d.reach, d.root_sigma);
+    }
+    template <int S> void up_s(size_t l, int trav) {
+        hipLaunchKernelGGL(k_fs_up<S>, sweep_grid(lvl[l].size()), dim3(SS_BLOCK), 0, st, d, d_lvl[l],
+                           (int)lvl[l].size(), trav, d.R, d.SS, d.cfv, d.root_sigma);
+    }
+    void launch_down(size_t l, int opp) {
+        switch ((d.nh[opp] + 63) / 64) {
+        case 1: down_s<1>(l, opp); break;
+        case 2: down_s<2>(l, opp); break;
+        case 3: down_s<3>(l, opp); break;
+        default: down_s<4>(l, opp); break;
+        }
+    }
+    void launch_up(size_t l, int trav) {
+        switch ((d.nh[trav] + 63) / 64) {
+        case 1: up_s<1>(l, trav); break;
+        case 2: up_s<2>(l, trav); break;
+        case 3: up_s<3>(l, trav); break;
+        default: up_s<4>(l, trav); break;
+        }
+    }
 
     // one iteration: traverser 0's pass, then traverser 1's, then t + 1
     void launch_iteration() {
+        const unsigned term_grid = (unsigned)std::max(1, 2 * cus);   // two workgroups' tables and rows fit a CU's LDS
         for (int trav = 0; trav < 2; ++trav) {
             const int opp = 1 - trav;
-            if (!dec.empty())
-                hipLaunchKernelGGL(k_fs_match, dim3(grid_for(dec.size() * (size_t)d.Mp)), dim3(SS_BLOCK), 0, st, d,
-                                   d_dec, (int)dec.size());
             hipLaunchKernelGGL(k_fs_root, dim3(1), dim3(SS_BLOCK), 0, st, d, opp);
             for (size_t l = 0; l < lvl.size(); ++l)
-                if (!lvl[l].empty())
-                    hipLaunchKernelGGL(k_fs_down, dim3(grid_for(lvl[l].size() * (size_t)d.Mp)), dim3(SS_BLOCK), 0, st,
-                                       d, d_lvl[l], (int)lvl[l].size(), opp);
-            if (!fold_sd.empty())
-                hipLaunchKernelGGL(k_fs_terminal, dim3((unsigned)std::min<size_t>(fold_sd.size(), 65535)),
-                                   dim3(SS_BLOCK), 0, st, d, d_term, (int)fold_sd.size(), trav);
+                if (!lvl[l].empty()) launch_down(l, opp);
+            if (!foldset.tasks.empty())
+                hipLaunchKernelGGL(k_fs_term<true>, dim3((unsigned)std::min<size_t>(foldset.tasks.size(), term_grid)),
+                                   dim3(SS_BLOCK), tab_bytes, st, d, foldset.d_tasks, (int)foldset.tasks.size(),
+                                   foldset.d_nodes, foldset.d_tab, d.reach, d.cfv, trav);
+            if (!sdset.tasks.empty())
+                hipLaunchKernelGGL(k_fs_term<false>, dim3((unsigned)std::min<size_t>(sdset.tasks.size(), term_grid)),
+                                   dim3(SS_BLOCK), tab_bytes, st, d, sdset.d_tasks, (int)sdset.tasks.size(),
+                                   sdset.d_nodes, sdset.d_tab, d.reach, d.cfv, trav);
             for (size_t l = lvl.size(); l-- > 0;)
-                if (!lvl[l].empty())
-                    hipLaunchKernelGGL(k_fs_up, dim3(grid_for(lvl[l].size() * (size_t)d.Mp)), dim3(SS_BLOCK), 0, st,
-                                       d, d_lvl[l], (int)lvl[l].size(), trav);
+                if (!lvl[l].empty()) launch_up(l, trav);
         }
         hipLaunchKernelGGL(k_fs_next_iter, dim3(1), dim3(64), 0, st, d.iter);
     }
@@ -575,8 +865,7 @@ int fs_solve_impl(FSTreeData *td, int iters, FSOutput *out, bool all) {
     if (root.type == FS_NODE_DECISION) {
         const int rp = root.player, nh = td->num_hands[rp], na = root.num_children;
         std::vector<float> sg((size_t)na * Mp);
-        SS_HIP(hipMemcpy(sg.data(), r.d.sigma + (size_t)r.abase[0] * Mp, sizeof(float) * sg.size(),
-                         hipMemcpyDeviceToHost));
+        SS_HIP(hipMemcpy(sg.data(), r.d.root_sigma, sizeof(float) * sg.size(), hipMemcpyDeviceToHost));
         out->root_num_actions = na;
         out->root_player = rp;
         out->root_strategy = (float *)malloc(sizeof(float) * std::max(1, na * nh));
