@@ -39,6 +39,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <tuple>
 #include <vector>
 
 namespace ss {
@@ -471,84 +472,102 @@ __global__ void __launch_bounds__(SS_BLOCK) k_fs_up(FsDev d, const int *__restri
     }
 }
 
-// Terminal values, fold (FOLD) or showdown, for one traverser: a task is the
-// set of terminal nodes on one board (tasks[j] = table, first, count into
-// `nodes`); its workgroup stages the board's codes for this traverser
-// ([no16][Mp] uint4, 16 opponent hands per lane word) in LDS once, then the
-// nodes' opponent reach rows 16 at a time, and computes four nodes at a time,
-// each lane one traverser hand (the reach values are LDS broadcasts). Per opponent
-// hand o, in the reference's order:
-//   fold      acc = fma(code, r_o, acc)                  (code 1 / 0:
-//             flop_solve.cu:737-779 adds r_o or skips it)
-//   showdown  acc = fma(fma(code, r_o, -r_o), pot/2, acc) (code 2 / 0 / 1:
-//             +-r_o pot/2 or nothing, flop_solve.cu:781-833)
-// A zero term leaves acc unchanged, so the sums are the reference's bit for
-// bit; the fold value is acc times the payoff.
-constexpr int FS_TASK_NODES = 16;
-constexpr int FS_ROW16 = 16 * ((FS_MAX_HANDS + 15) / 16);   // staged reach row: opponent hands padded to 16
+// 64 fused multiply-adds acc[m] = fma(bcast_k(rv[m]), w[k], acc[m]) for
+// k = 0..15 (outer) and m = 0..3: rv[m] holds 16 opponent values per 16-lane
+// row (lane l: value l % 16), and row_newbcast:k hands every lane of a row
+// that row's lane k, so one instruction both broadcasts the opponent value and
+// accumulates it. Written as one asm block because the compiler does not fold
+// a broadcast into a multiply-add; the leading s_nop covers the DPP read
+// hazards after a VALU write of rv or of EXEC. All 64 lanes must be active.
+#define FS_DPP_STEP(k, w)  \
+    "v_fmac_f32_dpp %0, %4, %" #w " row_newbcast:" #k " row_mask:0xf bank_mask:0xf\n" \
+    "v_fmac_f32_dpp %1, %5, %" #w " row_newbcast:" #k " row_mask:0xf bank_mask:0xf\n" \
+    "v_fmac_f32_dpp %2, %6, %" #w " row_newbcast:" #k " row_mask:0xf bank_mask:0xf\n" \
+    "v_fmac_f32_dpp %3, %7, %" #w " row_newbcast:" #k " row_mask:0xf bank_mask:0xf\n"
 
+__device__ __forceinline__ void fs_dpp_fma16(float (&acc)[4], const float (&rv)[4], const float (&w)[16]) {
+    asm volatile("s_nop 4\n" FS_DPP_STEP(0, 8) FS_DPP_STEP(1, 9) FS_DPP_STEP(2, 10) FS_DPP_STEP(3, 11)
+                     FS_DPP_STEP(4, 12) FS_DPP_STEP(5, 13) FS_DPP_STEP(6, 14) FS_DPP_STEP(7, 15)
+                         FS_DPP_STEP(8, 16) FS_DPP_STEP(9, 17) FS_DPP_STEP(10, 18) FS_DPP_STEP(11, 19)
+                             FS_DPP_STEP(12, 20) FS_DPP_STEP(13, 21) FS_DPP_STEP(14, 22) FS_DPP_STEP(15, 23)
+                 : "+v"(acc[0]), "+v"(acc[1]), "+v"(acc[2]), "+v"(acc[3])
+                 : "v"(rv[0]), "v"(rv[1]), "v"(rv[2]), "v"(rv[3]), "v"(w[0]), "v"(w[1]), "v"(w[2]), "v"(w[3]),
+                   "v"(w[4]), "v"(w[5]), "v"(w[6]), "v"(w[7]), "v"(w[8]), "v"(w[9]), "v"(w[10]), "v"(w[11]),
+                   "v"(w[12]), "v"(w[13]), "v"(w[14]), "v"(w[15]));
+}
+#undef FS_DPP_STEP
+
+// Terminal values, fold (FOLD) or showdown, for one traverser. A task is the
+// set of terminal nodes on one board (tasks[j] = table, first group, groups),
+// in groups of four nodes (groups[2 g] = the nodes, a short group repeating
+// its last; groups[2 g + 1] = {nodes in the group, pot}; the four nodes of a
+// showdown group have one pot). The workgroup stages the board's codes for
+// this traverser ([no16][Mp] uint4, 16 opponent hands per lane word) in LDS,
+// each lane is one traverser hand, and per opponent hand o, in the
+// reference's order:
+//   fold      acc = fma(r_o, code, acc), code 1 / 0: flop_solve.cu:737-779
+//             adds r_o or skips it; the value is acc times the payoff
+//   showdown  acc = fma(r_o, c pot/2, acc), c = code - 1 = +1 / -1 / 0:
+//             +-r_o pot/2 or nothing, flop_solve.cu:781-833
+// The product r_o * code (or r_o * c pot/2) is exact, and a zero term leaves
+// acc unchanged, so the sums are the reference's bit for bit.
 template <bool FOLD>
 __global__ void __launch_bounds__(SS_BLOCK) k_fs_term(FsDev d, const int4 *__restrict__ tasks, int ntasks,
-                                                      const int *__restrict__ nodes, const uint4 *__restrict__ tables,
+                                                      const int4 *__restrict__ groups, const uint4 *__restrict__ tables,
                                                       const float *__restrict__ reach, float *__restrict__ cfv,
                                                       int trav) {
     extern __shared__ uint4 s_tab[];
-    __shared__ float s_rch[FS_TASK_NODES][FS_ROW16];
-    const int n_trav = d.nh[trav], Mp = d.Mp, no16 = d.no16, rw = 16 * no16;
-    const int h = threadIdx.x;
+    const int n_trav = d.nh[trav], Mp = d.Mp, no16 = d.no16;
+    const int h = threadIdx.x, ht = min(h, Mp - 1), l16 = threadIdx.x & 15;
     const float half_start = d.starting_pot * 0.5f;
     for (int j = blockIdx.x; j < ntasks; j += gridDim.x) {
         const int4 tk = tasks[j];
         __syncthreads();   // the previous task's readers are done
         const uint4 *src = tables + ((size_t)tk.x * 2 + trav) * no16 * Mp;
         for (int i = threadIdx.x; i < no16 * Mp; i += blockDim.x) s_tab[i] = src[i];
-        for (int f = 0; f < tk.z; f += FS_TASK_NODES) {   // the board's nodes, 16 rows staged at a time
-          const int cnt = min(FS_TASK_NODES, tk.z - f), first = tk.y + f;
-          if (f) __syncthreads();   // the previous rows' readers are done
-          for (int i = threadIdx.x; i < cnt * rw; i += blockDim.x) {
-              const int m = i / rw, o = i - m * rw;   // o < rw <= Mp: inside the row
-              s_rch[m][o] = reach[(size_t)nodes[first + m] * Mp + o];
-          }
-          __syncthreads();
-          if (h >= n_trav) continue;
-          for (int g = 0; g < cnt; g += 4) {
-            int nn[4], mm[4];
-            float scale[4];   // showdown: pot / 2; fold: the payoff
+        __syncthreads();
+        // every lane computes (the broadcasts need all 64 active); lanes
+        // past the hand count read a valid column and store nothing
+        for (int g = tk.y; g < tk.y + tk.z; ++g) {
+            const int4 gn = groups[2 * g], gm = groups[2 * g + 1];
+            const int nn[4] = {gn.x, gn.y, gn.z, gn.w};
+            float scale[4];
 #pragma unroll
             for (int m = 0; m < 4; ++m) {
-                mm[m] = min(g + m, cnt - 1);   // a short group repeats its last node
-                nn[m] = nodes[first + mm[m]];
-                const int4 nd = d.node[nn[m]];
                 if (FOLD) {
-                    const int winner = (nd.x >> 4) - 1, loser = 1 - winner;
+                    const int winner = (d.node[nn[m]].x >> 4) - 1, loser = 1 - winner;
                     const int2 bt = d.bets[nn[m]];
                     scale[m] = trav == winner ? half_start + (float)(loser == 0 ? bt.x : bt.y)
                                               : -(half_start + (float)(trav == 0 ? bt.x : bt.y));
                 } else {
-                    scale[m] = nd.w * 0.5f;
+                    scale[m] = 1.f;
                 }
             }
-            float acc[4] = {0.f, 0.f, 0.f, 0.f};
+            const float hp = gm.y * 0.5f;
+            float acc[4] = {0.f, 0.f, 0.f, 0.f}, rv[4];
+#pragma unroll
+            for (int m = 0; m < 4; ++m) rv[m] = reach[(size_t)nn[m] * Mp + l16];
             for (int c = 0; c < no16; ++c) {
-                const uint4 q = s_tab[c * Mp + h];
+                // the next step's opponent values in flight during this step
+                const int cn = min(c + 1, no16 - 1);
+                float rn[4], w[16];
+#pragma unroll
+                for (int m = 0; m < 4; ++m) rn[m] = reach[(size_t)nn[m] * Mp + 16 * cn + l16];
+                const uint4 q = s_tab[c * Mp + ht];
                 const uint32_t qw[4] = {q.x, q.y, q.z, q.w};
 #pragma unroll
                 for (int k = 0; k < 16; ++k) {
                     const float code = (float)((qw[k >> 2] >> (8 * (k & 3))) & 255u);
-#pragma unroll
-                    for (int m = 0; m < 4; ++m) {
-                        const float r = s_rch[mm[m]][16 * c + k];
-                        if (FOLD)
-                            acc[m] = fmaf(code, r, acc[m]);
-                        else
-                            acc[m] = fmaf(fmaf(code, r, -r), scale[m], acc[m]);
-                    }
+                    w[k] = FOLD ? code : fmaf(code, hp, -hp);
                 }
-            }
+                fs_dpp_fma16(acc, rv, w);
 #pragma unroll
-            for (int m = 0; m < 4; ++m)
-                if (g + m < cnt) cfv[(size_t)nn[m] * Mp + h] = FOLD ? acc[m] * scale[m] : acc[m];
-          }
+                for (int m = 0; m < 4; ++m) rv[m] = rn[m];
+            }
+            if (h < n_trav)
+#pragma unroll
+                for (int m = 0; m < 4; ++m)
+                    if (m < gm.x) cfv[(size_t)nn[m] * Mp + h] = FOLD ? acc[m] * scale[m] : acc[m];
         }
     }
 }
@@ -570,24 +589,33 @@ struct FsRun {
     // device the same plus the per-board code tables.
     struct TermSet {
         std::vector<uint32_t> boards;
-        std::vector<int> sorted;
-        std::vector<int4> tasks;
+        std::vector<int4> tasks, groups;
         uint32_t *d_boards = nullptr;
-        int *d_nodes = nullptr;
-        int4 *d_tasks = nullptr;
+        int4 *d_tasks = nullptr, *d_groups = nullptr;
         uint4 *d_tab = nullptr;
-        void group(const std::vector<int> &nodes, const std::vector<uint32_t> &board) {
-            std::vector<std::pair<uint32_t, int>> keyed;
-            for (int n : nodes) keyed.push_back({board[n], n});
+        // by_pot: a group's four nodes share a pot (showdowns)
+        void group(const std::vector<int> &nodes, const std::vector<uint32_t> &board, const std::vector<int4> &node,
+                   bool by_pot) {
+            std::vector<std::tuple<uint32_t, int, int>> keyed;   // board, pot (or 0), node
+            for (int n : nodes) keyed.emplace_back(board[n], by_pot ? node[n].w : 0, n);
             std::sort(keyed.begin(), keyed.end());
             for (size_t i = 0; i < keyed.size();) {
                 size_t e = i;
-                while (e < keyed.size() && keyed[e].first == keyed[i].first) ++e;
-                tasks.push_back(make_int4((int)boards.size(), (int)i, (int)(e - i), 0));
-                boards.push_back(keyed[i].first);
+                while (e < keyed.size() && std::get<0>(keyed[e]) == std::get<0>(keyed[i])) ++e;
+                const int first = (int)groups.size() / 2;
+                for (size_t f = i; f < e;) {
+                    size_t g = f;
+                    while (g < e && g - f < 4 && std::get<1>(keyed[g]) == std::get<1>(keyed[f])) ++g;
+                    int nn[4];
+                    for (int m = 0; m < 4; ++m) nn[m] = std::get<2>(keyed[std::min(f + m, g - 1)]);
+                    groups.push_back(make_int4(nn[0], nn[1], nn[2], nn[3]));
+                    groups.push_back(make_int4((int)(g - f), std::get<1>(keyed[f]), 0, 0));
+                    f = g;
+                }
+                tasks.push_back(make_int4((int)boards.size(), first, (int)groups.size() / 2 - first, 0));
+                boards.push_back(std::get<0>(keyed[i]));
                 i = e;
             }
-            for (auto &k : keyed) sorted.push_back(k.second);
         }
     } sdset, foldset;
     std::vector<std::vector<int>> lvl;   // decision and chance nodes per depth
@@ -691,8 +719,8 @@ struct FsRun {
                 lvl[depth[n]].push_back(n);
         // terminals grouped by board (the nodes of one river betting tree
         // share it)
-        sdset.group(sd, board);
-        foldset.group(fold, board);
+        sdset.group(sd, board, node, true);
+        foldset.group(fold, board, node, false);
         std::vector<uint32_t> hands(2 * Mp, 0xFFFFu);
         std::vector<float> w(2 * Mp, 0.f);
         for (int p = 0; p < 2; ++p)
@@ -728,7 +756,7 @@ struct FsRun {
             return rc;
         for (TermSet *ts : {&sdset, &foldset})
             if ((rc = alloc(&ts->d_boards, ts->boards.size(), ts->boards.data())) ||
-                (rc = alloc(&ts->d_nodes, ts->sorted.size(), ts->sorted.data())) ||
+                (rc = alloc(&ts->d_groups, ts->groups.size(), ts->groups.data())) ||
                 (rc = alloc(&ts->d_tasks, ts->tasks.size(), ts->tasks.data())) ||
                 (rc = alloc(&ts->d_tab, ts->boards.size() * tab)))
                 return rc;
@@ -809,7 +837,7 @@ struct FsRun {
 
     // one iteration: traverser 0's pass, then traverser 1's, then t + 1
     void launch_iteration() {
-        const unsigned term_grid = (unsigned)std::max(1, 2 * cus);   // two workgroups' tables and rows fit a CU's LDS
+        const unsigned term_grid = (unsigned)std::max(1, 3 * cus);   // three staged tables fit a CU's LDS
         for (int trav = 0; trav < 2; ++trav) {
             const int opp = 1 - trav;
             hipLaunchKernelGGL(k_fs_root, dim3(1), dim3(SS_BLOCK), 0, st, d, opp);
@@ -818,11 +846,11 @@ struct FsRun {
             if (!foldset.tasks.empty())
                 hipLaunchKernelGGL(k_fs_term<true>, dim3((unsigned)std::min<size_t>(foldset.tasks.size(), term_grid)),
                                    dim3(SS_BLOCK), tab_bytes, st, d, foldset.d_tasks, (int)foldset.tasks.size(),
-                                   foldset.d_nodes, foldset.d_tab, d.reach, d.cfv, trav);
+                                   foldset.d_groups, foldset.d_tab, d.reach, d.cfv, trav);
             if (!sdset.tasks.empty())
                 hipLaunchKernelGGL(k_fs_term<false>, dim3((unsigned)std::min<size_t>(sdset.tasks.size(), term_grid)),
                                    dim3(SS_BLOCK), tab_bytes, st, d, sdset.d_tasks, (int)sdset.tasks.size(),
-                                   sdset.d_nodes, sdset.d_tab, d.reach, d.cfv, trav);
+                                   sdset.d_groups, sdset.d_tab, d.reach, d.cfv, trav);
             for (size_t l = lvl.size(); l-- > 0;)
                 if (!lvl[l].empty()) launch_up(l, trav);
         }
