@@ -356,6 +356,62 @@ def leaf_bench(reps=5):
     return rec
 
 
+FLOP_BOARD = [43, 51, 1]   # Qs As 2d
+
+
+def flop_spot(n_hands, lib=None):
+    """The flop-through-river workload of the reference's own flop benchmark
+    (bench/bench_gpu_flop.py:50-66): flop Qs As 2d, pot 650, stacks 9750, one
+    0.75-pot size with one raise per street; each player holds the first
+    n_hands two-card combos (c0 < c1) off the board, weight 1."""
+    import flop_solve
+    t = flop_solve.FlopTree(FLOP_BOARD, 650, 9750, [0.75], lib=lib)
+    combos = [(a, b) for a in range(52) for b in range(a + 1, 52) if a not in FLOP_BOARD and b not in FLOP_BOARD]
+    t.set_hands(combos[:n_hands], combos[:n_hands])
+    return t
+
+
+def flop_bench(hands=(80, 200), iters=200, reps=3, cpu_iters=1):
+    """Flop-through-river solves (include/flop_solve.h) on flop_spot: the wall
+    time of one fs_solve_gpu call (tree upload, per-board tables, the
+    captured iteration loop, read-back), the median of `reps` calls, at each
+    hand count; the per-iteration cost from the difference of a full and a
+    tenth-length call. CPU: the FP32 flop-solve oracle (one core) timed on
+    `cpu_iters` iterations of the smallest case and scaled to `iters`."""
+    import flop_solve
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import oracle_bind as ob
+    rec = {"workload": f"2-player flop through river with chance nodes: flop Qs As 2d, pot 650, stacks 9750, "
+                       f"0.75-pot size, 1 raise per street, {iters} iterations (the reference's "
+                       f"bench/bench_gpu_flop.py spot)", "iterations": iters, "cases": []}
+    for nh in hands:
+        t = flop_spot(nh)
+        flop_solve.solve(t, 2)
+        walls = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            flop_solve.solve(t, iters)
+            walls.append((time.perf_counter() - t0) * 1000)
+        t0 = time.perf_counter()
+        flop_solve.solve(t, iters // 10)
+        short = (time.perf_counter() - t0) * 1000
+        ms = float(np.median(walls))
+        rec["cases"].append({"hands_per_player": nh, "nodes": t.num_nodes, "gpu_ms_per_solve": round(ms, 2),
+                             "gpu_ms_per_iteration": round((ms - short) / (iters - iters // 10), 4)})
+        log(f"[bench] flop solve: {nh} hands, {t.num_nodes} nodes: {ms:.1f} ms per {iters}-iteration solve")
+        t.free()
+    t = flop_spot(hands[0])
+    t0 = time.perf_counter()
+    ob.fs_solve(t.td, cpu_iters)
+    cpu_ms = (time.perf_counter() - t0) * 1000 * iters / cpu_iters
+    t.free()
+    rec["cpu_oracle_ms_per_solve"] = round(cpu_ms, 1)
+    rec["cpu_kind"] = (f"port (C flop-solve oracle, FP32, 1 core) at {hands[0]} hands per player, "
+                       f"{cpu_iters} iteration(s) timed and scaled to {iters}")
+    rec["speedup_vs_cpu_oracle"] = round(cpu_ms / rec["cases"][0]["gpu_ms_per_solve"], 1)
+    return rec
+
+
 def traffic_per_launch(algo_per_launch, workload):
     """HBM bytes per CFR iteration from the committed counter profile
     (profiles/cfr_iteration_traffic.md: calibrated FETCH_SIZE + WRITE_SIZE of a
@@ -644,6 +700,15 @@ def error_report(a):
     return 0
 
 
+def _profile_flop(iters=50):
+    """One flop_spot solve at 200 hands per player for a kernel trace."""
+    import flop_solve
+    t = flop_spot(200)
+    flop_solve.solve(t, iters)
+    log(f"[profile] flop solve: {t.num_nodes} nodes, {iters} iterations")
+    return 0
+
+
 PROFILE_TARGETS = {
     "traffic": profile_traffic,
     "precision": profile_precision,
@@ -651,6 +716,7 @@ PROFILE_TARGETS = {
     "leaf": lambda a: _profile_leaf(False),
     "leaf-flop": lambda a: _profile_leaf(True),
     "error-report": error_report,
+    "flop": lambda a: _profile_flop(),
 }
 
 
@@ -678,6 +744,7 @@ def main():
     ap.add_argument("--precision-iters", type=int, default=1000)
     ap.add_argument("--precision-ref-boards", type=int, default=8, help="boards compared with the FP64 oracle")
     ap.add_argument("--no-leaf", action="store_true", help="skip the turn leaf-value producer run")
+    ap.add_argument("--no-flop", action="store_true", help="skip the flop-through-river solve run")
     ap.add_argument("--fresh-steps", type=int, default=2,
                     help="steps of the setup-inclusive rate (fresh subgames each step); 0 skips it")
     ap.add_argument("--profile-target", default=None, choices=sorted(PROFILE_TARGETS),
@@ -837,6 +904,10 @@ def main():
         line["leaf_values"] = leaf_bench()
         log(f"[bench] leaf values: {line['leaf_values']['gpu_ms_per_call']} ms per call on the GPU, "
             f"{line['leaf_values']['cpu_oracle_ms']} ms CPU oracle")
+
+    if not a.no_flop:
+        log("[bench] flop-through-river solves")
+        line["flop_solve"] = flop_bench()
 
     if not a.no_cpu:
         cores = a.cpu_cores or min(16, os.cpu_count() or 1)
