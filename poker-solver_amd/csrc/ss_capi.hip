@@ -249,9 +249,18 @@ struct BatchImpl {
         return SS_OK;
     }
 
+    // The make-up of the batch: subgames, players, and the decision and
+    // terminal task counts in quarter-octave buckets (batches drawn from one
+    // distribution share a key).
+    std::array<int64_t, 5> makeup_key() const {
+        const int ndec = pt.lvl_off.empty() ? 0 : pt.lvl_off[pt.L];
+        auto bucket = [](int x) { return (int64_t)std::lround(4.0 * std::log2(1.0 + x)); };
+        return {n, bucket(ndec), bucket(pt.nterm), pt.NPmax, (int64_t)cfg.precision};
+    }
+
     // Autotune the batched strategy on this batch: time two iterations of each
-    // (after one untimed iteration) and keep the faster. SS_BATCH_MODE=
-    // persistent|phased overrides.
+    // (after one untimed iteration) and keep the faster; the choice is kept per
+    // make-up for later batches. SS_BATCH_MODE=persistent|phased overrides.
     int choose_mode() {
         const char *env = getenv("SS_BATCH_MODE");
         if (env && !strcmp(env, "persistent")) { mode = 0; return SS_OK; }
@@ -261,6 +270,18 @@ struct BatchImpl {
             return rc ? rc : reset_state();
         }
         if (cfg.max_iterations < 3) { mode = 1; return SS_OK; }
+        static std::mutex mu;
+        static std::map<std::array<int64_t, 5>, int> chosen;
+        const auto key = makeup_key();
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            auto it = chosen.find(key);
+            if (it != chosen.end()) {
+                mode = it->second;
+                int rc = mode == 1 ? tune_shape() : SS_OK;
+                return rc ? rc : reset_state();
+            }
+        }
         float best = 1e30f;
         for (int m = 0; m < 2; ++m) {
             int rc = reset_state();
@@ -278,6 +299,10 @@ struct BatchImpl {
             SS_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
             if (log_level() > 0) fprintf(stderr, "[street_solve] batch mode %d: %.3f ms / 2 iterations\n", m, ms);
             if (ms < best) { best = ms; mode = m; }
+        }
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            chosen[key] = mode;
         }
         if (mode == 1) {
             int rc = tune_shape();
@@ -302,12 +327,7 @@ struct BatchImpl {
         const double t0 = now_ms();
         std::vector<Slot *> raw;
         for (auto &sl : slots) raw.push_back(sl.get());
-        // the make-up of the batch: subgames, players, and the decision and
-        // terminal task counts in quarter-octave buckets (batches drawn from
-        // one distribution share a key)
-        const int ndec = pt.lvl_off.empty() ? 0 : pt.lvl_off[pt.L];
-        auto bucket = [](int x) { return (int64_t)std::lround(4.0 * std::log2(1.0 + x)); };
-        const std::array<int64_t, 5> key{n, bucket(ndec), bucket(pt.nterm), pt.NPmax, (int64_t)cfg.precision};
+        const auto key = makeup_key();
         static std::mutex mu;
         static std::map<std::array<int64_t, 5>, PhasedTasks::Shape> cache;
         {
