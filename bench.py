@@ -118,7 +118,8 @@ def cpu_reference_baseline(cores, iters_per_solve=500):
 
 
 # ---------------------------------------------------------------- served form
-def served_bench(n, rate, seed, target, check_every, max_iters, iters_per_step, log_every=50, plain_fifo=False):
+def served_bench(n, rate, seed, target, check_every, max_iters, iters_per_step, log_every=50, plain_fifo=False,
+                 flags=0):
     """Poisson stream through ss_server_*: 70% rivers (2-3 players, as the
     batch mix) and 30% turns (2-3 players, continuation leaves N(0, 0.3 pot)).
     Requests are built before the clock starts and submitted at their arrival
@@ -145,7 +146,7 @@ def served_bench(n, rate, seed, target, check_every, max_iters, iters_per_step, 
             d = t_start + t_arr - time.perf_counter()
             if d > 0:
                 time.sleep(d)
-            tickets[i] = srv.submit(probs[i], max_iters, target=target, check_every=check_every)
+            tickets[i] = srv.submit(probs[i], max_iters, target=target, check_every=check_every, flags=flags)
 
     th = threading.Thread(target=submitter)
     th.start()
@@ -187,7 +188,7 @@ def served_bench(n, rate, seed, target, check_every, max_iters, iters_per_step, 
     if plain_fifo:
         service = []
         for i, pr in enumerate(probs):
-            r = ss_api.solve_plain(pr, max_iters, target=target, check_every=check_every)
+            r = ss_api.solve_plain(pr, max_iters, target=target, check_every=check_every, flags=flags)
             service.append(r.elapsed_ms)
             if (i + 1) % log_every == 0:
                 log(f"[bench] plain request loop {i + 1}/{n}")
@@ -731,6 +732,11 @@ def main():
     ap.add_argument("--seed", type=int, default=20261203 + 4000)
     ap.add_argument("--target", type=float, default=0.5, help="exploitability target, %% of pot")
     ap.add_argument("--check-every", type=int, default=25)
+    ap.add_argument("--fixed-checks", action="store_true",
+                    help="plain and served forms: check every --check-every iterations (default: the adaptive "
+                         "schedule, SS_FLAG_ADAPTIVE_CHECKS)")
+    ap.add_argument("--adaptive-batched", action="store_true",
+                    help="batched form: the adaptive check schedule (default: every --check-every iterations)")
     ap.add_argument("--max-iters", type=int, default=2000)
     ap.add_argument("--plain-sample", type=int, default=8)
     ap.add_argument("--cpu-cores", type=int, default=0, help="0 = min(16, host cores)")
@@ -760,13 +766,19 @@ def main():
               file=sys.stderr)
         return 2
 
+    import ss_abi
     import ss_api
     spots = make_spots(a.workload, a.subgames, a.seed)
     log(f"[bench] building {len(spots)} subgames ({a.workload})")
     problems = [ss_api.Problem.from_spot(s) for s in spots]
     nodes = [p.tree.num_nodes for p in problems]
     players = [p.num_players for p in problems]
-    batch = ss_api.Batch(problems, a.max_iters, target=a.target, check_every=a.check_every)
+    # check schedules: the batched form checks every subgame every check_every
+    # iterations (one check round serves the whole batch); the plain and served
+    # forms, whose checks are per subgame, use the adaptive schedule
+    bflags = ss_abi.SS_FLAG_ADAPTIVE_CHECKS if a.adaptive_batched else 0
+    flags = 0 if a.fixed_checks else ss_abi.SS_FLAG_ADAPTIVE_CHECKS
+    batch = ss_api.Batch(problems, a.max_iters, target=a.target, check_every=a.check_every, flags=bflags)
     log(f"[bench] uploaded; nodes mean {np.mean(nodes):.0f} max {max(nodes)}; players {np.bincount(players)[2:].tolist()}")
 
     for w in range(a.warmup):
@@ -792,7 +804,7 @@ def main():
         tf0 = time.perf_counter()
         for k, spots_k in enumerate(fresh):
             probs_k = [ss_api.Problem.from_spot(s) for s in spots_k]
-            bk = ss_api.Batch(probs_k, a.max_iters, target=a.target, check_every=a.check_every)
+            bk = ss_api.Batch(probs_k, a.max_iters, target=a.target, check_every=a.check_every, flags=bflags)
             reps_k, _ = bk.run()
             for i in range(len(probs_k)):
                 bk.output(i, reps_k[i])
@@ -819,7 +831,7 @@ def main():
     # plain form on a sample of the same subgames
     plain_ms = []
     for i in range(min(a.plain_sample, len(problems))):
-        r = ss_api.solve_plain(problems[i], a.max_iters, target=a.target, check_every=a.check_every)
+        r = ss_api.solve_plain(problems[i], a.max_iters, target=a.target, check_every=a.check_every, flags=flags)
         plain_ms.append(r.kernel_ms)
     plain_sps = 1000.0 / float(np.mean(plain_ms)) if plain_ms else None
     log(f"[bench] plain form: {np.mean(plain_ms):.1f} ms per solve on {len(plain_ms)} subgames")
@@ -846,6 +858,12 @@ def main():
             "subgames": len(problems),
             "target_exploitability_pct_pot": a.target,
             "check_every": a.check_every,
+            "check_schedule": {
+                "batched": "adaptive" if a.adaptive_batched else "every check_every iterations",
+                "plain_and_served": "every check_every iterations" if a.fixed_checks else
+                                    "adaptive (SS_FLAG_ADAPTIVE_CHECKS): first at check_every, then where each "
+                                    "subgame's exploitability would reach the target if it fell as t^-2, on the "
+                                    "5-iteration step, at most 4 x check_every apart"},
             "max_iterations": a.max_iters,
             "form": ("batched, phased: per traverser pass one launch per tree level / terminal class over the "
                      "(subgame, node) tasks of every live subgame, check_every iterations per hipGraph replay"
@@ -880,7 +898,8 @@ def main():
     if a.served_requests > 0:
         log(f"[bench] served form: {a.served_requests} requests at {a.served_rate}/s")
         line["served"] = served_bench(a.served_requests, a.served_rate, 20261203 + 5000, a.target,
-                                      a.check_every, a.max_iters, a.served_iters_per_step, plain_fifo=True)
+                                      a.check_every, a.max_iters, a.served_iters_per_step, plain_fifo=True,
+                                      flags=flags)
         log(f"[bench] served: {line['served']['solves_per_s']} solves/s, "
             f"p50 {line['served']['latency_ms']['p50']} ms, p99 {line['served']['latency_ms']['p99']} ms")
         pl = line["served"]["plain_request_loop"]
@@ -888,7 +907,7 @@ def main():
             f"p50 {pl['latency_ms']['p50']} ms, p99 {pl['latency_ms']['p99']} ms")
         log(f"[bench] served form capacity: the same {a.served_requests} requests submitted at once")
         cap = served_bench(a.served_requests, 1e9, 20261203 + 5000, a.target, a.check_every, a.max_iters,
-                           a.served_iters_per_step)
+                           a.served_iters_per_step, flags=flags)
         line["served"]["capacity_solves_per_s"] = cap["solves_per_s"]
         line["served"]["capacity_latency_ms"] = cap["latency_ms"]
         log(f"[bench] served capacity: {cap['solves_per_s']} solves/s")

@@ -140,6 +140,12 @@ typedef struct {
 } SSSolveConfig;
 
 #define SS_FLAG_NO_EXPL     1    /* skip the final exploitability evaluation */
+#define SS_FLAG_ADAPTIVE_CHECKS 2 /* first check at check_every, then each subgame's next
+                                     check where its exploitability would reach the target
+                                     if it fell as fast as t^-2 (rounded down to the
+                                     iteration step, at least one step and at most
+                                     4 * check_every later); without it checks are every
+                                     check_every iterations */
 
 typedef struct {
     int iterations;              /* iterations actually run */

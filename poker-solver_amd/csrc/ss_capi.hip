@@ -50,10 +50,13 @@ static int solve_plain(const HostProblem &hp, const SSSolveConfig &cfg, SSOutput
     const bool target = cfg.target_exploitability > 0;
     const int chunk = target ? std::max(1, cfg.check_every > 0 ? cfg.check_every : 50) : std::max(1, T);
     double kms = 0, expl = -1, gains[SS_MAX_PLAYERS] = {0, 0, 0, 0, 0, 0};
-    int done = 0, converged = 0, expl_at = -1;
+    int done = 0, converged = 0, expl_at = -1, due = chunk;
+    const bool adaptive = target && (cfg.flags & SS_FLAG_ADAPTIVE_CHECKS);
+    int astep = std::min(chunk, 10);   // the batched form's graph step: the largest divisor of chunk <= 10
+    while (chunk % astep) --astep;
     const double t_loop = now_ms();
     while (done < T) {
-        int k = std::min(chunk, T - done);
+        int k = std::min(due - done, T - done);
         rc = plain_run(s, k, done + k, g.st, &kms);
         if (rc) return rc;
         done += k;
@@ -65,6 +68,9 @@ static int solve_plain(const HostProblem &hp, const SSSolveConfig &cfg, SSOutput
                 converged = 1;
                 break;
             }
+            due = next_check_iter(done, expl, cfg.target_exploitability, chunk, astep, adaptive);
+        } else {
+            due = done + chunk;
         }
     }
     if (!(cfg.flags & SS_FLAG_NO_EXPL) && expl_at != done) {
@@ -172,7 +178,7 @@ struct BatchImpl {
             templ[i].iter = 0;
             templ[i].iter_end = T;
             templ[i].final_iter = T;
-            templ[i].snap_every = target ? chunk : 0;
+            templ[i].snap_every = target ? ((cfg.flags & SS_FLAG_ADAPTIVE_CHECKS) ? gsteps : chunk) : 0;
             templ[i].active = 1;
         }
         SS_HIP(hipMalloc(&d_slots, sizeof(SlotDesc) * std::max(1, n)));
@@ -403,12 +409,18 @@ struct BatchImpl {
         float ms = 0;
         SS_HIP(hipEventElapsedTime(&ms, ev[2], ev[3]));
         *check_ms += ms;
+        if (log_level() > 1) {
+            int nw = 0;
+            for (int i = 0; i < n; ++i) nw += sel[i] != 0;
+            fprintf(stderr, "[street_solve] check at %d: %d subgames, %.3f ms\n", at, nw, ms);
+        }
         SS_HIP(hipMemcpy(h_xout.data(), d_xout, sizeof(double) * h_xout.size(), hipMemcpyDeviceToHost));
         for (int i = 0; i < n; ++i) {
             if (!sel[i]) continue;
             expl[i] = expl_from_xout(h_xout.data() + 32 * (size_t)i, slots[i]->plan.NP, slots[i]->plan.starting_pot,
                                      &gains[(size_t)i * SS_MAX_PLAYERS]);
             expl_at[i] = at;
+            if (log_level() > 2) fprintf(stderr, "[street_solve] exploitability %d %d %.9g\n", i, at, expl[i]);
         }
         host_eval_ms += now_ms() - h0 - ms;
         return SS_OK;
@@ -438,12 +450,25 @@ struct BatchImpl {
         int64_t bytes = 0;
         int64_t live_bytes = 0;
         for (int i = 0; i < n; ++i) live_bytes += slots[i]->plan.algo_bytes_iter;
+        // iteration of each subgame's next check: every chunk iterations, or
+        // adaptive (each on a multiple of gsteps, so a replay count is whole)
+        const bool adaptive = target && (cfg.flags & SS_FLAG_ADAPTIVE_CHECKS);
+        std::vector<int> due(n, chunk);
         while (done < T) {
+            int next = done + chunk;
+            if (target) {
+                next = T;
+                for (int i = 0; i < n; ++i)
+                    if (live[i]) next = std::min(next, due[i]);
+                next = std::max(next, done + gsteps);
+            }
+            const int span = next - done;
+            const int reps = (span + gsteps - 1) / gsteps;
             SS_HIP(hipEventRecord(ev[0], st));
             if (use_graph) {
-                for (int k = 0; k < chunk / gsteps; ++k) SS_HIP(hipGraphLaunch(exec, st));
+                for (int k = 0; k < reps; ++k) SS_HIP(hipGraphLaunch(exec, st));
             } else {
-                for (int k = 0; k < chunk; ++k) {
+                for (int k = 0; k < reps * gsteps; ++k) {
                     int rc = launch_iteration(mode);
                     if (rc) return rc;
                 }
@@ -453,26 +478,33 @@ struct BatchImpl {
             float ms = 0;
             SS_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
             cfr_ms += ms;
-            int step = std::min(chunk, T - done);
+            if (log_level() > 1) {
+                int nl = 0;
+                for (int i = 0; i < n; ++i) nl += live[i];
+                fprintf(stderr, "[street_solve] iterations %d..%d: %d live, %.3f ms\n", done, done + span, nl, ms);
+            }
+            int step = std::min(reps * gsteps, T - done);
             done += step;
-            launches += chunk;
+            launches += reps * gsteps;
             bytes += live_bytes * step;
             for (int i = 0; i < n; ++i)
                 if (live[i]) iters[i] = done;
             if (target) {
-                for (int i = 0; i < n; ++i) want[i] = live[i];
+                for (int i = 0; i < n; ++i) want[i] = live[i] && (due[i] <= done || done >= T);
                 int rc = evaluate(want, done, &check_ms);
                 if (rc) return rc;
                 int nlive = 0;
                 static const int kZero = 0;
                 for (int i = 0; i < n; ++i) {
                     if (!live[i]) continue;
-                    if (expl[i] >= 0 && expl[i] <= cfg.target_exploitability) {
+                    if (want[i] && expl[i] >= 0 && expl[i] <= cfg.target_exploitability) {
                         conv[i] = 1;
                         live[i] = 0;
                         live_bytes -= slots[i]->plan.algo_bytes_iter;
                         SS_HIP(hipMemcpyAsync(&d_slots[i].active, &kZero, sizeof(int), hipMemcpyHostToDevice, st));
                     } else {
+                        if (want[i])
+                            due[i] = next_check_iter(done, expl[i], cfg.target_exploitability, chunk, gsteps, adaptive);
                         ++nlive;
                     }
                 }

@@ -128,6 +128,11 @@ int range_update_from_strategy(const float *d_SS, int row0, int na, int action, 
 
 // plain form
 int plain_run(Slot &s, int iters, int final_iter, hipStream_t st, double *kernel_ms);
+// Iteration of a subgame's next exploitability check after a check at
+// iteration t found e > target (% pot): t + every, or with adaptive checks
+// the iteration where e would reach the target if it fell as t^-2, rounded
+// down to a multiple of step, at least t + step and at most t + 4 * every.
+int next_check_iter(int t, double e, double target, int every, int step, bool adaptive);
 int plain_exploitability(Slot &s, hipStream_t st, double *expl_pct, double *gains);
 int read_output(Slot &s, const SlotDesc &desc_now, hipStream_t st, SSOutput *out, int legacy_root_ev);
 

@@ -1,6 +1,7 @@
 // ss_runtime.hip -- host runtime: problem intake, device plan, slot arenas,
 // the plain driver, exploitability and output extraction.
 #include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -567,6 +568,18 @@ double expl_from_xout(const double *xout, int np, int starting_pot, double *gain
 // ---------------------------------------------------------------------------
 // outputs in the reference layout (street_solve.cu:1556-1612)
 // ---------------------------------------------------------------------------
+int next_check_iter(int t, double e, double target, int every, int step, bool adaptive) {
+    if (!adaptive || !(e > target) || target <= 0) return t + every;
+    // exploitability of Linear CFR falls roughly as t^-a; over doubling
+    // intervals of the batch mix a stays below 2.2 for 99% of (subgame,
+    // interval) pairs, so assuming a = 2 rarely schedules a check after the
+    // crossing
+    const double r = std::sqrt(e / target);
+    const double tn = std::min((double)t + 4.0 * every, (double)t * r);
+    const int next = (int)(tn / step) * step;
+    return std::max(t + step, std::min(t + 4 * every, next));
+}
+
 int read_output(Slot &s, const SlotDesc &dnow, hipStream_t st, SSOutput *out, int) {
     const Plan &pl = s.plan;
     const size_t Mp = pl.Mp;

@@ -25,6 +25,7 @@ SS_PREC_FP32 = 0
 SS_PREC_BF16X2 = 1
 
 SS_FLAG_NO_EXPL = 1
+SS_FLAG_ADAPTIVE_CHECKS = 2
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG_DIR, "build", "street_solve.so")
