@@ -155,15 +155,27 @@ struct BatchImpl {
         SS_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         for (auto &e : ev) SS_HIP(hipEventCreate(&e));
         slots.resize(n);
-        for (int i = 0; i < n; ++i) {
-            slots[i].reset(new Slot());
-            int rc = slots[i]->setup(hps[i], st);
-            if (!rc && prec) rc = prec_setup(*slots[i], st);
-            if (rc) return set_error(rc, "subgame %d: %s", i, last_error_str());
-            if (target || want_final) {
-                rc = slots[i]->alloc_expl(st, 1);
-                if (rc) return rc;
+        const double c0 = now_ms();
+        {
+            // every subgame's plan, arena and uploads queued, one wait for the
+            // combo ranks, then the river tables and the FP64 buffers queued
+            PinnedStage stage;
+            for (int i = 0; i < n; ++i) {
+                slots[i].reset(new Slot());
+                int rc = slots[i]->setup_begin(hps[i], st, stage);
+                if (rc) return set_error(rc, "subgame %d: %s", i, last_error_str());
             }
+            SS_HIP(hipStreamSynchronize(st));
+            for (int i = 0; i < n; ++i) {
+                int rc = slots[i]->setup_finish(st, stage);
+                if (!rc && prec) rc = prec_setup(*slots[i], st);
+                if (rc) return set_error(rc, "subgame %d: %s", i, last_error_str());
+                if (target || want_final) {
+                    rc = slots[i]->alloc_expl(st, 1, stage);
+                    if (rc) return rc;
+                }
+            }
+            SS_HIP(hipStreamSynchronize(st));   // before the stage is released
         }
         if (target || want_final) {
             SS_HIP(hipMalloc(&d_xout, sizeof(double) * 32 * (size_t)n));
@@ -172,6 +184,7 @@ struct BatchImpl {
             h_xout.assign(32 * (size_t)n, 0.0);
             joint_done.assign(n, 0);
         }
+        const double c1 = now_ms();
         templ.resize(n);
         for (int i = 0; i < n; ++i) {
             templ[i] = slots[i]->desc;
@@ -187,17 +200,26 @@ struct BatchImpl {
         for (auto &sl : slots) raw.push_back(sl.get());
         int rc = build_phased_tasks(raw, pt, st);
         if (rc) return rc;
+        const double c2 = now_ms();
         xblocks = std::max(1, std::min(512, std::max(n, pt.nterm)));
         if (target || want_final)
             SS_HIP(hipMalloc(&scratch, sizeof(double) * (size_t)xblocks * SS_XSCRATCH_DOUBLES));
         rc = choose_mode();
         if (rc) return rc;
+        const double c3 = now_ms();
         const char *ng = getenv("SS_NO_GRAPH");
         use_graph = !(ng && atoi(ng) != 0);
         SS_HIP(hipMemcpyAsync(d_slots, templ.data(), sizeof(SlotDesc) * n, hipMemcpyHostToDevice, st));
         SS_HIP(hipStreamSynchronize(st));
         full_tasks = pt.lvl_off.empty() ? 0 : pt.lvl_off[pt.L] + pt.nterm;
-        return capture();
+        rc = capture();
+        if (log_level() > 0) {
+            double plan = 0;
+            for (auto &sl : slots) plan += sl->plan_ms;
+            fprintf(stderr, "[street_solve] batch create: subgames %.1f ms (plans %.1f), task lists %.1f, mode and "
+                            "shape %.1f, capture %.1f ms\n", c1 - c0, plan, c2 - c1, c3 - c2, now_ms() - c3);
+        }
+        return rc;
     }
 
     // capture `gsteps` iterations of the current task lists into the graph

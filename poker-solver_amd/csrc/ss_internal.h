@@ -94,6 +94,25 @@ struct PrecState {
     ~PrecState();
 };
 
+// Page-locked host staging for uploads: upload() copies host data into
+// pinned memory and queues the copy, so it does not wait for the stream (a
+// copy from pageable memory does); large blocks go straight from the
+// caller's memory. Keep the stage alive until the stream is synchronised.
+struct PinnedStage {
+    struct Chunk {
+        char *p = nullptr;
+        size_t cap = 0, used = 0;
+    };
+    std::vector<Chunk> chunks;
+    size_t chunk_bytes = (size_t)8 << 20;
+    PinnedStage() = default;
+    PinnedStage(const PinnedStage &) = delete;
+    PinnedStage &operator=(const PinnedStage &) = delete;
+    ~PinnedStage();
+    int alloc(size_t n, void **out);   // n bytes of pinned memory (256-byte aligned)
+    int upload(void *dst, const void *src, size_t n, hipStream_t st);
+};
+
 // A subgame resident on the device.
 struct Slot {
     std::unique_ptr<PrecState> prec;   // precision form only
@@ -106,10 +125,18 @@ struct Slot {
     size_t xarena_bytes = 0;
     double *xscratch = nullptr;   // per-workgroup FP64 terminal scratch
     int xscratch_blocks = 0;
+    double plan_ms = 0;           // host time of make_plan in setup
+    uint32_t *h_cstr = nullptr;   // combo ranks read back by setup_begin (pinned, in the stage)
+    size_t off_cstr = 0, off_gsort = 0, off_clist = 0, off_clen = 0, off_bnd = 0, off_cpos = 0, off_hmeta = 0;
 
     ~Slot();
+    // setup = setup_begin, a stream synchronisation, setup_finish; a batch
+    // runs every subgame's setup_begin before one synchronisation
     int setup(const HostProblem &hp, hipStream_t st);
+    int setup_begin(const HostProblem &hp, hipStream_t st, PinnedStage &stage);
+    int setup_finish(hipStream_t st, PinnedStage &stage);
     int alloc_expl(hipStream_t st, int blocks);
+    int alloc_expl(hipStream_t st, int blocks, PinnedStage &stage);
     int64_t footprint() const { return (int64_t)arena_bytes + (int64_t)xarena_bytes; }
 };
 

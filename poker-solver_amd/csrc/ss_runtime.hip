@@ -1,6 +1,7 @@
 // ss_runtime.hip -- host runtime: problem intake, device plan, slot arenas,
 // the plain driver, exploitability and output extraction.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -382,8 +383,60 @@ int64_t estimate_footprint(const HostProblem &hp) {
     return b + 64 * 1024;
 }
 
+int PinnedStage::alloc(size_t n, void **out) {
+    n = (n + 255) & ~(size_t)255;
+    for (auto &c : chunks)
+        if (c.cap - c.used >= n) {
+            *out = c.p + c.used;
+            c.used += n;
+            return SS_OK;
+        }
+    Chunk c;
+    c.cap = std::max(n, chunk_bytes);
+    SS_HIP(hipHostMalloc((void **)&c.p, c.cap, hipHostMallocDefault));
+    c.used = n;
+    *out = c.p;
+    chunks.push_back(c);
+    return SS_OK;
+}
+
+int PinnedStage::upload(void *dst, const void *src, size_t n, hipStream_t st) {
+    if (n == 0) return SS_OK;
+    if (n > chunk_bytes / 2) {   // large (leaf values): straight from the caller's memory
+        SS_HIP(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, st));
+        return SS_OK;
+    }
+    void *pin = nullptr;
+    int rc = alloc(n, &pin);
+    if (rc) return rc;
+    std::memcpy(pin, src, n);
+    SS_HIP(hipMemcpyAsync(dst, pin, n, hipMemcpyHostToDevice, st));
+    return SS_OK;
+}
+
+PinnedStage::~PinnedStage() {
+    for (auto &c : chunks) (void)hipHostFree(c.p);
+}
+
 int Slot::setup(const HostProblem &hp, hipStream_t st) {
+    PinnedStage stage;
+    int rc = setup_begin(hp, st, stage);
+    if (rc) return rc;
+    SS_HIP(hipStreamSynchronize(st));
+    rc = setup_finish(st, stage);
+    if (rc) return rc;
+    SS_HIP(hipStreamSynchronize(st));   // the stage's memory is released on return
+    return SS_OK;
+}
+
+// Plan, device arena, uploads of the plan's arrays and, on a river, the
+// device evaluation of every combo's rank, read back into pinned memory
+// (h_cstr): all queued on `st`, nothing waited for. setup_finish builds the
+// river tables from those ranks once the caller has synchronised the stream.
+int Slot::setup_begin(const HostProblem &hp, hipStream_t st, PinnedStage &stage) {
+    const auto t0 = std::chrono::steady_clock::now();
     int rc = make_plan(hp, plan);
+    plan_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (rc) return rc;
     Plan &pl = plan;
     const size_t Mp = pl.Mp;
@@ -416,25 +469,25 @@ int Slot::setup(const HostProblem &hp, hipStream_t st) {
     SS_HIP(hipMalloc(&d_desc, sizeof(SlotDesc)));
     char *a = arena;
     SS_HIP(hipMemsetAsync(arena, 0, arena_bytes, st));
-    SS_HIP(hipMemcpyAsync(a + o_nodes, pl.nodes.data(), sizeof(DevNode) * pl.N, hipMemcpyHostToDevice, st));
-    if (!pl.children.empty())
-        SS_HIP(hipMemcpyAsync(a + o_child, pl.children.data(), sizeof(int) * pl.children.size(),
-                              hipMemcpyHostToDevice, st));
-    SS_HIP(hipMemcpyAsync(a + o_declvl, pl.dec_lvl.data(), sizeof(int) * pl.dec_lvl.size(), hipMemcpyHostToDevice, st));
-    SS_HIP(hipMemcpyAsync(a + o_decoff, pl.dec_off.data(), sizeof(int) * pl.dec_off.size(), hipMemcpyHostToDevice, st));
-    SS_HIP(hipMemcpyAsync(a + o_term, pl.term.data(), sizeof(int) * pl.term.size(), hipMemcpyHostToDevice, st));
-    SS_HIP(hipMemcpyAsync(a + o_hinfo, pl.hinfo.data(), sizeof(uint32_t) * pl.NP * Mp, hipMemcpyHostToDevice, st));
-    SS_HIP(hipMemcpyAsync(a + o_w, pl.weights.data(), sizeof(float) * pl.NP * Mp, hipMemcpyHostToDevice, st));
-    if (pl.has_leaf)
-        SS_HIP(hipMemcpyAsync(a + o_leaf, pl.leaf_src, sizeof(float) * pl.leaf_count, hipMemcpyHostToDevice, st));
-    if (pl.frozen_src)
-        SS_HIP(hipMemcpyAsync(a + o_frozen, pl.frozen_src, sizeof(int) * pl.N, hipMemcpyHostToDevice, st));
-    SS_HIP(hipMemcpyAsync(a + o_rrow, pl.rrow.data(), sizeof(int) * pl.rrow.size(), hipMemcpyHostToDevice, st));
+#define SS_UP(dst, src, n)                                   \
+    do {                                                     \
+        int rc_ = stage.upload((dst), (src), (n), st);       \
+        if (rc_) return rc_;                                 \
+    } while (0)
+    SS_UP(a + o_nodes, pl.nodes.data(), sizeof(DevNode) * pl.N);
+    if (!pl.children.empty()) SS_UP(a + o_child, pl.children.data(), sizeof(int) * pl.children.size());
+    SS_UP(a + o_declvl, pl.dec_lvl.data(), sizeof(int) * pl.dec_lvl.size());
+    SS_UP(a + o_decoff, pl.dec_off.data(), sizeof(int) * pl.dec_off.size());
+    SS_UP(a + o_term, pl.term.data(), sizeof(int) * pl.term.size());
+    SS_UP(a + o_hinfo, pl.hinfo.data(), sizeof(uint32_t) * pl.NP * Mp);
+    SS_UP(a + o_w, pl.weights.data(), sizeof(float) * pl.NP * Mp);
+    if (pl.has_leaf) SS_UP(a + o_leaf, pl.leaf_src, sizeof(float) * pl.leaf_count);
+    if (pl.frozen_src) SS_UP(a + o_frozen, pl.frozen_src, sizeof(int) * pl.N);
+    SS_UP(a + o_rrow, pl.rrow.data(), sizeof(int) * pl.rrow.size());
     // root reach = raw weights (never overwritten: the forward sweep writes children only)
     for (int p = 0; p < pl.NP; ++p)
-        SS_HIP(hipMemcpyAsync(a + o_reach + sizeof(float) * (size_t)p * pl.N * Mp, pl.weights.data() + (size_t)p * Mp,
-                              sizeof(float) * Mp, hipMemcpyHostToDevice, st));
-
+        SS_UP(a + o_reach + sizeof(float) * (size_t)p * pl.N * Mp, pl.weights.data() + (size_t)p * Mp,
+              sizeof(float) * Mp);
     SlotDesc &d = desc;
     std::memset(&d, 0, sizeof(d));
     d.N = pl.N; d.NP = pl.NP; d.Mp = pl.Mp; d.maxh = pl.maxh;
@@ -468,20 +521,41 @@ int Slot::setup(const HostProblem &hp, hipStream_t st) {
     d.hmeta = pl.is_river ? (const uint4 *)(a + o_hmeta) : nullptr;
     d.active = 1;
 
+    h_cstr = nullptr;
+    off_cstr = o_cstr; off_gsort = o_gsort; off_clist = o_clist; off_clen = o_clen; off_bnd = o_bnd;
+    off_cpos = o_cpos; off_hmeta = o_hmeta;
     if (pl.is_river && pl.num_board == 5) {
-        // ranks of all combos on this board, evaluated on the device
+        // ranks of all combos on this board, evaluated on the device, read
+        // back into pinned memory without waiting
         int *dboard = (int *)(a + o_board);
-        SS_HIP(hipMemcpyAsync(dboard, pl.board, sizeof(int) * 5, hipMemcpyHostToDevice, st));
-        int rc2 = launch_combo_ranks(dboard, (uint32_t *)(a + o_cstr), st);
-        if (rc2) return rc2;
-        std::vector<uint32_t> cstr(SS_NCOMBO);
-        SS_HIP(hipMemcpyAsync(cstr.data(), a + o_cstr, sizeof(uint32_t) * SS_NCOMBO, hipMemcpyDeviceToHost, st));
-        SS_HIP(hipStreamSynchronize(st));
+        SS_UP(dboard, pl.board, sizeof(int) * 5);
+        rc = launch_combo_ranks(dboard, (uint32_t *)(a + o_cstr), st);
+        if (rc) return rc;
+        void *pin = nullptr;
+        rc = stage.alloc(sizeof(uint32_t) * SS_NCOMBO, &pin);
+        if (rc) return rc;
+        h_cstr = (uint32_t *)pin;
+        SS_HIP(hipMemcpyAsync(h_cstr, a + o_cstr, sizeof(uint32_t) * SS_NCOMBO, hipMemcpyDeviceToHost, st));
+    } else {
+        d.is_river = 0;   // showdowns off the river carry no ranks: value 0, as street_solve.cu:1520
+    }
+    return SS_OK;
+}
+
+// The river tables from the ranks setup_begin read back (the stream has been
+// synchronised since), uploaded through the stage, then the descriptor.
+int Slot::setup_finish(hipStream_t st, PinnedStage &stage) {
+    Plan &pl = plan;
+    const size_t Mp = pl.Mp;
+    SlotDesc &d = desc;
+    char *a = arena;
+    if (h_cstr) {
         RiverTables rt;
-        build_river_tables(cstr.data(), pl.lab, rt);
+        build_river_tables(h_cstr, pl.lab, rt);
+        h_cstr = nullptr;
         d.ng = rt.ng;
-        SS_HIP(hipMemcpyAsync(a + o_gsort, rt.gsort.data(), sizeof(uint16_t) * rt.gsort.size(), hipMemcpyHostToDevice, st));
-        SS_HIP(hipMemcpyAsync(a + o_cpos, rt.cpos.data(), 52 * 52, hipMemcpyHostToDevice, st));
+        SS_UP(a + off_gsort, rt.gsort.data(), sizeof(uint16_t) * rt.gsort.size());
+        SS_UP(a + off_cpos, rt.cpos.data(), 52 * 52);
         // hands of every player by ascending strength (dead hands last), one
         // record per hand with everything a showdown sweep needs
         std::vector<uint4> hm((size_t)pl.NP * Mp);
@@ -507,19 +581,24 @@ int Slot::setup(const HostProblem &hp, hipStream_t st) {
                 hm[(size_t)p * Mp + i] = r;
             }
         }
-        SS_HIP(hipMemcpyAsync(a + o_hmeta, hm.data(), sizeof(uint4) * hm.size(), hipMemcpyHostToDevice, st));
-        SS_HIP(hipMemcpyAsync(a + o_clist, rt.clist.data(), 52 * 52, hipMemcpyHostToDevice, st));
-        SS_HIP(hipMemcpyAsync(a + o_clen, rt.clen.data(), 52, hipMemcpyHostToDevice, st));
-        SS_HIP(hipMemcpyAsync(a + o_bnd, rt.bnd.data(), SS_BND_BYTES, hipMemcpyHostToDevice, st));
-        SS_HIP(hipStreamSynchronize(st));   // host tables go out of scope
-    } else {
-        d.is_river = 0;   // showdowns off the river carry no ranks: value 0, as street_solve.cu:1520
+        SS_UP(a + off_hmeta, hm.data(), sizeof(uint4) * hm.size());
+        SS_UP(a + off_clist, rt.clist.data(), 52 * 52);
+        SS_UP(a + off_clen, rt.clen.data(), 52);
+        SS_UP(a + off_bnd, rt.bnd.data(), SS_BND_BYTES);
     }
-    SS_HIP(hipMemcpyAsync(d_desc, &d, sizeof(d), hipMemcpyHostToDevice, st));
+    SS_UP(d_desc, &d, sizeof(d));
     return SS_OK;
 }
 
 int Slot::alloc_expl(hipStream_t st, int blocks) {
+    PinnedStage stage;
+    int rc = alloc_expl(st, blocks, stage);
+    if (rc) return rc;
+    SS_HIP(hipStreamSynchronize(st));   // the stage's memory is released on return
+    return SS_OK;
+}
+
+int Slot::alloc_expl(hipStream_t st, int blocks, PinnedStage &stage) {
     if (xarena && xscratch_blocks >= blocks) return SS_OK;
     if (xarena) { SS_HIP(hipFree(xarena)); xarena = nullptr; }
     const size_t Mp = plan.Mp;
@@ -544,13 +623,11 @@ int Slot::alloc_expl(hipStream_t st, int blocks) {
     std::vector<double> w(Mp);
     for (int p = 0; p < plan.NP; ++p) {
         for (size_t h = 0; h < Mp; ++h) w[h] = plan.weights[p * Mp + h];
-        SS_HIP(hipMemcpyAsync(desc.xreach + (size_t)p * plan.N * Mp, w.data(), sizeof(double) * Mp,
-                              hipMemcpyHostToDevice, st));
-        SS_HIP(hipStreamSynchronize(st));
+        SS_UP(desc.xreach + (size_t)p * plan.N * Mp, w.data(), sizeof(double) * Mp);
     }
     // refresh the device copy of the descriptor, keeping device-side progress
-    SS_HIP(hipMemcpyAsync((char *)d_desc + offsetof(SlotDesc, avg), &desc.avg,
-                          sizeof(SlotDesc) - offsetof(SlotDesc, avg), hipMemcpyHostToDevice, st));
+    SS_UP((char *)d_desc + offsetof(SlotDesc, avg), &desc.avg, sizeof(SlotDesc) - offsetof(SlotDesc, avg));
+#undef SS_UP
     return SS_OK;
 }
 
