@@ -480,6 +480,23 @@ def profile_served(a):
     return 0
 
 
+def profile_served_sweep(a, rates=(10.0, 25.0, 50.0, 75.0, 100.0, 150.0), n=200):
+    """Latency against load of the served form: the bench's Poisson stream
+    (70% river / 30% turn, 2-3 players, adaptive checks) at several arrival
+    rates, 200 requests each, with the plain-GPU request loop on the same
+    stream beside it. One JSON line with a record per rate."""
+    import ss_abi
+    rows = []
+    for i, rate in enumerate(rates):
+        r = served_bench(n, rate, 20261203 + 5000, 0.5, 25, 2000, 5, log_every=100, plain_fifo=(i == 0),
+                         flags=ss_abi.SS_FLAG_ADAPTIVE_CHECKS)
+        log(f"[sweep] {rate}/s: {r['solves_per_s']} solves/s, p50 {r['latency_ms']['p50']} ms, "
+            f"p99 {r['latency_ms']['p99']} ms")
+        rows.append(r)
+    print(json.dumps({"served_sweep": rows}), flush=True)
+    return 0
+
+
 def _profile_leaf(flop, reps=10):
     """`reps` calls of the turn (or flop) leaf-value producer on the 3-player
     full-range workload, then the two library calls alone on prepared
@@ -715,6 +732,7 @@ PROFILE_TARGETS = {
     "traffic": profile_traffic,
     "precision": profile_precision,
     "served": profile_served,
+    "served-sweep": profile_served_sweep,
     "leaf": lambda a: _profile_leaf(False),
     "leaf-flop": lambda a: _profile_leaf(True),
     "error-report": error_report,
