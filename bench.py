@@ -447,6 +447,10 @@ def profile_traffic(a, subgames=256, iters=20, copy_mib=256):
     assert lib.ssdbg_copy(copy_bytes, 1, ctypes.byref(ms)) == 0, ss_abi.last_error()
     probs = [ss_api.Problem.from_spot(s) for s in wl.batch_mix(subgames, max_players=3)]
     algo = sum(ss_api.plan_bytes(p)[0] for p in probs)
+    # the phased form with its automatic launch shapes, no trial iterations:
+    # the profiled dispatches are exactly the `iters` iterations
+    os.environ["SS_BATCH_MODE"] = "phased"
+    os.environ["SS_NO_AUTOTUNE"] = "1"
     b = ss_api.Batch(probs, iters, flags=ss_abi.SS_FLAG_NO_EXPL)
     reps, st = b.run()
     b.close()
