@@ -484,6 +484,24 @@ def profile_served(a):
     return 0
 
 
+def profile_graph(a, subgames=256, iters=25):
+    """The bench's mixed batch for `iters` iterations with every subgame live,
+    replayed from the captured graph (5 iterations per graph): a kernel trace
+    of the path the bench times, to compare with the direct launches
+    (SS_NO_GRAPH=1) of the other profiles."""
+    import ss_abi
+    import ss_api
+    import workloads as wl
+    probs = [ss_api.Problem.from_spot(s) for s in wl.batch_mix(subgames, max_players=3)]
+    os.environ["SS_BATCH_MODE"] = "phased"
+    os.environ["SS_NO_AUTOTUNE"] = "1"
+    b = ss_api.Batch(probs, iters, check_every=25, flags=ss_abi.SS_FLAG_NO_EXPL)
+    reps, st = b.run()
+    b.close()
+    print(json.dumps({"iterations": iters, "cfr_ms": st.cfr_ms, "ms_per_iteration": st.cfr_ms / iters}), flush=True)
+    return 0
+
+
 def profile_served_sweep(a, rates=(10.0, 25.0, 50.0, 75.0, 100.0, 150.0), n=200):
     """Latency against load of the served form: the bench's Poisson stream
     (70% river / 30% turn, 2-3 players, adaptive checks) at several arrival
@@ -737,6 +755,7 @@ PROFILE_TARGETS = {
     "precision": profile_precision,
     "served": profile_served,
     "served-sweep": profile_served_sweep,
+    "graph": profile_graph,
     "leaf": lambda a: _profile_leaf(False),
     "leaf-flop": lambda a: _profile_leaf(True),
     "error-report": error_report,
