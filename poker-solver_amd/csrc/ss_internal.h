@@ -110,6 +110,7 @@ struct PinnedStage {
     PinnedStage &operator=(const PinnedStage &) = delete;
     ~PinnedStage();
     int alloc(size_t n, void **out);   // n bytes of pinned memory (256-byte aligned)
+    void reset();                      // every chunk free again (its copies have completed)
     int upload(void *dst, const void *src, size_t n, hipStream_t st);
 };
 

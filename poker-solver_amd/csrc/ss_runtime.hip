@@ -418,14 +418,28 @@ PinnedStage::~PinnedStage() {
     for (auto &c : chunks) (void)hipHostFree(c.p);
 }
 
+// A thread's staging memory for single-subgame setups, kept between calls
+// (pinning pages costs more than the copies it serves); each call waits for
+// its copies before returning, so the next call can reuse it. Never freed:
+// released with the process.
+static PinnedStage &thread_stage() {
+    static thread_local PinnedStage *stage = new PinnedStage();
+    stage->reset();
+    return *stage;
+}
+
+void PinnedStage::reset() {
+    for (auto &c : chunks) c.used = 0;
+}
+
 int Slot::setup(const HostProblem &hp, hipStream_t st) {
-    PinnedStage stage;
+    PinnedStage &stage = thread_stage();
     int rc = setup_begin(hp, st, stage);
     if (rc) return rc;
     SS_HIP(hipStreamSynchronize(st));
     rc = setup_finish(st, stage);
     if (rc) return rc;
-    SS_HIP(hipStreamSynchronize(st));   // the stage's memory is released on return
+    SS_HIP(hipStreamSynchronize(st));   // the stage's memory is reused by the next call
     return SS_OK;
 }
 
@@ -591,10 +605,10 @@ int Slot::setup_finish(hipStream_t st, PinnedStage &stage) {
 }
 
 int Slot::alloc_expl(hipStream_t st, int blocks) {
-    PinnedStage stage;
+    PinnedStage &stage = thread_stage();
     int rc = alloc_expl(st, blocks, stage);
     if (rc) return rc;
-    SS_HIP(hipStreamSynchronize(st));   // the stage's memory is released on return
+    SS_HIP(hipStreamSynchronize(st));   // the stage's memory is reused by the next call
     return SS_OK;
 }
 

@@ -116,6 +116,8 @@ SMALL = [
     # (seed, players, hands per player, board cards, leaves)
     (0, 2, 12, 5, False), (1, 2, 9, 5, False), (2, 3, 6, 5, False), (3, 3, 7, 5, False),
     (4, 4, 4, 5, False), (5, 2, 10, 4, True), (6, 3, 5, 4, True),
+    # three to five opponents: fold and multiway showdown values from the odometer
+    (7, 5, 4, 5, False), (8, 6, 3, 5, False),
 ]
 
 
@@ -192,12 +194,12 @@ def test_phased_launch_shapes_equal_plain(shape, monkeypatch):
 
 
 def test_phased_batch_with_three_or_more_opponents_equals_plain(monkeypatch):
-    """4- and 5-player spots take the odometer terminals (k_term_odo) in the
+    """4-, 5- and 6-player spots take the odometer terminals (k_term_odo) in the
     phased form; results equal the plain form bit for bit and the state after
     one iteration matches the brute-force oracle."""
     monkeypatch.setenv("SS_BATCH_MODE", "phased")
     spots = [wl.small_spot(50, num_players=4, hands_per_player=5), wl.small_spot(51, num_players=5, hands_per_player=4),
-             wl.small_spot(52, num_players=3, hands_per_player=6)]
+             wl.small_spot(52, num_players=3, hands_per_player=6), wl.small_spot(53, num_players=6, hands_per_player=3)]
     probs = [ss_api.Problem.from_spot(s) for s in spots]
     b = ss_api.Batch(probs, 6, flags=1)
     reps, st = b.run()
@@ -298,6 +300,30 @@ def test_gpu_exploitability_matches_oracle_on_gpu_solution(npl):
     prob, oprob = _spot_problem(sp)
     res = ss_api.solve_plain(prob, 20)
     e_or, _ = ob.exploitability(oprob, res.avg_full(prob.tree.num_nodes), mode=ob.TERM_FAST)
+    assert abs(res.exploitability - e_or) <= 1e-6 * max(1.0, abs(e_or)), (res.exploitability, e_or)
+
+
+@pytest.mark.parametrize("npl,hp", [(4, 5), (5, 4), (6, 3)])
+def test_multiway_avg_and_exploitability_vs_brute_oracle(npl, hp):
+    """4-6 players (every fold node and the multiway showdowns have 3-5
+    opponents, evaluated by the odometer): the average strategy after 3
+    iterations within 1e-4 of the FP64 brute-force oracle on every info-set
+    row except those whose FP64 regrets sat exactly at regret matching's
+    discontinuity (all <= 0 with a 0; e.g. a hand tied with every live
+    opponent hand where pot / 2 equals its bet, whose closed-form value is
+    0 in exact arithmetic and +-1 ulp in FP32), and the GPU's FP64
+    exploitability of its solution equal to the oracle's evaluation of the
+    same strategy."""
+    sp = wl.small_spot(70 + npl, num_players=npl, hands_per_player=hp)
+    prob, oprob = _spot_problem(sp)
+    print(f"multiway: {npl} players, {hp} hands, {prob.tree.num_nodes} nodes", flush=True)
+    res = ss_api.solve_plain(prob, 3)
+    ref = ob.solve(oprob, 3, fp64=True, mode=ob.TERM_BRUTE)
+    avg = res.avg_full(prob.tree.num_nodes)
+    err = np.abs(avg - ref["avg"]).max(axis=1)          # [N, max_h]
+    ties = _tie_rows(prob, oprob, 3)
+    assert not ((err > 1e-4) & ~ties).any(), float(np.where(ties, 0, err).max())
+    e_or, _ = ob.exploitability(oprob, avg, mode=ob.TERM_BRUTE)
     assert abs(res.exploitability - e_or) <= 1e-6 * max(1.0, abs(e_or)), (res.exploitability, e_or)
 
 
