@@ -215,7 +215,7 @@ def load_library(path=None):
     if not os.path.exists(p):
         raise FileNotFoundError(
             f"street_solve library not found at {p}; build it with "
-            f"`python -c 'import __graft_entry__ as g; g.build()'`")
+            f"`python poker-solver_amd/build_lib.py`")
     lib = ctypes.CDLL(p)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

@@ -1,6 +1,6 @@
 """ctypes binding of the CPU oracle (oracle/liboracle.so). Test infrastructure:
-only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg use it,
-always as the checker."""
+only the tests, the smoke check and bench.py's CPU-baseline and error legs use
+it, always as the checker."""
 
 import ctypes
 import os
