@@ -789,8 +789,9 @@ def main():
     ap.add_argument("--fixed-checks", action="store_true",
                     help="plain and served forms: check every --check-every iterations (default: the adaptive "
                          "schedule, SS_FLAG_ADAPTIVE_CHECKS)")
-    ap.add_argument("--adaptive-batched", action="store_true",
-                    help="batched form: the adaptive check schedule (default: every --check-every iterations)")
+    ap.add_argument("--fixed-batched", action="store_true",
+                    help="batched form: check every subgame every --check-every iterations (default: the sparse "
+                         "schedule, SS_FLAG_SPARSE_CHECKS)")
     ap.add_argument("--max-iters", type=int, default=2000)
     ap.add_argument("--plain-sample", type=int, default=8)
     ap.add_argument("--cpu-cores", type=int, default=0, help="0 = min(16, host cores)")
@@ -827,10 +828,11 @@ def main():
     problems = [ss_api.Problem.from_spot(s) for s in spots]
     nodes = [p.tree.num_nodes for p in problems]
     players = [p.num_players for p in problems]
-    # check schedules: the batched form checks every subgame every check_every
-    # iterations (one check round serves the whole batch); the plain and served
-    # forms, whose checks are per subgame, use the adaptive schedule
-    bflags = ss_abi.SS_FLAG_ADAPTIVE_CHECKS if a.adaptive_batched else 0
+    # check schedules: the batched form keeps its check rounds on multiples of
+    # check_every (one round serves the whole batch) and skips, per subgame,
+    # the rounds before its predicted crossing; the plain and served forms,
+    # whose checks are per subgame, use the fine adaptive schedule
+    bflags = 0 if a.fixed_batched else ss_abi.SS_FLAG_SPARSE_CHECKS
     flags = 0 if a.fixed_checks else ss_abi.SS_FLAG_ADAPTIVE_CHECKS
     batch = ss_api.Batch(problems, a.max_iters, target=a.target, check_every=a.check_every, flags=bflags)
     log(f"[bench] uploaded; nodes mean {np.mean(nodes):.0f} max {max(nodes)}; players {np.bincount(players)[2:].tolist()}")
@@ -917,7 +919,10 @@ def main():
             "target_exploitability_pct_pot": a.target,
             "check_every": a.check_every,
             "check_schedule": {
-                "batched": "adaptive" if a.adaptive_batched else "every check_every iterations",
+                "batched": "every check_every iterations" if a.fixed_batched else
+                           "sparse (SS_FLAG_SPARSE_CHECKS): rounds on multiples of check_every; a subgame skips "
+                           "the rounds before the iteration where its exploitability would reach the target if it "
+                           "fell as t^-2, at most 3 in a row",
                 "plain_and_served": "every check_every iterations" if a.fixed_checks else
                                     "adaptive (SS_FLAG_ADAPTIVE_CHECKS): first at check_every, then where each "
                                     "subgame's exploitability would reach the target if it fell as t^-2, on the "

@@ -146,6 +146,10 @@ typedef struct {
                                      iteration step, at least one step and at most
                                      4 * check_every later); without it checks are every
                                      check_every iterations */
+#define SS_FLAG_SPARSE_CHECKS 4   /* checks stay on multiples of check_every, but a subgame
+                                     skips those before the iteration where its exploitability
+                                     would reach the target at t^-2 decay (at most 3 in a row);
+                                     with SS_FLAG_ADAPTIVE_CHECKS this one wins */
 
 typedef struct {
     int iterations;              /* iterations actually run */

@@ -51,9 +51,11 @@ static int solve_plain(const HostProblem &hp, const SSSolveConfig &cfg, SSOutput
     const int chunk = target ? std::max(1, cfg.check_every > 0 ? cfg.check_every : 50) : std::max(1, T);
     double kms = 0, expl = -1, gains[SS_MAX_PLAYERS] = {0, 0, 0, 0, 0, 0};
     int done = 0, converged = 0, expl_at = -1, due = chunk;
-    const bool adaptive = target && (cfg.flags & SS_FLAG_ADAPTIVE_CHECKS);
-    int astep = std::min(chunk, 10);   // the batched form's graph step: the largest divisor of chunk <= 10
-    while (chunk % astep) --astep;
+    int gstep = std::min(chunk, 10);   // the batched form's graph step: the largest divisor of chunk <= 10
+    while (chunk % gstep) --gstep;
+    bool adaptive = false;
+    const int astep = check_step(cfg.flags, chunk, gstep, &adaptive);
+    adaptive = adaptive && target;
     const double t_loop = now_ms();
     while (done < T) {
         int k = std::min(due - done, T - done);
@@ -191,6 +193,7 @@ struct BatchImpl {
             templ[i].iter = 0;
             templ[i].iter_end = T;
             templ[i].final_iter = T;
+            // root snapshots on every iteration a check can fall on (sparse checks stay on the chunk grid)
             templ[i].snap_every = target ? ((cfg.flags & SS_FLAG_ADAPTIVE_CHECKS) ? gsteps : chunk) : 0;
             templ[i].active = 1;
         }
@@ -474,7 +477,9 @@ struct BatchImpl {
         for (int i = 0; i < n; ++i) live_bytes += slots[i]->plan.algo_bytes_iter;
         // iteration of each subgame's next check: every chunk iterations, or
         // adaptive (each on a multiple of gsteps, so a replay count is whole)
-        const bool adaptive = target && (cfg.flags & SS_FLAG_ADAPTIVE_CHECKS);
+        bool adaptive = false;
+        const int cstep = check_step(cfg.flags, chunk, gsteps, &adaptive);
+        adaptive = adaptive && target;
         std::vector<int> due(n, chunk);
         while (done < T) {
             int next = done + chunk;
@@ -526,7 +531,7 @@ struct BatchImpl {
                         SS_HIP(hipMemcpyAsync(&d_slots[i].active, &kZero, sizeof(int), hipMemcpyHostToDevice, st));
                     } else {
                         if (want[i])
-                            due[i] = next_check_iter(done, expl[i], cfg.target_exploitability, chunk, gsteps, adaptive);
+                            due[i] = next_check_iter(done, expl[i], cfg.target_exploitability, chunk, cstep, adaptive);
                         ++nlive;
                     }
                 }

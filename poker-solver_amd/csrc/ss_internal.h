@@ -161,6 +161,10 @@ int plain_run(Slot &s, int iters, int final_iter, hipStream_t st, double *kernel
 // the iteration where e would reach the target if it fell as t^-2, rounded
 // down to a multiple of step, at least t + step and at most t + 4 * every.
 int next_check_iter(int t, double e, double target, int every, int step, bool adaptive);
+// The check grid of a solve: `step` for SS_FLAG_ADAPTIVE_CHECKS (the form's
+// iteration step), check_every for SS_FLAG_SPARSE_CHECKS; *adaptive tells
+// whether either is set.
+int check_step(int flags, int every, int fine_step, bool *adaptive);
 int plain_exploitability(Slot &s, hipStream_t st, double *expl_pct, double *gains);
 int read_output(Slot &s, const SlotDesc &desc_now, hipStream_t st, SSOutput *out, int legacy_root_ev);
 

@@ -671,6 +671,11 @@ int next_check_iter(int t, double e, double target, int every, int step, bool ad
     return std::max(t + step, std::min(t + 4 * every, next));
 }
 
+int check_step(int flags, int every, int fine_step, bool *adaptive) {
+    *adaptive = (flags & (SS_FLAG_ADAPTIVE_CHECKS | SS_FLAG_SPARSE_CHECKS)) != 0;
+    return (flags & SS_FLAG_SPARSE_CHECKS) ? std::max(1, every) : std::max(1, fine_step);
+}
+
 int read_output(Slot &s, const SlotDesc &dnow, hipStream_t st, SSOutput *out, int) {
     const Plan &pl = s.plan;
     const size_t Mp = pl.Mp;

@@ -232,7 +232,7 @@ int SSServer::admit_locked(std::unique_lock<std::mutex> &lk) {
             d.iter = 0;
             d.iter_end = std::max(0, r->cfg.max_iterations);
             d.final_iter = d.iter_end;
-            d.snap_every = (r->cfg.flags & SS_FLAG_ADAPTIVE_CHECKS) ? cfg.iters_per_step : r->cfg.check_every;
+            d.snap_every = cfg.iters_per_step;   // every step end: any check or retirement finds its root snapshot
             d.active = 1;
             if (hipMemcpyAsync(&d_slots[free_slot], &d, sizeof(d), hipMemcpyHostToDevice, st) != hipSuccess ||
                 hipStreamSynchronize(st) != hipSuccess)
@@ -333,9 +333,11 @@ int SSServer::check_and_retire() {
             r->expl = expl_from_xout(xo, r->dev->plan.NP, r->dev->plan.starting_pot, r->gains);
             r->joint_done = true;
             r->expl_at = r->iters;
-            if (r->cfg.flags & SS_FLAG_ADAPTIVE_CHECKS)
+            bool adaptive = false;
+            const int cstep = check_step(r->cfg.flags, std::max(1, r->cfg.check_every), cfg.iters_per_step, &adaptive);
+            if (adaptive)
                 r->next_check = next_check_iter(r->iters, r->expl, r->cfg.target_exploitability,
-                                                std::max(1, r->cfg.check_every), std::max(1, cfg.iters_per_step), true);
+                                                std::max(1, r->cfg.check_every), cstep, true);
             else
                 while (r->next_check <= r->iters && r->cfg.check_every > 0) r->next_check += r->cfg.check_every;
         }

@@ -26,6 +26,7 @@ SS_PREC_BF16X2 = 1
 
 SS_FLAG_NO_EXPL = 1
 SS_FLAG_ADAPTIVE_CHECKS = 2
+SS_FLAG_SPARSE_CHECKS = 4
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG_DIR, "build", "street_solve.so")

@@ -121,15 +121,19 @@ def test_served_cancel_queued():
         srv.close()
 
 
-def test_adaptive_checks_same_retirement_in_every_form():
+@pytest.mark.parametrize("sparse", [False, True])
+def test_adaptive_checks_same_retirement_in_every_form(sparse):
     """SS_FLAG_ADAPTIVE_CHECKS: the first check at check_every, later checks
     where each subgame's exploitability would reach the target at t^-2 decay,
     on the 5-iteration step. The schedule depends only on the subgame's own
     exploitability, so the plain, batched and served forms retire each subgame
     at the same iteration with the same results bit for bit; each retires at or
-    below the target, on the step."""
+    below the target, on the step. SS_FLAG_SPARSE_CHECKS keeps the checks on
+    multiples of check_every (25) and skips those before the predicted
+    crossing."""
     import ss_abi
-    flags = ss_abi.SS_FLAG_ADAPTIVE_CHECKS
+    flags = ss_abi.SS_FLAG_SPARSE_CHECKS if sparse else ss_abi.SS_FLAG_ADAPTIVE_CHECKS
+    step = 25 if sparse else 5
     spots = [wl.mix_river(wl.BASE_SEED + 4000 + i, max_players=3) for i in range(6)]
     probs = [ss_api.Problem.from_spot(s) for s in spots]
     b = ss_api.Batch(probs, 2000, target=0.5, check_every=25, flags=flags)
@@ -147,7 +151,7 @@ def test_adaptive_checks_same_retirement_in_every_form():
         print(f"subgame {i}: {p.num_players} players, adaptive {plain.iterations} / fixed {fixed.iterations} "
               f"iterations, exploitability {plain.exploitability:.4f}% pot", flush=True)
         assert plain.converged and plain.exploitability <= 0.5
-        assert plain.iterations % 5 == 0 and plain.iterations >= 25
+        assert plain.iterations % step == 0 and plain.iterations >= 25
         for g in (batched[i], served[i]):
             assert g.iterations == plain.iterations, i
             assert g.exploitability == pytest.approx(plain.exploitability, rel=1e-12), i
