@@ -98,6 +98,32 @@ static int solve_plain(const HostProblem &hp, const SSSolveConfig &cfg, SSOutput
     return SS_OK;
 }
 
+// The make-up of a batch, for reusing an autotuned choice: batches drawn from
+// one distribution have the same subgame and player counts and task counts
+// within a few percent, so a cached choice matches when those agree and
+// both task counts are within 25%.
+struct Makeup {
+    int n = 0, ndec = 0, nterm = 0, npmax = 0, precision = 0;
+    bool near(const Makeup &o) const {
+        auto close = [](int a, int b) { return 4 * std::abs(a - b) <= std::max(a, b); };
+        return n == o.n && npmax == o.npmax && precision == o.precision && close(ndec, o.ndec) &&
+               close(nterm, o.nterm);
+    }
+};
+template <typename V> struct MakeupCache {
+    std::vector<std::pair<Makeup, V>> entries;
+    const V *find(const Makeup &m) const {
+        for (auto &e : entries)
+            if (e.first.near(m)) return &e.second;
+        return nullptr;
+    }
+    void put(const Makeup &m, const V &v) {
+        for (auto &e : entries)
+            if (e.first.near(m)) { e.second = v; return; }
+        entries.emplace_back(m, v);
+    }
+};
+
 // Batched form: all subgames advance together, one persistent launch per
 // iteration; `gsteps` iterations are captured in a hipGraph and replayed
 // chunk / gsteps times between checks.
@@ -106,6 +132,7 @@ struct BatchImpl {
     int n = 0, chunk = 1;
     bool target = false, want_final = false;
     std::vector<std::unique_ptr<Slot>> slots;
+    std::unique_ptr<DevicePool> pool;   // the slots' device memory, in a few large allocations
     std::vector<SlotDesc> templ;
     SlotDesc *d_slots = nullptr;
     int *d_want = nullptr;
@@ -133,6 +160,7 @@ struct BatchImpl {
         for (auto &e : ev)
             if (e) (void)hipEventDestroy(e);
         slots.clear();
+        pool.reset();   // every slot's device memory
         if (d_slots) (void)hipFree(d_slots);
         if (d_want) (void)hipFree(d_want);
         if (scratch) (void)hipFree(scratch);
@@ -162,8 +190,10 @@ struct BatchImpl {
             // every subgame's plan, arena and uploads queued, one wait for the
             // combo ranks, then the river tables and the FP64 buffers queued
             PinnedStage stage;
+            pool.reset(new DevicePool());
             for (int i = 0; i < n; ++i) {
                 slots[i].reset(new Slot());
+                slots[i]->pool = pool.get();
                 int rc = slots[i]->setup_begin(hps[i], st, stage);
                 if (rc) return set_error(rc, "subgame %d: %s", i, last_error_str());
             }
@@ -283,10 +313,14 @@ struct BatchImpl {
     // The make-up of the batch: subgames, players, and the decision and
     // terminal task counts in quarter-octave buckets (batches drawn from one
     // distribution share a key).
-    std::array<int64_t, 5> makeup_key() const {
-        const int ndec = pt.lvl_off.empty() ? 0 : pt.lvl_off[pt.L];
-        auto bucket = [](int x) { return (int64_t)std::lround(4.0 * std::log2(1.0 + x)); };
-        return {n, bucket(ndec), bucket(pt.nterm), pt.NPmax, (int64_t)cfg.precision};
+    Makeup makeup() const {
+        Makeup m;
+        m.n = n;
+        m.ndec = pt.lvl_off.empty() ? 0 : pt.lvl_off[pt.L];
+        m.nterm = pt.nterm;
+        m.npmax = pt.NPmax;
+        m.precision = cfg.precision;
+        return m;
     }
 
     // Autotune the batched strategy on this batch: time two iterations of each
@@ -302,13 +336,12 @@ struct BatchImpl {
         }
         if (cfg.max_iterations < 3) { mode = 1; return SS_OK; }
         static std::mutex mu;
-        static std::map<std::array<int64_t, 5>, int> chosen;
-        const auto key = makeup_key();
+        static MakeupCache<int> chosen;
+        const Makeup key = makeup();
         {
             std::lock_guard<std::mutex> lk(mu);
-            auto it = chosen.find(key);
-            if (it != chosen.end()) {
-                mode = it->second;
+            if (const int *hit = chosen.find(key)) {
+                mode = *hit;
                 int rc = mode == 1 ? tune_shape() : SS_OK;
                 return rc ? rc : reset_state();
             }
@@ -333,7 +366,7 @@ struct BatchImpl {
         }
         {
             std::lock_guard<std::mutex> lk(mu);
-            chosen[key] = mode;
+            chosen.put(key, mode);
         }
         if (mode == 1) {
             int rc = tune_shape();
@@ -358,14 +391,13 @@ struct BatchImpl {
         const double t0 = now_ms();
         std::vector<Slot *> raw;
         for (auto &sl : slots) raw.push_back(sl.get());
-        const auto key = makeup_key();
+        const Makeup key = makeup();
         static std::mutex mu;
-        static std::map<std::array<int64_t, 5>, PhasedTasks::Shape> cache;
+        static MakeupCache<PhasedTasks::Shape> cache;
         {
             std::lock_guard<std::mutex> lk(mu);
-            auto it = cache.find(key);
-            if (it != cache.end()) {
-                pt.shape = it->second;
+            if (const PhasedTasks::Shape *hit = cache.find(key)) {
+                pt.shape = *hit;
                 int rc = build_phased_tasks(raw, pt, st);
                 tune_ms = now_ms() - t0;
                 return rc;
@@ -413,7 +445,7 @@ struct BatchImpl {
         rc = build_phased_tasks(raw, pt, st);
         {
             std::lock_guard<std::mutex> lk(mu);
-            cache[key] = best_shape;
+            cache.put(key, best_shape);
         }
         tune_ms = now_ms() - t0;
         return rc;

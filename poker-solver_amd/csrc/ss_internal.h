@@ -114,6 +114,23 @@ struct PinnedStage {
     int upload(void *dst, const void *src, size_t n, hipStream_t st);
 };
 
+// Device memory for many slots from a few large allocations: a batch's
+// subgames are created and destroyed together, and a hipMalloc / hipFree
+// per slot costs tens of microseconds (hipFree also waits for the device).
+struct DevicePool {
+    struct Chunk {
+        char *p = nullptr;
+        size_t cap = 0, used = 0;
+    };
+    std::vector<Chunk> chunks;
+    size_t chunk_bytes = (size_t)256 << 20;
+    DevicePool() = default;
+    DevicePool(const DevicePool &) = delete;
+    DevicePool &operator=(const DevicePool &) = delete;
+    ~DevicePool();
+    int alloc(size_t n, void **out);   // n bytes of device memory (256-byte aligned)
+};
+
 // A subgame resident on the device.
 struct Slot {
     std::unique_ptr<PrecState> prec;   // precision form only
@@ -127,6 +144,7 @@ struct Slot {
     double *xscratch = nullptr;   // per-workgroup FP64 terminal scratch
     int xscratch_blocks = 0;
     double plan_ms = 0;           // host time of make_plan in setup
+    DevicePool *pool = nullptr;   // when set, the device buffers come from it (and are not freed here)
     uint32_t *h_cstr = nullptr;   // combo ranks read back by setup_begin (pinned, in the stage)
     size_t off_cstr = 0, off_gsort = 0, off_clist = 0, off_clen = 0, off_bnd = 0, off_cpos = 0, off_hmeta = 0;
 

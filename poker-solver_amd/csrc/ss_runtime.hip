@@ -363,9 +363,38 @@ struct ArenaLayout {
 int launch_combo_ranks(const int *d_board, uint32_t *d_out, hipStream_t st);
 
 Slot::~Slot() {
+    if (pool) return;   // the pool's memory
     if (arena) (void)hipFree(arena);
     if (xarena) (void)hipFree(xarena);
     if (d_desc) (void)hipFree(d_desc);
+}
+
+int DevicePool::alloc(size_t n, void **out) {
+    n = (n + 255) & ~(size_t)255;
+    for (auto &c : chunks)
+        if (c.cap - c.used >= n) {
+            *out = c.p + c.used;
+            c.used += n;
+            return SS_OK;
+        }
+    Chunk c;
+    c.cap = std::max(n, chunk_bytes);
+    SS_HIP(hipMalloc((void **)&c.p, c.cap));
+    c.used = n;
+    *out = c.p;
+    chunks.push_back(c);
+    return SS_OK;
+}
+
+DevicePool::~DevicePool() {
+    for (auto &c : chunks) (void)hipFree(c.p);
+}
+
+// device memory of a slot: from its pool when it has one
+static int slot_alloc(DevicePool *pool, size_t n, void **out) {
+    if (pool) return pool->alloc(n, out);
+    SS_HIP(hipMalloc(out, n));
+    return SS_OK;
 }
 
 int64_t estimate_footprint(const HostProblem &hp) {
@@ -479,8 +508,9 @@ int Slot::setup_begin(const HostProblem &hp, hipStream_t st, PinnedStage &stage)
     size_t o_hmeta = pl.is_river ? L.take(sizeof(uint4) * pl.NP * Mp) : 0;
     size_t o_board = L.take(sizeof(int) * 8);
     arena_bytes = L.off;
-    SS_HIP(hipMalloc(&arena, arena_bytes));
-    SS_HIP(hipMalloc(&d_desc, sizeof(SlotDesc)));
+    rc = slot_alloc(pool, arena_bytes, (void **)&arena);
+    if (!rc) rc = slot_alloc(pool, sizeof(SlotDesc), (void **)&d_desc);
+    if (rc) return rc;
     char *a = arena;
     SS_HIP(hipMemsetAsync(arena, 0, arena_bytes, st));
 #define SS_UP(dst, src, n)                                   \
@@ -614,7 +644,8 @@ int Slot::alloc_expl(hipStream_t st, int blocks) {
 
 int Slot::alloc_expl(hipStream_t st, int blocks, PinnedStage &stage) {
     if (xarena && xscratch_blocks >= blocks) return SS_OK;
-    if (xarena) { SS_HIP(hipFree(xarena)); xarena = nullptr; }
+    if (xarena && !pool) SS_HIP(hipFree(xarena));   // pool memory stays until the pool goes
+    xarena = nullptr;
     const size_t Mp = plan.Mp;
     ArenaLayout L;
     size_t o_avg = L.take(sizeof(float) * (size_t)plan.A * Mp);
@@ -624,7 +655,10 @@ int Slot::alloc_expl(hipStream_t st, int blocks, PinnedStage &stage) {
     size_t o_xout = L.take(sizeof(double) * 32);
     size_t o_scr = L.take(sizeof(double) * (size_t)blocks * SS_XSCRATCH_DOUBLES);
     xarena_bytes = L.off;
-    SS_HIP(hipMalloc(&xarena, xarena_bytes));
+    {
+        int rc = slot_alloc(pool, xarena_bytes, (void **)&xarena);
+        if (rc) return rc;
+    }
     SS_HIP(hipMemsetAsync(xarena, 0, xarena_bytes, st));
     desc.avg = (float *)(xarena + o_avg);
     desc.xreach = (double *)(xarena + o_xr);

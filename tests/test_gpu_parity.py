@@ -513,8 +513,8 @@ def test_autotuned_launch_shape_equals_plain_and_is_cached():
     keeps the fastest; a second batch of the same make-up reuses the choice
     without tuning. Every shape gives the plain form's results bit for bit."""
     spots = [wl.river_2p(seed=90 + i) for i in range(3)] + [wl.river_3p(seed=93)]
-    spots += [wl.small_spot(94 + i, num_players=2 + i % 2, hands_per_player=11) for i in range(4)]
-    probs = [ss_api.Problem.from_spot(s) for s in spots]
+    spots += [wl.small_spot(94 + i, num_players=2 + i % 2, hands_per_player=11) for i in range(6)]
+    probs = [ss_api.Problem.from_spot(s) for s in spots]   # 10 subgames: no other test's batch size
     tuned = []
     for rep in range(2):
         b = ss_api.Batch(probs, 9, flags=1)
