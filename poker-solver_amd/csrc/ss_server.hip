@@ -226,6 +226,7 @@ int SSServer::admit_locked(std::unique_lock<std::mutex> &lk) {
         lk.unlock();
         r->dev.reset(new Slot());
         int rc = r->dev->setup(r->prob.hp, st);
+        if (!rc && r->cfg.precision == SS_PREC_BF16X2) rc = prec_setup(*r->dev, st);   // 2-player rivers only
         if (!rc) rc = r->dev->alloc_expl(st, 1);
         if (!rc) {
             SlotDesc d = r->dev->desc;
@@ -438,11 +439,14 @@ int64_t ss_server_submit(SSServer *s, const SSProblem *problem, const SSSolveCon
     int rc = r->prob.copy_from(problem);
     if (rc) return rc;
     r->cfg = *config;
-    if (r->cfg.precision != SS_PREC_FP32)
-        return set_error(SS_ERR_UNSUPPORTED, "the served form runs FP32 (precision form: ss_solve_gpu_ex)");
+    if (r->cfg.precision != SS_PREC_FP32 && r->cfg.precision != SS_PREC_BF16X2)
+        return set_error(SS_ERR_ARG, "unknown precision %d", r->cfg.precision);
     if (r->cfg.check_every <= 0) r->cfg.check_every = s->cfg.check_every;
     r->footprint = estimate_footprint(r->prob.hp);
     if (r->footprint < 0) return set_error(SS_ERR_ARG, "%s", last_error_str());
+    // the precision form's sign masks, packed reach and card-removal masses:
+    // under 4 MB even for 1326 combos and 128 showdown nodes
+    if (r->cfg.precision == SS_PREC_BF16X2) r->footprint += (int64_t)8 << 20;
     if (r->footprint > s->cfg.hbm_budget_bytes)
         return set_error(SS_ERR_NOMEM, "request needs %lld bytes of device state, budget is %lld",
                          (long long)r->footprint, (long long)s->cfg.hbm_budget_bytes);

@@ -240,9 +240,10 @@ class Server:
             raise SolveError(f"ss_server_create failed: {ss_abi.last_error()}")
         self.problems = {}
 
-    def submit(self, problem, iterations, target=0.005 * 100, check_every=0, flags=0):
+    def submit(self, problem, iterations, target=0.005 * 100, check_every=0, flags=0,
+               precision=ss_abi.SS_PREC_FP32):
         st = problem.struct()
-        cfg = config(iterations, target, check_every, flags=flags)
+        cfg = config(iterations, target, check_every, precision=precision, flags=flags)
         t = self.lib.ss_server_submit(self.h, ctypes.byref(st), ctypes.byref(cfg))
         _check(t, "ss_server_submit")
         self.problems[t] = problem

@@ -156,3 +156,33 @@ def test_adaptive_checks_same_retirement_in_every_form(sparse):
             assert g.iterations == plain.iterations, i
             assert g.exploitability == pytest.approx(plain.exploitability, rel=1e-12), i
             _same(g, plain)
+
+
+def test_served_precision_form_equals_plain():
+    """BF16 hi/lo precision requests (2-player rivers) served beside FP32
+    requests: each retires at the plain precision form's iteration with its
+    results bit for bit, the FP32 ones as their plain form; a 3-player
+    request asking for the precision form is refused at admission."""
+    import ss_abi
+    bf = ss_abi.SS_PREC_BF16X2
+    prec = [ss_api.Problem.from_spot(wl.river_2p(seed=300 + i)) for i in range(3)]
+    fp32 = [ss_api.Problem.from_spot(wl.mix_river(wl.BASE_SEED + 4000 + i, max_players=3)) for i in range(3)]
+    three = ss_api.Problem.from_spot(wl.river_3p(seed=301))
+    srv = ss_api.Server(max_slots=8, iters_per_step=5, check_every=25)
+    try:
+        tp = [srv.submit(p, 2000, target=0.5, check_every=25, precision=bf) for p in prec]
+        tf = [srv.submit(p, 2000, target=0.5, check_every=25) for p in fp32]
+        got = _wait(srv, tp + tf)
+        bad = srv.submit(three, 100, target=0.5, check_every=25, precision=bf)
+        with pytest.raises(ss_api.SolveError):
+            _wait(srv, [bad])
+    finally:
+        srv.close()
+    for i, p in enumerate(prec + fp32):
+        want = ss_api.solve_plain(p, 2000, target=0.5, check_every=25, precision=bf if i < 3 else ss_abi.SS_PREC_FP32)
+        g = got[i]
+        print(f"request {i}: {'bf16x2' if i < 3 else 'fp32'}, {g.iterations} iterations, "
+              f"exploitability {g.exploitability:.4f}% pot", flush=True)
+        assert g.iterations == want.iterations and g.converged
+        assert g.exploitability == pytest.approx(want.exploitability, rel=1e-12)
+        _same(g, want)
