@@ -414,6 +414,47 @@ def flop_bench(hands=(80, 200), iters=200, reps=3, cpu_iters=1):
     return rec
 
 
+def reference_latency_bench(reps=3):
+    """The realtime re-solve call at the setting of the reference's published
+    latencies (ARCHITECTURE.md:73, 116-117: RTX 3060, 200 hands, 200
+    iterations, ~67 ms river / ~236 ms flop): the drop-in StreetSolverGPU
+    (legacy ABI, ss_solve_gpu) on a 2-player spot, each player's range the
+    200 heaviest of U[0,1]-weighted combos, pot 10 BB, stacks 50 BB, sizes
+    0.33 / 0.75 / 1.5 pot (the class's default); turn and flop streets with
+    the 4 continuation strategies per player at the depth limit and zero leaf
+    values (the wrapper's fallback). Wall time of solve(200), median of
+    `reps` calls; tree build and range installation timed apart."""
+    import warnings
+    import street_solver_gpu as sg
+    import workloads as wl
+    rec = {"setting": "2 players, 200 hands each, 200 iterations, pot 10 BB, stacks 50 BB, sizes 0.33/0.75/1.5 pot, "
+                      "legacy ABI (ss_solve_gpu) through the drop-in StreetSolverGPU",
+           "published_rtx3060_ms": {"river": 67, "flop": 236}, "streets": {}}
+    for street, nb, seed in (("river", 5, 11), ("turn", 4, 12), ("flop", 3, 13)):
+        r = wl.rng_for(20261203 + 9000 + seed)
+        board = [int(c) for c in r.choice(52, nb, replace=False)]
+        combos = wl.all_combos(board)
+        ranges = [[(int(a), int(b), float(w)) for (a, b), w in zip(combos, r.uniform(0, 1, len(combos)))]
+                  for _ in range(2)]
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore")
+            t0 = time.perf_counter()
+            solver = sg.StreetSolverGPU(board=board, pot_bb=10.0, stack_bb=50.0, player_ranges=ranges)
+            build_ms = (time.perf_counter() - t0) * 1000
+            solver.solve(iterations=200)
+            walls = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                solver.solve(iterations=200)
+                walls.append((time.perf_counter() - t0) * 1000)
+        nodes = solver._tree.num_nodes
+        solver.free()
+        rec["streets"][street] = {"nodes": nodes, "solve_ms": round(float(np.median(walls)), 2),
+                                  "build_ms": round(build_ms, 2)}
+        log(f"[bench] reference-setting re-solve, {street}: {nodes} nodes, {np.median(walls):.1f} ms per solve(200)")
+    return rec
+
+
 def traffic_per_launch(algo_per_launch, workload):
     """HBM bytes per CFR iteration from the committed counter profile
     (profiles/cfr_iteration_traffic.md: calibrated FETCH_SIZE + WRITE_SIZE of a
@@ -756,6 +797,7 @@ PROFILE_TARGETS = {
     "served": profile_served,
     "served-sweep": profile_served_sweep,
     "graph": profile_graph,
+    "reference-latency": lambda a: print(json.dumps(reference_latency_bench()), flush=True) or 0,
     "leaf": lambda a: _profile_leaf(False),
     "leaf-flop": lambda a: _profile_leaf(True),
     "error-report": error_report,
@@ -990,6 +1032,8 @@ def main():
     if not a.no_flop:
         log("[bench] flop-through-river solves")
         line["flop_solve"] = flop_bench()
+        log("[bench] realtime re-solve at the reference's published setting")
+        line["reference_setting_latency"] = reference_latency_bench()
 
     if not a.no_cpu:
         cores = a.cpu_cores or min(16, os.cpu_count() or 1)
