@@ -28,10 +28,11 @@
 // stored. The terminals are (hand x opponent hand) sums in the reference's
 // order: at setup every distinct showdown board gets a table of
 // 1 + sign(rank difference) codes per (traverser hand, opponent hand), one
-// byte each, 0 / 1 card-disjointness codes serve the folds, and a terminal
-// workgroup stages its table in LDS once for many nodes, so a hand pair costs
-// a byte-to-float conversion and two fused multiply-adds whose zero terms
-// leave the sum unchanged: the results are the reference loop's bit for bit.
+// byte each, 0 / 1 card-disjointness codes serve the folds, and each lane of
+// a terminal workgroup keeps its hand's column of the table in registers for
+// up to 32 nodes, so a hand pair costs a fraction of a byte-to-float
+// conversion and one fused multiply-add whose zero terms leave the sum
+// unchanged: the results are the reference loop's bit for bit.
 #include "ss_internal.h"
 #include "ss_eval.h"
 #include "../../include/flop_solve.h"
@@ -332,22 +333,23 @@ __device__ __forceinline__ void fs_match_regs(const float *r, int na, float *sg)
 }
 
 // The opponent's reach down one level (fs_batch_propagate_reach,
-// flop_solve.cu:693-735): one wavefront per node, S segments of 64 hands per
+// flop_solve.cu:693-735): G wavefronts per node, S segments of 64 hands per
 // lane (their loads in flight together); the opponent's own decisions scale
 // by its current strategy, regret-matched here.
-template <int S>
+template <int S, int G>
 __global__ void __launch_bounds__(SS_BLOCK) k_fs_down(FsDev d, const int *__restrict__ recs, int nrec, int opp,
                                                       const float *__restrict__ R, float *__restrict__ reach,
                                                       float *__restrict__ root_sigma) {
     const int lane = threadIdx.x & 63, nh = d.nh[opp], Mp = d.Mp;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    for (int i = blockIdx.x * 4 + w; i < nrec; i += gridDim.x * 4) {
-        const int *rc = recs + (size_t)i * FS_REC;
+    for (int i = blockIdx.x * 4 + w; i < nrec * G; i += gridDim.x * 4) {
+        const int ni = G == 1 ? i : i / G, hb = G == 1 ? 0 : (i - ni * G) * S * 64;
+        const int *rc = recs + (size_t)ni * FS_REC;
         const int n = rc[0], meta = rc[1], na = meta >> 8;
         float pr[S];
 #pragma unroll
         for (int s = 0; s < S; ++s)
-            if (s * 64 + lane < nh) pr[s] = reach[(size_t)n * Mp + s * 64 + lane];
+            if (hb + s * 64 + lane < nh) pr[s] = reach[(size_t)n * Mp + hb + s * 64 + lane];
         if ((meta & 15) == FS_NODE_DECISION) {
             if (((meta >> 4) & 15) - 1 == opp) {
                 float r[S][FS_MAX_ACTIONS];
@@ -355,10 +357,10 @@ __global__ void __launch_bounds__(SS_BLOCK) k_fs_down(FsDev d, const int *__rest
                 for (int s = 0; s < S; ++s)
 #pragma unroll
                     for (int a = 0; a < FS_MAX_ACTIONS; ++a)
-                        if (a < na && s * 64 + lane < nh) r[s][a] = R[(size_t)(rc[2] + a) * Mp + s * 64 + lane];
+                        if (a < na && hb + s * 64 + lane < nh) r[s][a] = R[(size_t)(rc[2] + a) * Mp + hb + s * 64 + lane];
 #pragma unroll
                 for (int s = 0; s < S; ++s) {
-                    const int h = s * 64 + lane;
+                    const int h = hb + s * 64 + lane;
                     if (h >= nh) continue;
                     float sg[FS_MAX_ACTIONS];
                     fs_match_regs(r[s], na, sg);
@@ -374,7 +376,7 @@ __global__ void __launch_bounds__(SS_BLOCK) k_fs_down(FsDev d, const int *__rest
                 for (int s = 0; s < S; ++s)
 #pragma unroll
                     for (int a = 0; a < FS_MAX_ACTIONS; ++a)
-                        if (a < na && s * 64 + lane < nh) reach[(size_t)rc[3 + a] * Mp + s * 64 + lane] = pr[s];
+                        if (a < na && hb + s * 64 + lane < nh) reach[(size_t)rc[3 + a] * Mp + hb + s * 64 + lane] = pr[s];
             }
         } else {
             const int *kid = d.children + rc[2];
@@ -382,7 +384,7 @@ __global__ void __launch_bounds__(SS_BLOCK) k_fs_down(FsDev d, const int *__rest
                 const size_t row = (size_t)kid[a] * Mp;
 #pragma unroll
                 for (int s = 0; s < S; ++s)
-                    if (s * 64 + lane < nh) reach[row + s * 64 + lane] = pr[s];
+                    if (hb + s * 64 + lane < nh) reach[row + hb + s * 64 + lane] = pr[s];
             }
         }
     }
@@ -393,7 +395,7 @@ __global__ void __launch_bounds__(SS_BLOCK) k_fs_down(FsDev d, const int *__rest
 // those of the pass's start), the value, the regret and strategy-sum updates;
 // chance nodes average their children (their loads eight children at a time,
 // the sum in child order).
-template <int S>
+template <int S, int G>
 __global__ void __launch_bounds__(SS_BLOCK) k_fs_up(FsDev d, const int *__restrict__ recs, int nrec, int trav,
                                                     float *__restrict__ R, float *__restrict__ SS,
                                                     float *__restrict__ cfv, float *__restrict__ root_sigma) {
@@ -401,8 +403,9 @@ __global__ void __launch_bounds__(SS_BLOCK) k_fs_up(FsDev d, const int *__restri
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int t = *d.iter;
     const float disc = (float)t / ((float)t + 1.0f), tw = (float)t;
-    for (int i = blockIdx.x * 4 + w; i < nrec; i += gridDim.x * 4) {
-        const int *rc = recs + (size_t)i * FS_REC;
+    for (int i = blockIdx.x * 4 + w; i < nrec * G; i += gridDim.x * 4) {
+        const int ni = G == 1 ? i : i / G, hb = G == 1 ? 0 : (i - ni * G) * S * 64;
+        const int *rc = recs + (size_t)ni * FS_REC;
         const int n = rc[0], meta = rc[1], na = meta >> 8;
         if ((meta & 15) == FS_NODE_DECISION) {
             const bool mine = ((meta >> 4) & 15) - 1 == trav;
@@ -411,16 +414,16 @@ __global__ void __launch_bounds__(SS_BLOCK) k_fs_up(FsDev d, const int *__restri
             for (int s = 0; s < S; ++s)
 #pragma unroll
                 for (int a = 0; a < FS_MAX_ACTIONS; ++a)
-                    if (a < na && s * 64 + lane < nh) {
-                        cv[s][a] = cfv[(size_t)rc[3 + a] * Mp + s * 64 + lane];
+                    if (a < na && hb + s * 64 + lane < nh) {
+                        cv[s][a] = cfv[(size_t)rc[3 + a] * Mp + hb + s * 64 + lane];
                         if (mine) {
-                            r[s][a] = R[(size_t)(rc[2] + a) * Mp + s * 64 + lane];
-                            ss[s][a] = SS[(size_t)(rc[2] + a) * Mp + s * 64 + lane];
+                            r[s][a] = R[(size_t)(rc[2] + a) * Mp + hb + s * 64 + lane];
+                            ss[s][a] = SS[(size_t)(rc[2] + a) * Mp + hb + s * 64 + lane];
                         }
                     }
 #pragma unroll
             for (int s = 0; s < S; ++s) {
-                const int h = s * 64 + lane;
+                const int h = hb + s * 64 + lane;
                 if (h >= nh) continue;
                 float val = 0.f;
                 if (mine) {
@@ -457,7 +460,7 @@ __global__ void __launch_bounds__(SS_BLOCK) k_fs_up(FsDev d, const int *__restri
                         const size_t row = (size_t)kid[a0 + k] * Mp;
 #pragma unroll
                         for (int s = 0; s < S; ++s)
-                            if (s * 64 + lane < nh) v[k][s] = cfv[row + s * 64 + lane];
+                            if (hb + s * 64 + lane < nh) v[k][s] = cfv[row + hb + s * 64 + lane];
                     }
 #pragma unroll
                 for (int k = 0; k < 8; ++k)
@@ -467,7 +470,7 @@ __global__ void __launch_bounds__(SS_BLOCK) k_fs_up(FsDev d, const int *__restri
             }
 #pragma unroll
             for (int s = 0; s < S; ++s)
-                if (s * 64 + lane < nh) cfv[(size_t)n * Mp + s * 64 + lane] = acc[s] / (float)na;
+                if (hb + s * 64 + lane < nh) cfv[(size_t)n * Mp + hb + s * 64 + lane] = acc[s] / (float)na;
         }
     }
 }
@@ -497,37 +500,42 @@ __device__ __forceinline__ void fs_dpp_fma16(float (&acc)[4], const float (&rv)[
 }
 #undef FS_DPP_STEP
 
-// Terminal values, fold (FOLD) or showdown, for one traverser. A task is the
-// set of terminal nodes on one board (tasks[j] = table, first group, groups),
-// in groups of four nodes (groups[2 g] = the nodes, a short group repeating
-// its last; groups[2 g + 1] = {nodes in the group, pot}; the four nodes of a
-// showdown group have one pot). The workgroup stages the board's codes for
-// this traverser ([no16][Mp] uint4, 16 opponent hands per lane word) in LDS,
-// each lane is one traverser hand, and per opponent hand o, in the
-// reference's order:
+// Terminal values, fold (FOLD) or showdown, for one traverser. The terminal
+// nodes of a board are in groups of four (groups[2 g] = the nodes, a short
+// group repeating its last; groups[2 g + 1] = {nodes in the group, pot}; the
+// four nodes of a showdown group have one pot), and a task is at most eight
+// groups of one board (tasks[j] = table, first group, groups), so the tasks
+// outnumber the resident workgroups and balance by themselves. A workgroup
+// is Mp lanes, one traverser hand each; a lane loads its column of the
+// board's codes for this traverser ([no16][Mp] uint4, 16 opponent hands per
+// word) once per task and keeps it in registers for every group of the task.
+// Per opponent hand o, in the reference's order:
 //   fold      acc = fma(r_o, code, acc), code 1 / 0: flop_solve.cu:737-779
 //             adds r_o or skips it; the value is acc times the payoff
 //   showdown  acc = fma(r_o, c pot/2, acc), c = code - 1 = +1 / -1 / 0:
 //             +-r_o pot/2 or nothing, flop_solve.cu:781-833
 // The product r_o * code (or r_o * c pot/2) is exact, and a zero term leaves
 // acc unchanged, so the sums are the reference's bit for bit.
+constexpr int FS_NO16_MAX = (FS_MAX_HANDS + 15) / 16;
+constexpr int FS_TASK_GROUPS = 8;
+// Four wavefronts per SIMD (128 VGPRs): the register column and the
+// unrolled opponent loop use what the cap leaves; 3 or 5 measured slower.
 template <bool FOLD>
-__global__ void __launch_bounds__(SS_BLOCK) k_fs_term(FsDev d, const int4 *__restrict__ tasks, int ntasks,
-                                                      const int4 *__restrict__ groups, const uint4 *__restrict__ tables,
-                                                      const float *__restrict__ reach, float *__restrict__ cfv,
-                                                      int trav) {
-    extern __shared__ uint4 s_tab[];
+__global__ void __launch_bounds__(SS_BLOCK, 4) k_fs_term(FsDev d, const int4 *__restrict__ tasks, int ntasks,
+                                                        const int4 *__restrict__ groups,
+                                                        const uint4 *__restrict__ tables,
+                                                        const float *__restrict__ reach, float *__restrict__ cfv,
+                                                        int trav) {
     const int n_trav = d.nh[trav], Mp = d.Mp, no16 = d.no16;
-    const int h = threadIdx.x, ht = min(h, Mp - 1), l16 = threadIdx.x & 15;
+    const int h = threadIdx.x, l16 = threadIdx.x & 15;   // blockDim.x == Mp
     const float half_start = d.starting_pot * 0.5f;
     for (int j = blockIdx.x; j < ntasks; j += gridDim.x) {
         const int4 tk = tasks[j];
-        __syncthreads();   // the previous task's readers are done
-        const uint4 *src = tables + ((size_t)tk.x * 2 + trav) * no16 * Mp;
-        for (int i = threadIdx.x; i < no16 * Mp; i += blockDim.x) s_tab[i] = src[i];
-        __syncthreads();
-        // every lane computes (the broadcasts need all 64 active); lanes
-        // past the hand count read a valid column and store nothing
+        const uint4 *src = tables + ((size_t)tk.x * 2 + trav) * no16 * Mp + h;
+        uint4 q[FS_NO16_MAX];
+#pragma unroll
+        for (int c = 0; c < FS_NO16_MAX; ++c)
+            if (c < no16) q[c] = src[(size_t)c * Mp];
         for (int g = tk.y; g < tk.y + tk.z; ++g) {
             const int4 gn = groups[2 * g], gm = groups[2 * g + 1];
             const int nn[4] = {gn.x, gn.y, gn.z, gn.w};
@@ -547,23 +555,23 @@ __global__ void __launch_bounds__(SS_BLOCK) k_fs_term(FsDev d, const int4 *__res
             float acc[4] = {0.f, 0.f, 0.f, 0.f}, rv[4];
 #pragma unroll
             for (int m = 0; m < 4; ++m) rv[m] = reach[(size_t)nn[m] * Mp + l16];
-            for (int c = 0; c < no16; ++c) {
-                // the next step's opponent values in flight during this step
-                const int cn = min(c + 1, no16 - 1);
-                float rn[4], w[16];
 #pragma unroll
-                for (int m = 0; m < 4; ++m) rn[m] = reach[(size_t)nn[m] * Mp + 16 * cn + l16];
-                const uint4 q = s_tab[c * Mp + ht];
-                const uint32_t qw[4] = {q.x, q.y, q.z, q.w};
+            for (int c = 0; c < FS_NO16_MAX; ++c)
+                if (c < no16) {
+                    const int cn = min(c + 1, no16 - 1);
+                    float rn[4], w[16];
 #pragma unroll
-                for (int k = 0; k < 16; ++k) {
-                    const float code = (float)((qw[k >> 2] >> (8 * (k & 3))) & 255u);
-                    w[k] = FOLD ? code : fmaf(code, hp, -hp);
+                    for (int m = 0; m < 4; ++m) rn[m] = reach[(size_t)nn[m] * Mp + 16 * cn + l16];
+                    const uint32_t qw[4] = {q[c].x, q[c].y, q[c].z, q[c].w};
+#pragma unroll
+                    for (int k = 0; k < 16; ++k) {
+                        const float code = (float)((qw[k >> 2] >> (8 * (k & 3))) & 255u);
+                        w[k] = FOLD ? code : fmaf(code, hp, -hp);
+                    }
+                    fs_dpp_fma16(acc, rv, w);
+#pragma unroll
+                    for (int m = 0; m < 4; ++m) rv[m] = rn[m];
                 }
-                fs_dpp_fma16(acc, rv, w);
-#pragma unroll
-                for (int m = 0; m < 4; ++m) rv[m] = rn[m];
-            }
             if (h < n_trav)
 #pragma unroll
                 for (int m = 0; m < 4; ++m)
@@ -594,6 +602,14 @@ struct FsRun {
         int4 *d_tasks = nullptr, *d_groups = nullptr;
         uint4 *d_tab = nullptr;
         // by_pot: a group's four nodes share a pot (showdowns)
+        // tasks of at most `most` groups each, the largest first
+        void split(int most) {
+            std::vector<int4> out;
+            for (const int4 &t : tasks)
+                for (int o = 0; o < t.z; o += most) out.push_back(make_int4(t.x, t.y + o, std::min(most, t.z - o), 0));
+            std::stable_sort(out.begin(), out.end(), [](const int4 &a, const int4 &b) { return a.z > b.z; });
+            tasks.swap(out);
+        }
         void group(const std::vector<int> &nodes, const std::vector<uint32_t> &board, const std::vector<int4> &node,
                    bool by_pot) {
             std::vector<std::tuple<uint32_t, int, int>> keyed;   // board, pot (or 0), node
@@ -620,8 +636,6 @@ struct FsRun {
     } sdset, foldset;
     std::vector<std::vector<int>> lvl;   // decision and chance nodes per depth
     std::vector<int *> d_lvl;
-    int cus = 256;
-    size_t tab_bytes = 0;                // LDS of one staged table
     std::vector<int> abase;
     hipStream_t st = nullptr;
     hipGraph_t graph = nullptr;
@@ -651,9 +665,6 @@ struct FsRun {
         const int maxh = std::max(td->num_hands[0], td->num_hands[1]);
         const int Mp = (maxh + 63) / 64 * 64;
         SS_HIP(hipStreamCreate(&st));
-        int dev = 0;
-        SS_HIP(hipGetDevice(&dev));
-        SS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
         // host tables
         std::vector<int4> node(N);
         std::vector<int2> bets(N);
@@ -721,6 +732,8 @@ struct FsRun {
         // share it)
         sdset.group(sd, board, node, true);
         foldset.group(fold, board, node, false);
+        sdset.split(FS_TASK_GROUPS);
+        foldset.split(FS_TASK_GROUPS);
         std::vector<uint32_t> hands(2 * Mp, 0xFFFFu);
         std::vector<float> w(2 * Mp, 0.f);
         for (int p = 0; p < 2; ++p)
@@ -738,7 +751,6 @@ struct FsRun {
         d.nh[1] = td->num_hands[1];
         d.no16 = (maxh + 15) / 16;
         d.starting_pot = td->nodes[0].pot;
-        tab_bytes = sizeof(uint4) * (size_t)d.no16 * Mp;
         int4 *p_node;
         int2 *p_bets;
         int *p_kids, *p_abase, *p_iter;
@@ -805,52 +817,54 @@ struct FsRun {
     static unsigned grid_for(size_t work) {
         return (unsigned)std::max<size_t>(1, std::min<size_t>(8192, (work + SS_BLOCK - 1) / SS_BLOCK));
     }
-    // level sweeps: one node per wavefront, four per workgroup, every hand
-    // segment of the node in the same lane
-    static dim3 sweep_grid(size_t nodes) {
-        return dim3((unsigned)std::max<size_t>(1, std::min<size_t>(16384, (nodes + 3) / 4)));
+    // level sweeps: G wavefronts per node (four wavefronts per workgroup),
+    // S segments of 64 hands in each lane. The reach sweep keeps a node's
+    // segments in one lane (S = segments, G = 1); the value sweep, whose
+    // traverser rows need three times the registers, gives each segment its
+    // own wavefront (S = 1, G = segments) for the occupancy.
+    static dim3 sweep_grid(size_t tasks) {
+        return dim3((unsigned)std::max<size_t>(1, std::min<size_t>(16384, (tasks + 3) / 4)));
     }
-    template <int S> void down_s(size_t l, int opp) {
-        hipLaunchKernelGGL(k_fs_down<S>, sweep_grid(lvl[l].size()), dim3(SS_BLOCK), 0, st, d, d_lvl[l],
+    template <int S, int G> void down_s(size_t l, int opp) {
+        hipLaunchKernelGGL((k_fs_down<S, G>), sweep_grid(lvl[l].size() * G), dim3(SS_BLOCK), 0, st, d, d_lvl[l],
                            (int)lvl[l].size(), opp, d.R, d.reach, d.root_sigma);
     }
-    template <int S> void up_s(size_t l, int trav) {
-        hipLaunchKernelGGL(k_fs_up<S>, sweep_grid(lvl[l].size()), dim3(SS_BLOCK), 0, st, d, d_lvl[l],
+    template <int S, int G> void up_s(size_t l, int trav) {
+        hipLaunchKernelGGL((k_fs_up<S, G>), sweep_grid(lvl[l].size() * G), dim3(SS_BLOCK), 0, st, d, d_lvl[l],
                            (int)lvl[l].size(), trav, d.R, d.SS, d.cfv, d.root_sigma);
     }
     void launch_down(size_t l, int opp) {
         switch ((d.nh[opp] + 63) / 64) {
-        case 1: down_s<1>(l, opp); break;
-        case 2: down_s<2>(l, opp); break;
-        case 3: down_s<3>(l, opp); break;
-        default: down_s<4>(l, opp); break;
+        case 1: down_s<1, 1>(l, opp); break;
+        case 2: down_s<2, 1>(l, opp); break;
+        case 3: down_s<3, 1>(l, opp); break;
+        default: down_s<4, 1>(l, opp); break;
         }
     }
     void launch_up(size_t l, int trav) {
         switch ((d.nh[trav] + 63) / 64) {
-        case 1: up_s<1>(l, trav); break;
-        case 2: up_s<2>(l, trav); break;
-        case 3: up_s<3>(l, trav); break;
-        default: up_s<4>(l, trav); break;
+        case 1: up_s<1, 1>(l, trav); break;
+        case 2: up_s<1, 2>(l, trav); break;
+        case 3: up_s<1, 3>(l, trav); break;
+        default: up_s<1, 4>(l, trav); break;
         }
+    }
+    // terminal values: one workgroup per task, Mp lanes
+    template <bool FOLD> void launch_term(const TermSet &ts, int trav) {
+        if (!ts.tasks.empty())
+            hipLaunchKernelGGL(k_fs_term<FOLD>, dim3((unsigned)ts.tasks.size()), dim3(d.Mp), 0, st, d, ts.d_tasks,
+                               (int)ts.tasks.size(), ts.d_groups, ts.d_tab, d.reach, d.cfv, trav);
     }
 
     // one iteration: traverser 0's pass, then traverser 1's, then t + 1
     void launch_iteration() {
-        const unsigned term_grid = (unsigned)std::max(1, 3 * cus);   // three staged tables fit a CU's LDS
         for (int trav = 0; trav < 2; ++trav) {
             const int opp = 1 - trav;
             hipLaunchKernelGGL(k_fs_root, dim3(1), dim3(SS_BLOCK), 0, st, d, opp);
             for (size_t l = 0; l < lvl.size(); ++l)
                 if (!lvl[l].empty()) launch_down(l, opp);
-            if (!foldset.tasks.empty())
-                hipLaunchKernelGGL(k_fs_term<true>, dim3((unsigned)std::min<size_t>(foldset.tasks.size(), term_grid)),
-                                   dim3(SS_BLOCK), tab_bytes, st, d, foldset.d_tasks, (int)foldset.tasks.size(),
-                                   foldset.d_groups, foldset.d_tab, d.reach, d.cfv, trav);
-            if (!sdset.tasks.empty())
-                hipLaunchKernelGGL(k_fs_term<false>, dim3((unsigned)std::min<size_t>(sdset.tasks.size(), term_grid)),
-                                   dim3(SS_BLOCK), tab_bytes, st, d, sdset.d_tasks, (int)sdset.tasks.size(),
-                                   sdset.d_groups, sdset.d_tab, d.reach, d.cfv, trav);
+            launch_term<true>(foldset, trav);
+            launch_term<false>(sdset, trav);
             for (size_t l = lvl.size(); l-- > 0;)
                 if (!lvl[l].empty()) launch_up(l, trav);
         }
