@@ -6,9 +6,11 @@
 #include <cstring>
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <map>
 #include <memory>
 #include <mutex>
+#include <thread>
 #include <vector>
 
 #include "../../include/street_solve_debug.h"
@@ -168,6 +170,23 @@ struct BatchImpl {
         if (st) (void)hipStreamDestroy(st);
     }
 
+    // every subgame's river tables built on up to eight host threads (a
+    // failure is left for setup_finish, which builds them again and reports)
+    void prepare_all() {
+        std::atomic<int> next{0};
+        auto work = [&] {
+            for (int i; (i = next.fetch_add(1)) < n;) (void)slots[i]->prepare_tables();
+        };
+        const int want = std::min(8, std::max(1, (int)std::thread::hardware_concurrency()));
+        std::vector<std::thread> pool_threads;
+        try {
+            for (int t = 1; t < want && t < n; ++t) pool_threads.emplace_back(work);
+        } catch (...) {   // no more threads: the calling thread takes the rest
+        }
+        work();
+        for (auto &t : pool_threads) t.join();
+    }
+
     int create(const HostProblem *hps, int count, const SSSolveConfig &c) {
         if (c.precision != SS_PREC_FP32 && c.precision != SS_PREC_BF16X2)
             return set_error(SS_ERR_ARG, "unknown precision %d", c.precision);
@@ -186,6 +205,7 @@ struct BatchImpl {
         for (auto &e : ev) SS_HIP(hipEventCreate(&e));
         slots.resize(n);
         const double c0 = now_ms();
+        double cb = c0, cw = c0, cp = c0, cf = c0;   // setup phases: queued, ranks back, tables built, queued
         {
             // every subgame's plan, arena and uploads queued, one wait for the
             // combo ranks, then the river tables and the FP64 buffers queued
@@ -197,7 +217,11 @@ struct BatchImpl {
                 int rc = slots[i]->setup_begin(hps[i], st, stage);
                 if (rc) return set_error(rc, "subgame %d: %s", i, last_error_str());
             }
+            cb = now_ms();
             SS_HIP(hipStreamSynchronize(st));
+            cw = now_ms();
+            prepare_all();
+            cp = now_ms();
             for (int i = 0; i < n; ++i) {
                 int rc = slots[i]->setup_finish(st, stage);
                 if (!rc && prec) rc = prec_setup(*slots[i], st);
@@ -207,6 +231,7 @@ struct BatchImpl {
                     if (rc) return rc;
                 }
             }
+            cf = now_ms();
             SS_HIP(hipStreamSynchronize(st));   // before the stage is released
         }
         if (target || want_final) {
@@ -249,8 +274,11 @@ struct BatchImpl {
         if (log_level() > 0) {
             double plan = 0;
             for (auto &sl : slots) plan += sl->plan_ms;
-            fprintf(stderr, "[street_solve] batch create: subgames %.1f ms (plans %.1f), task lists %.1f, mode and "
-                            "shape %.1f, capture %.1f ms\n", c1 - c0, plan, c2 - c1, c3 - c2, now_ms() - c3);
+            fprintf(stderr, "[street_solve] batch create: subgames %.1f ms (plans %.1f, uploads queued %.1f, rank "
+                            "wait %.1f, tables %.1f on host threads + %.1f queued, final wait %.1f), task lists %.1f, mode "
+                            "and shape %.1f, capture %.1f ms\n", c1 - c0, plan, cb - c0, cw - cb, cp - cw, cf - cp,
+                    c1 - cf, c2 - c1, c3 - c2,
+                    now_ms() - c3);
         }
         return rc;
     }

@@ -147,6 +147,10 @@ struct Slot {
     DevicePool *pool = nullptr;   // when set, the device buffers come from it (and are not freed here)
     uint32_t *h_cstr = nullptr;   // combo ranks read back by setup_begin (pinned, in the stage)
     size_t off_cstr = 0, off_gsort = 0, off_clist = 0, off_clen = 0, off_bnd = 0, off_cpos = 0, off_hmeta = 0;
+    // river tables and per-hand showdown records built on the host from
+    // h_cstr by prepare_tables, held until setup_finish uploads them
+    std::unique_ptr<RiverTables> h_rt;
+    std::vector<uint4> h_hmeta;
 
     ~Slot();
     // setup = setup_begin, a stream synchronisation, setup_finish; a batch
@@ -154,6 +158,9 @@ struct Slot {
     int setup(const HostProblem &hp, hipStream_t st);
     int setup_begin(const HostProblem &hp, hipStream_t st, PinnedStage &stage);
     int setup_finish(hipStream_t st, PinnedStage &stage);
+    // the host half of setup_finish (no HIP calls, touches only this slot):
+    // a batch runs it for many subgames on several host threads at once
+    int prepare_tables();
     int alloc_expl(hipStream_t st, int blocks);
     int alloc_expl(hipStream_t st, int blocks, PinnedStage &stage);
     int64_t footprint() const { return (int64_t)arena_bytes + (int64_t)xarena_bytes; }

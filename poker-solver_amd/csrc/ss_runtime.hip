@@ -7,7 +7,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <initializer_list>
+#include <new>
 #include <numeric>
+#include <utility>
 
 #include "ss_internal.h"
 
@@ -518,16 +521,37 @@ int Slot::setup_begin(const HostProblem &hp, hipStream_t st, PinnedStage &stage)
         int rc_ = stage.upload((dst), (src), (n), st);       \
         if (rc_) return rc_;                                 \
     } while (0)
-    SS_UP(a + o_nodes, pl.nodes.data(), sizeof(DevNode) * pl.N);
-    if (!pl.children.empty()) SS_UP(a + o_child, pl.children.data(), sizeof(int) * pl.children.size());
-    SS_UP(a + o_declvl, pl.dec_lvl.data(), sizeof(int) * pl.dec_lvl.size());
-    SS_UP(a + o_decoff, pl.dec_off.data(), sizeof(int) * pl.dec_off.size());
-    SS_UP(a + o_term, pl.term.data(), sizeof(int) * pl.term.size());
-    SS_UP(a + o_hinfo, pl.hinfo.data(), sizeof(uint32_t) * pl.NP * Mp);
-    SS_UP(a + o_w, pl.weights.data(), sizeof(float) * pl.NP * Mp);
+    // the tree and range sections (nodes .. weights) and the frozen actions
+    // with the reach-row table (frozen .. rrow) are contiguous stretches of
+    // the arena: one staged image each (padding and absent parts zero)
+    {
+        auto image = [&](size_t lo, size_t hi, std::initializer_list<std::pair<size_t, std::pair<const void *, size_t>>>
+                                                    parts) -> int {
+            void *pin = nullptr;
+            int rc2 = stage.alloc(hi - lo, &pin);
+            if (rc2) return rc2;
+            char *img = (char *)pin;
+            std::memset(img, 0, hi - lo);
+            for (const auto &pt : parts)
+                if (pt.second.second) std::memcpy(img + (pt.first - lo), pt.second.first, pt.second.second);
+            SS_HIP(hipMemcpyAsync(a + lo, img, hi - lo, hipMemcpyHostToDevice, st));
+            return SS_OK;
+        };
+        rc = image(o_nodes, o_leaf,
+                   {{o_nodes, {pl.nodes.data(), sizeof(DevNode) * pl.N}},
+                    {o_child, {pl.children.data(), sizeof(int) * pl.children.size()}},
+                    {o_declvl, {pl.dec_lvl.data(), sizeof(int) * pl.dec_lvl.size()}},
+                    {o_decoff, {pl.dec_off.data(), sizeof(int) * pl.dec_off.size()}},
+                    {o_term, {pl.term.data(), sizeof(int) * pl.term.size()}},
+                    {o_hinfo, {pl.hinfo.data(), sizeof(uint32_t) * pl.NP * Mp}},
+                    {o_w, {pl.weights.data(), sizeof(float) * pl.NP * Mp}}});
+        if (!rc)
+            rc = image(o_frozen, o_R,
+                       {{o_frozen, {pl.frozen_src, pl.frozen_src ? sizeof(int) * pl.N : 0}},
+                        {o_rrow, {pl.rrow.data(), sizeof(int) * pl.rrow.size()}}});
+        if (rc) return rc;
+    }
     if (pl.has_leaf) SS_UP(a + o_leaf, pl.leaf_src, sizeof(float) * pl.leaf_count);
-    if (pl.frozen_src) SS_UP(a + o_frozen, pl.frozen_src, sizeof(int) * pl.N);
-    SS_UP(a + o_rrow, pl.rrow.data(), sizeof(int) * pl.rrow.size());
     // root reach = raw weights (never overwritten: the forward sweep writes children only)
     for (int p = 0; p < pl.NP; ++p)
         SS_UP(a + o_reach + sizeof(float) * (size_t)p * pl.N * Mp, pl.weights.data() + (size_t)p * Mp,
@@ -586,29 +610,24 @@ int Slot::setup_begin(const HostProblem &hp, hipStream_t st, PinnedStage &stage)
     return SS_OK;
 }
 
-// The river tables from the ranks setup_begin read back (the stream has been
-// synchronised since), uploaded through the stage, then the descriptor.
-int Slot::setup_finish(hipStream_t st, PinnedStage &stage) {
-    Plan &pl = plan;
-    const size_t Mp = pl.Mp;
-    SlotDesc &d = desc;
-    char *a = arena;
-    if (h_cstr) {
-        RiverTables rt;
-        build_river_tables(h_cstr, pl.lab, rt);
-        h_cstr = nullptr;
-        d.ng = rt.ng;
-        SS_UP(a + off_gsort, rt.gsort.data(), sizeof(uint16_t) * rt.gsort.size());
-        SS_UP(a + off_cpos, rt.cpos.data(), 52 * 52);
-        // hands of every player by ascending strength (dead hands last), one
-        // record per hand with everything a showdown sweep needs
-        std::vector<uint4> hm((size_t)pl.NP * Mp);
+// The host half of setup_finish: the river tables from the ranks setup_begin
+// read back (the stream has been synchronised since) and every player's
+// hands by ascending strength (dead hands last), one record per hand with
+// everything a showdown sweep needs.
+int Slot::prepare_tables() {
+    if (!h_cstr || h_rt) return SS_OK;
+    try {
+        const Plan &pl = plan;
+        const size_t Mp = pl.Mp;
+        std::unique_ptr<RiverTables> rt(new RiverTables);
+        build_river_tables(h_cstr, pl.lab, *rt);
+        std::vector<uint4> hm((size_t)pl.NP * Mp, make_uint4(0u, 0u, 0u, 0u));
         for (int p = 0; p < pl.NP; ++p) {
             std::vector<int> idx(pl.nh[p]);
             std::iota(idx.begin(), idx.end(), 0);
             auto rank = [&](int h) -> uint32_t {
                 uint32_t hi = pl.hinfo[(size_t)p * Mp + h];
-                return (hi & SS_LIVE_BIT) ? rt.cstr[(hi >> 12) & 2047] : 0u;
+                return (hi & SS_LIVE_BIT) ? rt->cstr[(hi >> 12) & 2047] : 0u;
             };
             auto key = [&](int h) -> uint32_t { uint32_t r = rank(h); return r ? r : 0xFFFFFFFFu; };
             std::stable_sort(idx.begin(), idx.end(), [&](int a2, int b2) { return key(a2) < key(b2); });
@@ -619,16 +638,47 @@ int Slot::setup_finish(hipStream_t st, PinnedStage &stage) {
                 const uint32_t ranked = rank(h) ? 1u : 0u;
                 uint4 r;
                 r.x = (uint32_t)h | ((hi & 63) << 11) | (((hi >> 6) & 63) << 17) | (ranked << 23);
-                r.y = ranked ? rt.kcls[kk] : 0u;
-                r.z = ranked ? (uint32_t)rt.glt[kk] : 0u;
-                r.w = ranked ? (uint32_t)rt.gle[kk] : 0u;
+                r.y = ranked ? rt->kcls[kk] : 0u;
+                r.z = ranked ? (uint32_t)rt->glt[kk] : 0u;
+                r.w = ranked ? (uint32_t)rt->gle[kk] : 0u;
                 hm[(size_t)p * Mp + i] = r;
             }
         }
-        SS_UP(a + off_hmeta, hm.data(), sizeof(uint4) * hm.size());
-        SS_UP(a + off_clist, rt.clist.data(), 52 * 52);
-        SS_UP(a + off_clen, rt.clen.data(), 52);
-        SS_UP(a + off_bnd, rt.bnd.data(), SS_BND_BYTES);
+        h_rt = std::move(rt);
+        h_hmeta.swap(hm);
+    } catch (const std::bad_alloc &) {
+        return set_error(SS_ERR_NOMEM, "out of host memory building the river tables");
+    }
+    return SS_OK;
+}
+
+// The river tables (prepare_tables, unless already run) uploaded as one image
+// of their contiguous stretch of the arena, then the descriptor.
+int Slot::setup_finish(hipStream_t st, PinnedStage &stage) {
+    SlotDesc &d = desc;
+    char *a = arena;
+    if (h_cstr) {
+        int rc = prepare_tables();
+        if (rc) return rc;
+        const RiverTables &rt = *h_rt;
+        d.ng = rt.ng;
+        // gsort, clist, clen, bnd, cpos, hmeta follow each other in the arena
+        const size_t lo = off_gsort, hi = off_hmeta + sizeof(uint4) * h_hmeta.size();
+        void *pin = nullptr;
+        rc = stage.alloc(hi - lo, &pin);
+        if (rc) return rc;
+        char *img = (char *)pin;
+        std::memset(img, 0, hi - lo);
+        std::memcpy(img + (off_gsort - lo), rt.gsort.data(), sizeof(uint16_t) * rt.gsort.size());
+        std::memcpy(img + (off_clist - lo), rt.clist.data(), 52 * 52);
+        std::memcpy(img + (off_clen - lo), rt.clen.data(), 52);
+        std::memcpy(img + (off_bnd - lo), rt.bnd.data(), SS_BND_BYTES);
+        std::memcpy(img + (off_cpos - lo), rt.cpos.data(), 52 * 52);
+        std::memcpy(img + (off_hmeta - lo), h_hmeta.data(), sizeof(uint4) * h_hmeta.size());
+        SS_HIP(hipMemcpyAsync(a + lo, img, hi - lo, hipMemcpyHostToDevice, st));
+        h_cstr = nullptr;
+        h_rt.reset();
+        std::vector<uint4>().swap(h_hmeta);
     }
     SS_UP(d_desc, &d, sizeof(d));
     return SS_OK;
