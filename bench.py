@@ -500,6 +500,32 @@ def profile_traffic(a, subgames=256, iters=20, copy_mib=256):
     return 0
 
 
+def profile_plain(a, iters=20, copy_mib=256):
+    """For rocprofv3 runs over the plain form's level sweeps (--kernel-trace,
+    then --pmc FETCH_SIZE, then --pmc WRITE_SIZE): one plain device copy of a
+    known size (ssdbg_copy, kernel k_copy: the dispatch that calibrates the
+    byte counters), then the 3-player 1081-combo river of the plain
+    benchmark configuration (pot 150 BB, stacks 500 BB, sizes {0.33, 0.75},
+    40% of combos live, max 2 raises) for `iters` plain-form iterations with
+    no exploitability evaluation: per traverser pass one k_forward launch per
+    level, k_terminals, one k_backward launch per level."""
+    import ctypes
+    import ss_abi
+    import ss_api
+    import workloads as wl
+    lib = ss_abi.load_library()
+    lib.ssdbg_copy.restype = ctypes.c_int
+    lib.ssdbg_copy.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
+    ms = ctypes.c_float()
+    assert lib.ssdbg_copy(copy_mib << 20, 1, ctypes.byref(ms)) == 0, ss_abi.last_error()
+    p = ss_api.Problem.from_spot(wl.river_3p())
+    r = ss_api.solve_plain(p, iters, flags=ss_abi.SS_FLAG_NO_EXPL)
+    td = p.tree.td
+    print(json.dumps({"copy_bytes": copy_mib << 20, "iterations": iters, "nodes": p.tree.num_nodes,
+                      "kernel_ms": r.kernel_ms, "levels": int(td.max_depth) + 1}), flush=True)
+    return 0
+
+
 def profile_precision(a, boards=64, iters=50):
     """64 two-player 1081-combo rivers (pot 100 BB, stacks 400 BB, sizes
     {0.5, 1}) for 50 iterations in one batched precision-form solve (BF16
@@ -802,6 +828,7 @@ PROFILE_TARGETS = {
     "leaf-flop": lambda a: _profile_leaf(True),
     "error-report": error_report,
     "flop": lambda a: _profile_flop(),
+    "plain": profile_plain,
 }
 
 
