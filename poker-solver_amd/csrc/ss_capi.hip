@@ -6,11 +6,9 @@
 #include <cstring>
 #include <algorithm>
 #include <array>
-#include <atomic>
 #include <map>
 #include <memory>
 #include <mutex>
-#include <thread>
 #include <vector>
 
 #include "../../include/street_solve_debug.h"
@@ -170,21 +168,11 @@ struct BatchImpl {
         if (st) (void)hipStreamDestroy(st);
     }
 
-    // every subgame's river tables built on up to eight host threads (a
-    // failure is left for setup_finish, which builds them again and reports)
+    // every subgame's river tables built on host threads
     void prepare_all() {
-        std::atomic<int> next{0};
-        auto work = [&] {
-            for (int i; (i = next.fetch_add(1)) < n;) (void)slots[i]->prepare_tables();
-        };
-        const int want = std::min(8, std::max(1, (int)std::thread::hardware_concurrency()));
-        std::vector<std::thread> pool_threads;
-        try {
-            for (int t = 1; t < want && t < n; ++t) pool_threads.emplace_back(work);
-        } catch (...) {   // no more threads: the calling thread takes the rest
-        }
-        work();
-        for (auto &t : pool_threads) t.join();
+        std::vector<Slot *> raw;
+        for (auto &sl : slots) raw.push_back(sl.get());
+        prepare_tables_all(raw);
     }
 
     int create(const HostProblem *hps, int count, const SSSolveConfig &c) {

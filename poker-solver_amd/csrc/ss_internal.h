@@ -165,6 +165,9 @@ struct Slot {
     int alloc_expl(hipStream_t st, int blocks, PinnedStage &stage);
     int64_t footprint() const { return (int64_t)arena_bytes + (int64_t)xarena_bytes; }
 };
+// prepare_tables of every listed slot (null entries skipped) on up to eight
+// host threads; a failure is left for setup_finish, which repeats and reports
+void prepare_tables_all(const std::vector<Slot *> &slots);
 
 int64_t estimate_footprint(const HostProblem &hp);
 

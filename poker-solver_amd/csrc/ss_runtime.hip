@@ -1,6 +1,7 @@
 // ss_runtime.hip -- host runtime: problem intake, device plan, slot arenas,
 // the plain driver, exploitability and output extraction.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -10,6 +11,7 @@
 #include <initializer_list>
 #include <new>
 #include <numeric>
+#include <thread>
 #include <utility>
 
 #include "ss_internal.h"
@@ -650,6 +652,22 @@ int Slot::prepare_tables() {
         return set_error(SS_ERR_NOMEM, "out of host memory building the river tables");
     }
     return SS_OK;
+}
+
+void prepare_tables_all(const std::vector<Slot *> &slots) {
+    std::atomic<size_t> next{0};
+    auto work = [&] {
+        for (size_t i; (i = next.fetch_add(1)) < slots.size();)
+            if (slots[i]) (void)slots[i]->prepare_tables();
+    };
+    const int want = std::min(8, std::max(1, (int)std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    try {
+        for (int t = 1; t < want && (size_t)t < slots.size(); ++t) th.emplace_back(work);
+    } catch (...) {   // no more threads: the calling thread takes the rest
+    }
+    work();
+    for (auto &t : th) t.join();
 }
 
 // The river tables (prepare_tables, unless already run) uploaded as one image

@@ -114,9 +114,19 @@ struct Request {
 struct SSServer {
     SSServerConfig cfg{};
     std::mutex mu;
-    std::condition_variable cv_work, cv_done;
+    std::condition_variable cv_work, cv_done, cv_read;
     std::thread worker;
     bool stop = false;
+    // Retired requests' outputs are read back by a second host thread on its
+    // own stream, so the loop moves on as soon as a request retires: the
+    // slot index is free at once, the request's device state (and its share
+    // of the HBM budget) when its copy is done.
+    std::thread reader;
+    std::deque<std::shared_ptr<ss::Request>> read_jobs;
+    int n_reading = 0;        // retired requests whose outputs are not read back yet
+    bool reader_stop = false;
+    hipStream_t rs = nullptr;
+    ss::PinnedStage adm_stage;   // admission uploads (worker thread only)
     int64_t next_ticket = 1;
     std::map<int64_t, std::shared_ptr<ss::Request>> reqs;
     std::deque<int64_t> queue;
@@ -133,17 +143,23 @@ struct SSServer {
     bool no_graph = false;    // SS_NO_GRAPH set: always launch directly (profilers)
     std::vector<std::shared_ptr<ss::Request>> resident;   // by slot index
     int64_t used_bytes = 0;
+    double t_admit = 0, t_step = 0, t_check = 0, t_expl = 0, t_read = 0;   // worker wall time per phase (logged at destroy)
+    int n_steps = 0, n_admits = 0;
     int fatal = 0;
     std::string fatal_msg;
 
     int init();
     void run();
     int admit_locked(std::unique_lock<std::mutex> &lk);
+    void setup_group(const std::vector<std::shared_ptr<ss::Request>> &grp, const std::vector<int> &slot_of,
+                     std::vector<int> &rcs, std::vector<std::string> &msgs);
     int refresh_tasks();
     void drop_graph();
     int step();
     int check_and_retire();
+    void release_slot(const std::shared_ptr<ss::Request> &r);
     void finish(const std::shared_ptr<ss::Request> &r, int state);
+    void read_loop();
     ~SSServer();
 };
 
@@ -151,6 +167,7 @@ using namespace ss;
 
 int SSServer::init() {
     SS_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    SS_HIP(hipStreamCreateWithFlags(&rs, hipStreamNonBlocking));
     const int n = cfg.max_slots;
     SS_HIP(hipMalloc(&d_slots, sizeof(SlotDesc) * n));
     SS_HIP(hipMemset(d_slots, 0, sizeof(SlotDesc) * n));   // active = 0 everywhere
@@ -182,80 +199,148 @@ int SSServer::refresh_tasks() {
 }
 
 SSServer::~SSServer() {
+    if (log_level() > 0)
+        fprintf(stderr, "[street_solve] server: %d admissions %.1f ms, %d steps %.1f ms, checks and retirement %.1f ms "
+                        "(exploitability %.1f, output read-back %.1f)\n",
+                n_admits, t_admit, n_steps, t_step, t_check, t_expl, t_read);
     drop_graph();
     for (auto &r : resident) r.reset();
     if (d_slots) (void)hipFree(d_slots);
     if (d_want) (void)hipFree(d_want);
     if (scratch) (void)hipFree(scratch);
     if (st) (void)hipStreamDestroy(st);
+    if (rs) (void)hipStreamDestroy(rs);
+}
+
+// the request's slot index inactive on the device and free for admission
+void SSServer::release_slot(const std::shared_ptr<Request> &r) {
+    if (r->slot < 0) return;
+    int zero = 0;
+    (void)hipMemcpyAsync(&d_slots[r->slot].active, &zero, sizeof(int), hipMemcpyHostToDevice, st);
+    (void)hipStreamSynchronize(st);
+    resident[r->slot].reset();
+    r->slot = -1;
+    dirty = true;
 }
 
 void SSServer::finish(const std::shared_ptr<Request> &r, int state) {
     r->state = state;
     r->t_done = now_ms();
-    if (r->slot >= 0) {
-        int zero = 0;
-        (void)hipMemcpyAsync(&d_slots[r->slot].active, &zero, sizeof(int), hipMemcpyHostToDevice, st);
-        (void)hipStreamSynchronize(st);
-        resident[r->slot].reset();
-        r->slot = -1;
-        dirty = true;
-    }
+    release_slot(r);
     if (r->dev) {
         used_bytes -= r->footprint;
         r->dev.reset();
     }
 }
 
-// Admit queued requests while a slot is free and the footprint fits.
-int SSServer::admit_locked(std::unique_lock<std::mutex> &lk) {
-    while (!queue.empty()) {
-        auto it = reqs.find(queue.front());
-        if (it == reqs.end() || it->second->cancel) {
-            if (it != reqs.end()) it->second->state = RQ_CANCELLED;
-            queue.pop_front();
-            cv_done.notify_all();
-            continue;
-        }
-        std::shared_ptr<Request> r = it->second;
-        int free_slot = -1;
-        for (int i = 0; i < cfg.max_slots; ++i)
-            if (!resident[i]) { free_slot = i; break; }
-        if (free_slot < 0 || used_bytes + r->footprint > cfg.hbm_budget_bytes) break;
-        queue.pop_front();
-        lk.unlock();
-        r->dev.reset(new Slot());
-        int rc = r->dev->setup(r->prob.hp, st);
-        if (!rc && r->cfg.precision == SS_PREC_BF16X2) rc = prec_setup(*r->dev, st);   // 2-player rivers only
-        if (!rc) rc = r->dev->alloc_expl(st, 1);
+// Device state of a group of admitted requests, set up together: every
+// request's plan, arena and uploads queued, one wait for the combo ranks,
+// the river tables on host threads, the tables, FP64 buffers and slot
+// descriptors queued, one wait. A request that fails gets its error in rcs
+// / msgs without stopping the others.
+void SSServer::setup_group(const std::vector<std::shared_ptr<Request>> &grp, const std::vector<int> &slot_of,
+                           std::vector<int> &rcs, std::vector<std::string> &msgs) {
+    const size_t g = grp.size();
+    auto fail = [&](size_t k, int rc) {
+        rcs[k] = rc;
+        msgs[k] = last_error_str();
+    };
+    PinnedStage &stage = adm_stage;   // kept between groups: pinning pages costs more than the copies
+    stage.reset();
+    for (size_t k = 0; k < g; ++k) {
+        grp[k]->dev.reset(new Slot());
+        int rc = grp[k]->dev->setup_begin(grp[k]->prob.hp, st, stage);
+        if (rc) fail(k, rc);
+    }
+    if (hipStreamSynchronize(st) != hipSuccess) {
+        const int rc = set_error(SS_ERR_HIP, "admission: stream synchronisation failed");
+        for (size_t k = 0; k < g; ++k)
+            if (!rcs[k]) fail(k, rc);
+        return;
+    }
+    std::vector<Slot *> raw(g, nullptr);
+    for (size_t k = 0; k < g; ++k)
+        if (!rcs[k]) raw[k] = grp[k]->dev.get();
+    prepare_tables_all(raw);
+    for (size_t k = 0; k < g; ++k) {
+        if (rcs[k]) continue;
+        Request &r = *grp[k];
+        int rc = r.dev->setup_finish(st, stage);
+        if (!rc && r.cfg.precision == SS_PREC_BF16X2) rc = prec_setup(*r.dev, st);   // 2-player rivers only
+        if (!rc) rc = r.dev->alloc_expl(st, 1, stage);
         if (!rc) {
-            SlotDesc d = r->dev->desc;
+            SlotDesc d = r.dev->desc;
             d.iter = 0;
-            d.iter_end = std::max(0, r->cfg.max_iterations);
+            d.iter_end = std::max(0, r.cfg.max_iterations);
             d.final_iter = d.iter_end;
             d.snap_every = cfg.iters_per_step;   // every step end: any check or retirement finds its root snapshot
             d.active = 1;
-            if (hipMemcpyAsync(&d_slots[free_slot], &d, sizeof(d), hipMemcpyHostToDevice, st) != hipSuccess ||
-                hipStreamSynchronize(st) != hipSuccess)
-                rc = set_error(SS_ERR_HIP, "slot upload failed");
+            rc = stage.upload(&d_slots[slot_of[k]], &d, sizeof(d), st);
         }
+        if (rc) fail(k, rc);
+    }
+    if (hipStreamSynchronize(st) != hipSuccess) {   // before the stage is released
+        const int rc = set_error(SS_ERR_HIP, "admission: slot upload failed");
+        for (size_t k = 0; k < g; ++k)
+            if (!rcs[k]) fail(k, rc);
+    }
+}
+
+// Admit queued requests, in order, while a slot is free and the footprint
+// fits: all that fit now are set up as one group.
+int SSServer::admit_locked(std::unique_lock<std::mutex> &lk) {
+    for (;;) {
+        std::vector<std::shared_ptr<Request>> grp;
+        std::vector<int> slot_of;
+        std::vector<char> taken(cfg.max_slots, 0);
+        for (int i = 0; i < cfg.max_slots; ++i) taken[i] = resident[i] != nullptr;
+        while (!queue.empty()) {
+            auto it = reqs.find(queue.front());
+            if (it == reqs.end() || it->second->cancel) {
+                if (it != reqs.end()) it->second->state = RQ_CANCELLED;
+                queue.pop_front();
+                cv_done.notify_all();
+                continue;
+            }
+            std::shared_ptr<Request> r = it->second;
+            int free_slot = -1;
+            for (int i = 0; i < cfg.max_slots; ++i)
+                if (!taken[i]) { free_slot = i; break; }
+            if (free_slot < 0 || used_bytes + r->footprint > cfg.hbm_budget_bytes) break;
+            queue.pop_front();
+            taken[free_slot] = 1;
+            used_bytes += r->footprint;   // held while the group is set up
+            grp.push_back(r);
+            slot_of.push_back(free_slot);
+        }
+        if (grp.empty()) break;
+        std::vector<int> rcs(grp.size(), SS_OK);
+        std::vector<std::string> msgs(grp.size());
+        lk.unlock();
+        const double ta = now_ms();
+        setup_group(grp, slot_of, rcs, msgs);
+        t_admit += now_ms() - ta;
+        n_admits += (int)grp.size();
         lk.lock();
-        if (rc) {
-            r->err = rc;
-            r->errmsg = last_error_str();
-            r->dev.reset();
-            r->state = RQ_ERROR;
-            r->t_done = now_ms();
-            cv_done.notify_all();
-            continue;
+        for (size_t k = 0; k < grp.size(); ++k) {
+            std::shared_ptr<Request> &r = grp[k];
+            if (rcs[k]) {
+                r->err = rcs[k];
+                r->errmsg = msgs[k];
+                r->dev.reset();
+                used_bytes -= r->footprint;
+                r->state = RQ_ERROR;
+                r->t_done = now_ms();
+                cv_done.notify_all();
+                continue;
+            }
+            r->slot = slot_of[k];
+            r->state = RQ_RUNNING;
+            r->t_admit = now_ms();
+            r->next_check = r->cfg.check_every;
+            resident[slot_of[k]] = r;
+            dirty = true;
         }
-        r->slot = free_slot;
-        r->state = RQ_RUNNING;
-        r->t_admit = now_ms();
-        r->next_check = r->cfg.check_every;
-        resident[free_slot] = r;
-        used_bytes += r->footprint;
-        dirty = true;
     }
     return SS_OK;
 }
@@ -319,6 +404,7 @@ int SSServer::check_and_retire() {
         bool final_eval = at_cap && !(r->cfg.flags & SS_FLAG_NO_EXPL);
         if ((at_check || final_eval) && r->expl_at != r->iters) { want[i] = 1; any = 1; }
     }
+    const double tx = now_ms();
     if (any) {
         SS_HIP(hipMemcpyAsync(d_want, want.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
         bool joint = false;
@@ -343,6 +429,7 @@ int SSServer::check_and_retire() {
                 while (r->next_check <= r->iters && r->cfg.check_every > 0) r->next_check += r->cfg.check_every;
         }
     }
+    t_expl += now_ms() - tx;
     for (int i = 0; i < n; ++i) {
         std::shared_ptr<Request> r = resident[i];
         if (!r) continue;
@@ -355,8 +442,7 @@ int SSServer::check_and_retire() {
             cancel = r->cancel;
         }
         if (!(conv || cap || cancel)) continue;
-        int rc = SS_OK;
-        if (!cancel) rc = read_output(*r->dev, r->dev->desc, st, &r->out, 0);
+        const double tr = now_ms();
         std::lock_guard<std::mutex> lk(mu);
         std::memset(&r->rep, 0, sizeof(r->rep));
         r->rep.iterations = r->iters;
@@ -365,12 +451,49 @@ int SSServer::check_and_retire() {
         for (int p = 0; p < SS_MAX_PLAYERS; ++p) r->rep.nashconv[p] = r->gains[p];
         r->rep.kernel_ms = r->kernel_ms;
         r->rep.algo_bytes = r->dev->plan.algo_bytes_iter * r->iters;
-        if (rc) { r->err = rc; r->errmsg = last_error_str(); }
-        finish(r, cancel ? RQ_CANCELLED : (rc ? RQ_ERROR : RQ_DONE));
-        r->rep.elapsed_ms = r->t_done - r->t_submit;
-        cv_done.notify_all();
+        if (cancel) {
+            finish(r, RQ_CANCELLED);
+            r->rep.elapsed_ms = r->t_done - r->t_submit;
+            cv_done.notify_all();
+        } else {
+            // no kernel touches the slot once it is inactive and the task
+            // lists are rebuilt (before the next step): the reader copies
+            // its outputs while the loop goes on
+            release_slot(r);
+            read_jobs.push_back(r);
+            ++n_reading;
+            cv_read.notify_one();
+        }
+        t_read += now_ms() - tr;
     }
     return SS_OK;
+}
+
+void SSServer::read_loop() {
+    std::unique_lock<std::mutex> lk(mu);
+    for (;;) {
+        cv_read.wait(lk, [&] { return reader_stop || !read_jobs.empty(); });
+        if (read_jobs.empty()) break;   // reader_stop and nothing left
+        std::shared_ptr<Request> r = read_jobs.front();
+        read_jobs.pop_front();
+        const bool cancel = r->cancel;
+        lk.unlock();
+        int rc = cancel ? SS_OK : read_output(*r->dev, r->dev->desc, rs, &r->out, 0);
+        std::string msg = rc ? std::string(last_error_str()) : std::string();
+        r->dev.reset();   // the request's device state
+        lk.lock();
+        used_bytes -= r->footprint;
+        if (rc) {
+            r->err = rc;
+            r->errmsg = msg;
+        }
+        r->state = cancel ? RQ_CANCELLED : (rc ? RQ_ERROR : RQ_DONE);
+        r->t_done = now_ms();
+        r->rep.elapsed_ms = r->t_done - r->t_submit;
+        --n_reading;
+        cv_done.notify_all();
+        cv_work.notify_all();   // budget freed: the worker may admit more
+    }
 }
 
 void SSServer::run() {
@@ -383,12 +506,20 @@ void SSServer::run() {
         if (!busy) {
             cv_done.notify_all();
             if (stop) break;
-            cv_work.wait(lk, [&] { return stop || !queue.empty(); });
+            // a queued request the budget holds back waits for a reader to
+            // free its share; an empty queue waits for a submission
+            const int reading = n_reading;
+            cv_work.wait(lk, [&] { return stop || (!queue.empty() && (reading == 0 || n_reading < reading)); });
             continue;
         }
         lk.unlock();
+        const double t0 = now_ms();
         int rc = step();
+        const double t1 = now_ms();
         if (!rc) rc = check_and_retire();
+        t_step += t1 - t0;
+        t_check += now_ms() - t1;
+        ++n_steps;
         lk.lock();
         if (rc) {
             fatal = rc;
@@ -428,7 +559,22 @@ SSServer *ss_server_create(const SSServerConfig *config) {
         delete s;
         return nullptr;
     }
-    s->worker = std::thread([s] { s->run(); });
+    try {
+        s->reader = std::thread([s] { s->read_loop(); });
+        s->worker = std::thread([s] { s->run(); });
+    } catch (...) {
+        if (s->reader.joinable()) {
+            {
+                std::lock_guard<std::mutex> lk(s->mu);
+                s->reader_stop = true;
+                s->cv_read.notify_all();
+            }
+            s->reader.join();
+        }
+        delete s;
+        set_error(SS_ERR_GENERIC, "could not start the server threads");
+        return nullptr;
+    }
     return s;
 }
 
@@ -501,7 +647,7 @@ int ss_server_drain(SSServer *s) {
     std::unique_lock<std::mutex> lk(s->mu);
     s->cv_done.wait(lk, [&] {
         if (s->fatal) return true;
-        if (!s->queue.empty()) return false;
+        if (!s->queue.empty() || s->n_reading > 0) return false;
         for (auto &r : s->resident)
             if (r) return false;
         return true;
@@ -519,6 +665,12 @@ void ss_server_destroy(SSServer *s) {
         s->cv_work.notify_all();
     }
     if (s->worker.joinable()) s->worker.join();
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->reader_stop = true;
+        s->cv_read.notify_all();
+    }
+    if (s->reader.joinable()) s->reader.join();
     for (auto &kv : s->reqs) {
         SSOutput &o = kv.second->out;
         std::free(o.root_strategy); std::free(o.root_ev);
