@@ -127,6 +127,9 @@ struct SSServer {
     bool reader_stop = false;
     hipStream_t rs = nullptr;
     ss::PinnedStage adm_stage;   // admission uploads (worker thread only)
+    hipStream_t as = nullptr;     // admissions set up while a step runs
+    // requests set up during a step, made resident after it (slot index)
+    std::vector<std::pair<std::shared_ptr<ss::Request>, int>> staged;
     int64_t next_ticket = 1;
     std::map<int64_t, std::shared_ptr<ss::Request>> reqs;
     std::deque<int64_t> queue;
@@ -152,7 +155,12 @@ struct SSServer {
     void run();
     int admit_locked(std::unique_lock<std::mutex> &lk);
     void setup_group(const std::vector<std::shared_ptr<ss::Request>> &grp, const std::vector<int> &slot_of,
-                     std::vector<int> &rcs, std::vector<std::string> &msgs);
+                     std::vector<int> &rcs, std::vector<std::string> &msgs, hipStream_t sst, bool upload_desc);
+    void take_admissible(std::vector<std::shared_ptr<ss::Request>> &grp, std::vector<int> &slot_of);
+    void fail_request_locked(const std::shared_ptr<ss::Request> &r, int rc, const std::string &msg);
+    void make_resident_locked(const std::shared_ptr<ss::Request> &r, int slot);
+    void overlap_admit();
+    int install_staged();
     int refresh_tasks();
     void drop_graph();
     int step();
@@ -168,6 +176,7 @@ using namespace ss;
 int SSServer::init() {
     SS_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     SS_HIP(hipStreamCreateWithFlags(&rs, hipStreamNonBlocking));
+    SS_HIP(hipStreamCreateWithFlags(&as, hipStreamNonBlocking));
     const int n = cfg.max_slots;
     SS_HIP(hipMalloc(&d_slots, sizeof(SlotDesc) * n));
     SS_HIP(hipMemset(d_slots, 0, sizeof(SlotDesc) * n));   // active = 0 everywhere
@@ -210,6 +219,7 @@ SSServer::~SSServer() {
     if (scratch) (void)hipFree(scratch);
     if (st) (void)hipStreamDestroy(st);
     if (rs) (void)hipStreamDestroy(rs);
+    if (as) (void)hipStreamDestroy(as);
 }
 
 // the request's slot index inactive on the device and free for admission
@@ -239,7 +249,7 @@ void SSServer::finish(const std::shared_ptr<Request> &r, int state) {
 // descriptors queued, one wait. A request that fails gets its error in rcs
 // / msgs without stopping the others.
 void SSServer::setup_group(const std::vector<std::shared_ptr<Request>> &grp, const std::vector<int> &slot_of,
-                           std::vector<int> &rcs, std::vector<std::string> &msgs) {
+                           std::vector<int> &rcs, std::vector<std::string> &msgs, hipStream_t sst, bool upload_desc) {
     const size_t g = grp.size();
     auto fail = [&](size_t k, int rc) {
         rcs[k] = rc;
@@ -249,10 +259,10 @@ void SSServer::setup_group(const std::vector<std::shared_ptr<Request>> &grp, con
     stage.reset();
     for (size_t k = 0; k < g; ++k) {
         grp[k]->dev.reset(new Slot());
-        int rc = grp[k]->dev->setup_begin(grp[k]->prob.hp, st, stage);
+        int rc = grp[k]->dev->setup_begin(grp[k]->prob.hp, sst, stage);
         if (rc) fail(k, rc);
     }
-    if (hipStreamSynchronize(st) != hipSuccess) {
+    if (hipStreamSynchronize(sst) != hipSuccess) {
         const int rc = set_error(SS_ERR_HIP, "admission: stream synchronisation failed");
         for (size_t k = 0; k < g; ++k)
             if (!rcs[k]) fail(k, rc);
@@ -265,10 +275,10 @@ void SSServer::setup_group(const std::vector<std::shared_ptr<Request>> &grp, con
     for (size_t k = 0; k < g; ++k) {
         if (rcs[k]) continue;
         Request &r = *grp[k];
-        int rc = r.dev->setup_finish(st, stage);
-        if (!rc && r.cfg.precision == SS_PREC_BF16X2) rc = prec_setup(*r.dev, st);   // 2-player rivers only
-        if (!rc) rc = r.dev->alloc_expl(st, 1, stage);
-        if (!rc) {
+        int rc = r.dev->setup_finish(sst, stage);
+        if (!rc && r.cfg.precision == SS_PREC_BF16X2) rc = prec_setup(*r.dev, sst);   // 2-player rivers only
+        if (!rc) rc = r.dev->alloc_expl(sst, 1, stage);
+        if (!rc && upload_desc) {
             SlotDesc d = r.dev->desc;
             d.iter = 0;
             d.iter_end = std::max(0, r.cfg.max_iterations);
@@ -279,68 +289,134 @@ void SSServer::setup_group(const std::vector<std::shared_ptr<Request>> &grp, con
         }
         if (rc) fail(k, rc);
     }
-    if (hipStreamSynchronize(st) != hipSuccess) {   // before the stage is released
+    if (hipStreamSynchronize(sst) != hipSuccess) {   // before the stage is reused
         const int rc = set_error(SS_ERR_HIP, "admission: slot upload failed");
         for (size_t k = 0; k < g; ++k)
             if (!rcs[k]) fail(k, rc);
     }
 }
 
-// Admit queued requests, in order, while a slot is free and the footprint
-// fits: all that fit now are set up as one group.
+// Queued requests, in order, while a slot is free and the footprint fits:
+// their slots and budget are reserved (caller holds the lock).
+void SSServer::take_admissible(std::vector<std::shared_ptr<Request>> &grp, std::vector<int> &slot_of) {
+    std::vector<char> taken(cfg.max_slots, 0);
+    for (int i = 0; i < cfg.max_slots; ++i) taken[i] = resident[i] != nullptr;
+    for (auto &sg : staged) taken[sg.second] = 1;
+    while (!queue.empty()) {
+        auto it = reqs.find(queue.front());
+        if (it == reqs.end() || it->second->cancel) {
+            if (it != reqs.end()) it->second->state = RQ_CANCELLED;
+            queue.pop_front();
+            cv_done.notify_all();
+            continue;
+        }
+        std::shared_ptr<Request> r = it->second;
+        int free_slot = -1;
+        for (int i = 0; i < cfg.max_slots; ++i)
+            if (!taken[i]) { free_slot = i; break; }
+        if (free_slot < 0 || used_bytes + r->footprint > cfg.hbm_budget_bytes) break;
+        queue.pop_front();
+        taken[free_slot] = 1;
+        used_bytes += r->footprint;   // held while the request is set up
+        grp.push_back(r);
+        slot_of.push_back(free_slot);
+    }
+}
+
+void SSServer::fail_request_locked(const std::shared_ptr<Request> &r, int rc, const std::string &msg) {
+    r->err = rc;
+    r->errmsg = msg;
+    r->dev.reset();
+    used_bytes -= r->footprint;
+    r->state = RQ_ERROR;
+    r->t_done = now_ms();
+    cv_done.notify_all();
+}
+
+void SSServer::make_resident_locked(const std::shared_ptr<Request> &r, int slot) {
+    r->slot = slot;
+    r->state = RQ_RUNNING;
+    r->t_admit = now_ms();
+    r->next_check = r->cfg.check_every;
+    resident[slot] = r;
+    dirty = true;
+}
+
+// Admit what fits now, each group set up on the loop's stream (the loop is
+// idle or between steps).
 int SSServer::admit_locked(std::unique_lock<std::mutex> &lk) {
     for (;;) {
         std::vector<std::shared_ptr<Request>> grp;
         std::vector<int> slot_of;
-        std::vector<char> taken(cfg.max_slots, 0);
-        for (int i = 0; i < cfg.max_slots; ++i) taken[i] = resident[i] != nullptr;
-        while (!queue.empty()) {
-            auto it = reqs.find(queue.front());
-            if (it == reqs.end() || it->second->cancel) {
-                if (it != reqs.end()) it->second->state = RQ_CANCELLED;
-                queue.pop_front();
-                cv_done.notify_all();
-                continue;
-            }
-            std::shared_ptr<Request> r = it->second;
-            int free_slot = -1;
-            for (int i = 0; i < cfg.max_slots; ++i)
-                if (!taken[i]) { free_slot = i; break; }
-            if (free_slot < 0 || used_bytes + r->footprint > cfg.hbm_budget_bytes) break;
-            queue.pop_front();
-            taken[free_slot] = 1;
-            used_bytes += r->footprint;   // held while the group is set up
-            grp.push_back(r);
-            slot_of.push_back(free_slot);
-        }
+        take_admissible(grp, slot_of);
         if (grp.empty()) break;
         std::vector<int> rcs(grp.size(), SS_OK);
         std::vector<std::string> msgs(grp.size());
         lk.unlock();
         const double ta = now_ms();
-        setup_group(grp, slot_of, rcs, msgs);
+        setup_group(grp, slot_of, rcs, msgs, st, true);
         t_admit += now_ms() - ta;
         n_admits += (int)grp.size();
         lk.lock();
         for (size_t k = 0; k < grp.size(); ++k) {
-            std::shared_ptr<Request> &r = grp[k];
-            if (rcs[k]) {
-                r->err = rcs[k];
-                r->errmsg = msgs[k];
-                r->dev.reset();
-                used_bytes -= r->footprint;
-                r->state = RQ_ERROR;
-                r->t_done = now_ms();
-                cv_done.notify_all();
-                continue;
-            }
-            r->slot = slot_of[k];
-            r->state = RQ_RUNNING;
-            r->t_admit = now_ms();
-            r->next_check = r->cfg.check_every;
-            resident[slot_of[k]] = r;
-            dirty = true;
+            if (rcs[k]) fail_request_locked(grp[k], rcs[k], msgs[k]);
+            else make_resident_locked(grp[k], slot_of[k]);
         }
+    }
+    return SS_OK;
+}
+
+// While a step runs on the loop's stream: set up what fits on the admission
+// stream (the subgames' own new memory only; their slot descriptors wait for
+// install_staged, after the step, since the step's kernels read every slot's
+// descriptor).
+void SSServer::overlap_admit() {
+    std::vector<std::shared_ptr<Request>> grp;
+    std::vector<int> slot_of;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        take_admissible(grp, slot_of);
+    }
+    if (grp.empty()) return;
+    std::vector<int> rcs(grp.size(), SS_OK);
+    std::vector<std::string> msgs(grp.size());
+    const double ta = now_ms();
+    setup_group(grp, slot_of, rcs, msgs, as, false);
+    t_admit += now_ms() - ta;
+    n_admits += (int)grp.size();
+    std::lock_guard<std::mutex> lk(mu);
+    for (size_t k = 0; k < grp.size(); ++k) {
+        if (rcs[k]) fail_request_locked(grp[k], rcs[k], msgs[k]);
+        else staged.emplace_back(grp[k], slot_of[k]);
+    }
+}
+
+// the requests overlap_admit set up: their descriptors uploaded, resident
+int SSServer::install_staged() {
+    std::vector<std::pair<std::shared_ptr<Request>, int>> sg;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        sg.swap(staged);
+    }
+    if (sg.empty()) return SS_OK;
+    PinnedStage &stage = adm_stage;
+    stage.reset();
+    int rc = SS_OK;
+    for (auto &e : sg) {
+        SlotDesc d = e.first->dev->desc;
+        d.iter = 0;
+        d.iter_end = std::max(0, e.first->cfg.max_iterations);
+        d.final_iter = d.iter_end;
+        d.snap_every = cfg.iters_per_step;   // every step end: any check or retirement finds its root snapshot
+        d.active = 1;
+        if (!rc) rc = stage.upload(&d_slots[e.second], &d, sizeof(d), st);
+    }
+    if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = set_error(SS_ERR_HIP, "slot upload failed");
+    const std::string msg = rc ? std::string(last_error_str()) : std::string();
+    std::lock_guard<std::mutex> lk(mu);
+    for (auto &e : sg) {
+        if (rc) fail_request_locked(e.first, rc, msg);
+        else make_resident_locked(e.first, e.second);
     }
     return SS_OK;
 }
@@ -379,6 +455,7 @@ int SSServer::step() {
         }
     }
     SS_HIP(hipEventRecord(e1, st));
+    overlap_admit();
     SS_HIP(hipEventSynchronize(e1));
     float ms = 0;
     SS_HIP(hipEventElapsedTime(&ms, e0, e1));
@@ -517,6 +594,7 @@ void SSServer::run() {
         int rc = step();
         const double t1 = now_ms();
         if (!rc) rc = check_and_retire();
+        if (!rc) rc = install_staged();
         t_step += t1 - t0;
         t_check += now_ms() - t1;
         ++n_steps;
@@ -530,6 +608,8 @@ void SSServer::run() {
                     r->errmsg = fatal_msg;
                     finish(r, RQ_ERROR);
                 }
+            for (auto &e : staged) fail_request_locked(e.first, rc, fatal_msg);
+            staged.clear();
             cv_done.notify_all();
         }
     }
@@ -647,9 +727,10 @@ int ss_server_drain(SSServer *s) {
     std::unique_lock<std::mutex> lk(s->mu);
     s->cv_done.wait(lk, [&] {
         if (s->fatal) return true;
-        if (!s->queue.empty() || s->n_reading > 0) return false;
-        for (auto &r : s->resident)
-            if (r) return false;
+        // queued (also while being set up) or running (also while its
+        // outputs are read back)
+        for (auto &kv : s->reqs)
+            if (kv.second->state == RQ_QUEUED || kv.second->state == RQ_RUNNING) return false;
         return true;
     });
     return s->fatal ? set_error(s->fatal, "server failed: %s", s->fatal_msg.c_str()) : SS_OK;
