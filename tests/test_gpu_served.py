@@ -186,3 +186,57 @@ def test_served_precision_form_equals_plain():
         assert g.iterations == want.iterations and g.converged
         assert g.exploitability == pytest.approx(want.exploitability, rel=1e-12)
         _same(g, want)
+
+
+def test_served_concurrent_clients_drain():
+    """Four client threads submit small requests at once and cancel every
+    fifth; drain returns only once every request has finished, its outputs
+    read back off the serving loop included, so every ticket then polls
+    final. Requests not cancelled equal the plain form bit for bit."""
+    import threading
+    n = 24
+    spots = [wl.small_spot(200 + i, num_players=2 + (i % 2), hands_per_player=7) for i in range(n)]
+    probs = [ss_api.Problem.from_spot(s) for s in spots]
+    srv = ss_api.Server(max_slots=6, iters_per_step=5)
+    tickets, cancelled, errors = {}, set(), []
+    lock = threading.Lock()
+
+    def client(k):
+        try:
+            for i in range(k, n, 4):
+                t = srv.submit(probs[i], 30, target=0.0)
+                with lock:
+                    tickets[i] = t
+                if i % 5 == 4:
+                    srv.cancel(t)
+                    with lock:
+                        cancelled.add(i)
+        except Exception as e:   # surfaced below
+            errors.append(e)
+
+    try:
+        th = [threading.Thread(target=client, args=(k,)) for k in range(4)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        assert not errors, errors
+        srv.drain()
+        got = {}
+        for i, t in tickets.items():
+            if i in cancelled:
+                try:
+                    srv.poll(t)   # finished before the cancel took effect, or cancelled
+                except ss_api.SolveError:
+                    pass
+                continue
+            r = srv.poll(t)
+            assert r is not None, f"request {i} not final after drain"
+            got[i] = r
+    finally:
+        srv.close()
+    assert len(got) == n - len(cancelled)
+    for i in sorted(got)[::3]:
+        want = ss_api.solve_plain(probs[i], 30)
+        assert got[i].iterations == 30
+        _same(got[i], want)
