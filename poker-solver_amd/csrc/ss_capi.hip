@@ -757,7 +757,14 @@ int ss_batch_run(SSBatch *batch, SSReport *reports, SSBatchStats *stats) {
 int ss_batch_output(SSBatch *batch, int index, SSOutput *output) {
     clear_error();
     if (!batch || !output || index < 0 || index >= batch->impl.n) return set_error(SS_ERR_ARG, "bad arguments");
-    return read_output(*batch->impl.slots[index], batch->impl.slots[index]->desc, batch->impl.st, output, 0);
+    // The batch has finished its work (run returns after it), so the copy
+    // may use a stream of the calling thread: callers reading many outputs
+    // from several threads keep several copies in flight. Each thread's
+    // stream is made once and kept for the life of the process.
+    static thread_local hipStream_t ts = nullptr;
+    if (!ts && hipStreamCreateWithFlags(&ts, hipStreamNonBlocking) != hipSuccess) ts = nullptr;
+    return read_output(*batch->impl.slots[index], batch->impl.slots[index]->desc, ts ? ts : batch->impl.st, output,
+                       0);
 }
 
 void ss_batch_destroy(SSBatch *batch) { delete batch; }

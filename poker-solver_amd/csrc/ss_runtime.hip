@@ -782,10 +782,20 @@ int read_output(Slot &s, const SlotDesc &dnow, hipStream_t st, SSOutput *out, in
     const Plan &pl = s.plan;
     const size_t Mp = pl.Mp;
     const int maxh = pl.maxh, NP = pl.NP;
-    std::vector<float> SS((size_t)pl.A * Mp), rsig(8 * Mp), cfv0(Mp);
-    SS_HIP(hipMemcpyAsync(SS.data(), s.desc.SS, sizeof(float) * SS.size(), hipMemcpyDeviceToHost, st));
-    SS_HIP(hipMemcpyAsync(rsig.data(), s.desc.rsig, sizeof(float) * rsig.size(), hipMemcpyDeviceToHost, st));
-    SS_HIP(hipMemcpyAsync(cfv0.data(), s.desc.cfv, sizeof(float) * Mp, hipMemcpyDeviceToHost, st));
+    // strategy sums, root snapshot and root values copied into this thread's
+    // pinned staging memory (a pageable destination would be staged by the
+    // runtime in synchronous chunks)
+    PinnedStage &stage = thread_stage();
+    const size_t nss = (size_t)pl.A * Mp;
+    void *p_ss = nullptr, *p_rsig = nullptr, *p_cfv = nullptr;
+    int rc = stage.alloc(sizeof(float) * std::max<size_t>(1, nss), &p_ss);
+    if (!rc) rc = stage.alloc(sizeof(float) * 8 * Mp, &p_rsig);
+    if (!rc) rc = stage.alloc(sizeof(float) * Mp, &p_cfv);
+    if (rc) return rc;
+    const float *SS = (const float *)p_ss, *rsig = (const float *)p_rsig, *cfv0 = (const float *)p_cfv;
+    if (nss) SS_HIP(hipMemcpyAsync(p_ss, s.desc.SS, sizeof(float) * nss, hipMemcpyDeviceToHost, st));
+    SS_HIP(hipMemcpyAsync(p_rsig, s.desc.rsig, sizeof(float) * 8 * Mp, hipMemcpyDeviceToHost, st));
+    SS_HIP(hipMemcpyAsync(p_cfv, s.desc.cfv, sizeof(float) * Mp, hipMemcpyDeviceToHost, st));
     SS_HIP(hipStreamSynchronize(st));
     (void)dnow;
     std::memset(out, 0, sizeof(*out));
