@@ -164,6 +164,27 @@ def test_batch_session_modes_equal_plain(mode, monkeypatch):
     b.close()
 
 
+def test_batch_rivers_and_turns_equal_plain():
+    """A batch mixing rivers (river tables built on host threads, uploaded
+    as one image) with turns (continuation leaves, no river tables) gives the
+    plain form's results bit for bit for every subgame, exploitability
+    included."""
+    spots = [wl.mix_river(wl.BASE_SEED + 4000 + 11, max_players=3), wl.river_2p(seed=95),
+             wl.mix_turn(wl.BASE_SEED + 200000 + 3, max_players=3), wl.mix_turn(wl.BASE_SEED + 200000 + 4, max_players=3),
+             wl.small_spot(41, num_players=3, hands_per_player=7, board_cards=4, use_leaves=True)]
+    probs = [ss_api.Problem.from_spot(s) for s in spots]
+    b = ss_api.Batch(probs, 20)
+    reps, _ = b.run()
+    for i, p in enumerate(probs):
+        got = b.output(i, reps[i])
+        want = ss_api.solve_plain(p, 20)
+        assert np.array_equal(got.avg, want.avg), i
+        assert np.array_equal(got.root_strategy, want.root_strategy), i
+        assert np.array_equal(got.root_ev, want.root_ev), i
+        assert got.exploitability == pytest.approx(want.exploitability, rel=1e-12), i
+    b.close()
+
+
 @pytest.mark.parametrize("shape", [(0, 1, -1), (0, 0, 1), (0, 0, 3), (1 << 30, 0, 4), (1 << 30, 0, 2)])
 def test_phased_launch_shapes_equal_plain(shape, monkeypatch):
     """Every launch shape of the phased form gives the plain form's results
